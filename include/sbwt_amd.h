@@ -1,0 +1,252 @@
+/*
+ * sbwt_amd.h -- C ABI of the MI355X (gfx950) SBWT-Search engine.
+ *
+ * The library replaces the reference's GPU seam: the device containers, the
+ * presearcher and the two searchers. Every entry point names the reference
+ * interface it replaces (paths relative to the SBWT-Search source tree).
+ *
+ * Conventions (as in the reference, src/IndexSearcher/IndexSearcher.cpp and
+ * src/Tools/GpuUtils.h):
+ *  - Every function returns 0 on success or a negative status; the message of
+ *    the last failure on the calling thread is sbwt_last_error(). HIP failures
+ *    read "GPUassert: <hipGetErrorString> at <file>:<line>" (GpuUtils.h:20-29).
+ *  - Index handles own their device memory and are shared read-only by any
+ *    number of searchers (IndexSearchMain.cpp:308-316). A searcher owns one
+ *    HIP stream and its batch buffers and is not thread-safe by itself.
+ *  - Calls taking host buffers are synchronous on return (IndexSearcher.cu:63-65).
+ *    Calls ending in _device take device pointers and only enqueue work on the
+ *    searcher's stream; sbwt_searcher_sync waits for it.
+ *  - Bit vectors are arrays of u64 words, bit i of the vector = bit i % 64 of
+ *    word i / 64 (the sdsl-lite bit_vector layout of the .sbwt/.tcolors files).
+ *  - Result sentinels: SBWT_NOT_FOUND (u64 max, printed "-1"), SBWT_INVALID
+ *    (u64 max - 1, printed "-2"), as in BinaryContinuousIndexResultsPrinter.
+ */
+#ifndef SBWT_AMD_H
+#define SBWT_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SBWT_NOT_FOUND UINT64_MAX
+#define SBWT_INVALID (UINT64_MAX - 1)
+#define SBWT_WALK_STUCK (UINT64_MAX - 3)
+
+typedef struct sbwt_index sbwt_index;
+typedef struct sbwt_searcher sbwt_searcher;
+
+/* Message of the last failure on this thread ("" if none). */
+const char *sbwt_last_error(void);
+
+/* Free and total bytes of the current device.
+ * Replaces gpu_utils::get_free_gpu_memory (src/Tools/GpuUtils.cu:7-12). */
+int sbwt_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
+
+/* ---- SBWT index -------------------------------------------------------- */
+
+/* Device index from the four ACGT bit vectors of an SBWT with num_bits nodes
+ * (ceil(num_bits/64) words each). Builds the rank layout, the c-map and the
+ * presearch table of presearch_letters letters (0 = the reference's 12, or k
+ * if k < 12). key_kmer_marks (nullable, ceil(num_bits/64) words) enables the
+ * key-k-mer move.
+ * Replaces GpuSbwtContainer(acgt, poppy, c_map, bits_total, bit_vector_size,
+ * kmer_size, key_kmer_marks) (src/SbwtContainer/GpuSbwtContainer.h:33-41,
+ * GpuSbwtContainer.cpp:16-52) plus Presearcher::presearch()
+ * (src/Presearcher/Presearcher.h:26-32, Presearcher.cpp:18-30); the c-map is
+ * computed as in SbwtBuilder.cpp:111,128-130. */
+int sbwt_index_create(const uint64_t *const acgt[4], uint64_t num_bits, uint32_t kmer_size,
+                      const uint64_t *key_kmer_marks, uint32_t presearch_letters,
+                      sbwt_index **out);
+
+/* Device index from a .sbwt/.tdbg plain-matrix v0.1 file and, if colors_path
+ * is not null, the key-k-mer marks of a .tcolors sdsl-hybrid-v4 file.
+ * Replaces SbwtBuilder(dbg, colors).get_cpu_sbwt()->to_gpu() + presearch
+ * (src/SbwtBuilder/SbwtBuilder.cpp:41-182, IndexSearchMain.cpp:87-103).
+ * Errors: "Error input is not a plain-matrix SBWT", "Error: wrong SBWT
+ * version", "The colors file has an incorrect format. Expected
+ * 'sdsl-hybrid-v4'" (SbwtBuilder.cpp:64,67,152-154). */
+int sbwt_index_load(const char *sbwt_path, const char *colors_path, uint32_t presearch_letters,
+                    sbwt_index **out);
+
+void sbwt_index_destroy(sbwt_index *index);
+
+typedef struct {
+  uint64_t num_nodes;         /* SbwtContainer::get_num_bits */
+  uint32_t kmer_size;         /* SbwtContainer::get_kmer_size */
+  uint32_t presearch_letters;
+  uint64_t c_map[5];
+  uint64_t device_bytes;
+  int has_key_kmer_marks;
+  int wide;                   /* node indexes need 64 bits */
+} sbwt_index_info;
+int sbwt_index_get_info(const sbwt_index *index, sbwt_index_info *info);
+
+/* ---- index search ------------------------------------------------------ */
+
+/* A searcher for batches of up to max_chars_per_batch chars and
+ * max_reads_per_batch reads. hip_stream (nullable) is the hipStream_t to run
+ * on; null creates a stream owned by the searcher.
+ * Replaces IndexSearcher(stream_id, container, max_chars_per_batch,
+ * move_to_key_kmer) (src/IndexSearcher/IndexSearcher.h:34-39, IndexSearcher.cpp:14-24). */
+int sbwt_searcher_create(sbwt_index *index, uint64_t max_chars_per_batch,
+                         uint64_t max_reads_per_batch, int move_to_key_kmer, void *hip_stream,
+                         sbwt_searcher **out);
+void sbwt_searcher_destroy(sbwt_searcher *searcher);
+int sbwt_searcher_sync(sbwt_searcher *searcher);
+
+/* Reference interface: bit_seqs (2-bit MSB-first, 32 chars per word, as built
+ * by ContinuousSeqToBitsConverter) and one start position per k-mer ->
+ * results[i] = SBWT index of the k-mer at positions[i] (after the key-k-mer
+ * move if enabled) or SBWT_NOT_FOUND. Host buffers; synchronous.
+ * Replaces IndexSearcher::search(bit_seqs, kmer_positions, results, batch_id)
+ * (src/IndexSearcher/IndexSearcher.h:41-46, IndexSearcher.cpp:26-51). */
+int sbwt_searcher_search(sbwt_searcher *searcher, const uint64_t *bit_seqs, uint64_t n_words,
+                         const uint64_t *kmer_positions, uint64_t n_positions, uint64_t *results);
+
+/* Same with device pointers; enqueued on the searcher's stream. */
+int sbwt_searcher_search_device(sbwt_searcher *searcher, const uint64_t *d_bit_seqs,
+                                const uint64_t *d_kmer_positions, uint64_t n_positions,
+                                uint64_t *d_results);
+
+/* Fused batch: ASCII reads (all reads concatenated, n_chars bytes) and the
+ * cumulative end offset of every read -> one classified result per k-mer, in
+ * read order: SBWT_INVALID if any of its k chars is not ACGTacgt, else the
+ * search result. Replaces, in one call, ContinuousSeqToBitsConverter
+ * (src/SeqToBitsConverter/ContinuousSeqToBitsConverter.cpp:65-107),
+ * PositionsBuilder (src/PositionsBuilder/PositionsBuilder.cpp:15-60),
+ * IndexSearcher::search and the invalid/not-found classification of
+ * ContinuousIndexResultsPrinter (src/IndexResultsPrinter/
+ * ContinuousIndexResultsPrinter.hpp:146-308). Host buffers; synchronous;
+ * *n_kmers receives the number of results written. */
+int sbwt_searcher_search_reads(sbwt_searcher *searcher, const char *chars, uint64_t n_chars,
+                               const uint64_t *read_ends, uint64_t n_reads, uint64_t *results,
+                               uint64_t *n_kmers);
+
+/* Same with device pointers; enqueued on the searcher's stream. d_results
+ * must hold n_chars entries. n_kmers (nullable, host) waits for the count. */
+int sbwt_searcher_search_reads_device(sbwt_searcher *searcher, const char *d_chars,
+                                      uint64_t n_chars, const uint64_t *d_read_ends,
+                                      uint64_t n_reads, uint64_t *d_results, uint64_t *n_kmers);
+
+/* Device time of the last call's kernels (ms): packing, k-mer offsets,
+ * search. The reference logs the same with GpuEvent pairs
+ * (src/IndexSearcher/IndexSearcher.cu:22,62,66-70). */
+int sbwt_searcher_last_timings(sbwt_searcher *searcher, float *pack_ms, float *offsets_ms,
+                               float *search_ms);
+
+/* Census mode for sbwt_searcher_search_reads(_device): when enabled, the next
+ * calls also count stats[0] k-mers searched, stats[1] SBWT steps taken (one
+ * rank-block line each), stats[2] key-walk steps and stats[3] invalid k-mers,
+ * the exact algorithmic work of the batch. Results are unchanged. No
+ * counterpart in the reference (its kernels keep no counters). */
+int sbwt_searcher_set_stats(sbwt_searcher *searcher, int enable);
+int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[4]);
+
+/* Device 2-bit packing alone: chars -> ceil(n/32)+2 words (MSB-first) and
+ * ceil(n/32)+2 u32 invalid masks (bit j%32 of word j/32 set if char j is not
+ * ACGTacgt). Replaces ContinuousSeqToBitsConverter::parallel_generate
+ * (ContinuousSeqToBitsConverter.cpp:65-107). hip_stream nullable. */
+int sbwt_pack_2bit_device(const char *d_chars, uint64_t n_chars, uint64_t *d_words,
+                          uint32_t *d_invalid, void *hip_stream);
+
+/* Device pointer to the cumulative k-mer count of every read of the last
+ * sbwt_searcher_search_reads(_device) call (n_reads entries), valid until the
+ * next call; it chains the index stage into the color stage on device. No
+ * counterpart in the reference (its stages meet through files). */
+int sbwt_searcher_kmer_ends_device(sbwt_searcher *searcher, const uint64_t **d_kmer_ends);
+
+/* ---- color index and color search -------------------------------------- */
+
+typedef struct sbwt_color_index sbwt_color_index;
+typedef struct sbwt_color_searcher sbwt_color_searcher;
+
+/* Device color index from a .tcolors sdsl-hybrid-v4 file.
+ * Replaces ColorIndexBuilder(filename).get_cpu_color_index_container() and
+ * CpuColorIndexContainer::to_gpu() -> GpuColorIndexContainer
+ * (src/ColorIndexBuilder/ColorIndexBuilder.cpp:13-86,
+ * src/ColorIndexContainer/GpuColorIndexContainer.cpp:13-86). */
+int sbwt_color_index_load(const char *tcolors_path, sbwt_color_index **out);
+void sbwt_color_index_destroy(sbwt_color_index *colors);
+
+typedef struct {
+  uint64_t num_colors;       /* GpuColorIndexContainer::num_colors (max color + 1) */
+  uint64_t num_color_sets;   /* GpuColorIndexContainer::num_color_sets */
+  uint64_t num_nodes;        /* length of the key-k-mer marks */
+  uint64_t device_bytes;
+} sbwt_color_index_info;
+int sbwt_color_index_get_info(const sbwt_color_index *colors, sbwt_color_index_info *info);
+
+/* Replaces ColorSearcher(stream_id, container, max_indexes_per_batch,
+ * max_seqs_per_batch) (src/ColorSearcher/ColorSearcher.h:36-41,
+ * ColorSearcher.cpp:16-36). hip_stream as in sbwt_searcher_create; give both
+ * searchers the same stream to chain the stages on the device. */
+int sbwt_color_searcher_create(sbwt_color_index *colors, uint64_t max_indexes_per_batch,
+                               uint64_t max_reads_per_batch, void *hip_stream,
+                               sbwt_color_searcher **out);
+void sbwt_color_searcher_destroy(sbwt_color_searcher *searcher);
+
+/* Reference interface: found SBWT (key) indexes, each read padded with
+ * u64 max to a multiple of 64, and the cumulative number of 64-index warps of
+ * every read with at least one found index (n_intervals = reads + 1) ->
+ * results[read * num_colors + color] = number of the read's indexes whose
+ * color set holds the color. Host buffers; synchronous.
+ * Replaces ColorSearcher::search(sbwt_index_idxs, warps_intervals, results,
+ * batch_id) (src/ColorSearcher/ColorSearcher.h:43-48, ColorSearcher.cpp:38-58). */
+int sbwt_color_searcher_search(sbwt_color_searcher *searcher, const uint64_t *warped_idxs,
+                               uint64_t n_idxs, const uint64_t *warp_intervals,
+                               uint64_t n_intervals, uint64_t *results);
+
+/* Threshold of the color stage: a read reports color c when its count of c is
+ * at least ceil(seq_size * threshold) with seq_size = found
+ * + include_not_found * not_found + include_invalid * invalid
+ * (src/ColorResultsPrinter/ContinuousColorResultsPrinter.hpp:235-263).
+ * first_read_quirk reproduces the reference printer's initial carry of u64 max
+ * (ContinuousColorResultsPrinter.hpp:62-64,178-180): the first read of a
+ * stream is thresholded with each of its three counts minus one (mod 2^64). */
+typedef struct {
+  double threshold;
+  int include_not_found;
+  int include_invalid;
+  int first_read_quirk;
+} sbwt_color_params;
+
+/* Fused pseudoalignment: the per-k-mer results of sbwt_searcher_search_reads
+ * (with the key-k-mer move) and the cumulative k-mer count of every read ->
+ * for read r the sorted color ids ids[offsets[r] .. offsets[r+1]) passing the
+ * threshold, and read_stats[3r..3r+2] = {found, not found, invalid}
+ * (nullable). Replaces ColorSearcher::search + d_post_process + the
+ * threshold of ContinuousColorResultsPrinter::do_print_seq in one pass, and
+ * writes a sorted u32 list per read instead of reads x num_colors u64 counts.
+ * Host buffers; synchronous; *n_ids = ids written. Fails if more than
+ * ids_capacity ids are needed. */
+int sbwt_color_searcher_pseudoalign(sbwt_color_searcher *searcher, const uint64_t *kmer_results,
+                                    const uint64_t *read_kmer_ends, uint64_t n_reads,
+                                    const sbwt_color_params *params, uint64_t *offsets,
+                                    uint32_t *ids, uint64_t ids_capacity, uint32_t *read_stats,
+                                    uint64_t *n_ids);
+
+/* Same on device buffers (d_offsets: n_reads + 1 entries); synchronous on the
+ * id count only. */
+int sbwt_color_searcher_pseudoalign_device(sbwt_color_searcher *searcher,
+                                           const uint64_t *d_kmer_results,
+                                           const uint64_t *d_read_kmer_ends, uint64_t n_reads,
+                                           const sbwt_color_params *params, uint64_t *d_offsets,
+                                           uint32_t *d_ids, uint64_t ids_capacity,
+                                           uint32_t *d_read_stats, uint64_t *n_ids);
+
+/* Raw per-read counts on device: d_counts[read * num_colors + color] (u32). */
+int sbwt_color_searcher_counts_device(sbwt_color_searcher *searcher,
+                                      const uint64_t *d_kmer_results,
+                                      const uint64_t *d_read_kmer_ends, uint64_t n_reads,
+                                      uint32_t *d_counts);
+
+/* Device time (ms) of the last call's color kernel. */
+int sbwt_color_searcher_last_time(sbwt_color_searcher *searcher, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

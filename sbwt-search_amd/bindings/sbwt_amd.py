@@ -1,0 +1,349 @@
+"""ctypes binding of libsbwt_amd.so (the C ABI in include/sbwt_amd.h).
+
+This is the Python-side FFI stub of the engine: it loads the in-tree shared
+library and raises if it is missing -- there is no CPU fallback. Device
+pointers are plain integers (e.g. torch.Tensor.data_ptr()).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REPO = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "csrc", "libsbwt_amd.so")
+HEADER = os.path.join(REPO, "include", "sbwt_amd.h")
+
+NOT_FOUND = 0xFFFFFFFFFFFFFFFF
+INVALID = 0xFFFFFFFFFFFFFFFE
+WALK_STUCK = 0xFFFFFFFFFFFFFFFC
+
+u64p = C.POINTER(C.c_uint64)
+u32p = C.POINTER(C.c_uint32)
+
+
+class IndexInfo(C.Structure):
+    _fields_ = [("num_nodes", C.c_uint64), ("kmer_size", C.c_uint32),
+                ("presearch_letters", C.c_uint32), ("c_map", C.c_uint64 * 5),
+                ("device_bytes", C.c_uint64), ("has_key_kmer_marks", C.c_int), ("wide", C.c_int)]
+
+
+class ColorIndexInfo(C.Structure):
+    _fields_ = [("num_colors", C.c_uint64), ("num_color_sets", C.c_uint64),
+                ("num_nodes", C.c_uint64), ("device_bytes", C.c_uint64)]
+
+
+class ColorParams(C.Structure):
+    _fields_ = [("threshold", C.c_double), ("include_not_found", C.c_int),
+                ("include_invalid", C.c_int), ("first_read_quirk", C.c_int)]
+
+
+class SbwtError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libsbwt_amd.so; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SbwtError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.sbwt_last_error.restype = C.c_char_p
+        L.sbwt_mem_info.argtypes = [u64p, u64p]
+        L.sbwt_index_create.argtypes = [u64p * 4, C.c_uint64, C.c_uint32, u64p, C.c_uint32,
+                                        C.POINTER(vp)]
+        L.sbwt_index_load.argtypes = [C.c_char_p, C.c_char_p, C.c_uint32, C.POINTER(vp)]
+        L.sbwt_index_destroy.argtypes = [vp]
+        L.sbwt_index_destroy.restype = None
+        L.sbwt_index_get_info.argtypes = [vp, C.POINTER(IndexInfo)]
+        L.sbwt_searcher_create.argtypes = [vp, C.c_uint64, C.c_uint64, C.c_int, vp, C.POINTER(vp)]
+        L.sbwt_searcher_destroy.argtypes = [vp]
+        L.sbwt_searcher_destroy.restype = None
+        L.sbwt_searcher_sync.argtypes = [vp]
+        L.sbwt_searcher_search.argtypes = [vp, u64p, C.c_uint64, u64p, C.c_uint64, u64p]
+        L.sbwt_searcher_search_device.argtypes = [vp, vp, vp, C.c_uint64, vp]
+        L.sbwt_searcher_search_reads.argtypes = [vp, C.c_char_p, C.c_uint64, u64p, C.c_uint64,
+                                                 u64p, u64p]
+        L.sbwt_searcher_search_reads_device.argtypes = [vp, vp, C.c_uint64, vp, C.c_uint64, vp,
+                                                        u64p]
+        L.sbwt_searcher_last_timings.argtypes = [vp, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                                 C.POINTER(C.c_float)]
+        L.sbwt_pack_2bit_device.argtypes = [vp, C.c_uint64, vp, vp, vp]
+        L.sbwt_searcher_kmer_ends_device.argtypes = [vp, C.POINTER(vp)]
+        L.sbwt_color_index_load.argtypes = [C.c_char_p, C.POINTER(vp)]
+        L.sbwt_color_index_destroy.argtypes = [vp]
+        L.sbwt_color_index_destroy.restype = None
+        L.sbwt_color_index_get_info.argtypes = [vp, C.POINTER(ColorIndexInfo)]
+        L.sbwt_color_searcher_create.argtypes = [vp, C.c_uint64, C.c_uint64, vp, C.POINTER(vp)]
+        L.sbwt_color_searcher_destroy.argtypes = [vp]
+        L.sbwt_color_searcher_destroy.restype = None
+        L.sbwt_color_searcher_search.argtypes = [vp, u64p, C.c_uint64, u64p, C.c_uint64, u64p]
+        L.sbwt_color_searcher_pseudoalign.argtypes = [vp, u64p, u64p, C.c_uint64,
+                                                      C.POINTER(ColorParams), u64p, u32p,
+                                                      C.c_uint64, u32p, u64p]
+        L.sbwt_color_searcher_pseudoalign_device.argtypes = [vp, vp, vp, C.c_uint64,
+                                                             C.POINTER(ColorParams), vp, vp,
+                                                             C.c_uint64, vp, u64p]
+        L.sbwt_color_searcher_counts_device.argtypes = [vp, vp, vp, C.c_uint64, vp]
+        L.sbwt_color_searcher_last_time.argtypes = [vp, C.POINTER(C.c_float)]
+        L.sbwt_searcher_set_stats.argtypes = [vp, C.c_int]
+        L.sbwt_searcher_last_stats.argtypes = [vp, u64p]
+        _lib = L
+    return _lib
+
+
+def header_symbols():
+    """Names of every function declared in include/sbwt_amd.h."""
+    import re
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(sbwt_[a-z0-9_]+)\s*\(", text)))
+
+
+def _check(rc):
+    if rc != 0:
+        raise SbwtError(lib().sbwt_last_error().decode())
+
+
+def _p64(a):
+    assert a.dtype == np.uint64 and a.flags.c_contiguous
+    return a.ctypes.data_as(u64p)
+
+
+def mem_info():
+    f, t = C.c_uint64(), C.c_uint64()
+    _check(lib().sbwt_mem_info(C.byref(f), C.byref(t)))
+    return f.value, t.value
+
+
+class Index:
+    """Device-resident SBWT index (rank blocks + presearch table + key marks)."""
+
+    def __init__(self, handle):
+        self.h = handle
+        info = IndexInfo()
+        _check(lib().sbwt_index_get_info(self.h, C.byref(info)))
+        self.num_nodes = info.num_nodes
+        self.k = info.kmer_size
+        self.presearch_letters = info.presearch_letters
+        self.c_map = [info.c_map[i] for i in range(5)]
+        self.device_bytes = info.device_bytes
+        self.has_key_marks = bool(info.has_key_kmer_marks)
+        self.wide = bool(info.wide)
+
+    @classmethod
+    def load(cls, sbwt_path, colors_path=None, presearch_letters=0):
+        h = C.c_void_p()
+        _check(lib().sbwt_index_load(os.fsencode(sbwt_path),
+                                     os.fsencode(colors_path) if colors_path else None,
+                                     presearch_letters, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def create(cls, acgt, num_bits, k, key_marks=None, presearch_letters=0):
+        keep = [np.ascontiguousarray(a, np.uint64) for a in acgt]
+        arr = (u64p * 4)(*[_p64(a) for a in keep])
+        km = np.ascontiguousarray(key_marks, np.uint64) if key_marks is not None else None
+        h = C.c_void_p()
+        _check(lib().sbwt_index_create(arr, num_bits, k, _p64(km) if km is not None else None,
+                                       presearch_letters, C.byref(h)))
+        return cls(h)
+
+    def close(self):
+        if self.h:
+            lib().sbwt_index_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def searcher(self, max_chars, max_reads, move_to_key_kmer=False, stream=None):
+        return Searcher(self, max_chars, max_reads, move_to_key_kmer, stream)
+
+
+class Searcher:
+    """One HIP stream + batch buffers (the reference's IndexSearcher)."""
+
+    def __init__(self, index, max_chars, max_reads, move_to_key_kmer=False, stream=None):
+        self.index = index
+        self.h = C.c_void_p()
+        _check(lib().sbwt_searcher_create(index.h, max_chars, max_reads, int(move_to_key_kmer),
+                                          C.c_void_p(stream) if stream else None,
+                                          C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().sbwt_searcher_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def search(self, bit_seqs, positions):
+        bit_seqs = np.ascontiguousarray(bit_seqs, np.uint64)
+        positions = np.ascontiguousarray(positions, np.uint64)
+        out = np.empty(len(positions), np.uint64)
+        _check(lib().sbwt_searcher_search(self.h, _p64(bit_seqs), len(bit_seqs), _p64(positions),
+                                          len(positions), _p64(out)))
+        return out
+
+    def search_reads(self, chars, read_ends):
+        chars = bytes(chars)
+        read_ends = np.ascontiguousarray(read_ends, np.uint64)
+        out = np.empty(max(len(chars), 1), np.uint64)
+        nk = C.c_uint64()
+        _check(lib().sbwt_searcher_search_reads(self.h, chars, len(chars), _p64(read_ends),
+                                                len(read_ends), _p64(out), C.byref(nk)))
+        return out[:nk.value]
+
+    def search_reads_device(self, d_chars, n_chars, d_read_ends, n_reads, d_results,
+                            want_count=False):
+        nk = C.c_uint64()
+        _check(lib().sbwt_searcher_search_reads_device(
+            self.h, C.c_void_p(d_chars), n_chars, C.c_void_p(d_read_ends), n_reads,
+            C.c_void_p(d_results), C.byref(nk) if want_count else None))
+        return nk.value if want_count else None
+
+    def search_device(self, d_bit_seqs, d_positions, n, d_results):
+        _check(lib().sbwt_searcher_search_device(self.h, C.c_void_p(d_bit_seqs),
+                                                 C.c_void_p(d_positions), n,
+                                                 C.c_void_p(d_results)))
+
+    def kmer_ends_device(self):
+        p = C.c_void_p()
+        _check(lib().sbwt_searcher_kmer_ends_device(self.h, C.byref(p)))
+        return p.value
+
+    def set_stats(self, enable):
+        _check(lib().sbwt_searcher_set_stats(self.h, int(enable)))
+
+    def last_stats(self):
+        out = np.zeros(4, np.uint64)
+        _check(lib().sbwt_searcher_last_stats(self.h, _p64(out)))
+        return {"searched": int(out[0]), "steps": int(out[1]), "walk_steps": int(out[2]),
+                "invalid": int(out[3])}
+
+    def sync(self):
+        _check(lib().sbwt_searcher_sync(self.h))
+
+    def last_timings(self):
+        a, b, c = C.c_float(), C.c_float(), C.c_float()
+        _check(lib().sbwt_searcher_last_timings(self.h, C.byref(a), C.byref(b), C.byref(c)))
+        return {"pack_ms": a.value, "offsets_ms": b.value, "search_ms": c.value}
+
+
+def pack_2bit_device(d_chars, n_chars, d_words, d_invalid, stream=None):
+    _check(lib().sbwt_pack_2bit_device(C.c_void_p(d_chars), n_chars, C.c_void_p(d_words),
+                                       C.c_void_p(d_invalid),
+                                       C.c_void_p(stream) if stream else None))
+
+
+class ColorIndex:
+    """Device-resident color index (.tcolors)."""
+
+    def __init__(self, path):
+        self.h = C.c_void_p()
+        _check(lib().sbwt_color_index_load(os.fsencode(path), C.byref(self.h)))
+        info = ColorIndexInfo()
+        _check(lib().sbwt_color_index_get_info(self.h, C.byref(info)))
+        self.num_colors = info.num_colors
+        self.num_color_sets = info.num_color_sets
+        self.num_nodes = info.num_nodes
+        self.device_bytes = info.device_bytes
+
+    def close(self):
+        if self.h:
+            lib().sbwt_color_index_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def searcher(self, max_indexes, max_reads, stream=None):
+        return ColorSearcher(self, max_indexes, max_reads, stream)
+
+
+def color_params(threshold=1.0, include_not_found=False, include_invalid=False,
+                 first_read_quirk=False):
+    return ColorParams(threshold, int(include_not_found), int(include_invalid),
+                       int(first_read_quirk))
+
+
+class ColorSearcher:
+    """One HIP stream + batch buffers (the reference's ColorSearcher)."""
+
+    def __init__(self, colors, max_indexes, max_reads, stream=None):
+        self.colors = colors
+        self.h = C.c_void_p()
+        _check(lib().sbwt_color_searcher_create(colors.h, max_indexes, max_reads,
+                                                C.c_void_p(stream) if stream else None,
+                                                C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().sbwt_color_searcher_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def search(self, warped, warp_intervals):
+        """Reference interface -> u64 counts [(len(warp_intervals) - 1), num_colors]."""
+        warped = np.ascontiguousarray(warped, np.uint64)
+        wi = np.ascontiguousarray(warp_intervals, np.uint64)
+        n = max(len(wi) - 1, 0)
+        out = np.zeros(max(n * self.colors.num_colors, 1), np.uint64)
+        _check(lib().sbwt_color_searcher_search(self.h, _p64(warped), len(warped), _p64(wi),
+                                                len(wi), _p64(out)))
+        return out[:n * self.colors.num_colors].reshape(n, self.colors.num_colors)
+
+    def pseudoalign(self, kmer_results, read_kmer_ends, params, ids_capacity=None):
+        res = np.ascontiguousarray(kmer_results, np.uint64)
+        ends = np.ascontiguousarray(read_kmer_ends, np.uint64)
+        n = len(ends)
+        cap = ids_capacity if ids_capacity is not None else max(n * self.colors.num_colors, 1)
+        offsets = np.zeros(n + 1, np.uint64)
+        ids = np.zeros(max(cap, 1), np.uint32)
+        stats = np.zeros(max(3 * n, 1), np.uint32)
+        nids = C.c_uint64()
+        _check(lib().sbwt_color_searcher_pseudoalign(
+            self.h, _p64(res) if len(res) else None, _p64(ends), n, C.byref(params),
+            _p64(offsets), ids.ctypes.data_as(u32p), cap, stats.ctypes.data_as(u32p),
+            C.byref(nids)))
+        lists = [ids[int(offsets[r]):int(offsets[r + 1])].tolist() for r in range(n)]
+        return lists, stats[:3 * n].reshape(n, 3), offsets, ids[:nids.value]
+
+    def pseudoalign_device(self, d_results, d_kmer_ends, n_reads, params, d_offsets, d_ids,
+                           ids_capacity, d_stats=None):
+        nids = C.c_uint64()
+        _check(lib().sbwt_color_searcher_pseudoalign_device(
+            self.h, C.c_void_p(d_results), C.c_void_p(d_kmer_ends), n_reads, C.byref(params),
+            C.c_void_p(d_offsets), C.c_void_p(d_ids), ids_capacity,
+            C.c_void_p(d_stats) if d_stats else None, C.byref(nids)))
+        return nids.value
+
+    def counts_device(self, d_results, d_kmer_ends, n_reads, d_counts):
+        _check(lib().sbwt_color_searcher_counts_device(self.h, C.c_void_p(d_results),
+                                                       C.c_void_p(d_kmer_ends), n_reads,
+                                                       C.c_void_p(d_counts)))
+
+    def last_time(self):
+        ms = C.c_float()
+        _check(lib().sbwt_color_searcher_last_time(self.h, C.byref(ms)))
+        return ms.value

@@ -1,0 +1,366 @@
+// extern "C" boundary of libsbwt_amd.so (declared in include/sbwt_amd.h).
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <string>
+
+#include "../../include/sbwt_amd.h"
+#include "color_search.hpp"
+#include "device_index.hpp"
+#include "index_search.hpp"
+#include "loaders.hpp"
+
+using sbwt_amd::DeviceIndex;
+using sbwt_amd::Searcher;
+
+struct sbwt_index {
+  DeviceIndex dev;
+  ~sbwt_index() { dev.release(); }
+};
+
+struct sbwt_searcher {
+  std::unique_ptr<Searcher> impl;
+};
+
+struct sbwt_color_index {
+  sbwt_amd::DeviceColors dev;
+  ~sbwt_color_index() { dev.release(); }
+};
+
+struct sbwt_color_searcher {
+  std::unique_ptr<sbwt_amd::ColorSearcher> impl;
+  const sbwt_color_index *colors = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F f) {
+  try {
+    f();
+    g_last_error.clear();
+    return 0;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "out of host memory";
+    return -2;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return -1;
+  }
+}
+
+void require(bool ok, const char *msg) {
+  if (!ok) throw std::runtime_error(msg);
+}
+
+// --threshold must lie in [0, 1] (ColorSearchArgumentParser.cpp:202-211).
+sbwt_amd::ColorParams to_params(const sbwt_color_params *p) {
+  sbwt_amd::ColorParams q;
+  if (p) {
+    if (!(p->threshold >= 0.0 && p->threshold <= 1.0))
+      throw std::runtime_error("threshold must be in [0, 1]");
+    q.threshold = p->threshold;
+    q.include_not_found = p->include_not_found != 0;
+    q.include_invalid = p->include_invalid != 0;
+    q.first_read_quirk = p->first_read_quirk != 0;
+  }
+  return q;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *sbwt_last_error(void) { return g_last_error.c_str(); }
+
+int sbwt_mem_info(uint64_t *free_bytes, uint64_t *total_bytes) {
+  return guarded([&] {
+    size_t f = 0, t = 0;
+    SBWT_HIP(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = f;
+    if (total_bytes) *total_bytes = t;
+  });
+}
+
+int sbwt_index_create(const uint64_t *const acgt[4], uint64_t num_bits, uint32_t kmer_size,
+                      const uint64_t *key_kmer_marks, uint32_t presearch_letters,
+                      sbwt_index **out) {
+  return guarded([&] {
+    require(out != nullptr && acgt != nullptr, "null argument");
+    require(num_bits >= 1, "an SBWT has at least one node");
+    require(kmer_size >= 1 && kmer_size <= 32, "k-mer size must be in [1, 32]");
+    for (int c = 0; c < 4; ++c) require(acgt[c] != nullptr, "null bit vector");
+    const uint32_t kp = presearch_letters ? presearch_letters : sbwt_amd::default_presearch_letters(kmer_size);
+    require(kp >= 1 && kp <= kmer_size && kp <= 14, "presearch letters must be in [1, min(k, 14)]");
+    auto idx = std::make_unique<sbwt_index>();
+    idx->dev.build(acgt, num_bits, kmer_size, kp, key_kmer_marks, key_kmer_marks ? num_bits : 0);
+    *out = idx.release();
+  });
+}
+
+int sbwt_index_load(const char *sbwt_path, const char *colors_path, uint32_t presearch_letters,
+                    sbwt_index **out) {
+  return guarded([&] {
+    require(out != nullptr && sbwt_path != nullptr, "null argument");
+    sbwt_amd::HostSbwt s = sbwt_amd::load_sbwt(sbwt_path);
+    sbwt_amd::HostBitVector km;
+    if (colors_path) {
+      km = sbwt_amd::load_key_kmer_marks(colors_path);
+      require(km.bits == s.num_bits, "key-k-mer marks and SBWT differ in length");
+    }
+    const uint64_t *acgt[4] = {s.acgt[0].data(), s.acgt[1].data(), s.acgt[2].data(), s.acgt[3].data()};
+    const uint32_t k = static_cast<uint32_t>(s.k);
+    const uint32_t kp = presearch_letters ? presearch_letters : sbwt_amd::default_presearch_letters(k);
+    require(kp >= 1 && kp <= k && kp <= 14, "presearch letters must be in [1, min(k, 14)]");
+    auto idx = std::make_unique<sbwt_index>();
+    idx->dev.build(acgt, s.num_bits, k, kp, colors_path ? km.words.data() : nullptr,
+                   colors_path ? km.bits : 0);
+    *out = idx.release();
+  });
+}
+
+void sbwt_index_destroy(sbwt_index *index) { delete index; }
+
+int sbwt_index_get_info(const sbwt_index *index, sbwt_index_info *info) {
+  return guarded([&] {
+    require(index && info, "null argument");
+    info->num_nodes = index->dev.n;
+    info->kmer_size = index->dev.k;
+    info->presearch_letters = index->dev.kp;
+    for (int i = 0; i < 5; ++i) info->c_map[i] = index->dev.c_map[i];
+    info->device_bytes = index->dev.device_bytes;
+    info->has_key_kmer_marks = index->dev.key_marks.blocks != nullptr;
+    info->wide = index->dev.wide;
+  });
+}
+
+int sbwt_searcher_create(sbwt_index *index, uint64_t max_chars_per_batch,
+                         uint64_t max_reads_per_batch, int move_to_key_kmer, void *hip_stream,
+                         sbwt_searcher **out) {
+  return guarded([&] {
+    require(index && out, "null argument");
+    require(max_chars_per_batch > 0 && max_reads_per_batch > 0, "Not enough memory");
+    auto s = std::make_unique<sbwt_searcher>();
+    s->impl = std::make_unique<Searcher>(&index->dev, max_chars_per_batch, max_reads_per_batch,
+                                         move_to_key_kmer != 0, static_cast<hipStream_t>(hip_stream));
+    *out = s.release();
+  });
+}
+
+void sbwt_searcher_destroy(sbwt_searcher *searcher) { delete searcher; }
+
+int sbwt_searcher_sync(sbwt_searcher *searcher) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    SBWT_HIP(hipStreamSynchronize(searcher->impl->stream()));
+  });
+}
+
+int sbwt_searcher_search(sbwt_searcher *searcher, const uint64_t *bit_seqs, uint64_t n_words,
+                         const uint64_t *kmer_positions, uint64_t n_positions, uint64_t *results) {
+  return guarded([&] {
+    require(searcher && (n_positions == 0 || (bit_seqs && kmer_positions && results)), "null argument");
+    searcher->impl->search_positions_host(bit_seqs, n_words, kmer_positions, n_positions, results);
+  });
+}
+
+int sbwt_searcher_search_device(sbwt_searcher *searcher, const uint64_t *d_bit_seqs,
+                                const uint64_t *d_kmer_positions, uint64_t n_positions,
+                                uint64_t *d_results) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    searcher->impl->search_positions_device(d_bit_seqs, d_kmer_positions, n_positions, d_results);
+  });
+}
+
+int sbwt_searcher_search_reads(sbwt_searcher *searcher, const char *chars, uint64_t n_chars,
+                               const uint64_t *read_ends, uint64_t n_reads, uint64_t *results,
+                               uint64_t *n_kmers) {
+  return guarded([&] {
+    require(searcher && (n_reads == 0 || (read_ends && results)), "null argument");
+    require(n_chars == 0 || chars, "null argument");
+    require(n_reads == 0 || read_ends[n_reads - 1] == n_chars, "read_ends must end at n_chars");
+    const uint64_t nk = searcher->impl->search_reads_host(chars, n_chars, read_ends, n_reads, results);
+    if (n_kmers) *n_kmers = nk;
+  });
+}
+
+int sbwt_searcher_search_reads_device(sbwt_searcher *searcher, const char *d_chars,
+                                      uint64_t n_chars, const uint64_t *d_read_ends,
+                                      uint64_t n_reads, uint64_t *d_results, uint64_t *n_kmers) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    searcher->impl->search_reads_device(d_chars, n_chars, d_read_ends, n_reads, d_results, n_kmers);
+  });
+}
+
+int sbwt_searcher_last_timings(sbwt_searcher *searcher, float *pack_ms, float *offsets_ms,
+                               float *search_ms) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    const sbwt_amd::SearchTimings t = searcher->impl->last_timings();
+    if (pack_ms) *pack_ms = t.pack_ms;
+    if (offsets_ms) *offsets_ms = t.offsets_ms;
+    if (search_ms) *search_ms = t.search_ms;
+  });
+}
+
+int sbwt_searcher_set_stats(sbwt_searcher *searcher, int enable) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    searcher->impl->set_stats(enable != 0);
+  });
+}
+
+int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[4]) {
+  return guarded([&] {
+    require(searcher && stats, "null argument");
+    searcher->impl->last_stats(stats);
+  });
+}
+
+int sbwt_pack_2bit_device(const char *d_chars, uint64_t n_chars, uint64_t *d_words,
+                          uint32_t *d_invalid, void *hip_stream) {
+  return guarded([&] {
+    sbwt_amd::launch_pack(d_chars, n_chars, d_words, d_invalid, static_cast<hipStream_t>(hip_stream));
+  });
+}
+
+int sbwt_searcher_kmer_ends_device(sbwt_searcher *searcher, const uint64_t **d_kmer_ends) {
+  return guarded([&] {
+    require(searcher && d_kmer_ends, "null argument");
+    *d_kmer_ends = searcher->impl->kmer_ends_device();
+  });
+}
+
+int sbwt_color_index_load(const char *tcolors_path, sbwt_color_index **out) {
+  return guarded([&] {
+    require(tcolors_path && out, "null argument");
+    sbwt_amd::HostColors host = sbwt_amd::load_colors(tcolors_path);
+    auto ci = std::make_unique<sbwt_color_index>();
+    ci->dev.build(host);
+    *out = ci.release();
+  });
+}
+
+void sbwt_color_index_destroy(sbwt_color_index *colors) { delete colors; }
+
+int sbwt_color_index_get_info(const sbwt_color_index *colors, sbwt_color_index_info *info) {
+  return guarded([&] {
+    require(colors && info, "null argument");
+    info->num_colors = colors->dev.num_colors;
+    info->num_color_sets = colors->dev.num_color_sets;
+    info->num_nodes = colors->dev.num_nodes;
+    info->device_bytes = colors->dev.device_bytes;
+  });
+}
+
+int sbwt_color_searcher_create(sbwt_color_index *colors, uint64_t max_indexes_per_batch,
+                               uint64_t max_reads_per_batch, void *hip_stream,
+                               sbwt_color_searcher **out) {
+  return guarded([&] {
+    require(colors && out, "null argument");
+    require(max_reads_per_batch > 0, "Not enough memory");
+    auto s = std::make_unique<sbwt_color_searcher>();
+    s->colors = colors;
+    s->impl = std::make_unique<sbwt_amd::ColorSearcher>(&colors->dev, max_indexes_per_batch,
+                                                        max_reads_per_batch,
+                                                        static_cast<hipStream_t>(hip_stream));
+    *out = s.release();
+  });
+}
+
+void sbwt_color_searcher_destroy(sbwt_color_searcher *searcher) { delete searcher; }
+
+int sbwt_color_searcher_search(sbwt_color_searcher *searcher, const uint64_t *warped_idxs,
+                               uint64_t n_idxs, const uint64_t *warp_intervals,
+                               uint64_t n_intervals, uint64_t *results) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    require(n_idxs == 0 || (warped_idxs && warp_intervals && results), "null argument");
+    searcher->impl->search_warped_host(warped_idxs, n_idxs, warp_intervals, n_intervals, results);
+  });
+}
+
+int sbwt_color_searcher_pseudoalign_device(sbwt_color_searcher *searcher,
+                                           const uint64_t *d_kmer_results,
+                                           const uint64_t *d_read_kmer_ends, uint64_t n_reads,
+                                           const sbwt_color_params *params, uint64_t *d_offsets,
+                                           uint32_t *d_ids, uint64_t ids_capacity,
+                                           uint32_t *d_read_stats, uint64_t *n_ids) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    const uint64_t n = searcher->impl->pseudoalign_device(d_kmer_results, d_read_kmer_ends, n_reads,
+                                                          to_params(params), d_offsets, d_ids,
+                                                          ids_capacity, d_read_stats);
+    if (n_ids) *n_ids = n;
+  });
+}
+
+int sbwt_color_searcher_pseudoalign(sbwt_color_searcher *searcher, const uint64_t *kmer_results,
+                                    const uint64_t *read_kmer_ends, uint64_t n_reads,
+                                    const sbwt_color_params *params, uint64_t *offsets,
+                                    uint32_t *ids, uint64_t ids_capacity, uint32_t *read_stats,
+                                    uint64_t *n_ids) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    if (n_ids) *n_ids = 0;
+    if (n_reads == 0) return;
+    require(read_kmer_ends && offsets && (ids || ids_capacity == 0), "null argument");
+    const uint64_t nk = read_kmer_ends[n_reads - 1];
+    for (uint64_t r = 1; r < n_reads; ++r)
+      require(read_kmer_ends[r] >= read_kmer_ends[r - 1], "read_kmer_ends must be non-decreasing");
+    require(nk == 0 || kmer_results, "null argument");
+    const sbwt_amd::ColorParams p = to_params(params);
+    hipStream_t st = searcher->impl->stream();
+    uint64_t *d_res = nullptr, *d_ends = nullptr, *d_off = nullptr;
+    uint32_t *d_ids = nullptr, *d_stats = nullptr;
+    auto cleanup = [&] {
+      void *b[] = {d_res, d_ends, d_off, d_ids, d_stats};
+      for (void *x : b)
+        if (x) (void)hipFree(x);
+    };
+    try {
+      SBWT_HIP(hipMalloc(&d_res, std::max<uint64_t>(nk, 1) * 8));
+      SBWT_HIP(hipMalloc(&d_ends, n_reads * 8));
+      SBWT_HIP(hipMalloc(&d_off, (n_reads + 1) * 8));
+      SBWT_HIP(hipMalloc(&d_ids, std::max<uint64_t>(ids_capacity, 1) * 4));
+      SBWT_HIP(hipMalloc(&d_stats, n_reads * 3 * 4));
+      if (nk) SBWT_HIP(hipMemcpyAsync(d_res, kmer_results, nk * 8, hipMemcpyHostToDevice, st));
+      SBWT_HIP(hipMemcpyAsync(d_ends, read_kmer_ends, n_reads * 8, hipMemcpyHostToDevice, st));
+      const uint64_t n = searcher->impl->pseudoalign_device(d_res, d_ends, n_reads, p, d_off, d_ids,
+                                                            ids_capacity, d_stats);
+      SBWT_HIP(hipMemcpyAsync(offsets, d_off, (n_reads + 1) * 8, hipMemcpyDeviceToHost, st));
+      if (n) SBWT_HIP(hipMemcpyAsync(ids, d_ids, n * 4, hipMemcpyDeviceToHost, st));
+      if (read_stats)
+        SBWT_HIP(hipMemcpyAsync(read_stats, d_stats, n_reads * 3 * 4, hipMemcpyDeviceToHost, st));
+      SBWT_HIP(hipStreamSynchronize(st));
+      if (n_ids) *n_ids = n;
+    } catch (...) {
+      cleanup();
+      throw;
+    }
+    cleanup();
+  });
+}
+
+int sbwt_color_searcher_counts_device(sbwt_color_searcher *searcher,
+                                      const uint64_t *d_kmer_results,
+                                      const uint64_t *d_read_kmer_ends, uint64_t n_reads,
+                                      uint32_t *d_counts) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    searcher->impl->counts_device(d_kmer_results, d_read_kmer_ends, n_reads, d_counts);
+  });
+}
+
+int sbwt_color_searcher_last_time(sbwt_color_searcher *searcher, float *ms) {
+  return guarded([&] {
+    require(searcher && ms, "null argument");
+    *ms = searcher->impl->last_search_ms();
+  });
+}
+
+}  // extern "C"

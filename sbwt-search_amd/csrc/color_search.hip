@@ -1,0 +1,491 @@
+// Color search kernels for gfx950: per-read color counts in LDS, threshold,
+// sorted-id compaction; plus the reference's warp-padded interface.
+#include <algorithm>
+#include <vector>
+
+#include "color_search.hpp"
+#include "device_fns.hpp"
+
+namespace sbwt_amd {
+
+namespace {
+
+constexpr u32 kWave = 64;
+constexpr u32 kLdsBytes = 64 * 1024;
+constexpr u64 kMaxGrid = 1u << 20;
+constexpr u32 kScanThreads = 256;
+constexpr u32 kScanItems = 16;
+constexpr u64 kScanChunk = static_cast<u64>(kScanThreads) * kScanItems;
+
+template <typename T>
+T *dev_upload(const T *host, u64 count) {
+  T *d = nullptr;
+  SBWT_HIP(hipMalloc(&d, std::max<u64>(count, 1) * sizeof(T)));
+  if (count) SBWT_HIP(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice));
+  return d;
+}
+
+inline bool host_bit(const std::vector<u64> &w, u64 i) { return (w[i >> 6] >> (i & 63)) & 1ULL; }
+
+inline u64 host_intv(const HostIntVector &v, u64 i) {
+  const u64 bit = i * v.width, w = bit >> 6;
+  const u32 off = static_cast<u32>(bit & 63);
+  u64 x = v.words[w] >> off;
+  if (off + v.width > 64) x |= v.words[w + 1] << (64 - off);
+  return v.width == 64 ? x : (x & ((1ULL << v.width) - 1));
+}
+
+// ---------------------------------------------------------------------------
+// Color set of a found (key) SBWT index: key-mark rank -> color_set_idx ->
+// set table (ColorSearcher.cuh:99-131). Adds `m` to the LDS counter of every
+// color of the set (the per-color ballot/popcount of ColorSearcher.cuh:175-205
+// summed over the read by ColorPostProcessor.cuh:21-43).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void add_color_set(const ColorView &cv, u64 s, u32 m, u32 *cnt) {
+  const u64 j = rank_marks(cv.key_marks, s);
+  const u64 cs = intv_get(cv.color_set_idxs, cv.csi_width, j);
+  if (cs >= cv.num_color_sets) return;
+  const uint4 e = cv.set_table[cs];
+  const u64 start = static_cast<u64>(e.x) | (static_cast<u64>(e.y) << 32);
+  const u32 len = e.z;
+  if (e.w & 1u) {
+    for (u32 b = 0; b < len; b += 64) {
+      const u64 bit = start + b, w = bit >> 6;
+      const u32 off = static_cast<u32>(bit & 63);
+      u64 x = cv.dense_arrays[w] >> off;
+      if (off) x |= cv.dense_arrays[w + 1] << (64 - off);
+      const u32 take = min(64u, len - b);
+      if (take < 64) x &= (1ULL << take) - 1;
+      while (x) {
+        const u32 c = b + static_cast<u32>(__builtin_ctzll(x));
+        if (c < cv.num_colors) atomicAdd(&cnt[c], m);
+        x &= x - 1;
+      }
+    }
+  } else {
+    for (u32 i = 0; i < len; ++i) {
+      const u64 c = intv_get(cv.sparse_arrays, cv.sparse_width, start + i);
+      if (c < cv.num_colors) atomicAdd(&cnt[c], m);
+    }
+  }
+}
+
+// One wave-wide pass over 64 consecutive result slots of a read: lanes whose
+// index equals the previous lane's share one color-set decode (consecutive
+// k-mers of a read usually walk to the same key k-mer).
+__device__ __forceinline__ void wave_add(const ColorView &cv, u64 v, bool found, u32 lane, u32 *cnt) {
+  const u64 prev = __shfl_up(v, 1, kWave);
+  const bool same = found && lane > 0 && prev == v;
+  const u64 same_mask = __ballot(same);
+  if (found && !same) {
+    const u64 rest = lane == kWave - 1 ? 0 : (same_mask >> (lane + 1));
+    const u32 run = 1 + (rest == ~0ULL ? 64u : static_cast<u32>(__builtin_ctzll(~rest)));
+    add_color_set(cv, v, run, cnt);
+  }
+}
+
+__device__ __forceinline__ u32 wave_sum(u32 x) {
+  for (u32 o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+  return x;
+}
+
+__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+struct KernelParams {
+  double threshold;
+  u32 include_not_found;
+  u32 include_invalid;
+  u32 first_read_quirk;
+};
+
+// Fused color search: one wave per read (grid-stride). COUNTS writes the raw
+// u32 counts; otherwise the read's colors with count >= ceil(seq_size * tau)
+// (ContinuousColorResultsPrinter.hpp:235-263) become a bitmap + count.
+template <bool COUNTS>
+__global__ __launch_bounds__(256) void k_color_reads(ColorView cv, const u64 *__restrict__ results,
+                                                      const u64 *__restrict__ kmer_ends, u64 n_reads,
+                                                      KernelParams p, u64 *__restrict__ bitmap,
+                                                      u64 *__restrict__ nout, u32 *__restrict__ stats,
+                                                      u32 *__restrict__ counts_out) {
+  extern __shared__ u32 lds[];
+  const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const u32 waves = blockDim.x / kWave;
+  const u32 nc = cv.num_colors;
+  u32 *cnt = lds + static_cast<u64>(wave) * nc;
+  for (u32 c = lane; c < nc; c += kWave) cnt[c] = 0;
+  wave_lds_fence();
+  const u32 words = (nc + 63) / 64;
+  for (u64 r = static_cast<u64>(blockIdx.x) * waves + wave; r < n_reads;
+       r += static_cast<u64>(gridDim.x) * waves) {
+    const u64 ks = r ? kmer_ends[r - 1] : 0, ke = kmer_ends[r];
+    u32 f = 0, nf = 0, inv = 0;
+    for (u64 base = ks; base < ke; base += kWave) {
+      const u64 i = base + lane;
+      const bool in = i < ke;
+      const u64 v = in ? results[i] : kInvalid;
+      const bool found = in && v < cv.num_nodes;
+      f += found;
+      inv += in && v == kInvalid;
+      nf += in && !found && v != kInvalid;
+      wave_add(cv, v, found, lane, cnt);
+    }
+    f = wave_sum(f);
+    nf = wave_sum(nf);
+    inv = wave_sum(inv);
+    wave_lds_fence();
+    if constexpr (COUNTS) {
+      for (u32 c = lane; c < nc; c += kWave) {
+        counts_out[r * nc + c] = cnt[c];
+        cnt[c] = 0;
+      }
+    } else {
+      u64 F = f, NF = nf, INV = inv;
+      if (p.first_read_quirk && r == 0) {
+        F -= 1;
+        NF -= 1;
+        INV -= 1;
+      }
+      const u64 seq = F + (p.include_not_found ? NF : 0) + (p.include_invalid ? INV : 0);
+      const u64 min_hits = static_cast<u64>(ceil(static_cast<double>(seq) * p.threshold));
+      u64 total = 0;
+      for (u32 w = 0; w < words; ++w) {
+        const u32 c = w * 64 + lane;
+        const bool hit = min_hits > 0 && c < nc && cnt[c] >= min_hits;
+        const u64 m = __ballot(hit);
+        if (lane == 0) bitmap[r * words + w] = m;
+        total += __popcll(m);
+        if (c < nc) cnt[c] = 0;
+      }
+      if (lane == 0) {
+        nout[r] = total;
+        if (stats) {
+          stats[3 * r] = f;
+          stats[3 * r + 1] = nf;
+          stats[3 * r + 2] = inv;
+        }
+      }
+    }
+    wave_lds_fence();
+  }
+}
+
+// Expand per-read hit bitmaps into sorted u32 color ids at CSR offsets.
+__global__ __launch_bounds__(256) void k_color_compact(const u64 *__restrict__ bitmap,
+                                                        const u64 *__restrict__ incl, u64 n_reads,
+                                                        u32 nc, u32 *__restrict__ ids,
+                                                        u64 *__restrict__ offsets) {
+  const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave, waves = blockDim.x / kWave;
+  const u32 words = (nc + 63) / 64;
+  for (u64 r = static_cast<u64>(blockIdx.x) * waves + wave; r < n_reads;
+       r += static_cast<u64>(gridDim.x) * waves) {
+    u64 pos = r ? incl[r - 1] : 0;
+    if (lane == 0) {
+      offsets[r + 1] = incl[r];
+      if (r == 0) offsets[0] = 0;
+    }
+    for (u32 w0 = 0; w0 < words; w0 += kWave) {
+      const u32 w = w0 + lane;
+      u64 x = w < words ? bitmap[r * words + w] : 0;
+      const u32 pc = static_cast<u32>(__popcll(x));
+      u32 pre = pc;  // inclusive wave scan
+      for (u32 o = 1; o < kWave; o <<= 1) {
+        const u32 y = __shfl_up(pre, o, kWave);
+        if (lane >= o) pre += y;
+      }
+      u64 at = pos + pre - pc;
+      while (x) {
+        ids[at++] = w * 64 + static_cast<u32>(__builtin_ctzll(x));
+        x &= x - 1;
+      }
+      pos += __shfl(pre, kWave - 1, kWave);
+    }
+  }
+}
+
+// Reference interface: one wave per colored read over its warps of 64
+// warp-padded indexes (u64 max = padding), u64 counts per (read, color).
+__global__ __launch_bounds__(256) void k_color_warped(ColorView cv, const u64 *__restrict__ warped,
+                                                       const u64 *__restrict__ intervals,
+                                                       u64 n_colored, u64 *__restrict__ counts) {
+  extern __shared__ u32 lds[];
+  const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave, waves = blockDim.x / kWave;
+  const u32 nc = cv.num_colors;
+  u32 *cnt = lds + static_cast<u64>(wave) * nc;
+  for (u32 c = lane; c < nc; c += kWave) cnt[c] = 0;
+  wave_lds_fence();
+  for (u64 q = static_cast<u64>(blockIdx.x) * waves + wave; q < n_colored;
+       q += static_cast<u64>(gridDim.x) * waves) {
+    for (u64 w = intervals[q]; w < intervals[q + 1]; ++w) {
+      const u64 v = warped[w * kWave + lane];
+      wave_add(cv, v, v < cv.num_nodes, lane, cnt);
+    }
+    wave_lds_fence();
+    for (u32 c = lane; c < nc; c += kWave) {
+      counts[q * nc + c] = cnt[c];
+      cnt[c] = 0;
+    }
+    wave_lds_fence();
+  }
+}
+
+template <u32 N>
+__device__ __forceinline__ u64 block_inclusive_scan(u64 v, u64 *sh) {
+  const u32 t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (u32 off = 1; off < N; off <<= 1) {
+    const u64 x = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += x;
+    __syncthreads();
+  }
+  const u64 r = sh[t];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const u64 *__restrict__ in, u64 n,
+                                                               u64 *__restrict__ sums) {
+  __shared__ u64 sh[kScanThreads];
+  const u64 first = blockIdx.x * kScanChunk + threadIdx.x * static_cast<u64>(kScanItems);
+  u64 s = 0;
+  for (u32 i = 0; i < kScanItems; ++i)
+    if (first + i < n) s += in[first + i];
+  const u64 tot = block_inclusive_scan<kScanThreads>(s, sh);
+  if (threadIdx.x == kScanThreads - 1) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_chunks(u64 *__restrict__ sums, u64 n_chunks) {
+  __shared__ u64 sh[1024];
+  __shared__ u64 tile_total;
+  u64 carry = 0;
+  for (u64 base = 0; base < n_chunks; base += 1024) {
+    const u64 i = base + threadIdx.x;
+    const u64 v = i < n_chunks ? sums[i] : 0;
+    const u64 inc = block_inclusive_scan<1024>(v, sh);
+    if (i < n_chunks) sums[i] = carry + inc - v;
+    if (threadIdx.x == 1023) tile_total = inc;
+    __syncthreads();
+    carry += tile_total;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_write(const u64 *__restrict__ in, u64 n,
+                                                              const u64 *__restrict__ offs,
+                                                              u64 *__restrict__ out) {
+  __shared__ u64 sh[kScanThreads];
+  const u64 first = blockIdx.x * kScanChunk + threadIdx.x * static_cast<u64>(kScanItems);
+  u64 v[kScanItems];
+  u64 s = 0;
+  for (u32 i = 0; i < kScanItems; ++i) {
+    v[i] = first + i < n ? in[first + i] : 0;
+    s += v[i];
+  }
+  const u64 inc = block_inclusive_scan<kScanThreads>(s, sh);
+  u64 run = offs[blockIdx.x] + inc - s;
+  for (u32 i = 0; i < kScanItems; ++i) {
+    run += v[i];
+    if (first + i < n) out[first + i] = run;
+  }
+}
+
+inline u32 grid_for(u64 items, u32 per_block) {
+  const u64 g = (items + per_block - 1) / per_block;
+  return static_cast<u32>(std::max<u64>(1, std::min<u64>(g, kMaxGrid)));
+}
+
+inline u32 waves_per_block(u64 nc) {
+  const u64 per_wave = std::max<u64>(nc, 1) * 4;
+  const u64 w = std::min<u64>(4, kLdsBytes / per_wave);
+  if (w == 0) throw std::runtime_error("more than 16384 colors is not supported");
+  return static_cast<u32>(w);
+}
+
+}  // namespace
+
+void scan_inclusive_u64(const u64 *d_in, u64 n, u64 *d_out, u64 *d_tmp, hipStream_t s) {
+  if (n == 0) return;
+  const u64 chunks = (n + kScanChunk - 1) / kScanChunk;
+  k_scan_reduce<<<static_cast<u32>(chunks), kScanThreads, 0, s>>>(d_in, n, d_tmp);
+  k_scan_chunks<<<1, 1024, 0, s>>>(d_tmp, chunks);
+  k_scan_write<<<static_cast<u32>(chunks), kScanThreads, 0, s>>>(d_in, n, d_tmp, d_out);
+  SBWT_HIP(hipGetLastError());
+}
+
+void DeviceColors::build(const HostColors &c) {
+  num_nodes = c.key_kmer_marks.bits;
+  num_color_sets = c.num_color_sets;
+  num_colors = c.num_colors;
+  if (num_colors > 0xFFFFFFFFULL) throw std::runtime_error("too many colors");
+  key_marks.build(c.key_kmer_marks.words.data(), c.key_kmer_marks.bits);
+  csi_width = c.color_set_idxs.width;
+  sparse_width = c.sparse_arrays.width;
+  std::vector<u64> csi = c.color_set_idxs.words;
+  csi.push_back(0);
+  color_set_idxs = dev_upload(csi.data(), csi.size());
+  std::vector<uint4> table(std::max<u64>(num_color_sets, 1));
+  u64 dense_rank = 0;
+  for (u64 cs = 0; cs < num_color_sets; ++cs) {
+    u64 start, end;
+    const bool dense = host_bit(c.is_dense_marks.words, cs);
+    if (dense) {
+      start = host_intv(c.dense_arrays_intervals, dense_rank);
+      end = host_intv(c.dense_arrays_intervals, dense_rank + 1);
+      ++dense_rank;
+    } else {
+      const u64 si = cs - dense_rank;
+      start = host_intv(c.sparse_arrays_intervals, si);
+      end = host_intv(c.sparse_arrays_intervals, si + 1);
+    }
+    if (end < start || end - start > 0xFFFFFFFFULL) throw std::runtime_error("malformed color set");
+    uint4 e;
+    e.x = static_cast<u32>(start);
+    e.y = static_cast<u32>(start >> 32);
+    e.z = static_cast<u32>(end - start);
+    e.w = dense ? 1u : 0u;
+    table[cs] = e;
+  }
+  set_table = dev_upload(table.data(), table.size());
+  std::vector<u64> dense = c.dense_arrays.words;
+  dense.push_back(0);
+  dense_arrays = dev_upload(dense.data(), dense.size());
+  std::vector<u64> sparse = c.sparse_arrays.words;
+  sparse.push_back(0);
+  sparse_arrays = dev_upload(sparse.data(), sparse.size());
+  device_bytes = key_marks.n_blocks * 64 + csi.size() * 8 + table.size() * 16 + dense.size() * 8 +
+                 sparse.size() * 8;
+}
+
+void DeviceColors::release() {
+  key_marks.release();
+  void *bufs[] = {color_set_idxs, set_table, dense_arrays, sparse_arrays};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  color_set_idxs = nullptr;
+  set_table = nullptr;
+  dense_arrays = nullptr;
+  sparse_arrays = nullptr;
+}
+
+ColorView DeviceColors::view() const {
+  ColorView v{};
+  v.key_marks = key_marks.blocks;
+  v.color_set_idxs = color_set_idxs;
+  v.set_table = set_table;
+  v.dense_arrays = dense_arrays;
+  v.sparse_arrays = sparse_arrays;
+  v.num_nodes = num_nodes;
+  v.num_color_sets = num_color_sets;
+  v.csi_width = csi_width;
+  v.sparse_width = sparse_width;
+  v.num_colors = static_cast<u32>(num_colors);
+  return v;
+}
+
+ColorSearcher::ColorSearcher(const DeviceColors *colors, u64 max_indexes, u64 max_reads,
+                             hipStream_t stream)
+    : colors_(colors),
+      stream_(stream),
+      own_stream_(stream == nullptr),
+      max_indexes_(max_indexes),
+      max_reads_(max_reads) {
+  (void)waves_per_block(colors_->num_colors);
+  if (own_stream_) SBWT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  const u64 words = (colors_->num_colors + 63) / 64;
+  SBWT_HIP(hipMalloc(&d_bitmap_, std::max<u64>(max_reads_ * words, 1) * 8));
+  SBWT_HIP(hipMalloc(&d_nout_, std::max<u64>(max_reads_, 1) * 8 * 2));
+  SBWT_HIP(hipMalloc(&d_scan_tmp_, (max_reads_ / kScanChunk + 2) * 8));
+  for (auto &e : ev_) SBWT_HIP(hipEventCreate(&e));
+}
+
+ColorSearcher::~ColorSearcher() {
+  for (auto &e : ev_) (void)hipEventDestroy(e);
+  void *bufs[] = {d_bitmap_, d_nout_, d_scan_tmp_, d_warped_, d_intervals_, d_counts_};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  if (own_stream_) (void)hipStreamDestroy(stream_);
+}
+
+void ColorSearcher::search_warped_host(const u64 *warped, u64 n_idxs, const u64 *warp_intervals,
+                                       u64 n_intervals, u64 *counts) {
+  if (n_idxs == 0 || n_intervals < 2) return;
+  if (n_idxs % kWave) throw std::runtime_error("warped indexes must be a multiple of 64");
+  if (n_idxs > max_indexes_ || n_intervals - 1 > max_reads_)
+    throw std::runtime_error("batch exceeds the color searcher's max indexes / reads");
+  const u64 n_colored = n_intervals - 1;
+  if (warp_intervals[n_colored] * kWave > n_idxs)
+    throw std::runtime_error("warp intervals exceed the warped indexes");
+  for (u64 q = 0; q < n_colored; ++q)
+    if (warp_intervals[q + 1] < warp_intervals[q]) throw std::runtime_error("warp intervals decrease");
+  const u64 nc = colors_->num_colors;
+  if (d_warped_) SBWT_HIP(hipFree(d_warped_));
+  if (d_intervals_) SBWT_HIP(hipFree(d_intervals_));
+  if (d_counts_) SBWT_HIP(hipFree(d_counts_));
+  SBWT_HIP(hipMalloc(&d_warped_, n_idxs * 8));
+  SBWT_HIP(hipMalloc(&d_intervals_, n_intervals * 8));
+  SBWT_HIP(hipMalloc(&d_counts_, std::max<u64>(n_colored * nc, 1) * 8));
+  SBWT_HIP(hipMemcpyAsync(d_warped_, warped, n_idxs * 8, hipMemcpyHostToDevice, stream_));
+  SBWT_HIP(hipMemcpyAsync(d_intervals_, warp_intervals, n_intervals * 8, hipMemcpyHostToDevice, stream_));
+  const u32 waves = waves_per_block(nc);
+  SBWT_HIP(hipEventRecord(ev_[0], stream_));
+  k_color_warped<<<grid_for(n_colored, waves), waves * kWave, waves * nc * 4, stream_>>>(
+      colors_->view(), d_warped_, d_intervals_, n_colored, d_counts_);
+  SBWT_HIP(hipGetLastError());
+  SBWT_HIP(hipEventRecord(ev_[1], stream_));
+  recorded_ = true;
+  SBWT_HIP(hipMemcpyAsync(counts, d_counts_, n_colored * nc * 8, hipMemcpyDeviceToHost, stream_));
+  SBWT_HIP(hipStreamSynchronize(stream_));
+}
+
+u64 ColorSearcher::pseudoalign_device(const u64 *d_results, const u64 *d_kmer_ends, u64 n_reads,
+                                      const ColorParams &p, u64 *d_offsets, u32 *d_ids,
+                                      u64 ids_capacity, u32 *d_stats) {
+  if (n_reads == 0) return 0;
+  if (n_reads > max_reads_) throw std::runtime_error("batch exceeds the color searcher's max reads");
+  const u32 nc = static_cast<u32>(colors_->num_colors);
+  const u32 waves = waves_per_block(nc);
+  KernelParams kp{p.threshold, p.include_not_found, p.include_invalid, p.first_read_quirk};
+  SBWT_HIP(hipEventRecord(ev_[0], stream_));
+  k_color_reads<false><<<grid_for(n_reads, waves), waves * kWave, waves * nc * 4, stream_>>>(
+      colors_->view(), d_results, d_kmer_ends, n_reads, kp, d_bitmap_, d_nout_, d_stats, nullptr);
+  SBWT_HIP(hipGetLastError());
+  SBWT_HIP(hipEventRecord(ev_[1], stream_));
+  recorded_ = true;
+  u64 *d_incl = d_nout_ + max_reads_;
+  scan_inclusive_u64(d_nout_, n_reads, d_incl, d_scan_tmp_, stream_);
+  u64 total = 0;
+  SBWT_HIP(hipMemcpyAsync(&total, d_incl + n_reads - 1, 8, hipMemcpyDeviceToHost, stream_));
+  SBWT_HIP(hipStreamSynchronize(stream_));
+  if (total > ids_capacity)
+    throw std::runtime_error("color id output needs " + std::to_string(total) + " entries, capacity " +
+                             std::to_string(ids_capacity));
+  k_color_compact<<<grid_for(n_reads, 4), 4 * kWave, 0, stream_>>>(d_bitmap_, d_incl, n_reads, nc,
+                                                                    d_ids, d_offsets);
+  SBWT_HIP(hipGetLastError());
+  return total;
+}
+
+void ColorSearcher::counts_device(const u64 *d_results, const u64 *d_kmer_ends, u64 n_reads,
+                                  u32 *d_counts) {
+  if (n_reads == 0) return;
+  const u32 nc = static_cast<u32>(colors_->num_colors);
+  const u32 waves = waves_per_block(nc);
+  KernelParams kp{1.0, 0, 0, 0};
+  SBWT_HIP(hipEventRecord(ev_[0], stream_));
+  k_color_reads<true><<<grid_for(n_reads, waves), waves * kWave, waves * nc * 4, stream_>>>(
+      colors_->view(), d_results, d_kmer_ends, n_reads, kp, nullptr, nullptr, nullptr, d_counts);
+  SBWT_HIP(hipGetLastError());
+  SBWT_HIP(hipEventRecord(ev_[1], stream_));
+  recorded_ = true;
+}
+
+float ColorSearcher::last_search_ms() {
+  float ms = 0;
+  if (!recorded_) return ms;
+  SBWT_HIP(hipEventSynchronize(ev_[1]));
+  SBWT_HIP(hipEventElapsedTime(&ms, ev_[0], ev_[1]));
+  return ms;
+}
+
+}  // namespace sbwt_amd

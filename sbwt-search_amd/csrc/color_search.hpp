@@ -1,0 +1,95 @@
+// Color search (pseudoalignment) on the MI355X: device color index and the
+// ColorSearcher that replaces src/ColorSearcher/ColorSearcher.{h,cpp,cu}.
+#pragma once
+
+#include "device_index.hpp"
+#include "loaders.hpp"
+
+namespace sbwt_amd {
+
+// One 16-byte entry per color set: {start lo, start hi, length, dense flag}.
+// Dense: bits [start, start+length) of dense_arrays, color c present iff bit
+// start+c. Sparse: sorted ids sparse_arrays[start .. start+length).
+// This fuses the reference's is_dense bit, is_dense rank and the two interval
+// reads (src/ColorSearcher/ColorSearcher.cuh:111-131, 208-290) into one line.
+struct ColorView {
+  const u64 *key_marks;       // mark blocks of the key-k-mer marks
+  const u64 *color_set_idxs;  // sdsl int_vector words (+ padding)
+  const uint4 *set_table;
+  const u64 *dense_arrays;
+  const u64 *sparse_arrays;
+  u64 num_nodes;
+  u64 num_color_sets;
+  u32 csi_width;
+  u32 sparse_width;
+  u32 num_colors;
+};
+
+struct DeviceColors {
+  DeviceMarks key_marks;
+  u64 *color_set_idxs = nullptr;
+  uint4 *set_table = nullptr;
+  u64 *dense_arrays = nullptr;
+  u64 *sparse_arrays = nullptr;
+  u64 num_nodes = 0, num_color_sets = 0, num_colors = 0;
+  u32 csi_width = 0, sparse_width = 0;
+  u64 device_bytes = 0;
+  void build(const HostColors &c);
+  void release();
+  ColorView view() const;
+};
+
+struct ColorParams {
+  double threshold = 1.0;     // tau, --threshold (ColorSearchArgumentParser.cpp:126-148)
+  bool include_not_found = false;
+  bool include_invalid = false;
+  // The reference's printer starts its carried per-read counts at u64 max
+  // (ContinuousColorResultsPrinter.hpp:62-64,178-180), so the first read of a
+  // stream is thresholded with found-1, not_found-1, invalid-1 (mod 2^64).
+  bool first_read_quirk = false;
+};
+
+class ColorSearcher {
+ public:
+  ColorSearcher(const DeviceColors *colors, u64 max_indexes, u64 max_reads, hipStream_t stream);
+  ~ColorSearcher();
+
+  // Reference interface: warp-padded found indexes + cumulative warp counts
+  // per colored read -> u64 counts[(n_intervals - 1) * num_colors]
+  // (ColorSearcher::search, ColorSearcher.cpp:38-58). Host buffers, synchronous.
+  void search_warped_host(const u64 *warped, u64 n_idxs, const u64 *warp_intervals,
+                          u64 n_intervals, u64 *counts);
+
+  // Fused form on device buffers: per-k-mer index results (index /
+  // kNotFound / kInvalid, after the key-k-mer move) and cumulative k-mer
+  // counts per read -> per-read thresholded color ids, sorted, as a CSR
+  // (d_offsets[n_reads + 1], d_ids), plus per-read {found, not_found, invalid}.
+  // Returns the total number of ids (synchronous on the count).
+  u64 pseudoalign_device(const u64 *d_results, const u64 *d_kmer_ends, u64 n_reads,
+                         const ColorParams &p, u64 *d_offsets, u32 *d_ids, u64 ids_capacity,
+                         u32 *d_stats);
+  // Counts only (no threshold): u32 counts[n_reads * num_colors] on device.
+  void counts_device(const u64 *d_results, const u64 *d_kmer_ends, u64 n_reads, u32 *d_counts);
+
+  hipStream_t stream() const { return stream_; }
+  float last_search_ms();
+
+ private:
+  const DeviceColors *colors_;
+  hipStream_t stream_;
+  bool own_stream_;
+  u64 max_indexes_, max_reads_;
+  u64 *d_bitmap_ = nullptr;   // [max_reads][words per read]
+  u64 *d_nout_ = nullptr;     // per read
+  u64 *d_scan_tmp_ = nullptr;
+  // staging for the reference interface
+  u64 *d_warped_ = nullptr, *d_intervals_ = nullptr, *d_counts_ = nullptr;
+  hipEvent_t ev_[2];
+  bool recorded_ = false;
+};
+
+// Inclusive scan of u64 values on the device (three kernels); tmp must hold
+// n / 4096 + 2 entries.
+void scan_inclusive_u64(const u64 *d_in, u64 n, u64 *d_out, u64 *d_tmp, hipStream_t s);
+
+}  // namespace sbwt_amd

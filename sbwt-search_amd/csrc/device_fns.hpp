@@ -1,0 +1,125 @@
+// Device functions over the MI355X index layout: rank on ACGT rank blocks and
+// mark blocks, the SBWT step, bit reads and sdsl int_vector reads.
+#pragma once
+
+#include "common.hpp"
+
+namespace sbwt_amd {
+
+// Number of 1s among the first `o` (0..95) bits of a rank-block entry.
+__device__ __forceinline__ u32 entry_popc(const uint4 e, u32 o) {
+  const u64 lo = (static_cast<u64>(e.z) << 32) | e.y;
+  const u64 mlo = o >= 64 ? ~0ULL : ((1ULL << o) - 1);
+  const u32 mhi = o > 64 ? ((1u << (o - 64)) - 1) : 0u;
+  return static_cast<u32>(__popcll(lo & mlo)) + static_cast<u32>(__popc(e.w & mhi));
+}
+
+__device__ __forceinline__ bool entry_bit(const uint4 e, u32 o) {
+  const u32 w = o < 32 ? e.y : (o < 64 ? e.z : e.w);
+  return (w >> (o & 31)) & 1u;
+}
+
+template <typename Idx>
+__device__ __forceinline__ Idx block_of(Idx i) {
+  return i / static_cast<Idx>(kBlockNodes);
+}
+
+// Absolute count before block b of letter c: narrow indexes store absolute
+// counts; wide ones add the hyperblock base.
+template <typename Idx>
+__device__ __forceinline__ Idx entry_base(const IndexView &ix, uint4 e, Idx b, u32 c) {
+  if constexpr (sizeof(Idx) == 4) {
+    return e.x;
+  } else {
+    return ix.hyper_base[(b >> kHyperShift) * 4 + c] + e.x;
+  }
+}
+
+template <typename Idx>
+__device__ __forceinline__ Idx cmap_of(const IndexView &ix, u32 c) {
+  const u64 v = c == 0 ? ix.c_map[0] : c == 1 ? ix.c_map[1] : c == 2 ? ix.c_map[2] : ix.c_map[3];
+  return static_cast<Idx>(v);
+}
+
+// rank_c over [0, i): src/UtilityKernels/Rank.cuh:26-65 restated on the
+// rank-block layout (one 16-byte load).
+template <typename Idx>
+__device__ __forceinline__ Idx rank_acgt(const IndexView &ix, u32 c, Idx i) {
+  const Idx b = block_of(i);
+  const u32 o = static_cast<u32>(i - b * kBlockNodes);
+  const uint4 e = ix.blocks[static_cast<u64>(b) * 4 + c];
+  return entry_base(ix, e, b, c) + entry_popc(e, o);
+}
+
+// One SBWT step with letter c on the interval [l, r]
+// (src/IndexSearcher/IndexSearcher.cuh:48-55). When l and r+1 fall in the
+// same block the entry is loaded once.
+template <typename Idx>
+__device__ __forceinline__ void sbwt_step(const IndexView &ix, u32 c, Idx &l, Idx &r) {
+  const Idx cm = cmap_of<Idx>(ix, c);
+  const Idx r1 = r + 1;
+  const Idx bl = block_of(l), br = block_of(r1);
+  const uint4 el = ix.blocks[static_cast<u64>(bl) * 4 + c];
+  uint4 er = el;
+  if (br != bl) er = ix.blocks[static_cast<u64>(br) * 4 + c];
+  l = cm + entry_base(ix, el, bl, c) + entry_popc(el, static_cast<u32>(l - bl * kBlockNodes));
+  r = cm + entry_base(ix, er, br, c) + entry_popc(er, static_cast<u32>(r1 - br * kBlockNodes)) - 1;
+}
+
+// rank over [0, i) on mark blocks {u64 count, 7 words}.
+__device__ __forceinline__ u64 rank_marks(const u64 *mb, u64 i) {
+  const u64 b = i / kMarkBits;
+  const u32 o = static_cast<u32>(i - b * kMarkBits);
+  const u64 *blk = mb + b * 8;
+  u64 r = blk[0];
+  const u32 full = o >> 6;
+#pragma unroll
+  for (u32 w = 0; w < 7; ++w) {
+    const u64 x = blk[1 + w];
+    if (w < full) r += __popcll(x);
+    else if (w == full && (o & 63)) r += __popcll(x & ((1ULL << (o & 63)) - 1));
+  }
+  return r;
+}
+
+__device__ __forceinline__ bool mark_bit(const u64 *mb, u64 i) {
+  const u64 b = i / kMarkBits;
+  const u32 o = static_cast<u32>(i - b * kMarkBits);
+  return (mb[b * 8 + 1 + (o >> 6)] >> (o & 63)) & 1ULL;
+}
+
+// Key-k-mer move (src/IndexSearcher/IndexSearcher.cuh:60-70): while node l
+// is not a key k-mer, follow its first outgoing edge in A,C,G,T order. A row
+// with no edge ends the walk with kWalkStuck (the reference loops forever).
+template <typename Idx>
+__device__ __forceinline__ u64 key_walk(const IndexView &ix, Idx l, u64 max_steps, u32 &walked) {
+  for (u64 s = 0; s <= max_steps; ++s, ++walked) {
+    if (mark_bit(ix.key_marks, static_cast<u64>(l))) return static_cast<u64>(l);
+    const Idx b = block_of(l);
+    const u32 o = static_cast<u32>(l - b * kBlockNodes);
+    const uint4 *blk = ix.blocks + static_cast<u64>(b) * 4;
+    u32 c = 4;
+#pragma unroll
+    for (u32 cc = 0; cc < 4; ++cc) {
+      if (c == 4 && entry_bit(blk[cc], o)) c = cc;
+    }
+    if (c == 4) return kWalkStuck;
+    const uint4 e = blk[c];
+    l = cmap_of<Idx>(ix, c) + entry_base(ix, e, b, c) + entry_popc(e, o);
+  }
+  return kWalkStuck;
+}
+
+// sdsl int_vector<> element i of width w (LSB-first, may straddle two words):
+// src/UtilityKernels/VariableLengthIntegerIndex.cuh:23-36.
+__device__ __forceinline__ u64 intv_get(const u64 *v, u32 width, u64 i) {
+  const u64 bit = i * width, w = bit >> 6;
+  const u32 off = static_cast<u32>(bit & 63);
+  u64 x = v[w] >> off;
+  if (off + width > 64) x |= v[w + 1] << (64 - off);
+  return width == 64 ? x : (x & ((1ULL << width) - 1));
+}
+
+__device__ __forceinline__ bool plain_bit(const u64 *bv, u64 i) { return (bv[i >> 6] >> (i & 63)) & 1ULL; }
+
+}  // namespace sbwt_amd

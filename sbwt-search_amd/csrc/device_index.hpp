@@ -1,0 +1,52 @@
+// Device-resident SBWT index: ACGT rank blocks, presearch table, key-k-mer
+// mark blocks. Replaces the reference's GpuSbwtContainer
+// (src/SbwtContainer/GpuSbwtContainer.cpp:16-52) and Presearcher
+// (src/Presearcher/Presearcher.cpp:18-30) with an MI355X layout.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace sbwt_amd {
+
+// Mark blocks of a single bit vector: per 448 bits one 64-byte line
+// {u64 ones before the block, 7 words}. rank over [0, i) = one line.
+struct DeviceMarks {
+  u64 *blocks = nullptr;  // [n_blocks][8]
+  u64 n_bits = 0, n_blocks = 0, total = 0;
+  void build(const u64 *words, u64 n_bits);
+  void release();
+};
+
+struct DeviceIndex {
+  int device = 0;
+  u64 n = 0;         // nodes
+  u32 k = 0;
+  u32 kp = 0;        // presearch prefix length
+  bool wide = false; // node indexes need 64 bits (n >= 2^32)
+  u64 c_map[5] = {0, 0, 0, 0, 0};
+  uint4 *blocks = nullptr;
+  u64 n_blocks = 0;
+  u64 *hyper_base = nullptr;
+  u64 n_hyper = 0;
+  void *presearch = nullptr;
+  u64 presearch_bytes = 0;
+  DeviceMarks key_marks;
+  u64 device_bytes = 0;
+
+  // acgt: four host bit vectors of ceil(n/64) words (bit i of word w = node
+  // 64w + i, as in the .sbwt file). key_marks may be null.
+  void build(const u64 *const acgt[4], u64 n, u32 k, u32 kp, const u64 *key_marks_words,
+             u64 key_marks_bits);
+  void release();
+  IndexView view() const;
+};
+
+// Presearch prefix length used when the caller does not choose one: the
+// reference's 12 letters (src/Global/GlobalDefinitions.h.in:17), or k if k < 12.
+u32 default_presearch_letters(u32 k);
+
+}  // namespace sbwt_amd

@@ -1,0 +1,474 @@
+// Index search kernels for gfx950 and the Searcher that drives them.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "device_fns.hpp"
+#include "index_search.hpp"
+
+namespace sbwt_amd {
+
+namespace {
+
+constexpr u32 kScanThreads = 256;
+constexpr u32 kScanItems = 16;
+constexpr u64 kScanChunk = static_cast<u64>(kScanThreads) * kScanItems;
+constexpr u64 kMaxGrid = 1u << 20;  // workgroups per launch; kernels grid-stride past it
+
+// ---------------------------------------------------------------------------
+// 2-bit packing: src/SeqToBitsConverter/CharToBits.h:21-40 and
+// ContinuousSeqToBitsConverter.cpp:90-107. Char j -> word j/32, bits
+// [63-2(j%32), 62-2(j%32)]; A/a 0, C/c 1, G/g 2, T/t 3; any other byte packs
+// as 0 and sets bit j%32 of invalid word j/32. One thread per output word.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void pack4(u32 bytes4, u32 j0, u64 &v, u32 &inv) {
+#pragma unroll
+  for (u32 q = 0; q < 4; ++q) {
+    const u32 ch = (bytes4 >> (8 * q)) & 0xFFu;
+    const u32 u = ch & 0xDFu;
+    const bool ok = (u == 0x41u) | (u == 0x43u) | (u == 0x47u) | (u == 0x54u);
+    const u32 code = ((u >> 1) ^ (u >> 2)) & 3u;
+    const u32 j = j0 + q;
+    v |= ok ? (static_cast<u64>(code) << (62 - 2 * j)) : 0ULL;
+    inv |= ok ? 0u : (1u << j);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_pack(const u8 *__restrict__ chars, u64 n, u64 n_words,
+                                               bool aligned, u64 *__restrict__ packed,
+                                               u32 *__restrict__ invalid) {
+  for (u64 w = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; w < n_words;
+       w += static_cast<u64>(gridDim.x) * blockDim.x) {
+    const u64 base = w * 32;
+    u64 v = 0;
+    u32 inv = 0;
+    if (aligned && base + 32 <= n) {
+      const uint4 a = *reinterpret_cast<const uint4 *>(chars + base);
+      const uint4 b = *reinterpret_cast<const uint4 *>(chars + base + 16);
+      pack4(a.x, 0, v, inv);
+      pack4(a.y, 4, v, inv);
+      pack4(a.z, 8, v, inv);
+      pack4(a.w, 12, v, inv);
+      pack4(b.x, 16, v, inv);
+      pack4(b.y, 20, v, inv);
+      pack4(b.z, 24, v, inv);
+      pack4(b.w, 28, v, inv);
+    } else {
+      for (u32 j = 0; j < 32 && base + j < n; ++j) {
+        const u32 u = chars[base + j] & 0xDFu;
+        const bool ok = (u == 0x41u) | (u == 0x43u) | (u == 0x47u) | (u == 0x54u);
+        v |= ok ? (static_cast<u64>(((u >> 1) ^ (u >> 2)) & 3u) << (62 - 2 * j)) : 0ULL;
+        inv |= ok ? 0u : (1u << j);
+      }
+    }
+    packed[w] = v;
+    invalid[w] = inv;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Cumulative k-mer counts per read (the positions of
+// src/PositionsBuilder/PositionsBuilder.cpp:15-60 made implicit): read r of
+// length len has max(0, len - k + 1) k-mers. Three-phase scan.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ u64 kmers_of_read(const u64 *ends, u64 r, u32 k) {
+  const u64 st = r ? ends[r - 1] : 0, len = ends[r] >= st ? ends[r] - st : 0;
+  return len >= k ? len - k + 1 : 0;
+}
+
+template <u32 N>
+__device__ __forceinline__ u64 block_inclusive_scan(u64 v, u64 *sh) {
+  const u32 t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (u32 off = 1; off < N; off <<= 1) {
+    const u64 x = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += x;
+    __syncthreads();
+  }
+  const u64 r = sh[t];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const u64 *__restrict__ ends, u64 n,
+                                                               u32 k, u64 *__restrict__ sums) {
+  __shared__ u64 sh[kScanThreads];
+  const u64 first = blockIdx.x * kScanChunk + threadIdx.x * static_cast<u64>(kScanItems);
+  u64 s = 0;
+  for (u32 i = 0; i < kScanItems; ++i)
+    if (first + i < n) s += kmers_of_read(ends, first + i, k);
+  const u64 tot = block_inclusive_scan<kScanThreads>(s, sh);
+  if (threadIdx.x == kScanThreads - 1) sums[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_chunks(u64 *__restrict__ sums, u64 n_chunks) {
+  __shared__ u64 sh[1024];
+  u64 carry = 0;
+  for (u64 base = 0; base < n_chunks; base += 1024) {
+    const u64 i = base + threadIdx.x;
+    const u64 v = i < n_chunks ? sums[i] : 0;
+    const u64 inc = block_inclusive_scan<1024>(v, sh);
+    if (i < n_chunks) sums[i] = carry + inc - v;
+    __shared__ u64 tile_total;
+    if (threadIdx.x == 1023) tile_total = inc;
+    __syncthreads();
+    carry += tile_total;
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_write(const u64 *__restrict__ ends, u64 n, u32 k,
+                                                              const u64 *__restrict__ offs,
+                                                              u64 *__restrict__ out) {
+  __shared__ u64 sh[kScanThreads];
+  const u64 first = blockIdx.x * kScanChunk + threadIdx.x * static_cast<u64>(kScanItems);
+  u64 v[kScanItems];
+  u64 s = 0;
+  for (u32 i = 0; i < kScanItems; ++i) {
+    v[i] = first + i < n ? kmers_of_read(ends, first + i, k) : 0;
+    s += v[i];
+  }
+  const u64 inc = block_inclusive_scan<kScanThreads>(s, sh);
+  u64 run = offs[blockIdx.x] + inc - s;
+  for (u32 i = 0; i < kScanItems; ++i) {
+    run += v[i];
+    if (first + i < n) out[first + i] = run;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Search of one k-mer whose 2k bits sit at the top of `window`:
+// src/IndexSearcher/IndexSearcher.cuh:36-71. The presearch table gives the
+// interval after kp letters; each remaining letter is one SBWT step. Once the
+// interval is empty it stays empty (l = r + 1 is preserved by the step), so
+// the loop stops there: the result is the same not-found sentinel.
+// ---------------------------------------------------------------------------
+// `steps` and `walked` count the SBWT steps and key-walk steps taken.
+template <typename Idx, bool KEY>
+__device__ __forceinline__ u64 search_window(const IndexView &ix, u64 window, u32 &steps,
+                                             u32 &walked) {
+  const u64 pre = window >> (64 - 2 * ix.kp);
+  Idx l, r;
+  if constexpr (sizeof(Idx) == 4) {
+    const uint2 p = static_cast<const uint2 *>(ix.presearch)[pre];
+    l = p.x;
+    r = p.y;
+  } else {
+    const ulonglong2 p = static_cast<const ulonglong2 *>(ix.presearch)[pre];
+    l = p.x;
+    r = p.y;
+  }
+  for (u32 j = ix.kp; j < ix.k && l <= r; ++j) {
+    const u32 c = static_cast<u32>(window >> (62 - 2 * j)) & 3u;
+    sbwt_step<Idx>(ix, c, l, r);
+    ++steps;
+  }
+  if (l > r) return kNotFound;
+  if constexpr (KEY) {
+    return key_walk<Idx>(ix, l, ix.num_nodes, walked);
+  } else {
+    return static_cast<u64>(l);
+  }
+}
+
+__device__ __forceinline__ u64 window_at(const u64 *__restrict__ packed, u64 p) {
+  const u64 w = p >> 5;
+  const u32 sh = static_cast<u32>(p & 31) * 2;
+  const u64 a = packed[w];
+  return sh ? ((a << sh) | (packed[w + 1] >> (64 - sh))) : a;
+}
+
+// Reference interface: explicit positions (IndexSearcher.cuh:22-72).
+template <typename Idx, bool KEY>
+__global__ __launch_bounds__(kSearchThreads) void k_search_positions(
+    IndexView ix, const u64 *__restrict__ packed, const u64 *__restrict__ positions, u64 n,
+    u64 *__restrict__ out) {
+  u32 steps = 0, walked = 0;
+  for (u64 i = static_cast<u64>(blockIdx.x) * kSearchThreads + threadIdx.x; i < n;
+       i += static_cast<u64>(gridDim.x) * kSearchThreads) {
+    out[i] = search_window<Idx, KEY>(ix, window_at(packed, positions[i]), steps, walked);
+  }
+}
+
+// Fused reads form: workgroup w handles k-mers [256 w, 256 w + 256). Threads 0
+// and 255 locate the reads of the first and last k-mer by binary search over
+// the cumulative k-mer counts; every lane then searches that short range. A
+// k-mer with an invalid char is kInvalid without a search
+// (ContinuousIndexResultsPrinter.hpp:199-223, 289-308). With STATS the kernel
+// also adds {k-mers searched, SBWT steps, key-walk steps, invalid k-mers} into
+// stats[0..3] (a census run for the algorithmic byte count; same results).
+template <typename Idx, bool KEY, bool STATS>
+__global__ __launch_bounds__(kSearchThreads) void k_search_reads(
+    IndexView ix, const u64 *__restrict__ packed, const u32 *__restrict__ invalid,
+    const u64 *__restrict__ char_ends, const u64 *__restrict__ kmer_ends, u64 n_reads,
+    u64 n_chars, u64 *__restrict__ out, u64 *__restrict__ stats) {
+  __shared__ u64 sh_r[2];
+  __shared__ u64 sh_stats[4];
+  if (STATS && threadIdx.x < 4) sh_stats[threadIdx.x] = 0;
+  u32 steps = 0, walked = 0, searched = 0, invalids = 0;
+  // k-mers never outnumber chars; the bound keeps malformed read ends in range
+  const u64 n_kmers = min(kmer_ends[n_reads - 1], n_chars);
+  const u64 n_tiles = (n_kmers + kSearchThreads - 1) / kSearchThreads;
+  const u32 t = threadIdx.x;
+  const u64 kmask = (ix.k >= 64) ? ~0ULL : ((1ULL << ix.k) - 1);
+  for (u64 tile = blockIdx.x; tile < n_tiles; tile += gridDim.x) {
+    const u64 g0 = tile * kSearchThreads;
+    if (t == 0 || t == kSearchThreads - 1) {
+      const u64 g = t == 0 ? g0 : min(g0 + kSearchThreads - 1, n_kmers - 1);
+      u64 lo = 0, hi = n_reads;
+      while (lo < hi) {
+        const u64 mid = (lo + hi) >> 1;
+        if (kmer_ends[mid] > g) hi = mid;
+        else lo = mid + 1;
+      }
+      sh_r[t == 0 ? 0 : 1] = lo;
+    }
+    __syncthreads();
+    const u64 g = g0 + t;
+    u64 lo = sh_r[0], hi = sh_r[1];
+    __syncthreads();
+    if (g < n_kmers) {
+      while (lo < hi) {
+        const u64 mid = (lo + hi) >> 1;
+        if (kmer_ends[mid] > g) hi = mid;
+        else lo = mid + 1;
+      }
+      const u64 r = lo;
+      const u64 kstart = r ? kmer_ends[r - 1] : 0;
+      const u64 cstart = r ? char_ends[r - 1] : 0;
+      const u64 p = cstart + (g - kstart);
+      if (p + ix.k > n_chars) {
+        out[g] = kInvalid;
+        continue;
+      }
+      const u64 iw = p >> 5;
+      const u64 ib = ((static_cast<u64>(invalid[iw + 1]) << 32) | invalid[iw]) >> (p & 31);
+      if (ib & kmask) {
+        out[g] = kInvalid;
+        ++invalids;
+      } else {
+        out[g] = search_window<Idx, KEY>(ix, window_at(packed, p), steps, walked);
+        ++searched;
+      }
+    }
+  }
+  if constexpr (STATS) {
+    __syncthreads();
+    atomicAdd(reinterpret_cast<unsigned long long *>(&sh_stats[0]), static_cast<u64>(searched));
+    atomicAdd(reinterpret_cast<unsigned long long *>(&sh_stats[1]), static_cast<u64>(steps));
+    atomicAdd(reinterpret_cast<unsigned long long *>(&sh_stats[2]), static_cast<u64>(walked));
+    atomicAdd(reinterpret_cast<unsigned long long *>(&sh_stats[3]), static_cast<u64>(invalids));
+    __syncthreads();
+    if (threadIdx.x < 4)
+      atomicAdd(reinterpret_cast<unsigned long long *>(stats + threadIdx.x), sh_stats[threadIdx.x]);
+  }
+}
+
+inline u32 grid_for(u64 items, u32 per_block) {
+  const u64 g = (items + per_block - 1) / per_block;
+  return static_cast<u32>(std::max<u64>(1, std::min<u64>(g, kMaxGrid)));
+}
+
+template <typename F>
+void dispatch_search(bool wide, bool key, F f) {
+  if (wide) {
+    if (key) f.template operator()<u64, true>();
+    else f.template operator()<u64, false>();
+  } else {
+    if (key) f.template operator()<u32, true>();
+    else f.template operator()<u32, false>();
+  }
+}
+
+template <typename F>
+void dispatch_search3(bool wide, bool key, bool stats, F f) {
+  dispatch_search(wide, key, [&]<typename Idx, bool KEY>() {
+    if (stats) f.template operator()<Idx, KEY, true>();
+    else f.template operator()<Idx, KEY, false>();
+  });
+}
+
+}  // namespace
+
+void launch_pack(const char *d_chars, u64 n_chars, u64 *d_packed, u32 *d_invalid, hipStream_t s) {
+  const u64 n_words = (n_chars + 31) / 32 + 2;  // two zero words of padding
+  const bool aligned = (reinterpret_cast<uintptr_t>(d_chars) % 16) == 0;
+  k_pack<<<grid_for(n_words, 256), 256, 0, s>>>(reinterpret_cast<const u8 *>(d_chars), n_chars,
+                                                n_words, aligned, d_packed, d_invalid);
+  SBWT_HIP(hipGetLastError());
+}
+
+Searcher::Searcher(const DeviceIndex *index, u64 max_chars, u64 max_reads, bool move_to_key_kmer,
+                   hipStream_t stream)
+    : index_(index),
+      stream_(stream),
+      own_stream_(stream == nullptr),
+      move_to_key_(move_to_key_kmer),
+      max_chars_(max_chars),
+      max_reads_(max_reads) {
+  if (move_to_key_ && !index_->key_marks.blocks)
+    throw std::runtime_error("move to key k-mer requested but the index has no key-k-mer marks");
+  if (own_stream_) SBWT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  const u64 words = max_chars_ / 32 + 3;
+  SBWT_HIP(hipMalloc(&d_packed_, words * 8));
+  SBWT_HIP(hipMalloc(&d_invalid_, words * 4));
+  SBWT_HIP(hipMalloc(&d_kmer_ends_, std::max<u64>(max_reads_, 1) * 8));
+  scan_tmp_len_ = max_reads_ / kScanChunk + 2;
+  SBWT_HIP(hipMalloc(&d_scan_tmp_, scan_tmp_len_ * 8));
+  for (auto &e : ev_) SBWT_HIP(hipEventCreate(&e));
+  SBWT_HIP(hipMalloc(&d_stats_, 4 * 8));
+}
+
+void Searcher::last_stats(u64 out[4]) {
+  SBWT_HIP(hipMemcpyAsync(out, d_stats_, 4 * 8, hipMemcpyDeviceToHost, stream_));
+  SBWT_HIP(hipStreamSynchronize(stream_));
+}
+
+Searcher::~Searcher() {
+  for (auto &e : ev_) (void)hipEventDestroy(e);
+  void *bufs[] = {d_packed_, d_invalid_, d_kmer_ends_, d_scan_tmp_, d_chars_,
+                  d_read_ends_, d_results_, d_positions_, d_bits_, d_stats_};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  if (own_stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Searcher::ensure_staging(u64 chars, u64 reads, u64 positions) {
+  if (chars > staged_chars_) {
+    if (d_chars_) SBWT_HIP(hipFree(d_chars_));
+    if (d_results_) SBWT_HIP(hipFree(d_results_));
+    if (d_bits_) SBWT_HIP(hipFree(d_bits_));
+    SBWT_HIP(hipMalloc(&d_chars_, chars + 64));
+    SBWT_HIP(hipMalloc(&d_results_, (chars + 1) * 8));
+    SBWT_HIP(hipMalloc(&d_bits_, (chars / 32 + 3) * 8));
+    staged_chars_ = chars;
+  }
+  if (reads > staged_reads_) {
+    if (d_read_ends_) SBWT_HIP(hipFree(d_read_ends_));
+    SBWT_HIP(hipMalloc(&d_read_ends_, (reads + 1) * 8));
+    staged_reads_ = reads;
+  }
+  if (positions > staged_positions_) {
+    if (d_positions_) SBWT_HIP(hipFree(d_positions_));
+    if (positions > staged_chars_) {
+      if (d_results_) SBWT_HIP(hipFree(d_results_));
+      SBWT_HIP(hipMalloc(&d_results_, (positions + 1) * 8));
+      staged_chars_ = positions;
+      if (d_chars_) SBWT_HIP(hipFree(d_chars_));
+      SBWT_HIP(hipMalloc(&d_chars_, positions + 64));
+      if (d_bits_) SBWT_HIP(hipFree(d_bits_));
+      SBWT_HIP(hipMalloc(&d_bits_, (positions / 32 + 3) * 8));
+    }
+    SBWT_HIP(hipMalloc(&d_positions_, (positions + 1) * 8));
+    staged_positions_ = positions;
+  }
+}
+
+void Searcher::search_positions_device(const u64 *d_bit_seqs, const u64 *d_positions, u64 n,
+                                       u64 *d_results) {
+  if (n == 0) return;
+  const IndexView ix = index_->view();
+  if (timing_)
+    for (int e = 0; e < 3; ++e) SBWT_HIP(hipEventRecord(ev_[e], stream_));
+  dispatch_search(index_->wide, move_to_key_, [&]<typename Idx, bool KEY>() {
+    k_search_positions<Idx, KEY><<<grid_for(n, kSearchThreads), kSearchThreads, 0, stream_>>>(
+        ix, d_bit_seqs, d_positions, n, d_results);
+  });
+  SBWT_HIP(hipGetLastError());
+  if (timing_) {
+    SBWT_HIP(hipEventRecord(ev_[3], stream_));
+    events_recorded_ = true;
+  }
+}
+
+void Searcher::search_positions_host(const u64 *bit_seqs, u64 n_words, const u64 *positions, u64 n,
+                                     u64 *results) {
+  if (n == 0) return;
+  for (u64 i = 0; i < n; ++i)
+    if (positions[i] + index_->k > n_words * 32)
+      throw std::runtime_error("k-mer position " + std::to_string(positions[i]) +
+                               " reads past the end of bit_seqs");
+  ensure_staging(std::max<u64>(n_words * 32, 32), 0, n);
+  SBWT_HIP(hipMemsetAsync(d_bits_, 0, (staged_chars_ / 32 + 3) * 8, stream_));
+  SBWT_HIP(hipMemcpyAsync(d_bits_, bit_seqs, n_words * 8, hipMemcpyHostToDevice, stream_));
+  SBWT_HIP(hipMemcpyAsync(d_positions_, positions, n * 8, hipMemcpyHostToDevice, stream_));
+  search_positions_device(d_bits_, d_positions_, n, d_results_);
+  SBWT_HIP(hipMemcpyAsync(results, d_results_, n * 8, hipMemcpyDeviceToHost, stream_));
+  SBWT_HIP(hipStreamSynchronize(stream_));
+}
+
+u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d_read_ends,
+                                  u64 n_reads, u64 *d_results, u64 *n_kmers_host) {
+  if (n_reads == 0) {
+    if (n_kmers_host) *n_kmers_host = 0;
+    return 0;
+  }
+  if (n_chars > max_chars_ || n_reads > max_reads_)
+    throw std::runtime_error("batch exceeds the searcher's max_chars_per_batch / max_reads_per_batch");
+  const IndexView ix = index_->view();
+  if (timing_) SBWT_HIP(hipEventRecord(ev_[0], stream_));
+  launch_pack(d_chars, n_chars, d_packed_, d_invalid_, stream_);
+  if (timing_) SBWT_HIP(hipEventRecord(ev_[1], stream_));
+  const u64 chunks = (n_reads + kScanChunk - 1) / kScanChunk;
+  k_scan_reduce<<<static_cast<u32>(chunks), kScanThreads, 0, stream_>>>(d_read_ends, n_reads, ix.k,
+                                                                         d_scan_tmp_);
+  k_scan_chunks<<<1, 1024, 0, stream_>>>(d_scan_tmp_, chunks);
+  k_scan_write<<<static_cast<u32>(chunks), kScanThreads, 0, stream_>>>(d_read_ends, n_reads, ix.k,
+                                                                        d_scan_tmp_, d_kmer_ends_);
+  SBWT_HIP(hipGetLastError());
+  if (timing_) SBWT_HIP(hipEventRecord(ev_[2], stream_));
+  // n_chars bounds the k-mer count; surplus workgroups leave at once
+  if (stats_) SBWT_HIP(hipMemsetAsync(d_stats_, 0, 4 * 8, stream_));
+  dispatch_search3(index_->wide, move_to_key_, stats_, [&]<typename Idx, bool KEY, bool STATS>() {
+    k_search_reads<Idx, KEY, STATS>
+        <<<grid_for(n_chars, kSearchThreads), kSearchThreads, 0, stream_>>>(
+            ix, d_packed_, d_invalid_, d_read_ends, d_kmer_ends_, n_reads, n_chars, d_results,
+            d_stats_);
+  });
+  SBWT_HIP(hipGetLastError());
+  if (timing_) {
+    SBWT_HIP(hipEventRecord(ev_[3], stream_));
+    events_recorded_ = true;
+  }
+  u64 nk = 0;
+  if (n_kmers_host) {
+    SBWT_HIP(hipMemcpyAsync(&nk, d_kmer_ends_ + (n_reads - 1), 8, hipMemcpyDeviceToHost, stream_));
+    SBWT_HIP(hipStreamSynchronize(stream_));
+    *n_kmers_host = nk;
+  }
+  return nk;
+}
+
+u64 Searcher::search_reads_host(const char *chars, u64 n_chars, const u64 *read_ends, u64 n_reads,
+                                u64 *results) {
+  u64 nk = 0, st = 0;
+  for (u64 r = 0; r < n_reads; ++r) {
+    if (read_ends[r] < st || read_ends[r] > n_chars)
+      throw std::runtime_error("read_ends must be non-decreasing and end at n_chars");
+    const u64 len = read_ends[r] - st;
+    if (len >= index_->k) nk += len - index_->k + 1;
+    st = read_ends[r];
+  }
+  if (n_reads == 0) return 0;
+  ensure_staging(std::max<u64>(n_chars, 1), n_reads, 0);
+  if (n_chars) SBWT_HIP(hipMemcpyAsync(d_chars_, chars, n_chars, hipMemcpyHostToDevice, stream_));
+  SBWT_HIP(hipMemcpyAsync(d_read_ends_, read_ends, n_reads * 8, hipMemcpyHostToDevice, stream_));
+  search_reads_device(d_chars_, n_chars, d_read_ends_, n_reads, d_results_, nullptr);
+  if (nk) SBWT_HIP(hipMemcpyAsync(results, d_results_, nk * 8, hipMemcpyDeviceToHost, stream_));
+  SBWT_HIP(hipStreamSynchronize(stream_));
+  return nk;
+}
+
+SearchTimings Searcher::last_timings() {
+  SearchTimings t;
+  if (!events_recorded_) return t;
+  SBWT_HIP(hipEventSynchronize(ev_[3]));
+  SBWT_HIP(hipEventElapsedTime(&t.pack_ms, ev_[0], ev_[1]));
+  SBWT_HIP(hipEventElapsedTime(&t.offsets_ms, ev_[1], ev_[2]));
+  SBWT_HIP(hipEventElapsedTime(&t.search_ms, ev_[2], ev_[3]));
+  return t;
+}
+
+}  // namespace sbwt_amd

@@ -1,0 +1,77 @@
+// Index search: 2-bit packing, per-read k-mer offsets and the SBWT search
+// kernels, driven by a Searcher that owns one HIP stream and its batch
+// buffers (the reference's IndexSearcher, src/IndexSearcher/IndexSearcher.h:25-46).
+#pragma once
+
+#include "device_index.hpp"
+
+namespace sbwt_amd {
+
+struct SearchTimings {
+  float pack_ms = 0, offsets_ms = 0, search_ms = 0;
+};
+
+class Searcher {
+ public:
+  Searcher(const DeviceIndex *index, u64 max_chars, u64 max_reads, bool move_to_key_kmer,
+           hipStream_t stream);
+  ~Searcher();
+
+  // Reference interface: packed 2-bit sequence + explicit k-mer start
+  // positions -> one u64 per position (node index or kNotFound), host
+  // buffers, synchronous (IndexSearcher::search, IndexSearcher.cpp:26-51).
+  void search_positions_host(const u64 *bit_seqs, u64 n_words, const u64 *positions, u64 n,
+                             u64 *results);
+  // Fused form: ASCII reads + cumulative read ends -> classified result per
+  // k-mer (index / kNotFound / kInvalid), all buffers on the device, enqueued
+  // on the searcher's stream. Returns the number of k-mers.
+  u64 search_reads_device(const char *d_chars, u64 n_chars, const u64 *d_read_ends, u64 n_reads,
+                          u64 *d_results, u64 *n_kmers_host);
+  // Same with host buffers, synchronous.
+  u64 search_reads_host(const char *chars, u64 n_chars, const u64 *read_ends, u64 n_reads,
+                        u64 *results);
+  // Device positions form (explicit positions already on the device).
+  void search_positions_device(const u64 *d_bit_seqs, const u64 *d_positions, u64 n,
+                               u64 *d_results);
+
+  hipStream_t stream() const { return stream_; }
+  SearchTimings last_timings();
+  void set_timing(bool on) { timing_ = on; }
+  // Census mode: the fused search also counts {k-mers searched, SBWT steps,
+  // key-walk steps, invalid k-mers}; last_stats reads them (synchronous).
+  void set_stats(bool on) { stats_ = on; }
+  void last_stats(u64 out[4]);
+  const u64 *kmer_ends_device() const { return d_kmer_ends_; }
+  u64 max_chars() const { return max_chars_; }
+  u64 max_reads() const { return max_reads_; }
+
+ private:
+  void ensure_staging(u64 chars, u64 reads, u64 positions);
+  const DeviceIndex *index_;
+  hipStream_t stream_;
+  bool own_stream_;
+  bool move_to_key_;
+  bool timing_ = true;
+  bool stats_ = false;
+  u64 *d_stats_ = nullptr;
+  u64 max_chars_, max_reads_;
+  u64 *d_packed_ = nullptr;
+  u32 *d_invalid_ = nullptr;
+  u64 *d_kmer_ends_ = nullptr;
+  u64 *d_scan_tmp_ = nullptr;
+  u64 scan_tmp_len_ = 0;
+  // staging for host-buffer calls
+  char *d_chars_ = nullptr;
+  u64 *d_read_ends_ = nullptr;
+  u64 *d_results_ = nullptr;
+  u64 *d_positions_ = nullptr;
+  u64 *d_bits_ = nullptr;
+  u64 staged_chars_ = 0, staged_reads_ = 0, staged_positions_ = 0;
+  hipEvent_t ev_[4];
+  bool events_recorded_ = false;
+};
+
+// Device kernels exposed for reuse by the color stage and tests.
+void launch_pack(const char *d_chars, u64 n_chars, u64 *d_packed, u32 *d_invalid, hipStream_t s);
+
+}  // namespace sbwt_amd

@@ -1,0 +1,195 @@
+"""Index search on the MI355X through the C ABI, bit-exact against the oracle.
+
+Every test prints the case it runs before touching the GPU.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+import sbwt_amd as S
+import workloads as W
+from conftest import FIXTURES
+
+pytestmark = pytest.mark.gpu
+
+INDEX_FIXTURES = ["fasta1.fna", "fasta2.fna", "fasta3.fna", "fasta4.fna", "fastq1.fnq"]
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def fixture_pair():
+    path = os.path.join(FIXTURES, "search_test_index.sbwt")
+    return S.Index.load(path), O.OracleSbwt(path)
+
+
+@pytest.mark.parametrize("name", INDEX_FIXTURES)
+def test_reference_fixtures_byte_identical(fixture_pair, name):
+    say(f"fixture {name}")
+    idx, orc = fixture_pair
+    chars, ends = O.read_seqfile(os.path.join(FIXTURES, "index_search", name))
+    s = idx.searcher(max(len(chars), 1), max(len(ends), 1))
+    res = s.search_reads(chars, ends)
+    assert np.array_equal(res, orc.index_reads(chars, ends))
+    out = O.ascii_index_output(res, ends, idx.k)
+    exp = open(os.path.join(FIXTURES, "index_search", "expected",
+                            name.split(".")[0] + ".txt"), "rb").read()
+    assert out == exp
+
+
+@pytest.fixture(scope="module")
+def genome_index(tmp_path_factory):
+    g = W.genome_random(0, 200_000)
+    codes, ends = W.concat([g])
+    b = W.BuiltSbwt(codes, ends, 31)
+    path = str(tmp_path_factory.mktemp("idx") / "g.sbwt")
+    b.write(path)
+    return g, b, path, O.OracleSbwt(path)
+
+
+def _reads(g, n, length, seed, random_frac=0.5, err=0.01, n_rate=0.001):
+    lens = np.full(n, length, np.uint32) if np.isscalar(length) else np.asarray(length, np.uint32)
+    chars, ends, _ = W.reads(seed, g, np.array([len(g)], np.uint64), lens, random_frac, err,
+                             n_rate)
+    return chars.tobytes(), ends
+
+
+@pytest.mark.parametrize("kp", [12, 1, 7, 10, 14])
+def test_reads_parity_presearch_widths(genome_index, kp):
+    g, b, path, orc = genome_index
+    say(f"reads parity kp={kp}")
+    idx = S.Index.load(path, presearch_letters=kp)
+    chars, ends = _reads(g, 3000, 150, seed=kp)
+    s = idx.searcher(len(chars), len(ends))
+    res = s.search_reads(chars, ends)
+    exp = orc.index_reads(chars, ends, kp=12)
+    assert len(res) == len(exp) == 3000 * 120
+    assert np.array_equal(res, exp)
+    found = np.count_nonzero(res < O.INVALID)
+    assert found > 0.2 * len(res)
+
+
+def test_positions_interface_parity(genome_index):
+    """The reference's IndexSearcher::search interface: bit_seqs + positions."""
+    g, b, path, orc = genome_index
+    say("positions interface")
+    idx = S.Index.load(path)
+    chars, ends = _reads(g, 500, 150, seed=3)
+    packed, _ = O.pack(chars)
+    pos = O.positions(ends, 31)
+    rng = np.random.default_rng(1)
+    pos = np.ascontiguousarray(rng.permutation(pos))
+    s = idx.searcher(len(chars), len(ends))
+    res = s.search(packed, pos)
+    assert np.array_equal(res, orc.search(packed, pos, kp=12))
+    with pytest.raises(S.SbwtError):
+        s.search(packed[:2], np.array([1000], np.uint64))
+
+
+def test_ragged_reads_and_edge_cases(genome_index):
+    g, b, path, orc = genome_index
+    say("ragged reads")
+    idx = S.Index.load(path)
+    lens = [0, 1, 30, 31, 32, 33, 63, 64, 65, 200, 1000, 5000, 0, 31]
+    chars, ends = _reads(g, len(lens), lens, seed=5, random_frac=0.3, n_rate=0.01)
+    chars = bytearray(chars)
+    # lower case, all-N and high bytes inside some reads
+    st = int(ends[8])
+    chars[st:st + 20] = chars[st:st + 20].lower()
+    st = int(ends[9])
+    chars[st + 100:st + 140] = b"N" * 40
+    chars[st + 500] = 0xC1
+    chars = bytes(chars)
+    s = idx.searcher(len(chars) + 100, len(ends) + 10)
+    res = s.search_reads(chars, ends)
+    assert np.array_equal(res, orc.index_reads(chars, ends))
+    say("empty batch")
+    assert len(s.search_reads(b"", np.zeros(0, np.uint64))) == 0
+    say("reads all shorter than k")
+    r2 = s.search_reads(b"ACGT" * 10, np.array([5, 10, 40], np.uint64))
+    assert len(r2) == 10 and np.array_equal(r2, orc.index_reads(b"ACGT" * 10,
+                                                                np.array([5, 10, 40], np.uint64)))
+
+
+def test_searcher_reuse_shrinking_batches(genome_index):
+    g, b, path, orc = genome_index
+    say("searcher reuse")
+    idx = S.Index.load(path)
+    s = idx.searcher(2_000_000, 20_000)
+    for n, seed in [(5000, 1), (10, 2), (1200, 3)]:
+        chars, ends = _reads(g, n, 150, seed=seed)
+        assert np.array_equal(s.search_reads(chars, ends), orc.index_reads(chars, ends))
+    with pytest.raises(S.SbwtError):
+        chars, ends = _reads(g, 30_000, 150, seed=9)
+        s.search_reads(chars, ends)
+
+
+def test_wide_index_kernels(genome_index, monkeypatch):
+    """64-bit node-index kernels (needed from 2^32 nodes) on a small index."""
+    g, b, path, orc = genome_index
+    say("forced 64-bit index")
+    monkeypatch.setenv("SBWT_AMD_FORCE_WIDE", "1")
+    idx = S.Index.load(path)
+    assert idx.wide
+    chars, ends = _reads(g, 2000, 150, seed=21)
+    s = idx.searcher(len(chars), len(ends))
+    assert np.array_equal(s.search_reads(chars, ends), orc.index_reads(chars, ends))
+
+
+def test_create_from_bit_vectors_matches_load(genome_index):
+    g, b, path, orc = genome_index
+    say("create from bit vectors")
+    idx = S.Index.create([b.bits(c) for c in range(4)], b.n, 31)
+    assert idx.c_map == orc.c_map()
+    chars, ends = _reads(g, 1000, 150, seed=8)
+    s = idx.searcher(len(chars), len(ends))
+    assert np.array_equal(s.search_reads(chars, ends), orc.index_reads(chars, ends))
+
+
+@pytest.mark.parametrize("d", [1, 3, 20])
+def test_key_kmer_move_parity(d, tmp_path):
+    """-k: move every found k-mer to its key k-mer (IndexSearcher.cuh:60-70)."""
+    say(f"key k-mer walk d={d}")
+    root = W.genome_random(1, 30_000)
+    gens = [root] + [W.genome_mutate(root, s, 0.02) for s in (2, 3, 4)]
+    codes, ends = W.concat(gens)
+    b = W.BuiltSbwt(codes, ends, 31)
+    col = b.colors(np.arange(len(gens), dtype=np.uint32), d=d)
+    sp, cp = str(tmp_path / "k.sbwt"), str(tmp_path / "k.tcolors")
+    b.write(sp)
+    col.write(cp)
+    orc = O.OracleSbwt(sp)
+    km = col.key_marks()
+    idx = S.Index.load(sp, cp)
+    assert idx.has_key_marks
+    chars, ends2, _ = W.reads(5, codes, ends, np.full(2000, 120, np.uint32), 0.2, 0.01, 0.001)
+    chars = chars.tobytes()
+    s = idx.searcher(len(chars), len(ends2), move_to_key_kmer=True)
+    res = s.search_reads(chars, ends2)
+    exp = orc.index_reads(chars, ends2, key_marks=np.concatenate([km, np.zeros(8, np.uint64)]))
+    assert np.array_equal(res, exp)
+    assert not np.any(res == O.WALK_STUCK)
+
+
+def test_device_buffers_with_torch(genome_index):
+    """Device-resident form: torch allocates, the library computes."""
+    import torch
+    g, b, path, orc = genome_index
+    say("device buffers via torch")
+    idx = S.Index.load(path)
+    chars, ends = _reads(g, 4000, 150, seed=33)
+    dc = torch.frombuffer(bytearray(chars), dtype=torch.uint8).cuda()
+    de = torch.from_numpy(ends.astype(np.int64)).cuda()
+    dr = torch.empty(len(chars), dtype=torch.int64, device="cuda")
+    s = idx.searcher(len(chars), len(ends))
+    nk = s.search_reads_device(dc.data_ptr(), len(chars), de.data_ptr(), len(ends),
+                               dr.data_ptr(), want_count=True)
+    assert nk == 4000 * 120
+    got = dr[:nk].cpu().numpy().view(np.uint64)
+    assert np.array_equal(got, orc.index_reads(chars, ends))
+    t = s.last_timings()
+    assert t["search_ms"] > 0
