@@ -44,6 +44,7 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--presearch-letters", type=int, default=12)
+    ap.add_argument("--algorithm", default="streaming", choices=["streaming", "per-kmer"])
     return ap.parse_args()
 
 
@@ -81,7 +82,8 @@ def main():
     log(f"SBWT: {built.n} nodes, {built.n_kmers} k-mers ({time.time() - t0:.1f} s)")
     bits = [built.bits(c) for c in range(4)]
     t0 = time.time()
-    idx = S.Index.create(bits, built.n, configs.K, presearch_letters=args.presearch_letters)
+    idx = S.Index.create(bits, built.n, configs.K, presearch_letters=args.presearch_letters,
+                         suffix_group_starts=built.bits(4))
     log(f"device index: {idx.device_bytes / 2**20:.1f} MiB, wide={idx.wide} "
         f"({time.time() - t0:.1f} s)")
 
@@ -91,6 +93,7 @@ def main():
     d_out = torch.empty(wl.n_chars, dtype=torch.int64, device=dev)
     n_kmers = wl.n_kmers()
     s = idx.searcher(wl.n_chars, wl.n_reads)
+    s.set_algorithm(S.ALGO_STREAMING if args.algorithm == "streaming" else S.ALGO_PER_KMER)
 
     def step():
         s.search_reads_device(d_chars.data_ptr(), wl.n_chars, d_ends.data_ptr(), wl.n_reads,
@@ -102,9 +105,14 @@ def main():
     s.sync()
     stats = s.last_stats()
     s.set_stats(False)
-    assert stats["searched"] + stats["invalid"] == n_kmers, stats
-    algo_bytes = (LINE * (stats["searched"] + stats["steps"]) + 3 * LINE * stats["walk_steps"]
-                  + 8 * n_kmers + 0.375 * wl.n_chars)
+    resolved = (stats["full_searches"] + stats["streamed"] + stats["invalid"]
+                - stats["stream_fallbacks"])
+    assert resolved == n_kmers, stats
+    # one 64-byte line per presearch lookup, SBWT step and streamed step, two per
+    # key-walk step (mark + rank block); u64 output; packed input and invalid mask
+    lines = (stats["full_searches"] + stats["steps"] + stats["streamed"]
+             + stats["stream_extra_lines"] + 2 * stats["walk_steps"])
+    algo_bytes = LINE * lines + 8 * n_kmers + 0.375 * wl.n_chars
 
     for _ in range(args.warmup):
         step()
@@ -152,6 +160,7 @@ def main():
             "chars_per_step": wl.n_chars,
             "found_fraction": found / n_kmers,
             "presearch_letters": idx.presearch_letters,
+            "algorithm": args.algorithm,
             "search_kernel_ms": kern_ms,
             "reads_per_s": wl.n_reads / (elapsed / args.steps),
         },

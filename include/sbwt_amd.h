@@ -51,15 +51,17 @@ int sbwt_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
  * (ceil(num_bits/64) words each). Builds the rank layout, the c-map and the
  * presearch table of presearch_letters letters (0 = the reference's 12, or k
  * if k < 12). key_kmer_marks (nullable, ceil(num_bits/64) words) enables the
- * key-k-mer move.
+ * key-k-mer move. suffix_group_starts (nullable, the .sbwt file's vector of
+ * the same name) lets the streaming search step from nodes that do not start
+ * their suffix group; without it such steps fall back to a full search.
  * Replaces GpuSbwtContainer(acgt, poppy, c_map, bits_total, bit_vector_size,
  * kmer_size, key_kmer_marks) (src/SbwtContainer/GpuSbwtContainer.h:33-41,
  * GpuSbwtContainer.cpp:16-52) plus Presearcher::presearch()
  * (src/Presearcher/Presearcher.h:26-32, Presearcher.cpp:18-30); the c-map is
  * computed as in SbwtBuilder.cpp:111,128-130. */
 int sbwt_index_create(const uint64_t *const acgt[4], uint64_t num_bits, uint32_t kmer_size,
-                      const uint64_t *key_kmer_marks, uint32_t presearch_letters,
-                      sbwt_index **out);
+                      const uint64_t *key_kmer_marks, const uint64_t *suffix_group_starts,
+                      uint32_t presearch_letters, sbwt_index **out);
 
 /* Device index from a .sbwt/.tdbg plain-matrix v0.1 file and, if colors_path
  * is not null, the key-k-mer marks of a .tcolors sdsl-hybrid-v4 file.
@@ -138,12 +140,24 @@ int sbwt_searcher_last_timings(sbwt_searcher *searcher, float *pack_ms, float *o
                                float *search_ms);
 
 /* Census mode for sbwt_searcher_search_reads(_device): when enabled, the next
- * calls also count stats[0] k-mers searched, stats[1] SBWT steps taken (one
- * rank-block line each), stats[2] key-walk steps and stats[3] invalid k-mers,
- * the exact algorithmic work of the batch. Results are unchanged. No
- * counterpart in the reference (its kernels keep no counters). */
+ * calls also count the exact algorithmic work of the batch: stats[0] full
+ * searches (one presearch line each), stats[1] their SBWT steps (one
+ * rank-block line each), stats[2] key-walk steps, stats[3] invalid k-mers,
+ * stats[4] k-mers stepped from the previous k-mer's node (one line each),
+ * stats[5] extra lines of those steps, stats[6] of those steps that fell back
+ * to a full search. Results are unchanged. No counterpart in the reference
+ * (its kernels keep no counters). */
 int sbwt_searcher_set_stats(sbwt_searcher *searcher, int enable);
-int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[4]);
+int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[8]);
+
+/* Search algorithm of the reads path: SBWT_ALGO_STREAMING (default) steps each
+ * k-mer of a read from the previous k-mer's node and searches from the
+ * presearch table only after a miss; SBWT_ALGO_PER_KMER searches every k-mer
+ * independently, as the reference's d_search does (IndexSearcher.cuh:22-72).
+ * Both return identical results. No counterpart in the reference. */
+#define SBWT_ALGO_STREAMING 0
+#define SBWT_ALGO_PER_KMER 1
+int sbwt_searcher_set_algorithm(sbwt_searcher *searcher, int algorithm);
 
 /* Device 2-bit packing alone: chars -> ceil(n/32)+2 words (MSB-first) and
  * ceil(n/32)+2 u32 invalid masks (bit j%32 of word j/32 set if char j is not

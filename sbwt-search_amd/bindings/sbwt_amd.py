@@ -14,6 +14,8 @@ REPO = os.path.dirname(PKG)
 LIB_PATH = os.path.join(PKG, "csrc", "libsbwt_amd.so")
 HEADER = os.path.join(REPO, "include", "sbwt_amd.h")
 
+ALGO_STREAMING = 0
+ALGO_PER_KMER = 1
 NOT_FOUND = 0xFFFFFFFFFFFFFFFF
 INVALID = 0xFFFFFFFFFFFFFFFE
 WALK_STUCK = 0xFFFFFFFFFFFFFFFC
@@ -55,8 +57,8 @@ def lib():
         vp = C.c_void_p
         L.sbwt_last_error.restype = C.c_char_p
         L.sbwt_mem_info.argtypes = [u64p, u64p]
-        L.sbwt_index_create.argtypes = [u64p * 4, C.c_uint64, C.c_uint32, u64p, C.c_uint32,
-                                        C.POINTER(vp)]
+        L.sbwt_index_create.argtypes = [u64p * 4, C.c_uint64, C.c_uint32, u64p, u64p,
+                                        C.c_uint32, C.POINTER(vp)]
         L.sbwt_index_load.argtypes = [C.c_char_p, C.c_char_p, C.c_uint32, C.POINTER(vp)]
         L.sbwt_index_destroy.argtypes = [vp]
         L.sbwt_index_destroy.restype = None
@@ -93,6 +95,7 @@ def lib():
         L.sbwt_color_searcher_last_time.argtypes = [vp, C.POINTER(C.c_float)]
         L.sbwt_searcher_set_stats.argtypes = [vp, C.c_int]
         L.sbwt_searcher_last_stats.argtypes = [vp, u64p]
+        L.sbwt_searcher_set_algorithm.argtypes = [vp, C.c_int]
         _lib = L
     return _lib
 
@@ -144,12 +147,16 @@ class Index:
         return cls(h)
 
     @classmethod
-    def create(cls, acgt, num_bits, k, key_marks=None, presearch_letters=0):
+    def create(cls, acgt, num_bits, k, key_marks=None, presearch_letters=0,
+               suffix_group_starts=None):
         keep = [np.ascontiguousarray(a, np.uint64) for a in acgt]
         arr = (u64p * 4)(*[_p64(a) for a in keep])
         km = np.ascontiguousarray(key_marks, np.uint64) if key_marks is not None else None
+        sg = np.ascontiguousarray(suffix_group_starts, np.uint64) \
+            if suffix_group_starts is not None else None
         h = C.c_void_p()
         _check(lib().sbwt_index_create(arr, num_bits, k, _p64(km) if km is not None else None,
+                                       _p64(sg) if sg is not None else None,
                                        presearch_letters, C.byref(h)))
         return cls(h)
 
@@ -228,10 +235,15 @@ class Searcher:
         _check(lib().sbwt_searcher_set_stats(self.h, int(enable)))
 
     def last_stats(self):
-        out = np.zeros(4, np.uint64)
+        out = np.zeros(8, np.uint64)
         _check(lib().sbwt_searcher_last_stats(self.h, _p64(out)))
-        return {"searched": int(out[0]), "steps": int(out[1]), "walk_steps": int(out[2]),
-                "invalid": int(out[3])}
+        return {"full_searches": int(out[0]), "steps": int(out[1]), "walk_steps": int(out[2]),
+                "invalid": int(out[3]), "streamed": int(out[4]), "stream_extra_lines": int(out[5]),
+                "stream_fallbacks": int(out[6])}
+
+    def set_algorithm(self, algorithm):
+        """0 = streaming (default), 1 = per-k-mer (the reference's search shape)."""
+        _check(lib().sbwt_searcher_set_algorithm(self.h, int(algorithm)))
 
     def sync(self):
         _check(lib().sbwt_searcher_sync(self.h))

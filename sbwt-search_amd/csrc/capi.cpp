@@ -85,8 +85,8 @@ int sbwt_mem_info(uint64_t *free_bytes, uint64_t *total_bytes) {
 }
 
 int sbwt_index_create(const uint64_t *const acgt[4], uint64_t num_bits, uint32_t kmer_size,
-                      const uint64_t *key_kmer_marks, uint32_t presearch_letters,
-                      sbwt_index **out) {
+                      const uint64_t *key_kmer_marks, const uint64_t *suffix_group_starts,
+                      uint32_t presearch_letters, sbwt_index **out) {
   return guarded([&] {
     require(out != nullptr && acgt != nullptr, "null argument");
     require(num_bits >= 1, "an SBWT has at least one node");
@@ -95,7 +95,8 @@ int sbwt_index_create(const uint64_t *const acgt[4], uint64_t num_bits, uint32_t
     const uint32_t kp = presearch_letters ? presearch_letters : sbwt_amd::default_presearch_letters(kmer_size);
     require(kp >= 1 && kp <= kmer_size && kp <= 14, "presearch letters must be in [1, min(k, 14)]");
     auto idx = std::make_unique<sbwt_index>();
-    idx->dev.build(acgt, num_bits, kmer_size, kp, key_kmer_marks, key_kmer_marks ? num_bits : 0);
+    idx->dev.build(acgt, num_bits, kmer_size, kp, key_kmer_marks, key_kmer_marks ? num_bits : 0,
+                   suffix_group_starts);
     *out = idx.release();
   });
 }
@@ -116,7 +117,7 @@ int sbwt_index_load(const char *sbwt_path, const char *colors_path, uint32_t pre
     require(kp >= 1 && kp <= k && kp <= 14, "presearch letters must be in [1, min(k, 14)]");
     auto idx = std::make_unique<sbwt_index>();
     idx->dev.build(acgt, s.num_bits, k, kp, colors_path ? km.words.data() : nullptr,
-                   colors_path ? km.bits : 0);
+                   colors_path ? km.bits : 0, s.suffix_group_starts.data());
     *out = idx.release();
   });
 }
@@ -214,7 +215,15 @@ int sbwt_searcher_set_stats(sbwt_searcher *searcher, int enable) {
   });
 }
 
-int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[4]) {
+int sbwt_searcher_set_algorithm(sbwt_searcher *searcher, int algorithm) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    require(algorithm == SBWT_ALGO_STREAMING || algorithm == SBWT_ALGO_PER_KMER, "unknown algorithm");
+    searcher->impl->set_algorithm(algorithm);
+  });
+}
+
+int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[8]) {
   return guarded([&] {
     require(searcher && stats, "null argument");
     searcher->impl->last_stats(stats);

@@ -88,8 +88,4 @@ class ColorSearcher {
   bool recorded_ = false;
 };
 
-// Inclusive scan of u64 values on the device (three kernels); tmp must hold
-// n / 4096 + 2 entries.
-void scan_inclusive_u64(const u64 *d_in, u64 n, u64 *d_out, u64 *d_tmp, hipStream_t s);
-
 }  // namespace sbwt_amd

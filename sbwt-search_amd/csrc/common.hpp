@@ -45,6 +45,7 @@ struct IndexView {
   const u64 *hyper_base;      // [n_hyper][4] absolute count at hyperblock start
   const void *presearch;      // [4^kp] of {left,right} (uint2 or ulonglong2)
   const u64 *key_marks;       // mark blocks of the key-k-mer marks, or null
+  const u64 *sgs;             // suffix-group starts (plain bit vector), or null
   u64 c_map[5];
   u64 num_nodes;
   u32 k;

@@ -96,7 +96,7 @@ void DeviceMarks::release() {
 }
 
 void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const u64 *km_words,
-                        u64 km_bits) {
+                        u64 km_bits, const u64 *sgs_words) {
   SBWT_HIP(hipGetDevice(&device));
   n = n_;
   k = k_;
@@ -142,6 +142,14 @@ void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const
     key_marks.build(km_words, km_bits);
     device_bytes += key_marks.n_blocks * 64;
   }
+  if (sgs_words) {
+    std::vector<u64> w((n + 63) / 64 + 2, 0);
+    for (u64 i = 0; i < (n + 63) / 64; ++i) w[i] = sgs_words[i];
+    if (n % 64) w[n / 64] &= (1ULL << (n % 64)) - 1;
+    w[0] |= 1;  // the root starts the first group
+    sgs = upload(w.data(), w.size());
+    device_bytes += w.size() * 8;
+  }
   // presearch table on the device
   const u64 count = 1ULL << (2 * kp);
   const IndexView ix = view();
@@ -166,6 +174,8 @@ void DeviceIndex::release() {
   if (blocks) (void)hipFree(blocks);
   if (hyper_base) (void)hipFree(hyper_base);
   if (presearch) (void)hipFree(presearch);
+  if (sgs) (void)hipFree(sgs);
+  sgs = nullptr;
   blocks = nullptr;
   hyper_base = nullptr;
   presearch = nullptr;
@@ -178,6 +188,7 @@ IndexView DeviceIndex::view() const {
   v.hyper_base = hyper_base;
   v.presearch = presearch;
   v.key_marks = key_marks.blocks;
+  v.sgs = sgs;
   for (int i = 0; i < 5; ++i) v.c_map[i] = c_map[i];
   v.num_nodes = n;
   v.k = k;

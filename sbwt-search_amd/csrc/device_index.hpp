@@ -35,12 +35,14 @@ struct DeviceIndex {
   void *presearch = nullptr;
   u64 presearch_bytes = 0;
   DeviceMarks key_marks;
+  u64 *sgs = nullptr;  // suffix-group starts, when known
   u64 device_bytes = 0;
 
   // acgt: four host bit vectors of ceil(n/64) words (bit i of word w = node
-  // 64w + i, as in the .sbwt file). key_marks may be null.
+  // 64w + i, as in the .sbwt file). key_marks and sgs_words (the suffix-group
+  // starts, which let the streaming search step from any node) may be null.
   void build(const u64 *const acgt[4], u64 n, u32 k, u32 kp, const u64 *key_marks_words,
-             u64 key_marks_bits);
+             u64 key_marks_bits, const u64 *sgs_words);
   void release();
   IndexView view() const;
 };

@@ -5,6 +5,7 @@
 
 #include "device_fns.hpp"
 #include "index_search.hpp"
+#include "scan.hpp"
 
 namespace sbwt_amd {
 
@@ -266,6 +267,234 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_reads(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Streaming search with persistent lanes. Work items are chunks of up to
+// kChunkKmers consecutive k-mers of one read. A lane walks its chunk k-mer by
+// k-mer; when k-mer x was found at node v, the next k-mer y = x[1..k-1] c is
+// one SBWT step from the suffix group of v: if v carries edge c, y is node
+// C[c] + rank_c(v) (one rank-block line); if v carries another edge, y is
+// absent; if v carries none, the group start gs (the last set bit of the
+// suffix-group starts at or before v) decides the same way. A k-mer that
+// follows an absent, invalid or chunk-initial one is searched from its
+// presearch interval as in search_window. Results equal the per-k-mer
+// search: each k-mer's node is unique. When a lane finishes its chunk it
+// takes the next one from a device-wide counter (one atomic per wave).
+// ---------------------------------------------------------------------------
+constexpr u32 kChunkKmers = 256;
+
+__device__ __forceinline__ u32 lanes_below(u64 mask) {
+  return __builtin_amdgcn_mbcnt_hi(static_cast<u32>(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo(static_cast<u32>(mask), 0u));
+}
+
+// Last suffix-group start at or before v (bit 0 is always set).
+__device__ __forceinline__ u64 group_start(const u64 *__restrict__ sgs, u64 v) {
+  u64 w = v >> 6;
+  u64 x = sgs[w] & ((2ULL << (v & 63)) - 1);  // bits at or below v (all for v % 64 == 63)
+  while (x == 0) x = sgs[--w];
+  return (w << 6) + 63 - static_cast<u64>(__builtin_clzll(x));
+}
+
+enum StreamStep : u32 { kStepFound = 0, kStepAbsent = 1, kStepUnknown = 2 };
+
+// Successor of found node v by letter c; `lines` counts extra lines read.
+template <typename Idx>
+__device__ __forceinline__ u32 stream_step(const IndexView &ix, Idx v, u32 c, Idx &next, u32 &lines) {
+  const Idx b = block_of(v);
+  const u32 o = static_cast<u32>(v - b * kBlockNodes);
+  const uint4 *blk = ix.blocks + static_cast<u64>(b) * 4;
+  const uint4 e = blk[c];
+  if (entry_bit(e, o)) {
+    next = cmap_of<Idx>(ix, c) + entry_base(ix, e, b, c) + entry_popc(e, o);
+    return kStepFound;
+  }
+  bool any = false;
+#pragma unroll
+  for (u32 cc = 0; cc < 4; ++cc)
+    if (cc != c) any |= entry_bit(blk[cc], o);
+  if (any) return kStepAbsent;  // v leads its group and has no edge c
+  if (!ix.sgs) return kStepUnknown;
+  const Idx gs = static_cast<Idx>(group_start(ix.sgs, static_cast<u64>(v)));
+  ++lines;
+  if (gs == v) return kStepAbsent;  // a group without outgoing edges
+  const Idx b2 = block_of(gs);
+  const u32 o2 = static_cast<u32>(gs - b2 * kBlockNodes);
+  const uint4 e2 = ix.blocks[static_cast<u64>(b2) * 4 + c];
+  ++lines;
+  if (!entry_bit(e2, o2)) return kStepAbsent;
+  next = cmap_of<Idx>(ix, c) + entry_base(ix, e2, b2, c) + entry_popc(e2, o2);
+  return kStepFound;
+}
+
+// Per read: number of chunks.
+__global__ __launch_bounds__(256) void k_chunk_count(const u64 *__restrict__ kmer_ends, u64 n_reads,
+                                                      u64 *__restrict__ counts) {
+  for (u64 r = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; r < n_reads;
+       r += static_cast<u64>(gridDim.x) * blockDim.x) {
+    const u64 nk = kmer_ends[r] - (r ? kmer_ends[r - 1] : 0);
+    counts[r] = (nk + kChunkKmers - 1) / kChunkKmers;
+  }
+}
+
+// Per read: its chunks {first k-mer (output slot), char position | count << 48}.
+__global__ __launch_bounds__(256) void k_chunk_fill(const u64 *__restrict__ kmer_ends,
+                                                     const u64 *__restrict__ char_ends, u64 n_reads,
+                                                     const u64 *__restrict__ chunk_ends,
+                                                     ulonglong2 *__restrict__ chunks) {
+  for (u64 r = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; r < n_reads;
+       r += static_cast<u64>(gridDim.x) * blockDim.x) {
+    const u64 ks = r ? kmer_ends[r - 1] : 0, nk = kmer_ends[r] - ks;
+    const u64 cs = r ? char_ends[r - 1] : 0;
+    u64 at = r ? chunk_ends[r - 1] : 0;
+    for (u64 off = 0; off < nk; off += kChunkKmers, ++at) {
+      const u64 cnt = min(static_cast<u64>(kChunkKmers), nk - off);
+      ulonglong2 ch;
+      ch.x = ks + off;
+      ch.y = (cs + off) | (cnt << 48);
+      chunks[at] = ch;
+    }
+  }
+}
+
+__global__ void k_reset_counter(u64 *c) { *c = 0; }
+
+template <typename Idx, bool KEY, bool STATS>
+__global__ __launch_bounds__(kSearchThreads) void k_search_stream(
+    IndexView ix, const u64 *__restrict__ packed, const u32 *__restrict__ invalid,
+    const ulonglong2 *__restrict__ chunks, const u64 *__restrict__ chunk_ends, u64 n_reads,
+    u64 *__restrict__ counter, u64 *__restrict__ out, u64 *__restrict__ stats) {
+  enum : u32 { kIdle = 0, kNew = 1, kSearch = 2, kDone = 3 };
+  const u64 n_chunks = chunk_ends[n_reads - 1];
+  const u32 lane = threadIdx.x & 63;
+  const u64 inv_mask = ix.k >= 64 ? ~0ULL : ((1ULL << ix.k) - 1);
+  u32 mode = kIdle;
+  u64 g = 0, p = 0, window = 0;
+  u32 left = 0, j = 0;
+  Idx l = 0, r = 0, v = 0;
+  bool prev = false;
+  u32 st_full = 0, st_steps = 0, st_walk = 0, st_inv = 0, st_stream = 0, st_extra = 0, st_fb = 0;
+  for (;;) {
+    const u64 idle = __ballot(mode == kIdle);
+    if (idle) {
+      u64 base = 0;
+      if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long *>(counter),
+                                      static_cast<unsigned long long>(__popcll(idle)));
+      base = __shfl(base, 0, 64);
+      if (mode == kIdle) {
+        const u64 ci = base + lanes_below(idle);
+        if (ci < n_chunks) {
+          const ulonglong2 ch = chunks[ci];
+          g = ch.x;
+          p = ch.y & ((1ULL << 48) - 1);
+          left = static_cast<u32>(ch.y >> 48);
+          prev = false;
+          mode = kNew;
+        } else {
+          mode = kDone;
+        }
+      }
+    }
+    if (__ballot(mode != kDone) == 0) break;
+    u64 result = 0;
+    bool emit = false;
+    if (mode == kNew) {
+      window = window_at(packed, p);
+      const u64 iw = p >> 5;
+      const u64 ib = ((static_cast<u64>(invalid[iw + 1]) << 32) | invalid[iw]) >> (p & 31);
+      bool full = true;
+      if (ib & inv_mask) {
+        result = kInvalid;
+        emit = true;
+        prev = false;
+        full = false;
+        if (STATS) ++st_inv;
+      } else if (prev) {
+        const u32 c = static_cast<u32>(window >> (64 - 2 * ix.k)) & 3u;
+        Idx nx = 0;
+        u32 extra = 0;
+        const u32 how = stream_step<Idx>(ix, v, c, nx, extra);
+        if (STATS) {
+          ++st_stream;
+          st_extra += extra;
+        }
+        if (how == kStepFound) {
+          v = nx;
+          if constexpr (KEY) {
+            u32 w = 0;
+            result = key_walk<Idx>(ix, v, ix.num_nodes, w);
+            if (STATS) st_walk += w;
+          } else {
+            result = static_cast<u64>(v);
+          }
+          emit = true;
+          full = false;
+        } else if (how == kStepAbsent) {
+          result = kNotFound;
+          emit = true;
+          prev = false;
+          full = false;
+        } else if (STATS) {
+          ++st_fb;
+        }
+      }
+      if (full) {
+        const u64 pre = window >> (64 - 2 * ix.kp);
+        if constexpr (sizeof(Idx) == 4) {
+          const uint2 pr = static_cast<const uint2 *>(ix.presearch)[pre];
+          l = pr.x;
+          r = pr.y;
+        } else {
+          const ulonglong2 pr = static_cast<const ulonglong2 *>(ix.presearch)[pre];
+          l = pr.x;
+          r = pr.y;
+        }
+        j = ix.kp;
+        if (STATS) ++st_full;
+        mode = kSearch;
+      }
+    } else if (mode == kSearch) {
+      const u32 c = static_cast<u32>(window >> (62 - 2 * j)) & 3u;
+      sbwt_step<Idx>(ix, c, l, r);
+      ++j;
+      if (STATS) ++st_steps;
+    }
+    if (mode == kSearch) {  // a search just started or stepped: resolve if done
+      if (l > r) {
+        result = kNotFound;
+        emit = true;
+        prev = false;
+      } else if (j == ix.k) {
+        v = l;
+        prev = true;
+        if constexpr (KEY) {
+          u32 w = 0;
+          result = key_walk<Idx>(ix, v, ix.num_nodes, w);
+          if (STATS) st_walk += w;
+        } else {
+          result = static_cast<u64>(v);
+        }
+        emit = true;
+      }
+    }
+    if (emit) {
+      out[g] = result;
+      ++g;
+      ++p;
+      --left;
+      mode = left ? kNew : kIdle;
+    }
+  }
+  if constexpr (STATS) {
+    atomicAdd(reinterpret_cast<unsigned long long *>(stats + 0), static_cast<u64>(st_full));
+    atomicAdd(reinterpret_cast<unsigned long long *>(stats + 1), static_cast<u64>(st_steps));
+    atomicAdd(reinterpret_cast<unsigned long long *>(stats + 2), static_cast<u64>(st_walk));
+    atomicAdd(reinterpret_cast<unsigned long long *>(stats + 3), static_cast<u64>(st_inv));
+    atomicAdd(reinterpret_cast<unsigned long long *>(stats + 4), static_cast<u64>(st_stream));
+    atomicAdd(reinterpret_cast<unsigned long long *>(stats + 5), static_cast<u64>(st_extra));
+    atomicAdd(reinterpret_cast<unsigned long long *>(stats + 6), static_cast<u64>(st_fb));
+  }
+}
+
 inline u32 grid_for(u64 items, u32 per_block) {
   const u64 g = (items + per_block - 1) / per_block;
   return static_cast<u32>(std::max<u64>(1, std::min<u64>(g, kMaxGrid)));
@@ -318,18 +547,28 @@ Searcher::Searcher(const DeviceIndex *index, u64 max_chars, u64 max_reads, bool 
   scan_tmp_len_ = max_reads_ / kScanChunk + 2;
   SBWT_HIP(hipMalloc(&d_scan_tmp_, scan_tmp_len_ * 8));
   for (auto &e : ev_) SBWT_HIP(hipEventCreate(&e));
-  SBWT_HIP(hipMalloc(&d_stats_, 4 * 8));
+  SBWT_HIP(hipMalloc(&d_stats_, 8 * 8));
+  max_chunks_ = max_chars_ / kChunkKmers + max_reads_ + 1;
+  SBWT_HIP(hipMalloc(&d_chunks_, max_chunks_ * 16));
+  SBWT_HIP(hipMalloc(&d_chunk_counts_, std::max<u64>(max_reads_, 1) * 8 * 2));
+  SBWT_HIP(hipMalloc(&d_chunk_scan_tmp_, scan_tmp_entries(max_reads_) * 8));
+  SBWT_HIP(hipMalloc(&d_counter_, 8));
+  int dev = 0, cus = 0;
+  SBWT_HIP(hipGetDevice(&dev));
+  SBWT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  persistent_blocks_ = static_cast<u32>(std::max(1, cus) * 8);
 }
 
-void Searcher::last_stats(u64 out[4]) {
-  SBWT_HIP(hipMemcpyAsync(out, d_stats_, 4 * 8, hipMemcpyDeviceToHost, stream_));
+void Searcher::last_stats(u64 out[8]) {
+  SBWT_HIP(hipMemcpyAsync(out, d_stats_, 8 * 8, hipMemcpyDeviceToHost, stream_));
   SBWT_HIP(hipStreamSynchronize(stream_));
 }
 
 Searcher::~Searcher() {
   for (auto &e : ev_) (void)hipEventDestroy(e);
-  void *bufs[] = {d_packed_, d_invalid_, d_kmer_ends_, d_scan_tmp_, d_chars_,
-                  d_read_ends_, d_results_, d_positions_, d_bits_, d_stats_};
+  void *bufs[] = {d_packed_, d_invalid_, d_kmer_ends_, d_scan_tmp_, d_chars_, d_read_ends_,
+                  d_results_, d_positions_, d_bits_, d_stats_, d_chunks_, d_chunk_counts_,
+                  d_chunk_scan_tmp_, d_counter_};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (own_stream_) (void)hipStreamDestroy(stream_);
@@ -419,14 +658,31 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
                                                                         d_scan_tmp_, d_kmer_ends_);
   SBWT_HIP(hipGetLastError());
   if (timing_) SBWT_HIP(hipEventRecord(ev_[2], stream_));
-  // n_chars bounds the k-mer count; surplus workgroups leave at once
-  if (stats_) SBWT_HIP(hipMemsetAsync(d_stats_, 0, 4 * 8, stream_));
-  dispatch_search3(index_->wide, move_to_key_, stats_, [&]<typename Idx, bool KEY, bool STATS>() {
-    k_search_reads<Idx, KEY, STATS>
-        <<<grid_for(n_chars, kSearchThreads), kSearchThreads, 0, stream_>>>(
-            ix, d_packed_, d_invalid_, d_read_ends, d_kmer_ends_, n_reads, n_chars, d_results,
-            d_stats_);
-  });
+  if (stats_) SBWT_HIP(hipMemsetAsync(d_stats_, 0, 8 * 8, stream_));
+  if (algorithm_ == kAlgoStreaming) {
+    // chunk table of (read, k-mer range) work items, then the persistent search
+    u64 *d_chunk_ends = d_chunk_counts_ + max_reads_;
+    k_chunk_count<<<grid_for(n_reads, 256), 256, 0, stream_>>>(d_kmer_ends_, n_reads, d_chunk_counts_);
+    scan_inclusive_u64(d_chunk_counts_, n_reads, d_chunk_ends, d_chunk_scan_tmp_, stream_);
+    k_chunk_fill<<<grid_for(n_reads, 256), 256, 0, stream_>>>(d_kmer_ends_, d_read_ends, n_reads,
+                                                              d_chunk_ends, d_chunks_);
+    k_reset_counter<<<1, 1, 0, stream_>>>(d_counter_);
+    SBWT_HIP(hipGetLastError());
+    if (timing_) SBWT_HIP(hipEventRecord(ev_[2], stream_));
+    dispatch_search3(index_->wide, move_to_key_, stats_, [&]<typename Idx, bool KEY, bool STATS>() {
+      k_search_stream<Idx, KEY, STATS><<<persistent_blocks_, kSearchThreads, 0, stream_>>>(
+          ix, d_packed_, d_invalid_, d_chunks_, d_chunk_ends, n_reads, d_counter_, d_results,
+          d_stats_);
+    });
+  } else {
+    // n_chars bounds the k-mer count; surplus workgroups leave at once
+    dispatch_search3(index_->wide, move_to_key_, stats_, [&]<typename Idx, bool KEY, bool STATS>() {
+      k_search_reads<Idx, KEY, STATS>
+          <<<grid_for(n_chars, kSearchThreads), kSearchThreads, 0, stream_>>>(
+              ix, d_packed_, d_invalid_, d_read_ends, d_kmer_ends_, n_reads, n_chars, d_results,
+              d_stats_);
+    });
+  }
   SBWT_HIP(hipGetLastError());
   if (timing_) {
     SBWT_HIP(hipEventRecord(ev_[3], stream_));

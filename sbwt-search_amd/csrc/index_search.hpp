@@ -7,6 +7,12 @@
 
 namespace sbwt_amd {
 
+// Search algorithms of the fused reads path: streaming persistent lanes
+// (consecutive k-mers of a read step from the previous k-mer's node) or one
+// independent search per k-mer (the reference's per-thread search).
+constexpr int kAlgoStreaming = 0;
+constexpr int kAlgoPerKmer = 1;
+
 struct SearchTimings {
   float pack_ms = 0, offsets_ms = 0, search_ms = 0;
 };
@@ -37,10 +43,14 @@ class Searcher {
   hipStream_t stream() const { return stream_; }
   SearchTimings last_timings();
   void set_timing(bool on) { timing_ = on; }
-  // Census mode: the fused search also counts {k-mers searched, SBWT steps,
-  // key-walk steps, invalid k-mers}; last_stats reads them (synchronous).
+  // Census mode: the fused search also counts, in stats[0..6], full searches
+  // (presearch lines), their SBWT steps, key-walk steps, invalid k-mers,
+  // streamed k-mers, extra lines of streamed steps and streamed steps that fell
+  // back to a full search; last_stats reads them (synchronous).
   void set_stats(bool on) { stats_ = on; }
-  void last_stats(u64 out[4]);
+  void set_algorithm(int a) { algorithm_ = a; }
+  int algorithm() const { return algorithm_; }
+  void last_stats(u64 out[8]);
   const u64 *kmer_ends_device() const { return d_kmer_ends_; }
   u64 max_chars() const { return max_chars_; }
   u64 max_reads() const { return max_reads_; }
@@ -53,7 +63,14 @@ class Searcher {
   bool move_to_key_;
   bool timing_ = true;
   bool stats_ = false;
+  int algorithm_ = kAlgoStreaming;
   u64 *d_stats_ = nullptr;
+  u64 max_chunks_ = 0;
+  ulonglong2 *d_chunks_ = nullptr;
+  u64 *d_chunk_counts_ = nullptr;  // [counts | inclusive chunk ends]
+  u64 *d_chunk_scan_tmp_ = nullptr;
+  u64 *d_counter_ = nullptr;
+  u32 persistent_blocks_ = 0;
   u64 max_chars_, max_reads_;
   u64 *d_packed_ = nullptr;
   u32 *d_invalid_ = nullptr;

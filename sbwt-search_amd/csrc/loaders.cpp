@@ -82,7 +82,11 @@ HostSbwt load_sbwt(const std::string &path) {
     in.words(s.acgt[c], bits);
   }
   for (int i = 0; i < 4; ++i) in.skip_bit_vector();  // rank structures (SbwtBuilder.cpp:97)
-  in.skip_bit_vector();                               // suffix group starts
+  {  // suffix group starts
+    const uint64_t bits = in.u64();
+    if (bits != s.num_bits) throw std::runtime_error("Malformed SBWT: suffix group starts length");
+    in.words(s.suffix_group_starts, bits);
+  }
   in.skip(in.u64());                                  // C map
   in.skip(in.u64());                                  // kmer_prefix_calc
   in.skip(24);                                        // precalc_k, n_nodes, n_kmers

@@ -15,6 +15,7 @@ struct HostSbwt {
   uint64_t num_bits = 0;
   uint64_t k = 0;
   std::vector<uint64_t> acgt[4];
+  std::vector<uint64_t> suffix_group_starts;  // bit i set: node i starts a suffix group
 };
 
 struct HostIntVector {

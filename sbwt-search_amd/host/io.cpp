@@ -1,0 +1,327 @@
+// Host I/O of the sbwt_search command line (see io.hpp).
+#include "io.hpp"
+
+#include <zlib.h>
+
+#include <algorithm>
+#include <charconv>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <stdexcept>
+#include <sys/stat.h>
+
+namespace sbwt_host {
+
+namespace {
+constexpr u64 kNotFound = ~0ULL;
+constexpr u64 kInvalid = ~0ULL - 1;
+constexpr u64 kNewline = ~0ULL - 2;
+constexpr size_t kFlush = 1 << 24;
+
+void write_all(FILE *f, const void *p, size_t n) {
+  if (n && fwrite(p, 1, n, f) != n) throw std::runtime_error("write failed");
+}
+}  // namespace
+
+void write_sized_string(FILE *f, const std::string &s) {
+  const u64 n = s.size();
+  write_all(f, &n, 8);
+  write_all(f, s.data(), n);
+}
+
+// ------------------------------------------------------------------ reader
+SeqReader::SeqReader(const std::string &path) : buf_(1 << 20) {
+  gzFile g = gzopen(path.c_str(), "rb");
+  if (!g) throw std::runtime_error("Unable to open file " + path);
+  gzbuffer(g, 1 << 20);
+  gz_ = g;
+}
+
+SeqReader::~SeqReader() {
+  if (gz_) gzclose(static_cast<gzFile>(gz_));
+}
+
+int SeqReader::getc() {
+  if (pos_ == len_) {
+    if (eof_) return -1;
+    const int n = gzread(static_cast<gzFile>(gz_), buf_.data(), static_cast<unsigned>(buf_.size()));
+    if (n <= 0) {
+      eof_ = true;
+      return -1;
+    }
+    len_ = static_cast<size_t>(n);
+    pos_ = 0;
+  }
+  return static_cast<unsigned char>(buf_[pos_++]);
+}
+
+bool SeqReader::next(std::vector<char> &chars) {
+  int c;
+  if (!started_) {
+    started_ = true;
+    while ((c = getc()) != -1 && c != '>' && c != '@') {
+    }
+    if (c == -1) return false;
+  } else if (pending_ == -1) {
+    return false;
+  }
+  while ((c = getc()) != -1 && c != '\n') {  // header line
+  }
+  u64 seq_len = 0;
+  pending_ = -1;
+  if (c == -1) return true;
+  for (;;) {
+    c = getc();
+    if (c == -1) break;
+    if (c == '>' || c == '@') {
+      pending_ = c;
+      break;
+    }
+    if (c == '+') {  // FASTQ quality
+      while ((c = getc()) != -1 && c != '\n') {
+      }
+      u64 q = 0;
+      while (c != -1 && q < seq_len) {
+        while ((c = getc()) != -1 && c != '\n') ++q;
+      }
+      while ((c = getc()) != -1 && c != '>' && c != '@') {
+      }
+      pending_ = c;
+      break;
+    }
+    if (c == '\n') continue;
+    chars.push_back(static_cast<char>(c));
+    ++seq_len;
+    while ((c = getc()) != -1 && c != '\n') {
+      chars.push_back(static_cast<char>(c));
+      ++seq_len;
+    }
+    if (c == -1) break;
+  }
+  return true;
+}
+
+// ------------------------------------------------------------ index writer
+IndexWriter::IndexWriter(const std::string &path, const std::string &mode, bool headers) {
+  static const char *ext[3] = {".txt", ".bin", ".bool"};
+  static const char *fmt[3] = {"ascii", "binary", "bool"};
+  static const char *ver[3] = {"v1.0", "v1.0", "v2.0"};
+  if (mode == "ascii") mode_ = 0;
+  else if (mode == "binary") mode_ = 1;
+  else if (mode == "bool") mode_ = 2;
+  else throw std::runtime_error("Invalid value passed by user for argument print_mode");
+  f_ = fopen((path + ext[mode_]).c_str(), "wb");
+  if (!f_) throw std::runtime_error("Unable to open file " + path + ext[mode_]);
+  if (headers) {
+    write_sized_string(f_, fmt[mode_]);
+    write_sized_string(f_, ver[mode_]);
+  }
+  out_.reserve(kFlush + 64);
+}
+
+IndexWriter::~IndexWriter() {
+  try {
+    flush();
+  } catch (...) {
+  }
+  fclose(f_);
+}
+
+void IndexWriter::result(u64 v) {
+  if (mode_ == 0) {
+    if (!first_in_line_) out_.push_back(' ');
+    if (v == kInvalid) {
+      out_.push_back('-');
+      out_.push_back('2');
+    } else if (v == kNotFound) {
+      out_.push_back('-');
+      out_.push_back('1');
+    } else {
+      char tmp[24];
+      auto r = std::to_chars(tmp, tmp + sizeof tmp, v);
+      out_.insert(out_.end(), tmp, r.ptr);
+    }
+  } else if (mode_ == 1) {
+    const char *p = reinterpret_cast<const char *>(&v);
+    out_.insert(out_.end(), p, p + 8);
+  } else {
+    out_.push_back(v == kInvalid ? '2' : v == kNotFound ? '1' : '0');
+  }
+  first_in_line_ = false;
+  if (out_.size() >= kFlush) flush();
+}
+
+void IndexWriter::end_read() {
+  if (mode_ == 1) {
+    const u64 v = kNewline;
+    const char *p = reinterpret_cast<const char *>(&v);
+    out_.insert(out_.end(), p, p + 8);
+  } else {
+    out_.push_back('\n');
+  }
+  first_in_line_ = true;
+  if (out_.size() >= kFlush) flush();
+}
+
+void IndexWriter::flush() {
+  write_all(f_, out_.data(), out_.size());
+  out_.clear();
+}
+
+// ------------------------------------------------------------ color writer
+ColorWriter::ColorWriter(const std::string &path, const std::string &mode, bool headers,
+                         u64 num_colors)
+    : num_colors_(num_colors) {
+  static const char *ext[3] = {".txt", ".bin", ".csv"};
+  if (mode == "ascii") mode_ = 0;
+  else if (mode == "binary") mode_ = 1;
+  else if (mode == "csv") mode_ = 2;
+  else throw std::runtime_error("Invalid value passed by user for argument print_mode");
+  f_ = fopen((path + ext[mode_]).c_str(), "wb");
+  if (!f_) throw std::runtime_error("Unable to open file " + path + ext[mode_]);
+  if (mode_ == 2) {  // always written (CsvContinuousColorResultsPrinter.cpp:60-75)
+    std::string h;
+    for (u64 i = 1; i <= num_colors_; ++i) {
+      h += std::to_string(i);
+      h += i == num_colors_ ? '\n' : ',';
+    }
+    write_all(f_, h.data(), h.size());
+  } else if (headers) {
+    write_sized_string(f_, mode_ == 0 ? "ascii" : "binary");
+    write_sized_string(f_, "v1.0");
+  }
+  out_.reserve(kFlush + 64);
+}
+
+ColorWriter::~ColorWriter() {
+  try {
+    flush();
+  } catch (...) {
+  }
+  fclose(f_);
+}
+
+void ColorWriter::read(const u32 *ids, u64 n) {
+  if (mode_ == 0) {
+    for (u64 i = 0; i < n; ++i) {
+      if (i) out_.push_back(' ');
+      char tmp[16];
+      auto r = std::to_chars(tmp, tmp + sizeof tmp, ids[i]);
+      out_.insert(out_.end(), tmp, r.ptr);
+    }
+    out_.push_back('\n');
+  } else if (mode_ == 1) {
+    for (u64 i = 0; i < n; ++i) {
+      const u64 v = ids[i];
+      const char *p = reinterpret_cast<const char *>(&v);
+      out_.insert(out_.end(), p, p + 8);
+    }
+    const u64 end = ~0ULL;
+    const char *p = reinterpret_cast<const char *>(&end);
+    out_.insert(out_.end(), p, p + 8);
+  } else {
+    const size_t at = out_.size();
+    for (u64 c = 0; c < num_colors_; ++c) {
+      out_.push_back('0');
+      out_.push_back(c + 1 == num_colors_ ? '\n' : ',');
+    }
+    for (u64 i = 0; i < n; ++i) out_[at + 2 * ids[i]] = '1';
+  }
+  if (out_.size() >= kFlush) flush();
+}
+
+void ColorWriter::flush() {
+  write_all(f_, out_.data(), out_.size());
+  out_.clear();
+}
+
+// ----------------------------------------------------- index file reading
+IndexFile read_index_file(const std::string &path) {
+  std::ifstream in(path, std::ios::binary);
+  if (!in) throw std::runtime_error("Unable to open file " + path);
+  std::vector<char> data((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+  size_t p = 0;
+  auto sized = [&]() {
+    u64 n;
+    if (p + 8 > data.size()) throw std::runtime_error("Invalid file format in file: " + path);
+    std::memcpy(&n, data.data() + p, 8);
+    p += 8;
+    if (n > 64 || p + n > data.size()) throw std::runtime_error("Invalid file format in file: " + path);
+    std::string s(data.data() + p, n);
+    p += n;
+    return s;
+  };
+  const std::string fmt = sized();
+  const std::string ver = sized();
+  if (ver != "v1.0") throw std::runtime_error("The file has an incompatible version number");
+  IndexFile out;
+  if (fmt == "ascii") {  // AsciiIndexFileParser.cpp:33-70
+    u64 n = 0;
+    while (p < data.size()) {
+      const char c = data[p++];
+      if (c == '-') {
+        const char d = p < data.size() ? data[p++] : 0;
+        if (d == '1') out.results.push_back(kNotFound);
+        else if (d == '2') out.results.push_back(kInvalid);
+        while (p < data.size() && data[p] >= '0') ++p;
+        ++n;
+      } else if (c >= '0') {
+        u64 v = static_cast<u64>(c - '0');
+        while (p < data.size() && data[p] >= '0') v = v * 10 + static_cast<u64>(data[p++] - '0');
+        out.results.push_back(v);
+        ++n;
+      } else if (c == '\n') {
+        out.kmer_ends.push_back(out.results.size());
+      }
+    }
+    (void)n;
+  } else if (fmt == "binary") {  // BinaryIndexFileParser.cpp:33-63
+    while (p + 8 <= data.size()) {
+      u64 v;
+      std::memcpy(&v, data.data() + p, 8);
+      p += 8;
+      if (v == kNewline) out.kmer_ends.push_back(out.results.size());
+      else out.results.push_back(v);
+    }
+  } else {
+    throw std::runtime_error("Invalid file format in file: " + path);
+  }
+  return out;
+}
+
+std::vector<std::string> expand_list(const std::string &name) {
+  const std::string ext = ".list";
+  if (name.size() >= ext.size() && name.compare(name.size() - ext.size(), ext.size(), ext) == 0) {
+    std::ifstream in(name);
+    if (!in) throw std::runtime_error("Unable to open file " + name);
+    std::vector<std::string> out;
+    std::string line;
+    while (std::getline(in, line)) out.push_back(line);
+    return out;
+  }
+  return {name};
+}
+
+u64 file_size(const std::string &path) {
+  struct stat st;
+  if (stat(path.c_str(), &st) != 0) return 0;
+  return static_cast<u64>(st.st_size);
+}
+
+std::vector<std::vector<u64>> partition_by_size(const std::vector<std::string> &files, u64 parts) {
+  std::map<u64, std::vector<u64>> by_size;
+  for (u64 i = 0; i < files.size(); ++i) by_size[file_size(files[i])].push_back(i);
+  std::vector<std::vector<u64>> out(parts);
+  std::vector<u64> load(parts, 0);
+  for (auto it = by_size.rbegin(); it != by_size.rend(); ++it) {
+    for (u64 f : it->second) {
+      const u64 s = std::min_element(load.begin(), load.end()) - load.begin();
+      out[s].push_back(f);
+      load[s] += it->first;
+    }
+  }
+  return out;
+}
+
+}  // namespace sbwt_host

@@ -1,0 +1,97 @@
+// Host I/O of the sbwt_search command line: sequence files, index-result files
+// and the output formats of the reference's printers.
+#pragma once
+
+#include <cstdint>
+#include <cstdio>
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace sbwt_host {
+
+using u32 = uint32_t;
+using u64 = uint64_t;
+
+// FASTA/FASTQ reader (optionally gzipped). The reference delegates to
+// kseqpp_REad v1.6.0 (src/SequenceFileParser/ContinuousSequenceFileParser.cpp
+// :125,152); the record grammar is kseq's: skip to the first '>' or '@', the
+// header is the rest of that line, the sequence is the concatenation of the
+// following lines up to a line starting with '>', '@' or '+'; after '+' the
+// quality lines are consumed until they are as long as the sequence.
+class SeqReader {
+ public:
+  explicit SeqReader(const std::string &path);
+  ~SeqReader();
+  // Appends the next record's sequence to `chars`; false at end of file.
+  bool next(std::vector<char> &chars);
+
+ private:
+  int getc();
+  void *gz_ = nullptr;
+  std::vector<char> buf_;
+  size_t pos_ = 0, len_ = 0;
+  bool eof_ = false;
+  bool started_ = false;
+  int pending_ = -1;  // first char of the next header, already read
+};
+
+// A batch of whole reads (or pieces of one long read).
+struct ReadBatch {
+  std::vector<char> chars;
+  std::vector<u64> ends;         // cumulative read ends
+  std::vector<u64> piece_of;     // per entry: read number in the file
+  bool continues_previous = false;  // first entry continues the last read of the previous batch
+};
+
+// Output writers for the index stage (src/IndexResultsPrinter/*):
+// ascii v1.0, binary v1.0, bool v2.0, headers = u64 length + string.
+class IndexWriter {
+ public:
+  IndexWriter(const std::string &path, const std::string &mode, bool headers);
+  ~IndexWriter();
+  void result(u64 v);   // one k-mer result
+  void end_read();      // read separator
+  void flush();
+
+ private:
+  FILE *f_;
+  int mode_;  // 0 ascii, 1 binary, 2 bool
+  bool first_in_line_ = true;
+  std::vector<char> out_;
+};
+
+// Output writers for the color stage (src/ColorResultsPrinter/*): ascii v1.0
+// ("c1 c2 ...\n"), binary v1.0 (u64 ids, u64 max per read), csv (header
+// 1..num_colors, one 0/1 row per read).
+class ColorWriter {
+ public:
+  ColorWriter(const std::string &path, const std::string &mode, bool headers, u64 num_colors);
+  ~ColorWriter();
+  void read(const u32 *ids, u64 n);
+  void flush();
+
+ private:
+  FILE *f_;
+  int mode_;  // 0 ascii, 1 binary, 2 csv
+  u64 num_colors_;
+  std::vector<char> out_;
+};
+
+// Index-stage result file (ascii or binary, with headers) read back as one
+// result per k-mer (index, u64 max not found, u64 max - 1 invalid) and the
+// number of k-mers of every read: src/IndexFileParser/{Ascii,Binary}IndexFileParser.cpp.
+struct IndexFile {
+  std::vector<u64> results;
+  std::vector<u64> kmer_ends;
+};
+IndexFile read_index_file(const std::string &path);
+
+void write_sized_string(FILE *f, const std::string &s);
+std::vector<std::string> expand_list(const std::string &name);
+u64 file_size(const std::string &path);
+// LPT partition of files into streams by size, largest first
+// (src/FilesizeLoadBalancer/FilesizeLoadBalancer.cpp:23-55).
+std::vector<std::vector<u64>> partition_by_size(const std::vector<std::string> &files, u64 parts);
+
+}  // namespace sbwt_host

@@ -1,0 +1,285 @@
+// sbwt_search: the drop-in command line of the MI355X engine.
+//   sbwt_search index  -q <fasta/q(.gz)|.list> -i <.sbwt|.tdbg> -o <prefix|.list>
+//                      [-k <.tcolors>] [-p ascii|binary|bool] [--no-headers] [-s streams]
+//                      [--batch-chars N]
+//   sbwt_search colors -q <index results|.list> -k <.tcolors> -o <prefix|.list>
+//                      [-p ascii|binary|csv] [-t threshold] [--include-not-found]
+//                      [--include-invalid] [--no-headers] [-s streams]
+// Option names and defaults follow src/ArgumentParser/IndexSearchArgumentParser.cpp:29-178
+// and ColorSearchArgumentParser.cpp:27-180; the memory options (-u -m -c -g -r)
+// are accepted and bound the batch sizes.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/sbwt_amd.h"
+#include "io.hpp"
+
+using sbwt_host::u32;
+using sbwt_host::u64;
+
+namespace {
+
+struct Args {
+  std::string mode;
+  std::map<std::string, std::string> kv;
+  std::map<std::string, bool> flags;
+  std::string get(const std::string &k, const std::string &def = "") const {
+    auto it = kv.find(k);
+    return it == kv.end() ? def : it->second;
+  }
+  bool flag(const std::string &k) const { return flags.count(k) > 0; }
+};
+
+const std::map<std::string, std::string> kShort = {
+    {"q", "query-file"},          {"i", "index-file"},      {"o", "output-prefix"},
+    {"u", "unavailable-main-memory"}, {"m", "max-main-memory"}, {"c", "cpu-memory-percentage"},
+    {"r", "base-pairs-per-seq"},  {"g", "gpu-memory-percentage"}, {"s", "streams"},
+    {"p", "print-mode"},          {"k", "colors-file"},     {"t", "threshold"},
+    {"h", "help"}};
+const char *kFlags[] = {"no-headers", "include-not-found", "include-invalid", "help"};
+
+Args parse(int argc, char **argv) {
+  Args a;
+  if (argc < 2) throw std::runtime_error("usage: sbwt_search index|colors [options]");
+  a.mode = argv[1];
+  for (int i = 2; i < argc; ++i) {
+    std::string s = argv[i], key;
+    if (s.rfind("--", 0) == 0) key = s.substr(2);
+    else if (s.size() == 2 && s[0] == '-' && kShort.count(s.substr(1))) key = kShort.at(s.substr(1));
+    else continue;  // unrecognised options are allowed, as in the reference
+    std::string val;
+    const size_t eq = key.find('=');
+    if (eq != std::string::npos) {
+      val = key.substr(eq + 1);
+      key = key.substr(0, eq);
+      a.kv[key] = val;
+      continue;
+    }
+    bool is_flag = false;
+    for (const char *f : kFlags) is_flag |= key == f;
+    if (is_flag) {
+      a.flags[key] = true;
+    } else if (i + 1 < argc) {
+      a.kv[key] = argv[++i];
+    }
+  }
+  return a;
+}
+
+void check(int rc) {
+  if (rc != 0) throw std::runtime_error(sbwt_last_error());
+}
+
+double fraction(const Args &a, const std::string &key, const std::string &def, const char *msg) {
+  const double v = std::stod(a.get(key, def));
+  if (v < 0 || v > 1) {
+    std::fprintf(stderr, "%s\n", msg);
+    std::exit(1);
+  }
+  return v;
+}
+
+// chars per index-search batch from free device memory (about 13 bytes per
+// char: input, results, packed sequence, invalid mask, offsets).
+u64 chars_per_batch(double gpu_fraction, u64 streams) {
+  u64 free_b = 0, total = 0;
+  check(sbwt_mem_info(&free_b, &total));
+  const u64 budget = static_cast<u64>(static_cast<double>(free_b) * gpu_fraction) / std::max<u64>(streams, 1);
+  const u64 n = std::min<u64>(budget / 13, 1ULL << 30);
+  if (n == 0) throw std::runtime_error("Not enough memory");
+  return n;
+}
+
+// ---------------------------------------------------------------- index
+void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<std::string> &in,
+                  const std::vector<std::string> &out, const std::vector<u64> &files,
+                  const std::string &mode, bool headers, u64 max_chars) {
+  const u64 max_reads = std::max<u64>(max_chars / 16, 1024);
+  sbwt_searcher *s = nullptr;
+  check(sbwt_searcher_create(index, max_chars, max_reads, move_to_key, nullptr, &s));
+  std::vector<u64> results(max_chars + 1);
+  std::vector<char> chars, read;
+  std::vector<u64> ends;
+  std::vector<char> closes;  // entry ends its read (else the next entry continues it)
+  for (u64 f : files) {
+    sbwt_host::SeqReader reader(in[f]);
+    sbwt_host::IndexWriter writer(out[f], mode, headers);
+    auto flush_batch = [&] {
+      if (ends.empty()) return;
+      u64 nk = 0;
+      check(sbwt_searcher_search_reads(s, chars.data(), chars.size(), ends.data(), ends.size(),
+                                       results.data(), &nk));
+      u64 ri = 0, cs = 0;
+      for (u64 e = 0; e < ends.size(); ++e) {
+        const u64 len = ends[e] - cs;
+        const u64 n = len >= k ? len - k + 1 : 0;
+        for (u64 j = 0; j < n; ++j) writer.result(results[ri++]);
+        if (closes[e]) writer.end_read();
+        cs = ends[e];
+      }
+      chars.clear();
+      ends.clear();
+      closes.clear();
+    };
+    while (true) {
+      read.clear();
+      if (!reader.next(read)) break;
+      // a read longer than the room left is cut into pieces overlapping by k-1
+      // chars (the carry-over of ContinuousSequenceFileParser.cpp:82-112)
+      u64 st = 0;
+      while (true) {
+        const u64 room = max_chars - chars.size();
+        if (room < k && !ends.empty()) {
+          flush_batch();
+          continue;
+        }
+        const u64 take = std::min<u64>(read.size() - st, room);
+        chars.insert(chars.end(), read.begin() + st, read.begin() + st + take);
+        ends.push_back(chars.size());
+        const bool last = st + take == read.size();
+        closes.push_back(last);
+        if (last) break;
+        st += take - (k - 1);
+        flush_batch();
+      }
+      if (chars.size() >= max_chars || ends.size() >= max_reads) flush_batch();
+    }
+    flush_batch();
+  }
+  sbwt_searcher_destroy(s);
+}
+
+int run_index(const Args &a) {
+  const std::string q = a.get("query-file"), idx = a.get("index-file"), o = a.get("output-prefix");
+  if (q.empty() || idx.empty() || o.empty())
+    throw std::runtime_error("query-file, index-file and output-prefix are required");
+  const std::string colors = a.get("colors-file");
+  const std::string mode = a.get("print-mode", "ascii");
+  if (mode != "ascii" && mode != "binary" && mode != "bool")
+    throw std::runtime_error("Invalid value passed by user for argument print_mode");
+  fraction(a, "cpu-memory-percentage", "0.8", "Invalid value for cpu-memory-percentage. Must be between 0 and 1.");
+  const double gpu = fraction(a, "gpu-memory-percentage", "0.95",
+                              "Invalid value for gpu-memory-percentage. Must be between 0 and 1.");
+  const auto in = sbwt_host::expand_list(q), out = sbwt_host::expand_list(o);
+  if (in.size() != out.size()) throw std::runtime_error("Input and output file sizes differ");
+  const u64 streams = std::max<u64>(1, std::min<u64>(in.size(), std::stoull(a.get("streams", "4"))));
+  sbwt_index *index = nullptr;
+  check(sbwt_index_load(idx.c_str(), colors.empty() ? nullptr : colors.c_str(), 0, &index));
+  sbwt_index_info info;
+  check(sbwt_index_get_info(index, &info));
+  u64 max_chars = chars_per_batch(gpu, streams);
+  // --batch-chars N (an extension) caps the chars per device batch
+  if (!a.get("batch-chars").empty())
+    max_chars = std::max<u64>(std::min<u64>(max_chars, std::stoull(a.get("batch-chars"))), info.kmer_size);
+  const auto parts = sbwt_host::partition_by_size(in, streams);
+  std::vector<std::thread> th;
+  std::vector<std::string> errors(streams);
+  for (u64 p = 0; p < streams; ++p)
+    th.emplace_back([&, p] {
+      try {
+        index_stream(index, info.kmer_size, !colors.empty(), in, out, parts[p], mode,
+                     !a.flag("no-headers"), max_chars);
+      } catch (const std::exception &e) {
+        errors[p] = e.what();
+      }
+    });
+  for (auto &t : th) t.join();
+  sbwt_index_destroy(index);
+  for (auto &e : errors)
+    if (!e.empty()) throw std::runtime_error(e);
+  return 0;
+}
+
+// --------------------------------------------------------------- colors
+void colors_stream(sbwt_color_index *ci, u64 num_colors, const std::vector<std::string> &in,
+                   const std::vector<std::string> &out, const std::vector<u64> &files,
+                   const std::string &mode, bool headers, const sbwt_color_params &params) {
+  sbwt_color_searcher *cs = nullptr;
+  bool first_batch = true;
+  for (u64 f : files) {
+    sbwt_host::IndexFile idx = sbwt_host::read_index_file(in[f]);
+    sbwt_host::ColorWriter writer(out[f], mode, headers, num_colors);
+    const u64 n = idx.kmer_ends.size();
+    if (n == 0) continue;
+    if (cs) sbwt_color_searcher_destroy(cs);
+    check(sbwt_color_searcher_create(ci, idx.results.size() + 64, n, nullptr, &cs));
+    sbwt_color_params p = params;
+    p.first_read_quirk = first_batch ? 1 : 0;  // the reference's carry starts once per stream
+    first_batch = false;
+    std::vector<u64> offsets(n + 1);
+    u64 cap = 0;
+    for (u64 r = 0, st = 0; r < n; ++r) {  // ids <= min(colors, found) per read
+      cap += std::min<u64>(num_colors, idx.kmer_ends[r] - st);
+      st = idx.kmer_ends[r];
+    }
+    std::vector<u32> ids(std::max<u64>(cap, 1));
+    u64 nids = 0;
+    check(sbwt_color_searcher_pseudoalign(cs, idx.results.data(), idx.kmer_ends.data(), n, &p,
+                                          offsets.data(), ids.data(), cap, nullptr, &nids));
+    for (u64 r = 0; r < n; ++r) writer.read(ids.data() + offsets[r], offsets[r + 1] - offsets[r]);
+  }
+  if (cs) sbwt_color_searcher_destroy(cs);
+}
+
+int run_colors(const Args &a) {
+  const std::string q = a.get("query-file"), k = a.get("colors-file"), o = a.get("output-prefix");
+  if (q.empty() || k.empty() || o.empty())
+    throw std::runtime_error("query-file, colors-file and output-prefix are required");
+  const std::string mode = a.get("print-mode", "ascii");
+  if (mode != "ascii" && mode != "binary" && mode != "csv")
+    throw std::runtime_error("Invalid value passed by user for argument print_mode");
+  const double t = std::stod(a.get("threshold", "1"));
+  if (t < 0 || t > 1) {
+    std::fprintf(stderr, "Invalid value for threshold, must be between 1 and 0 (both included)\n");
+    std::exit(1);
+  }
+  fraction(a, "cpu-memory-percentage", "0.8", "Invalid value for cpu-memory-percentage. Must be between 0 and 1.");
+  fraction(a, "gpu-memory-percentage", "0.95", "Invalid value for gpu-memory-percentage. Must be between 0 and 1.");
+  const auto in = sbwt_host::expand_list(q), out = sbwt_host::expand_list(o);
+  if (in.size() != out.size()) throw std::runtime_error("Input and output file sizes differ");
+  const u64 streams = std::max<u64>(1, std::min<u64>(in.size(), std::stoull(a.get("streams", "4"))));
+  sbwt_color_index *ci = nullptr;
+  check(sbwt_color_index_load(k.c_str(), &ci));
+  sbwt_color_index_info info;
+  check(sbwt_color_index_get_info(ci, &info));
+  sbwt_color_params params{t, a.flag("include-not-found"), a.flag("include-invalid"), 0};
+  const auto parts = sbwt_host::partition_by_size(in, streams);
+  std::vector<std::thread> th;
+  std::vector<std::string> errors(streams);
+  for (u64 p = 0; p < streams; ++p)
+    th.emplace_back([&, p] {
+      try {
+        colors_stream(ci, info.num_colors, in, out, parts[p], mode, !a.flag("no-headers"), params);
+      } catch (const std::exception &e) {
+        errors[p] = e.what();
+      }
+    });
+  for (auto &x : th) x.join();
+  sbwt_color_index_destroy(ci);
+  for (auto &e : errors)
+    if (!e.empty()) throw std::runtime_error(e);
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  try {
+    const Args a = parse(argc, argv);
+    if (a.mode == "index") return run_index(a);
+    if (a.mode == "colors") return run_colors(a);
+    std::fprintf(stderr, "usage: sbwt_search index|colors [options]\n");
+    return 1;
+  } catch (const std::exception &e) {
+    std::fprintf(stderr, "%s\n", e.what());
+    return 1;
+  }
+}

@@ -1,0 +1,87 @@
+"""The sbwt_search command line on the MI355X: byte-identical to the reference's
+expected files and to the oracle."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+from conftest import FIXTURES, REPO
+
+pytestmark = pytest.mark.gpu
+CLI = os.path.join(REPO, "sbwt-search_amd", "host", "sbwt_search")
+NAMES = ["fasta1.fna", "fasta2.fna", "fasta3.fna", "fasta4.fna", "fastq1.fnq"]
+
+
+def run(args):
+    print("[case] sbwt_search " + " ".join(args), flush=True)
+    r = subprocess.run([CLI] + args, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return r
+
+
+@pytest.mark.parametrize("batch", [None, 40, 31])
+def test_index_cli_matches_reference_expected(tmp_path, batch):
+    sbwt = os.path.join(FIXTURES, "search_test_index.sbwt")
+    for name in NAMES:
+        out = str(tmp_path / name)
+        extra = ["--batch-chars", str(batch)] if batch else []
+        run(["index", "-q", os.path.join(FIXTURES, "index_search", name), "-i", sbwt, "-o", out]
+            + extra)
+        exp = open(os.path.join(FIXTURES, "index_search", "expected",
+                                name.split(".")[0] + ".txt"), "rb").read()
+        assert open(out + ".txt", "rb").read() == exp, name
+
+
+def test_index_cli_list_streams_and_formats(tmp_path):
+    """5 files over 3 streams via .list; binary and bool equal the oracle's files."""
+    sbwt = os.path.join(FIXTURES, "search_test_index.sbwt")
+    orc = O.OracleSbwt(sbwt)
+    ins = [os.path.join(FIXTURES, "index_search", n) for n in NAMES]
+    outs = [str(tmp_path / n) for n in NAMES]
+    (tmp_path / "in.list").write_text("\n".join(ins) + "\n")
+    (tmp_path / "out.list").write_text("\n".join(outs) + "\n")
+    for mode, ext, fmt in (("binary", ".bin", 1), ("bool", ".bool", 2), ("ascii", ".txt", 0)):
+        run(["index", "-q", str(tmp_path / "in.list"), "-i", sbwt, "-o", str(tmp_path / "out.list"),
+             "-p", mode, "-s", "3"])
+        for i, o in zip(ins, outs):
+            orc.index_file(i, o + ".orc", fmt, True)
+            assert open(o + ext, "rb").read() == open(o + ".orc", "rb").read(), (mode, i)
+
+
+def test_index_cli_gzip_and_no_headers(tmp_path):
+    import gzip
+    sbwt = os.path.join(FIXTURES, "search_test_index.sbwt")
+    src = os.path.join(FIXTURES, "index_search", "fasta2.fna")
+    gz = str(tmp_path / "f.fna.gz")
+    with open(src, "rb") as f, gzip.open(gz, "wb") as g:
+        g.write(f.read())
+    run(["index", "-q", gz, "-i", sbwt, "-o", str(tmp_path / "o"), "--no-headers"])
+    exp = open(os.path.join(FIXTURES, "index_search", "expected", "fasta2.txt"), "rb").read()
+    assert open(str(tmp_path / "o.txt"), "rb").read() == exp[25:]
+
+
+def test_colors_cli_matches_oracle(tmp_path):
+    from colorlib import ColoredIndex, kmer_ends_of, read_stats, threshold_lists
+    ci = ColoredIndex(tmp_path, n_genomes=6, length=15_000, d=4)
+    chars, ends = ci.reads(300, 120, seed=9)
+    fa = tmp_path / "r.fna"
+    st = 0
+    with open(fa, "wb") as f:
+        for i, e in enumerate(ends):
+            f.write(b">r%d\n" % i + chars[st:int(e)] + b"\n")
+            st = int(e)
+    run(["index", "-q", str(fa), "-i", ci.sbwt_path, "-k", ci.colors_path, "-o",
+         str(tmp_path / "idx")])
+    res = ci.index_results(chars, ends)
+    assert open(str(tmp_path / "idx.txt"), "rb").read() == \
+        O.ascii_index_output(res, ends, 31)
+    run(["colors", "-q", str(tmp_path / "idx.txt"), "-k", ci.colors_path, "-o",
+         str(tmp_path / "col"), "-t", "0.7"])
+    ke = kmer_ends_of(ends)
+    exp = threshold_lists(ci.oracle_counts(res, ke), read_stats(res, ke), 0.7,
+                          first_read_quirk=True)
+    body = open(str(tmp_path / "col.txt"), "rb").read()[25:].decode().split("\n")[:-1]
+    assert [[int(x) for x in line.split()] for line in body] == exp

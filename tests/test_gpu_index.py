@@ -58,13 +58,18 @@ def _reads(g, n, length, seed, random_frac=0.5, err=0.01, n_rate=0.001):
     return chars.tobytes(), ends
 
 
+ALGOS = [S.ALGO_STREAMING, S.ALGO_PER_KMER]
+
+
+@pytest.mark.parametrize("algo", ALGOS, ids=["streaming", "per-kmer"])
 @pytest.mark.parametrize("kp", [12, 1, 7, 10, 14])
-def test_reads_parity_presearch_widths(genome_index, kp):
+def test_reads_parity_presearch_widths(genome_index, kp, algo):
     g, b, path, orc = genome_index
-    say(f"reads parity kp={kp}")
+    say(f"reads parity kp={kp} algo={algo}")
     idx = S.Index.load(path, presearch_letters=kp)
     chars, ends = _reads(g, 3000, 150, seed=kp)
     s = idx.searcher(len(chars), len(ends))
+    s.set_algorithm(algo)
     res = s.search_reads(chars, ends)
     exp = orc.index_reads(chars, ends, kp=12)
     assert len(res) == len(exp) == 3000 * 120
@@ -90,9 +95,10 @@ def test_positions_interface_parity(genome_index):
         s.search(packed[:2], np.array([1000], np.uint64))
 
 
-def test_ragged_reads_and_edge_cases(genome_index):
+@pytest.mark.parametrize("algo", ALGOS, ids=["streaming", "per-kmer"])
+def test_ragged_reads_and_edge_cases(genome_index, algo):
     g, b, path, orc = genome_index
-    say("ragged reads")
+    say(f"ragged reads algo={algo}")
     idx = S.Index.load(path)
     lens = [0, 1, 30, 31, 32, 33, 63, 64, 65, 200, 1000, 5000, 0, 31]
     chars, ends = _reads(g, len(lens), lens, seed=5, random_frac=0.3, n_rate=0.01)
@@ -105,6 +111,7 @@ def test_ragged_reads_and_edge_cases(genome_index):
     chars[st + 500] = 0xC1
     chars = bytes(chars)
     s = idx.searcher(len(chars) + 100, len(ends) + 10)
+    s.set_algorithm(algo)
     res = s.search_reads(chars, ends)
     assert np.array_equal(res, orc.index_reads(chars, ends))
     say("empty batch")
@@ -128,16 +135,66 @@ def test_searcher_reuse_shrinking_batches(genome_index):
         s.search_reads(chars, ends)
 
 
-def test_wide_index_kernels(genome_index, monkeypatch):
+@pytest.mark.parametrize("algo", ALGOS, ids=["streaming", "per-kmer"])
+def test_wide_index_kernels(genome_index, monkeypatch, algo):
     """64-bit node-index kernels (needed from 2^32 nodes) on a small index."""
     g, b, path, orc = genome_index
-    say("forced 64-bit index")
+    say(f"forced 64-bit index algo={algo}")
     monkeypatch.setenv("SBWT_AMD_FORCE_WIDE", "1")
     idx = S.Index.load(path)
     assert idx.wide
     chars, ends = _reads(g, 2000, 150, seed=21)
     s = idx.searcher(len(chars), len(ends))
+    s.set_algorithm(algo)
     assert np.array_equal(s.search_reads(chars, ends), orc.index_reads(chars, ends))
+
+
+def test_long_reads_span_many_chunks(tmp_path):
+    """Reads of up to 20 kbp (many 256-k-mer work items) from similar genomes."""
+    say("long reads, many chunks")
+    root = W.genome_random(9, 60_000)
+    gens = [root] + [W.genome_mutate(root, s, 0.02, 0.001, 10) for s in (1, 2, 3)]
+    codes, ends = W.concat(gens)
+    b = W.BuiltSbwt(codes, ends, 31)
+    path = str(tmp_path / "l.sbwt")
+    b.write(path)
+    orc = O.OracleSbwt(path)
+    idx = S.Index.load(path)
+    lens = W.lognormal_lengths(4, 300, 2000, 1.2, 31, 20_000)
+    chars, rends, _ = W.reads(6, codes, ends, lens, 0.05, 0.01, 0.0005)
+    chars = chars.tobytes()
+    s = idx.searcher(len(chars), len(rends))
+    s.set_stats(True)
+    res = s.search_reads(chars, rends)
+    st = s.last_stats()
+    assert np.array_equal(res, orc.index_reads(chars, rends))
+    assert st["streamed"] > st["full_searches"]  # most k-mers step from their predecessor
+
+
+def test_streaming_without_suffix_group_starts(tmp_path):
+    """Index built from the ACGT vectors alone: steps from nodes that do not start
+    their suffix group fall back to full searches; results are unchanged."""
+    say("streaming without suffix-group starts")
+    root = W.genome_random(12, 40_000)
+    gens = [root] + [W.genome_mutate(root, s, 0.03) for s in (1, 2, 3, 4)]
+    codes, ends = W.concat(gens)
+    b = W.BuiltSbwt(codes, ends, 31)
+    path = str(tmp_path / "v.sbwt")
+    b.write(path)
+    orc = O.OracleSbwt(path)
+    chars, rends, _ = W.reads(7, codes, ends, np.full(3000, 150, np.uint32), 0.05, 0.005, 0.0)
+    chars = chars.tobytes()
+    exp = orc.index_reads(chars, rends)
+    for sgs in (None, b.bits(4)):
+        idx = S.Index.create([b.bits(c) for c in range(4)], b.n, 31, suffix_group_starts=sgs)
+        s = idx.searcher(len(chars), len(rends))
+        s.set_stats(True)
+        assert np.array_equal(s.search_reads(chars, rends), exp)
+        st = s.last_stats()
+        if sgs is None:
+            assert st["stream_fallbacks"] > 0
+        else:
+            assert st["stream_fallbacks"] == 0 and st["stream_extra_lines"] > 0
 
 
 def test_create_from_bit_vectors_matches_load(genome_index):
@@ -150,10 +207,11 @@ def test_create_from_bit_vectors_matches_load(genome_index):
     assert np.array_equal(s.search_reads(chars, ends), orc.index_reads(chars, ends))
 
 
+@pytest.mark.parametrize("algo", ALGOS, ids=["streaming", "per-kmer"])
 @pytest.mark.parametrize("d", [1, 3, 20])
-def test_key_kmer_move_parity(d, tmp_path):
+def test_key_kmer_move_parity(d, algo, tmp_path):
     """-k: move every found k-mer to its key k-mer (IndexSearcher.cuh:60-70)."""
-    say(f"key k-mer walk d={d}")
+    say(f"key k-mer walk d={d} algo={algo}")
     root = W.genome_random(1, 30_000)
     gens = [root] + [W.genome_mutate(root, s, 0.02) for s in (2, 3, 4)]
     codes, ends = W.concat(gens)
@@ -169,6 +227,7 @@ def test_key_kmer_move_parity(d, tmp_path):
     chars, ends2, _ = W.reads(5, codes, ends, np.full(2000, 120, np.uint32), 0.2, 0.01, 0.001)
     chars = chars.tobytes()
     s = idx.searcher(len(chars), len(ends2), move_to_key_kmer=True)
+    s.set_algorithm(algo)
     res = s.search_reads(chars, ends2)
     exp = orc.index_reads(chars, ends2, key_marks=np.concatenate([km, np.zeros(8, np.uint64)]))
     assert np.array_equal(res, exp)
