@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--presearch-letters", type=int, default=12)
     ap.add_argument("--algorithm", default="streaming", choices=["streaming", "per-kmer"])
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run an elementwise copy of this many MiB (a kernel of known "
+                         "HBM traffic, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     return ap.parse_args()
 
 
@@ -134,6 +137,15 @@ def main():
     found = int(np.count_nonzero(res < np.uint64(S.INVALID)))
     log(f"{ms_per_step:.2f} ms/step, search kernel {kern_ms:.2f} ms, {value / 1e9:.3f} G k-mers/s,"
         f" found {found / n_kmers:.3f}")
+
+    if args.calibrate_copy_mib:
+        n = args.calibrate_copy_mib << 20
+        src = torch.ones(n // 8, dtype=torch.int64, device=dev)
+        dst = torch.empty_like(src)
+        torch.add(src, 0, out=dst)  # reads n bytes, writes n bytes
+        torch.cuda.synchronize()
+        log(f"calibration copy: {n} bytes read, {n} bytes written")
+        del src, dst
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -260,6 +260,47 @@ int sbwt_color_searcher_counts_device(sbwt_color_searcher *searcher,
 /* Device time (ms) of the last call's color kernel. */
 int sbwt_color_searcher_last_time(sbwt_color_searcher *searcher, float *ms);
 
+/* ---- serving loop -------------------------------------------------------- */
+
+/* A request loop over resident indexes: submitted batches of reads queue up
+ * and are admitted to worker slots (each its own HIP stream, searchers and
+ * pinned staging) as slots free up, so requests that arrive while others run
+ * overlap on the device. The number of slots is min(slots, what
+ * memory_fraction of the free device memory holds). Requests larger than a
+ * slot run as several batches in that slot. No counterpart in the reference,
+ * which processes its input files in fixed streams (IndexSearchMain.cpp:414-448). */
+typedef struct sbwt_server sbwt_server;
+typedef struct {
+  uint32_t slots;
+  uint64_t max_chars_per_slot;
+  uint64_t max_reads_per_slot;
+  double memory_fraction;
+  int move_to_key_kmer;
+  sbwt_color_params color;   /* used when a color index is attached */
+} sbwt_server_config;
+
+typedef struct {
+  uint64_t n_kmers;
+  const uint64_t *kmer_results;   /* as sbwt_searcher_search_reads */
+  uint64_t n_reads;
+  const uint64_t *color_offsets;  /* n_reads + 1 entries, null without colors */
+  const uint32_t *color_ids;
+  double submit_s, start_s, end_s; /* seconds since the server started */
+} sbwt_server_result;
+
+int sbwt_server_create(sbwt_index *index, sbwt_color_index *colors /* nullable */,
+                       const sbwt_server_config *config, sbwt_server **out);
+/* Copies the request and returns at once with its id. */
+int sbwt_server_submit(sbwt_server *server, const char *chars, uint64_t n_chars,
+                       const uint64_t *read_ends, uint64_t n_reads, uint64_t *request_id);
+/* Waits for the request; the result's buffers stay valid until release. A
+ * failed request (e.g. malformed read ends) returns its error; other requests
+ * are unaffected. */
+int sbwt_server_wait(sbwt_server *server, uint64_t request_id, sbwt_server_result *out);
+int sbwt_server_release(sbwt_server *server, uint64_t request_id);
+int sbwt_server_get_slots(sbwt_server *server, uint32_t *slots, uint64_t *peak_in_flight);
+void sbwt_server_destroy(sbwt_server *server);
+
 #ifdef __cplusplus
 }
 #endif

@@ -9,6 +9,15 @@ Each builder returns plain numpy data; nothing here touches the GPU.
       indels of length 1-10 (seeds 1..50); union SBWT k=31 (~95 M nodes);
       10,000,000 reads x 150 bp: 95% from a random genome/strand with 1%
       substitutions, 5% uniform random (reads seed 1).
+  C3  color search: genomes derived from seeded uniform roots with 1-5%
+      substitutions (rate uniform per genome), one color per genome, key k-mers
+      every d = 20 on non-branching paths; reads of length U[50, 300] drawn from
+      random genomes with 1% error (reads seed 3). Full scale is 1,000 genomes of
+      2-5 Mbp from 20 roots; smaller genome counts/lengths keep the same recipe.
+  C4  batched form: the C2 genomes and their color index (50 colors, d = 20);
+      1,000,000 reads with log-normal lengths (median 2,000 bp, sigma 1.2,
+      clipped to [31, 100,000]) and 4,000,000 reads x 150 bp, interleaved in a
+      seeded random order (reads seed 4); sampled as in C2.
 """
 import numpy as np
 
@@ -63,3 +72,34 @@ def c2(n_reads=10_000_000, n_genomes=50, length=2_000_000):
                     "one seeded root (2% substitutions, 0.1% indels of 1-10 bp), union SBWT "
                     f"k=31; {n_reads} reads x 150 bp (95% sampled with 1% substitutions, 5% "
                     "random)", codes, ends, chars, rends)
+
+
+def c3_genomes(n_genomes=1000, n_roots=20, min_len=2_000_000, max_len=5_000_000):
+    roots = [W.genome_random(100 + i, max_len) for i in range(n_roots)]
+    gens = []
+    for g in range(n_genomes):
+        length = int(min_len + W.lib().bld_draw(3, 11, g) % (max_len - min_len + 1))
+        rate = 0.01 + 0.04 * (W.lib().bld_draw(3, 12, g) >> 11) * 2.0 ** -53
+        gens.append(W.genome_mutate(roots[g % n_roots][:length], 1000 + g, rate))
+    return W.concat(gens)
+
+
+def c3(n_reads=5_000_000, n_genomes=1000, n_roots=20, min_len=2_000_000, max_len=5_000_000):
+    codes, ends = c3_genomes(n_genomes, n_roots, min_len, max_len)
+    lens = W.uniform_lengths(3, n_reads, 50, 300)
+    chars, rends, _ = W.reads(3, codes, ends, lens, random_frac=0.0, err_rate=0.01)
+    return Workload("C3", f"color search: {n_genomes} genomes of {min_len}-{max_len} bp from "
+                    f"{n_roots} seeded roots (1-5% substitutions), one color each, d=20; "
+                    f"{n_reads} reads of U[50,300] bp with 1% error", codes, ends, chars, rends)
+
+
+def c4(n_long=1_000_000, n_short=4_000_000, n_genomes=50, length=2_000_000):
+    codes, ends = c2_genomes(n_genomes, length)
+    long_lens = W.lognormal_lengths(4, n_long, 2000.0, 1.2, 31, 100_000)
+    lens = np.concatenate([long_lens, np.full(n_short, 150, np.uint32)])
+    lens = np.ascontiguousarray(lens[W.permutation(4, len(lens)).astype(np.int64)])
+    chars, rends, _ = W.reads(4, codes, ends, lens, random_frac=0.05, err_rate=0.01)
+    return Workload("C4", f"batched form: C2 genomes + color index (50 colors, d=20); "
+                    f"{n_long} log-normal reads (median 2 kbp, sigma 1.2, [31, 100 kbp]) "
+                    f"interleaved with {n_short} reads x 150 bp", codes, ends, chars, rends)
+

@@ -40,6 +40,18 @@ class ColorParams(C.Structure):
                 ("include_invalid", C.c_int), ("first_read_quirk", C.c_int)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("slots", C.c_uint32), ("max_chars_per_slot", C.c_uint64),
+                ("max_reads_per_slot", C.c_uint64), ("memory_fraction", C.c_double),
+                ("move_to_key_kmer", C.c_int), ("color", ColorParams)]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("n_kmers", C.c_uint64), ("kmer_results", u64p), ("n_reads", C.c_uint64),
+                ("color_offsets", u64p), ("color_ids", u32p), ("submit_s", C.c_double),
+                ("start_s", C.c_double), ("end_s", C.c_double)]
+
+
 class SbwtError(RuntimeError):
     pass
 
@@ -53,6 +65,10 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise SbwtError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+        # One HIP runtime per process: torch ships its own libamdhip64 (same SONAME as
+        # the system one the library links). Loading torch first makes the library
+        # bind to that copy, so torch tensors and library kernels share one runtime.
+        import torch  # noqa: F401
         L = C.CDLL(LIB_PATH)
         vp = C.c_void_p
         L.sbwt_last_error.restype = C.c_char_p
@@ -93,6 +109,13 @@ def lib():
                                                              C.c_uint64, vp, u64p]
         L.sbwt_color_searcher_counts_device.argtypes = [vp, vp, vp, C.c_uint64, vp]
         L.sbwt_color_searcher_last_time.argtypes = [vp, C.POINTER(C.c_float)]
+        L.sbwt_server_create.argtypes = [vp, vp, C.POINTER(ServerConfig), C.POINTER(vp)]
+        L.sbwt_server_submit.argtypes = [vp, C.c_char_p, C.c_uint64, u64p, C.c_uint64, u64p]
+        L.sbwt_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(ServerResult)]
+        L.sbwt_server_release.argtypes = [vp, C.c_uint64]
+        L.sbwt_server_get_slots.argtypes = [vp, C.POINTER(C.c_uint32), u64p]
+        L.sbwt_server_destroy.argtypes = [vp]
+        L.sbwt_server_destroy.restype = None
         L.sbwt_searcher_set_stats.argtypes = [vp, C.c_int]
         L.sbwt_searcher_last_stats.argtypes = [vp, u64p]
         L.sbwt_searcher_set_algorithm.argtypes = [vp, C.c_int]
@@ -359,3 +382,65 @@ class ColorSearcher:
         ms = C.c_float()
         _check(lib().sbwt_color_searcher_last_time(self.h, C.byref(ms)))
         return ms.value
+
+
+class Server:
+    """Serving loop over resident indexes (sbwt_server_*)."""
+
+    def __init__(self, index, colors=None, slots=4, max_chars_per_slot=1 << 25,
+                 max_reads_per_slot=1 << 20, memory_fraction=0.9, move_to_key_kmer=False,
+                 params=None):
+        self.index, self.colors = index, colors
+        cfg = ServerConfig(slots, max_chars_per_slot, max_reads_per_slot, memory_fraction,
+                           int(move_to_key_kmer), params or color_params())
+        self.h = C.c_void_p()
+        _check(lib().sbwt_server_create(index.h, colors.h if colors else None, C.byref(cfg),
+                                        C.byref(self.h)))
+        n = C.c_uint32()
+        _check(lib().sbwt_server_get_slots(self.h, C.byref(n), None))
+        self.slots = n.value
+
+    def submit(self, chars, read_ends):
+        chars = bytes(chars)
+        ends = np.ascontiguousarray(read_ends, np.uint64)
+        rid = C.c_uint64()
+        _check(lib().sbwt_server_submit(self.h, chars, len(chars), _p64(ends) if len(ends) else None,
+                                        len(ends), C.byref(rid)))
+        return rid.value
+
+    def wait(self, rid):
+        """-> dict(kmers, color_lists or None, submit_s, start_s, end_s); raises on failure."""
+        r = ServerResult()
+        rc = lib().sbwt_server_wait(self.h, rid, C.byref(r))
+        try:
+            if rc != 0:
+                raise SbwtError(lib().sbwt_last_error().decode())
+            kmers = np.ctypeslib.as_array(r.kmer_results, shape=(r.n_kmers,)).copy() \
+                if r.n_kmers else np.zeros(0, np.uint64)
+            lists = None
+            if r.color_offsets:
+                off = np.ctypeslib.as_array(r.color_offsets, shape=(r.n_reads + 1,))
+                n_ids = int(off[-1])
+                ids = np.ctypeslib.as_array(r.color_ids, shape=(n_ids,)).copy() if n_ids else \
+                    np.zeros(0, np.uint32)
+                lists = [ids[int(off[i]):int(off[i + 1])].tolist() for i in range(r.n_reads)]
+            return {"kmers": kmers, "color_lists": lists, "submit_s": r.submit_s,
+                    "start_s": r.start_s, "end_s": r.end_s}
+        finally:
+            lib().sbwt_server_release(self.h, rid)
+
+    def peak_in_flight(self):
+        n, p = C.c_uint32(), C.c_uint64()
+        _check(lib().sbwt_server_get_slots(self.h, C.byref(n), C.byref(p)))
+        return p.value
+
+    def close(self):
+        if self.h:
+            lib().sbwt_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -35,6 +35,7 @@ def lib():
         L.bld_lognormal_lengths.argtypes = [C.c_uint64, C.c_uint64, C.c_double, C.c_double,
                                             C.c_uint32, C.c_uint32, u32p]
         L.bld_uniform_lengths.argtypes = [C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32, u32p]
+        L.bld_permutation.argtypes = [C.c_uint64, C.c_uint64, u64p]
         L.bld_sbwt_build.argtypes = [u8p, u64p, C.c_uint64, C.c_uint32, C.c_int, C.c_int]
         L.bld_sbwt_build.restype = C.c_void_p
         L.bld_sbwt_free.argtypes = [C.c_void_p]
@@ -182,3 +183,9 @@ class BuiltColors:
         if getattr(self, "h", None):
             lib().bld_colors_free(self.h)
             self.h = None
+
+
+def permutation(seed, n):
+    out = np.empty(n, np.uint64)
+    lib().bld_permutation(seed, n, _p(out, u64p))
+    return out

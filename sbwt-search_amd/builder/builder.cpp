@@ -218,6 +218,12 @@ extern "C" uint64_t bld_genome_mutate(const uint8_t *root, uint64_t len, uint64_
   return o;
 }
 
+extern "C" void bld_permutation(uint64_t seed, uint64_t n, uint64_t *out) {
+  for (uint64_t i = 0; i < n; ++i) out[i] = i;
+  Rng r(seed, 9);
+  for (uint64_t i = n; i > 1; --i) std::swap(out[i - 1], out[r.next() % i]);
+}
+
 extern "C" void bld_uniform_lengths(uint64_t seed, uint64_t n, uint32_t lo, uint32_t hi,
                                     uint32_t *out) {
 #pragma omp parallel for schedule(static)

@@ -42,6 +42,9 @@ uint64_t bld_reads(uint64_t seed, const uint8_t *genomes, const uint64_t *genome
                    double err_rate, double n_rate, int mode_alternate, char *out, uint64_t *ends,
                    uint32_t *src_genome);
 
+/* A seeded uniform permutation of 0..n-1 (Fisher-Yates). */
+void bld_permutation(uint64_t seed, uint64_t n, uint64_t *out);
+
 /* Log-normal read lengths exp(N(log(median), sigma)) clipped to [lo, hi]. */
 void bld_lognormal_lengths(uint64_t seed, uint64_t n, double median, double sigma, uint32_t lo,
                            uint32_t hi, uint32_t *out);

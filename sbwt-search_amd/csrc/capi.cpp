@@ -9,6 +9,7 @@
 #include "device_index.hpp"
 #include "index_search.hpp"
 #include "loaders.hpp"
+#include "server.hpp"
 
 using sbwt_amd::DeviceIndex;
 using sbwt_amd::Searcher;
@@ -25,6 +26,10 @@ struct sbwt_searcher {
 struct sbwt_color_index {
   sbwt_amd::DeviceColors dev;
   ~sbwt_color_index() { dev.release(); }
+};
+
+struct sbwt_server {
+  std::unique_ptr<sbwt_amd::Server> impl;
 };
 
 struct sbwt_color_searcher {
@@ -371,5 +376,64 @@ int sbwt_color_searcher_last_time(sbwt_color_searcher *searcher, float *ms) {
     *ms = searcher->impl->last_search_ms();
   });
 }
+
+int sbwt_server_create(sbwt_index *index, sbwt_color_index *colors,
+                       const sbwt_server_config *config, sbwt_server **out) {
+  return guarded([&] {
+    require(index && config && out, "null argument");
+    sbwt_amd::ServerConfig c;
+    c.slots = config->slots;
+    c.max_chars_per_slot = config->max_chars_per_slot;
+    c.max_reads_per_slot = config->max_reads_per_slot;
+    c.memory_fraction = config->memory_fraction;
+    c.move_to_key_kmer = config->move_to_key_kmer != 0;
+    if (colors) c.color = to_params(&config->color);
+    auto s = std::make_unique<sbwt_server>();
+    s->impl = std::make_unique<sbwt_amd::Server>(&index->dev, colors ? &colors->dev : nullptr, c);
+    *out = s.release();
+  });
+}
+
+int sbwt_server_submit(sbwt_server *server, const char *chars, uint64_t n_chars,
+                       const uint64_t *read_ends, uint64_t n_reads, uint64_t *request_id) {
+  return guarded([&] {
+    require(server && request_id && (n_chars == 0 || chars) && (n_reads == 0 || read_ends),
+            "null argument");
+    *request_id = server->impl->submit(chars, n_chars, read_ends, n_reads);
+  });
+}
+
+int sbwt_server_wait(sbwt_server *server, uint64_t request_id, sbwt_server_result *out) {
+  return guarded([&] {
+    require(server && out, "null argument");
+    auto r = server->impl->wait(request_id);
+    out->n_kmers = r->kmer_results.size();
+    out->kmer_results = r->kmer_results.data();
+    out->n_reads = r->read_ends.size();
+    out->color_offsets = r->color_offsets.empty() ? nullptr : r->color_offsets.data();
+    out->color_ids = r->color_ids.data();
+    out->submit_s = r->submit_s;
+    out->start_s = r->start_s;
+    out->end_s = r->end_s;
+    if (!r->error.empty()) throw std::runtime_error(r->error);
+  });
+}
+
+int sbwt_server_release(sbwt_server *server, uint64_t request_id) {
+  return guarded([&] {
+    require(server, "null argument");
+    server->impl->release(request_id);
+  });
+}
+
+int sbwt_server_get_slots(sbwt_server *server, uint32_t *slots, uint64_t *peak_in_flight) {
+  return guarded([&] {
+    require(server, "null argument");
+    if (slots) *slots = server->impl->slots();
+    if (peak_in_flight) *peak_in_flight = server->impl->admitted_peak();
+  });
+}
+
+void sbwt_server_destroy(sbwt_server *server) { delete server; }
 
 }  // extern "C"

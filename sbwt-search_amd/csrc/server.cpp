@@ -1,0 +1,191 @@
+// Serving loop (see server.hpp).
+#include "server.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+
+namespace sbwt_amd {
+
+Server::Server(const DeviceIndex *index, const DeviceColors *colors, const ServerConfig &cfg)
+    : index_(index), colors_(colors), cfg_(cfg), t0_(std::chrono::steady_clock::now()) {
+  if (cfg_.slots == 0 || cfg_.max_chars_per_slot == 0 || cfg_.max_reads_per_slot == 0)
+    throw std::runtime_error("Not enough memory");
+  if (cfg_.move_to_key_kmer && !index_->key_marks.blocks)
+    throw std::runtime_error("move to key k-mer requested but the index has no key-k-mer marks");
+  const u64 nc = colors_ ? colors_->num_colors : 0;
+  const u64 C = cfg_.max_chars_per_slot, R = cfg_.max_reads_per_slot;
+  const u64 ids_cap = colors_ ? std::min<u64>(C, R * nc) : 0;
+  // device bytes of one slot: inputs, results, searcher and color-searcher buffers
+  const u64 per_slot = C * (1 + 8 + 1) + R * (8 * 6) + C / 256 * 16 +
+                       (colors_ ? R * ((nc + 63) / 64 * 8 + 24) + ids_cap * 4 : 0) + (8u << 20);
+  size_t free_b = 0, total = 0;
+  SBWT_HIP(hipMemGetInfo(&free_b, &total));
+  const u64 budget = static_cast<u64>(static_cast<double>(free_b) * cfg_.memory_fraction);
+  const u64 fit = budget / per_slot;
+  if (fit == 0) throw std::runtime_error("Not enough memory");
+  const u32 n_slots = static_cast<u32>(std::min<u64>(cfg_.slots, fit));
+  for (u32 i = 0; i < n_slots; ++i) {
+    auto s = std::make_unique<Slot>();
+    SBWT_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    s->searcher = std::make_unique<Searcher>(index_, C, R, cfg_.move_to_key_kmer, s->stream);
+    s->searcher->set_timing(false);
+    if (colors_) s->colors = std::make_unique<ColorSearcher>(colors_, C, R, s->stream);
+    SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_chars), C + 64, hipHostMallocDefault));
+    SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_ends), R * 8, hipHostMallocDefault));
+    SBWT_HIP(hipMalloc(&s->d_chars, C + 64));
+    SBWT_HIP(hipMalloc(&s->d_ends, R * 8));
+    SBWT_HIP(hipMalloc(&s->d_results, (C + 1) * 8));
+    if (colors_) {
+      s->ids_capacity = std::max<u64>(ids_cap, 1);
+      SBWT_HIP(hipMalloc(&s->d_offsets, (R + 1) * 8));
+      SBWT_HIP(hipMalloc(&s->d_ids, s->ids_capacity * 4));
+    }
+    slots_.push_back(std::move(s));
+  }
+  for (auto &s : slots_) {
+    Slot *sp = s.get();
+    s->worker = std::thread([this, sp] { run_slot(*sp); });
+  }
+}
+
+Server::~Server() {
+  {
+    std::lock_guard<std::mutex> g(mu_);
+    stop_ = true;
+  }
+  cv_work_.notify_all();
+  for (auto &s : slots_) {
+    if (s->worker.joinable()) s->worker.join();
+    s->colors.reset();
+    s->searcher.reset();
+    void *dev[] = {s->d_chars, s->d_ends, s->d_results, s->d_offsets, s->d_ids};
+    for (void *p : dev)
+      if (p) (void)hipFree(p);
+    if (s->h_chars) (void)hipHostFree(s->h_chars);
+    if (s->h_ends) (void)hipHostFree(s->h_ends);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+  }
+}
+
+double Server::now() const {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+}
+
+u64 Server::submit(const char *chars, u64 n_chars, const u64 *read_ends, u64 n_reads) {
+  auto r = std::make_shared<Request>();
+  r->chars.assign(chars, chars + n_chars);
+  r->read_ends.assign(read_ends, read_ends + n_reads);
+  std::lock_guard<std::mutex> g(mu_);
+  r->id = next_id_++;
+  r->submit_s = now();
+  requests_[r->id] = r;
+  queue_.push_back(r);
+  cv_work_.notify_one();
+  return r->id;
+}
+
+std::shared_ptr<Request> Server::wait(u64 id) {
+  std::unique_lock<std::mutex> g(mu_);
+  auto it = requests_.find(id);
+  if (it == requests_.end()) throw std::runtime_error("unknown request id " + std::to_string(id));
+  auto r = it->second;
+  cv_done_.wait(g, [&] { return r->done; });
+  return r;
+}
+
+void Server::release(u64 id) {
+  std::lock_guard<std::mutex> g(mu_);
+  requests_.erase(id);
+}
+
+void Server::run_slot(Slot &s) {
+  for (;;) {
+    std::shared_ptr<Request> r;
+    {
+      std::unique_lock<std::mutex> g(mu_);
+      cv_work_.wait(g, [&] { return stop_ || !queue_.empty(); });
+      if (queue_.empty()) return;  // stop_ and drained
+      r = queue_.front();
+      queue_.pop_front();
+      peak_in_flight_ = std::max(peak_in_flight_, ++in_flight_);
+    }
+    r->start_s = now();
+    try {
+      process(s, *r);
+    } catch (const std::exception &e) {
+      r->error = e.what();
+      (void)hipStreamSynchronize(s.stream);
+    }
+    {
+      std::lock_guard<std::mutex> g(mu_);
+      r->end_s = now();
+      r->done = true;
+      --in_flight_;
+    }
+    cv_done_.notify_all();
+  }
+}
+
+// One request through one slot: batches of whole reads up to the slot's
+// capacity, each packed, searched (and color-searched) on the slot's stream.
+void Server::process(Slot &s, Request &r) {
+  const u64 n_reads = r.read_ends.size();
+  const u64 n_chars = r.chars.size();
+  const u64 k = index_->k;
+  u64 prev = 0, total_kmers = 0;
+  for (u64 i = 0; i < n_reads; ++i) {
+    if (r.read_ends[i] < prev || r.read_ends[i] > n_chars)
+      throw std::runtime_error("read_ends must be non-decreasing and end at n_chars");
+    if (r.read_ends[i] - prev > cfg_.max_chars_per_slot)
+      throw std::runtime_error("a read is longer than the slot capacity");
+    const u64 len = r.read_ends[i] - prev;
+    total_kmers += len >= k ? len - k + 1 : 0;
+    prev = r.read_ends[i];
+  }
+  if (n_reads && r.read_ends[n_reads - 1] != n_chars) throw std::runtime_error("read_ends must end at n_chars");
+  r.kmer_results.resize(total_kmers);
+  if (colors_) {
+    r.color_offsets.assign(n_reads + 1, 0);
+    r.color_ids.clear();
+  }
+  u64 read0 = 0, kmer_out = 0;
+  while (read0 < n_reads) {
+    // whole reads up to the slot capacity
+    const u64 c0 = read0 ? r.read_ends[read0 - 1] : 0;
+    u64 read1 = read0;
+    while (read1 < n_reads && read1 - read0 < cfg_.max_reads_per_slot &&
+           r.read_ends[read1] - c0 <= cfg_.max_chars_per_slot)
+      ++read1;
+    const u64 nr = read1 - read0, nc = r.read_ends[read1 - 1] - c0;
+    std::memcpy(s.h_chars, r.chars.data() + c0, nc);
+    for (u64 i = 0; i < nr; ++i) s.h_ends[i] = r.read_ends[read0 + i] - c0;
+    SBWT_HIP(hipMemcpyAsync(s.d_chars, s.h_chars, nc, hipMemcpyHostToDevice, s.stream));
+    SBWT_HIP(hipMemcpyAsync(s.d_ends, s.h_ends, nr * 8, hipMemcpyHostToDevice, s.stream));
+    u64 nk = 0;
+    s.searcher->search_reads_device(s.d_chars, nc, s.d_ends, nr, s.d_results, &nk);
+    if (nk)
+      SBWT_HIP(hipMemcpyAsync(r.kmer_results.data() + kmer_out, s.d_results, nk * 8,
+                              hipMemcpyDeviceToHost, s.stream));
+    if (colors_) {
+      const u64 n_ids = s.colors->pseudoalign_device(s.d_results, s.searcher->kmer_ends_device(), nr,
+                                                     cfg_.color, s.d_offsets, s.d_ids,
+                                                     s.ids_capacity, nullptr);
+      std::vector<u64> off(nr + 1);
+      const u64 base = r.color_ids.size();
+      r.color_ids.resize(base + n_ids);
+      SBWT_HIP(hipMemcpyAsync(off.data(), s.d_offsets, (nr + 1) * 8, hipMemcpyDeviceToHost, s.stream));
+      if (n_ids)
+        SBWT_HIP(hipMemcpyAsync(r.color_ids.data() + base, s.d_ids, n_ids * 4, hipMemcpyDeviceToHost,
+                                s.stream));
+      SBWT_HIP(hipStreamSynchronize(s.stream));
+      for (u64 i = 0; i < nr; ++i) r.color_offsets[read0 + i + 1] = base + off[i + 1];
+    } else {
+      SBWT_HIP(hipStreamSynchronize(s.stream));
+    }
+    kmer_out += nk;
+    read0 = read1;
+  }
+}
+
+}  // namespace sbwt_amd

@@ -1,0 +1,89 @@
+// Serving loop: the index (and color index) stay resident; requests (batches
+// of reads) are queued and admitted to worker slots as slots and device memory
+// free up, so requests that arrive while others run overlap on the device.
+// No counterpart in the reference, which processes files in fixed streams.
+#pragma once
+
+#include <chrono>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "color_search.hpp"
+#include "index_search.hpp"
+
+namespace sbwt_amd {
+
+struct ServerConfig {
+  u32 slots = 4;                    // requests in flight on the device at once
+  u64 max_chars_per_slot = 1 << 25;
+  u64 max_reads_per_slot = 1 << 20;
+  double memory_fraction = 0.9;     // of free device memory the slots may take
+  bool move_to_key_kmer = false;
+  ColorParams color;
+};
+
+struct Request {
+  u64 id = 0;
+  std::vector<char> chars;
+  std::vector<u64> read_ends;
+  // results
+  std::vector<u64> kmer_results;
+  std::vector<u64> color_offsets;
+  std::vector<u32> color_ids;
+  std::string error;
+  bool done = false;
+  double submit_s = 0, start_s = 0, end_s = 0;  // seconds since server start
+};
+
+class Server {
+ public:
+  Server(const DeviceIndex *index, const DeviceColors *colors, const ServerConfig &cfg);
+  ~Server();
+  u64 submit(const char *chars, u64 n_chars, const u64 *read_ends, u64 n_reads);
+  // Blocks until request id is done; the request stays until release().
+  std::shared_ptr<Request> wait(u64 id);
+  void release(u64 id);
+  u32 slots() const { return static_cast<u32>(slots_.size()); }
+  u64 admitted_peak() const { return peak_in_flight_; }
+
+ private:
+  struct Slot {
+    std::unique_ptr<Searcher> searcher;
+    std::unique_ptr<ColorSearcher> colors;
+    hipStream_t stream = nullptr;
+    char *h_chars = nullptr;   // pinned staging
+    u64 *h_ends = nullptr;
+    char *d_chars = nullptr;
+    u64 *d_ends = nullptr;
+    u64 *d_results = nullptr;
+    u64 *d_offsets = nullptr;
+    u32 *d_ids = nullptr;
+    u64 ids_capacity = 0;
+    std::thread worker;
+  };
+  void run_slot(Slot &s);
+  void process(Slot &s, Request &r);
+  double now() const;
+
+  const DeviceIndex *index_;
+  const DeviceColors *colors_;
+  ServerConfig cfg_;
+  std::vector<std::unique_ptr<Slot>> slots_;
+  std::mutex mu_;
+  std::condition_variable cv_work_, cv_done_;
+  std::deque<std::shared_ptr<Request>> queue_;
+  std::map<u64, std::shared_ptr<Request>> requests_;
+  u64 next_id_ = 1;
+  u64 in_flight_ = 0, peak_in_flight_ = 0;
+  bool stop_ = false;
+  std::chrono::steady_clock::time_point t0_;
+};
+
+}  // namespace sbwt_amd

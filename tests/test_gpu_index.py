@@ -116,10 +116,13 @@ def test_ragged_reads_and_edge_cases(genome_index, algo):
     assert np.array_equal(res, orc.index_reads(chars, ends))
     say("empty batch")
     assert len(s.search_reads(b"", np.zeros(0, np.uint64))) == 0
+    say("reads shorter than k around a longer one")
+    short = b"ACGT" * 10 + b"ACGTA"
+    ends2 = np.array([5, 10, 45], np.uint64)  # 5, 5 and 35 chars: 0, 0 and 5 k-mers
+    r2 = s.search_reads(short, ends2)
+    assert len(r2) == 5 and np.array_equal(r2, orc.index_reads(short, ends2))
     say("reads all shorter than k")
-    r2 = s.search_reads(b"ACGT" * 10, np.array([5, 10, 40], np.uint64))
-    assert len(r2) == 10 and np.array_equal(r2, orc.index_reads(b"ACGT" * 10,
-                                                                np.array([5, 10, 40], np.uint64)))
+    assert len(s.search_reads(b"ACGT" * 10, np.array([5, 10, 40], np.uint64))) == 0
 
 
 def test_searcher_reuse_shrinking_batches(genome_index):

@@ -1,0 +1,69 @@
+"""Serving loop on the MI355X: every request's result equals its result alone."""
+import numpy as np
+import pytest
+
+import sbwt_amd as S
+from colorlib import ColoredIndex, kmer_ends_of, read_stats, threshold_lists
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+@pytest.fixture(scope="module")
+def served(tmp_path_factory):
+    ci = ColoredIndex(tmp_path_factory.mktemp("srv"), n_genomes=6, length=20_000, d=8)
+    return ci, S.Index.load(ci.sbwt_path, ci.colors_path), S.ColorIndex(ci.colors_path)
+
+
+def _requests(ci, n, seed):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        n_reads = int(np.exp(rng.uniform(0, np.log(3000))))
+        lens = rng.integers(75, 251, n_reads)
+        out.append(ci.reads(n_reads, lens, seed=seed * 1000 + i))
+    return out
+
+
+def test_served_results_equal_alone(served):
+    ci, idx, col = served
+    say("serving: mix vs alone")
+    reqs = _requests(ci, 24, seed=1)
+    params = S.color_params(0.7)
+    srv = S.Server(idx, col, slots=4, max_chars_per_slot=200_000, max_reads_per_slot=2000,
+                   move_to_key_kmer=True, params=params)
+    assert srv.slots >= 2
+    ids = [srv.submit(c, e) for c, e in reqs]
+    got = [srv.wait(i) for i in ids]
+    assert srv.peak_in_flight() >= 2  # requests overlapped on the device
+    alone = idx.searcher(200_000 * 4, 50_000, move_to_key_kmer=True)
+    cs = col.searcher(200_000 * 4, 50_000)
+    for (chars, ends), g in zip(reqs, got):
+        res = alone.search_reads(chars, ends)
+        assert np.array_equal(g["kmers"], res)
+        assert np.array_equal(res, ci.index_results(chars, ends))
+        ke = kmer_ends_of(ends)
+        lists, _, _, _ = cs.pseudoalign(res, ke, params)
+        assert g["color_lists"] == lists
+        assert lists == threshold_lists(ci.oracle_counts(res, ke), read_stats(res, ke), 0.7)
+        assert g["submit_s"] <= g["start_s"] <= g["end_s"]
+
+
+def test_malformed_request_does_not_disturb_others(served):
+    ci, idx, col = served
+    say("serving: malformed request in the mix")
+    reqs = _requests(ci, 6, seed=2)
+    srv = S.Server(idx, col, slots=3, max_chars_per_slot=200_000, max_reads_per_slot=2000,
+                   move_to_key_kmer=True, params=S.color_params(0.7))
+    ids = [srv.submit(c, e) for c, e in reqs[:3]]
+    bad_chars, bad_ends = reqs[3]
+    bad = srv.submit(bad_chars, bad_ends[::-1].copy())  # decreasing read ends
+    ids += [srv.submit(c, e) for c, e in reqs[4:]]
+    with pytest.raises(S.SbwtError):
+        srv.wait(bad)
+    alone = idx.searcher(200_000 * 4, 50_000, move_to_key_kmer=True)
+    for rid, (chars, ends) in zip(ids, reqs[:3] + reqs[4:]):
+        assert np.array_equal(srv.wait(rid)["kmers"], alone.search_reads(chars, ends))
