@@ -39,6 +39,24 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--mode", default="index", choices=["index", "batched", "serve", "colors"],
+                    help="index: the headline line (default); batched: C4 plain vs batched "
+                         "form; serve: Poisson request stream, served vs plain sequential; "
+                         "colors: the color stage on the C3 recipe")
+    ap.add_argument("--color-genomes", type=int, default=200)
+    ap.add_argument("--color-roots", type=int, default=20)
+    ap.add_argument("--color-min-len", type=int, default=2_000_000)
+    ap.add_argument("--color-max-len", type=int, default=5_000_000)
+    ap.add_argument("--color-reads", type=int, default=5_000_000)
+    ap.add_argument("--batch-chars", type=int, default=600_000_000)
+    ap.add_argument("--long-reads", type=int, default=1_000_000)
+    ap.add_argument("--short-reads", type=int, default=4_000_000)
+    ap.add_argument("--serve-genomes", type=int, default=200)
+    ap.add_argument("--serve-roots", type=int, default=10)
+    ap.add_argument("--serve-length", type=int, default=5_000_000)
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-max-reads", type=int, default=100_000)
     ap.add_argument("--reads", type=int, default=0, help="override the number of reads")
     ap.add_argument("--cpu-sample-reads", type=int, default=200_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -74,6 +92,12 @@ def main():
     import torch
     import workloads as W
 
+    if args.mode == "batched":
+        return mode_batched(args, configs, S, torch, W)
+    if args.mode == "serve":
+        return mode_serve(args, configs, S, torch, W)
+    if args.mode == "colors":
+        return mode_colors(args, configs, S, torch, W)
     t0 = time.time()
     if args.config == "c1":
         wl = configs.c1(**({"n_reads": args.reads} if args.reads else {}))
@@ -213,6 +237,120 @@ def cpu_baseline(args, wl, bits, built, gpu_res):
             "sample": f"first {n_sample} reads ({len(ref)} k-mers) of the same batch; "
                       "pack + positions + presearch-12 search + classify, results checked "
                       "equal to the GPU's"}
+
+
+def _colored_index(W, S, built, color_of, d, tag):
+    """Color index of a built SBWT, written to a temporary .tcolors and loaded."""
+    import tempfile
+    t0 = time.time()
+    col = built.colors(np.asarray(color_of, np.uint32), d=d)
+    tmp = tempfile.mkdtemp(prefix=f"sbwt_{tag}_")
+    path = os.path.join(tmp, "index.tcolors")
+    col.write(path)
+    km = col.key_marks()
+    idx = S.Index.create([built.bits(c) for c in range(4)], built.n, 31, key_marks=km,
+                         suffix_group_starts=built.bits(4))
+    cidx = S.ColorIndex(path)
+    log(f"colors: {col.num_colors} colors, {col.num_sets} sets ({col.num_dense} dense), "
+        f"{time.time() - t0:.1f} s")
+    os.remove(path)
+    os.rmdir(tmp)
+    return idx, cidx
+
+
+def mode_batched(args, configs, S, torch, W):
+    """C4: plain per-batch launches vs the batched persistent form (reads/s)."""
+    import modes
+    t0 = time.time()
+    wl = configs.c4(args.long_reads, args.short_reads)
+    log(f"workload {wl.name}: {wl.n_reads} reads, {wl.n_chars} chars ({time.time() - t0:.1f} s)")
+    built = W.BuiltSbwt(wl.genomes, wl.genome_ends, configs.K)
+    log(f"SBWT: {built.n} nodes")
+    idx, cidx = _colored_index(W, S, built, np.arange(len(wl.genome_ends)), 20, "c4")
+    r = modes.run_batched(S, torch, wl, idx, cidx, args.batch_chars, log=log)
+    print(json.dumps({"metric": "reads/s (index + color), batched vs plain", "mode": "batched",
+                      "config": {"workload": wl.name + " " + wl.description,
+                                 "sbwt_nodes": int(built.n)}, **r}), flush=True)
+    return 0
+
+
+def mode_serve(args, configs, S, torch, W):
+    """C5-style serving: Poisson requests, served (continuous batching) vs plain sequential."""
+    import modes
+    t0 = time.time()
+    codes, ends = configs.c3_genomes(args.serve_genomes, args.serve_roots, args.serve_length,
+                                     args.serve_length)
+    built = W.BuiltSbwt(codes, ends, configs.K)
+    log(f"serve index: {built.n} nodes ({time.time() - t0:.1f} s)")
+    idx, cidx = _colored_index(W, S, built, np.arange(len(ends)), 20, "serve")
+
+    def reads(seed, n):
+        lens = W.uniform_lengths(seed, n, 75, 250)
+        c, e, _ = W.reads(seed, codes, ends, lens, random_frac=0.0, err_rate=0.01)
+        return c.tobytes(), e
+
+    r = modes.run_serve(S, idx, cidx, reads, args.serve_rate, args.serve_requests,
+                        args.serve_max_reads, 5, S.color_params(0.7), log=log)
+    print(json.dumps({"metric": "p50/p99 request latency + reads/s, served vs plain",
+                      "mode": "serve", "config": {
+                          "index": f"{args.serve_genomes} genomes x {args.serve_length} bp from "
+                                   f"{args.serve_roots} roots (1-5% substitutions), "
+                                   f"{built.n} nodes, {args.serve_genomes} colors",
+                          "requests": "Poisson, N log-uniform in [1, max] reads of U[75,250] bp"},
+                      **r}), flush=True)
+    return 0
+
+
+def mode_colors(args, configs, S, torch, W):
+    """C3 recipe: index search with the key-k-mer move, then the color stage (reads/s)."""
+    t0 = time.time()
+    wl = configs.c3(args.color_reads, args.color_genomes, args.color_roots, args.color_min_len,
+                    args.color_max_len)
+    log(f"workload {wl.name}: {wl.n_reads} reads ({time.time() - t0:.1f} s)")
+    t0 = time.time()
+    built = W.BuiltSbwt(wl.genomes, wl.genome_ends, configs.K)
+    log(f"SBWT: {built.n} nodes ({time.time() - t0:.1f} s)")
+    idx, cidx = _colored_index(W, S, built, np.arange(len(wl.genome_ends)), 20, "c3")
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream()
+    s = idx.searcher(wl.n_chars, wl.n_reads, move_to_key_kmer=True, stream=stream.cuda_stream)
+    cs = cidx.searcher(wl.n_chars, wl.n_reads, stream=stream.cuda_stream)
+    d_chars = torch.from_numpy(wl.chars).to(dev)
+    d_ends = torch.from_numpy(wl.read_ends.view(np.int64)).to(dev)
+    d_res = torch.empty(wl.n_chars, dtype=torch.int64, device=dev)
+    d_off = torch.empty(wl.n_reads + 1, dtype=torch.int64, device=dev)
+    cap = min(wl.n_reads * cidx.num_colors, wl.n_chars)
+    d_ids = torch.empty(cap, dtype=torch.int32, device=dev)
+    params = S.color_params(0.7)
+    times = {"index": [], "color": [], "total": []}
+    n_ids = 0
+    for it in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        s.search_reads_device(d_chars.data_ptr(), wl.n_chars, d_ends.data_ptr(), wl.n_reads,
+                              d_res.data_ptr())
+        stream.synchronize()
+        t1 = time.perf_counter()
+        n_ids = cs.pseudoalign_device(d_res.data_ptr(), s.kmer_ends_device(), wl.n_reads, params,
+                                      d_off.data_ptr(), d_ids.data_ptr(), cap)
+        stream.synchronize()
+        t2 = time.perf_counter()
+        if it >= args.warmup:
+            times["index"].append(t1 - t0)
+            times["color"].append(t2 - t1)
+            times["total"].append(t2 - t0)
+    med = {k: float(np.median(v)) for k, v in times.items()}
+    out = {"metric": "reads/s (color)", "mode": "colors",
+           "value": wl.n_reads / med["total"], "unit": "reads/s",
+           "color_stage_reads_per_s": wl.n_reads / med["color"],
+           "index_stage_kmers_per_s": wl.n_kmers() / med["index"],
+           "color_kernel_ms": cs.last_time(), "ids_per_step": n_ids,
+           "config": {"workload": wl.name + " " + wl.description, "sbwt_nodes": int(built.n),
+                      "num_colors": int(cidx.num_colors),
+                      "color_sets": int(cidx.num_color_sets), "threshold": 0.7},
+           "seconds": med}
+    print(json.dumps(out), flush=True)
+    return 0
 
 
 def configs_k():

@@ -1,0 +1,188 @@
+"""Measurement modes beyond the headline index search (used by bench.py --mode ...).
+
+batched: the uneven-length workload (C4) through the plain form -- one launch per
+  batch of the per-k-mer search, then the color search, batch after batch -- and the
+  batched form -- the streaming persistent-lane search over a device work queue of
+  (read, k-mer range) chunks plus the color search, with batches alternating over two
+  streams so one batch's tail overlaps the next one's start. Outputs are checked equal.
+colors: the color stage alone on a colored workload (C3 recipe), reads/s.
+serve: a Poisson stream of requests (N log-uniform in [1, max] reads of U[75, 250] bp)
+  through the serving loop vs plain sequential processing; p50/p99 latency, reads/s.
+"""
+import time
+
+import numpy as np
+
+
+def _split_batches(read_ends, max_chars):
+    """[(read0, read1)] of whole reads with at most max_chars chars per batch."""
+    out = []
+    n = len(read_ends)
+    r0 = 0
+    ends = read_ends.astype(np.int64)
+    while r0 < n:
+        c0 = int(ends[r0 - 1]) if r0 else 0
+        r1 = int(np.searchsorted(ends, c0 + max_chars, side="right"))
+        r1 = max(r1, r0 + 1)
+        out.append((r0, r1))
+        r0 = r1
+    return out
+
+
+def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print):
+    dev = torch.device("cuda:0")
+    d_chars = torch.from_numpy(wl.chars).to(dev)
+    ends64 = wl.read_ends.astype(np.int64)
+    batches = _split_batches(wl.read_ends, batch_chars)
+    # per-batch relative read ends, resident on the device like the reads
+    rel = []
+    for r0, r1 in batches:
+        c0 = int(ends64[r0 - 1]) if r0 else 0
+        rel.append((c0, int(ends64[r1 - 1]) - c0, torch.from_numpy(ends64[r0:r1] - c0).to(dev)))
+    max_chars = max(n for _, n, _ in rel)
+    max_reads = max(r1 - r0 for r0, r1 in batches)
+    nc = col.num_colors
+    params = S.color_params(tau)
+
+    def make_lane(algo):
+        stream = torch.cuda.Stream()
+        s = idx.searcher(max_chars, max_reads, move_to_key_kmer=True, stream=stream.cuda_stream)
+        s.set_algorithm(algo)
+        cs = col.searcher(max_chars, max_reads, stream=stream.cuda_stream)
+        res = torch.empty(max_chars, dtype=torch.int64, device=dev)
+        off = torch.empty(max_reads + 1, dtype=torch.int64, device=dev)
+        cap = max_reads * nc
+        ids = torch.empty(cap, dtype=torch.int32, device=dev)
+        return {"stream": stream, "s": s, "cs": cs, "res": res, "off": off, "ids": ids,
+                "cap": cap}
+
+    def run_batch(lane, b):
+        c0, n, d_ends = rel[b]
+        r0, r1 = batches[b]
+        lane["s"].search_reads_device(d_chars.data_ptr() + c0, n, d_ends.data_ptr(), r1 - r0,
+                                      lane["res"].data_ptr())
+        return lane["cs"].pseudoalign_device(lane["res"].data_ptr(), lane["s"].kmer_ends_device(),
+                                             r1 - r0, params, lane["off"].data_ptr(),
+                                             lane["ids"].data_ptr(), lane["cap"])
+
+    def checksum(lane, b, n_ids):
+        ids = lane["ids"][:n_ids].to(torch.int64)
+        return (int(n_ids), int(ids.sum().item()) if n_ids else 0)
+
+    # plain: one launch per batch, per-k-mer search, batch after batch
+    plain = make_lane(S.ALGO_PER_KMER)
+    sums_plain = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for b in range(len(batches)):
+        n_ids = run_batch(plain, b)
+        plain["stream"].synchronize()
+        sums_plain.append(checksum(plain, b, n_ids))
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    log(f"plain: {len(batches)} batches in {t_plain:.2f} s")
+
+    # batched: streaming persistent search, two streams in flight
+    lanes = [make_lane(S.ALGO_STREAMING), make_lane(S.ALGO_STREAMING)]
+    import threading
+    sums_b = [None] * len(batches)
+
+    def worker(li):
+        lane = lanes[li]
+        for b in range(li, len(batches), 2):
+            n_ids = run_batch(lane, b)
+            lane["stream"].synchronize()
+            sums_b[b] = checksum(lane, b, n_ids)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    th = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
+    for t in th:
+        t.start()
+    for t in th:
+        t.join()
+    torch.cuda.synchronize()
+    t_batched = time.perf_counter() - t0
+    log(f"batched: {len(batches)} batches in {t_batched:.2f} s")
+    if sums_b != sums_plain:
+        raise SystemExit("batched and plain color outputs differ")
+    return {"reads": wl.n_reads, "batches": len(batches), "batch_chars": batch_chars,
+            "plain_s": t_plain, "batched_s": t_batched,
+            "plain_reads_per_s": wl.n_reads / t_plain,
+            "batched_reads_per_s": wl.n_reads / t_batched,
+            "speedup": t_plain / t_batched}
+
+
+def run_serve(S, idx, col, wl_reads, rate_per_s, n_requests, max_reads, seed, params,
+              slots=4, log=print):
+    """wl_reads(seed, n_reads) -> (chars, ends). Poisson arrivals at rate_per_s."""
+    import threading
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(0, np.log(max_reads), n_requests)).astype(np.int64)
+    sizes = np.maximum(sizes, 1)
+    gaps = rng.exponential(1.0 / rate_per_s, n_requests)
+    reqs = [wl_reads(seed * 100_000 + i, int(sizes[i])) for i in range(n_requests)]
+    total_reads = int(sizes.sum())
+    max_chars = max(len(c) for c, _ in reqs)
+    out = {}
+
+    # served: submit on the arrival schedule, slots refill as they free up
+    srv = S.Server(idx, col, slots=slots, max_chars_per_slot=max(max_chars, 1 << 20),
+                   max_reads_per_slot=max(max_reads, 1), move_to_key_kmer=col is not None,
+                   params=params)
+    lat = [None] * n_requests
+    ids = [None] * n_requests
+    done = threading.Event()
+
+    def collector():
+        for i in range(n_requests):
+            while ids[i] is None:
+                time.sleep(0.0005)
+            r = srv.wait(ids[i])
+            lat[i] = r["end_s"] - r["submit_s"]
+        done.set()
+
+    th = threading.Thread(target=collector)
+    th.start()
+    t0 = time.perf_counter()
+    arrive = t0
+    for i, (c, e) in enumerate(reqs):
+        arrive += gaps[i]
+        while time.perf_counter() < arrive:
+            time.sleep(0.0002)
+        ids[i] = srv.submit(c, e)
+    done.wait()
+    th.join()
+    t_served = time.perf_counter() - t0
+    out["served"] = {"slots": srv.slots, "peak_in_flight": srv.peak_in_flight(),
+                     "p50_ms": 1000 * float(np.percentile(lat, 50)),
+                     "p99_ms": 1000 * float(np.percentile(lat, 99)),
+                     "reads_per_s": total_reads / t_served, "wall_s": t_served}
+    srv.close()
+    log(f"served: {out['served']}")
+
+    # plain: one request at a time on one searcher, same arrival schedule
+    s = idx.searcher(max(max_chars, 1 << 20), max(max_reads, 1), move_to_key_kmer=col is not None)
+    cs = col.searcher(max(max_chars, 1 << 20), max(max_reads, 1)) if col is not None else None
+    lat2 = []
+    t0 = time.perf_counter()
+    arrive = t0
+    for i, (c, e) in enumerate(reqs):
+        arrive += gaps[i]
+        while time.perf_counter() < arrive:
+            time.sleep(0.0002)
+        res = s.search_reads(c, e)
+        if cs is not None:
+            lens = np.diff(np.concatenate([[0], e.astype(np.int64)]))
+            ke = np.cumsum(np.maximum(lens - idx.k + 1, 0)).astype(np.uint64)
+            cs.pseudoalign(res, ke, params, ids_capacity=max(len(res), 1))
+        lat2.append(time.perf_counter() - arrive)
+    t_plain = time.perf_counter() - t0
+    out["plain"] = {"p50_ms": 1000 * float(np.percentile(lat2, 50)),
+                    "p99_ms": 1000 * float(np.percentile(lat2, 99)),
+                    "reads_per_s": total_reads / t_plain, "wall_s": t_plain}
+    log(f"plain: {out['plain']}")
+    out["requests"] = n_requests
+    out["reads"] = total_reads
+    out["rate_per_s"] = rate_per_s
+    return out

@@ -9,7 +9,6 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
-#include <unordered_map>
 #include <vector>
 
 #include "radix.hpp"
@@ -487,15 +486,6 @@ struct bld_colors {
   vector<uint8_t> dense;
 };
 
-namespace {
-struct VecHash {
-  size_t operator()(const vector<uint32_t> &v) const {
-    uint64_t h = 0x12345678ULL + v.size();
-    for (uint32_t x : v) h = mix64(h ^ x);
-    return static_cast<size_t>(h);
-  }
-};
-}  // namespace
 
 extern "C" bld_colors *bld_colors_build(const bld_sbwt *s, const uint8_t *codes,
                                         const uint64_t *seq_ends, uint64_t n_seqs,
@@ -539,41 +529,103 @@ extern "C" bld_colors *bld_colors_build(const bld_sbwt *s, const uint8_t *codes,
     radix_sort_pairs(keys.data(), idx.data(), keys.size(), 64, T);
     col.swap(idx);
   }
-  // 2. color set of every real node, deduplicated into set ids
-  vector<uint32_t> node_set(n, UINT32_MAX);
-  std::unordered_map<vector<uint32_t>, uint32_t, VecHash> set_id;
-  C->set_offsets.push_back(0);
-  vector<uint32_t> cur;
-  for (uint64_t i = 0, j = 0; i < s->n_kmers; ++i) {
-    const uint64_t x = s->kmers[i];
-    while (j < keys.size() && keys[j] < x) ++j;
-    cur.clear();
-    while (j < keys.size() && keys[j] == x) {
-      if (cur.empty() || cur.back() != col[j]) cur.push_back(col[j]);
-      ++j;
+  // 2. color set of every real node, deduplicated into set ids. Pairs are
+  // sorted by (k-mer, color); drop repeated pairs, then the distinct k-mers of
+  // the pair list are exactly s->kmers in order.
+  {
+    uint64_t w = 0;
+    for (uint64_t i = 0; i < keys.size(); ++i) {
+      if (keys[i] == UINT64_MAX && k < 32) break;  // skipped windows sort last
+      if (w && keys[w - 1] == keys[i] && col[w - 1] == col[i]) continue;
+      keys[w] = keys[i];
+      col[w] = col[i];
+      ++w;
     }
-    auto it = set_id.find(cur);
-    uint32_t id;
-    if (it == set_id.end()) {
-      id = static_cast<uint32_t>(set_id.size());
-      set_id.emplace(cur, id);
-      C->set_ids.insert(C->set_ids.end(), cur.begin(), cur.end());
-      C->set_offsets.push_back(C->set_ids.size());
-    } else {
-      id = it->second;
+    keys.resize(w);
+    col.resize(w);
+  }
+  const uint64_t nk = s->n_kmers;
+  vector<uint64_t> run(nk + 1, 0);  // colors of k-mer i: col[run[i] .. run[i+1])
+  {
+    uint64_t r = 0;
+    for (uint64_t j = 0; j < keys.size(); ++j)
+      if (j == 0 || keys[j] != keys[j - 1]) run[r++] = j;
+    if (r != nk) {
+      delete C;
+      return nullptr;
     }
-    node_set[s->node_of_kmer_index(i)] = id;
+    run[nk] = keys.size();
   }
   keys.clear();
   keys.shrink_to_fit();
+  vector<uint64_t> h(nk);
+  vector<uint32_t> order(nk);
+#pragma omp parallel for schedule(static) num_threads(T)
+  for (uint64_t i = 0; i < nk; ++i) {
+    uint64_t x = 0x12345678ULL + (run[i + 1] - run[i]);
+    for (uint64_t j = run[i]; j < run[i + 1]; ++j) x = mix64(x ^ col[j]);
+    h[i] = x;
+    order[i] = static_cast<uint32_t>(i);
+  }
+  if (nk >= (1ULL << 32)) {
+    delete C;
+    return nullptr;
+  }
+  radix_sort_pairs(h.data(), order.data(), nk, 64, T);
+  vector<uint32_t> kmer_set(nk);
+  C->set_offsets.push_back(0);
+  auto same_list = [&](uint64_t a, uint64_t b) {
+    const uint64_t la = run[a + 1] - run[a], lb = run[b + 1] - run[b];
+    return la == lb && std::equal(col.begin() + run[a], col.begin() + run[a + 1], col.begin() + run[b]);
+  };
+  for (uint64_t i = 0; i < nk;) {
+    uint64_t e = i;
+    while (e < nk && h[e] == h[i]) ++e;
+    // equal hashes: one set per distinct list (lists compared exactly)
+    vector<uint64_t> heads;
+    vector<uint32_t> ids;
+    for (uint64_t q = i; q < e; ++q) {
+      const uint64_t kq = order[q];
+      uint32_t id = UINT32_MAX;
+      for (size_t t = 0; t < heads.size(); ++t)
+        if (same_list(heads[t], kq)) id = ids[t];
+      if (id == UINT32_MAX) {
+        id = static_cast<uint32_t>(C->set_offsets.size() - 1);
+        C->set_ids.insert(C->set_ids.end(), col.begin() + run[kq], col.begin() + run[kq + 1]);
+        C->set_offsets.push_back(C->set_ids.size());
+        heads.push_back(kq);
+        ids.push_back(id);
+      }
+      kmer_set[kq] = id;
+    }
+    i = e;
+  }
+  h.clear();
+  h.shrink_to_fit();
+  order.clear();
+  order.shrink_to_fit();
   col.clear();
   col.shrink_to_fit();
+  run.clear();
+  run.shrink_to_fit();
+  vector<uint32_t> node_set(n, UINT32_MAX);
+#pragma omp parallel for schedule(static) num_threads(T)
+  for (uint64_t i = 0; i < nk; ++i) node_set[s->node_of_kmer_index(i)] = kmer_set[i];
+  kmer_set.clear();
+  kmer_set.shrink_to_fit();
   // 3. key k-mer marks
   const uint64_t words = (n + 63) / 64;
   C->key_marks.assign(words + 1, 0);
+  auto mark = [&](uint64_t v) {
+    __atomic_fetch_or(&C->key_marks[v >> 6], 1ULL << (v & 63), __ATOMIC_RELAXED);
+  };
+  auto marked = [&](uint64_t v) {
+    return (__atomic_load_n(&C->key_marks[v >> 6], __ATOMIC_RELAXED) >> (v & 63)) & 1;
+  };
   WordRank wr[4];
   for (int c = 0; c < 4; ++c) wr[c].build(s->bits[c]);
   vector<uint64_t> succ(n, UINT64_MAX);  // unique successor of unmarked real nodes
+#pragma omp parallel for schedule(static, 4096) num_threads(T)
   for (uint64_t v = 0; v < n; ++v) {
     if (node_set[v] == UINT32_MAX) continue;  // dummy
     const bool alone = bit(s->bits[4], v) && (v + 1 == n || bit(s->bits[4], v + 1));
@@ -584,38 +636,43 @@ extern "C" bld_colors *bld_colors_build(const bld_sbwt *s, const uint8_t *codes,
         c1 = c;
       }
     if (!alone || outdeg != 1) {
-      set_bit(C->key_marks, v);
+      mark(v);
       continue;
     }
     const uint64_t w = s->c_map[c1] + wr[c1].rank(v);
     if (node_set[w] != node_set[v]) {
-      set_bit(C->key_marks, v);
+      mark(v);
       continue;
     }
     succ[v] = w;
   }
-  // every d-th node along the remaining chains and cycles
+  // every d-th node along the remaining chains (in-degree <= 1, so chains are
+  // disjoint and walk in parallel), then the cycles
   vector<uint8_t> has_pred(n, 0), seen(n, 0);
+#pragma omp parallel for schedule(static, 4096) num_threads(T)
   for (uint64_t v = 0; v < n; ++v)
-    if (succ[v] != UINT64_MAX && !bit(C->key_marks, succ[v])) has_pred[succ[v]] = 1;
+    if (succ[v] != UINT64_MAX && !marked(succ[v])) has_pred[succ[v]] = 1;
   const uint32_t dd = d ? d : 1;
   auto walk = [&](uint64_t v, bool cycle) {
     uint64_t steps = 0;
     if (cycle) {
-      set_bit(C->key_marks, v);
+      mark(v);
       seen[v] = 1;
       v = succ[v];
     }
-    while (v != UINT64_MAX && !seen[v] && !bit(C->key_marks, v)) {
+    while (v != UINT64_MAX && !seen[v] && !marked(v)) {
       seen[v] = 1;
-      if (++steps % dd == 0) set_bit(C->key_marks, v);
+      if (++steps % dd == 0) mark(v);
       v = succ[v];
     }
   };
+#pragma omp parallel for schedule(dynamic, 4096) num_threads(T)
   for (uint64_t v = 0; v < n; ++v)
-    if (succ[v] != UINT64_MAX && !bit(C->key_marks, v) && !has_pred[v] && !seen[v]) walk(v, false);
+    if (succ[v] != UINT64_MAX && !has_pred[v] && !marked(v)) walk(v, false);
   for (uint64_t v = 0; v < n; ++v)
-    if (succ[v] != UINT64_MAX && !bit(C->key_marks, v) && !seen[v]) walk(v, true);
+    if (succ[v] != UINT64_MAX && !marked(v) && !seen[v]) walk(v, true);
+  succ.clear();
+  succ.shrink_to_fit();
   // 4. color_set_idxs over key nodes; dense/sparse choice per set
   for (uint64_t v = 0; v < n; ++v)
     if (bit(C->key_marks, v)) C->color_set_idxs.push_back(node_set[v] == UINT32_MAX ? 0 : node_set[v]);

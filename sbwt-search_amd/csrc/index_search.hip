@@ -272,13 +272,14 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_reads(
 // kChunkKmers consecutive k-mers of one read. A lane walks its chunk k-mer by
 // k-mer; when k-mer x was found at node v, the next k-mer y = x[1..k-1] c is
 // one SBWT step from the suffix group of v: if v carries edge c, y is node
-// C[c] + rank_c(v) (one rank-block line); if v carries another edge, y is
-// absent; if v carries none, the group start gs (the last set bit of the
-// suffix-group starts at or before v) decides the same way. A k-mer that
-// follows an absent, invalid or chunk-initial one is searched from its
-// presearch interval as in search_window. Results equal the per-k-mer
-// search: each k-mer's node is unique. When a lane finishes its chunk it
-// takes the next one from a device-wide counter (one atomic per wave).
+// C[c] + rank_c(v) (one rank-block line); if v starts its suffix group and
+// lacks c, y is absent; otherwise the group start gs (the last set bit of the
+// suffix-group starts at or before v, which carries the group's edges)
+// decides the same way. A k-mer that follows an absent, invalid or
+// chunk-initial one is searched from its presearch interval as in
+// search_window. Results equal the per-k-mer search: each k-mer's node is
+// unique. When a lane finishes its chunk it takes the next one from a
+// device-wide counter (one atomic per wave).
 // ---------------------------------------------------------------------------
 constexpr u32 kChunkKmers = 256;
 
@@ -287,52 +288,15 @@ __device__ __forceinline__ u32 lanes_below(u64 mask) {
                                    __builtin_amdgcn_mbcnt_lo(static_cast<u32>(mask), 0u));
 }
 
-// Last suffix-group start at or before v (bit 0 is always set).
-__device__ __forceinline__ u64 group_start(const u64 *__restrict__ sgs, u64 v) {
-  u64 w = v >> 6;
-  u64 x = sgs[w] & ((2ULL << (v & 63)) - 1);  // bits at or below v (all for v % 64 == 63)
-  while (x == 0) x = sgs[--w];
-  return (w << 6) + 63 - static_cast<u64>(__builtin_clzll(x));
-}
-
-enum StreamStep : u32 { kStepFound = 0, kStepAbsent = 1, kStepUnknown = 2 };
-
-// Successor of found node v by letter c; `lines` counts extra lines read.
-template <typename Idx>
-__device__ __forceinline__ u32 stream_step(const IndexView &ix, Idx v, u32 c, Idx &next, u32 &lines) {
-  const Idx b = block_of(v);
-  const u32 o = static_cast<u32>(v - b * kBlockNodes);
-  const uint4 *blk = ix.blocks + static_cast<u64>(b) * 4;
-  const uint4 e = blk[c];
-  if (entry_bit(e, o)) {
-    next = cmap_of<Idx>(ix, c) + entry_base(ix, e, b, c) + entry_popc(e, o);
-    return kStepFound;
-  }
-  bool any = false;
-#pragma unroll
-  for (u32 cc = 0; cc < 4; ++cc)
-    if (cc != c) any |= entry_bit(blk[cc], o);
-  if (any) return kStepAbsent;  // v leads its group and has no edge c
-  if (!ix.sgs) return kStepUnknown;
-  const Idx gs = static_cast<Idx>(group_start(ix.sgs, static_cast<u64>(v)));
-  ++lines;
-  if (gs == v) return kStepAbsent;  // a group without outgoing edges
-  const Idx b2 = block_of(gs);
-  const u32 o2 = static_cast<u32>(gs - b2 * kBlockNodes);
-  const uint4 e2 = ix.blocks[static_cast<u64>(b2) * 4 + c];
-  ++lines;
-  if (!entry_bit(e2, o2)) return kStepAbsent;
-  next = cmap_of<Idx>(ix, c) + entry_base(ix, e2, b2, c) + entry_popc(e2, o2);
-  return kStepFound;
-}
-
-// Per read: number of chunks.
-__global__ __launch_bounds__(256) void k_chunk_count(const u64 *__restrict__ kmer_ends, u64 n_reads,
-                                                      u64 *__restrict__ counts) {
+// Per read: number of chunks (none for a read ending past n_chars, so malformed
+// read ends never send the search outside the batch).
+__global__ __launch_bounds__(256) void k_chunk_count(const u64 *__restrict__ kmer_ends,
+                                                      const u64 *__restrict__ char_ends, u64 n_reads,
+                                                      u64 n_chars, u64 *__restrict__ counts) {
   for (u64 r = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; r < n_reads;
        r += static_cast<u64>(gridDim.x) * blockDim.x) {
     const u64 nk = kmer_ends[r] - (r ? kmer_ends[r - 1] : 0);
-    counts[r] = (nk + kChunkKmers - 1) / kChunkKmers;
+    counts[r] = char_ends[r] > n_chars ? 0 : (nk + kChunkKmers - 1) / kChunkKmers;
   }
 }
 
@@ -358,19 +322,38 @@ __global__ __launch_bounds__(256) void k_chunk_fill(const u64 *__restrict__ kmer
 
 __global__ void k_reset_counter(u64 *c) { *c = 0; }
 
+// Every loop iteration, each lane issues exactly two independent 16-byte
+// loads whose addresses depend on its state, so a wave pays one memory round
+// trip per iteration however its lanes diverge:
+//   kSeq0/kSeq1  64 packed chars + their 64 invalid bits (a lane keeps 128
+//                chars in registers; k-mer windows are shifted out of them)
+//   kStream      rank-block entry of letter c at v  + suffix-group-start bits at v
+//   kGsScan      suffix-group-start bits further left (group start not yet seen)
+//   kGsEntry     rank-block entry of letter c at the group start
+//   kPresearch   presearch interval of the k-mer's first kp letters
+//   kStep        rank-block entries of letter c at l and at r + 1
 template <typename Idx, bool KEY, bool STATS>
 __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     IndexView ix, const u64 *__restrict__ packed, const u32 *__restrict__ invalid,
     const ulonglong2 *__restrict__ chunks, const u64 *__restrict__ chunk_ends, u64 n_reads,
     u64 *__restrict__ counter, u64 *__restrict__ out, u64 *__restrict__ stats) {
-  enum : u32 { kIdle = 0, kNew = 1, kSearch = 2, kDone = 3 };
+  enum : u32 { kIdle, kSeq0, kSeq1, kStream, kGsScan, kGsEntry, kPresearch, kStep, kDone };
   const u64 n_chunks = chunk_ends[n_reads - 1];
   const u32 lane = threadIdx.x & 63;
-  const u64 inv_mask = ix.k >= 64 ? ~0ULL : ((1ULL << ix.k) - 1);
+  const u32 k = ix.k, kp = ix.kp;
+  const u64 inv_mask = (1ULL << k) - 1;
+  const uint4 *packed4 = reinterpret_cast<const uint4 *>(packed);   // 64 chars each
+  const uint4 *invalid4 = reinterpret_cast<const uint4 *>(invalid); // 128 chars each
+  const uint4 *sgs4 = reinterpret_cast<const uint4 *>(ix.sgs);      // 128 nodes each
+  const uint4 *pre4 = reinterpret_cast<const uint4 *>(ix.presearch);
+  const uint4 *dummy = ix.blocks;
   u32 mode = kIdle;
-  u64 g = 0, p = 0, window = 0;
-  u32 left = 0, j = 0;
-  Idx l = 0, r = 0, v = 0;
+  u64 g = 0, p = 0, buf = 0;          // output slot, char position, buffer start (64-aligned)
+  u64 w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // packed chars [buf, buf + 128)
+  u64 i0 = 0, i1 = 0;                  // their invalid bits
+  u64 window = 0, gs_at = 0;
+  u32 left = 0, j = 0, c = 0;
+  Idx l = 0, r = 0, v = 0, gs = 0;
   bool prev = false;
   u32 st_full = 0, st_steps = 0, st_walk = 0, st_inv = 0, st_stream = 0, st_extra = 0, st_fb = 0;
   for (;;) {
@@ -387,101 +370,234 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
           g = ch.x;
           p = ch.y & ((1ULL << 48) - 1);
           left = static_cast<u32>(ch.y >> 48);
+          buf = p & ~63ULL;
           prev = false;
-          mode = kNew;
+          mode = kSeq0;
         } else {
           mode = kDone;
         }
       }
     }
     if (__ballot(mode != kDone) == 0) break;
-    u64 result = 0;
-    bool emit = false;
-    if (mode == kNew) {
-      window = window_at(packed, p);
-      const u64 iw = p >> 5;
-      const u64 ib = ((static_cast<u64>(invalid[iw + 1]) << 32) | invalid[iw]) >> (p & 31);
-      bool full = true;
-      if (ib & inv_mask) {
-        result = kInvalid;
-        emit = true;
-        prev = false;
-        full = false;
-        if (STATS) ++st_inv;
-      } else if (prev) {
-        const u32 c = static_cast<u32>(window >> (64 - 2 * ix.k)) & 3u;
-        Idx nx = 0;
-        u32 extra = 0;
-        const u32 how = stream_step<Idx>(ix, v, c, nx, extra);
-        if (STATS) {
-          ++st_stream;
-          st_extra += extra;
-        }
-        if (how == kStepFound) {
-          v = nx;
-          if constexpr (KEY) {
-            u32 w = 0;
-            result = key_walk<Idx>(ix, v, ix.num_nodes, w);
-            if (STATS) st_walk += w;
+    // ---- the two loads of this iteration
+    const uint4 *a1 = dummy, *a2 = dummy;
+    switch (mode) {
+      case kSeq0:
+        a1 = packed4 + (buf >> 6);
+        a2 = invalid4 + (buf >> 7);
+        break;
+      case kSeq1:
+        a1 = packed4 + (buf >> 6) + 1;
+        a2 = invalid4 + ((buf + 64) >> 7);
+        break;
+      case kStream:
+        a1 = ix.blocks + static_cast<u64>(block_of(v)) * 4 + c;
+        a2 = ix.sgs ? sgs4 + (static_cast<u64>(v) >> 7) : a1;
+        break;
+      case kGsScan:
+        a1 = a2 = sgs4 + (gs_at >> 7);
+        break;
+      case kGsEntry:
+        a1 = a2 = ix.blocks + static_cast<u64>(block_of(gs)) * 4 + c;
+        break;
+      case kPresearch: {
+        const u64 pre = window >> (64 - 2 * kp);
+        a1 = a2 = pre4 + (sizeof(Idx) == 4 ? (pre >> 1) : pre);
+        break;
+      }
+      case kStep:
+        a1 = ix.blocks + static_cast<u64>(block_of(l)) * 4 + c;
+        a2 = ix.blocks + static_cast<u64>(block_of(static_cast<Idx>(r + 1))) * 4 + c;
+        break;
+      default:
+        break;
+    }
+    const uint4 d1 = *a1;
+    const uint4 d2 = *a2;
+    // ---- advance the lane's state
+    bool emitted = false, decide = false;
+    u64 res = 0;
+    Idx found_at = 0;
+    bool found = false;
+    switch (mode) {
+      case kSeq0: {
+        w0 = static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32);
+        w1 = static_cast<u64>(d1.z) | (static_cast<u64>(d1.w) << 32);
+        i0 = (buf & 64) ? (static_cast<u64>(d2.z) | (static_cast<u64>(d2.w) << 32))
+                        : (static_cast<u64>(d2.x) | (static_cast<u64>(d2.y) << 32));
+        mode = kSeq1;
+        break;
+      }
+      case kSeq1: {
+        w2 = static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32);
+        w3 = static_cast<u64>(d1.z) | (static_cast<u64>(d1.w) << 32);
+        i1 = ((buf + 64) & 64) ? (static_cast<u64>(d2.z) | (static_cast<u64>(d2.w) << 32))
+                               : (static_cast<u64>(d2.x) | (static_cast<u64>(d2.y) << 32));
+        decide = true;
+        break;
+      }
+      case kStream: {
+        const Idx b = block_of(v);
+        const u32 o = static_cast<u32>(v - b * kBlockNodes);
+        if (STATS) ++st_stream;
+        if (entry_bit(d1, o)) {
+          found_at = cmap_of<Idx>(ix, c) + entry_base(ix, d1, b, c) + entry_popc(d1, o);
+          found = true;
+        } else if (!ix.sgs) {
+          if (STATS) ++st_fb;
+          mode = kPresearch;  // the group start is unknown: search afresh
+        } else {
+          const u32 bit = static_cast<u32>(static_cast<u64>(v) & 127);
+          const u64 lo = static_cast<u64>(d2.x) | (static_cast<u64>(d2.y) << 32);
+          const u64 hi = static_cast<u64>(d2.z) | (static_cast<u64>(d2.w) << 32);
+          const u64 word = bit < 64 ? lo : hi;
+          if ((word >> (bit & 63)) & 1ULL) {  // v starts its group and lacks edge c
+            res = kNotFound;
+            emitted = true;
+            prev = false;
           } else {
-            result = static_cast<u64>(v);
+            // last group start below v within these 128 nodes, if any
+            u64 m = word & ((1ULL << (bit & 63)) - 1);
+            u32 pos = 0;
+            bool hit = false;
+            if (m) {
+              pos = (bit & 64) + 63 - static_cast<u32>(__builtin_clzll(m));
+              hit = true;
+            } else if (bit >= 64 && lo) {
+              pos = 63 - static_cast<u32>(__builtin_clzll(lo));
+              hit = true;
+            }
+            if (STATS) ++st_extra;
+            if (hit) {
+              gs = static_cast<Idx>((static_cast<u64>(v) & ~127ULL) + pos);
+              mode = kGsEntry;
+            } else {
+              gs_at = (static_cast<u64>(v) & ~127ULL) - 1;
+              mode = kGsScan;
+            }
           }
-          emit = true;
-          full = false;
-        } else if (how == kStepAbsent) {
-          result = kNotFound;
-          emit = true;
+        }
+        break;
+      }
+      case kGsScan: {
+        const u64 lo = static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32);
+        const u64 hi = static_cast<u64>(d1.z) | (static_cast<u64>(d1.w) << 32);
+        if (STATS) ++st_extra;
+        if (hi | lo) {
+          const u32 pos = hi ? 127 - static_cast<u32>(__builtin_clzll(hi))
+                             : 63 - static_cast<u32>(__builtin_clzll(lo));
+          gs = static_cast<Idx>((gs_at & ~127ULL) + pos);
+          mode = kGsEntry;
+        } else {
+          gs_at -= 128;  // bit 0 of the vector is set, so this ends
+        }
+        break;
+      }
+      case kGsEntry: {
+        const Idx b = block_of(gs);
+        const u32 o = static_cast<u32>(gs - b * kBlockNodes);
+        if (STATS) ++st_extra;
+        if (entry_bit(d1, o)) {
+          found_at = cmap_of<Idx>(ix, c) + entry_base(ix, d1, b, c) + entry_popc(d1, o);
+          found = true;
+        } else {
+          res = kNotFound;
+          emitted = true;
           prev = false;
-          full = false;
-        } else if (STATS) {
-          ++st_fb;
         }
+        break;
       }
-      if (full) {
-        const u64 pre = window >> (64 - 2 * ix.kp);
+      case kPresearch: {
+        const u64 pre = window >> (64 - 2 * kp);
         if constexpr (sizeof(Idx) == 4) {
-          const uint2 pr = static_cast<const uint2 *>(ix.presearch)[pre];
-          l = pr.x;
-          r = pr.y;
+          l = (pre & 1) ? d1.z : d1.x;
+          r = (pre & 1) ? d1.w : d1.y;
         } else {
-          const ulonglong2 pr = static_cast<const ulonglong2 *>(ix.presearch)[pre];
-          l = pr.x;
-          r = pr.y;
+          l = static_cast<Idx>(static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32));
+          r = static_cast<Idx>(static_cast<u64>(d1.z) | (static_cast<u64>(d1.w) << 32));
         }
-        j = ix.kp;
+        j = kp;
         if (STATS) ++st_full;
-        mode = kSearch;
+        mode = kStep;  // resolved below
+        break;
       }
-    } else if (mode == kSearch) {
-      const u32 c = static_cast<u32>(window >> (62 - 2 * j)) & 3u;
-      sbwt_step<Idx>(ix, c, l, r);
-      ++j;
-      if (STATS) ++st_steps;
+      case kStep: {
+        const Idx cm = cmap_of<Idx>(ix, c);
+        const Idx r1 = r + 1;
+        const Idx bl = block_of(l), br = block_of(r1);
+        l = cm + entry_base(ix, d1, bl, c) + entry_popc(d1, static_cast<u32>(l - bl * kBlockNodes));
+        r = cm + entry_base(ix, d2, br, c) + entry_popc(d2, static_cast<u32>(r1 - br * kBlockNodes)) - 1;
+        ++j;
+        if (STATS) ++st_steps;
+        break;
+      }
+      default:
+        break;
     }
-    if (mode == kSearch) {  // a search just started or stepped: resolve if done
+    if (mode == kStep && !found && !emitted) {  // a search just started or stepped
       if (l > r) {
-        result = kNotFound;
-        emit = true;
+        res = kNotFound;
+        emitted = true;
         prev = false;
-      } else if (j == ix.k) {
-        v = l;
-        prev = true;
-        if constexpr (KEY) {
-          u32 w = 0;
-          result = key_walk<Idx>(ix, v, ix.num_nodes, w);
-          if (STATS) st_walk += w;
-        } else {
-          result = static_cast<u64>(v);
-        }
-        emit = true;
+      } else if (j == k) {
+        found_at = l;
+        found = true;
+      } else {
+        c = static_cast<u32>(window >> (62 - 2 * j)) & 3u;
       }
     }
-    if (emit) {
-      out[g] = result;
-      ++g;
-      ++p;
-      --left;
-      mode = left ? kNew : kIdle;
+    if (found) {
+      v = found_at;
+      prev = true;
+      if constexpr (KEY) {
+        u32 w = 0;
+        res = key_walk<Idx>(ix, v, ix.num_nodes, w);
+        if (STATS) st_walk += w;
+      } else {
+        res = static_cast<u64>(v);
+      }
+      emitted = true;
+    }
+    // ---- emit, then set up the next k-mer (invalid k-mers resolve here)
+    while (emitted || decide) {
+      if (emitted) {
+        out[g] = res;
+        ++g;
+        ++p;
+        --left;
+        emitted = false;
+        if (left == 0) {
+          mode = kIdle;
+          break;
+        }
+      }
+      decide = false;
+      if (p - buf >= 64) {  // slide the 128-char buffer by 64 chars
+        buf += 64;
+        w0 = w2;
+        w1 = w3;
+        i0 = i1;
+        mode = kSeq1;
+        break;
+      }
+      const u32 q = static_cast<u32>(p - buf);
+      const u64 wa = (q & 32) ? w1 : w0, wb = (q & 32) ? w2 : w1;
+      const u32 sh = 2 * (q & 31);
+      window = sh ? ((wa << sh) | (wb >> (64 - sh))) : wa;
+      const u64 iv = q ? ((i0 >> q) | (i1 << (64 - q))) : i0;
+      if (iv & inv_mask) {
+        res = kInvalid;
+        prev = false;
+        emitted = true;
+        if (STATS) ++st_inv;
+        continue;
+      }
+      if (prev) {
+        c = static_cast<u32>(window >> (64 - 2 * k)) & 3u;
+        mode = kStream;
+      } else {
+        mode = kPresearch;
+      }
     }
   }
   if constexpr (STATS) {
@@ -522,7 +638,7 @@ void dispatch_search3(bool wide, bool key, bool stats, F f) {
 }  // namespace
 
 void launch_pack(const char *d_chars, u64 n_chars, u64 *d_packed, u32 *d_invalid, hipStream_t s) {
-  const u64 n_words = (n_chars + 31) / 32 + 2;  // two zero words of padding
+  const u64 n_words = (n_chars + 31) / 32 + 4;  // zero words of padding past the end
   const bool aligned = (reinterpret_cast<uintptr_t>(d_chars) % 16) == 0;
   k_pack<<<grid_for(n_words, 256), 256, 0, s>>>(reinterpret_cast<const u8 *>(d_chars), n_chars,
                                                 n_words, aligned, d_packed, d_invalid);
@@ -540,7 +656,7 @@ Searcher::Searcher(const DeviceIndex *index, u64 max_chars, u64 max_reads, bool 
   if (move_to_key_ && !index_->key_marks.blocks)
     throw std::runtime_error("move to key k-mer requested but the index has no key-k-mer marks");
   if (own_stream_) SBWT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  const u64 words = max_chars_ / 32 + 3;
+  const u64 words = max_chars_ / 32 + 8;
   SBWT_HIP(hipMalloc(&d_packed_, words * 8));
   SBWT_HIP(hipMalloc(&d_invalid_, words * 4));
   SBWT_HIP(hipMalloc(&d_kmer_ends_, std::max<u64>(max_reads_, 1) * 8));
@@ -662,7 +778,8 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
   if (algorithm_ == kAlgoStreaming) {
     // chunk table of (read, k-mer range) work items, then the persistent search
     u64 *d_chunk_ends = d_chunk_counts_ + max_reads_;
-    k_chunk_count<<<grid_for(n_reads, 256), 256, 0, stream_>>>(d_kmer_ends_, n_reads, d_chunk_counts_);
+    k_chunk_count<<<grid_for(n_reads, 256), 256, 0, stream_>>>(d_kmer_ends_, d_read_ends, n_reads,
+                                                               n_chars, d_chunk_counts_);
     scan_inclusive_u64(d_chunk_counts_, n_reads, d_chunk_ends, d_chunk_scan_tmp_, stream_);
     k_chunk_fill<<<grid_for(n_reads, 256), 256, 0, stream_>>>(d_kmer_ends_, d_read_ends, n_reads,
                                                               d_chunk_ends, d_chunks_);
