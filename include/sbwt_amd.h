@@ -50,7 +50,9 @@ int sbwt_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
 /* Device index from the four ACGT bit vectors of an SBWT with num_bits nodes
  * (ceil(num_bits/64) words each). Builds the rank layout, the c-map and the
  * presearch table of presearch_letters letters (0 = the reference's 12, or k
- * if k < 12). key_kmer_marks (nullable, ceil(num_bits/64) words) enables the
+ * if k < 12; at most 16: 4^kp intervals of 8 bytes, 16 if num_bits >= 2^32).
+ * Longer prefixes cut SBWT steps from every search; results do not depend on
+ * it. key_kmer_marks (nullable, ceil(num_bits/64) words) enables the
  * key-k-mer move. suffix_group_starts (nullable, the .sbwt file's vector of
  * the same name) lets the streaming search step from nodes that do not start
  * their suffix group; without it such steps fall back to a full search.

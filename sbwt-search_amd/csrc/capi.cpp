@@ -98,7 +98,7 @@ int sbwt_index_create(const uint64_t *const acgt[4], uint64_t num_bits, uint32_t
     require(kmer_size >= 1 && kmer_size <= 32, "k-mer size must be in [1, 32]");
     for (int c = 0; c < 4; ++c) require(acgt[c] != nullptr, "null bit vector");
     const uint32_t kp = presearch_letters ? presearch_letters : sbwt_amd::default_presearch_letters(kmer_size);
-    require(kp >= 1 && kp <= kmer_size && kp <= 14, "presearch letters must be in [1, min(k, 14)]");
+    require(kp >= 1 && kp <= kmer_size && kp <= 16, "presearch letters must be in [1, min(k, 16)]");
     auto idx = std::make_unique<sbwt_index>();
     idx->dev.build(acgt, num_bits, kmer_size, kp, key_kmer_marks, key_kmer_marks ? num_bits : 0,
                    suffix_group_starts);
@@ -119,7 +119,7 @@ int sbwt_index_load(const char *sbwt_path, const char *colors_path, uint32_t pre
     const uint64_t *acgt[4] = {s.acgt[0].data(), s.acgt[1].data(), s.acgt[2].data(), s.acgt[3].data()};
     const uint32_t k = static_cast<uint32_t>(s.k);
     const uint32_t kp = presearch_letters ? presearch_letters : sbwt_amd::default_presearch_letters(k);
-    require(kp >= 1 && kp <= k && kp <= 14, "presearch letters must be in [1, min(k, 14)]");
+    require(kp >= 1 && kp <= k && kp <= 16, "presearch letters must be in [1, min(k, 16)]");
     auto idx = std::make_unique<sbwt_index>();
     idx->dev.build(acgt, s.num_bits, k, kp, colors_path ? km.words.data() : nullptr,
                    colors_path ? km.bits : 0, s.suffix_group_starts.data());

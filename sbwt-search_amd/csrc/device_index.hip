@@ -56,20 +56,62 @@ T *upload(const T *host, u64 count) {
 // Presearch: interval of every kp-letter prefix x (first letter in the top two
 // of its 2*kp bits), src/Presearcher/Presearcher.cuh:19-41.
 template <typename Idx, typename Pair>
-__global__ __launch_bounds__(256) void k_presearch(IndexView ix, Pair *out, u64 count) {
-  const u64 x = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (x >= count) return;
-  u32 c = static_cast<u32>(x >> (2 * ix.kp - 2)) & 3u;
-  Idx l = static_cast<Idx>(ix.c_map[c]);
-  Idx r = static_cast<Idx>(ix.c_map[c + 1] - 1);
-  for (int sh = 2 * static_cast<int>(ix.kp) - 4; sh >= 0; sh -= 2) {
-    c = static_cast<u32>(x >> sh) & 3u;
-    sbwt_step<Idx>(ix, c, l, r);
+__global__ __launch_bounds__(256) void k_presearch(IndexView ix, u32 kp, Pair *out, u64 count) {
+  for (u64 x = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; x < count;
+       x += static_cast<u64>(gridDim.x) * blockDim.x) {
+    u32 c = static_cast<u32>(x >> (2 * kp - 2)) & 3u;
+    Idx l = static_cast<Idx>(ix.c_map[c]);
+    Idx r = static_cast<Idx>(ix.c_map[c + 1] - 1);
+    for (int sh = 2 * static_cast<int>(kp) - 4; sh >= 0; sh -= 2) {
+      c = static_cast<u32>(x >> sh) & 3u;
+      sbwt_step<Idx>(ix, c, l, r);
+    }
+    Pair p;
+    p.x = l;
+    p.y = r;
+    out[x] = p;
   }
-  Pair p;
-  p.x = l;
-  p.y = r;
-  out[x] = p;
+}
+
+// Longer prefixes from a base table of kb-letter intervals: the interval of
+// x is that of its first kb letters stepped by the remaining ones. Stepping
+// stops once the interval is empty (l > r), which every later step keeps.
+template <typename Idx, typename Pair>
+__global__ __launch_bounds__(256) void k_presearch_extend(IndexView ix, const Pair *__restrict__ base,
+                                                          u32 kb, u32 kp, Pair *out, u64 count) {
+  const u32 extra = 2 * (kp - kb);
+  for (u64 x = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; x < count;
+       x += static_cast<u64>(gridDim.x) * blockDim.x) {
+    const Pair b = base[x >> extra];
+    Idx l = static_cast<Idx>(b.x), r = static_cast<Idx>(b.y);
+    for (int sh = static_cast<int>(extra) - 2; sh >= 0 && l <= r; sh -= 2)
+      sbwt_step<Idx>(ix, static_cast<u32>(x >> sh) & 3u, l, r);
+    Pair p;
+    p.x = l;
+    p.y = r;
+    out[x] = p;
+  }
+}
+
+template <typename Idx, typename Pair>
+void build_presearch(const IndexView &ix, u32 kp, Pair *out) {
+  constexpr u32 kBase = 12;
+  const u64 count = 1ULL << (2 * kp);
+  const u32 grid = static_cast<u32>(std::min<u64>((count + 255) / 256, 1u << 20));
+  if (kp <= kBase) {
+    k_presearch<Idx, Pair><<<grid, 256>>>(ix, kp, out, count);
+    SBWT_HIP(hipGetLastError());
+    return;
+  }
+  const u64 base_count = 1ULL << (2 * kBase);
+  Pair *base = nullptr;
+  SBWT_HIP(hipMalloc(&base, base_count * sizeof(Pair)));
+  k_presearch<Idx, Pair><<<static_cast<u32>(base_count / 256), 256>>>(ix, kBase, base, base_count);
+  SBWT_HIP(hipGetLastError());
+  k_presearch_extend<Idx, Pair><<<grid, 256>>>(ix, base, kBase, kp, out, count);
+  SBWT_HIP(hipGetLastError());
+  SBWT_HIP(hipDeviceSynchronize());
+  SBWT_HIP(hipFree(base));
 }
 
 }  // namespace
@@ -153,17 +195,14 @@ void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const
   // presearch table on the device
   const u64 count = 1ULL << (2 * kp);
   const IndexView ix = view();
-  const u64 grid = (count + 255) / 256;
   if (wide) {
     presearch_bytes = count * sizeof(ulonglong2);
     SBWT_HIP(hipMalloc(&presearch, presearch_bytes));
-    k_presearch<u64, ulonglong2><<<dim3(static_cast<u32>(grid)), dim3(256)>>>(
-        ix, static_cast<ulonglong2 *>(presearch), count);
+    build_presearch<u64, ulonglong2>(ix, kp, static_cast<ulonglong2 *>(presearch));
   } else {
     presearch_bytes = count * sizeof(uint2);
     SBWT_HIP(hipMalloc(&presearch, presearch_bytes));
-    k_presearch<u32, uint2><<<dim3(static_cast<u32>(grid)), dim3(256)>>>(
-        ix, static_cast<uint2 *>(presearch), count);
+    build_presearch<u32, uint2>(ix, kp, static_cast<uint2 *>(presearch));
   }
   SBWT_HIP(hipGetLastError());
   SBWT_HIP(hipDeviceSynchronize());

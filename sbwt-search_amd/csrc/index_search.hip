@@ -325,6 +325,7 @@ __global__ void k_reset_counter(u64 *c) { *c = 0; }
 // Every loop iteration, each lane issues exactly two independent 16-byte
 // loads whose addresses depend on its state, so a wave pays one memory round
 // trip per iteration however its lanes diverge:
+//   kChunk       the chunk's descriptor (output slot, char position, count)
 //   kSeq0/kSeq1  64 packed chars + their 64 invalid bits (a lane keeps 128
 //                chars in registers; k-mer windows are shifted out of them)
 //   kStream      rank-block entry of letter c at v  + suffix-group-start bits at v
@@ -332,12 +333,19 @@ __global__ void k_reset_counter(u64 *c) { *c = 0; }
 //   kGsEntry     rank-block entry of letter c at the group start
 //   kPresearch   presearch interval of the k-mer's first kp letters
 //   kStep        rank-block entries of letter c at l and at r + 1
+// Work distribution: each wave owns a range of chunks and hands them to its
+// lanes as they finish (ballot + popcount, no atomics). The first half of the
+// chunks is split evenly over the waves up front; the rest is taken in ranges
+// of 64 from a device-wide counter, one atomic per range issued beside the
+// loads. Device-scope atomics on one address serialize at the memory side, so
+// a counter touched once per lane refill stalls every wave; per 64 chunks it
+// only balances the tail.
 template <typename Idx, bool KEY, bool STATS>
 __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     IndexView ix, const u64 *__restrict__ packed, const u32 *__restrict__ invalid,
     const ulonglong2 *__restrict__ chunks, const u64 *__restrict__ chunk_ends, u64 n_reads,
     u64 *__restrict__ counter, u64 *__restrict__ out, u64 *__restrict__ stats) {
-  enum : u32 { kIdle, kSeq0, kSeq1, kStream, kGsScan, kGsEntry, kPresearch, kStep, kDone };
+  enum : u32 { kChunk, kSeq0, kSeq1, kStream, kGsScan, kGsEntry, kPresearch, kStep, kDone };
   const u64 n_chunks = chunk_ends[n_reads - 1];
   const u32 lane = threadIdx.x & 63;
   const u32 k = ix.k, kp = ix.kp;
@@ -346,9 +354,20 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
   const uint4 *invalid4 = reinterpret_cast<const uint4 *>(invalid); // 128 chars each
   const uint4 *sgs4 = reinterpret_cast<const uint4 *>(ix.sgs);      // 128 nodes each
   const uint4 *pre4 = reinterpret_cast<const uint4 *>(ix.presearch);
+  const uint4 *chunks4 = reinterpret_cast<const uint4 *>(chunks);
   const uint4 *dummy = ix.blocks;
-  u32 mode = kIdle;
-  u64 g = 0, p = 0, buf = 0;          // output slot, char position, buffer start (64-aligned)
+  // the wave's unassigned chunks [wnext, wend); the first half of all chunks
+  // is split evenly, the rest comes from the counter (offset by that half)
+  const u64 n_waves = static_cast<u64>(gridDim.x) * (kSearchThreads / 64);
+  const u64 wave = static_cast<u64>(blockIdx.x) * (kSearchThreads / 64) + threadIdx.x / 64;
+  const u64 share = n_chunks / (2 * n_waves);
+  const u64 dyn0 = share * n_waves;
+  u64 wnext = wave * share, wend = wnext + share;
+  bool dry = false;  // the counter is past the last chunk
+  u64 ci = 0;        // the lane's chunk (valid once want is false in kChunk)
+  u32 mode = kChunk;
+  bool want = true;  // the lane waits for a chunk
+  u64 g = 0, p = 0, buf = 0;           // output slot, char position, buffer start (64-aligned)
   u64 w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // packed chars [buf, buf + 128)
   u64 i0 = 0, i1 = 0;                  // their invalid bits
   u64 window = 0, gs_at = 0;
@@ -357,31 +376,37 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
   bool prev = false;
   u32 st_full = 0, st_steps = 0, st_walk = 0, st_inv = 0, st_stream = 0, st_extra = 0, st_fb = 0;
   for (;;) {
-    const u64 idle = __ballot(mode == kIdle);
-    if (idle) {
-      u64 base = 0;
-      if (lane == 0) base = atomicAdd(reinterpret_cast<unsigned long long *>(counter),
-                                      static_cast<unsigned long long>(__popcll(idle)));
-      base = __shfl(base, 0, 64);
-      if (mode == kIdle) {
-        const u64 ci = base + lanes_below(idle);
-        if (ci < n_chunks) {
-          const ulonglong2 ch = chunks[ci];
-          g = ch.x;
-          p = ch.y & ((1ULL << 48) - 1);
-          left = static_cast<u32>(ch.y >> 48);
-          buf = p & ~63ULL;
-          prev = false;
-          mode = kSeq0;
-        } else {
-          mode = kDone;
-        }
+    // hand the wave's chunks to lanes that wait for one
+    const u64 need = __ballot(want);
+    if (need != 0) {
+      const u64 avail = wend - wnext;
+      if (want && lanes_below(need) < avail) {
+        ci = wnext + lanes_below(need);
+        want = false;
+      } else if (want && dry) {
+        want = false;
+        mode = kDone;
       }
+      wnext += min(static_cast<u64>(__popcll(need)), avail);
     }
     if (__ballot(mode != kDone) == 0) break;
+    // an empty range is refilled by one atomic, answered with this iteration's loads
+    const bool refill = !dry && wnext == wend;
+    u64 tbase = 0;
+    if (refill && lane == 0)
+      tbase = atomicAdd(reinterpret_cast<unsigned long long *>(counter), 64ULL);
     // ---- the two loads of this iteration
+    // nodes of the entry loads (0 in other states: indexes stay in range)
+    const Idx n1 = mode == kStream ? v : (mode == kGsEntry ? gs : (mode == kStep ? l : Idx(0)));
+    const Idx n2 = mode == kStep ? static_cast<Idx>(r + 1) : Idx(0);
+    const Idx b1 = block_of(n1), b2 = block_of(n2);
+    const u32 o1 = static_cast<u32>(n1 - b1 * kBlockNodes), o2 = static_cast<u32>(n2 - b2 * kBlockNodes);
+    const uint4 *e1 = ix.blocks + static_cast<u64>(b1) * 4 + c;
     const uint4 *a1 = dummy, *a2 = dummy;
     switch (mode) {
+      case kChunk:
+        if (!want) a1 = a2 = chunks4 + ci;
+        break;
       case kSeq0:
         a1 = packed4 + (buf >> 6);
         a2 = invalid4 + (buf >> 7);
@@ -391,14 +416,14 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         a2 = invalid4 + ((buf + 64) >> 7);
         break;
       case kStream:
-        a1 = ix.blocks + static_cast<u64>(block_of(v)) * 4 + c;
-        a2 = ix.sgs ? sgs4 + (static_cast<u64>(v) >> 7) : a1;
+        a1 = e1;
+        a2 = ix.sgs ? sgs4 + (static_cast<u64>(v) >> 7) : e1;
         break;
       case kGsScan:
         a1 = a2 = sgs4 + (gs_at >> 7);
         break;
       case kGsEntry:
-        a1 = a2 = ix.blocks + static_cast<u64>(block_of(gs)) * 4 + c;
+        a1 = a2 = e1;
         break;
       case kPresearch: {
         const u64 pre = window >> (64 - 2 * kp);
@@ -406,42 +431,63 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         break;
       }
       case kStep:
-        a1 = ix.blocks + static_cast<u64>(block_of(l)) * 4 + c;
-        a2 = ix.blocks + static_cast<u64>(block_of(static_cast<Idx>(r + 1))) * 4 + c;
+        a1 = e1;
+        a2 = ix.blocks + static_cast<u64>(b2) * 4 + c;
         break;
       default:
         break;
     }
-    const uint4 d1 = *a1;
-    const uint4 d2 = *a2;
+    uint4 d1 = *a1;
+    uint4 d2 = *a2;
+    // keep the loaded words in registers: re-loading them later in a branch
+    // would add a dependent round trip
+    asm volatile("" : "+v"(d1.x), "+v"(d1.y), "+v"(d1.z), "+v"(d1.w), "+v"(d2.x), "+v"(d2.y),
+                 "+v"(d2.z), "+v"(d2.w));
+    if (refill) {
+      tbase = dyn0 + __shfl(tbase, 0, 64);
+      if (tbase >= n_chunks) {
+        dry = true;
+      } else {
+        wnext = tbase;
+        wend = min(tbase + 64, n_chunks);
+      }
+    }
+    // ---- rank arithmetic shared by the entry states (kStream, kGsEntry, kStep)
+    const Idx cm = cmap_of<Idx>(ix, c);
+    const Idx x1 = cm + entry_base(ix, d1, b1, c) + entry_popc(d1, o1);
+    const bool bit1 = entry_bit(d1, o1);
     // ---- advance the lane's state
-    bool emitted = false, decide = false;
-    u64 res = 0;
-    Idx found_at = 0;
-    bool found = false;
+    bool found = false, notfound = false, decide = false, resolve = false;
+    Idx found_at = x1;
     switch (mode) {
-      case kSeq0: {
+      case kChunk:
+        if (!want) {
+          g = static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32);
+          const u64 y = static_cast<u64>(d1.z) | (static_cast<u64>(d1.w) << 32);
+          p = y & ((1ULL << 48) - 1);
+          left = static_cast<u32>(y >> 48);
+          buf = p & ~63ULL;
+          prev = false;
+          mode = kSeq0;
+        }
+        break;
+      case kSeq0:
         w0 = static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32);
         w1 = static_cast<u64>(d1.z) | (static_cast<u64>(d1.w) << 32);
         i0 = (buf & 64) ? (static_cast<u64>(d2.z) | (static_cast<u64>(d2.w) << 32))
                         : (static_cast<u64>(d2.x) | (static_cast<u64>(d2.y) << 32));
         mode = kSeq1;
         break;
-      }
-      case kSeq1: {
+      case kSeq1:
         w2 = static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32);
         w3 = static_cast<u64>(d1.z) | (static_cast<u64>(d1.w) << 32);
         i1 = ((buf + 64) & 64) ? (static_cast<u64>(d2.z) | (static_cast<u64>(d2.w) << 32))
                                : (static_cast<u64>(d2.x) | (static_cast<u64>(d2.y) << 32));
         decide = true;
         break;
-      }
-      case kStream: {
-        const Idx b = block_of(v);
-        const u32 o = static_cast<u32>(v - b * kBlockNodes);
+      case kStream:
         if (STATS) ++st_stream;
-        if (entry_bit(d1, o)) {
-          found_at = cmap_of<Idx>(ix, c) + entry_base(ix, d1, b, c) + entry_popc(d1, o);
+        if (bit1) {
           found = true;
         } else if (!ix.sgs) {
           if (STATS) ++st_fb;
@@ -452,24 +498,18 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
           const u64 hi = static_cast<u64>(d2.z) | (static_cast<u64>(d2.w) << 32);
           const u64 word = bit < 64 ? lo : hi;
           if ((word >> (bit & 63)) & 1ULL) {  // v starts its group and lacks edge c
-            res = kNotFound;
-            emitted = true;
-            prev = false;
+            notfound = true;
           } else {
             // last group start below v within these 128 nodes, if any
-            u64 m = word & ((1ULL << (bit & 63)) - 1);
-            u32 pos = 0;
-            bool hit = false;
-            if (m) {
-              pos = (bit & 64) + 63 - static_cast<u32>(__builtin_clzll(m));
-              hit = true;
-            } else if (bit >= 64 && lo) {
-              pos = 63 - static_cast<u32>(__builtin_clzll(lo));
-              hit = true;
-            }
+            const u64 m = word & ((1ULL << (bit & 63)) - 1);
             if (STATS) ++st_extra;
-            if (hit) {
-              gs = static_cast<Idx>((static_cast<u64>(v) & ~127ULL) + pos);
+            if (m) {
+              gs = static_cast<Idx>((static_cast<u64>(v) & ~127ULL) + (bit & 64) + 63 -
+                                    static_cast<u32>(__builtin_clzll(m)));
+              mode = kGsEntry;
+            } else if (bit >= 64 && lo) {
+              gs = static_cast<Idx>((static_cast<u64>(v) & ~127ULL) + 63 -
+                                    static_cast<u32>(__builtin_clzll(lo)));
               mode = kGsEntry;
             } else {
               gs_at = (static_cast<u64>(v) & ~127ULL) - 1;
@@ -478,7 +518,6 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
           }
         }
         break;
-      }
       case kGsScan: {
         const u64 lo = static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32);
         const u64 hi = static_cast<u64>(d1.z) | (static_cast<u64>(d1.w) << 32);
@@ -493,20 +532,11 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         }
         break;
       }
-      case kGsEntry: {
-        const Idx b = block_of(gs);
-        const u32 o = static_cast<u32>(gs - b * kBlockNodes);
+      case kGsEntry:
         if (STATS) ++st_extra;
-        if (entry_bit(d1, o)) {
-          found_at = cmap_of<Idx>(ix, c) + entry_base(ix, d1, b, c) + entry_popc(d1, o);
-          found = true;
-        } else {
-          res = kNotFound;
-          emitted = true;
-          prev = false;
-        }
+        found = bit1;
+        notfound = !bit1;
         break;
-      }
       case kPresearch: {
         const u64 pre = window >> (64 - 2 * kp);
         if constexpr (sizeof(Idx) == 4) {
@@ -518,27 +548,23 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         }
         j = kp;
         if (STATS) ++st_full;
-        mode = kStep;  // resolved below
+        mode = kStep;
+        resolve = true;
         break;
       }
-      case kStep: {
-        const Idx cm = cmap_of<Idx>(ix, c);
-        const Idx r1 = r + 1;
-        const Idx bl = block_of(l), br = block_of(r1);
-        l = cm + entry_base(ix, d1, bl, c) + entry_popc(d1, static_cast<u32>(l - bl * kBlockNodes));
-        r = cm + entry_base(ix, d2, br, c) + entry_popc(d2, static_cast<u32>(r1 - br * kBlockNodes)) - 1;
+      case kStep:
+        l = x1;
+        r = cm + entry_base(ix, d2, b2, c) + entry_popc(d2, o2) - 1;
         ++j;
         if (STATS) ++st_steps;
+        resolve = true;
         break;
-      }
       default:
         break;
     }
-    if (mode == kStep && !found && !emitted) {  // a search just started or stepped
+    if (resolve) {  // a search just started or stepped
       if (l > r) {
-        res = kNotFound;
-        emitted = true;
-        prev = false;
+        notfound = true;
       } else if (j == k) {
         found_at = l;
         found = true;
@@ -546,6 +572,9 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         c = static_cast<u32>(window >> (62 - 2 * j)) & 3u;
       }
     }
+    u64 res = kNotFound;
+    bool emitted = notfound;
+    if (notfound) prev = false;
     if (found) {
       v = found_at;
       prev = true;
@@ -566,8 +595,9 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         ++p;
         --left;
         emitted = false;
-        if (left == 0) {
-          mode = kIdle;
+        if (left == 0) {  // chunk done: wait for the next one
+          want = true;
+          mode = kChunk;
           break;
         }
       }

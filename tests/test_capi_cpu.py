@@ -54,3 +54,14 @@ def test_load_rejects_wrong_colors_magic(tmp_path):
     assert rc != 0
     assert S.lib().sbwt_last_error() == \
         b"The colors file has an incorrect format. Expected 'sdsl-hybrid-v4'"
+
+
+@pytest.mark.parametrize("kp,k", [(17, 31), (9, 8)])
+def test_create_rejects_presearch_letters_out_of_range(kp, k):
+    """The presearch prefix is at most min(k, 16) letters; checked before device work."""
+    words = (ctypes.c_uint64 * 1)(0b1011)
+    acgt = (ctypes.POINTER(ctypes.c_uint64) * 4)(*([ctypes.cast(words, ctypes.POINTER(ctypes.c_uint64))] * 4))
+    h = ctypes.c_void_p()
+    rc = S.lib().sbwt_index_create(acgt, 4, k, None, None, kp, ctypes.byref(h))
+    assert rc != 0
+    assert S.lib().sbwt_last_error() == b"presearch letters must be in [1, min(k, 16)]"

@@ -62,7 +62,7 @@ ALGOS = [S.ALGO_STREAMING, S.ALGO_PER_KMER]
 
 
 @pytest.mark.parametrize("algo", ALGOS, ids=["streaming", "per-kmer"])
-@pytest.mark.parametrize("kp", [12, 1, 7, 10, 14])
+@pytest.mark.parametrize("kp", [12, 1, 7, 10, 13, 16])
 def test_reads_parity_presearch_widths(genome_index, kp, algo):
     g, b, path, orc = genome_index
     say(f"reads parity kp={kp} algo={algo}")
