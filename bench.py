@@ -26,7 +26,7 @@ sys.path.insert(0, os.path.join(REPO, "sbwt-search_amd", "bindings"))
 
 METRIC = "k-mers/s (index) + reads/s (color) on 1 MI355X; batched vs plain; p99 latency"
 HBM_PEAK_GBS = 8000.0
-LINE = 64  # bytes per rank-block line / presearch line
+LINE = 128  # bytes per random access: one MI355X L2 line (node line or presearch line)
 
 
 def log(msg):
@@ -61,7 +61,9 @@ def parse():
     ap.add_argument("--cpu-sample-reads", type=int, default=200_000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--presearch-letters", type=int, default=12)
+    ap.add_argument("--presearch-letters", type=int, default=0,
+                    help="presearch prefix letters (0: the library's choice for the index size; "
+                         "the reference fixes 12)")
     ap.add_argument("--algorithm", default="streaming", choices=["streaming", "per-kmer"])
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run an elementwise copy of this many MiB (a kernel of known "
@@ -111,7 +113,8 @@ def main():
     t0 = time.time()
     idx = S.Index.create(bits, built.n, configs.K, presearch_letters=args.presearch_letters,
                          suffix_group_starts=built.bits(4))
-    log(f"device index: {idx.device_bytes / 2**20:.1f} MiB, wide={idx.wide} "
+    log(f"device index: {idx.device_bytes / 2**20:.1f} MiB, wide={idx.wide}, "
+        f"presearch {idx.presearch_letters} letters "
         f"({time.time() - t0:.1f} s)")
 
     dev = torch.device("cuda:0")
@@ -135,10 +138,10 @@ def main():
     resolved = (stats["full_searches"] + stats["streamed"] + stats["invalid"]
                 - stats["stream_fallbacks"])
     assert resolved == n_kmers, stats
-    # one 64-byte line per presearch lookup, SBWT step and streamed step, two per
-    # key-walk step (mark + rank block); u64 output; packed input and invalid mask
+    # one 128-byte line per presearch lookup, SBWT step, streamed step, extra
+    # group-start line and key-walk step; u64 output; packed input and invalid mask
     lines = (stats["full_searches"] + stats["steps"] + stats["streamed"]
-             + stats["stream_extra_lines"] + 2 * stats["walk_steps"])
+             + stats["stream_extra_lines"] + stats["walk_steps"])
     algo_bytes = LINE * lines + 8 * n_kmers + 0.375 * wl.n_chars
 
     for _ in range(args.warmup):

@@ -49,10 +49,12 @@ int sbwt_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
 
 /* Device index from the four ACGT bit vectors of an SBWT with num_bits nodes
  * (ceil(num_bits/64) words each). Builds the rank layout, the c-map and the
- * presearch table of presearch_letters letters (0 = the reference's 12, or k
- * if k < 12; at most 16: 4^kp intervals of 8 bytes, 16 if num_bits >= 2^32).
- * Longer prefixes cut SBWT steps from every search; results do not depend on
- * it. key_kmer_marks (nullable, ceil(num_bits/64) words) enables the
+ * presearch table of presearch_letters letters, at most min(k, 16): 4^kp
+ * intervals of 8 bytes (16 if num_bits >= 2^32). Longer prefixes cut SBWT
+ * steps from every search; results do not depend on it. 0 chooses about
+ * log4(num_bits) + 2 letters within [min(k, 12), min(k, 16)], the table
+ * taking at most an eighth of device memory (the reference fixes 12).
+ * key_kmer_marks (nullable, ceil(num_bits/64) words) enables the
  * key-k-mer move. suffix_group_starts (nullable, the .sbwt file's vector of
  * the same name) lets the streaming search step from nodes that do not start
  * their suffix group; without it such steps fall back to a full search.
@@ -144,11 +146,13 @@ int sbwt_searcher_last_timings(sbwt_searcher *searcher, float *pack_ms, float *o
 /* Census mode for sbwt_searcher_search_reads(_device): when enabled, the next
  * calls also count the exact algorithmic work of the batch: stats[0] full
  * searches (one presearch line each), stats[1] their SBWT steps (one
- * rank-block line each), stats[2] key-walk steps, stats[3] invalid k-mers,
- * stats[4] k-mers stepped from the previous k-mer's node (one line each),
- * stats[5] extra lines of those steps, stats[6] of those steps that fell back
- * to a full search. Results are unchanged. No counterpart in the reference
- * (its kernels keep no counters). */
+ * 128-byte node line each), stats[2] key-walk steps (one node line each),
+ * stats[3] invalid k-mers, stats[4] k-mers stepped from the previous k-mer's
+ * node (one node line each), stats[5] extra lines of those steps (the suffix
+ * group began in an earlier line), stats[6] of those steps that fell back to
+ * a full search, stats[7] of those steps the ones from a node that does not
+ * start its suffix group. Results are unchanged. No counterpart in the reference (its
+ * kernels keep no counters). */
 int sbwt_searcher_set_stats(sbwt_searcher *searcher, int enable);
 int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[8]);
 

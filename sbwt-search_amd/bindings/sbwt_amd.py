@@ -262,7 +262,7 @@ class Searcher:
         _check(lib().sbwt_searcher_last_stats(self.h, _p64(out)))
         return {"full_searches": int(out[0]), "steps": int(out[1]), "walk_steps": int(out[2]),
                 "invalid": int(out[3]), "streamed": int(out[4]), "stream_extra_lines": int(out[5]),
-                "stream_fallbacks": int(out[6])}
+                "stream_fallbacks": int(out[6]), "group_steps": int(out[7])}
 
     def set_algorithm(self, algorithm):
         """0 = streaming (default), 1 = per-k-mer (the reference's search shape)."""

@@ -97,8 +97,8 @@ int sbwt_index_create(const uint64_t *const acgt[4], uint64_t num_bits, uint32_t
     require(num_bits >= 1, "an SBWT has at least one node");
     require(kmer_size >= 1 && kmer_size <= 32, "k-mer size must be in [1, 32]");
     for (int c = 0; c < 4; ++c) require(acgt[c] != nullptr, "null bit vector");
-    const uint32_t kp = presearch_letters ? presearch_letters : sbwt_amd::default_presearch_letters(kmer_size);
-    require(kp >= 1 && kp <= kmer_size && kp <= 16, "presearch letters must be in [1, min(k, 16)]");
+    const uint32_t kp = presearch_letters;  // 0: chosen from the index size
+    require(kp <= kmer_size && kp <= 16, "presearch letters must be in [1, min(k, 16)]");
     auto idx = std::make_unique<sbwt_index>();
     idx->dev.build(acgt, num_bits, kmer_size, kp, key_kmer_marks, key_kmer_marks ? num_bits : 0,
                    suffix_group_starts);
@@ -118,8 +118,8 @@ int sbwt_index_load(const char *sbwt_path, const char *colors_path, uint32_t pre
     }
     const uint64_t *acgt[4] = {s.acgt[0].data(), s.acgt[1].data(), s.acgt[2].data(), s.acgt[3].data()};
     const uint32_t k = static_cast<uint32_t>(s.k);
-    const uint32_t kp = presearch_letters ? presearch_letters : sbwt_amd::default_presearch_letters(k);
-    require(kp >= 1 && kp <= k && kp <= 16, "presearch letters must be in [1, min(k, 16)]");
+    const uint32_t kp = presearch_letters;  // 0: chosen from the index size
+    require(kp <= k && kp <= 16, "presearch letters must be in [1, min(k, 16)]");
     auto idx = std::make_unique<sbwt_index>();
     idx->dev.build(acgt, s.num_bits, k, kp, colors_path ? km.words.data() : nullptr,
                    colors_path ? km.bits : 0, s.suffix_group_starts.data());
@@ -137,7 +137,7 @@ int sbwt_index_get_info(const sbwt_index *index, sbwt_index_info *info) {
     info->presearch_letters = index->dev.kp;
     for (int i = 0; i < 5; ++i) info->c_map[i] = index->dev.c_map[i];
     info->device_bytes = index->dev.device_bytes;
-    info->has_key_kmer_marks = index->dev.key_marks.blocks != nullptr;
+    info->has_key_kmer_marks = index->dev.has_marks;
     info->wide = index->dev.wide;
   });
 }

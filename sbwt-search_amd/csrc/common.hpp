@@ -28,28 +28,41 @@ constexpr u64 kWalkStuck = ~0ULL - 3;   // key walk reached a row with no edge a
   } while (0)
 
 // ---------------------------------------------------------------------------
-// Device layout of the SBWT ("ACGT rank blocks").
-// Block b covers nodes [96 b, 96 b + 96) and is one 64-byte line holding, for
-// each letter c, a 16-byte entry {count, bits[0..31], bits[32..63],
-// bits[64..95]}: count = number of 1s of letter c before the block, relative
-// to the block's hyperblock (2^25 blocks), so rank_c(i) is one 16-byte load
-// plus one small table read. The key-k-mer marks (and the color index's
-// marks) use 64-byte "mark blocks": {u64 count, 7 words} for 448 bits.
+// Device layout of the SBWT ("node lines").
+// Line b covers nodes [96 b, 96 b + 96) and is one 128-byte line (the L2 line
+// of the MI355X, so every query on a node costs one line) of eight 16-byte
+// slots:
+//   slot c (0..3)  letter c: {count, bits[0..31], bits[32..63], bits[64..95]}
+//                  count = 1s of letter c before the line
+//   slot 4         suffix-group starts: {delta, bits[0..95]}, delta = distance
+//                  from the line's first node back to the start of its suffix
+//                  group (kFarGroup if unknown or too far)
+//   slot 5         key-k-mer marks: {count, bits[0..95]}
+//   slots 6, 7     zero
+// Counts are relative to the line's hyperblock (2^25 lines) whose absolute
+// bases sit in a small table; indexes below 2^32 nodes have a single
+// hyperblock with base 0, so a rank is one 16-byte load and a popcount.
+// The color index keeps its marks in 64-byte "mark blocks": {u64 count,
+// 7 words} for 448 bits.
 // ---------------------------------------------------------------------------
 constexpr u32 kBlockNodes = 96;
-constexpr u32 kHyperShift = 25;  // blocks per hyperblock = 2^25 (3.2 G nodes)
+constexpr u32 kLineSlots = 8;
+constexpr u32 kSlotGroups = 4;
+constexpr u32 kSlotMarks = 5;
+constexpr u32 kHyperShift = 25;  // lines per hyperblock = 2^25 (3.2 G nodes)
 constexpr u32 kMarkBits = 448;
+constexpr u32 kFarGroup = 0xFFFFFFFFu;
 
 struct IndexView {
-  const uint4 *blocks;        // [n_blocks][4] entries
-  const u64 *hyper_base;      // [n_hyper][4] absolute count at hyperblock start
+  const uint4 *blocks;        // [n_blocks][8] slots
+  const u64 *hyper_base;      // [n_hyper][8] absolute counts at hyperblock start (slots 0-3, 5)
   const void *presearch;      // [4^kp] of {left,right} (uint2 or ulonglong2)
-  const u64 *key_marks;       // mark blocks of the key-k-mer marks, or null
-  const u64 *sgs;             // suffix-group starts (plain bit vector), or null
   u64 c_map[5];
   u64 num_nodes;
   u32 k;
   u32 kp;
+  u32 has_groups;             // slot 4 holds the suffix-group starts
+  u32 has_marks;              // slot 5 holds the key-k-mer marks
 };
 
 // Threads per workgroup of the search kernels: 4 wave64s.

@@ -1,5 +1,5 @@
-// Device functions over the MI355X index layout: rank on ACGT rank blocks and
-// mark blocks, the SBWT step, bit reads and sdsl int_vector reads.
+// Device functions over the MI355X index layout: rank on node lines and mark
+// blocks, the SBWT step, bit reads and sdsl int_vector reads.
 #pragma once
 
 #include "common.hpp"
@@ -24,14 +24,14 @@ __device__ __forceinline__ Idx block_of(Idx i) {
   return i / static_cast<Idx>(kBlockNodes);
 }
 
-// Absolute count before block b of letter c: narrow indexes store absolute
-// counts; wide ones add the hyperblock base.
+// Absolute count before line b of slot `slot` (letters 0..3, marks 5):
+// narrow indexes store absolute counts; wide ones add the hyperblock base.
 template <typename Idx>
-__device__ __forceinline__ Idx entry_base(const IndexView &ix, uint4 e, Idx b, u32 c) {
+__device__ __forceinline__ Idx entry_base(const IndexView &ix, uint4 e, Idx b, u32 slot) {
   if constexpr (sizeof(Idx) == 4) {
     return e.x;
   } else {
-    return ix.hyper_base[(b >> kHyperShift) * 4 + c] + e.x;
+    return ix.hyper_base[(b >> kHyperShift) * kLineSlots + slot] + e.x;
   }
 }
 
@@ -41,27 +41,31 @@ __device__ __forceinline__ Idx cmap_of(const IndexView &ix, u32 c) {
   return static_cast<Idx>(v);
 }
 
+__device__ __forceinline__ const uint4 *line_slot(const IndexView &ix, u64 b, u32 slot) {
+  return ix.blocks + b * kLineSlots + slot;
+}
+
 // rank_c over [0, i): src/UtilityKernels/Rank.cuh:26-65 restated on the
-// rank-block layout (one 16-byte load).
+// node-line layout (one 16-byte load).
 template <typename Idx>
 __device__ __forceinline__ Idx rank_acgt(const IndexView &ix, u32 c, Idx i) {
   const Idx b = block_of(i);
   const u32 o = static_cast<u32>(i - b * kBlockNodes);
-  const uint4 e = ix.blocks[static_cast<u64>(b) * 4 + c];
+  const uint4 e = *line_slot(ix, b, c);
   return entry_base(ix, e, b, c) + entry_popc(e, o);
 }
 
 // One SBWT step with letter c on the interval [l, r]
 // (src/IndexSearcher/IndexSearcher.cuh:48-55). When l and r+1 fall in the
-// same block the entry is loaded once.
+// same line the entry is loaded once.
 template <typename Idx>
 __device__ __forceinline__ void sbwt_step(const IndexView &ix, u32 c, Idx &l, Idx &r) {
   const Idx cm = cmap_of<Idx>(ix, c);
   const Idx r1 = r + 1;
   const Idx bl = block_of(l), br = block_of(r1);
-  const uint4 el = ix.blocks[static_cast<u64>(bl) * 4 + c];
+  const uint4 el = *line_slot(ix, bl, c);
   uint4 er = el;
-  if (br != bl) er = ix.blocks[static_cast<u64>(br) * 4 + c];
+  if (br != bl) er = *line_slot(ix, br, c);
   l = cm + entry_base(ix, el, bl, c) + entry_popc(el, static_cast<u32>(l - bl * kBlockNodes));
   r = cm + entry_base(ix, er, br, c) + entry_popc(er, static_cast<u32>(r1 - br * kBlockNodes)) - 1;
 }
@@ -91,20 +95,23 @@ __device__ __forceinline__ bool mark_bit(const u64 *mb, u64 i) {
 // Key-k-mer move (src/IndexSearcher/IndexSearcher.cuh:60-70): while node l
 // is not a key k-mer, follow its first outgoing edge in A,C,G,T order. A row
 // with no edge ends the walk with kWalkStuck (the reference loops forever).
+// Each step reads one node line: the marks slot and the four letter slots.
 template <typename Idx>
 __device__ __forceinline__ u64 key_walk(const IndexView &ix, Idx l, u64 max_steps, u32 &walked) {
   for (u64 s = 0; s <= max_steps; ++s, ++walked) {
-    if (mark_bit(ix.key_marks, static_cast<u64>(l))) return static_cast<u64>(l);
     const Idx b = block_of(l);
     const u32 o = static_cast<u32>(l - b * kBlockNodes);
-    const uint4 *blk = ix.blocks + static_cast<u64>(b) * 4;
+    const uint4 *line = line_slot(ix, b, 0);
+    const uint4 m = line[kSlotMarks];
+    const uint4 e0 = line[0], e1 = line[1], e2 = line[2], e3 = line[3];
+    if (entry_bit(m, o)) return static_cast<u64>(l);
     u32 c = 4;
-#pragma unroll
-    for (u32 cc = 0; cc < 4; ++cc) {
-      if (c == 4 && entry_bit(blk[cc], o)) c = cc;
-    }
+    uint4 e = e0;
+    if (entry_bit(e3, o)) { c = 3; e = e3; }
+    if (entry_bit(e2, o)) { c = 2; e = e2; }
+    if (entry_bit(e1, o)) { c = 1; e = e1; }
+    if (entry_bit(e0, o)) { c = 0; e = e0; }
     if (c == 4) return kWalkStuck;
-    const uint4 e = blk[c];
     l = cmap_of<Idx>(ix, c) + entry_base(ix, e, b, c) + entry_popc(e, o);
   }
   return kWalkStuck;

@@ -16,7 +16,15 @@ void throw_hip(hipError_t e, const char *file, int line) {
                            std::to_string(line));
 }
 
-u32 default_presearch_letters(u32 k) { return k < 12 ? k : 12; }
+u32 default_presearch_letters(u32 k, u64 n, bool wide, u64 device_bytes) {
+  const u32 lo = std::min<u32>(k, 12), hi = std::min<u32>(k, 16);
+  u32 log4 = 0;
+  while (log4 < 32 && (1ULL << (2 * log4)) < n) ++log4;  // ceil(log4(n))
+  u32 kp = std::clamp<u32>(log4 + 2, lo, hi);
+  const u64 entry = wide ? 16 : 8;
+  while (kp > lo && (entry << (2 * kp)) > device_bytes / 8) --kp;
+  return kp;
+}
 
 namespace {
 
@@ -147,30 +155,57 @@ void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const
   // selects them for any index so the 64-bit kernels can be validated small.
   const char *force = std::getenv("SBWT_AMD_FORCE_WIDE");
   wide = n >= (1ULL << 32) || (force && force[0] == '1');
-  n_blocks = n / kBlockNodes + 1;  // the block of node n exists: rank(n) is defined
-  // per-block popcounts, then per-letter prefix sums
-  std::vector<uint4> host(n_blocks * 4);
+  if (kp == 0) {
+    size_t free_b = 0, total_b = 0;
+    SBWT_HIP(hipMemGetInfo(&free_b, &total_b));
+    kp = default_presearch_letters(k, n, wide, total_b);
+  }
+  n_blocks = n / kBlockNodes + 1;  // the line of node n exists: rank(n) is defined
+  has_marks = km_words != nullptr;
+  has_groups = sgs_words != nullptr;
+  // line bits; counts are filled by the prefix pass below
+  std::vector<uint4> host(n_blocks * kLineSlots);
+  const u64 *vecs[6] = {acgt[0], acgt[1], acgt[2], acgt[3], sgs_words, km_words};
+  const u64 vec_bits[6] = {n, n, n, n, n, km_words ? km_bits : 0};
+  std::vector<u32> last_start(n_blocks);  // last group start in the line + 1 (0: none)
   parallel_for(n_blocks, [&](u64 b) {
-    for (u32 c = 0; c < 4; ++c) {
-      const u64 lo = get_bits(acgt[c], n, b * kBlockNodes, 64);
-      const u64 hi = get_bits(acgt[c], n, b * kBlockNodes + 64, 32);
-      uint4 e;
-      e.x = 0;
-      e.y = static_cast<u32>(lo);
-      e.z = static_cast<u32>(lo >> 32);
-      e.w = static_cast<u32>(hi);
-      host[b * 4 + c] = e;
+    for (u32 s = 0; s < 6; ++s) {
+      uint4 e{0, 0, 0, 0};
+      if (vecs[s]) {
+        u64 lo = get_bits(vecs[s], vec_bits[s], b * kBlockNodes, 64);
+        const u64 hi = get_bits(vecs[s], vec_bits[s], b * kBlockNodes + 64, 32);
+        if (s == kSlotGroups && b == 0) lo |= 1;  // the root starts the first group
+        e.y = static_cast<u32>(lo);
+        e.z = static_cast<u32>(lo >> 32);
+        e.w = static_cast<u32>(hi);
+        if (s == kSlotGroups)
+          last_start[b] = hi ? 65 + (31 - __builtin_clz(static_cast<u32>(hi)))
+                          : lo ? 1 + (63 - __builtin_clzll(lo)) : 0;
+      }
+      host[b * kLineSlots + s] = e;
     }
   });
   n_hyper = (n_blocks >> kHyperShift) + 1;
-  std::vector<u64> hyper(n_hyper * 4, 0);
-  u64 run[4] = {0, 0, 0, 0};
+  std::vector<u64> hyper(n_hyper * kLineSlots, 0);
+  u64 run[6] = {0, 0, 0, 0, 0, 0};
+  u64 group = 0;  // start of the group of the current line's first node
   for (u64 b = 0; b < n_blocks; ++b) {
-    for (u32 c = 0; c < 4; ++c) {
-      if ((b & ((1ULL << kHyperShift) - 1)) == 0) hyper[(b >> kHyperShift) * 4 + c] = run[c];
-      uint4 &e = host[b * 4 + c];
-      e.x = static_cast<u32>(wide ? run[c] - hyper[(b >> kHyperShift) * 4 + c] : run[c]);
-      run[c] += __builtin_popcount(e.y) + __builtin_popcount(e.z) + __builtin_popcount(e.w);
+    for (u32 s = 0; s < 6; ++s) {
+      uint4 &e = host[b * kLineSlots + s];
+      if (s == kSlotGroups) {
+        if (!has_groups) {
+          e.x = kFarGroup;
+          continue;
+        }
+        const u64 first = b * kBlockNodes;
+        const u64 delta = (e.y & 1) ? 0 : first - group;
+        e.x = delta < kFarGroup ? static_cast<u32>(delta) : kFarGroup;
+        if (last_start[b]) group = first + last_start[b] - 1;
+        continue;
+      }
+      if ((b & ((1ULL << kHyperShift) - 1)) == 0) hyper[(b >> kHyperShift) * kLineSlots + s] = run[s];
+      e.x = static_cast<u32>(wide ? run[s] - hyper[(b >> kHyperShift) * kLineSlots + s] : run[s]);
+      run[s] += __builtin_popcount(e.y) + __builtin_popcount(e.z) + __builtin_popcount(e.w);
     }
   }
   c_map[0] = 1;  // src/SbwtBuilder/SbwtBuilder.cpp:111,128-130
@@ -180,18 +215,6 @@ void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const
   device_bytes = host.size() * sizeof(uint4) + hyper.size() * 8;
   host.clear();
   host.shrink_to_fit();
-  if (km_words) {
-    key_marks.build(km_words, km_bits);
-    device_bytes += key_marks.n_blocks * 64;
-  }
-  if (sgs_words) {
-    std::vector<u64> w((n + 63) / 64 + 2, 0);
-    for (u64 i = 0; i < (n + 63) / 64; ++i) w[i] = sgs_words[i];
-    if (n % 64) w[n / 64] &= (1ULL << (n % 64)) - 1;
-    w[0] |= 1;  // the root starts the first group
-    sgs = upload(w.data(), w.size());
-    device_bytes += w.size() * 8;
-  }
   // presearch table on the device
   const u64 count = 1ULL << (2 * kp);
   const IndexView ix = view();
@@ -213,12 +236,9 @@ void DeviceIndex::release() {
   if (blocks) (void)hipFree(blocks);
   if (hyper_base) (void)hipFree(hyper_base);
   if (presearch) (void)hipFree(presearch);
-  if (sgs) (void)hipFree(sgs);
-  sgs = nullptr;
   blocks = nullptr;
   hyper_base = nullptr;
   presearch = nullptr;
-  key_marks.release();
 }
 
 IndexView DeviceIndex::view() const {
@@ -226,8 +246,8 @@ IndexView DeviceIndex::view() const {
   v.blocks = blocks;
   v.hyper_base = hyper_base;
   v.presearch = presearch;
-  v.key_marks = key_marks.blocks;
-  v.sgs = sgs;
+  v.has_groups = has_groups;
+  v.has_marks = has_marks;
   for (int i = 0; i < 5; ++i) v.c_map[i] = c_map[i];
   v.num_nodes = n;
   v.k = k;

@@ -1,5 +1,5 @@
-// Device-resident SBWT index: ACGT rank blocks, presearch table, key-k-mer
-// mark blocks. Replaces the reference's GpuSbwtContainer
+// Device-resident SBWT index: node lines (ACGT rank entries, suffix-group
+// starts and key-k-mer marks per 96 nodes) and the presearch table. Replaces the reference's GpuSbwtContainer
 // (src/SbwtContainer/GpuSbwtContainer.cpp:16-52) and Presearcher
 // (src/Presearcher/Presearcher.cpp:18-30) with an MI355X layout.
 #pragma once
@@ -28,14 +28,14 @@ struct DeviceIndex {
   u32 kp = 0;        // presearch prefix length
   bool wide = false; // node indexes need 64 bits (n >= 2^32)
   u64 c_map[5] = {0, 0, 0, 0, 0};
-  uint4 *blocks = nullptr;
+  uint4 *blocks = nullptr;  // [n_blocks][8] slots: one 128-byte line per 96 nodes
   u64 n_blocks = 0;
   u64 *hyper_base = nullptr;
   u64 n_hyper = 0;
   void *presearch = nullptr;
   u64 presearch_bytes = 0;
-  DeviceMarks key_marks;
-  u64 *sgs = nullptr;  // suffix-group starts, when known
+  bool has_marks = false;   // key-k-mer marks are in the lines
+  bool has_groups = false;  // suffix-group starts are in the lines
   u64 device_bytes = 0;
 
   // acgt: four host bit vectors of ceil(n/64) words (bit i of word w = node
@@ -47,8 +47,12 @@ struct DeviceIndex {
   IndexView view() const;
 };
 
-// Presearch prefix length used when the caller does not choose one: the
-// reference's 12 letters (src/Global/GlobalDefinitions.h.in:17), or k if k < 12.
-u32 default_presearch_letters(u32 k);
+// Presearch prefix length used when the caller does not choose one. The
+// reference fixes 12 letters (src/Global/GlobalDefinitions.h.in:17); here the
+// prefix grows with the index to about log4(n) + 2 letters (between 12 and 16,
+// at most k), so that a prefix interval holds a node or none and most searches
+// start with few SBWT steps left, as long as the table takes at most an eighth
+// of device memory (8 bytes per prefix, 16 for 64-bit indexes).
+u32 default_presearch_letters(u32 k, u64 n, bool wide, u64 device_bytes);
 
 }  // namespace sbwt_amd

@@ -328,11 +328,11 @@ __global__ void k_reset_counter(u64 *c) { *c = 0; }
 //   kChunk       the chunk's descriptor (output slot, char position, count)
 //   kSeq0/kSeq1  64 packed chars + their 64 invalid bits (a lane keeps 128
 //                chars in registers; k-mer windows are shifted out of them)
-//   kStream      rank-block entry of letter c at v  + suffix-group-start bits at v
-//   kGsScan      suffix-group-start bits further left (group start not yet seen)
-//   kGsEntry     rank-block entry of letter c at the group start
+//   kStream      letter slot c and suffix-group slot of v's node line (one line)
+//   kGsEntry     letter slot c of the line holding v's group start, when the
+//                group began in an earlier line
 //   kPresearch   presearch interval of the k-mer's first kp letters
-//   kStep        rank-block entries of letter c at l and at r + 1
+//   kStep        letter slot c of the lines of l and of r + 1
 // Work distribution: each wave owns a range of chunks and hands them to its
 // lanes as they finish (ballot + popcount, no atomics). The first half of the
 // chunks is split evenly over the waves up front; the rest is taken in ranges
@@ -345,14 +345,13 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     IndexView ix, const u64 *__restrict__ packed, const u32 *__restrict__ invalid,
     const ulonglong2 *__restrict__ chunks, const u64 *__restrict__ chunk_ends, u64 n_reads,
     u64 *__restrict__ counter, u64 *__restrict__ out, u64 *__restrict__ stats) {
-  enum : u32 { kChunk, kSeq0, kSeq1, kStream, kGsScan, kGsEntry, kPresearch, kStep, kDone };
+  enum : u32 { kChunk, kSeq0, kSeq1, kStream, kGsEntry, kPresearch, kStep, kDone };
   const u64 n_chunks = chunk_ends[n_reads - 1];
   const u32 lane = threadIdx.x & 63;
   const u32 k = ix.k, kp = ix.kp;
   const u64 inv_mask = (1ULL << k) - 1;
   const uint4 *packed4 = reinterpret_cast<const uint4 *>(packed);   // 64 chars each
   const uint4 *invalid4 = reinterpret_cast<const uint4 *>(invalid); // 128 chars each
-  const uint4 *sgs4 = reinterpret_cast<const uint4 *>(ix.sgs);      // 128 nodes each
   const uint4 *pre4 = reinterpret_cast<const uint4 *>(ix.presearch);
   const uint4 *chunks4 = reinterpret_cast<const uint4 *>(chunks);
   const uint4 *dummy = ix.blocks;
@@ -370,11 +369,12 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
   u64 g = 0, p = 0, buf = 0;           // output slot, char position, buffer start (64-aligned)
   u64 w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // packed chars [buf, buf + 128)
   u64 i0 = 0, i1 = 0;                  // their invalid bits
-  u64 window = 0, gs_at = 0;
+  u64 window = 0;
   u32 left = 0, j = 0, c = 0;
   Idx l = 0, r = 0, v = 0, gs = 0;
   bool prev = false;
   u32 st_full = 0, st_steps = 0, st_walk = 0, st_inv = 0, st_stream = 0, st_extra = 0, st_fb = 0;
+  u32 st_group = 0;
   for (;;) {
     // hand the wave's chunks to lanes that wait for one
     const u64 need = __ballot(want);
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     const Idx n2 = mode == kStep ? static_cast<Idx>(r + 1) : Idx(0);
     const Idx b1 = block_of(n1), b2 = block_of(n2);
     const u32 o1 = static_cast<u32>(n1 - b1 * kBlockNodes), o2 = static_cast<u32>(n2 - b2 * kBlockNodes);
-    const uint4 *e1 = ix.blocks + static_cast<u64>(b1) * 4 + c;
+    const uint4 *e1 = line_slot(ix, b1, c);
     const uint4 *a1 = dummy, *a2 = dummy;
     switch (mode) {
       case kChunk:
@@ -417,10 +417,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         break;
       case kStream:
         a1 = e1;
-        a2 = ix.sgs ? sgs4 + (static_cast<u64>(v) >> 7) : e1;
-        break;
-      case kGsScan:
-        a1 = a2 = sgs4 + (gs_at >> 7);
+        a2 = ix.has_groups ? line_slot(ix, b1, kSlotGroups) : e1;
         break;
       case kGsEntry:
         a1 = a2 = e1;
@@ -432,7 +429,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
       }
       case kStep:
         a1 = e1;
-        a2 = ix.blocks + static_cast<u64>(b2) * 4 + c;
+        a2 = line_slot(ix, b2, c);
         break;
       default:
         break;
@@ -489,49 +486,37 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         if (STATS) ++st_stream;
         if (bit1) {
           found = true;
-        } else if (!ix.sgs) {
+        } else if (!ix.has_groups) {
           if (STATS) ++st_fb;
           mode = kPresearch;  // the group start is unknown: search afresh
         } else {
-          const u32 bit = static_cast<u32>(static_cast<u64>(v) & 127);
-          const u64 lo = static_cast<u64>(d2.x) | (static_cast<u64>(d2.y) << 32);
-          const u64 hi = static_cast<u64>(d2.z) | (static_cast<u64>(d2.w) << 32);
-          const u64 word = bit < 64 ? lo : hi;
-          if ((word >> (bit & 63)) & 1ULL) {  // v starts its group and lacks edge c
-            notfound = true;
-          } else {
-            // last group start below v within these 128 nodes, if any
-            const u64 m = word & ((1ULL << (bit & 63)) - 1);
-            if (STATS) ++st_extra;
-            if (m) {
-              gs = static_cast<Idx>((static_cast<u64>(v) & ~127ULL) + (bit & 64) + 63 -
-                                    static_cast<u32>(__builtin_clzll(m)));
-              mode = kGsEntry;
-            } else if (bit >= 64 && lo) {
-              gs = static_cast<Idx>((static_cast<u64>(v) & ~127ULL) + 63 -
-                                    static_cast<u32>(__builtin_clzll(lo)));
-              mode = kGsEntry;
+          // v's group start: the last group-start bit at or before v in this
+          // line, else delta nodes before the line
+          const u32 q = o1 + 1;
+          const u64 glo = (static_cast<u64>(d2.y) | (static_cast<u64>(d2.z) << 32)) &
+                          (q >= 64 ? ~0ULL : ((1ULL << q) - 1));
+          const u32 ghi = q > 64 ? (d2.w & ((q - 64 >= 32) ? ~0u : ((1u << (q - 64)) - 1))) : 0u;
+          if (ghi | glo) {
+            const u32 pos = ghi ? 95 - static_cast<u32>(__builtin_clz(ghi))
+                                : 63 - static_cast<u32>(__builtin_clzll(glo));
+            if (STATS && pos != o1) ++st_group;
+            // the group's edges sit at its start (v itself when pos == o1)
+            if (pos != o1 && entry_bit(d1, pos)) {
+              found_at = cm + entry_base(ix, d1, b1, c) + entry_popc(d1, pos);
+              found = true;
             } else {
-              gs_at = (static_cast<u64>(v) & ~127ULL) - 1;
-              mode = kGsScan;
+              notfound = true;
             }
+          } else if (d2.x == kFarGroup) {
+            if (STATS) ++st_fb;
+            mode = kPresearch;
+          } else {
+            gs = static_cast<Idx>(static_cast<u64>(b1) * kBlockNodes - d2.x);
+            if (STATS) ++st_group;
+            mode = kGsEntry;
           }
         }
         break;
-      case kGsScan: {
-        const u64 lo = static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32);
-        const u64 hi = static_cast<u64>(d1.z) | (static_cast<u64>(d1.w) << 32);
-        if (STATS) ++st_extra;
-        if (hi | lo) {
-          const u32 pos = hi ? 127 - static_cast<u32>(__builtin_clzll(hi))
-                             : 63 - static_cast<u32>(__builtin_clzll(lo));
-          gs = static_cast<Idx>((gs_at & ~127ULL) + pos);
-          mode = kGsEntry;
-        } else {
-          gs_at -= 128;  // bit 0 of the vector is set, so this ends
-        }
-        break;
-      }
       case kGsEntry:
         if (STATS) ++st_extra;
         found = bit1;
@@ -638,6 +623,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     atomicAdd(reinterpret_cast<unsigned long long *>(stats + 4), static_cast<u64>(st_stream));
     atomicAdd(reinterpret_cast<unsigned long long *>(stats + 5), static_cast<u64>(st_extra));
     atomicAdd(reinterpret_cast<unsigned long long *>(stats + 6), static_cast<u64>(st_fb));
+    atomicAdd(reinterpret_cast<unsigned long long *>(stats + 7), static_cast<u64>(st_group));
   }
 }
 
@@ -683,7 +669,7 @@ Searcher::Searcher(const DeviceIndex *index, u64 max_chars, u64 max_reads, bool 
       move_to_key_(move_to_key_kmer),
       max_chars_(max_chars),
       max_reads_(max_reads) {
-  if (move_to_key_ && !index_->key_marks.blocks)
+  if (move_to_key_ && !index_->has_marks)
     throw std::runtime_error("move to key k-mer requested but the index has no key-k-mer marks");
   if (own_stream_) SBWT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   const u64 words = max_chars_ / 32 + 8;

@@ -11,7 +11,7 @@ Server::Server(const DeviceIndex *index, const DeviceColors *colors, const Serve
     : index_(index), colors_(colors), cfg_(cfg), t0_(std::chrono::steady_clock::now()) {
   if (cfg_.slots == 0 || cfg_.max_chars_per_slot == 0 || cfg_.max_reads_per_slot == 0)
     throw std::runtime_error("Not enough memory");
-  if (cfg_.move_to_key_kmer && !index_->key_marks.blocks)
+  if (cfg_.move_to_key_kmer && !index_->has_marks)
     throw std::runtime_error("move to key k-mer requested but the index has no key-k-mer marks");
   const u64 nc = colors_ ? colors_->num_colors : 0;
   const u64 C = cfg_.max_chars_per_slot, R = cfg_.max_reads_per_slot;

@@ -176,7 +176,9 @@ def test_long_reads_span_many_chunks(tmp_path):
 
 def test_streaming_without_suffix_group_starts(tmp_path):
     """Index built from the ACGT vectors alone: steps from nodes that do not start
-    their suffix group fall back to full searches; results are unchanged."""
+    their suffix group fall back to full searches; results are unchanged. With
+    the starts, such steps resolve from the group start (in the node's line or
+    an earlier one) and never fall back."""
     say("streaming without suffix-group starts")
     root = W.genome_random(12, 40_000)
     gens = [root] + [W.genome_mutate(root, s, 0.03) for s in (1, 2, 3, 4)]
@@ -197,7 +199,15 @@ def test_streaming_without_suffix_group_starts(tmp_path):
         if sgs is None:
             assert st["stream_fallbacks"] > 0
         else:
-            assert st["stream_fallbacks"] == 0 and st["stream_extra_lines"] > 0
+            assert st["stream_fallbacks"] == 0 and st["group_steps"] > 0
+            assert st["stream_extra_lines"] <= st["group_steps"]
+            # every k-mer of the genomes: includes groups that begin in the line
+            # before the node's own line
+            gchars = W.ACGT[codes].tobytes()
+            s2 = idx.searcher(len(gchars), len(ends))
+            s2.set_stats(True)
+            assert np.array_equal(s2.search_reads(gchars, ends), orc.index_reads(gchars, ends))
+            assert s2.last_stats()["stream_extra_lines"] > 0
 
 
 def test_create_from_bit_vectors_matches_load(genome_index):
