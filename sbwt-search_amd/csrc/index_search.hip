@@ -282,6 +282,9 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_reads(
 // device-wide counter (one atomic per wave).
 // ---------------------------------------------------------------------------
 constexpr u32 kChunkKmers = 256;
+// Persistent workgroups per CU: 16 waves per CU keep the random line fetches
+// near the memory system's rate; more only add L2 pressure.
+constexpr int kStreamBlocksPerCu = 4;
 
 __device__ __forceinline__ u32 lanes_below(u64 mask) {
   return __builtin_amdgcn_mbcnt_hi(static_cast<u32>(mask >> 32),
@@ -333,6 +336,11 @@ __global__ void k_reset_counter(u64 *c) { *c = 0; }
 //                group began in an earlier line
 //   kPresearch   presearch interval of the k-mer's first kp letters
 //   kStep        letter slot c of the lines of l and of r + 1
+// Output: a lane's results are consecutive slots of out. They are staged in
+// LDS (one column per lane, 8 results) and written as whole 64-byte groups
+// with four 16-byte stores; partial groups at chunk ends go out one by one.
+// Scattered 8-byte stores from every lane would keep more partly written
+// lines open than the L2 holds, and each eviction would write a fragment.
 // Work distribution: each wave owns a range of chunks and hands them to its
 // lanes as they finish (ballot + popcount, no atomics). The first half of the
 // chunks is split evenly over the waves up front; the rest is taken in ranges
@@ -373,6 +381,10 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
   u32 left = 0, j = 0, c = 0;
   Idx l = 0, r = 0, v = 0, gs = 0;
   bool prev = false;
+  __shared__ u64 obuf[8][kSearchThreads];  // staged results, obuf[slot][thread]
+  const u32 tid = threadIdx.x;
+  const u64 phase = (reinterpret_cast<uintptr_t>(out) >> 3) & 7;  // out's offset in a 64-byte group
+  u64 gbeg = 0;  // first staged output slot
   u32 st_full = 0, st_steps = 0, st_walk = 0, st_inv = 0, st_stream = 0, st_extra = 0, st_fb = 0;
   u32 st_group = 0;
   for (;;) {
@@ -460,6 +472,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
       case kChunk:
         if (!want) {
           g = static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32);
+          gbeg = g;
           const u64 y = static_cast<u64>(d1.z) | (static_cast<u64>(d1.w) << 32);
           p = y & ((1ULL << 48) - 1);
           left = static_cast<u32>(y >> 48);
@@ -575,7 +588,22 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     // ---- emit, then set up the next k-mer (invalid k-mers resolve here)
     while (emitted || decide) {
       if (emitted) {
-        out[g] = res;
+        const u32 slot = static_cast<u32>((g + phase) & 7);
+        obuf[slot][tid] = res;
+        if (slot == 7 || left == 1) {  // a 64-byte group or the chunk is complete
+          if (slot == 7 && g - gbeg == 7) {
+            uint4 *o4 = reinterpret_cast<uint4 *>(out + gbeg);  // 64-byte aligned
+#pragma unroll
+            for (u32 q = 0; q < 4; ++q) {
+              const u64 a = obuf[2 * q][tid], b = obuf[2 * q + 1][tid];
+              o4[q] = make_uint4(static_cast<u32>(a), static_cast<u32>(a >> 32), static_cast<u32>(b),
+                                 static_cast<u32>(b >> 32));
+            }
+          } else {
+            for (u64 i = gbeg; i <= g; ++i) out[i] = obuf[(i + phase) & 7][tid];
+          }
+          gbeg = g + 1;
+        }
         ++g;
         ++p;
         --left;
@@ -688,7 +716,7 @@ Searcher::Searcher(const DeviceIndex *index, u64 max_chars, u64 max_reads, bool 
   int dev = 0, cus = 0;
   SBWT_HIP(hipGetDevice(&dev));
   SBWT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  persistent_blocks_ = static_cast<u32>(std::max(1, cus) * 8);
+  persistent_blocks_ = static_cast<u32>(std::max(1, cus) * kStreamBlocksPerCu);
 }
 
 void Searcher::last_stats(u64 out[8]) {

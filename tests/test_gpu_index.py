@@ -265,3 +265,32 @@ def test_device_buffers_with_torch(genome_index):
     assert np.array_equal(got, orc.index_reads(chars, ends))
     t = s.last_timings()
     assert t["search_ms"] > 0
+
+
+@pytest.mark.parametrize("algo", ALGOS, ids=["streaming", "per-kmer"])
+@pytest.mark.parametrize("offset_words", [1, 3, 7])
+def test_output_at_any_word_offset(genome_index, algo, offset_words):
+    """Results staged per 64-byte group land right for an output pointer at any
+    8-byte offset, and nothing outside the k-mers' slots is written."""
+    import torch
+    g, b, path, orc = genome_index
+    say(f"output offset {offset_words} words algo={algo}")
+    idx = S.Index.load(path)
+    lens = np.array([31, 45, 150, 97, 31, 300, 64, 200] * 40, np.uint32)
+    chars, ends, _ = W.reads(41 + offset_words, g, np.array([len(g)], np.uint64), lens, 0.1, 0.01,
+                             0.002)
+    chars = chars.tobytes()
+    exp = orc.index_reads(chars, ends)
+    dc = torch.frombuffer(bytearray(chars), dtype=torch.uint8).cuda()
+    de = torch.from_numpy(ends.astype(np.int64)).cuda()
+    sentinel = -7
+    buf = torch.full((len(exp) + 16,), sentinel, dtype=torch.int64, device="cuda")
+    s = idx.searcher(len(chars), len(ends))
+    s.set_algorithm(algo)
+    nk = s.search_reads_device(dc.data_ptr(), len(chars), de.data_ptr(), len(ends),
+                               buf.data_ptr() + 8 * offset_words, want_count=True)
+    assert nk == len(exp)
+    host = buf.cpu().numpy()
+    assert np.array_equal(host[offset_words:offset_words + nk].view(np.uint64), exp)
+    assert (host[:offset_words] == sentinel).all()
+    assert (host[offset_words + nk:] == sentinel).all()
