@@ -135,13 +135,11 @@ def main():
     s.sync()
     stats = s.last_stats()
     s.set_stats(False)
-    resolved = (stats["full_searches"] + stats["streamed"] + stats["invalid"]
-                - stats["stream_fallbacks"])
-    assert resolved == n_kmers, stats
-    # one 128-byte line per presearch lookup, SBWT step, streamed step, extra
-    # group-start line and key-walk step; u64 output; packed input and invalid mask
-    lines = (stats["full_searches"] + stats["steps"] + stats["streamed"]
-             + stats["stream_extra_lines"] + stats["walk_steps"])
+    assert S.Searcher.resolved(stats) == n_kmers, stats
+    # one 128-byte line per presearch lookup (searches and absence checks), SBWT
+    # step, streamed step, extra group-start line and key-walk step; u64 output;
+    # packed input and invalid mask
+    lines = S.Searcher.lines(stats)
     algo_bytes = LINE * lines + 8 * n_kmers + 0.375 * wl.n_chars
 
     for _ in range(args.warmup):

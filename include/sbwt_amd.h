@@ -151,10 +151,14 @@ int sbwt_searcher_last_timings(sbwt_searcher *searcher, float *pack_ms, float *o
  * node (one node line each), stats[5] extra lines of those steps (the suffix
  * group began in an earlier line), stats[6] of those steps that fell back to
  * a full search, stats[7] of those steps the ones from a node that does not
- * start its suffix group. Results are unchanged. No counterpart in the reference (its
+ * start its suffix group, stats[8] absence checks (one presearch line each:
+ * a kp-letter substring of the k-mer whose interval is empty is in no k-mer
+ * of the index, so every k-mer containing it is not found), stats[9] k-mers
+ * settled by such a check, stats[10] further k-mers settled by an earlier
+ * check without a lookup; stats[11..15] are zero. Results are unchanged. No counterpart in the reference (its
  * kernels keep no counters). */
 int sbwt_searcher_set_stats(sbwt_searcher *searcher, int enable);
-int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[8]);
+int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[16]);
 
 /* Search algorithm of the reads path: SBWT_ALGO_STREAMING (default) steps each
  * k-mer of a read from the previous k-mer's node and searches from the

@@ -258,11 +258,25 @@ class Searcher:
         _check(lib().sbwt_searcher_set_stats(self.h, int(enable)))
 
     def last_stats(self):
-        out = np.zeros(8, np.uint64)
+        out = np.zeros(16, np.uint64)
         _check(lib().sbwt_searcher_last_stats(self.h, _p64(out)))
         return {"full_searches": int(out[0]), "steps": int(out[1]), "walk_steps": int(out[2]),
                 "invalid": int(out[3]), "streamed": int(out[4]), "stream_extra_lines": int(out[5]),
-                "stream_fallbacks": int(out[6]), "group_steps": int(out[7])}
+                "stream_fallbacks": int(out[6]), "group_steps": int(out[7]),
+                "absence_checks": int(out[8]), "absence_hits": int(out[9]),
+                "absent_skipped": int(out[10])}
+
+    @staticmethod
+    def resolved(st):
+        """K-mers a census accounts for: each is settled exactly one way."""
+        return (st["full_searches"] + st["streamed"] - st["stream_fallbacks"] + st["invalid"]
+                + st["absence_hits"] + st["absent_skipped"])
+
+    @staticmethod
+    def lines(st):
+        """128-byte lines a census reads: presearch and node lines."""
+        return (st["full_searches"] + st["steps"] + st["streamed"] + st["stream_extra_lines"]
+                + st["walk_steps"] + st["absence_checks"])
 
     def set_algorithm(self, algorithm):
         """0 = streaming (default), 1 = per-k-mer (the reference's search shape)."""

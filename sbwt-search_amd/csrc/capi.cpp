@@ -228,7 +228,7 @@ int sbwt_searcher_set_algorithm(sbwt_searcher *searcher, int algorithm) {
   });
 }
 
-int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[8]) {
+int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[16]) {
   return guarded([&] {
     require(searcher && stats, "null argument");
     searcher->impl->last_stats(stats);

@@ -65,6 +65,9 @@ struct IndexView {
   u32 has_marks;              // slot 5 holds the key-k-mer marks
 };
 
+// Words of the search census (sbwt_searcher_last_stats).
+constexpr u32 kStatsWords = 16;
+
 // Threads per workgroup of the search kernels: 4 wave64s.
 constexpr u32 kSearchThreads = 256;
 

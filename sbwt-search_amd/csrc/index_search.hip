@@ -352,8 +352,9 @@ template <typename Idx, bool KEY, bool STATS>
 __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     IndexView ix, const u64 *__restrict__ packed, const u32 *__restrict__ invalid,
     const ulonglong2 *__restrict__ chunks, const u64 *__restrict__ chunk_ends, u64 n_reads,
-    u64 *__restrict__ counter, u64 *__restrict__ out, u64 *__restrict__ stats) {
-  enum : u32 { kChunk, kSeq0, kSeq1, kStream, kGsEntry, kPresearch, kStep, kDone };
+    u64 *__restrict__ counter, u64 *__restrict__ out, u64 *__restrict__ stats, u32 absence) {
+  enum : u32 { kChunk, kSeq0, kSeq1, kStream, kGsEntry, kPresearch, kTail, kStep, kDone };
+  enum : u32 { kHintNone, kHintMiss, kHintPre };  // what the last miss left known
   const u64 n_chunks = chunk_ends[n_reads - 1];
   const u32 lane = threadIdx.x & 63;
   const u32 k = ix.k, kp = ix.kp;
@@ -377,7 +378,10 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
   u64 g = 0, p = 0, buf = 0;           // output slot, char position, buffer start (64-aligned)
   u64 w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // packed chars [buf, buf + 128)
   u64 i0 = 0, i1 = 0;                  // their invalid bits
-  u64 window = 0;
+  u64 window = 0, pre_idx = 0;       // current window; presearch entry to read
+  u64 dead_end = 0;                   // windows starting before dead_end are not found
+  u64 miss_at = 0, tq = 0;            // char that ended the last miss; checked kp-mer start
+  u32 hint = kHintNone;
   u32 left = 0, j = 0, c = 0;
   Idx l = 0, r = 0, v = 0, gs = 0;
   bool prev = false;
@@ -386,7 +390,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
   const u64 phase = (reinterpret_cast<uintptr_t>(out) >> 3) & 7;  // out's offset in a 64-byte group
   u64 gbeg = 0;  // first staged output slot
   u32 st_full = 0, st_steps = 0, st_walk = 0, st_inv = 0, st_stream = 0, st_extra = 0, st_fb = 0;
-  u32 st_group = 0;
+  u32 st_group = 0, st_tail = 0, st_tail_hit = 0, st_skip = 0;
   for (;;) {
     // hand the wave's chunks to lanes that wait for one
     const u64 need = __ballot(want);
@@ -434,11 +438,10 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
       case kGsEntry:
         a1 = a2 = e1;
         break;
-      case kPresearch: {
-        const u64 pre = window >> (64 - 2 * kp);
-        a1 = a2 = pre4 + (sizeof(Idx) == 4 ? (pre >> 1) : pre);
+      case kPresearch:
+      case kTail:
+        a1 = a2 = pre4 + (sizeof(Idx) == 4 ? (pre_idx >> 1) : pre_idx);
         break;
-      }
       case kStep:
         a1 = e1;
         a2 = line_slot(ix, b2, c);
@@ -478,6 +481,8 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
           left = static_cast<u32>(y >> 48);
           buf = p & ~63ULL;
           prev = false;
+          hint = kHintNone;
+          dead_end = 0;
           mode = kSeq0;
         }
         break;
@@ -501,6 +506,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
           found = true;
         } else if (!ix.has_groups) {
           if (STATS) ++st_fb;
+          pre_idx = window >> (64 - 2 * kp);
           mode = kPresearch;  // the group start is unknown: search afresh
         } else {
           // v's group start: the last group-start bit at or before v in this
@@ -522,6 +528,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
             }
           } else if (d2.x == kFarGroup) {
             if (STATS) ++st_fb;
+            pre_idx = window >> (64 - 2 * kp);
             mode = kPresearch;
           } else {
             gs = static_cast<Idx>(static_cast<u64>(b1) * kBlockNodes - d2.x);
@@ -535,21 +542,33 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         found = bit1;
         notfound = !bit1;
         break;
-      case kPresearch: {
-        const u64 pre = window >> (64 - 2 * kp);
+      case kPresearch:
+      case kTail:
         if constexpr (sizeof(Idx) == 4) {
-          l = (pre & 1) ? d1.z : d1.x;
-          r = (pre & 1) ? d1.w : d1.y;
+          l = (pre_idx & 1) ? d1.z : d1.x;
+          r = (pre_idx & 1) ? d1.w : d1.y;
         } else {
           l = static_cast<Idx>(static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32));
           r = static_cast<Idx>(static_cast<u64>(d1.z) | (static_cast<u64>(d1.w) << 32));
+        }
+        if (mode == kTail) {
+          // absence check: the kp-mer at tq lies in every window [p, tq]
+          if (STATS) ++st_tail;
+          if (l > r) {
+            if (STATS) ++st_tail_hit;
+            dead_end = tq + 1;
+            notfound = true;
+          } else {
+            pre_idx = window >> (64 - 2 * kp);
+            mode = kPresearch;
+          }
+          break;
         }
         j = kp;
         if (STATS) ++st_full;
         mode = kStep;
         resolve = true;
         break;
-      }
       case kStep:
         l = x1;
         r = cm + entry_base(ix, d2, b2, c) + entry_popc(d2, o2) - 1;
@@ -563,6 +582,9 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     if (resolve) {  // a search just started or stepped
       if (l > r) {
         notfound = true;
+        // the miss ended at char p + j - 1, or within the presearch prefix
+        hint = j == kp ? kHintPre : kHintMiss;
+        miss_at = p + j - 1;
       } else if (j == k) {
         found_at = l;
         found = true;
@@ -572,10 +594,16 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     }
     u64 res = kNotFound;
     bool emitted = notfound;
+    if (notfound && (mode == kStream || mode == kGsEntry)) {
+      // the k-mer's last char made the miss: its predecessor was found
+      hint = kHintMiss;
+      miss_at = p + k - 1;
+    }
     if (notfound) prev = false;
     if (found) {
       v = found_at;
       prev = true;
+      hint = kHintNone;
       if constexpr (KEY) {
         u32 w = 0;
         res = key_walk<Idx>(ix, v, ix.num_nodes, w);
@@ -631,15 +659,42 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
       if (iv & inv_mask) {
         res = kInvalid;
         prev = false;
+        hint = kHintNone;
         emitted = true;
         if (STATS) ++st_inv;
+        continue;
+      }
+      if (p < dead_end) {  // holds a kp-mer that is in no k-mer of the index
+        res = kNotFound;
+        emitted = true;
+        if (STATS) ++st_skip;
         continue;
       }
       if (prev) {
         c = static_cast<u32>(window >> (64 - 2 * k)) & 3u;
         mode = kStream;
       } else {
-        mode = kPresearch;
+        // Absence check before a full search: the kp-mer at t lies in every
+        // window from p to t, so if its presearch interval is empty all of
+        // them are not found. After a miss that ended at char miss_at the
+        // check takes the last kp-mer of this window that still holds that
+        // char; with nothing known, the window's last kp-mer. After a miss
+        // in the presearch prefix the next window's prefix likely holds the
+        // same bad char, so it is searched directly.
+        u64 t = p;
+        if (absence && hint == kHintNone) t = p + k - kp;
+        if (absence && hint == kHintMiss && miss_at >= p) t = min(miss_at, p + k - kp);
+        if (t > p) {
+          tq = t;
+          const u32 x = static_cast<u32>(t - buf);  // < 96: the kp-mer ends inside this window
+          const u64 ta = x < 32 ? w0 : (x < 64 ? w1 : w2), tb = x < 32 ? w1 : (x < 64 ? w2 : w3);
+          const u32 tsh = 2 * (x & 31);
+          pre_idx = (tsh ? ((ta << tsh) | (tb >> (64 - tsh))) : ta) >> (64 - 2 * kp);
+          mode = kTail;
+        } else {
+          pre_idx = window >> (64 - 2 * kp);
+          mode = kPresearch;
+        }
       }
     }
   }
@@ -652,7 +707,16 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     atomicAdd(reinterpret_cast<unsigned long long *>(stats + 5), static_cast<u64>(st_extra));
     atomicAdd(reinterpret_cast<unsigned long long *>(stats + 6), static_cast<u64>(st_fb));
     atomicAdd(reinterpret_cast<unsigned long long *>(stats + 7), static_cast<u64>(st_group));
+    atomicAdd(reinterpret_cast<unsigned long long *>(stats + 8), static_cast<u64>(st_tail));
+    atomicAdd(reinterpret_cast<unsigned long long *>(stats + 9), static_cast<u64>(st_tail_hit));
+    atomicAdd(reinterpret_cast<unsigned long long *>(stats + 10), static_cast<u64>(st_skip));
   }
+}
+
+// Absence checks pay when most kp-mers are in no k-mer of the index: with
+// 4^kp >= 8 n at most about one kp-mer in eight has a non-empty interval.
+inline bool absence_checks(const DeviceIndex &ix) {
+  return ix.kp < ix.k && (1ULL << (2 * ix.kp)) >= 8 * ix.n;
 }
 
 inline u32 grid_for(u64 items, u32 per_block) {
@@ -707,7 +771,7 @@ Searcher::Searcher(const DeviceIndex *index, u64 max_chars, u64 max_reads, bool 
   scan_tmp_len_ = max_reads_ / kScanChunk + 2;
   SBWT_HIP(hipMalloc(&d_scan_tmp_, scan_tmp_len_ * 8));
   for (auto &e : ev_) SBWT_HIP(hipEventCreate(&e));
-  SBWT_HIP(hipMalloc(&d_stats_, 8 * 8));
+  SBWT_HIP(hipMalloc(&d_stats_, kStatsWords * 8));
   max_chunks_ = max_chars_ / kChunkKmers + max_reads_ + 1;
   SBWT_HIP(hipMalloc(&d_chunks_, max_chunks_ * 16));
   SBWT_HIP(hipMalloc(&d_chunk_counts_, std::max<u64>(max_reads_, 1) * 8 * 2));
@@ -719,8 +783,8 @@ Searcher::Searcher(const DeviceIndex *index, u64 max_chars, u64 max_reads, bool 
   persistent_blocks_ = static_cast<u32>(std::max(1, cus) * kStreamBlocksPerCu);
 }
 
-void Searcher::last_stats(u64 out[8]) {
-  SBWT_HIP(hipMemcpyAsync(out, d_stats_, 8 * 8, hipMemcpyDeviceToHost, stream_));
+void Searcher::last_stats(u64 out[kStatsWords]) {
+  SBWT_HIP(hipMemcpyAsync(out, d_stats_, kStatsWords * 8, hipMemcpyDeviceToHost, stream_));
   SBWT_HIP(hipStreamSynchronize(stream_));
 }
 
@@ -818,7 +882,7 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
                                                                         d_scan_tmp_, d_kmer_ends_);
   SBWT_HIP(hipGetLastError());
   if (timing_) SBWT_HIP(hipEventRecord(ev_[2], stream_));
-  if (stats_) SBWT_HIP(hipMemsetAsync(d_stats_, 0, 8 * 8, stream_));
+  if (stats_) SBWT_HIP(hipMemsetAsync(d_stats_, 0, kStatsWords * 8, stream_));
   if (algorithm_ == kAlgoStreaming) {
     // chunk table of (read, k-mer range) work items, then the persistent search
     u64 *d_chunk_ends = d_chunk_counts_ + max_reads_;
@@ -833,7 +897,7 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
     dispatch_search3(index_->wide, move_to_key_, stats_, [&]<typename Idx, bool KEY, bool STATS>() {
       k_search_stream<Idx, KEY, STATS><<<persistent_blocks_, kSearchThreads, 0, stream_>>>(
           ix, d_packed_, d_invalid_, d_chunks_, d_chunk_ends, n_reads, d_counter_, d_results,
-          d_stats_);
+          d_stats_, absence_checks(*index_) ? 1u : 0u);
     });
   } else {
     // n_chars bounds the k-mer count; surplus workgroups leave at once

@@ -43,14 +43,13 @@ class Searcher {
   hipStream_t stream() const { return stream_; }
   SearchTimings last_timings();
   void set_timing(bool on) { timing_ = on; }
-  // Census mode: the fused search also counts, in stats[0..6], full searches
-  // (presearch lines), their SBWT steps, key-walk steps, invalid k-mers,
-  // streamed k-mers, extra lines of streamed steps and streamed steps that fell
-  // back to a full search; last_stats reads them (synchronous).
+  // Census mode: the fused search also counts its work per k-mer class (the
+  // words are listed with sbwt_searcher_last_stats); last_stats reads them
+  // (synchronous).
   void set_stats(bool on) { stats_ = on; }
   void set_algorithm(int a) { algorithm_ = a; }
   int algorithm() const { return algorithm_; }
-  void last_stats(u64 out[8]);
+  void last_stats(u64 out[kStatsWords]);
   const u64 *kmer_ends_device() const { return d_kmer_ends_; }
   u64 max_chars() const { return max_chars_; }
   u64 max_reads() const { return max_reads_; }

@@ -172,6 +172,7 @@ def test_long_reads_span_many_chunks(tmp_path):
     st = s.last_stats()
     assert np.array_equal(res, orc.index_reads(chars, rends))
     assert st["streamed"] > st["full_searches"]  # most k-mers step from their predecessor
+    assert S.Searcher.resolved(st) == len(res)
 
 
 def test_streaming_without_suffix_group_starts(tmp_path):
@@ -294,3 +295,21 @@ def test_output_at_any_word_offset(genome_index, algo, offset_words):
     assert np.array_equal(host[offset_words:offset_words + nk].view(np.uint64), exp)
     assert (host[:offset_words] == sentinel).all()
     assert (host[offset_words + nk:] == sentinel).all()
+
+
+def test_absence_checks_settle_windows(genome_index):
+    """Random reads and reads with errors: one presearch lookup that finds an
+    absent kp-mer settles every window holding it; results equal the oracle's and
+    the census accounts for every k-mer exactly once."""
+    g, b, path, orc = genome_index
+    say("absence checks")
+    idx = S.Index.load(path)
+    assert (1 << (2 * idx.presearch_letters)) >= 8 * idx.num_nodes
+    chars, ends = _reads(g, 3000, 150, seed=77, random_frac=0.3, err=0.02, n_rate=0.002)
+    s = idx.searcher(len(chars), len(ends))
+    s.set_stats(True)
+    res = s.search_reads(chars, ends)
+    st = s.last_stats()
+    assert np.array_equal(res, orc.index_reads(chars, ends))
+    assert st["absence_hits"] > 0 and st["absent_skipped"] > st["absence_hits"]
+    assert S.Searcher.resolved(st) == len(res)

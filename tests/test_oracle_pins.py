@@ -195,3 +195,24 @@ def test_sbwt_writer_round_trip(tmp_path):
     assert np.array_equal(s.sgs()[: (b.n + 63) // 64], b.bits(4))
     cm = s.c_map()
     assert cm[0] == 1 and cm[4] == b.n
+
+
+@pytest.mark.parametrize("seed,glen,k,kp", [(4, 400, 9, 5), (5, 3000, 15, 6), (6, 800, 31, 7)])
+def test_empty_presearch_interval_iff_in_no_kmer(seed, glen, k, kp, tmp_path):
+    """A kp-letter string has an empty SBWT interval exactly when no k-mer of the
+    index (reverse complements included) contains it: the dummy nodes make every
+    prefix of a k-mer end some node. This is what lets one presearch lookup settle
+    every window that holds the string as not found."""
+    g, b = _tiny_sbwt(seed, glen, k)
+    path = str(tmp_path / "t.sbwt")
+    b.write(path)
+    left, right = O.OracleSbwt(path).presearch(kp)
+    kmers = [x for x in _node_strings(g, k) if "$" not in x]
+    inside = {x[i:i + kp] for x in kmers for i in range(k - kp + 1)}
+    n_absent = 0
+    for x in range(4 ** kp):
+        s = "".join("ACGT"[(x >> (2 * (kp - 1 - i))) & 3] for i in range(kp))
+        empty = int(left[x]) > int(right[x])
+        assert empty == (s not in inside), s
+        n_absent += empty
+    assert 0 < n_absent < 4 ** kp
