@@ -614,27 +614,32 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
       emitted = true;
     }
     // ---- emit, then set up the next k-mer (invalid k-mers resolve here)
+    // stage one result in LDS; a complete 64-byte group (or the chunk's end)
+    // goes out to HBM
+    auto stage = [&](u64 res_) {
+      const u32 slot = static_cast<u32>((g + phase) & 7);
+      obuf[slot][tid] = res_;
+      if (slot == 7 || left == 1) {
+        if (slot == 7 && g - gbeg == 7) {
+          uint4 *o4 = reinterpret_cast<uint4 *>(out + gbeg);  // 64-byte aligned
+#pragma unroll
+          for (u32 q = 0; q < 4; ++q) {
+            const u64 a = obuf[2 * q][tid], b = obuf[2 * q + 1][tid];
+            o4[q] = make_uint4(static_cast<u32>(a), static_cast<u32>(a >> 32), static_cast<u32>(b),
+                               static_cast<u32>(b >> 32));
+          }
+        } else {
+          for (u64 i = gbeg; i <= g; ++i) out[i] = obuf[(i + phase) & 7][tid];
+        }
+        gbeg = g + 1;
+      }
+      ++g;
+      ++p;
+      --left;
+    };
     while (emitted || decide) {
       if (emitted) {
-        const u32 slot = static_cast<u32>((g + phase) & 7);
-        obuf[slot][tid] = res;
-        if (slot == 7 || left == 1) {  // a 64-byte group or the chunk is complete
-          if (slot == 7 && g - gbeg == 7) {
-            uint4 *o4 = reinterpret_cast<uint4 *>(out + gbeg);  // 64-byte aligned
-#pragma unroll
-            for (u32 q = 0; q < 4; ++q) {
-              const u64 a = obuf[2 * q][tid], b = obuf[2 * q + 1][tid];
-              o4[q] = make_uint4(static_cast<u32>(a), static_cast<u32>(a >> 32), static_cast<u32>(b),
-                                 static_cast<u32>(b >> 32));
-            }
-          } else {
-            for (u64 i = gbeg; i <= g; ++i) out[i] = obuf[(i + phase) & 7][tid];
-          }
-          gbeg = g + 1;
-        }
-        ++g;
-        ++p;
-        --left;
+        stage(res);
         emitted = false;
         if (left == 0) {  // chunk done: wait for the next one
           want = true;
@@ -643,6 +648,23 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         }
       }
       decide = false;
+      // windows settled by an earlier absence check: a tight loop, since a
+      // run (up to k - kp + 1 windows) would otherwise hold the whole wave in
+      // this loop's full body once per window
+      while (p < dead_end && p - buf < 64 && left > 0) {
+        const u32 q = static_cast<u32>(p - buf);
+        const bool bad = ((q ? ((i0 >> q) | (i1 << (64 - q))) : i0) & inv_mask) != 0;
+        if (STATS) {
+          if (bad) ++st_inv;
+          else ++st_skip;
+        }
+        stage(bad ? kInvalid : kNotFound);
+      }
+      if (left == 0) {
+        want = true;
+        mode = kChunk;
+        break;
+      }
       if (p - buf >= 64) {  // slide the 128-char buffer by 64 chars
         buf += 64;
         w0 = w2;
@@ -662,12 +684,6 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         hint = kHintNone;
         emitted = true;
         if (STATS) ++st_inv;
-        continue;
-      }
-      if (p < dead_end) {  // holds a kp-mer that is in no k-mer of the index
-        res = kNotFound;
-        emitted = true;
-        if (STATS) ++st_skip;
         continue;
       }
       if (prev) {
