@@ -58,7 +58,8 @@ def parse():
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--serve-max-reads", type=int, default=100_000)
     ap.add_argument("--reads", type=int, default=0, help="override the number of reads")
-    ap.add_argument("--cpu-sample-reads", type=int, default=200_000)
+    ap.add_argument("--cpu-sample-reads", type=int, default=1_500_000,
+                    help="reads of the batch timed on the CPU oracle (about 10 s of host work)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--presearch-letters", type=int, default=0,
@@ -71,15 +72,20 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profile(name):
-    """HBM bytes per search launch from the committed rocprofv3 --pmc summary."""
-    path = os.path.join(REPO, "profiles", name)
+def traffic_from_profile(config, algorithm, presearch_letters):
+    """HBM bytes per search launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a
+    copy of known size) from the committed summary of the same workload, or None."""
+    path = os.path.join(REPO, "profiles", f"search_{algorithm}_{config}_pmc.json")
     if not os.path.exists(path):
         return None
     try:
-        return json.load(open(path)).get("hbm_bytes_per_launch")
+        d = json.load(open(path))
     except (ValueError, OSError):
         return None
+    if (d.get("config"), d.get("algorithm"), d.get("presearch_letters")) != \
+            (config, algorithm, presearch_letters):
+        return None
+    return d.get("hbm_bytes_per_launch")
 
 
 def main():
@@ -207,7 +213,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic_from_profile(f"search_reads_{wl.name.lower()}_pmc.json"),
+            "traffic": traffic_from_profile(wl.name.lower(), args.algorithm.replace("-", "_"),
+                                            idx.presearch_letters),
             "algorithmic_bytes_per_launch": algo_bytes,
             "census": stats,
         },
