@@ -216,6 +216,11 @@ def main():
             "traffic": traffic_from_profile(wl.name.lower(), args.algorithm.replace("-", "_"),
                                             idx.presearch_letters),
             "algorithmic_bytes_per_launch": algo_bytes,
+            # the reference algorithm's cost model (one 64-byte line per presearch and
+            # per SBWT step of every valid k-mer, u64 out): what an independent search
+            # of each k-mer would read
+            "reference_model_bytes_per_launch": (64 * (configs.K - 12) + 64 + 8)
+            * (n_kmers - stats["invalid"]) + 8 * stats["invalid"],
             "census": stats,
         },
         "cpu_baseline": cpu,
@@ -241,10 +246,27 @@ def cpu_baseline(args, wl, bits, built, gpu_res):
     if not np.array_equal(ref, gpu_res[:len(ref)]):
         raise SystemExit("GPU results differ from the CPU oracle on the baseline sample")
     log(f"cpu baseline: {len(ref)} k-mers in {dt:.2f} s on {threads} threads")
+    # one thread on a tenth of the sample
+    n1 = max(1, n_sample // 10)
+    e1 = int(wl.read_ends[n1 - 1])
+    t0 = time.perf_counter()
+    ref1 = orc.index_reads(wl.chars[:e1].tobytes(), wl.read_ends[:n1].copy(), kp=12, threads=1)
+    dt1 = time.perf_counter() - t0
+    log(f"cpu baseline: {len(ref1)} k-mers in {dt1:.2f} s on 1 thread")
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {"value": len(ref) / dt, "unit": "k-mers/s", "cores": threads, "kind": "port",
             "sample": f"first {n_sample} reads ({len(ref)} k-mers) of the same batch; "
                       "pack + positions + presearch-12 search + classify, results checked "
-                      "equal to the GPU's"}
+                      "equal to the GPU's",
+            "value_1_thread": len(ref1) / dt1,
+            "host": f"{model}, {os.cpu_count()} logical cpus"}
 
 
 def _colored_index(W, S, built, color_of, d, tag):
