@@ -17,6 +17,11 @@ using u64 = uint64_t;
 constexpr u64 kNotFound = ~0ULL;        // reference "-1" (IndexSearcher.cuh:57)
 constexpr u64 kInvalid = ~0ULL - 1;     // reference "-2" (BinaryContinuousIndexResultsPrinter)
 constexpr u64 kWalkStuck = ~0ULL - 3;   // key walk reached a row with no edge and no mark
+// Key-destination table entries that are not node indexes: the walk from the
+// node is longer than the table's build cap (walk at query time), or stuck.
+constexpr u32 kKeyLong32 = 0xFFFFFFFFu, kKeyStuck32 = 0xFFFFFFFEu;
+constexpr u64 kKeyLong64 = ~0ULL;
+constexpr u32 kKeyTableSteps = 4096;  // walk cap of the table build
 
 // HIP failures become std::runtime_error("GPUassert: <msg> at <file>:<line>"),
 // the message shape of the reference's GPU_CHECK (src/Tools/GpuUtils.h:20-29).
@@ -57,6 +62,7 @@ struct IndexView {
   const uint4 *blocks;        // [n_blocks][8] slots
   const u64 *hyper_base;      // [n_hyper][8] absolute counts at hyperblock start (slots 0-3, 5)
   const void *presearch;      // [4^kp] of {left,right} (uint2 or ulonglong2)
+  const void *key_dest;       // [n] key-k-mer move destination (u32 or u64), or null
   u64 c_map[5];
   u64 num_nodes;
   u32 k;

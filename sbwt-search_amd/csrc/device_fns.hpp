@@ -117,6 +117,53 @@ __device__ __forceinline__ u64 key_walk(const IndexView &ix, Idx l, u64 max_step
   return kWalkStuck;
 }
 
+// The key walk of node l, at most max_steps steps; kKeyLong64 when the cap
+// ends it (the table build records such nodes for a walk at query time).
+template <typename Idx>
+__device__ __forceinline__ u64 key_walk_capped(const IndexView &ix, Idx l, u32 max_steps) {
+  for (u32 s = 0; s < max_steps; ++s) {
+    const Idx b = block_of(l);
+    const u32 o = static_cast<u32>(l - b * kBlockNodes);
+    const uint4 *line = line_slot(ix, b, 0);
+    const uint4 m = line[kSlotMarks];
+    const uint4 e0 = line[0], e1 = line[1], e2 = line[2], e3 = line[3];
+    if (entry_bit(m, o)) return static_cast<u64>(l);
+    u32 c = 4;
+    uint4 e = e0;
+    if (entry_bit(e3, o)) { c = 3; e = e3; }
+    if (entry_bit(e2, o)) { c = 2; e = e2; }
+    if (entry_bit(e1, o)) { c = 1; e = e1; }
+    if (entry_bit(e0, o)) { c = 0; e = e0; }
+    if (c == 4) return kWalkStuck;
+    l = cmap_of<Idx>(ix, c) + entry_base(ix, e, b, c) + entry_popc(e, o);
+  }
+  return kKeyLong64;
+}
+
+// Decoded key-destination table entry of node l: its key node, kWalkStuck, or
+// kKeyLong64 (walk at query time).
+template <typename Idx>
+__device__ __forceinline__ u64 key_dest_entry(const IndexView &ix, Idx l) {
+  if constexpr (sizeof(Idx) == 4) {
+    const u32 d = static_cast<const u32 *>(ix.key_dest)[l];
+    return d == kKeyLong32 ? kKeyLong64 : (d == kKeyStuck32 ? kWalkStuck : static_cast<u64>(d));
+  } else {
+    return static_cast<const u64 *>(ix.key_dest)[l];
+  }
+}
+
+// The key-k-mer move of a found node: one table read, or the walk when the
+// index has no table or the node's walk is longer than its cap.
+template <typename Idx>
+__device__ __forceinline__ u64 key_move(const IndexView &ix, Idx l, u32 &walked) {
+  if (ix.key_dest) {
+    ++walked;
+    const u64 d = key_dest_entry(ix, l);
+    if (d != kKeyLong64) return d;
+  }
+  return key_walk<Idx>(ix, l, ix.num_nodes, walked);
+}
+
 // sdsl int_vector<> element i of width w (LSB-first, may straddle two words):
 // src/UtilityKernels/VariableLengthIntegerIndex.cuh:23-36.
 __device__ __forceinline__ u64 intv_get(const u64 *v, u32 width, u64 i) {

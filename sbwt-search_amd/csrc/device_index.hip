@@ -101,6 +101,23 @@ __global__ __launch_bounds__(256) void k_presearch_extend(IndexView ix, const Pa
   }
 }
 
+// Key-k-mer move destination of every node (the walk of
+// src/IndexSearcher/IndexSearcher.cuh:60-70 done once per node), so a found
+// k-mer's move costs one table read. Walks longer than kKeyTableSteps are
+// left to the query (kKeyLong*).
+template <typename Idx>
+__global__ __launch_bounds__(256) void k_key_dest(IndexView ix, Idx *out, u64 n, u32 cap) {
+  for (u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += static_cast<u64>(gridDim.x) * blockDim.x) {
+    const u64 d = key_walk_capped<Idx>(ix, static_cast<Idx>(i), cap);
+    if constexpr (sizeof(Idx) == 4) {
+      out[i] = d == kKeyLong64 ? kKeyLong32 : (d == kWalkStuck ? kKeyStuck32 : static_cast<u32>(d));
+    } else {
+      out[i] = d;
+    }
+  }
+}
+
 template <typename Idx, typename Pair>
 void build_presearch(const IndexView &ix, u32 kp, Pair *out) {
   constexpr u32 kBase = 12;
@@ -230,12 +247,35 @@ void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const
   SBWT_HIP(hipGetLastError());
   SBWT_HIP(hipDeviceSynchronize());
   device_bytes += presearch_bytes;
+  // key-destination table when the index has key marks and it takes at most
+  // an eighth of device memory
+  if (has_marks) {
+    size_t free_b = 0, total_b = 0;
+    SBWT_HIP(hipMemGetInfo(&free_b, &total_b));
+    const u64 bytes = n * (wide ? 8 : 4) + 16;
+    if (bytes <= total_b / 8 && bytes < free_b / 2) {
+      SBWT_HIP(hipMalloc(&key_dest, bytes));
+      const IndexView kv = view();
+      const u32 grid = static_cast<u32>(std::min<u64>((n + 255) / 256, 1u << 20));
+      // SBWT_AMD_KEY_TABLE_STEPS lowers the walk cap so that the query-time
+      // walk of long entries can be validated on small indexes
+      const char *cap_env = std::getenv("SBWT_AMD_KEY_TABLE_STEPS");
+      const u32 cap = cap_env ? static_cast<u32>(std::strtoul(cap_env, nullptr, 10)) : kKeyTableSteps;
+      if (wide) k_key_dest<u64><<<grid, 256>>>(kv, static_cast<u64 *>(key_dest), n, cap);
+      else k_key_dest<u32><<<grid, 256>>>(kv, static_cast<u32 *>(key_dest), n, cap);
+      SBWT_HIP(hipGetLastError());
+      SBWT_HIP(hipDeviceSynchronize());
+      device_bytes += bytes;
+    }
+  }
 }
 
 void DeviceIndex::release() {
   if (blocks) (void)hipFree(blocks);
   if (hyper_base) (void)hipFree(hyper_base);
   if (presearch) (void)hipFree(presearch);
+  if (key_dest) (void)hipFree(key_dest);
+  key_dest = nullptr;
   blocks = nullptr;
   hyper_base = nullptr;
   presearch = nullptr;
@@ -246,6 +286,7 @@ IndexView DeviceIndex::view() const {
   v.blocks = blocks;
   v.hyper_base = hyper_base;
   v.presearch = presearch;
+  v.key_dest = key_dest;
   v.has_groups = has_groups;
   v.has_marks = has_marks;
   for (int i = 0; i < 5; ++i) v.c_map[i] = c_map[i];

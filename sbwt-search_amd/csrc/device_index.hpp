@@ -35,6 +35,7 @@ struct DeviceIndex {
   void *presearch = nullptr;
   u64 presearch_bytes = 0;
   bool has_marks = false;   // key-k-mer marks are in the lines
+  void *key_dest = nullptr; // per node: its key-k-mer move destination (u32, or u64 if wide)
   bool has_groups = false;  // suffix-group starts are in the lines
   u64 device_bytes = 0;
 

@@ -168,7 +168,7 @@ __device__ __forceinline__ u64 search_window(const IndexView &ix, u64 window, u3
   }
   if (l > r) return kNotFound;
   if constexpr (KEY) {
-    return key_walk<Idx>(ix, l, ix.num_nodes, walked);
+    return key_move<Idx>(ix, l, walked);
   } else {
     return static_cast<u64>(l);
   }
@@ -353,7 +353,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     IndexView ix, const u64 *__restrict__ packed, const u32 *__restrict__ invalid,
     const ulonglong2 *__restrict__ chunks, const u64 *__restrict__ chunk_ends, u64 n_reads,
     u64 *__restrict__ counter, u64 *__restrict__ out, u64 *__restrict__ stats, u32 absence) {
-  enum : u32 { kChunk, kSeq0, kSeq1, kStream, kGsEntry, kPresearch, kTail, kStep, kDone };
+  enum : u32 { kChunk, kSeq0, kSeq1, kStream, kGsEntry, kPresearch, kTail, kStep, kKey, kDone };
   enum : u32 { kHintNone, kHintMiss, kHintPre };  // what the last miss left known
   const u64 n_chunks = chunk_ends[n_reads - 1];
   const u32 lane = threadIdx.x & 63;
@@ -446,6 +446,10 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         a1 = e1;
         a2 = line_slot(ix, b2, c);
         break;
+      case kKey:  // the 16 bytes of the key-destination table holding v's entry
+        a1 = a2 = static_cast<const uint4 *>(ix.key_dest) +
+                  (sizeof(Idx) == 4 ? (static_cast<u64>(v) >> 2) : (static_cast<u64>(v) >> 1));
+        break;
       default:
         break;
     }
@@ -470,6 +474,8 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     const bool bit1 = entry_bit(d1, o1);
     // ---- advance the lane's state
     bool found = false, notfound = false, decide = false, resolve = false;
+    bool key_done = false;  // kKey: the table gave the found k-mer's key node
+    u64 key_res = 0;
     Idx found_at = x1;
     switch (mode) {
       case kChunk:
@@ -576,6 +582,24 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         if (STATS) ++st_steps;
         resolve = true;
         break;
+      case kKey:
+        if constexpr (KEY) {
+          u64 dest;
+          if constexpr (sizeof(Idx) == 4) {
+            const u32 q = static_cast<u32>(v) & 3;
+            const u32 e = q == 0 ? d1.x : (q == 1 ? d1.y : (q == 2 ? d1.z : d1.w));
+            dest = e == kKeyLong32 ? kKeyLong64 : (e == kKeyStuck32 ? kWalkStuck : static_cast<u64>(e));
+          } else {
+            dest = (static_cast<u64>(v) & 1) ? (static_cast<u64>(d1.z) | (static_cast<u64>(d1.w) << 32))
+                                              : (static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32));
+          }
+          u32 w = 1;
+          if (dest == kKeyLong64) dest = key_walk<Idx>(ix, v, ix.num_nodes, w);
+          if (STATS) st_walk += w;
+          key_res = dest;
+          key_done = true;
+        }
+        break;
       default:
         break;
     }
@@ -592,8 +616,8 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         c = static_cast<u32>(window >> (62 - 2 * j)) & 3u;
       }
     }
-    u64 res = kNotFound;
-    bool emitted = notfound;
+    u64 res = key_done ? key_res : kNotFound;
+    bool emitted = notfound || key_done;
     if (notfound && (mode == kStream || mode == kGsEntry)) {
       // the k-mer's last char made the miss: its predecessor was found
       hint = kHintMiss;
@@ -605,13 +629,18 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
       prev = true;
       hint = kHintNone;
       if constexpr (KEY) {
-        u32 w = 0;
-        res = key_walk<Idx>(ix, v, ix.num_nodes, w);
-        if (STATS) st_walk += w;
+        if (ix.key_dest) {
+          mode = kKey;  // emitted when the table entry arrives
+        } else {
+          u32 w = 0;
+          res = key_walk<Idx>(ix, v, ix.num_nodes, w);
+          if (STATS) st_walk += w;
+          emitted = true;
+        }
       } else {
         res = static_cast<u64>(v);
+        emitted = true;
       }
-      emitted = true;
     }
     // ---- emit, then set up the next k-mer (invalid k-mers resolve here)
     // stage one result in LDS; a complete 64-byte group (or the chunk's end)

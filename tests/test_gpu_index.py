@@ -221,11 +221,16 @@ def test_create_from_bit_vectors_matches_load(genome_index):
     assert np.array_equal(s.search_reads(chars, ends), orc.index_reads(chars, ends))
 
 
+@pytest.mark.parametrize("cap", [None, "2", "0"], ids=["table", "table-cap2", "walk-only"])
 @pytest.mark.parametrize("algo", ALGOS, ids=["streaming", "per-kmer"])
 @pytest.mark.parametrize("d", [1, 3, 20])
-def test_key_kmer_move_parity(d, algo, tmp_path):
-    """-k: move every found k-mer to its key k-mer (IndexSearcher.cuh:60-70)."""
-    say(f"key k-mer walk d={d} algo={algo}")
+def test_key_kmer_move_parity(d, algo, cap, tmp_path, monkeypatch):
+    """-k: move every found k-mer to its key k-mer (IndexSearcher.cuh:60-70). The
+    move reads a per-node destination table built at load; walks longer than its
+    cap (lowered here through SBWT_AMD_KEY_TABLE_STEPS) are walked at query time."""
+    if cap is not None:
+        monkeypatch.setenv("SBWT_AMD_KEY_TABLE_STEPS", cap)
+    say(f"key k-mer walk d={d} algo={algo} cap={cap}")
     root = W.genome_random(1, 30_000)
     gens = [root] + [W.genome_mutate(root, s, 0.02) for s in (2, 3, 4)]
     codes, ends = W.concat(gens)
