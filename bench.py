@@ -66,6 +66,8 @@ def parse():
                     help="presearch prefix letters (0: the library's choice for the index size; "
                          "the reference fixes 12)")
     ap.add_argument("--algorithm", default="streaming", choices=["streaming", "per-kmer"])
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shape instead of autotuning it on the batch")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run an elementwise copy of this many MiB (a kernel of known "
                          "HBM traffic, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -134,6 +136,16 @@ def main():
     def step():
         s.search_reads_device(d_chars.data_ptr(), wl.n_chars, d_ends.data_ptr(), wl.n_reads,
                               d_out.data_ptr())
+
+    launch = None
+    if args.algorithm == "streaming" and not args.no_autotune:
+        ch, bpc, ms = s.autotune(d_chars.data_ptr(), wl.n_chars, d_ends.data_ptr(), wl.n_reads,
+                                 d_out.data_ptr())
+        launch = {"chunk_kmers": ch, "blocks_per_cu": bpc}
+        log(f"autotune: {ch} k-mers per work item, {bpc} workgroups per CU ({ms:.2f} ms)")
+    elif args.algorithm == "streaming":
+        ch, bpc = s.launch()
+        launch = {"chunk_kmers": ch, "blocks_per_cu": bpc}
 
     # census (untimed): exact work of this batch for the algorithmic byte count
     s.set_stats(True)
@@ -204,6 +216,7 @@ def main():
             "found_fraction": found / n_kmers,
             "presearch_letters": idx.presearch_letters,
             "algorithm": args.algorithm,
+            "launch": launch,
             "search_kernel_ms": kern_ms,
             "reads_per_s": wl.n_reads / (elapsed / args.steps),
         },

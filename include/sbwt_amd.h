@@ -169,6 +169,22 @@ int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[16]);
 #define SBWT_ALGO_PER_KMER 1
 int sbwt_searcher_set_algorithm(sbwt_searcher *searcher, int algorithm);
 
+/* Launch shape of the streaming search: k-mers per work item (chunk_kmers in
+ * [64, 65535], default 256) and persistent 256-lane workgroups per CU
+ * (blocks_per_cu in [1, 8], default 4). Results do not depend on it. No
+ * counterpart in the reference (its launch is one thread per k-mer). */
+int sbwt_searcher_set_launch(sbwt_searcher *searcher, uint32_t chunk_kmers, uint32_t blocks_per_cu);
+int sbwt_searcher_get_launch(sbwt_searcher *searcher, uint32_t *chunk_kmers, uint32_t *blocks_per_cu);
+
+/* Autotuner: runs the streaming search of one caller batch (device buffers,
+ * as sbwt_searcher_search_reads_device) for each shape of a small grid
+ * (chunk_kmers 128..1024 x blocks_per_cu 2..8), keeps the fastest search time
+ * and returns it (outputs nullable). d_results holds the batch's results,
+ * identical for every shape. Synchronous. No counterpart in the reference. */
+int sbwt_searcher_autotune(sbwt_searcher *searcher, const char *d_chars, uint64_t n_chars,
+                           const uint64_t *d_read_ends, uint64_t n_reads, uint64_t *d_results,
+                           uint32_t *chunk_kmers, uint32_t *blocks_per_cu, float *search_ms);
+
 /* Device 2-bit packing alone: chars -> ceil(n/32)+2 words (MSB-first) and
  * ceil(n/32)+2 u32 invalid masks (bit j%32 of word j/32 set if char j is not
  * ACGTacgt). Replaces ContinuousSeqToBitsConverter::parallel_generate

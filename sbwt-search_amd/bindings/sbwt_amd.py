@@ -119,6 +119,11 @@ def lib():
         L.sbwt_searcher_set_stats.argtypes = [vp, C.c_int]
         L.sbwt_searcher_last_stats.argtypes = [vp, u64p]
         L.sbwt_searcher_set_algorithm.argtypes = [vp, C.c_int]
+        L.sbwt_searcher_set_launch.argtypes = [vp, C.c_uint32, C.c_uint32]
+        L.sbwt_searcher_get_launch.argtypes = [vp, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
+        L.sbwt_searcher_autotune.argtypes = [vp, C.c_void_p, C.c_uint64, C.c_void_p, C.c_uint64,
+                                             C.c_void_p, C.POINTER(C.c_uint32),
+                                             C.POINTER(C.c_uint32), C.POINTER(C.c_float)]
         _lib = L
     return _lib
 
@@ -281,6 +286,21 @@ class Searcher:
     def set_algorithm(self, algorithm):
         """0 = streaming (default), 1 = per-k-mer (the reference's search shape)."""
         _check(lib().sbwt_searcher_set_algorithm(self.h, int(algorithm)))
+
+    def set_launch(self, chunk_kmers, blocks_per_cu):
+        _check(lib().sbwt_searcher_set_launch(self.h, int(chunk_kmers), int(blocks_per_cu)))
+
+    def launch(self):
+        ch, bpc = C.c_uint32(), C.c_uint32()
+        _check(lib().sbwt_searcher_get_launch(self.h, C.byref(ch), C.byref(bpc)))
+        return int(ch.value), int(bpc.value)
+
+    def autotune(self, d_chars, n_chars, d_read_ends, n_reads, d_results):
+        """Device pointers (ints). Returns (chunk_kmers, blocks_per_cu, search_ms)."""
+        ch, bpc, ms = C.c_uint32(), C.c_uint32(), C.c_float()
+        _check(lib().sbwt_searcher_autotune(self.h, d_chars, n_chars, d_read_ends, n_reads,
+                                            d_results, C.byref(ch), C.byref(bpc), C.byref(ms)))
+        return int(ch.value), int(bpc.value), float(ms.value)
 
     def sync(self):
         _check(lib().sbwt_searcher_sync(self.h))

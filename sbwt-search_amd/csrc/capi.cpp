@@ -228,6 +228,36 @@ int sbwt_searcher_set_algorithm(sbwt_searcher *searcher, int algorithm) {
   });
 }
 
+int sbwt_searcher_set_launch(sbwt_searcher *searcher, uint32_t chunk_kmers, uint32_t blocks_per_cu) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    searcher->impl->set_launch(chunk_kmers, blocks_per_cu);
+  });
+}
+
+int sbwt_searcher_get_launch(sbwt_searcher *searcher, uint32_t *chunk_kmers, uint32_t *blocks_per_cu) {
+  return guarded([&] {
+    require(searcher && chunk_kmers && blocks_per_cu, "null argument");
+    const sbwt_amd::LaunchShape l = searcher->impl->launch();
+    *chunk_kmers = l.chunk_kmers;
+    *blocks_per_cu = l.blocks_per_cu;
+  });
+}
+
+int sbwt_searcher_autotune(sbwt_searcher *searcher, const char *d_chars, uint64_t n_chars,
+                           const uint64_t *d_read_ends, uint64_t n_reads, uint64_t *d_results,
+                           uint32_t *chunk_kmers, uint32_t *blocks_per_cu, float *search_ms) {
+  return guarded([&] {
+    require(searcher && d_read_ends && d_results && (d_chars || n_chars == 0), "null argument");
+    require(n_reads > 0, "autotune needs a batch with at least one read");
+    const sbwt_amd::LaunchShape l =
+        searcher->impl->autotune(d_chars, n_chars, d_read_ends, n_reads, d_results);
+    if (chunk_kmers) *chunk_kmers = l.chunk_kmers;
+    if (blocks_per_cu) *blocks_per_cu = l.blocks_per_cu;
+    if (search_ms) *search_ms = l.search_ms;
+  });
+}
+
 int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[16]) {
   return guarded([&] {
     require(searcher && stats, "null argument");

@@ -278,13 +278,9 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_reads(
 // decides the same way. A k-mer that follows an absent, invalid or
 // chunk-initial one is searched from its presearch interval as in
 // search_window. Results equal the per-k-mer search: each k-mer's node is
-// unique. When a lane finishes its chunk it takes the next one from a
-// device-wide counter (one atomic per wave).
+// unique. The chunk length (k-mers per work item) and the number of
+// persistent workgroups per CU are launch parameters (set_launch, autotune).
 // ---------------------------------------------------------------------------
-constexpr u32 kChunkKmers = 256;
-// Persistent workgroups per CU: 16 waves per CU keep the random line fetches
-// near the memory system's rate; more only add L2 pressure.
-constexpr int kStreamBlocksPerCu = 4;
 
 __device__ __forceinline__ u32 lanes_below(u64 mask) {
   return __builtin_amdgcn_mbcnt_hi(static_cast<u32>(mask >> 32),
@@ -295,26 +291,26 @@ __device__ __forceinline__ u32 lanes_below(u64 mask) {
 // read ends never send the search outside the batch).
 __global__ __launch_bounds__(256) void k_chunk_count(const u64 *__restrict__ kmer_ends,
                                                       const u64 *__restrict__ char_ends, u64 n_reads,
-                                                      u64 n_chars, u64 *__restrict__ counts) {
+                                                      u64 n_chars, u32 chunk, u64 *__restrict__ counts) {
   for (u64 r = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; r < n_reads;
        r += static_cast<u64>(gridDim.x) * blockDim.x) {
     const u64 nk = kmer_ends[r] - (r ? kmer_ends[r - 1] : 0);
-    counts[r] = char_ends[r] > n_chars ? 0 : (nk + kChunkKmers - 1) / kChunkKmers;
+    counts[r] = char_ends[r] > n_chars ? 0 : (nk + chunk - 1) / chunk;
   }
 }
 
 // Per read: its chunks {first k-mer (output slot), char position | count << 48}.
 __global__ __launch_bounds__(256) void k_chunk_fill(const u64 *__restrict__ kmer_ends,
                                                      const u64 *__restrict__ char_ends, u64 n_reads,
-                                                     const u64 *__restrict__ chunk_ends,
+                                                     const u64 *__restrict__ chunk_ends, u32 chunk,
                                                      ulonglong2 *__restrict__ chunks) {
   for (u64 r = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; r < n_reads;
        r += static_cast<u64>(gridDim.x) * blockDim.x) {
     const u64 ks = r ? kmer_ends[r - 1] : 0, nk = kmer_ends[r] - ks;
     const u64 cs = r ? char_ends[r - 1] : 0;
     u64 at = r ? chunk_ends[r - 1] : 0;
-    for (u64 off = 0; off < nk; off += kChunkKmers, ++at) {
-      const u64 cnt = min(static_cast<u64>(kChunkKmers), nk - off);
+    for (u64 off = 0; off < nk; off += chunk, ++at) {
+      const u64 cnt = min(static_cast<u64>(chunk), nk - off);
       ulonglong2 ch;
       ch.x = ks + off;
       ch.y = (cs + off) | (cnt << 48);
@@ -817,15 +813,53 @@ Searcher::Searcher(const DeviceIndex *index, u64 max_chars, u64 max_reads, bool 
   SBWT_HIP(hipMalloc(&d_scan_tmp_, scan_tmp_len_ * 8));
   for (auto &e : ev_) SBWT_HIP(hipEventCreate(&e));
   SBWT_HIP(hipMalloc(&d_stats_, kStatsWords * 8));
-  max_chunks_ = max_chars_ / kChunkKmers + max_reads_ + 1;
+  max_chunks_ = max_chars_ / kMinChunkKmers + max_reads_ + 1;
   SBWT_HIP(hipMalloc(&d_chunks_, max_chunks_ * 16));
   SBWT_HIP(hipMalloc(&d_chunk_counts_, std::max<u64>(max_reads_, 1) * 8 * 2));
   SBWT_HIP(hipMalloc(&d_chunk_scan_tmp_, scan_tmp_entries(max_reads_) * 8));
   SBWT_HIP(hipMalloc(&d_counter_, 8));
-  int dev = 0, cus = 0;
+  int dev = 0;
   SBWT_HIP(hipGetDevice(&dev));
-  SBWT_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-  persistent_blocks_ = static_cast<u32>(std::max(1, cus) * kStreamBlocksPerCu);
+  SBWT_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
+  set_launch(kDefaultChunkKmers, kDefaultBlocksPerCu);
+}
+
+void Searcher::set_launch(u32 chunk_kmers, u32 blocks_per_cu) {
+  if (chunk_kmers < kMinChunkKmers || chunk_kmers > kMaxChunkKmers)
+    throw std::runtime_error("chunk_kmers must be in [64, 65535]");
+  if (blocks_per_cu < 1 || blocks_per_cu > kMaxBlocksPerCu)
+    throw std::runtime_error("blocks_per_cu must be in [1, 8]");
+  chunk_kmers_ = chunk_kmers;
+  blocks_per_cu_ = blocks_per_cu;
+  persistent_blocks_ = static_cast<u32>(std::max(1, cus_)) * blocks_per_cu;
+}
+
+LaunchShape Searcher::autotune(const char *d_chars, u64 n_chars, const u64 *d_read_ends, u64 n_reads,
+                               u64 *d_results) {
+  static constexpr u32 kChunks[] = {128, 256, 512, 1024};
+  static constexpr u32 kBlocks[] = {2, 3, 4, 6, 8};
+  const bool timing = timing_;
+  const int algorithm = algorithm_;
+  timing_ = true;
+  algorithm_ = kAlgoStreaming;
+  LaunchShape best{chunk_kmers_, blocks_per_cu_, 0.0f};
+  bool have = false;
+  for (u32 ch : kChunks) {
+    for (u32 bpc : kBlocks) {
+      set_launch(ch, bpc);
+      float ms = 0.0f;
+      for (int rep = 0; rep < 2; ++rep) {  // the second run is timed
+        search_reads_device(d_chars, n_chars, d_read_ends, n_reads, d_results, nullptr);
+        ms = last_timings().search_ms;
+      }
+      if (!have || ms < best.search_ms) best = {ch, bpc, ms};
+      have = true;
+    }
+  }
+  set_launch(best.chunk_kmers, best.blocks_per_cu);
+  timing_ = timing;
+  algorithm_ = algorithm;
+  return best;
 }
 
 void Searcher::last_stats(u64 out[kStatsWords]) {
@@ -932,10 +966,10 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
     // chunk table of (read, k-mer range) work items, then the persistent search
     u64 *d_chunk_ends = d_chunk_counts_ + max_reads_;
     k_chunk_count<<<grid_for(n_reads, 256), 256, 0, stream_>>>(d_kmer_ends_, d_read_ends, n_reads,
-                                                               n_chars, d_chunk_counts_);
+                                                               n_chars, chunk_kmers_, d_chunk_counts_);
     scan_inclusive_u64(d_chunk_counts_, n_reads, d_chunk_ends, d_chunk_scan_tmp_, stream_);
     k_chunk_fill<<<grid_for(n_reads, 256), 256, 0, stream_>>>(d_kmer_ends_, d_read_ends, n_reads,
-                                                              d_chunk_ends, d_chunks_);
+                                                              d_chunk_ends, chunk_kmers_, d_chunks_);
     k_reset_counter<<<1, 1, 0, stream_>>>(d_counter_);
     SBWT_HIP(hipGetLastError());
     if (timing_) SBWT_HIP(hipEventRecord(ev_[2], stream_));

@@ -17,6 +17,17 @@ struct SearchTimings {
   float pack_ms = 0, offsets_ms = 0, search_ms = 0;
 };
 
+// Launch shape of the streaming search (work item = up to chunk_kmers
+// consecutive k-mers of one read; blocks_per_cu persistent 256-lane
+// workgroups per CU, each holding 16 KiB of LDS for staged results).
+struct LaunchShape {
+  u32 chunk_kmers;
+  u32 blocks_per_cu;
+  float search_ms;  // measured by autotune
+};
+constexpr u32 kMinChunkKmers = 64, kMaxChunkKmers = 65535, kDefaultChunkKmers = 256;
+constexpr u32 kMaxBlocksPerCu = 8, kDefaultBlocksPerCu = 4;
+
 class Searcher {
  public:
   Searcher(const DeviceIndex *index, u64 max_chars, u64 max_reads, bool move_to_key_kmer,
@@ -39,6 +50,14 @@ class Searcher {
   // Device positions form (explicit positions already on the device).
   void search_positions_device(const u64 *d_bit_seqs, const u64 *d_positions, u64 n,
                                u64 *d_results);
+  // Launch shape of the streaming search: k-mers per work item and persistent
+  // workgroups per CU. autotune times the search of one caller batch for each
+  // shape of a small grid, keeps the fastest and returns it (results of the
+  // batch are in d_results, the same for every shape).
+  void set_launch(u32 chunk_kmers, u32 blocks_per_cu);
+  LaunchShape launch() const { return {chunk_kmers_, blocks_per_cu_, 0.0f}; }
+  LaunchShape autotune(const char *d_chars, u64 n_chars, const u64 *d_read_ends, u64 n_reads,
+                       u64 *d_results);
 
   hipStream_t stream() const { return stream_; }
   SearchTimings last_timings();
@@ -70,6 +89,9 @@ class Searcher {
   u64 *d_chunk_scan_tmp_ = nullptr;
   u64 *d_counter_ = nullptr;
   u32 persistent_blocks_ = 0;
+  int cus_ = 1;
+  u32 chunk_kmers_ = kDefaultChunkKmers;
+  u32 blocks_per_cu_ = kDefaultBlocksPerCu;
   u64 max_chars_, max_reads_;
   u64 *d_packed_ = nullptr;
   u32 *d_invalid_ = nullptr;

@@ -318,3 +318,44 @@ def test_absence_checks_settle_windows(genome_index):
     assert np.array_equal(res, orc.index_reads(chars, ends))
     assert st["absence_hits"] > 0 and st["absent_skipped"] > st["absence_hits"]
     assert S.Searcher.resolved(st) == len(res)
+
+
+@pytest.mark.parametrize("chunk,bpc", [(64, 1), (100, 3), (256, 4), (1000, 8), (65535, 2)])
+def test_launch_shapes_give_identical_results(genome_index, chunk, bpc):
+    """Work-item length and persistent workgroups per CU change only the speed."""
+    g, b, path, orc = genome_index
+    say(f"launch shape chunk={chunk} bpc={bpc}")
+    idx = S.Index.load(path)
+    lens = np.concatenate([np.full(500, 150, np.uint32), W.uniform_lengths(3, 100, 31, 3000)])
+    chars, ends, _ = W.reads(61, g, np.array([len(g)], np.uint64), lens, 0.2, 0.01, 0.001)
+    chars = chars.tobytes()
+    s = idx.searcher(len(chars), len(ends))
+    s.set_launch(chunk, bpc)
+    assert s.launch() == (chunk, bpc)
+    assert np.array_equal(s.search_reads(chars, ends), orc.index_reads(chars, ends))
+
+
+def test_launch_shape_bounds(genome_index):
+    g, b, path, orc = genome_index
+    s = S.Index.load(path).searcher(1000, 10)
+    for bad in ((63, 4), (65536, 4), (256, 0), (256, 9)):
+        with pytest.raises(S.SbwtError):
+            s.set_launch(*bad)
+
+
+def test_autotune_keeps_results(genome_index):
+    import torch
+    g, b, path, orc = genome_index
+    say("autotune")
+    idx = S.Index.load(path)
+    chars, ends = _reads(g, 3000, 150, seed=91)
+    dc = torch.frombuffer(bytearray(chars), dtype=torch.uint8).cuda()
+    de = torch.from_numpy(ends.astype(np.int64)).cuda()
+    dr = torch.empty(len(chars), dtype=torch.int64, device="cuda")
+    s = idx.searcher(len(chars), len(ends))
+    ch, bpc, ms = s.autotune(dc.data_ptr(), len(chars), de.data_ptr(), len(ends), dr.data_ptr())
+    assert s.launch() == (ch, bpc) and ch in (128, 256, 512, 1024) and bpc in (2, 3, 4, 6, 8)
+    assert ms > 0
+    exp = orc.index_reads(chars, ends)
+    assert np.array_equal(dr[:len(exp)].cpu().numpy().view(np.uint64), exp)
+    assert np.array_equal(s.search_reads(chars, ends), exp)
