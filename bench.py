@@ -321,8 +321,7 @@ def mode_serve(args, configs, S, torch, W):
     """C5-style serving: Poisson requests, served (continuous batching) vs plain sequential."""
     import modes
     t0 = time.time()
-    codes, ends = configs.c3_genomes(args.serve_genomes, args.serve_roots, args.serve_length,
-                                     args.serve_length)
+    codes, ends = configs.c5_genomes(args.serve_genomes, args.serve_roots, args.serve_length)
     built = W.BuiltSbwt(codes, ends, configs.K)
     log(f"serve index: {built.n} nodes ({time.time() - t0:.1f} s)")
     idx, cidx = _colored_index(W, S, built, np.arange(len(ends)), 20, "serve")
@@ -337,7 +336,7 @@ def mode_serve(args, configs, S, torch, W):
     print(json.dumps({"metric": "p50/p99 request latency + reads/s, served vs plain",
                       "mode": "serve", "config": {
                           "index": f"{args.serve_genomes} genomes x {args.serve_length} bp from "
-                                   f"{args.serve_roots} roots (1-5% substitutions), "
+                                   f"{args.serve_roots} roots (3% substitutions), "
                                    f"{built.n} nodes, {args.serve_genomes} colors",
                           "requests": "Poisson, N log-uniform in [1, max] reads of U[75,250] bp"},
                       **r}), flush=True)

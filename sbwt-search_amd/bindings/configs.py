@@ -18,6 +18,9 @@ Each builder returns plain numpy data; nothing here touches the GPU.
       1,000,000 reads with log-normal lengths (median 2,000 bp, sigma 1.2,
       clipped to [31, 100,000]) and 4,000,000 reads x 150 bp, interleaved in a
       seeded random order (reads seed 4); sampled as in C2.
+  C5  serving index: 200 genomes of 5 Mbp, genome g = root g mod 10 with 3%
+      substitutions, one color per genome; requests of N log-uniform in [1, 10^5]
+      reads of length U[75, 250].
 """
 import numpy as np
 
@@ -91,6 +94,13 @@ def c3(n_reads=5_000_000, n_genomes=1000, n_roots=20, min_len=2_000_000, max_len
     return Workload("C3", f"color search: {n_genomes} genomes of {min_len}-{max_len} bp from "
                     f"{n_roots} seeded roots (1-5% substitutions), one color each, d=20; "
                     f"{n_reads} reads of U[50,300] bp with 1% error", codes, ends, chars, rends)
+
+
+def c5_genomes(n_genomes=200, n_roots=10, length=5_000_000, rate=0.03):
+    """Serving index: genome g is root g mod n_roots with `rate` substitutions."""
+    roots = [W.genome_random(200 + i, length) for i in range(n_roots)]
+    gens = [W.genome_mutate(roots[g % n_roots], 2000 + g, rate) for g in range(n_genomes)]
+    return W.concat(gens)
 
 
 def c4(n_long=1_000_000, n_short=4_000_000, n_genomes=50, length=2_000_000):
