@@ -39,10 +39,12 @@ def parse():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
-    ap.add_argument("--mode", default="index", choices=["index", "batched", "serve", "colors"],
+    ap.add_argument("--mode", default="index",
+                    choices=["index", "batched", "serve", "colors", "cli"],
                     help="index: the headline line (default); batched: C4 plain vs batched "
                          "form; serve: Poisson request stream, served vs plain sequential; "
-                         "colors: the color stage on the C3 recipe")
+                         "colors: the color stage on the C3 recipe; cli: the sbwt_search "
+                         "command end to end on C2 written as FASTA")
     ap.add_argument("--color-genomes", type=int, default=200)
     ap.add_argument("--color-roots", type=int, default=20)
     ap.add_argument("--color-min-len", type=int, default=2_000_000)
@@ -106,6 +108,8 @@ def main():
         return mode_batched(args, configs, S, torch, W)
     if args.mode == "serve":
         return mode_serve(args, configs, S, torch, W)
+    if args.mode == "cli":
+        return mode_cli(args, configs, S, torch, W)
     if args.mode == "colors":
         return mode_colors(args, configs, S, torch, W)
     t0 = time.time()
@@ -391,6 +395,53 @@ def mode_colors(args, configs, S, torch, W):
                       "num_colors": int(cidx.num_colors),
                       "color_sets": int(cidx.num_color_sets), "threshold": 0.7},
            "seconds": med}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def mode_cli(args, configs, S, torch, W):
+    """C2 reads as a FASTA file through `sbwt_search index` (file to file)."""
+    import subprocess
+    import tempfile
+    wl = configs.c2(args.reads or 10_000_000)
+    lens = np.diff(np.concatenate([[0], wl.read_ends.astype(np.int64)]))
+    if not np.all(lens == lens[0]):
+        raise SystemExit("cli mode writes fixed-length reads")
+    built = W.BuiltSbwt(wl.genomes, wl.genome_ends, configs.K)
+    tmp = tempfile.mkdtemp(prefix="sbwt_cli_")
+    sbwt_path, fasta = os.path.join(tmp, "index.sbwt"), os.path.join(tmp, "reads.fna")
+    built.write(sbwt_path)
+    n, L = wl.n_reads, int(lens[0])
+    rec = np.empty((n, L + 4), np.uint8)
+    rec[:, 0], rec[:, 1], rec[:, 2] = ord(">"), ord("r"), ord("\n")
+    rec[:, 3:3 + L] = wl.chars.reshape(n, L)
+    rec[:, 3 + L] = ord("\n")
+    rec.tofile(fasta)
+    in_bytes = rec.nbytes
+    del rec
+    exe = os.path.join(REPO, "sbwt-search_amd", "host", "sbwt_search")
+    out = {"metric": "sbwt_search index end to end (FASTA file to results file)", "mode": "cli",
+           "config": {"workload": wl.name + " " + wl.description, "fasta_bytes": in_bytes,
+                      "tmpdir": tmp}}
+    for fmt in ("ascii", "binary"):
+        prefix = os.path.join(tmp, "out_" + fmt)
+        env = dict(os.environ, SBWT_SEARCH_TIMING="1")
+        t0 = time.perf_counter()
+        r = subprocess.run([exe, "index", "-q", fasta, "-i", sbwt_path, "-o", prefix, "-p", fmt],
+                           env=env, capture_output=True, text=True)
+        dt = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise SystemExit(f"sbwt_search failed: {r.stderr}")
+        out_file = prefix + (".txt" if fmt == "ascii" else ".bin")
+        size = os.path.getsize(out_file)
+        log(f"cli {fmt}: {dt:.2f} s, {r.stderr.strip()}")
+        out[fmt] = {"seconds": dt, "reads_per_s": n / dt, "kmers_per_s": wl.n_kmers() / dt,
+                    "input_MB_per_s": in_bytes / dt / 1e6, "output_bytes": size,
+                    "stages": r.stderr.strip()}
+        os.remove(out_file)
+    os.remove(fasta)
+    os.remove(sbwt_path)
+    os.rmdir(tmp)
     print(json.dumps(out), flush=True)
     return 0
 

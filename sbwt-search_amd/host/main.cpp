@@ -9,6 +9,8 @@
 // and ColorSearchArgumentParser.cpp:27-180; the memory options (-u -m -c -g -r)
 // are accepted and bound the batch sizes.
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -98,6 +100,23 @@ u64 chars_per_batch(double gpu_fraction, u64 streams) {
   return n;
 }
 
+// Stage times of the index command (seconds summed over streams), printed to
+// stderr when SBWT_SEARCH_TIMING is set.
+struct StageTimes {
+  std::atomic<u64> parse_ns{0}, search_ns{0}, format_ns{0};
+};
+StageTimes g_times;
+
+struct Clock {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  u64 lap() {
+    const auto t = std::chrono::steady_clock::now();
+    const u64 ns = static_cast<u64>(std::chrono::duration_cast<std::chrono::nanoseconds>(t - t0).count());
+    t0 = t;
+    return ns;
+  }
+};
+
 // ---------------------------------------------------------------- index
 void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<std::string> &in,
                   const std::vector<std::string> &out, const std::vector<u64> &files,
@@ -112,11 +131,14 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
   for (u64 f : files) {
     sbwt_host::SeqReader reader(in[f]);
     sbwt_host::IndexWriter writer(out[f], mode, headers);
+    Clock clk;
     auto flush_batch = [&] {
       if (ends.empty()) return;
+      g_times.parse_ns += clk.lap();
       u64 nk = 0;
       check(sbwt_searcher_search_reads(s, chars.data(), chars.size(), ends.data(), ends.size(),
                                        results.data(), &nk));
+      g_times.search_ns += clk.lap();
       u64 ri = 0, cs = 0;
       for (u64 e = 0; e < ends.size(); ++e) {
         const u64 len = ends[e] - cs;
@@ -125,6 +147,7 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
         if (closes[e]) writer.end_read();
         cs = ends[e];
       }
+      g_times.format_ns += clk.lap();
       chars.clear();
       ends.clear();
       closes.clear();
@@ -195,6 +218,10 @@ int run_index(const Args &a) {
   sbwt_index_destroy(index);
   for (auto &e : errors)
     if (!e.empty()) throw std::runtime_error(e);
+  if (std::getenv("SBWT_SEARCH_TIMING"))
+    std::fprintf(stderr, "stage seconds (summed over %llu streams): parse %.3f search %.3f format+write %.3f\n",
+                 static_cast<unsigned long long>(streams), g_times.parse_ns * 1e-9,
+                 g_times.search_ns * 1e-9, g_times.format_ns * 1e-9);
   return 0;
 }
 
