@@ -192,6 +192,55 @@ int sbwt_searcher_autotune(sbwt_searcher *searcher, const char *d_chars, uint64_
 int sbwt_pack_2bit_device(const char *d_chars, uint64_t n_chars, uint64_t *d_words,
                           uint32_t *d_invalid, void *hip_stream);
 
+/* ---- Device formatting of index results ------------------------------ */
+
+/* Formats index results into the bytes of the reference's index output files
+ * (headers excluded; write them first): ascii "v v v\n" with -1 not found and
+ * -2 invalid, binary u64 per result and u64 max - 2 per read end, bool '0'
+ * found / '1' not found / '2' invalid and '\n' per read end. Replaces the
+ * per-result loops of ContinuousIndexResultsPrinter::process_batch
+ * (src/ResultsPrinter/ContinuousIndexResultsPrinter.hpp:146-308) and the
+ * Ascii/Binary/Bool printers' converters (AsciiContinuousIndexResultsPrinter.cpp:50-85,
+ * BinaryContinuousIndexResultsPrinter.cpp:52-78, BoolContinuousIndexResultsPrinter.cpp:43-69).
+ * hip_stream nullable (the formatter then owns one). */
+#define SBWT_FORMAT_ASCII 0
+#define SBWT_FORMAT_BINARY 1
+#define SBWT_FORMAT_BOOL 2
+typedef struct sbwt_formatter sbwt_formatter;
+int sbwt_formatter_create(uint64_t max_kmers, uint64_t max_pieces, void *hip_stream,
+                          sbwt_formatter **out);
+void sbwt_formatter_destroy(sbwt_formatter *formatter);
+
+/* One batch of read pieces: piece e holds the results
+ * d_results[d_piece_kmer_ends[e-1] .. d_piece_kmer_ends[e]) (cumulative, non-
+ * decreasing; sbwt_searcher_kmer_ends_device gives them for a search batch)
+ * and ends its read when d_piece_closes[e] != 0, else the next piece continues
+ * the same line; line_open: the first piece continues a line on which the
+ * previous batch already wrote results. *n_bytes receives the batch's byte count; the bytes go to
+ * d_out only when d_out is not null and capacity >= *n_bytes (a call with
+ * d_out = NULL sizes the buffer). Synchronous. */
+int sbwt_formatter_format(sbwt_formatter *formatter, int mode, const uint64_t *d_results,
+                          const uint64_t *d_piece_kmer_ends, const uint8_t *d_piece_closes,
+                          uint64_t n_pieces, int line_open, char *d_out, uint64_t capacity,
+                          uint64_t *n_bytes);
+
+/* Host batch in, formatted index output bytes out: the search of
+ * sbwt_searcher_search_reads (results stay on the device), then the formatter
+ * on its pieces = the batch's reads (read_closes[e] != 0 when read e is
+ * complete; 0 when the next read, in this batch or the next, continues its
+ * line), one copy back. *n_bytes receives the byte count; the bytes are
+ * copied to out when capacity >= *n_bytes, else they wait in the formatter for
+ * sbwt_formatter_copy_last. Synchronous. Replaces one batch of the
+ * reference's ContinuousIndexSearcher + ContinuousIndexResultsPrinter
+ * (src/IndexSearcher/ContinuousIndexSearcher.cpp:67-74,
+ * ContinuousIndexResultsPrinter.hpp:146-308). */
+int sbwt_searcher_search_reads_format(sbwt_searcher *searcher, sbwt_formatter *formatter, int mode,
+                                      const char *chars, uint64_t n_chars,
+                                      const uint64_t *read_ends, const uint8_t *read_closes,
+                                      uint64_t n_reads, int line_open, char *out,
+                                      uint64_t capacity, uint64_t *n_bytes);
+int sbwt_formatter_copy_last(sbwt_formatter *formatter, char *out, uint64_t capacity);
+
 /* Device pointer to the cumulative k-mer count of every read of the last
  * sbwt_searcher_search_reads(_device) call (n_reads entries), valid until the
  * next call; it chains the index stage into the color stage on device. No

@@ -93,6 +93,11 @@ def lib():
                                                  C.POINTER(C.c_float)]
         L.sbwt_pack_2bit_device.argtypes = [vp, C.c_uint64, vp, vp, vp]
         L.sbwt_searcher_kmer_ends_device.argtypes = [vp, C.POINTER(vp)]
+        L.sbwt_formatter_create.argtypes = [C.c_uint64, C.c_uint64, vp, C.POINTER(vp)]
+        L.sbwt_formatter_destroy.argtypes = [vp]
+        L.sbwt_formatter_destroy.restype = None
+        L.sbwt_formatter_format.argtypes = [vp, C.c_int, vp, vp, vp, C.c_uint64, C.c_int, vp,
+                                            C.c_uint64, C.POINTER(C.c_uint64)]
         L.sbwt_color_index_load.argtypes = [C.c_char_p, C.POINTER(vp)]
         L.sbwt_color_index_destroy.argtypes = [vp]
         L.sbwt_color_index_destroy.restype = None
@@ -315,6 +320,48 @@ def pack_2bit_device(d_chars, n_chars, d_words, d_invalid, stream=None):
     _check(lib().sbwt_pack_2bit_device(C.c_void_p(d_chars), n_chars, C.c_void_p(d_words),
                                        C.c_void_p(d_invalid),
                                        C.c_void_p(stream) if stream else None))
+
+
+FORMAT_ASCII, FORMAT_BINARY, FORMAT_BOOL = 0, 1, 2
+
+
+class Formatter:
+    """Device formatter of index results into the index output bytes."""
+
+    def __init__(self, max_kmers, max_pieces, stream=None):
+        self.h = C.c_void_p()
+        _check(lib().sbwt_formatter_create(max_kmers, max_pieces, stream, C.byref(self.h)))
+
+    def format_device(self, mode, d_results, d_piece_kmer_ends, d_piece_closes, n_pieces,
+                      line_open, d_out=None, capacity=0):
+        """Device pointers (ints). Returns the byte count (written if d_out fits it)."""
+        n = C.c_uint64()
+        _check(lib().sbwt_formatter_format(self.h, mode, d_results, d_piece_kmer_ends,
+                                           d_piece_closes, n_pieces, int(line_open), d_out,
+                                           capacity, C.byref(n)))
+        return int(n.value)
+
+    def format(self, mode, results, piece_kmer_ends, piece_closes, line_open=False):
+        """Host arrays in, bytes out (via torch device buffers)."""
+        import torch
+        res = torch.from_numpy(np.ascontiguousarray(results, np.uint64).view(np.int64)).cuda()
+        ke = torch.from_numpy(np.ascontiguousarray(piece_kmer_ends, np.uint64).view(np.int64)).cuda()
+        cl = torch.from_numpy(np.ascontiguousarray(piece_closes, np.uint8)).cuda()
+        n = self.format_device(mode, res.data_ptr(), ke.data_ptr(), cl.data_ptr(), len(ke),
+                               line_open)
+        out = torch.empty(max(n, 1), dtype=torch.uint8, device="cuda")
+        n2 = self.format_device(mode, res.data_ptr(), ke.data_ptr(), cl.data_ptr(), len(ke),
+                                line_open, out.data_ptr(), n)
+        assert n2 == n
+        return out[:n].cpu().numpy().tobytes()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sbwt_formatter_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 class ColorIndex:

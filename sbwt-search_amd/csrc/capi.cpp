@@ -7,6 +7,7 @@
 #include "../../include/sbwt_amd.h"
 #include "color_search.hpp"
 #include "device_index.hpp"
+#include "format.hpp"
 #include "index_search.hpp"
 #include "loaders.hpp"
 #include "server.hpp"
@@ -30,6 +31,10 @@ struct sbwt_color_index {
 
 struct sbwt_server {
   std::unique_ptr<sbwt_amd::Server> impl;
+};
+
+struct sbwt_formatter {
+  std::unique_ptr<sbwt_amd::Formatter> impl;
 };
 
 struct sbwt_color_searcher {
@@ -269,6 +274,60 @@ int sbwt_pack_2bit_device(const char *d_chars, uint64_t n_chars, uint64_t *d_wor
                           uint32_t *d_invalid, void *hip_stream) {
   return guarded([&] {
     sbwt_amd::launch_pack(d_chars, n_chars, d_words, d_invalid, static_cast<hipStream_t>(hip_stream));
+  });
+}
+
+int sbwt_formatter_create(uint64_t max_kmers, uint64_t max_pieces, void *hip_stream,
+                          sbwt_formatter **out) {
+  return guarded([&] {
+    require(out != nullptr, "null argument");
+    require(max_pieces >= 1, "max_pieces must be at least 1");
+    auto f = std::make_unique<sbwt_formatter>();
+    f->impl = std::make_unique<sbwt_amd::Formatter>(max_kmers, max_pieces,
+                                                    static_cast<hipStream_t>(hip_stream));
+    *out = f.release();
+  });
+}
+
+void sbwt_formatter_destroy(sbwt_formatter *formatter) { delete formatter; }
+
+int sbwt_formatter_format(sbwt_formatter *formatter, int mode, const uint64_t *d_results,
+                          const uint64_t *d_piece_kmer_ends, const uint8_t *d_piece_closes,
+                          uint64_t n_pieces, int line_open, char *d_out, uint64_t capacity,
+                          uint64_t *n_bytes) {
+  return guarded([&] {
+    require(formatter && n_bytes, "null argument");
+    require(mode == SBWT_FORMAT_ASCII || mode == SBWT_FORMAT_BINARY || mode == SBWT_FORMAT_BOOL,
+            "unknown output format");
+    require(n_pieces == 0 || (d_piece_kmer_ends && d_piece_closes), "null argument");
+    *n_bytes = formatter->impl->format(mode, d_results, d_piece_kmer_ends, d_piece_closes, n_pieces,
+                                       line_open != 0, d_out, capacity);
+  });
+}
+
+int sbwt_searcher_search_reads_format(sbwt_searcher *searcher, sbwt_formatter *formatter, int mode,
+                                      const char *chars, uint64_t n_chars,
+                                      const uint64_t *read_ends, const uint8_t *read_closes,
+                                      uint64_t n_reads, int line_open, char *out,
+                                      uint64_t capacity, uint64_t *n_bytes) {
+  return guarded([&] {
+    require(searcher && formatter && n_bytes, "null argument");
+    require(mode == SBWT_FORMAT_ASCII || mode == SBWT_FORMAT_BINARY || mode == SBWT_FORMAT_BOOL,
+            "unknown output format");
+    require(n_reads == 0 || (read_ends && read_closes && (chars || n_chars == 0)), "null argument");
+    *n_bytes = 0;
+    if (n_reads == 0) return;
+    searcher->impl->search_reads_host(chars, n_chars, read_ends, n_reads, nullptr);
+    *n_bytes = formatter->impl->format_to_host(mode, searcher->impl->results_device(),
+                                               searcher->impl->kmer_ends_device(), read_closes,
+                                               n_reads, line_open != 0, out, capacity);
+  });
+}
+
+int sbwt_formatter_copy_last(sbwt_formatter *formatter, char *out, uint64_t capacity) {
+  return guarded([&] {
+    require(formatter && (out || capacity == 0), "null argument");
+    formatter->impl->copy_last(out, capacity);
   });
 }
 

@@ -16,6 +16,7 @@ using u64 = uint64_t;
 
 constexpr u64 kNotFound = ~0ULL;        // reference "-1" (IndexSearcher.cuh:57)
 constexpr u64 kInvalid = ~0ULL - 1;     // reference "-2" (BinaryContinuousIndexResultsPrinter)
+constexpr u64 kNewline = ~0ULL - 2;     // end of a read in the binary index format (same file)
 constexpr u64 kWalkStuck = ~0ULL - 3;   // key walk reached a row with no edge and no mark
 // Key-destination table entries that are not node indexes: the walk from the
 // node is longer than the table's build cap (walk at query time), or stuck.

@@ -1016,7 +1016,8 @@ u64 Searcher::search_reads_host(const char *chars, u64 n_chars, const u64 *read_
   if (n_chars) SBWT_HIP(hipMemcpyAsync(d_chars_, chars, n_chars, hipMemcpyHostToDevice, stream_));
   SBWT_HIP(hipMemcpyAsync(d_read_ends_, read_ends, n_reads * 8, hipMemcpyHostToDevice, stream_));
   search_reads_device(d_chars_, n_chars, d_read_ends_, n_reads, d_results_, nullptr);
-  if (nk) SBWT_HIP(hipMemcpyAsync(results, d_results_, nk * 8, hipMemcpyDeviceToHost, stream_));
+  if (nk && results)
+    SBWT_HIP(hipMemcpyAsync(results, d_results_, nk * 8, hipMemcpyDeviceToHost, stream_));
   SBWT_HIP(hipStreamSynchronize(stream_));
   return nk;
 }

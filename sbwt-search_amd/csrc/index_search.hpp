@@ -45,6 +45,7 @@ class Searcher {
   u64 search_reads_device(const char *d_chars, u64 n_chars, const u64 *d_read_ends, u64 n_reads,
                           u64 *d_results, u64 *n_kmers_host);
   // Same with host buffers, synchronous.
+  // results may be null: they then stay on the device (results_device()).
   u64 search_reads_host(const char *chars, u64 n_chars, const u64 *read_ends, u64 n_reads,
                         u64 *results);
   // Device positions form (explicit positions already on the device).
@@ -70,6 +71,7 @@ class Searcher {
   int algorithm() const { return algorithm_; }
   void last_stats(u64 out[kStatsWords]);
   const u64 *kmer_ends_device() const { return d_kmer_ends_; }
+  const u64 *results_device() const { return d_results_; }
   u64 max_chars() const { return max_chars_; }
   u64 max_reads() const { return max_reads_; }
 

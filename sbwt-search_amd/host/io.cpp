@@ -103,71 +103,22 @@ bool SeqReader::next(std::vector<char> &chars) {
 }
 
 // ------------------------------------------------------------ index writer
-IndexWriter::IndexWriter(const std::string &path, const std::string &mode, bool headers) {
+IndexFileOut::IndexFileOut(const std::string &path, int mode, bool headers) {
   static const char *ext[3] = {".txt", ".bin", ".bool"};
   static const char *fmt[3] = {"ascii", "binary", "bool"};
   static const char *ver[3] = {"v1.0", "v1.0", "v2.0"};
-  if (mode == "ascii") mode_ = 0;
-  else if (mode == "binary") mode_ = 1;
-  else if (mode == "bool") mode_ = 2;
-  else throw std::runtime_error("Invalid value passed by user for argument print_mode");
-  f_ = fopen((path + ext[mode_]).c_str(), "wb");
-  if (!f_) throw std::runtime_error("Unable to open file " + path + ext[mode_]);
+  if (mode < 0 || mode > 2) throw std::runtime_error("Invalid value passed by user for argument print_mode");
+  f_ = fopen((path + ext[mode]).c_str(), "wb");
+  if (!f_) throw std::runtime_error("Unable to open file " + path + ext[mode]);
   if (headers) {
-    write_sized_string(f_, fmt[mode_]);
-    write_sized_string(f_, ver[mode_]);
+    write_sized_string(f_, fmt[mode]);
+    write_sized_string(f_, ver[mode]);
   }
-  out_.reserve(kFlush + 64);
 }
 
-IndexWriter::~IndexWriter() {
-  try {
-    flush();
-  } catch (...) {
-  }
-  fclose(f_);
-}
+IndexFileOut::~IndexFileOut() { fclose(f_); }
 
-void IndexWriter::result(u64 v) {
-  if (mode_ == 0) {
-    if (!first_in_line_) out_.push_back(' ');
-    if (v == kInvalid) {
-      out_.push_back('-');
-      out_.push_back('2');
-    } else if (v == kNotFound) {
-      out_.push_back('-');
-      out_.push_back('1');
-    } else {
-      char tmp[24];
-      auto r = std::to_chars(tmp, tmp + sizeof tmp, v);
-      out_.insert(out_.end(), tmp, r.ptr);
-    }
-  } else if (mode_ == 1) {
-    const char *p = reinterpret_cast<const char *>(&v);
-    out_.insert(out_.end(), p, p + 8);
-  } else {
-    out_.push_back(v == kInvalid ? '2' : v == kNotFound ? '1' : '0');
-  }
-  first_in_line_ = false;
-  if (out_.size() >= kFlush) flush();
-}
-
-void IndexWriter::end_read() {
-  if (mode_ == 1) {
-    const u64 v = kNewline;
-    const char *p = reinterpret_cast<const char *>(&v);
-    out_.insert(out_.end(), p, p + 8);
-  } else {
-    out_.push_back('\n');
-  }
-  first_in_line_ = true;
-  if (out_.size() >= kFlush) flush();
-}
-
-void IndexWriter::flush() {
-  write_all(f_, out_.data(), out_.size());
-  out_.clear();
-}
+void IndexFileOut::write(const char *p, u64 n) { write_all(f_, p, n); }
 
 // ------------------------------------------------------------ color writer
 ColorWriter::ColorWriter(const std::string &path, const std::string &mode, bool headers,

@@ -36,29 +36,17 @@ class SeqReader {
   int pending_ = -1;  // first char of the next header, already read
 };
 
-// A batch of whole reads (or pieces of one long read).
-struct ReadBatch {
-  std::vector<char> chars;
-  std::vector<u64> ends;         // cumulative read ends
-  std::vector<u64> piece_of;     // per entry: read number in the file
-  bool continues_previous = false;  // first entry continues the last read of the previous batch
-};
-
-// Output writers for the index stage (src/IndexResultsPrinter/*):
-// ascii v1.0, binary v1.0, bool v2.0, headers = u64 length + string.
-class IndexWriter {
+// Index output file (src/IndexResultsPrinter/*): the header (u64 length +
+// "ascii"/"binary"/"bool", u64 length + "v1.0"/"v1.0"/"v2.0") unless
+// headers are off, then the formatted bytes (formatted on the device).
+class IndexFileOut {
  public:
-  IndexWriter(const std::string &path, const std::string &mode, bool headers);
-  ~IndexWriter();
-  void result(u64 v);   // one k-mer result
-  void end_read();      // read separator
-  void flush();
+  IndexFileOut(const std::string &path, int mode, bool headers);
+  ~IndexFileOut();
+  void write(const char *p, u64 n);
 
  private:
   FILE *f_;
-  int mode_;  // 0 ascii, 1 binary, 2 bool
-  bool first_in_line_ = true;
-  std::vector<char> out_;
 };
 
 // Output writers for the color stage (src/ColorResultsPrinter/*): ascii v1.0

@@ -12,6 +12,9 @@
 #include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -26,6 +29,7 @@
 
 using sbwt_host::u32;
 using sbwt_host::u64;
+using u8 = uint8_t;
 
 namespace {
 
@@ -89,19 +93,20 @@ double fraction(const Args &a, const std::string &key, const std::string &def, c
   return v;
 }
 
-// chars per index-search batch from free device memory (about 13 bytes per
-// char: input, results, packed sequence, invalid mask, offsets).
+// chars per index-search batch from free device memory: input, packed
+// sequence, invalid mask, results and offsets (about 13 bytes per char) plus
+// the formatted output (up to 21 bytes per k-mer in ascii).
 u64 chars_per_batch(double gpu_fraction, u64 streams) {
   u64 free_b = 0, total = 0;
   check(sbwt_mem_info(&free_b, &total));
   const u64 budget = static_cast<u64>(static_cast<double>(free_b) * gpu_fraction) / std::max<u64>(streams, 1);
-  const u64 n = std::min<u64>(budget / 13, 1ULL << 30);
+  const u64 n = std::min<u64>(budget / 34, 1ULL << 28);
   if (n == 0) throw std::runtime_error("Not enough memory");
   return n;
 }
 
-// Stage times of the index command (seconds summed over streams), printed to
-// stderr when SBWT_SEARCH_TIMING is set.
+// Busy time of each stage of the index command (seconds summed over
+// streams), printed to stderr when SBWT_SEARCH_TIMING is set.
 struct StageTimes {
   std::atomic<u64> parse_ns{0}, search_ns{0}, format_ns{0};
 };
@@ -117,67 +122,195 @@ struct Clock {
   }
 };
 
-// ---------------------------------------------------------------- index
-void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<std::string> &in,
-                  const std::vector<std::string> &out, const std::vector<u64> &files,
-                  const std::string &mode, bool headers, u64 max_chars) {
-  const u64 max_reads = std::max<u64>(max_chars / 16, 1024);
-  sbwt_searcher *s = nullptr;
-  check(sbwt_searcher_create(index, max_chars, max_reads, move_to_key, nullptr, &s));
-  std::vector<u64> results(max_chars + 1);
-  std::vector<char> chars, read;
-  std::vector<u64> ends;
-  std::vector<char> closes;  // entry ends its read (else the next entry continues it)
-  for (u64 f : files) {
-    sbwt_host::SeqReader reader(in[f]);
-    sbwt_host::IndexWriter writer(out[f], mode, headers);
-    Clock clk;
-    auto flush_batch = [&] {
-      if (ends.empty()) return;
-      g_times.parse_ns += clk.lap();
-      u64 nk = 0;
-      check(sbwt_searcher_search_reads(s, chars.data(), chars.size(), ends.data(), ends.size(),
-                                       results.data(), &nk));
-      g_times.search_ns += clk.lap();
-      u64 ri = 0, cs = 0;
-      for (u64 e = 0; e < ends.size(); ++e) {
-        const u64 len = ends[e] - cs;
-        const u64 n = len >= k ? len - k + 1 : 0;
-        for (u64 j = 0; j < n; ++j) writer.result(results[ri++]);
-        if (closes[e]) writer.end_read();
-        cs = ends[e];
-      }
-      g_times.format_ns += clk.lap();
-      chars.clear();
-      ends.clear();
-      closes.clear();
-    };
-    while (true) {
-      read.clear();
-      if (!reader.next(read)) break;
-      // a read longer than the room left is cut into pieces overlapping by k-1
-      // chars (the carry-over of ContinuousSequenceFileParser.cpp:82-112)
-      u64 st = 0;
-      while (true) {
-        const u64 room = max_chars - chars.size();
-        if (room < k && !ends.empty()) {
-          flush_batch();
-          continue;
-        }
-        const u64 take = std::min<u64>(read.size() - st, room);
-        chars.insert(chars.end(), read.begin() + st, read.begin() + st + take);
-        ends.push_back(chars.size());
-        const bool last = st + take == read.size();
-        closes.push_back(last);
-        if (last) break;
-        st += take - (k - 1);
-        flush_batch();
-      }
-      if (chars.size() >= max_chars || ends.size() >= max_reads) flush_batch();
-    }
-    flush_batch();
+// Blocking queue of bounded size between the stages of one stream; abort()
+// wakes every waiter (a failing stage stops the others).
+template <typename T>
+class Channel {
+ public:
+  explicit Channel(size_t cap) : cap_(cap) {}
+  bool push(T v) {
+    std::unique_lock<std::mutex> g(mu_);
+    cv_push_.wait(g, [&] { return q_.size() < cap_ || aborted_; });
+    if (aborted_) return false;
+    q_.push_back(std::move(v));
+    cv_pop_.notify_one();
+    return true;
   }
-  sbwt_searcher_destroy(s);
+  bool pop(T &v) {
+    std::unique_lock<std::mutex> g(mu_);
+    cv_pop_.wait(g, [&] { return !q_.empty() || closed_ || aborted_; });
+    if (aborted_ || q_.empty()) return false;
+    v = std::move(q_.front());
+    q_.pop_front();
+    cv_push_.notify_one();
+    return true;
+  }
+  void close() {
+    std::lock_guard<std::mutex> g(mu_);
+    closed_ = true;
+    cv_pop_.notify_all();
+  }
+  void abort() {
+    std::lock_guard<std::mutex> g(mu_);
+    aborted_ = true;
+    cv_pop_.notify_all();
+    cv_push_.notify_all();
+  }
+
+ private:
+  size_t cap_;
+  std::mutex mu_;
+  std::condition_variable cv_push_, cv_pop_;
+  std::deque<T> q_;
+  bool closed_ = false, aborted_ = false;
+};
+
+// One device batch: whole reads, or pieces of a long read (closes[e] == 0:
+// the next entry continues the same read, and so the same output line).
+struct InBatch {
+  u64 file = 0;
+  bool first = false, last = false;  // first / last batch of its file
+  std::vector<char> chars;
+  std::vector<u64> ends;
+  std::vector<u8> closes;
+};
+
+struct OutChunk {
+  u64 file = 0;
+  bool first = false, last = false;
+  std::vector<char> bytes;
+};
+
+// One stream of the index command, three stages on three threads: the reader
+// parses reads into batches; the device stage searches a batch and formats
+// its results into the output bytes on the device; the writer appends them
+// to the file. Parsing, device work and writing of different batches overlap.
+void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<std::string> &in,
+                  const std::vector<std::string> &out, const std::vector<u64> &files, int mode,
+                  bool headers, u64 max_chars) {
+  const u64 max_reads = std::max<u64>(max_chars / 16, 1024);
+  Channel<InBatch> to_device(2);
+  Channel<OutChunk> to_writer(2);
+  std::string error;
+  std::mutex error_mu;
+  auto fail = [&](const std::exception &e) {
+    {
+      std::lock_guard<std::mutex> g(error_mu);
+      if (error.empty()) error = e.what();
+    }
+    to_device.abort();
+    to_writer.abort();
+  };
+
+  std::thread device([&] {
+    sbwt_searcher *s = nullptr;
+    sbwt_formatter *fmt = nullptr;
+    try {
+      check(sbwt_searcher_create(index, max_chars, max_reads, move_to_key, nullptr, &s));
+      check(sbwt_formatter_create(max_chars, max_reads, nullptr, &fmt));
+      bool line_open = false;
+      std::vector<char> spare;
+      InBatch b;
+      while (to_device.pop(b)) {
+        Clock clk;
+        OutChunk o;
+        o.file = b.file;
+        o.first = b.first;
+        o.last = b.last;
+        if (b.first) line_open = false;
+        if (!b.ends.empty()) {
+          o.bytes.swap(spare);
+          o.bytes.resize(o.bytes.capacity());
+          u64 n = 0;
+          check(sbwt_searcher_search_reads_format(s, fmt, mode, b.chars.data(), b.chars.size(),
+                                                  b.ends.data(), b.closes.data(), b.ends.size(),
+                                                  line_open, o.bytes.data(), o.bytes.size(), &n));
+          if (n > o.bytes.size()) {
+            o.bytes.resize(n + n / 4);
+            check(sbwt_formatter_copy_last(fmt, o.bytes.data(), o.bytes.size()));
+          }
+          o.bytes.resize(n);
+          line_open = b.closes.back() == 0;
+        }
+        g_times.search_ns += clk.lap();
+        if (!to_writer.push(std::move(o))) break;
+      }
+      to_writer.close();
+    } catch (const std::exception &e) {
+      fail(e);
+    }
+    if (fmt) sbwt_formatter_destroy(fmt);
+    if (s) sbwt_searcher_destroy(s);
+  });
+
+  std::thread writer([&] {
+    try {
+      std::unique_ptr<sbwt_host::IndexFileOut> f;
+      OutChunk o;
+      while (to_writer.pop(o)) {
+        Clock clk;
+        if (o.first) f = std::make_unique<sbwt_host::IndexFileOut>(out[o.file], mode, headers);
+        if (!o.bytes.empty()) f->write(o.bytes.data(), o.bytes.size());
+        if (o.last) f.reset();
+        g_times.format_ns += clk.lap();
+      }
+    } catch (const std::exception &e) {
+      fail(e);
+    }
+  });
+
+  // reader (this thread)
+  try {
+    for (size_t fi = 0; fi < files.size(); ++fi) {
+      const u64 f = files[fi];
+      sbwt_host::SeqReader reader(in[f]);
+      InBatch b;
+      b.file = f;
+      b.first = true;
+      Clock clk;
+      bool ok = true;
+      auto flush_batch = [&](bool last) {
+        b.last = last;
+        g_times.parse_ns += clk.lap();
+        const bool pushed = to_device.push(std::move(b));
+        b = InBatch();
+        b.file = f;
+        return pushed;
+      };
+      std::vector<char> read;
+      while (ok) {
+        read.clear();
+        if (!reader.next(read)) break;
+        // a read longer than the room left is cut into pieces overlapping by
+        // k-1 chars (the carry-over of ContinuousSequenceFileParser.cpp:82-112)
+        u64 st = 0;
+        while (ok) {
+          const u64 room = max_chars - b.chars.size();
+          if (room < k && !b.ends.empty()) {
+            ok = flush_batch(false);
+            continue;
+          }
+          const u64 take = std::min<u64>(read.size() - st, room);
+          b.chars.insert(b.chars.end(), read.begin() + st, read.begin() + st + take);
+          b.ends.push_back(b.chars.size());
+          const bool last = st + take == read.size();
+          b.closes.push_back(last ? 1 : 0);
+          if (last) break;
+          st += take - (k - 1);
+          ok = flush_batch(false);
+        }
+        if (ok && (b.chars.size() >= max_chars || b.ends.size() >= max_reads)) ok = flush_batch(false);
+      }
+      if (ok) ok = flush_batch(true);  // the file's last batch (possibly empty)
+      if (!ok) break;
+    }
+    to_device.close();
+  } catch (const std::exception &e) {
+    fail(e);
+  }
+  device.join();
+  writer.join();
+  if (!error.empty()) throw std::runtime_error(error);
 }
 
 int run_index(const Args &a) {
@@ -185,9 +318,11 @@ int run_index(const Args &a) {
   if (q.empty() || idx.empty() || o.empty())
     throw std::runtime_error("query-file, index-file and output-prefix are required");
   const std::string colors = a.get("colors-file");
-  const std::string mode = a.get("print-mode", "ascii");
-  if (mode != "ascii" && mode != "binary" && mode != "bool")
+  const std::string mode_name = a.get("print-mode", "ascii");
+  if (mode_name != "ascii" && mode_name != "binary" && mode_name != "bool")
     throw std::runtime_error("Invalid value passed by user for argument print_mode");
+  const int mode = mode_name == "ascii" ? SBWT_FORMAT_ASCII
+                   : mode_name == "binary" ? SBWT_FORMAT_BINARY : SBWT_FORMAT_BOOL;
   fraction(a, "cpu-memory-percentage", "0.8", "Invalid value for cpu-memory-percentage. Must be between 0 and 1.");
   const double gpu = fraction(a, "gpu-memory-percentage", "0.95",
                               "Invalid value for gpu-memory-percentage. Must be between 0 and 1.");
@@ -219,7 +354,7 @@ int run_index(const Args &a) {
   for (auto &e : errors)
     if (!e.empty()) throw std::runtime_error(e);
   if (std::getenv("SBWT_SEARCH_TIMING"))
-    std::fprintf(stderr, "stage seconds (summed over %llu streams): parse %.3f search %.3f format+write %.3f\n",
+    std::fprintf(stderr, "stage busy seconds (summed over %llu streams; stages overlap): parse %.3f search+format %.3f write %.3f\n",
                  static_cast<unsigned long long>(streams), g_times.parse_ns * 1e-9,
                  g_times.search_ns * 1e-9, g_times.format_ns * 1e-9);
   return 0;

@@ -216,3 +216,14 @@ def test_empty_presearch_interval_iff_in_no_kmer(seed, glen, k, kp, tmp_path):
         assert empty == (s not in inside), s
         n_absent += empty
     assert 0 < n_absent < 4 ** kp
+
+
+@pytest.mark.parametrize("name", INDEX_FIXTURES)
+def test_ascii_helper_matches_reference_expected(fixture_sbwt, name):
+    """The test-side ascii writer reproduces the reference's expected files from
+    the oracle's results (so GPU tests may compare formatted bytes with it)."""
+    chars, ends = O.read_seqfile(os.path.join(FIXTURES, "index_search", name))
+    res = fixture_sbwt.index_reads(chars, ends)
+    expected = open(os.path.join(FIXTURES, "index_search", "expected",
+                                 name.split(".")[0] + ".txt"), "rb").read()
+    assert O.ascii_index_output(res, ends, 30) == expected
