@@ -192,6 +192,12 @@ int sbwt_searcher_autotune(sbwt_searcher *searcher, const char *d_chars, uint64_
 int sbwt_pack_2bit_device(const char *d_chars, uint64_t n_chars, uint64_t *d_words,
                           uint32_t *d_invalid, void *hip_stream);
 
+/* Page-locked host memory (hipHostMalloc) for the host side of batch copies:
+ * copies from and to it run at full link speed. The reference keeps its host
+ * batches in PinnedVector (src/Tools/PinnedVector.h). */
+int sbwt_host_alloc(uint64_t bytes, void **out);
+void sbwt_host_free(void *ptr);
+
 /* ---- Device formatting of index results ------------------------------ */
 
 /* Formats index results into the bytes of the reference's index output files

@@ -1,6 +1,7 @@
 // extern "C" boundary of libsbwt_amd.so (declared in include/sbwt_amd.h).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <memory>
 #include <string>
 
@@ -275,6 +276,18 @@ int sbwt_pack_2bit_device(const char *d_chars, uint64_t n_chars, uint64_t *d_wor
   return guarded([&] {
     sbwt_amd::launch_pack(d_chars, n_chars, d_words, d_invalid, static_cast<hipStream_t>(hip_stream));
   });
+}
+
+int sbwt_host_alloc(uint64_t bytes, void **out) {
+  return guarded([&] {
+    require(out != nullptr, "null argument");
+    *out = nullptr;
+    SBWT_HIP(hipHostMalloc(out, std::max<uint64_t>(bytes, 1), hipHostMallocDefault));
+  });
+}
+
+void sbwt_host_free(void *ptr) {
+  if (ptr) (void)hipHostFree(ptr);
 }
 
 int sbwt_formatter_create(uint64_t max_kmers, uint64_t max_pieces, void *hip_stream,

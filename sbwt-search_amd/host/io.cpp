@@ -42,18 +42,47 @@ SeqReader::~SeqReader() {
   if (gz_) gzclose(static_cast<gzFile>(gz_));
 }
 
-int SeqReader::getc() {
-  if (pos_ == len_) {
-    if (eof_) return -1;
-    const int n = gzread(static_cast<gzFile>(gz_), buf_.data(), static_cast<unsigned>(buf_.size()));
-    if (n <= 0) {
-      eof_ = true;
-      return -1;
-    }
-    len_ = static_cast<size_t>(n);
-    pos_ = 0;
+bool SeqReader::fill() {
+  if (pos_ < len_) return true;
+  if (eof_) return false;
+  const int n = gzread(static_cast<gzFile>(gz_), buf_.data(), static_cast<unsigned>(buf_.size()));
+  if (n <= 0) {
+    eof_ = true;
+    return false;
   }
+  len_ = static_cast<size_t>(n);
+  pos_ = 0;
+  return true;
+}
+
+int SeqReader::getc() {
+  if (!fill()) return -1;
   return static_cast<unsigned char>(buf_[pos_++]);
+}
+
+int SeqReader::peek() {
+  if (!fill()) return -1;
+  return static_cast<unsigned char>(buf_[pos_]);
+}
+
+// Consumes the rest of the line and its '\n'; appends its chars to `out`
+// when not null. Returns the number of chars before the '\n'.
+u64 SeqReader::take_line(std::vector<char> *out) {
+  u64 n = 0;
+  while (fill()) {
+    const char *p = buf_.data() + pos_;
+    const size_t avail = len_ - pos_;
+    const void *nl = std::memchr(p, '\n', avail);
+    const size_t m = nl ? static_cast<size_t>(static_cast<const char *>(nl) - p) : avail;
+    if (out) out->insert(out->end(), p, p + m);
+    n += m;
+    pos_ += m;
+    if (nl) {
+      ++pos_;  // the '\n'
+      break;
+    }
+  }
+  return n;
 }
 
 bool SeqReader::next(std::vector<char> &chars) {
@@ -66,38 +95,26 @@ bool SeqReader::next(std::vector<char> &chars) {
   } else if (pending_ == -1) {
     return false;
   }
-  while ((c = getc()) != -1 && c != '\n') {  // header line
-  }
-  u64 seq_len = 0;
   pending_ = -1;
-  if (c == -1) return true;
+  take_line(nullptr);  // header line
+  u64 seq_len = 0;
   for (;;) {
-    c = getc();
+    c = peek();
     if (c == -1) break;
     if (c == '>' || c == '@') {
-      pending_ = c;
+      pending_ = getc();
       break;
     }
-    if (c == '+') {  // FASTQ quality
-      while ((c = getc()) != -1 && c != '\n') {
-      }
+    if (c == '+') {  // FASTQ quality: lines until as long as the sequence
+      take_line(nullptr);
       u64 q = 0;
-      while (c != -1 && q < seq_len) {
-        while ((c = getc()) != -1 && c != '\n') ++q;
-      }
+      while (q < seq_len && peek() != -1) q += take_line(nullptr);
       while ((c = getc()) != -1 && c != '>' && c != '@') {
       }
       pending_ = c;
       break;
     }
-    if (c == '\n') continue;
-    chars.push_back(static_cast<char>(c));
-    ++seq_len;
-    while ((c = getc()) != -1 && c != '\n') {
-      chars.push_back(static_cast<char>(c));
-      ++seq_len;
-    }
-    if (c == -1) break;
+    seq_len += take_line(&chars);
   }
   return true;
 }

@@ -27,7 +27,10 @@ class SeqReader {
   bool next(std::vector<char> &chars);
 
  private:
+  bool fill();
   int getc();
+  int peek();
+  u64 take_line(std::vector<char> *out);
   void *gz_ = nullptr;
   std::vector<char> buf_;
   size_t pos_ = 0, len_ = 0;

@@ -100,7 +100,7 @@ u64 chars_per_batch(double gpu_fraction, u64 streams) {
   u64 free_b = 0, total = 0;
   check(sbwt_mem_info(&free_b, &total));
   const u64 budget = static_cast<u64>(static_cast<double>(free_b) * gpu_fraction) / std::max<u64>(streams, 1);
-  const u64 n = std::min<u64>(budget / 34, 1ULL << 28);
+  const u64 n = std::min<u64>(budget / 34, 1ULL << 27);
   if (n == 0) throw std::runtime_error("Not enough memory");
   return n;
 }
@@ -156,6 +156,11 @@ class Channel {
     cv_pop_.notify_all();
     cv_push_.notify_all();
   }
+  // Items left in the channel (after the stages have stopped).
+  std::deque<T> drain() {
+    std::lock_guard<std::mutex> g(mu_);
+    return std::move(q_);
+  }
 
  private:
   size_t cap_;
@@ -165,12 +170,32 @@ class Channel {
   bool closed_ = false, aborted_ = false;
 };
 
+// Page-locked host buffer (copies to and from the device at link speed).
+struct Pinned {
+  char *p = nullptr;
+  u64 cap = 0;
+  void ensure(u64 n) {
+    if (n <= cap) return;
+    release();
+    void *q = nullptr;
+    check(sbwt_host_alloc(n, &q));
+    p = static_cast<char *>(q);
+    cap = n;
+  }
+  void release() {
+    sbwt_host_free(p);
+    p = nullptr;
+    cap = 0;
+  }
+};
+
 // One device batch: whole reads, or pieces of a long read (closes[e] == 0:
 // the next entry continues the same read, and so the same output line).
 struct InBatch {
   u64 file = 0;
   bool first = false, last = false;  // first / last batch of its file
-  std::vector<char> chars;
+  Pinned chars;
+  u64 n_chars = 0;
   std::vector<u64> ends;
   std::vector<u8> closes;
 };
@@ -178,7 +203,8 @@ struct InBatch {
 struct OutChunk {
   u64 file = 0;
   bool first = false, last = false;
-  std::vector<char> bytes;
+  Pinned bytes;
+  u64 n = 0;
 };
 
 // One stream of the index command, three stages on three threads: the reader
@@ -189,8 +215,16 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
                   const std::vector<std::string> &out, const std::vector<u64> &files, int mode,
                   bool headers, u64 max_chars) {
   const u64 max_reads = std::max<u64>(max_chars / 16, 1024);
-  Channel<InBatch> to_device(2);
-  Channel<OutChunk> to_writer(2);
+  constexpr size_t kBuffers = 3;  // per direction: one filling, one in the queue, one in use
+  Channel<InBatch> to_device(1);
+  Channel<OutChunk> to_writer(1);
+  Channel<Pinned> free_in(kBuffers), free_out(kBuffers);
+  for (size_t i = 0; i < kBuffers; ++i) {
+    Pinned in_buf;
+    in_buf.ensure(max_chars + 64);
+    free_in.push(in_buf);
+    free_out.push(Pinned());
+  }
   std::string error;
   std::mutex error_mu;
   auto fail = [&](const std::exception &e) {
@@ -200,6 +234,8 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
     }
     to_device.abort();
     to_writer.abort();
+    free_in.abort();
+    free_out.abort();
   };
 
   std::thread device([&] {
@@ -209,7 +245,6 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
       check(sbwt_searcher_create(index, max_chars, max_reads, move_to_key, nullptr, &s));
       check(sbwt_formatter_create(max_chars, max_reads, nullptr, &fmt));
       bool line_open = false;
-      std::vector<char> spare;
       InBatch b;
       while (to_device.pop(b)) {
         Clock clk;
@@ -217,23 +252,21 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
         o.file = b.file;
         o.first = b.first;
         o.last = b.last;
+        if (!free_out.pop(o.bytes)) break;
         if (b.first) line_open = false;
         if (!b.ends.empty()) {
-          o.bytes.swap(spare);
-          o.bytes.resize(o.bytes.capacity());
-          u64 n = 0;
-          check(sbwt_searcher_search_reads_format(s, fmt, mode, b.chars.data(), b.chars.size(),
+          check(sbwt_searcher_search_reads_format(s, fmt, mode, b.chars.p, b.n_chars,
                                                   b.ends.data(), b.closes.data(), b.ends.size(),
-                                                  line_open, o.bytes.data(), o.bytes.size(), &n));
-          if (n > o.bytes.size()) {
-            o.bytes.resize(n + n / 4);
-            check(sbwt_formatter_copy_last(fmt, o.bytes.data(), o.bytes.size()));
+                                                  line_open, o.bytes.p, o.bytes.cap, &o.n));
+          if (o.n > o.bytes.cap) {
+            o.bytes.ensure(o.n + o.n / 4);
+            check(sbwt_formatter_copy_last(fmt, o.bytes.p, o.bytes.cap));
           }
-          o.bytes.resize(n);
           line_open = b.closes.back() == 0;
         }
+        if (!free_in.push(b.chars)) break;
         g_times.search_ns += clk.lap();
-        if (!to_writer.push(std::move(o))) break;
+        if (!to_writer.push(o)) break;
       }
       to_writer.close();
     } catch (const std::exception &e) {
@@ -250,9 +283,10 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
       while (to_writer.pop(o)) {
         Clock clk;
         if (o.first) f = std::make_unique<sbwt_host::IndexFileOut>(out[o.file], mode, headers);
-        if (!o.bytes.empty()) f->write(o.bytes.data(), o.bytes.size());
+        if (o.n) f->write(o.bytes.p, o.n);
         if (o.last) f.reset();
         g_times.format_ns += clk.lap();
+        if (!free_out.push(o.bytes)) break;
       }
     } catch (const std::exception &e) {
       fail(e);
@@ -261,23 +295,25 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
 
   // reader (this thread)
   try {
+    std::vector<char> read;
     for (size_t fi = 0; fi < files.size(); ++fi) {
       const u64 f = files[fi];
       sbwt_host::SeqReader reader(in[f]);
       InBatch b;
+      bool ok = free_in.pop(b.chars);
       b.file = f;
       b.first = true;
       Clock clk;
-      bool ok = true;
+      // hands the batch to the device stage; a batch that is not the file's
+      // last one is followed by a fresh buffer from the pool
       auto flush_batch = [&](bool last) {
         b.last = last;
         g_times.parse_ns += clk.lap();
-        const bool pushed = to_device.push(std::move(b));
+        if (!to_device.push(std::move(b))) return false;
         b = InBatch();
         b.file = f;
-        return pushed;
+        return last || free_in.pop(b.chars);
       };
-      std::vector<char> read;
       while (ok) {
         read.clear();
         if (!reader.next(read)) break;
@@ -285,23 +321,25 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
         // k-1 chars (the carry-over of ContinuousSequenceFileParser.cpp:82-112)
         u64 st = 0;
         while (ok) {
-          const u64 room = max_chars - b.chars.size();
+          const u64 room = max_chars - b.n_chars;
           if (room < k && !b.ends.empty()) {
             ok = flush_batch(false);
             continue;
           }
           const u64 take = std::min<u64>(read.size() - st, room);
-          b.chars.insert(b.chars.end(), read.begin() + st, read.begin() + st + take);
-          b.ends.push_back(b.chars.size());
+          if (take) std::memcpy(b.chars.p + b.n_chars, read.data() + st, take);
+          b.n_chars += take;
+          b.ends.push_back(b.n_chars);
           const bool last = st + take == read.size();
           b.closes.push_back(last ? 1 : 0);
           if (last) break;
           st += take - (k - 1);
           ok = flush_batch(false);
         }
-        if (ok && (b.chars.size() >= max_chars || b.ends.size() >= max_reads)) ok = flush_batch(false);
+        if (ok && (b.n_chars >= max_chars || b.ends.size() >= max_reads)) ok = flush_batch(false);
       }
       if (ok) ok = flush_batch(true);  // the file's last batch (possibly empty)
+      else if (b.chars.p) b.chars.release();  // stopped by a failure elsewhere
       if (!ok) break;
     }
     to_device.close();
@@ -310,6 +348,11 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
   }
   device.join();
   writer.join();
+  // every buffer is back in its pool (or stranded in a queue after a failure)
+  for (auto &x : free_in.drain()) x.release();
+  for (auto &x : free_out.drain()) x.release();
+  for (auto &x : to_device.drain()) x.chars.release();
+  for (auto &x : to_writer.drain()) x.bytes.release();
   if (!error.empty()) throw std::runtime_error(error);
 }
 

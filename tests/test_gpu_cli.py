@@ -35,17 +35,21 @@ def test_index_cli_matches_reference_expected(tmp_path, batch):
         assert open(out + ".txt", "rb").read() == exp, name
 
 
-def test_index_cli_list_streams_and_formats(tmp_path):
-    """5 files over 3 streams via .list; binary and bool equal the oracle's files."""
+@pytest.mark.parametrize("streams,repeat", [("3", 1), ("1", 2)])
+def test_index_cli_list_streams_and_formats(tmp_path, streams, repeat):
+    """Files of a .list over several streams, or many files on one stream (its
+    batch buffers are recycled across files); every format equals the oracle's
+    files."""
     sbwt = os.path.join(FIXTURES, "search_test_index.sbwt")
     orc = O.OracleSbwt(sbwt)
-    ins = [os.path.join(FIXTURES, "index_search", n) for n in NAMES]
-    outs = [str(tmp_path / n) for n in NAMES]
+    names = NAMES * repeat
+    ins = [os.path.join(FIXTURES, "index_search", n) for n in names]
+    outs = [str(tmp_path / f"{i}_{n}") for i, n in enumerate(names)]
     (tmp_path / "in.list").write_text("\n".join(ins) + "\n")
     (tmp_path / "out.list").write_text("\n".join(outs) + "\n")
     for mode, ext, fmt in (("binary", ".bin", 1), ("bool", ".bool", 2), ("ascii", ".txt", 0)):
         run(["index", "-q", str(tmp_path / "in.list"), "-i", sbwt, "-o", str(tmp_path / "out.list"),
-             "-p", mode, "-s", "3"])
+             "-p", mode, "-s", streams, "--batch-chars", "64"])
         for i, o in zip(ins, outs):
             orc.index_file(i, o + ".orc", fmt, True)
             assert open(o + ext, "rb").read() == open(o + ".orc", "rb").read(), (mode, i)
