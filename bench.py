@@ -70,6 +70,8 @@ def parse():
     ap.add_argument("--algorithm", default="streaming", choices=["streaming", "per-kmer"])
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape instead of autotuning it on the batch")
+    ap.add_argument("--launch", default="",
+                    help="CHUNK,BLOCKS: fixed launch shape of the streaming search (no autotune)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run an elementwise copy of this many MiB (a kernel of known "
                          "HBM traffic, to scale rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -142,7 +144,11 @@ def main():
                               d_out.data_ptr())
 
     launch = None
-    if args.algorithm == "streaming" and not args.no_autotune:
+    if args.algorithm == "streaming" and args.launch:
+        ch, bpc = (int(x) for x in args.launch.split(","))
+        s.set_launch(ch, bpc)
+        launch = {"chunk_kmers": ch, "blocks_per_cu": bpc}
+    elif args.algorithm == "streaming" and not args.no_autotune:
         ch, bpc, ms = s.autotune(d_chars.data_ptr(), wl.n_chars, d_ends.data_ptr(), wl.n_reads,
                                  d_out.data_ptr())
         launch = {"chunk_kmers": ch, "blocks_per_cu": bpc}
