@@ -348,7 +348,10 @@ int sbwt_color_searcher_last_time(sbwt_color_searcher *searcher, float *ms);
  * pinned staging) as slots free up, so requests that arrive while others run
  * overlap on the device. The number of slots is min(slots, what
  * memory_fraction of the free device memory holds). Requests larger than a
- * slot run as several batches in that slot. No counterpart in the reference,
+ * slot run as several batches in that slot. Batches of up to 2^20 chars
+ * replay a HIP graph captured once per slot and power-of-two (chars, reads)
+ * bucket (staging copies, search, color stage, copies back in one launch;
+ * no_graphs = 1 launches them one by one). No counterpart in the reference,
  * which processes its input files in fixed streams (IndexSearchMain.cpp:414-448). */
 typedef struct sbwt_server sbwt_server;
 typedef struct {
@@ -358,6 +361,7 @@ typedef struct {
   double memory_fraction;
   int move_to_key_kmer;
   sbwt_color_params color;   /* used when a color index is attached */
+  int no_graphs;
 } sbwt_server_config;
 
 typedef struct {

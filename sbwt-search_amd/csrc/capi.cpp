@@ -489,6 +489,7 @@ int sbwt_server_create(sbwt_index *index, sbwt_color_index *colors,
     c.max_reads_per_slot = config->max_reads_per_slot;
     c.memory_fraction = config->memory_fraction;
     c.move_to_key_kmer = config->move_to_key_kmer != 0;
+    c.graphs = config->no_graphs == 0;
     if (colors) c.color = to_params(&config->color);
     auto s = std::make_unique<sbwt_server>();
     s->impl = std::make_unique<sbwt_amd::Server>(&index->dev, colors ? &colors->dev : nullptr, c);

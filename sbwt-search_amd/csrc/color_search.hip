@@ -167,10 +167,11 @@ __global__ __launch_bounds__(256) void k_color_reads(ColorView cv, const u64 *__
   }
 }
 
-// Expand per-read hit bitmaps into sorted u32 color ids at CSR offsets.
+// Expand per-read hit bitmaps into sorted u32 color ids at CSR offsets; ids
+// past `capacity` are not written (offsets still give the full count).
 __global__ __launch_bounds__(256) void k_color_compact(const u64 *__restrict__ bitmap,
                                                         const u64 *__restrict__ incl, u64 n_reads,
-                                                        u32 nc, u32 *__restrict__ ids,
+                                                        u32 nc, u32 *__restrict__ ids, u64 capacity,
                                                         u64 *__restrict__ offsets) {
   const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave, waves = blockDim.x / kWave;
   const u32 words = (nc + 63) / 64;
@@ -192,7 +193,8 @@ __global__ __launch_bounds__(256) void k_color_compact(const u64 *__restrict__ b
       }
       u64 at = pos + pre - pc;
       while (x) {
-        ids[at++] = w * 64 + static_cast<u32>(__builtin_ctzll(x));
+        if (at < capacity) ids[at] = w * 64 + static_cast<u32>(__builtin_ctzll(x));
+        ++at;
         x &= x - 1;
       }
       pos += __shfl(pre, kWave - 1, kWave);
@@ -373,12 +375,14 @@ u64 ColorSearcher::pseudoalign_device(const u64 *d_results, const u64 *d_kmer_en
   const u32 nc = static_cast<u32>(colors_->num_colors);
   const u32 waves = waves_per_block(nc);
   KernelParams kp{p.threshold, p.include_not_found, p.include_invalid, p.first_read_quirk};
-  SBWT_HIP(hipEventRecord(ev_[0], stream_));
+  if (timing_) SBWT_HIP(hipEventRecord(ev_[0], stream_));
   k_color_reads<false><<<grid_for(n_reads, waves), waves * kWave, waves * nc * 4, stream_>>>(
       colors_->view(), d_results, d_kmer_ends, n_reads, kp, d_bitmap_, d_nout_, d_stats, nullptr);
   SBWT_HIP(hipGetLastError());
-  SBWT_HIP(hipEventRecord(ev_[1], stream_));
-  recorded_ = true;
+  if (timing_) {
+    SBWT_HIP(hipEventRecord(ev_[1], stream_));
+    recorded_ = true;
+  }
   u64 *d_incl = d_nout_ + max_reads_;
   scan_inclusive_u64(d_nout_, n_reads, d_incl, d_scan_tmp_, stream_);
   u64 total = 0;
@@ -388,9 +392,32 @@ u64 ColorSearcher::pseudoalign_device(const u64 *d_results, const u64 *d_kmer_en
     throw std::runtime_error("color id output needs " + std::to_string(total) + " entries, capacity " +
                              std::to_string(ids_capacity));
   k_color_compact<<<grid_for(n_reads, 4), 4 * kWave, 0, stream_>>>(d_bitmap_, d_incl, n_reads, nc,
-                                                                    d_ids, d_offsets);
+                                                                    d_ids, ids_capacity, d_offsets);
   SBWT_HIP(hipGetLastError());
   return total;
+}
+
+void ColorSearcher::pseudoalign_device_async(const u64 *d_results, const u64 *d_kmer_ends,
+                                             u64 n_reads, const ColorParams &p, u64 *d_offsets,
+                                             u32 *d_ids, u64 ids_capacity, u32 *d_stats) {
+  if (n_reads == 0) return;
+  if (n_reads > max_reads_) throw std::runtime_error("batch exceeds the color searcher's max reads");
+  const u32 nc = static_cast<u32>(colors_->num_colors);
+  const u32 waves = waves_per_block(nc);
+  KernelParams kp{p.threshold, p.include_not_found, p.include_invalid, p.first_read_quirk};
+  if (timing_) SBWT_HIP(hipEventRecord(ev_[0], stream_));
+  k_color_reads<false><<<grid_for(n_reads, waves), waves * kWave, waves * nc * 4, stream_>>>(
+      colors_->view(), d_results, d_kmer_ends, n_reads, kp, d_bitmap_, d_nout_, d_stats, nullptr);
+  SBWT_HIP(hipGetLastError());
+  if (timing_) {
+    SBWT_HIP(hipEventRecord(ev_[1], stream_));
+    recorded_ = true;
+  }
+  u64 *d_incl = d_nout_ + max_reads_;
+  scan_inclusive_u64(d_nout_, n_reads, d_incl, d_scan_tmp_, stream_);
+  k_color_compact<<<grid_for(n_reads, 4), 4 * kWave, 0, stream_>>>(d_bitmap_, d_incl, n_reads, nc,
+                                                                    d_ids, ids_capacity, d_offsets);
+  SBWT_HIP(hipGetLastError());
 }
 
 void ColorSearcher::counts_device(const u64 *d_results, const u64 *d_kmer_ends, u64 n_reads,

@@ -68,11 +68,18 @@ class ColorSearcher {
   u64 pseudoalign_device(const u64 *d_results, const u64 *d_kmer_ends, u64 n_reads,
                          const ColorParams &p, u64 *d_offsets, u32 *d_ids, u64 ids_capacity,
                          u32 *d_stats);
+  // The same without the host sync: nothing checks capacity on the host (ids
+  // past it are not written) and the total is d_offsets[n_reads] once the
+  // stream gets there. Capturable into a HIP graph (with timing off).
+  void pseudoalign_device_async(const u64 *d_results, const u64 *d_kmer_ends, u64 n_reads,
+                                const ColorParams &p, u64 *d_offsets, u32 *d_ids, u64 ids_capacity,
+                                u32 *d_stats);
   // Counts only (no threshold): u32 counts[n_reads * num_colors] on device.
   void counts_device(const u64 *d_results, const u64 *d_kmer_ends, u64 n_reads, u32 *d_counts);
 
   hipStream_t stream() const { return stream_; }
   float last_search_ms();
+  void set_timing(bool on) { timing_ = on; }
 
  private:
   const DeviceColors *colors_;
@@ -86,6 +93,7 @@ class ColorSearcher {
   u64 *d_warped_ = nullptr, *d_intervals_ = nullptr, *d_counts_ = nullptr;
   hipEvent_t ev_[2];
   bool recorded_ = false;
+  bool timing_ = true;
 };
 
 }  // namespace sbwt_amd

@@ -30,9 +30,17 @@ Server::Server(const DeviceIndex *index, const DeviceColors *colors, const Serve
     SBWT_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
     s->searcher = std::make_unique<Searcher>(index_, C, R, cfg_.move_to_key_kmer, s->stream);
     s->searcher->set_timing(false);
-    if (colors_) s->colors = std::make_unique<ColorSearcher>(colors_, C, R, s->stream);
+    if (colors_) {
+      s->colors = std::make_unique<ColorSearcher>(colors_, C, R, s->stream);
+      s->colors->set_timing(false);
+    }
     SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_chars), C + 64, hipHostMallocDefault));
     SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_ends), R * 8, hipHostMallocDefault));
+    if (cfg_.graphs) {
+      const u64 gc = std::min<u64>(C, kGraphMaxChars), gr = std::min<u64>(R, kGraphMaxChars);
+      SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_results), gc * 8 + 8, hipHostMallocDefault));
+      SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_offsets), (gr + 1) * 8, hipHostMallocDefault));
+    }
     SBWT_HIP(hipMalloc(&s->d_chars, C + 64));
     SBWT_HIP(hipMalloc(&s->d_ends, R * 8));
     SBWT_HIP(hipMalloc(&s->d_results, (C + 1) * 8));
@@ -57,6 +65,7 @@ Server::~Server() {
   cv_work_.notify_all();
   for (auto &s : slots_) {
     if (s->worker.joinable()) s->worker.join();
+    for (auto &g : s->graphs) (void)hipGraphExecDestroy(g.second);
     s->colors.reset();
     s->searcher.reset();
     void *dev[] = {s->d_chars, s->d_ends, s->d_results, s->d_offsets, s->d_ids};
@@ -64,6 +73,8 @@ Server::~Server() {
       if (p) (void)hipFree(p);
     if (s->h_chars) (void)hipHostFree(s->h_chars);
     if (s->h_ends) (void)hipHostFree(s->h_ends);
+    if (s->h_results) (void)hipHostFree(s->h_results);
+    if (s->h_offsets) (void)hipHostFree(s->h_offsets);
     if (s->stream) (void)hipStreamDestroy(s->stream);
   }
 }
@@ -160,32 +171,109 @@ void Server::process(Slot &s, Request &r) {
     const u64 nr = read1 - read0, nc = r.read_ends[read1 - 1] - c0;
     std::memcpy(s.h_chars, r.chars.data() + c0, nc);
     for (u64 i = 0; i < nr; ++i) s.h_ends[i] = r.read_ends[read0 + i] - c0;
-    SBWT_HIP(hipMemcpyAsync(s.d_chars, s.h_chars, nc, hipMemcpyHostToDevice, s.stream));
-    SBWT_HIP(hipMemcpyAsync(s.d_ends, s.h_ends, nr * 8, hipMemcpyHostToDevice, s.stream));
-    u64 nk = 0;
-    s.searcher->search_reads_device(s.d_chars, nc, s.d_ends, nr, s.d_results, &nk);
-    if (nk)
-      SBWT_HIP(hipMemcpyAsync(r.kmer_results.data() + kmer_out, s.d_results, nk * 8,
-                              hipMemcpyDeviceToHost, s.stream));
-    if (colors_) {
-      const u64 n_ids = s.colors->pseudoalign_device(s.d_results, s.searcher->kmer_ends_device(), nr,
-                                                     cfg_.color, s.d_offsets, s.d_ids,
-                                                     s.ids_capacity, nullptr);
-      std::vector<u64> off(nr + 1);
-      const u64 base = r.color_ids.size();
-      r.color_ids.resize(base + n_ids);
-      SBWT_HIP(hipMemcpyAsync(off.data(), s.d_offsets, (nr + 1) * 8, hipMemcpyDeviceToHost, s.stream));
-      if (n_ids)
-        SBWT_HIP(hipMemcpyAsync(r.color_ids.data() + base, s.d_ids, n_ids * 4, hipMemcpyDeviceToHost,
-                                s.stream));
-      SBWT_HIP(hipStreamSynchronize(s.stream));
-      for (u64 i = 0; i < nr; ++i) r.color_offsets[read0 + i + 1] = base + off[i + 1];
-    } else {
-      SBWT_HIP(hipStreamSynchronize(s.stream));
-    }
-    kmer_out += nk;
+    // power-of-two bucket of the batch: small batches replay a captured graph
+    u64 cb = kGraphMinChars, rb = kGraphMinReads;
+    while (cb < nc) cb <<= 1;
+    while (rb < nr) rb <<= 1;
+    const bool graph = cfg_.graphs && cb <= kGraphMaxChars && cb <= cfg_.max_chars_per_slot &&
+                       rb <= cfg_.max_reads_per_slot && rb <= kGraphMaxChars;
+    kmer_out += graph ? run_batch_graph(s, r, nc, nr, read0, kmer_out, cb, rb)
+                      : run_batch(s, r, nc, nr, read0, kmer_out);
     read0 = read1;
   }
+}
+
+u64 Server::run_batch(Slot &s, Request &r, u64 nc, u64 nr, u64 read0, u64 kmer_out) {
+  SBWT_HIP(hipMemcpyAsync(s.d_chars, s.h_chars, nc, hipMemcpyHostToDevice, s.stream));
+  SBWT_HIP(hipMemcpyAsync(s.d_ends, s.h_ends, nr * 8, hipMemcpyHostToDevice, s.stream));
+  u64 nk = 0;
+  s.searcher->search_reads_device(s.d_chars, nc, s.d_ends, nr, s.d_results, &nk);
+  if (nk)
+    SBWT_HIP(hipMemcpyAsync(r.kmer_results.data() + kmer_out, s.d_results, nk * 8,
+                            hipMemcpyDeviceToHost, s.stream));
+  if (colors_) {
+    const u64 n_ids = s.colors->pseudoalign_device(s.d_results, s.searcher->kmer_ends_device(), nr,
+                                                   cfg_.color, s.d_offsets, s.d_ids,
+                                                   s.ids_capacity, nullptr);
+    std::vector<u64> off(nr + 1);
+    const u64 base = r.color_ids.size();
+    r.color_ids.resize(base + n_ids);
+    SBWT_HIP(hipMemcpyAsync(off.data(), s.d_offsets, (nr + 1) * 8, hipMemcpyDeviceToHost, s.stream));
+    if (n_ids)
+      SBWT_HIP(hipMemcpyAsync(r.color_ids.data() + base, s.d_ids, n_ids * 4, hipMemcpyDeviceToHost,
+                              s.stream));
+    SBWT_HIP(hipStreamSynchronize(s.stream));
+    for (u64 i = 0; i < nr; ++i) r.color_offsets[read0 + i + 1] = base + off[i + 1];
+  } else {
+    SBWT_HIP(hipStreamSynchronize(s.stream));
+  }
+  return nk;
+}
+
+// The batch is padded to its bucket with empty reads (they have no k-mers and
+// no color ids); chars past nc in the staging buffer belong to no read.
+u64 Server::run_batch_graph(Slot &s, Request &r, u64 nc, u64 nr, u64 read0, u64 kmer_out, u64 cb,
+                            u64 rb) {
+  for (u64 i = nr; i < rb; ++i) s.h_ends[i] = nc;
+  u64 nk = 0;
+  const u64 k = index_->k;
+  for (u64 i = 0, st = 0; i < nr; ++i) {
+    const u64 len = s.h_ends[i] - st;
+    nk += len >= k ? len - k + 1 : 0;
+    st = s.h_ends[i];
+  }
+  SBWT_HIP(hipGraphLaunch(graph_for(s, cb, rb), s.stream));
+  SBWT_HIP(hipStreamSynchronize(s.stream));
+  if (nk) std::memcpy(r.kmer_results.data() + kmer_out, s.h_results, nk * 8);
+  if (colors_) {
+    const u64 n_ids = s.h_offsets[nr];
+    if (n_ids > s.ids_capacity)
+      throw std::runtime_error("color id output needs " + std::to_string(n_ids) +
+                               " entries, capacity " + std::to_string(s.ids_capacity));
+    const u64 base = r.color_ids.size();
+    r.color_ids.resize(base + n_ids);
+    if (n_ids) {
+      SBWT_HIP(hipMemcpyAsync(r.color_ids.data() + base, s.d_ids, n_ids * 4, hipMemcpyDeviceToHost,
+                              s.stream));
+      SBWT_HIP(hipStreamSynchronize(s.stream));
+    }
+    for (u64 i = 0; i < nr; ++i) r.color_offsets[read0 + i + 1] = base + s.h_offsets[i + 1];
+  }
+  return nk;
+}
+
+// Captures, once per bucket, the staging copies, the search, the color stage
+// and the copies back on the slot's stream (timing and census are off in the
+// slots, so only kernels and copies are recorded).
+hipGraphExec_t Server::graph_for(Slot &s, u64 cb, u64 rb) {
+  const auto key = std::make_pair(cb, rb);
+  auto it = s.graphs.find(key);
+  if (it != s.graphs.end()) return it->second;
+  hipGraph_t graph = nullptr;
+  SBWT_HIP(hipStreamBeginCapture(s.stream, hipStreamCaptureModeThreadLocal));
+  try {
+    SBWT_HIP(hipMemcpyAsync(s.d_chars, s.h_chars, cb, hipMemcpyHostToDevice, s.stream));
+    SBWT_HIP(hipMemcpyAsync(s.d_ends, s.h_ends, rb * 8, hipMemcpyHostToDevice, s.stream));
+    s.searcher->search_reads_device(s.d_chars, cb, s.d_ends, rb, s.d_results, nullptr);
+    SBWT_HIP(hipMemcpyAsync(s.h_results, s.d_results, cb * 8, hipMemcpyDeviceToHost, s.stream));
+    if (colors_) {
+      s.colors->pseudoalign_device_async(s.d_results, s.searcher->kmer_ends_device(), rb, cfg_.color,
+                                         s.d_offsets, s.d_ids, s.ids_capacity, nullptr);
+      SBWT_HIP(hipMemcpyAsync(s.h_offsets, s.d_offsets, (rb + 1) * 8, hipMemcpyDeviceToHost,
+                              s.stream));
+    }
+  } catch (...) {
+    (void)hipStreamEndCapture(s.stream, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  SBWT_HIP(hipStreamEndCapture(s.stream, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  SBWT_HIP(e);
+  s.graphs[key] = exec;
+  return exec;
 }
 
 }  // namespace sbwt_amd

@@ -13,6 +13,7 @@
 #include <mutex>
 #include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "color_search.hpp"
@@ -27,7 +28,14 @@ struct ServerConfig {
   double memory_fraction = 0.9;     // of free device memory the slots may take
   bool move_to_key_kmer = false;
   ColorParams color;
+  // Batches of up to kGraphMaxChars chars replay a HIP graph captured once per
+  // (chars, reads) power-of-two bucket and slot: copies in, search, color
+  // stage and copies out in one launch.
+  bool graphs = true;
 };
+
+constexpr u64 kGraphMaxChars = 1 << 20;
+constexpr u64 kGraphMinChars = 1 << 12, kGraphMinReads = 1 << 6;
 
 struct Request {
   u64 id = 0;
@@ -66,10 +74,20 @@ class Server {
     u64 *d_offsets = nullptr;
     u32 *d_ids = nullptr;
     u64 ids_capacity = 0;
+    u64 *h_results = nullptr;  // pinned, graph batches
+    u64 *h_offsets = nullptr;
+    std::map<std::pair<u64, u64>, hipGraphExec_t> graphs;  // by (chars, reads) bucket
     std::thread worker;
   };
   void run_slot(Slot &s);
   void process(Slot &s, Request &r);
+  // One batch of whole reads staged in s.h_chars / s.h_ends (nc chars, nr
+  // reads, the request's reads from read0); its results go to the request at
+  // k-mer kmer_out (and its color lists). Returns the batch's k-mers.
+  u64 run_batch(Slot &s, Request &r, u64 nc, u64 nr, u64 read0, u64 kmer_out);
+  u64 run_batch_graph(Slot &s, Request &r, u64 nc, u64 nr, u64 read0, u64 kmer_out, u64 cb,
+                      u64 rb);
+  hipGraphExec_t graph_for(Slot &s, u64 cb, u64 rb);
   double now() const;
 
   const DeviceIndex *index_;

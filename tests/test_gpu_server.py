@@ -67,3 +67,30 @@ def test_malformed_request_does_not_disturb_others(served):
     alone = idx.searcher(200_000 * 4, 50_000, move_to_key_kmer=True)
     for rid, (chars, ends) in zip(ids, reqs[:3] + reqs[4:]):
         assert np.array_equal(srv.wait(rid)["kmers"], alone.search_reads(chars, ends))
+
+
+def test_graph_replays_equal_eager(served):
+    """Small batches replay a HIP graph captured once per slot and size bucket.
+    The same requests served several times (so every graph is replayed, with
+    direct searches on the device in between) equal a server without graphs."""
+    ci, idx, col = served
+    say("serving: graph replays vs eager")
+    rng = np.random.default_rng(9)
+    reqs = []
+    for i in range(12):
+        n_reads = int(rng.integers(1, 400))
+        reqs.append(ci.reads(n_reads, rng.integers(20, 251, n_reads), seed=9000 + i))
+    params = S.color_params(0.7)
+    kw = dict(slots=2, max_chars_per_slot=1 << 20, max_reads_per_slot=1 << 14,
+              move_to_key_kmer=True, params=params)
+    srv_g = S.Server(idx, col, graphs=True, **kw)
+    srv_e = S.Server(idx, col, graphs=False, **kw)
+    other = idx.searcher(1 << 20, 1 << 14, move_to_key_kmer=True)
+    for rep in range(3):
+        for chars, ends in reqs:
+            g = srv_g.wait(srv_g.submit(chars, ends))
+            e = srv_e.wait(srv_e.submit(chars, ends))
+            assert np.array_equal(g["kmers"], e["kmers"]), rep
+            assert g["color_lists"] == e["color_lists"], rep
+            assert np.array_equal(g["kmers"], ci.index_results(chars, ends))
+            other.search_reads(chars, ends)  # eager device work between replays
