@@ -57,7 +57,8 @@ def parse():
     ap.add_argument("--serve-roots", type=int, default=10)
     ap.add_argument("--serve-length", type=int, default=5_000_000)
     ap.add_argument("--serve-requests", type=int, default=200)
-    ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--serve-rates", default="50,1000",
+                    help="Poisson arrival rates (requests/s) to run, comma separated")
     ap.add_argument("--serve-max-reads", type=int, default=100_000)
     ap.add_argument("--reads", type=int, default=0, help="override the number of reads")
     ap.add_argument("--cpu-sample-reads", type=int, default=1_500_000,
@@ -341,15 +342,17 @@ def mode_serve(args, configs, S, torch, W):
         c, e, _ = W.reads(seed, codes, ends, lens, random_frac=0.0, err_rate=0.01)
         return c.tobytes(), e
 
-    r = modes.run_serve(S, idx, cidx, reads, args.serve_rate, args.serve_requests,
-                        args.serve_max_reads, 5, S.color_params(0.7), log=log)
+    runs = []
+    for rate in (float(x) for x in args.serve_rates.split(",")):
+        runs.append(modes.run_serve(S, idx, cidx, reads, rate, args.serve_requests,
+                                    args.serve_max_reads, 5, S.color_params(0.7), log=log))
     print(json.dumps({"metric": "p50/p99 request latency + reads/s, served vs plain",
                       "mode": "serve", "config": {
                           "index": f"{args.serve_genomes} genomes x {args.serve_length} bp from "
                                    f"{args.serve_roots} roots (3% substitutions), "
                                    f"{built.n} nodes, {args.serve_genomes} colors",
                           "requests": "Poisson, N log-uniform in [1, max] reads of U[75,250] bp"},
-                      **r}), flush=True)
+                      "runs": runs}), flush=True)
     return 0
 
 

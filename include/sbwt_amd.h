@@ -160,13 +160,17 @@ int sbwt_searcher_last_timings(sbwt_searcher *searcher, float *pack_ms, float *o
 int sbwt_searcher_set_stats(sbwt_searcher *searcher, int enable);
 int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[16]);
 
-/* Search algorithm of the reads path: SBWT_ALGO_STREAMING (default) steps each
- * k-mer of a read from the previous k-mer's node and searches from the
- * presearch table only after a miss; SBWT_ALGO_PER_KMER searches every k-mer
- * independently, as the reference's d_search does (IndexSearcher.cuh:22-72).
- * Both return identical results. No counterpart in the reference. */
+/* Search algorithm of the reads path: SBWT_ALGO_STREAMING steps each k-mer of
+ * a read from the previous k-mer's node and searches from the presearch table
+ * only after a miss (throughput: one line per found k-mer); SBWT_ALGO_PER_KMER
+ * searches every k-mer independently, as the reference's d_search does
+ * (IndexSearcher.cuh:22-72) (latency: a small batch takes one search's
+ * time); SBWT_ALGO_AUTO (default) takes streaming for batches of at least
+ * 2^21 chars and per k-mer below. All return identical results. No
+ * counterpart in the reference. */
 #define SBWT_ALGO_STREAMING 0
 #define SBWT_ALGO_PER_KMER 1
+#define SBWT_ALGO_AUTO 2
 int sbwt_searcher_set_algorithm(sbwt_searcher *searcher, int algorithm);
 
 /* Launch shape of the streaming search: k-mers per work item (chunk_kmers in

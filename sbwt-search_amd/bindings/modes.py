@@ -115,7 +115,9 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print):
 
 def run_serve(S, idx, col, wl_reads, rate_per_s, n_requests, max_reads, seed, params,
               slots=4, log=print):
-    """wl_reads(seed, n_reads) -> (chars, ends). Poisson arrivals at rate_per_s."""
+    """wl_reads(seed, n_reads) -> (chars, ends). Poisson arrivals at rate_per_s.
+    The same request stream goes through the server with HIP graphs, the server
+    without them, and plain sequential processing on one searcher."""
     import threading
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(0, np.log(max_reads), n_requests)).astype(np.int64)
@@ -126,40 +128,46 @@ def run_serve(S, idx, col, wl_reads, rate_per_s, n_requests, max_reads, seed, pa
     max_chars = max(len(c) for c, _ in reqs)
     out = {}
 
-    # served: submit on the arrival schedule, slots refill as they free up
-    srv = S.Server(idx, col, slots=slots, max_chars_per_slot=max(max_chars, 1 << 20),
-                   max_reads_per_slot=max(max_reads, 1), move_to_key_kmer=col is not None,
-                   params=params)
-    lat = [None] * n_requests
-    ids = [None] * n_requests
-    done = threading.Event()
+    def served(graphs):
+        # submit on the arrival schedule; slots refill as they free up
+        srv = S.Server(idx, col, slots=slots, max_chars_per_slot=max(max_chars, 1 << 20),
+                       max_reads_per_slot=max(max_reads, 1), move_to_key_kmer=col is not None,
+                       params=params, graphs=graphs)
+        lat = [None] * n_requests
+        ids = [None] * n_requests
+        done = threading.Event()
 
-    def collector():
-        for i in range(n_requests):
-            while ids[i] is None:
-                time.sleep(0.0005)
-            r = srv.wait(ids[i])
-            lat[i] = r["end_s"] - r["submit_s"]
-        done.set()
+        def collector():
+            for i in range(n_requests):
+                while ids[i] is None:
+                    time.sleep(0.0005)
+                r = srv.wait(ids[i])
+                lat[i] = r["end_s"] - r["submit_s"]
+            done.set()
 
-    th = threading.Thread(target=collector)
-    th.start()
-    t0 = time.perf_counter()
-    arrive = t0
-    for i, (c, e) in enumerate(reqs):
-        arrive += gaps[i]
-        while time.perf_counter() < arrive:
-            time.sleep(0.0002)
-        ids[i] = srv.submit(c, e)
-    done.wait()
-    th.join()
-    t_served = time.perf_counter() - t0
-    out["served"] = {"slots": srv.slots, "peak_in_flight": srv.peak_in_flight(),
-                     "p50_ms": 1000 * float(np.percentile(lat, 50)),
-                     "p99_ms": 1000 * float(np.percentile(lat, 99)),
-                     "reads_per_s": total_reads / t_served, "wall_s": t_served}
-    srv.close()
+        th = threading.Thread(target=collector)
+        th.start()
+        t0 = time.perf_counter()
+        arrive = t0
+        for i, (c, e) in enumerate(reqs):
+            arrive += gaps[i]
+            while time.perf_counter() < arrive:
+                time.sleep(0.0002)
+            ids[i] = srv.submit(c, e)
+        done.wait()
+        th.join()
+        wall = time.perf_counter() - t0
+        r = {"slots": srv.slots, "peak_in_flight": srv.peak_in_flight(),
+             "p50_ms": 1000 * float(np.percentile(lat, 50)),
+             "p99_ms": 1000 * float(np.percentile(lat, 99)),
+             "reads_per_s": total_reads / wall, "wall_s": wall}
+        srv.close()
+        return r
+
+    out["served"] = served(True)
     log(f"served: {out['served']}")
+    out["served_no_graphs"] = served(False)
+    log(f"served without graphs: {out['served_no_graphs']}")
 
     # plain: one request at a time on one searcher, same arrival schedule
     s = idx.searcher(max(max_chars, 1 << 20), max(max_reads, 1), move_to_key_kmer=col is not None)

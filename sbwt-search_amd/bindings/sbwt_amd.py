@@ -16,6 +16,7 @@ HEADER = os.path.join(REPO, "include", "sbwt_amd.h")
 
 ALGO_STREAMING = 0
 ALGO_PER_KMER = 1
+ALGO_AUTO = 2  # default: streaming from 2^21 chars per batch, per k-mer below
 NOT_FOUND = 0xFFFFFFFFFFFFFFFF
 INVALID = 0xFFFFFFFFFFFFFFFE
 WALK_STUCK = 0xFFFFFFFFFFFFFFFC

@@ -229,7 +229,9 @@ int sbwt_searcher_set_stats(sbwt_searcher *searcher, int enable) {
 int sbwt_searcher_set_algorithm(sbwt_searcher *searcher, int algorithm) {
   return guarded([&] {
     require(searcher, "null argument");
-    require(algorithm == SBWT_ALGO_STREAMING || algorithm == SBWT_ALGO_PER_KMER, "unknown algorithm");
+    require(algorithm == SBWT_ALGO_STREAMING || algorithm == SBWT_ALGO_PER_KMER ||
+                algorithm == SBWT_ALGO_AUTO,
+            "unknown algorithm");
     searcher->impl->set_algorithm(algorithm);
   });
 }

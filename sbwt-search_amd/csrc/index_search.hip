@@ -841,7 +841,7 @@ LaunchShape Searcher::autotune(const char *d_chars, u64 n_chars, const u64 *d_re
   const bool timing = timing_;
   const int algorithm = algorithm_;
   timing_ = true;
-  algorithm_ = kAlgoStreaming;
+  algorithm_ = kAlgoStreaming;  // the shape only affects the streaming search
   LaunchShape best{chunk_kmers_, blocks_per_cu_, 0.0f};
   bool have = false;
   for (u32 ch : kChunks) {
@@ -962,7 +962,9 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
   SBWT_HIP(hipGetLastError());
   if (timing_) SBWT_HIP(hipEventRecord(ev_[2], stream_));
   if (stats_) SBWT_HIP(hipMemsetAsync(d_stats_, 0, kStatsWords * 8, stream_));
-  if (algorithm_ == kAlgoStreaming) {
+  const bool streaming = algorithm_ == kAlgoStreaming ||
+                         (algorithm_ == kAlgoAuto && n_chars >= kAutoStreamingChars);
+  if (streaming) {
     // chunk table of (read, k-mer range) work items, then the persistent search
     u64 *d_chunk_ends = d_chunk_counts_ + max_reads_;
     k_chunk_count<<<grid_for(n_reads, 256), 256, 0, stream_>>>(d_kmer_ends_, d_read_ends, n_reads,
@@ -973,8 +975,13 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
     k_reset_counter<<<1, 1, 0, stream_>>>(d_counter_);
     SBWT_HIP(hipGetLastError());
     if (timing_) SBWT_HIP(hipEventRecord(ev_[2], stream_));
+    // no more workgroups than there can be chunks for (a bound from the host:
+    // reads plus chars per chunk)
+    const u64 chunk_bound = n_reads + n_chars / chunk_kmers_ + 1;
+    const u32 blocks = static_cast<u32>(
+        std::min<u64>(persistent_blocks_, (chunk_bound + kSearchThreads - 1) / kSearchThreads));
     dispatch_search3(index_->wide, move_to_key_, stats_, [&]<typename Idx, bool KEY, bool STATS>() {
-      k_search_stream<Idx, KEY, STATS><<<persistent_blocks_, kSearchThreads, 0, stream_>>>(
+      k_search_stream<Idx, KEY, STATS><<<blocks, kSearchThreads, 0, stream_>>>(
           ix, d_packed_, d_invalid_, d_chunks_, d_chunk_ends, n_reads, d_counter_, d_results,
           d_stats_, absence_checks(*index_) ? 1u : 0u);
     });

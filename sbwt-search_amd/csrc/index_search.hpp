@@ -12,6 +12,12 @@ namespace sbwt_amd {
 // independent search per k-mer (the reference's per-thread search).
 constexpr int kAlgoStreaming = 0;
 constexpr int kAlgoPerKmer = 1;
+// Auto: streaming for batches of at least kAutoStreamingChars chars (its
+// throughput needs many reads in flight), per k-mer below (every k-mer in
+// parallel: a small batch finishes in one search's latency instead of its
+// longest read's chain of steps).
+constexpr int kAlgoAuto = 2;
+constexpr u64 kAutoStreamingChars = 1ULL << 21;
 
 struct SearchTimings {
   float pack_ms = 0, offsets_ms = 0, search_ms = 0;
@@ -83,7 +89,7 @@ class Searcher {
   bool move_to_key_;
   bool timing_ = true;
   bool stats_ = false;
-  int algorithm_ = kAlgoStreaming;
+  int algorithm_ = kAlgoAuto;
   u64 *d_stats_ = nullptr;
   u64 max_chunks_ = 0;
   ulonglong2 *d_chunks_ = nullptr;

@@ -167,6 +167,7 @@ def test_long_reads_span_many_chunks(tmp_path):
     chars, rends, _ = W.reads(6, codes, ends, lens, 0.05, 0.01, 0.0005)
     chars = chars.tobytes()
     s = idx.searcher(len(chars), len(rends))
+    s.set_algorithm(S.ALGO_STREAMING)
     s.set_stats(True)
     res = s.search_reads(chars, rends)
     st = s.last_stats()
@@ -194,6 +195,7 @@ def test_streaming_without_suffix_group_starts(tmp_path):
     for sgs in (None, b.bits(4)):
         idx = S.Index.create([b.bits(c) for c in range(4)], b.n, 31, suffix_group_starts=sgs)
         s = idx.searcher(len(chars), len(rends))
+        s.set_algorithm(S.ALGO_STREAMING)
         s.set_stats(True)
         assert np.array_equal(s.search_reads(chars, rends), exp)
         st = s.last_stats()
@@ -206,6 +208,7 @@ def test_streaming_without_suffix_group_starts(tmp_path):
             # before the node's own line
             gchars = W.ACGT[codes].tobytes()
             s2 = idx.searcher(len(gchars), len(ends))
+            s2.set_algorithm(S.ALGO_STREAMING)
             s2.set_stats(True)
             assert np.array_equal(s2.search_reads(gchars, ends), orc.index_reads(gchars, ends))
             assert s2.last_stats()["stream_extra_lines"] > 0
@@ -312,6 +315,7 @@ def test_absence_checks_settle_windows(genome_index):
     assert (1 << (2 * idx.presearch_letters)) >= 8 * idx.num_nodes
     chars, ends = _reads(g, 3000, 150, seed=77, random_frac=0.3, err=0.02, n_rate=0.002)
     s = idx.searcher(len(chars), len(ends))
+    s.set_algorithm(S.ALGO_STREAMING)
     s.set_stats(True)
     res = s.search_reads(chars, ends)
     st = s.last_stats()
@@ -330,6 +334,7 @@ def test_launch_shapes_give_identical_results(genome_index, chunk, bpc):
     chars, ends, _ = W.reads(61, g, np.array([len(g)], np.uint64), lens, 0.2, 0.01, 0.001)
     chars = chars.tobytes()
     s = idx.searcher(len(chars), len(ends))
+    s.set_algorithm(S.ALGO_STREAMING)
     s.set_launch(chunk, bpc)
     assert s.launch() == (chunk, bpc)
     assert np.array_equal(s.search_reads(chars, ends), orc.index_reads(chars, ends))
@@ -359,3 +364,20 @@ def test_autotune_keeps_results(genome_index):
     exp = orc.index_reads(chars, ends)
     assert np.array_equal(dr[:len(exp)].cpu().numpy().view(np.uint64), exp)
     assert np.array_equal(s.search_reads(chars, ends), exp)
+
+
+def test_auto_algorithm_by_batch_size(genome_index):
+    """The default picks per-k-mer search for small batches and streaming for
+    batches of 2^21 chars and more; results are the same."""
+    g, b, path, orc = genome_index
+    say("auto algorithm")
+    idx = S.Index.load(path)
+    small_c, small_e = _reads(g, 1000, 150, seed=71)
+    big_c, big_e = _reads(g, 15_000, 150, seed=72)
+    assert len(small_c) < (1 << 21) <= len(big_c)
+    s = idx.searcher(len(big_c), len(big_e))
+    s.set_stats(True)
+    assert np.array_equal(s.search_reads(small_c, small_e), orc.index_reads(small_c, small_e))
+    assert s.last_stats()["streamed"] == 0
+    assert np.array_equal(s.search_reads(big_c, big_e), orc.index_reads(big_c, big_e))
+    assert s.last_stats()["streamed"] > 0
