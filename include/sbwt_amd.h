@@ -345,6 +345,93 @@ int sbwt_color_searcher_counts_device(sbwt_color_searcher *searcher,
 /* Device time (ms) of the last call's color kernel. */
 int sbwt_color_searcher_last_time(sbwt_color_searcher *searcher, float *ms);
 
+/* ---- colors stage from index-result files --------------------------------- */
+
+/* Bodies of the index stage's output files (after the two sized header
+ * strings) parsed on the device into the color search's input: per-k-mer
+ * results (found index, u64 max for a counted "-1", u64 max - 1 for "-2") and
+ * cumulative result counts at each read end. Replaces the reference's
+ * AsciiIndexFileParser / BinaryIndexFileParser::generate_batch
+ * (src/IndexFileParser/AsciiIndexFileParser.cpp:33-98,
+ * BinaryIndexFileParser.cpp:33-70) and the warp layout of
+ * IndexFileParser.cpp:36-74, which the fused color search does not need.
+ * Ascii follows the reference's char loop: any char >= '0' (signed) is a
+ * digit; '-' takes the next char ('1' counts not found, '2' invalid), skips
+ * the digits after it and consumes one more char, which ends the read if it
+ * is '\n'; any other '\n' ends a read. Binary: u64 words, u64 max - 2 ends a
+ * read, a partial last word is not read. Results after the last read end
+ * belong to no read. max_bytes_per_batch < 2^32. */
+#define SBWT_INDEX_FILE_ASCII 0
+#define SBWT_INDEX_FILE_BINARY 1
+typedef struct sbwt_index_file_parser sbwt_index_file_parser;
+int sbwt_index_file_parser_create(uint64_t max_bytes_per_batch, uint64_t max_reads_per_batch,
+                                  void *hip_stream, sbwt_index_file_parser **out);
+void sbwt_index_file_parser_destroy(sbwt_index_file_parser *parser);
+/* d_body: device bytes, 16-byte aligned (read in whole 16-byte chunks).
+ * *d_results / *d_read_kmer_ends point into the parser (valid until its next
+ * call); synchronous on the counts. */
+int sbwt_index_file_parser_parse_device(sbwt_index_file_parser *parser, int format,
+                                        const char *d_body, uint64_t n_bytes,
+                                        const uint64_t **d_results,
+                                        const uint64_t **d_read_kmer_ends, uint64_t *n_results,
+                                        uint64_t *n_reads);
+/* Host bytes in, host arrays out (capacities in entries); synchronous. */
+int sbwt_index_file_parser_parse(sbwt_index_file_parser *parser, int format, const char *body,
+                                 uint64_t n_bytes, uint64_t *results, uint64_t results_capacity,
+                                 uint64_t *read_kmer_ends, uint64_t reads_capacity,
+                                 uint64_t *n_results, uint64_t *n_reads);
+/* Longest prefix of body[0, n_bytes) that a batch may end with: it ends right
+ * after a read end ('\n' not preceded by '-' in ascii; a u64 max - 2 word in
+ * binary) and holds at most max_reads read ends. 0 if there is none. Host
+ * only, no device work. The reference's parser instead stops anywhere and
+ * carries the open read into the next batch
+ * (ContinuousColorResultsPrinter.hpp:175-185). */
+uint64_t sbwt_index_file_batch_cut(int format, const char *body, uint64_t n_bytes,
+                                   uint64_t max_reads);
+
+/* Color id lists (the CSR of sbwt_color_searcher_pseudoalign_device) as the
+ * reference's color printers write them, file headers excluded: ascii ids
+ * joined by ' ' and a '\n' per read; binary a u64 per id and u64 max per read;
+ * csv a row of num_colors "0"/"1" joined by ',' and ended by '\n'
+ * (src/ColorResultsPrinter/AsciiContinuousColorResultsPrinter.cpp:62-79,
+ * BinaryContinuousColorResultsPrinter.cpp:51-62,
+ * CsvContinuousColorResultsPrinter.cpp:40-50,78-110). */
+#define SBWT_COLOR_FORMAT_ASCII 0
+#define SBWT_COLOR_FORMAT_BINARY 1
+#define SBWT_COLOR_FORMAT_CSV 2
+typedef struct sbwt_color_formatter sbwt_color_formatter;
+int sbwt_color_formatter_create(uint64_t max_ids, uint64_t max_reads, void *hip_stream,
+                                sbwt_color_formatter **out);
+void sbwt_color_formatter_destroy(sbwt_color_formatter *formatter);
+/* *n_bytes receives the byte count; the bytes go to d_out only when d_out is
+ * not null and capacity >= *n_bytes. Synchronous on the count. */
+int sbwt_color_formatter_format(sbwt_color_formatter *formatter, int mode,
+                                const uint64_t *d_offsets, const uint32_t *d_ids,
+                                uint64_t n_reads, uint64_t num_colors, char *d_out,
+                                uint64_t capacity, uint64_t *n_bytes);
+
+/* One batch of the colors command on one stream: index-file body bytes (host)
+ * -> parse -> color search -> printer bytes (host). *n_bytes receives the
+ * byte count (copied to out when capacity suffices, else kept for
+ * sbwt_color_file_searcher_copy_last), *n_reads the reads of the batch. The
+ * batch must end at a read end (sbwt_index_file_batch_cut). params->
+ * first_read_quirk belongs on a stream's first batch only. Replaces one batch
+ * of the reference's ContinuousIndexFileParser + ContinuousColorSearcher +
+ * ContinuousColorResultsPrinter (ContinuousIndexFileParser.cpp:48-120,
+ * ContinuousColorSearcher.cpp:73-80, ContinuousColorResultsPrinter.hpp
+ * :167-263). */
+typedef struct sbwt_color_file_searcher sbwt_color_file_searcher;
+int sbwt_color_file_searcher_create(sbwt_color_index *colors, uint64_t max_bytes_per_batch,
+                                    uint64_t max_reads_per_batch, void *hip_stream,
+                                    sbwt_color_file_searcher **out);
+void sbwt_color_file_searcher_destroy(sbwt_color_file_searcher *searcher);
+int sbwt_color_file_searcher_batch(sbwt_color_file_searcher *searcher, int in_format,
+                                   const char *body, uint64_t n_bytes,
+                                   const sbwt_color_params *params, int out_mode, char *out,
+                                   uint64_t capacity, uint64_t *n_bytes_out, uint64_t *n_reads);
+int sbwt_color_file_searcher_copy_last(sbwt_color_file_searcher *searcher, char *out,
+                                       uint64_t capacity);
+
 /* ---- serving loop -------------------------------------------------------- */
 
 /* A request loop over resident indexes: submitted batches of reads queue up

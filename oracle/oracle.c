@@ -677,3 +677,113 @@ uint64_t orc_color_min_hits(uint64_t found, uint64_t not_found, uint64_t invalid
                             (uint64_t)(include_invalid != 0) * invalid;
   return (uint64_t)ceil((double)seq_size * threshold);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Index-result file bodies (the colors stage's input), after the two sized  */
+/* header strings.                                                          */
+/* Ascii: the char loop of src/IndexFileParser/AsciiIndexFileParser.cpp     */
+/* :33-70 with parse_number (:91-98) and skip_until_next_whitespace         */
+/* (:85-89). Chars are signed; "a number" is any char >= '0'. '-' consumes  */
+/* the next char (whatever it is: '1' counts a not-found, '2' an invalid),   */
+/* then every char >= '0', then one more char, which ends the read if it is */
+/* '\n'. A number's terminator is read again by the loop. A read ends at    */
+/* each '\n' met by the loop (IndexFileParser.cpp:36-40).                   */
+/* Binary: u64 words (BinaryIndexFileParser.cpp:33-70): u64max not found,   */
+/* u64max-1 invalid, u64max-2 end of read, anything else found; a partial   */
+/* last word is not read.                                                   */
+/* results[] gets, in file order, the found indexes and ORC_NOT_FOUND /     */
+/* ORC_INVALID for the counted "-1" / "-2"; kmer_ends[r] = results before   */
+/* the end of read r. Results after the last read end belong to no read.   */
+/* Capacity: results <= n + 1, reads <= n + 1. Returns the reads.           */
+/* ------------------------------------------------------------------------ */
+uint64_t orc_parse_index_body(const unsigned char *body, uint64_t n, int format,
+                              uint64_t *results, uint64_t *n_results, uint64_t *kmer_ends) {
+  uint64_t nr = 0, nl = 0;
+  if (format == 1) {
+    for (uint64_t w = 0; w + 8 <= n; w += 8) {
+      uint64_t v;
+      memcpy(&v, body + w, 8);
+      if (v == ORC_NEWLINE) kmer_ends[nl++] = nr;
+      else results[nr++] = v;
+    }
+    *n_results = nr;
+    return nl;
+  }
+  uint64_t p = 0;
+  while (p < n) {
+    signed char c = (signed char)body[p++];
+    if (c == '-') {
+      if (p < n) {
+        const signed char d = (signed char)body[p++];
+        if (d == '1') results[nr++] = ORC_NOT_FOUND;
+        else if (d == '2') results[nr++] = ORC_INVALID;
+      }
+      c = 0;
+      while (p < n && (c = (signed char)body[p++]) >= '0') c = 0;
+      if (c == '\n') kmer_ends[nl++] = nr;
+      continue;
+    }
+    if (c == '\n') {
+      kmer_ends[nl++] = nr;
+      continue;
+    }
+    if (c >= '0') {
+      uint64_t v = (uint64_t)(c - '0');
+      while (p < n && (signed char)body[p] >= '0') v = v * 10 + (uint64_t)((signed char)body[p++] - '0');
+      results[nr++] = v;
+    }
+  }
+  *n_results = nr;
+  return nl;
+}
+
+/* Thresholded color ids of n_reads reads (ids[offsets[r] .. offsets[r+1]),   */
+/* ascending) as the color printers write them (without file headers):      */
+/* mode 0 ascii, src/ColorResultsPrinter/AsciiContinuousColorResultsPrinter */
+/* .cpp:62-79 with ContinuousColorResultsPrinter.hpp:237-263 (' ' between   */
+/* ids, '\n' per read); 1 binary, BinaryContinuousColorResultsPrinter.cpp   */
+/* :51-62 (u64 per id, u64max per read); 2 csv, CsvContinuousColorResults-  */
+/* Printer.cpp:40-50,78-110 (a row of num_colors "0"/"1" joined by ',' and  */
+/* ended by '\n'). out may be NULL (size only). Returns the byte count.     */
+uint64_t orc_color_lists_format(const uint64_t *offsets, const uint32_t *ids, uint64_t n_reads,
+                                uint64_t num_colors, int mode, char *out) {
+  uint64_t b = 0;
+  char tmp[24];
+  for (uint64_t r = 0; r < n_reads; ++r) {
+    if (mode == 2) {
+      if (out) {
+        for (uint64_t c = 0; c < num_colors; ++c) {
+          out[b + 2 * c] = '0';
+          out[b + 2 * c + 1] = c + 1 == num_colors ? '\n' : ',';
+        }
+        for (uint64_t i = offsets[r]; i < offsets[r + 1]; ++i) out[b + 2 * (uint64_t)ids[i]] = '1';
+      }
+      b += 2 * num_colors;
+      continue;
+    }
+    for (uint64_t i = offsets[r]; i < offsets[r + 1]; ++i) {
+      if (mode == 1) {
+        const uint64_t v = ids[i];
+        if (out) memcpy(out + b, &v, 8);
+        b += 8;
+        continue;
+      }
+      if (i > offsets[r]) {
+        if (out) out[b] = ' ';
+        ++b;
+      }
+      const int m = snprintf(tmp, sizeof tmp, "%u", ids[i]);
+      if (out) memcpy(out + b, tmp, (size_t)m);
+      b += (uint64_t)m;
+    }
+    if (mode == 1) {
+      const uint64_t end = UINT64_MAX;
+      if (out) memcpy(out + b, &end, 8);
+      b += 8;
+    } else {
+      if (out) out[b] = '\n';
+      ++b;
+    }
+  }
+  return b;
+}

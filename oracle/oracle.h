@@ -132,6 +132,16 @@ uint64_t orc_warp_layout(const uint64_t *results, const uint64_t *read_kmer_ends
 void orc_color_counts(const orc_colors *c, const uint64_t *results,
                       const uint64_t *read_kmer_ends, uint64_t n_reads, uint64_t *counts,
                       int threads);
+/* Body of an ascii (format 0) or binary (format 1) index-result file ->
+ * per-k-mer results (found index / ORC_NOT_FOUND / ORC_INVALID) and cumulative
+ * result counts at each read end (AsciiIndexFileParser.cpp:33-98,
+ * BinaryIndexFileParser.cpp:33-70). Returns the number of reads. */
+uint64_t orc_parse_index_body(const unsigned char *body, uint64_t n, int format,
+                              uint64_t *results, uint64_t *n_results, uint64_t *kmer_ends);
+/* Color id lists -> printer bytes, mode 0 ascii / 1 binary / 2 csv (no file
+ * headers); out may be NULL. Returns the byte count. */
+uint64_t orc_color_lists_format(const uint64_t *offsets, const uint32_t *ids, uint64_t n_reads,
+                                uint64_t num_colors, int mode, char *out);
 /* ceil(seq_size * threshold) with the reference's double arithmetic. */
 uint64_t orc_color_min_hits(uint64_t found, uint64_t not_found, uint64_t invalid,
                             double threshold, int include_not_found, int include_invalid);

@@ -115,6 +115,28 @@ def lib():
                                                              C.c_uint64, vp, u64p]
         L.sbwt_color_searcher_counts_device.argtypes = [vp, vp, vp, C.c_uint64, vp]
         L.sbwt_color_searcher_last_time.argtypes = [vp, C.POINTER(C.c_float)]
+        L.sbwt_index_file_parser_create.argtypes = [C.c_uint64, C.c_uint64, vp, C.POINTER(vp)]
+        L.sbwt_index_file_parser_destroy.argtypes = [vp]
+        L.sbwt_index_file_parser_destroy.restype = None
+        L.sbwt_index_file_parser_parse_device.argtypes = [vp, C.c_int, vp, C.c_uint64,
+                                                          C.POINTER(vp), C.POINTER(vp), u64p, u64p]
+        L.sbwt_index_file_parser_parse.argtypes = [vp, C.c_int, C.c_char_p, C.c_uint64, u64p,
+                                                   C.c_uint64, u64p, C.c_uint64, u64p, u64p]
+        L.sbwt_index_file_batch_cut.argtypes = [C.c_int, C.c_char_p, C.c_uint64, C.c_uint64]
+        L.sbwt_index_file_batch_cut.restype = C.c_uint64
+        L.sbwt_color_formatter_create.argtypes = [C.c_uint64, C.c_uint64, vp, C.POINTER(vp)]
+        L.sbwt_color_formatter_destroy.argtypes = [vp]
+        L.sbwt_color_formatter_destroy.restype = None
+        L.sbwt_color_formatter_format.argtypes = [vp, C.c_int, vp, vp, C.c_uint64, C.c_uint64, vp,
+                                                  C.c_uint64, u64p]
+        L.sbwt_color_file_searcher_create.argtypes = [vp, C.c_uint64, C.c_uint64, vp,
+                                                      C.POINTER(vp)]
+        L.sbwt_color_file_searcher_destroy.argtypes = [vp]
+        L.sbwt_color_file_searcher_destroy.restype = None
+        L.sbwt_color_file_searcher_batch.argtypes = [vp, C.c_int, C.c_char_p, C.c_uint64,
+                                                     C.POINTER(ColorParams), C.c_int, C.c_char_p,
+                                                     C.c_uint64, u64p, u64p]
+        L.sbwt_color_file_searcher_copy_last.argtypes = [vp, C.c_char_p, C.c_uint64]
         L.sbwt_server_create.argtypes = [vp, vp, C.POINTER(ServerConfig), C.POINTER(vp)]
         L.sbwt_server_submit.argtypes = [vp, C.c_char_p, C.c_uint64, u64p, C.c_uint64, u64p]
         L.sbwt_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(ServerResult)]
@@ -526,3 +548,122 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+INDEX_FILE_ASCII, INDEX_FILE_BINARY = 0, 1
+COLOR_FORMAT_ASCII, COLOR_FORMAT_BINARY, COLOR_FORMAT_CSV = 0, 1, 2
+
+
+def index_file_batch_cut(fmt, body, max_reads):
+    """Longest prefix of an index-file body that a batch may end with."""
+    return int(lib().sbwt_index_file_batch_cut(fmt, bytes(body), len(body), max_reads))
+
+
+class IndexFileParser:
+    """Device parser of index-result file bodies (the colors stage's input)."""
+
+    def __init__(self, max_bytes, max_reads, stream=None):
+        self.h = C.c_void_p()
+        _check(lib().sbwt_index_file_parser_create(max_bytes, max_reads,
+                                                   C.c_void_p(stream) if stream else None,
+                                                   C.byref(self.h)))
+        self.max_bytes = max_bytes
+        self.max_reads = max_reads
+
+    def parse(self, fmt, body):
+        """Host bytes in -> (results u64[], read_kmer_ends u64[])."""
+        body = bytes(body)
+        res = np.zeros(len(body) // 2 + 1, np.uint64)
+        ends = np.zeros(self.max_reads, np.uint64)
+        nr, nl = C.c_uint64(), C.c_uint64()
+        _check(lib().sbwt_index_file_parser_parse(self.h, fmt, body, len(body), _p64(res), len(res),
+                                                  _p64(ends), len(ends), C.byref(nr), C.byref(nl)))
+        return res[:nr.value].copy(), ends[:nl.value].copy()
+
+    def parse_device(self, fmt, d_body, n_bytes):
+        """Device body (int pointer, 16-byte aligned) -> (d_results, d_ends, n_results, n_reads)."""
+        dr, de = C.c_void_p(), C.c_void_p()
+        nr, nl = C.c_uint64(), C.c_uint64()
+        _check(lib().sbwt_index_file_parser_parse_device(self.h, fmt, C.c_void_p(d_body), n_bytes,
+                                                         C.byref(dr), C.byref(de), C.byref(nr),
+                                                         C.byref(nl)))
+        return dr.value, de.value, int(nr.value), int(nl.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sbwt_index_file_parser_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class ColorFormatter:
+    """Device printer of color id lists (ascii / binary / csv bytes)."""
+
+    def __init__(self, max_ids, max_reads, stream=None):
+        self.h = C.c_void_p()
+        _check(lib().sbwt_color_formatter_create(max_ids, max_reads,
+                                                 C.c_void_p(stream) if stream else None,
+                                                 C.byref(self.h)))
+
+    def format(self, mode, offsets, ids, num_colors):
+        """Host CSR in, bytes out (via torch device buffers)."""
+        import torch
+        off = torch.from_numpy(np.ascontiguousarray(offsets, np.uint64).view(np.int64)).cuda()
+        idv = torch.from_numpy(np.ascontiguousarray(ids if len(ids) else [0], np.uint32)
+                               .view(np.int32)).cuda()
+        n_reads = len(offsets) - 1
+        n = C.c_uint64()
+        _check(lib().sbwt_color_formatter_format(self.h, mode, C.c_void_p(off.data_ptr()),
+                                                 C.c_void_p(idv.data_ptr()), n_reads, num_colors,
+                                                 None, 0, C.byref(n)))
+        out = torch.empty(max(n.value, 1), dtype=torch.uint8, device="cuda")
+        n2 = C.c_uint64()
+        _check(lib().sbwt_color_formatter_format(self.h, mode, C.c_void_p(off.data_ptr()),
+                                                 C.c_void_p(idv.data_ptr()), n_reads, num_colors,
+                                                 C.c_void_p(out.data_ptr()), n.value,
+                                                 C.byref(n2)))
+        torch.cuda.synchronize()
+        return out[:n.value].cpu().numpy().tobytes()
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sbwt_color_formatter_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class ColorFileSearcher:
+    """The colors command's device stage: index-file body -> color printer bytes."""
+
+    def __init__(self, colors, max_bytes, max_reads, stream=None):
+        self.colors = colors
+        self.h = C.c_void_p()
+        _check(lib().sbwt_color_file_searcher_create(colors.h, max_bytes, max_reads,
+                                                     C.c_void_p(stream) if stream else None,
+                                                     C.byref(self.h)))
+
+    def batch(self, in_fmt, body, params, out_mode):
+        """Returns (bytes, n_reads)."""
+        body = bytes(body)
+        cap = max(1024, 4 * len(body))
+        out = C.create_string_buffer(cap)
+        nb, nr = C.c_uint64(), C.c_uint64()
+        _check(lib().sbwt_color_file_searcher_batch(self.h, in_fmt, body, len(body),
+                                                    C.byref(params), out_mode, out, cap,
+                                                    C.byref(nb), C.byref(nr)))
+        if nb.value > cap:
+            out = C.create_string_buffer(nb.value)
+            _check(lib().sbwt_color_file_searcher_copy_last(self.h, out, nb.value))
+        return out.raw[:nb.value], int(nr.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sbwt_color_file_searcher_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()

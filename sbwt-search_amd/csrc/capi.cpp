@@ -9,6 +9,7 @@
 #include "color_search.hpp"
 #include "device_index.hpp"
 #include "format.hpp"
+#include "index_parse.hpp"
 #include "index_search.hpp"
 #include "loaders.hpp"
 #include "server.hpp"
@@ -41,6 +42,30 @@ struct sbwt_formatter {
 struct sbwt_color_searcher {
   std::unique_ptr<sbwt_amd::ColorSearcher> impl;
   const sbwt_color_index *colors = nullptr;
+};
+
+struct sbwt_index_file_parser {
+  std::unique_ptr<sbwt_amd::IndexFileParser> impl;
+};
+
+struct sbwt_color_formatter {
+  std::unique_ptr<sbwt_amd::ColorFormatter> impl;
+};
+
+// The colors command's device stage: parser, color search and printer on the
+// parser's stream, with the color id CSR between them.
+struct sbwt_color_file_searcher {
+  const sbwt_color_index *colors = nullptr;
+  std::unique_ptr<sbwt_amd::IndexFileParser> parser;
+  std::unique_ptr<sbwt_amd::ColorSearcher> search;
+  std::unique_ptr<sbwt_amd::ColorFormatter> printer;
+  uint64_t *d_offsets = nullptr;
+  uint32_t *d_ids = nullptr;
+  uint64_t ids_capacity = 0;
+  ~sbwt_color_file_searcher() {
+    if (d_offsets) (void)hipFree(d_offsets);
+    if (d_ids) (void)hipFree(d_ids);
+  }
 };
 
 namespace {
@@ -478,6 +503,148 @@ int sbwt_color_searcher_last_time(sbwt_color_searcher *searcher, float *ms) {
   return guarded([&] {
     require(searcher && ms, "null argument");
     *ms = searcher->impl->last_search_ms();
+  });
+}
+
+int sbwt_index_file_parser_create(uint64_t max_bytes_per_batch, uint64_t max_reads_per_batch,
+                                  void *hip_stream, sbwt_index_file_parser **out) {
+  return guarded([&] {
+    require(out, "null argument");
+    require(max_reads_per_batch > 0, "max_reads_per_batch must be positive");
+    auto p = std::make_unique<sbwt_index_file_parser>();
+    p->impl = std::make_unique<sbwt_amd::IndexFileParser>(max_bytes_per_batch, max_reads_per_batch,
+                                                          static_cast<hipStream_t>(hip_stream));
+    *out = p.release();
+  });
+}
+
+void sbwt_index_file_parser_destroy(sbwt_index_file_parser *parser) { delete parser; }
+
+int sbwt_index_file_parser_parse_device(sbwt_index_file_parser *parser, int format,
+                                        const char *d_body, uint64_t n_bytes,
+                                        const uint64_t **d_results,
+                                        const uint64_t **d_read_kmer_ends, uint64_t *n_results,
+                                        uint64_t *n_reads) {
+  return guarded([&] {
+    require(parser && d_results && d_read_kmer_ends && n_results && n_reads, "null argument");
+    require(n_bytes == 0 || d_body, "null argument");
+    *n_reads = parser->impl->parse_device(format, d_body, n_bytes, n_results);
+    *d_results = parser->impl->results();
+    *d_read_kmer_ends = parser->impl->kmer_ends();
+  });
+}
+
+int sbwt_index_file_parser_parse(sbwt_index_file_parser *parser, int format, const char *body,
+                                 uint64_t n_bytes, uint64_t *results, uint64_t results_capacity,
+                                 uint64_t *read_kmer_ends, uint64_t reads_capacity,
+                                 uint64_t *n_results, uint64_t *n_reads) {
+  return guarded([&] {
+    require(parser && n_results && n_reads, "null argument");
+    require(n_bytes == 0 || body, "null argument");
+    uint64_t nr = 0;
+    const uint64_t nl = parser->impl->parse_host(format, body, n_bytes, &nr);
+    require(nr <= results_capacity && nl <= reads_capacity, "output arrays too small for the batch");
+    require((nr == 0 || results) && (nl == 0 || read_kmer_ends), "null argument");
+    hipStream_t st = parser->impl->stream();
+    if (nr) SBWT_HIP(hipMemcpyAsync(results, parser->impl->results(), nr * 8, hipMemcpyDeviceToHost, st));
+    if (nl)
+      SBWT_HIP(hipMemcpyAsync(read_kmer_ends, parser->impl->kmer_ends(), nl * 8,
+                              hipMemcpyDeviceToHost, st));
+    SBWT_HIP(hipStreamSynchronize(st));
+    *n_results = nr;
+    *n_reads = nl;
+  });
+}
+
+uint64_t sbwt_index_file_batch_cut(int format, const char *body, uint64_t n_bytes,
+                                   uint64_t max_reads) {
+  if (!body || (format != SBWT_INDEX_FILE_ASCII && format != SBWT_INDEX_FILE_BINARY)) return 0;
+  return sbwt_amd::IndexFileParser::batch_cut(format, body, n_bytes, max_reads);
+}
+
+int sbwt_color_formatter_create(uint64_t max_ids, uint64_t max_reads, void *hip_stream,
+                                sbwt_color_formatter **out) {
+  return guarded([&] {
+    require(out, "null argument");
+    auto f = std::make_unique<sbwt_color_formatter>();
+    f->impl = std::make_unique<sbwt_amd::ColorFormatter>(max_ids, max_reads,
+                                                         static_cast<hipStream_t>(hip_stream));
+    *out = f.release();
+  });
+}
+
+void sbwt_color_formatter_destroy(sbwt_color_formatter *formatter) { delete formatter; }
+
+int sbwt_color_formatter_format(sbwt_color_formatter *formatter, int mode,
+                                const uint64_t *d_offsets, const uint32_t *d_ids,
+                                uint64_t n_reads, uint64_t num_colors, char *d_out,
+                                uint64_t capacity, uint64_t *n_bytes) {
+  return guarded([&] {
+    require(formatter && n_bytes, "null argument");
+    require(n_reads == 0 || (d_offsets && d_ids), "null argument");
+    *n_bytes = formatter->impl->format(mode, d_offsets, d_ids, n_reads, num_colors, d_out, capacity);
+  });
+}
+
+int sbwt_color_file_searcher_create(sbwt_color_index *colors, uint64_t max_bytes_per_batch,
+                                    uint64_t max_reads_per_batch, void *hip_stream,
+                                    sbwt_color_file_searcher **out) {
+  return guarded([&] {
+    require(colors && out, "null argument");
+    require(max_reads_per_batch > 0, "max_reads_per_batch must be positive");
+    auto f = std::make_unique<sbwt_color_file_searcher>();
+    f->colors = colors;
+    f->parser = std::make_unique<sbwt_amd::IndexFileParser>(max_bytes_per_batch, max_reads_per_batch,
+                                                            static_cast<hipStream_t>(hip_stream));
+    hipStream_t st = f->parser->stream();
+    f->search = std::make_unique<sbwt_amd::ColorSearcher>(&colors->dev, f->parser->max_results(),
+                                                          max_reads_per_batch, st);
+    f->search->set_timing(false);
+    // a read reports at most min(num_colors, its found results) ids
+    const uint64_t nc = std::max<uint64_t>(colors->dev.num_colors, 1);
+    f->ids_capacity = max_reads_per_batch > f->parser->max_results() / nc
+                          ? f->parser->max_results()
+                          : max_reads_per_batch * nc;
+    f->printer = std::make_unique<sbwt_amd::ColorFormatter>(f->ids_capacity, max_reads_per_batch, st);
+    SBWT_HIP(hipMalloc(&f->d_offsets, (max_reads_per_batch + 1) * 8));
+    SBWT_HIP(hipMalloc(&f->d_ids, std::max<uint64_t>(f->ids_capacity, 1) * 4));
+    *out = f.release();
+  });
+}
+
+void sbwt_color_file_searcher_destroy(sbwt_color_file_searcher *searcher) { delete searcher; }
+
+int sbwt_color_file_searcher_batch(sbwt_color_file_searcher *searcher, int in_format,
+                                   const char *body, uint64_t n_bytes,
+                                   const sbwt_color_params *params, int out_mode, char *out,
+                                   uint64_t capacity, uint64_t *n_bytes_out, uint64_t *n_reads) {
+  return guarded([&] {
+    require(searcher && n_bytes_out && n_reads, "null argument");
+    require(n_bytes == 0 || body, "null argument");
+    require(out_mode == SBWT_COLOR_FORMAT_ASCII || out_mode == SBWT_COLOR_FORMAT_BINARY ||
+                out_mode == SBWT_COLOR_FORMAT_CSV,
+            "unknown color output format");
+    const sbwt_amd::ColorParams p = to_params(params);
+    *n_bytes_out = 0;
+    uint64_t nr = 0;
+    const uint64_t nl = searcher->parser->parse_host(in_format, body, n_bytes, &nr);
+    *n_reads = nl;
+    if (nl == 0) return;
+    searcher->search->pseudoalign_device_async(searcher->parser->results(),
+                                               searcher->parser->kmer_ends(), nl, p,
+                                               searcher->d_offsets, searcher->d_ids,
+                                               searcher->ids_capacity, nullptr);
+    *n_bytes_out = searcher->printer->format_to_host(out_mode, searcher->d_offsets, searcher->d_ids,
+                                                     nl, searcher->colors->dev.num_colors, out,
+                                                     capacity);
+  });
+}
+
+int sbwt_color_file_searcher_copy_last(sbwt_color_file_searcher *searcher, char *out,
+                                       uint64_t capacity) {
+  return guarded([&] {
+    require(searcher && out, "null argument");
+    searcher->printer->copy_last(out, capacity);
   });
 }
 

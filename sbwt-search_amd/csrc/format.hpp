@@ -47,4 +47,43 @@ class Formatter {
   u64 out_capacity_ = 0, last_bytes_ = 0;
 };
 
+// Color id lists of a batch of reads (the color stage's CSR: d_offsets[r] ..
+// d_offsets[r+1] into d_ids, ascending per read) as the reference's color
+// printers write them, without file headers
+// (src/ColorResultsPrinter/AsciiContinuousColorResultsPrinter.cpp:62-79,
+// BinaryContinuousColorResultsPrinter.cpp:51-62,
+// CsvContinuousColorResultsPrinter.cpp:40-50,78-110, line structure
+// ContinuousColorResultsPrinter.hpp:237-263):
+//   ascii   ids separated by ' ', '\n' at the end of every read
+//   binary  one u64 per id, u64 max at the end of every read
+//   csv     per read num_colors "0"/"1" joined by ',' and ended by '\n'
+constexpr int kColorFormatAscii = 0, kColorFormatBinary = 1, kColorFormatCsv = 2;
+
+class ColorFormatter {
+ public:
+  ColorFormatter(u64 max_ids, u64 max_reads, hipStream_t stream);
+  ~ColorFormatter();
+  ColorFormatter(const ColorFormatter &) = delete;
+  ColorFormatter &operator=(const ColorFormatter &) = delete;
+  // Bytes of the batch; written to d_out when not null and capacity
+  // suffices. Synchronous on the byte count.
+  u64 format(int mode, const u64 *d_offsets, const u32 *d_ids, u64 n_reads, u64 num_colors,
+             char *d_out, u64 capacity);
+  // Into the formatter's own device buffer (grown as needed), then to
+  // host_out when capacity suffices; copy_last copies it later otherwise.
+  u64 format_to_host(int mode, const u64 *d_offsets, const u32 *d_ids, u64 n_reads,
+                     u64 num_colors, char *host_out, u64 capacity);
+  void copy_last(char *host_out, u64 capacity);
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  u64 max_items_, max_reads_;
+  hipStream_t stream_;
+  bool own_stream_;
+  u64 *d_block_sums_ = nullptr;
+  u64 *d_scan_tmp_ = nullptr;
+  char *d_out_ = nullptr;
+  u64 out_capacity_ = 0, last_bytes_ = 0;
+};
+
 }  // namespace sbwt_amd

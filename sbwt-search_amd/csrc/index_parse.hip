@@ -1,0 +1,398 @@
+// Device parser of index-result files (see index_parse.hpp).
+// Ascii: four kernels over 4 KiB blocks of 256 threads x 16-byte segments:
+//   maps    each block's state map (composition of its bytes' maps)
+//   states  one workgroup scans the block maps into each block's start state
+//   count   per block: results and read ends (each thread walks its segment
+//           from its start state: the block's start state through the
+//           exclusive prefix of its segment maps)
+//   write   the same walk, writing at offsets from the scan of the counts
+// Binary: count and write kernels over u64 words.
+#include "index_parse.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+
+#include "scan.hpp"
+
+namespace sbwt_amd {
+namespace {
+
+constexpr u32 kParseThreads = 256;
+constexpr u32 kSegBytes = 16;
+constexpr u64 kBlockBytes = u64{kParseThreads} * kSegBytes;   // ascii block: 4 KiB
+constexpr u64 kBinaryBlockBytes = u64{kParseThreads} * 8;      // binary block: 256 words
+constexpr u32 kStatesThreads = 1024;
+
+// Automaton states: top level, inside a number, after '-', skipping the
+// digits after the char that follows '-'.
+constexpr u32 kTop = 0, kNum = 1, kDash = 2, kSkip = 3;
+// A state map packs the next state of start state s in bits [2s, 2s + 2).
+constexpr u32 kMapIdentity = 0 | 1 << 2 | 2 << 4 | 3 << 6;
+constexpr u32 kMapDigit = kNum | kNum << 2 | kSkip << 4 | kSkip << 6;
+constexpr u32 kMapMinus = kDash | kDash << 2 | kSkip << 4 | kTop << 6;
+constexpr u32 kMapOther = kTop | kTop << 2 | kSkip << 4 | kTop << 6;  // '\n' and the rest
+
+__device__ __forceinline__ bool is_digit(u8 b) { return static_cast<signed char>(b) >= '0'; }
+
+__device__ __forceinline__ u32 char_map(u8 b) {
+  return is_digit(b) ? kMapDigit : (b == '-' ? kMapMinus : kMapOther);
+}
+
+__device__ __forceinline__ u32 next_state(u32 map, u32 s) { return (map >> (2 * s)) & 3u; }
+
+// f, then g.
+__device__ __forceinline__ u32 compose(u32 f, u32 g) {
+  u32 h = 0;
+#pragma unroll
+  for (u32 s = 0; s < 4; ++s) h |= next_state(g, next_state(f, s)) << (2 * s);
+  return h;
+}
+
+// The bytes of segment `seg` (zero past n); returns how many are in range.
+__device__ __forceinline__ u32 load_segment(const u8 *body, u64 n, u64 seg, u8 (&b)[kSegBytes]) {
+  const u64 beg = seg * kSegBytes;
+  if (beg >= n) return 0;
+  const u32 len = static_cast<u32>(min<u64>(kSegBytes, n - beg));
+  const uint4 v = *reinterpret_cast<const uint4 *>(body + beg);
+  const u32 w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (u32 i = 0; i < kSegBytes; ++i) b[i] = i < len ? static_cast<u8>(w[i / 4] >> (8 * (i % 4))) : 0;
+  return len;
+}
+
+__device__ __forceinline__ u32 segment_map(const u8 (&b)[kSegBytes], u32 len) {
+  u32 m = kMapIdentity;
+#pragma unroll
+  for (u32 i = 0; i < kSegBytes; ++i)
+    if (i < len) m = compose(m, char_map(b[i]));
+  return m;
+}
+
+// Exclusive prefix (in thread order) of the maps of a workgroup of WAVES
+// waves; *total gets the composition of all of them.
+template <u32 WAVES>
+__device__ __forceinline__ u32 block_scan_maps(u32 m, u32 *sh, u32 *total) {
+  const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  u32 incl = m;
+#pragma unroll
+  for (u32 d = 1; d < 64; d <<= 1) {
+    const u32 y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl = compose(y, incl);
+  }
+  if (lane == 63) sh[wave] = incl;
+  __syncthreads();
+  u32 before = kMapIdentity, all = kMapIdentity;
+#pragma unroll
+  for (u32 w = 0; w < WAVES; ++w) {
+    if (w < wave) before = compose(before, sh[w]);
+    all = compose(all, sh[w]);
+  }
+  __syncthreads();
+  u32 excl = __shfl_up(incl, 1, 64);
+  if (lane == 0) excl = kMapIdentity;
+  *total = all;
+  return compose(before, excl);
+}
+
+// Workgroup sum (every thread gets it) and exclusive prefix of x.
+__device__ __forceinline__ u32 block_prefix_sum(u32 x, u32 *sh, u32 *total) {
+  const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  u32 incl = x;
+#pragma unroll
+  for (u32 d = 1; d < 64; d <<= 1) {
+    const u32 y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63) sh[wave] = incl;
+  __syncthreads();
+  u32 before = 0, all = 0;
+#pragma unroll
+  for (u32 w = 0; w < kParseThreads / 64; ++w) {
+    if (w < wave) before += sh[w];
+    all += sh[w];
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - x;
+}
+
+// u64 value of the digit run at body[pos] (reference parse_number: wrapping
+// arithmetic over every char >= '0').
+__device__ __forceinline__ u64 parse_number(const u8 *body, u64 n, u64 pos) {
+  u64 v = 0;
+  for (u64 j = pos; j < n; ++j) {
+    const u8 c = body[j];
+    if (!is_digit(c)) break;
+    v = v * 10 + static_cast<u64>(static_cast<signed char>(c) - '0');
+  }
+  return v;
+}
+
+// Runs the automaton over one segment from state st, counting results (r)
+// and read ends (nl); with WRITE, also stores them at rbase + r and
+// nlbase + nl.
+template <bool WRITE>
+__device__ __forceinline__ void walk_segment(const u8 *body, u64 n, u64 beg, const u8 (&b)[kSegBytes],
+                                             u32 len, u32 st, u32 &r, u32 &nl, u64 *results,
+                                             u64 rbase, u64 *kends, u64 nlbase) {
+#pragma unroll
+  for (u32 i = 0; i < kSegBytes; ++i) {
+    if (i < len) {
+      const u8 c = b[i];
+      // what this char emits: a result (counted '-1' / '-2', or a number's
+      // first digit) or a read end
+      const bool dash_result = st == kDash && (c == '1' || c == '2');
+      const bool number = st == kTop && is_digit(c);
+      const bool read_end = st != kDash && c == '\n';
+      if constexpr (WRITE) {
+        if (dash_result) results[rbase + r] = c == '1' ? kNotFound : kInvalid;
+        if (number) results[rbase + r] = parse_number(body, n, beg + i);
+        if (read_end) kends[nlbase + nl] = rbase + r;
+      }
+      r += (dash_result || number) ? 1u : 0u;
+      nl += read_end ? 1u : 0u;
+      st = next_state(char_map(c), st);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kParseThreads) void k_ascii_maps(const u8 *__restrict__ body, u64 n,
+                                                              u64 n_blocks, u8 *__restrict__ maps) {
+  __shared__ u32 sh[kParseThreads / 64];
+  for (u64 blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+    u8 b[kSegBytes];
+    const u32 len = load_segment(body, n, blk * kParseThreads + threadIdx.x, b);
+    u32 total;
+    block_scan_maps<kParseThreads / 64>(segment_map(b, len), sh, &total);
+    if (threadIdx.x == 0) maps[blk] = static_cast<u8>(total);
+  }
+}
+
+// One workgroup: block maps -> block start states (in place is not used:
+// states go to their own array).
+__global__ __launch_bounds__(kStatesThreads) void k_ascii_states(const u8 *__restrict__ maps,
+                                                                 u64 n_blocks,
+                                                                 u8 *__restrict__ states) {
+  __shared__ u32 sh[kStatesThreads / 64];
+  const u64 per = (n_blocks + kStatesThreads - 1) / kStatesThreads;
+  const u64 beg = min<u64>(n_blocks, threadIdx.x * per), end = min<u64>(n_blocks, beg + per);
+  u32 m = kMapIdentity;
+  for (u64 i = beg; i < end; ++i) m = compose(m, maps[i]);
+  u32 total;
+  const u32 before = block_scan_maps<kStatesThreads / 64>(m, sh, &total);
+  u32 st = next_state(before, kTop);
+  for (u64 i = beg; i < end; ++i) {
+    states[i] = static_cast<u8>(st);
+    st = next_state(maps[i], st);
+  }
+}
+
+// Start state of this thread's segment, whose bytes are b[0, len).
+__device__ __forceinline__ u32 segment_start(u64 blk, const u8 *states, u32 *sh,
+                                             const u8 (&b)[kSegBytes], u32 len) {
+  u32 total;
+  const u32 before = block_scan_maps<kParseThreads / 64>(segment_map(b, len), sh, &total);
+  return next_state(before, states[blk]);
+}
+
+__global__ __launch_bounds__(kParseThreads) void k_ascii_count(const u8 *__restrict__ body, u64 n,
+                                                               u64 n_blocks,
+                                                               const u8 *__restrict__ states,
+                                                               u64 *__restrict__ counts) {
+  __shared__ u32 sh[kParseThreads / 64];
+  for (u64 blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+    u8 b[kSegBytes];
+    const u32 len = load_segment(body, n, blk * kParseThreads + threadIdx.x, b);
+    const u32 st = segment_start(blk, states, sh, b, len);
+    u32 r = 0, nl = 0;
+    const u64 beg = (blk * kParseThreads + threadIdx.x) * kSegBytes;
+    walk_segment<false>(body, n, beg, b, len, st, r, nl, nullptr, 0, nullptr, 0);
+    u32 tr, tn;
+    block_prefix_sum(r, sh, &tr);
+    block_prefix_sum(nl, sh, &tn);
+    if (threadIdx.x == 0) counts[blk] = (u64{tr} << 32) | tn;
+  }
+}
+
+__global__ __launch_bounds__(kParseThreads) void k_ascii_write(const u8 *__restrict__ body, u64 n,
+                                                               u64 n_blocks,
+                                                               const u8 *__restrict__ states,
+                                                               const u64 *__restrict__ incl,
+                                                               u64 *__restrict__ results,
+                                                               u64 *__restrict__ kends) {
+  __shared__ u32 sh[kParseThreads / 64];
+  for (u64 blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+    u8 b[kSegBytes];
+    const u32 len = load_segment(body, n, blk * kParseThreads + threadIdx.x, b);
+    const u32 st = segment_start(blk, states, sh, b, len);
+    u32 r = 0, nl = 0;
+    const u64 beg = (blk * kParseThreads + threadIdx.x) * kSegBytes;
+    walk_segment<false>(body, n, beg, b, len, st, r, nl, nullptr, 0, nullptr, 0);
+    u32 tr, tn;
+    const u32 r_before = block_prefix_sum(r, sh, &tr);
+    const u32 nl_before = block_prefix_sum(nl, sh, &tn);
+    const u64 base = blk ? incl[blk - 1] : 0;
+    r = 0;
+    nl = 0;
+    walk_segment<true>(body, n, beg, b, len, st, r, nl, results, (base >> 32) + r_before, kends,
+                       (base & 0xFFFFFFFFu) + nl_before);
+  }
+}
+
+__global__ __launch_bounds__(kParseThreads) void k_binary_count(const u64 *__restrict__ words,
+                                                                u64 n_words, u64 n_blocks,
+                                                                u64 *__restrict__ counts) {
+  __shared__ u32 sh[kParseThreads / 64];
+  for (u64 blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+    const u64 i = blk * kParseThreads + threadIdx.x;
+    const u32 nl = i < n_words && words[i] == kNewline ? 1u : 0u;
+    const u32 r = i < n_words && !nl ? 1u : 0u;
+    u32 tr, tn;
+    block_prefix_sum(r, sh, &tr);
+    block_prefix_sum(nl, sh, &tn);
+    if (threadIdx.x == 0) counts[blk] = (u64{tr} << 32) | tn;
+  }
+}
+
+__global__ __launch_bounds__(kParseThreads) void k_binary_write(const u64 *__restrict__ words,
+                                                                u64 n_words, u64 n_blocks,
+                                                                const u64 *__restrict__ incl,
+                                                                u64 *__restrict__ results,
+                                                                u64 *__restrict__ kends) {
+  __shared__ u32 sh[kParseThreads / 64];
+  for (u64 blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+    const u64 i = blk * kParseThreads + threadIdx.x;
+    const u64 w = i < n_words ? words[i] : 0;
+    const u32 nl = i < n_words && w == kNewline ? 1u : 0u;
+    const u32 r = i < n_words && !nl ? 1u : 0u;
+    u32 tr, tn;
+    const u32 r_before = block_prefix_sum(r, sh, &tr);
+    const u32 nl_before = block_prefix_sum(nl, sh, &tn);
+    const u64 base = blk ? incl[blk - 1] : 0;
+    const u64 ri = (base >> 32) + r_before;
+    if (r) results[ri] = w;
+    if (nl) kends[(base & 0xFFFFFFFFu) + nl_before] = ri;
+  }
+}
+
+u32 grid_of(u64 n_blocks) { return static_cast<u32>(std::min<u64>(n_blocks, 1u << 16)); }
+
+}  // namespace
+
+IndexFileParser::IndexFileParser(u64 max_bytes, u64 max_reads, hipStream_t stream)
+    : max_bytes_(max_bytes), max_reads_(max_reads), stream_(stream), own_stream_(stream == nullptr) {
+  if (max_bytes_ == 0 || max_bytes_ >= (1ULL << 32))
+    throw std::runtime_error("index file batches must hold between 1 byte and 4 GiB - 1");
+  // every ascii result starts at its own byte and the next one starts at
+  // least two bytes later; a binary result is 8 bytes
+  max_results_ = max_bytes_ / 2 + 1;
+  if (own_stream_) SBWT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  // binary blocks are the smaller, so they bound the block count
+  block_cap_ = max_bytes_ / kBinaryBlockBytes + 2;
+  const u64 blocks = block_cap_;
+  SBWT_HIP(hipMalloc(&d_body_, max_bytes_ + 64));
+  SBWT_HIP(hipMalloc(&d_results_, max_results_ * 8));
+  SBWT_HIP(hipMalloc(&d_kends_, std::max<u64>(max_reads_, 1) * 8));
+  SBWT_HIP(hipMalloc(&d_block_maps_, blocks * 2));
+  SBWT_HIP(hipMalloc(&d_block_counts_, blocks * 2 * 8));
+  SBWT_HIP(hipMalloc(&d_scan_tmp_, scan_tmp_entries(blocks) * 8));
+}
+
+IndexFileParser::~IndexFileParser() {
+  void *bufs[] = {d_body_, d_results_, d_kends_, d_block_maps_, d_block_counts_, d_scan_tmp_};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+}
+
+u64 IndexFileParser::parse_device(int format, const char *d_body, u64 n_bytes, u64 *n_results) {
+  *n_results = 0;
+  if (format != kIndexFileAscii && format != kIndexFileBinary)
+    throw std::runtime_error("index file format must be ascii or binary");
+  if (n_bytes > max_bytes_) throw std::runtime_error("index file batch exceeds the parser's max bytes");
+  if (reinterpret_cast<uintptr_t>(d_body) % 16)
+    throw std::runtime_error("index file body must be 16-byte aligned on the device");
+  const u8 *body = reinterpret_cast<const u8 *>(d_body);
+  const u64 units = format == kIndexFileAscii ? n_bytes : n_bytes / 8;
+  if (units == 0) return 0;
+  const u64 n_blocks = format == kIndexFileAscii ? (n_bytes + kBlockBytes - 1) / kBlockBytes
+                                                 : (units + kParseThreads - 1) / kParseThreads;
+  if (n_blocks > block_cap_) throw std::runtime_error("index file batch exceeds the parser's blocks");
+  u8 *states = d_block_maps_ + block_cap_;
+  u64 *incl = d_block_counts_ + block_cap_;
+  const u32 grid = grid_of(n_blocks);
+  if (format == kIndexFileAscii) {
+    k_ascii_maps<<<grid, kParseThreads, 0, stream_>>>(body, n_bytes, n_blocks, d_block_maps_);
+    k_ascii_states<<<1, kStatesThreads, 0, stream_>>>(d_block_maps_, n_blocks, states);
+    k_ascii_count<<<grid, kParseThreads, 0, stream_>>>(body, n_bytes, n_blocks, states,
+                                                       d_block_counts_);
+  } else {
+    k_binary_count<<<grid, kParseThreads, 0, stream_>>>(reinterpret_cast<const u64 *>(body), units,
+                                                        n_blocks, d_block_counts_);
+  }
+  SBWT_HIP(hipGetLastError());
+  scan_inclusive_u64(d_block_counts_, n_blocks, incl, d_scan_tmp_, stream_);
+  u64 tot = 0;
+  SBWT_HIP(hipMemcpyAsync(&tot, incl + n_blocks - 1, 8, hipMemcpyDeviceToHost, stream_));
+  SBWT_HIP(hipStreamSynchronize(stream_));
+  const u64 nr = tot >> 32, nl = tot & 0xFFFFFFFFu;
+  if (nl > max_reads_) throw std::runtime_error("index file batch holds more reads than the parser's max reads");
+  if (nr > max_results_) throw std::runtime_error("index file batch holds more results than the parser can");
+  if (format == kIndexFileAscii) {
+    k_ascii_write<<<grid, kParseThreads, 0, stream_>>>(body, n_bytes, n_blocks, states, incl,
+                                                       d_results_, d_kends_);
+  } else {
+    k_binary_write<<<grid, kParseThreads, 0, stream_>>>(reinterpret_cast<const u64 *>(body), units,
+                                                        n_blocks, incl, d_results_, d_kends_);
+  }
+  SBWT_HIP(hipGetLastError());
+  *n_results = nr;
+  return nl;
+}
+
+u64 IndexFileParser::parse_host(int format, const char *body, u64 n_bytes, u64 *n_results) {
+  *n_results = 0;
+  if (n_bytes > max_bytes_) throw std::runtime_error("index file batch exceeds the parser's max bytes");
+  if (n_bytes)
+    SBWT_HIP(hipMemcpyAsync(d_body_, body, n_bytes, hipMemcpyHostToDevice, stream_));
+  return parse_device(format, d_body_, n_bytes, n_results);
+}
+
+u64 IndexFileParser::batch_cut(int format, const char *body, u64 n, u64 max_reads) {
+  if (format == kIndexFileBinary) {
+    u64 cut = 0, reads = 0;
+    for (u64 w = 0; w + 8 <= n && reads < max_reads; w += 8) {
+      u64 v;
+      std::memcpy(&v, body + w, 8);
+      if (v == kNewline) {
+        ++reads;
+        cut = w + 8;
+      }
+    }
+    return cut;
+  }
+  // A read end is at most one per '\n' byte: stop at the max_reads-th one,
+  // then step back to a '\n' that is not the char consumed after a '-'.
+  u64 limit = n, seen = 0;
+  for (u64 p = 0; p < n;) {
+    const u64 span = std::min<u64>(n - p, 1 << 16);
+    const u64 c = static_cast<u64>(std::count(body + p, body + p + span, '\n'));
+    if (seen + c <= max_reads) {
+      seen += c;
+      p += span;
+      continue;
+    }
+    for (u64 q = p; q < p + span; ++q)
+      if (body[q] == '\n' && ++seen == max_reads) {
+        limit = q + 1;
+        break;
+      }
+    break;
+  }
+  for (u64 c = limit; c > 0; --c)
+    if (body[c - 1] == '\n' && (c < 2 || body[c - 2] != '-')) return c;
+  return 0;
+}
+
+}  // namespace sbwt_amd

@@ -4,7 +4,6 @@
 #include <zlib.h>
 
 #include <algorithm>
-#include <charconv>
 #include <cstring>
 #include <fstream>
 #include <map>
@@ -14,11 +13,6 @@
 namespace sbwt_host {
 
 namespace {
-constexpr u64 kNotFound = ~0ULL;
-constexpr u64 kInvalid = ~0ULL - 1;
-constexpr u64 kNewline = ~0ULL - 2;
-constexpr size_t kFlush = 1 << 24;
-
 void write_all(FILE *f, const void *p, size_t n) {
   if (n && fwrite(p, 1, n, f) != n) throw std::runtime_error("write failed");
 }
@@ -137,125 +131,72 @@ IndexFileOut::~IndexFileOut() { fclose(f_); }
 
 void IndexFileOut::write(const char *p, u64 n) { write_all(f_, p, n); }
 
-// ------------------------------------------------------------ color writer
-ColorWriter::ColorWriter(const std::string &path, const std::string &mode, bool headers,
-                         u64 num_colors)
-    : num_colors_(num_colors) {
+// ------------------------------------------------------------ color output
+ColorFileOut::ColorFileOut(const std::string &path, int mode, bool headers, u64 num_colors) {
   static const char *ext[3] = {".txt", ".bin", ".csv"};
-  if (mode == "ascii") mode_ = 0;
-  else if (mode == "binary") mode_ = 1;
-  else if (mode == "csv") mode_ = 2;
-  else throw std::runtime_error("Invalid value passed by user for argument print_mode");
-  f_ = fopen((path + ext[mode_]).c_str(), "wb");
-  if (!f_) throw std::runtime_error("Unable to open file " + path + ext[mode_]);
-  if (mode_ == 2) {  // always written (CsvContinuousColorResultsPrinter.cpp:60-75)
+  if (mode < 0 || mode > 2) throw std::runtime_error("Invalid value passed by user for argument print_mode");
+  f_ = fopen((path + ext[mode]).c_str(), "wb");
+  if (!f_) throw std::runtime_error("Unable to open file " + path + ext[mode]);
+  if (mode == 2) {
     std::string h;
-    for (u64 i = 1; i <= num_colors_; ++i) {
+    for (u64 i = 1; i <= num_colors; ++i) {
       h += std::to_string(i);
-      h += i == num_colors_ ? '\n' : ',';
+      h += i == num_colors ? '\n' : ',';
     }
     write_all(f_, h.data(), h.size());
   } else if (headers) {
-    write_sized_string(f_, mode_ == 0 ? "ascii" : "binary");
+    write_sized_string(f_, mode == 0 ? "ascii" : "binary");
     write_sized_string(f_, "v1.0");
   }
-  out_.reserve(kFlush + 64);
 }
 
-ColorWriter::~ColorWriter() {
-  try {
-    flush();
-  } catch (...) {
-  }
-  fclose(f_);
-}
+ColorFileOut::~ColorFileOut() { fclose(f_); }
 
-void ColorWriter::read(const u32 *ids, u64 n) {
-  if (mode_ == 0) {
-    for (u64 i = 0; i < n; ++i) {
-      if (i) out_.push_back(' ');
-      char tmp[16];
-      auto r = std::to_chars(tmp, tmp + sizeof tmp, ids[i]);
-      out_.insert(out_.end(), tmp, r.ptr);
-    }
-    out_.push_back('\n');
-  } else if (mode_ == 1) {
-    for (u64 i = 0; i < n; ++i) {
-      const u64 v = ids[i];
-      const char *p = reinterpret_cast<const char *>(&v);
-      out_.insert(out_.end(), p, p + 8);
-    }
-    const u64 end = ~0ULL;
-    const char *p = reinterpret_cast<const char *>(&end);
-    out_.insert(out_.end(), p, p + 8);
-  } else {
-    const size_t at = out_.size();
-    for (u64 c = 0; c < num_colors_; ++c) {
-      out_.push_back('0');
-      out_.push_back(c + 1 == num_colors_ ? '\n' : ',');
-    }
-    for (u64 i = 0; i < n; ++i) out_[at + 2 * ids[i]] = '1';
-  }
-  if (out_.size() >= kFlush) flush();
-}
+void ColorFileOut::write(const char *p, u64 n) { write_all(f_, p, n); }
 
-void ColorWriter::flush() {
-  write_all(f_, out_.data(), out_.size());
-  out_.clear();
-}
-
-// ----------------------------------------------------- index file reading
-IndexFile read_index_file(const std::string &path) {
-  std::ifstream in(path, std::ios::binary);
-  if (!in) throw std::runtime_error("Unable to open file " + path);
-  std::vector<char> data((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
-  size_t p = 0;
+// ------------------------------------------------------ index file reading
+IndexFileIn::IndexFileIn(const std::string &path) {
+  f_ = fopen(path.c_str(), "rb");
+  if (!f_) throw std::runtime_error("Unable to open file " + path);
   auto sized = [&]() {
-    u64 n;
-    if (p + 8 > data.size()) throw std::runtime_error("Invalid file format in file: " + path);
-    std::memcpy(&n, data.data() + p, 8);
-    p += 8;
-    if (n > 64 || p + n > data.size()) throw std::runtime_error("Invalid file format in file: " + path);
-    std::string s(data.data() + p, n);
-    p += n;
+    u64 n = 0;
+    if (fread(&n, 8, 1, f_) != 1 || n > 64) {
+      fclose(f_);
+      throw std::runtime_error("Invalid file format in file: " + path);
+    }
+    std::string s(n, '\0');
+    if (n && fread(&s[0], 1, n, f_) != n) {
+      fclose(f_);
+      throw std::runtime_error("Invalid file format in file: " + path);
+    }
     return s;
   };
   const std::string fmt = sized();
-  const std::string ver = sized();
-  if (ver != "v1.0") throw std::runtime_error("The file has an incompatible version number");
-  IndexFile out;
-  if (fmt == "ascii") {  // AsciiIndexFileParser.cpp:33-70
-    u64 n = 0;
-    while (p < data.size()) {
-      const char c = data[p++];
-      if (c == '-') {
-        const char d = p < data.size() ? data[p++] : 0;
-        if (d == '1') out.results.push_back(kNotFound);
-        else if (d == '2') out.results.push_back(kInvalid);
-        while (p < data.size() && data[p] >= '0') ++p;
-        ++n;
-      } else if (c >= '0') {
-        u64 v = static_cast<u64>(c - '0');
-        while (p < data.size() && data[p] >= '0') v = v * 10 + static_cast<u64>(data[p++] - '0');
-        out.results.push_back(v);
-        ++n;
-      } else if (c == '\n') {
-        out.kmer_ends.push_back(out.results.size());
-      }
-    }
-    (void)n;
-  } else if (fmt == "binary") {  // BinaryIndexFileParser.cpp:33-63
-    while (p + 8 <= data.size()) {
-      u64 v;
-      std::memcpy(&v, data.data() + p, 8);
-      p += 8;
-      if (v == kNewline) out.kmer_ends.push_back(out.results.size());
-      else out.results.push_back(v);
-    }
-  } else {
+  if (fmt == "ascii") format_ = 0;
+  else if (fmt == "binary") format_ = 1;
+  else {
+    fclose(f_);
     throw std::runtime_error("Invalid file format in file: " + path);
   }
-  return out;
+  if (sized() != "v1.0") {
+    fclose(f_);
+    throw std::runtime_error("The file has an incompatible version number");
+  }
+}
+
+IndexFileIn::~IndexFileIn() { fclose(f_); }
+
+u64 IndexFileIn::read(char *p, u64 n) {
+  u64 got = 0;
+  while (got < n) {
+    const size_t m = fread(p + got, 1, n - got, f_);
+    if (m == 0) {
+      if (ferror(f_)) throw std::runtime_error("read failed");
+      break;
+    }
+    got += m;
+  }
+  return got;
 }
 
 std::vector<std::string> expand_list(const std::string &name) {

@@ -52,31 +52,37 @@ class IndexFileOut {
   FILE *f_;
 };
 
-// Output writers for the color stage (src/ColorResultsPrinter/*): ascii v1.0
-// ("c1 c2 ...\n"), binary v1.0 (u64 ids, u64 max per read), csv (header
-// 1..num_colors, one 0/1 row per read).
-class ColorWriter {
+// Color output file (src/ColorResultsPrinter/*): ascii and binary start with
+// the header (u64 length + "ascii"/"binary", u64 length + "v1.0") unless
+// headers are off; csv always starts with the row "1,2,...,num_colors"
+// (CsvContinuousColorResultsPrinter.cpp:60-75). Then the printer bytes
+// (formatted on the device). mode: 0 ascii (.txt), 1 binary (.bin), 2 csv (.csv).
+class ColorFileOut {
  public:
-  ColorWriter(const std::string &path, const std::string &mode, bool headers, u64 num_colors);
-  ~ColorWriter();
-  void read(const u32 *ids, u64 n);
-  void flush();
+  ColorFileOut(const std::string &path, int mode, bool headers, u64 num_colors);
+  ~ColorFileOut();
+  void write(const char *p, u64 n);
 
  private:
   FILE *f_;
-  int mode_;  // 0 ascii, 1 binary, 2 csv
-  u64 num_colors_;
-  std::vector<char> out_;
 };
 
-// Index-stage result file (ascii or binary, with headers) read back as one
-// result per k-mer (index, u64 max not found, u64 max - 1 invalid) and the
-// number of k-mers of every read: src/IndexFileParser/{Ascii,Binary}IndexFileParser.cpp.
-struct IndexFile {
-  std::vector<u64> results;
-  std::vector<u64> kmer_ends;
+// Index-result file opened for the colors stage: its two sized header strings
+// are read and checked (ContinuousIndexFileParser.cpp:84-99,
+// AsciiIndexFileParser.cpp:25-30, BinaryIndexFileParser.cpp:25-30); read()
+// then returns body bytes.
+class IndexFileIn {
+ public:
+  explicit IndexFileIn(const std::string &path);
+  ~IndexFileIn();
+  int format() const { return format_; }  // 0 ascii, 1 binary
+  // Up to n body bytes into p; fewer only at the end of the file.
+  u64 read(char *p, u64 n);
+
+ private:
+  FILE *f_;
+  int format_ = 0;
 };
-IndexFile read_index_file(const std::string &path);
 
 void write_sized_string(FILE *f, const std::string &s);
 std::vector<std::string> expand_list(const std::string &name);

@@ -4,7 +4,7 @@
 //                      [--batch-chars N]
 //   sbwt_search colors -q <index results|.list> -k <.tcolors> -o <prefix|.list>
 //                      [-p ascii|binary|csv] [-t threshold] [--include-not-found]
-//                      [--include-invalid] [--no-headers] [-s streams]
+//                      [--include-invalid] [--no-headers] [-s streams] [--batch-bytes N]
 // Option names and defaults follow src/ArgumentParser/IndexSearchArgumentParser.cpp:29-178
 // and ColorSearchArgumentParser.cpp:27-180; the memory options (-u -m -c -g -r)
 // are accepted and bound the batch sizes.
@@ -404,50 +404,195 @@ int run_index(const Args &a) {
 }
 
 // --------------------------------------------------------------- colors
-void colors_stream(sbwt_color_index *ci, u64 num_colors, const std::vector<std::string> &in,
-                   const std::vector<std::string> &out, const std::vector<u64> &files,
-                   const std::string &mode, bool headers, const sbwt_color_params &params) {
-  sbwt_color_searcher *cs = nullptr;
-  bool first_batch = true;
-  for (u64 f : files) {
-    sbwt_host::IndexFile idx = sbwt_host::read_index_file(in[f]);
-    sbwt_host::ColorWriter writer(out[f], mode, headers, num_colors);
-    const u64 n = idx.kmer_ends.size();
-    if (n == 0) continue;
-    if (cs) sbwt_color_searcher_destroy(cs);
-    check(sbwt_color_searcher_create(ci, idx.results.size() + 64, n, nullptr, &cs));
-    sbwt_color_params p = params;
-    p.first_read_quirk = first_batch ? 1 : 0;  // the reference's carry starts once per stream
-    first_batch = false;
-    std::vector<u64> offsets(n + 1);
-    u64 cap = 0;
-    for (u64 r = 0, st = 0; r < n; ++r) {  // ids <= min(colors, found) per read
-      cap += std::min<u64>(num_colors, idx.kmer_ends[r] - st);
-      st = idx.kmer_ends[r];
-    }
-    std::vector<u32> ids(std::max<u64>(cap, 1));
-    u64 nids = 0;
-    check(sbwt_color_searcher_pseudoalign(cs, idx.results.data(), idx.kmer_ends.data(), n, &p,
-                                          offsets.data(), ids.data(), cap, nullptr, &nids));
-    for (u64 r = 0; r < n; ++r) writer.read(ids.data() + offsets[r], offsets[r + 1] - offsets[r]);
+// One colors batch: body bytes of one index-result file that end at a read
+// end (or at the end of the file).
+struct ColorInBatch {
+  u64 file = 0;
+  bool first = false, last = false;  // first / last batch of its file
+  int format = 0;
+  Pinned body;
+  u64 n = 0;
+};
+
+// Whether bytes of an index-file body may hold a read end: a '\n' in ascii
+// (not every '\n' is one), a u64 max - 2 word in binary.
+bool holds_read_end(int format, const char *p, u64 n) {
+  if (format == SBWT_INDEX_FILE_ASCII) return std::memchr(p, '\n', n) != nullptr;
+  for (u64 w = 0; w + 8 <= n; w += 8) {
+    u64 v;
+    std::memcpy(&v, p + w, 8);
+    if (v == ~0ULL - 2) return true;
   }
-  if (cs) sbwt_color_searcher_destroy(cs);
+  return false;
+}
+
+// One stream of the colors command, three stages on three threads: the reader
+// cuts index-file bodies into batches at read ends; the device stage parses a
+// batch, runs the color search and prints the color ids into the output
+// bytes, all on the device; the writer appends the bytes to the file.
+void colors_stream(sbwt_color_index *ci, u64 num_colors, const std::vector<std::string> &in,
+                   const std::vector<std::string> &out, const std::vector<u64> &files, int mode,
+                   bool headers, const sbwt_color_params &params, u64 max_bytes, u64 max_reads) {
+  constexpr size_t kBuffers = 3;
+  Channel<ColorInBatch> to_device(1);
+  Channel<OutChunk> to_writer(1);
+  Channel<Pinned> free_in(kBuffers), free_out(kBuffers);
+  for (size_t i = 0; i < kBuffers; ++i) {
+    Pinned in_buf;
+    in_buf.ensure(max_bytes + 64);
+    free_in.push(in_buf);
+    free_out.push(Pinned());
+  }
+  std::string error;
+  std::mutex error_mu;
+  auto fail = [&](const std::exception &e) {
+    {
+      std::lock_guard<std::mutex> g(error_mu);
+      if (error.empty()) error = e.what();
+    }
+    to_device.abort();
+    to_writer.abort();
+    free_in.abort();
+    free_out.abort();
+  };
+
+  std::thread device([&] {
+    sbwt_color_file_searcher *fs = nullptr;
+    try {
+      check(sbwt_color_file_searcher_create(ci, max_bytes, max_reads, nullptr, &fs));
+      // the reference's printer carries u64 max into the first read of the
+      // stream (ContinuousColorResultsPrinter.hpp:62-64,175-180)
+      bool first_read = true;
+      ColorInBatch b;
+      while (to_device.pop(b)) {
+        Clock clk;
+        OutChunk o;
+        o.file = b.file;
+        o.first = b.first;
+        o.last = b.last;
+        if (!free_out.pop(o.bytes)) break;
+        sbwt_color_params p = params;
+        p.first_read_quirk = first_read ? 1 : 0;
+        u64 reads = 0;
+        const u64 want = std::max<u64>(b.n * 2, 1 << 16);
+        o.bytes.ensure(want);
+        check(sbwt_color_file_searcher_batch(fs, b.format, b.body.p, b.n, &p, mode, o.bytes.p,
+                                             o.bytes.cap, &o.n, &reads));
+        if (o.n > o.bytes.cap) {
+          o.bytes.ensure(o.n + o.n / 4);
+          check(sbwt_color_file_searcher_copy_last(fs, o.bytes.p, o.bytes.cap));
+        }
+        if (reads) first_read = false;
+        if (!free_in.push(b.body)) break;
+        g_times.search_ns += clk.lap();
+        if (!to_writer.push(o)) break;
+      }
+      to_writer.close();
+    } catch (const std::exception &e) {
+      fail(e);
+    }
+    if (fs) sbwt_color_file_searcher_destroy(fs);
+  });
+
+  std::thread writer([&] {
+    try {
+      std::unique_ptr<sbwt_host::ColorFileOut> f;
+      OutChunk o;
+      while (to_writer.pop(o)) {
+        Clock clk;
+        if (o.first) f = std::make_unique<sbwt_host::ColorFileOut>(out[o.file], mode, headers, num_colors);
+        if (o.n) f->write(o.bytes.p, o.n);
+        if (o.last) f.reset();
+        g_times.format_ns += clk.lap();
+        if (!free_out.push(o.bytes)) break;
+      }
+    } catch (const std::exception &e) {
+      fail(e);
+    }
+  });
+
+  // reader (this thread)
+  try {
+    bool ok = true;
+    for (size_t fi = 0; ok && fi < files.size(); ++fi) {
+      const u64 f = files[fi];
+      sbwt_host::IndexFileIn file(in[f]);
+      ColorInBatch b;
+      ok = free_in.pop(b.body);
+      b.file = f;
+      b.first = true;
+      b.format = file.format();
+      bool eof = false;
+      Clock clk;
+      while (ok) {
+        if (!eof) {
+          const u64 got = file.read(b.body.p + b.n, max_bytes - b.n);
+          eof = b.n + got < max_bytes;
+          b.n += got;
+        }
+        u64 cut = sbwt_index_file_batch_cut(b.format, b.body.p, b.n, max_reads);
+        if (eof) {
+          // the rest is the file's last batch unless it holds more read
+          // ends than a batch may (then cut after the first max_reads)
+          const bool more = cut > 0 && cut < b.n && holds_read_end(b.format, b.body.p + cut, b.n - cut);
+          if (!more) {
+            b.last = true;
+            g_times.parse_ns += clk.lap();
+            ok = to_device.push(std::move(b));
+            break;
+          }
+        }
+        if (cut == 0)
+          throw std::runtime_error("no read end within one batch (" + std::to_string(max_bytes) +
+                                   " bytes) of index file " + in[f] + "; raise --batch-bytes");
+        ColorInBatch next;
+        if (!free_in.pop(next.body)) {
+          ok = false;
+          break;
+        }
+        next.file = f;
+        next.format = b.format;
+        next.n = b.n - cut;
+        if (next.n) std::memcpy(next.body.p, b.body.p + cut, next.n);
+        b.n = cut;
+        g_times.parse_ns += clk.lap();
+        Pinned sent = b.body;  // back to the pool through the device stage, or freed here
+        ok = to_device.push(std::move(b));
+        b = std::move(next);
+        if (!ok) sent.release();
+      }
+      if (!ok && b.body.p) b.body.release();
+    }
+    to_device.close();
+  } catch (const std::exception &e) {
+    fail(e);
+  }
+  device.join();
+  writer.join();
+  for (auto &x : free_in.drain()) x.release();
+  for (auto &x : free_out.drain()) x.release();
+  for (auto &x : to_device.drain()) x.body.release();
+  for (auto &x : to_writer.drain()) x.bytes.release();
+  if (!error.empty()) throw std::runtime_error(error);
 }
 
 int run_colors(const Args &a) {
   const std::string q = a.get("query-file"), k = a.get("colors-file"), o = a.get("output-prefix");
   if (q.empty() || k.empty() || o.empty())
     throw std::runtime_error("query-file, colors-file and output-prefix are required");
-  const std::string mode = a.get("print-mode", "ascii");
-  if (mode != "ascii" && mode != "binary" && mode != "csv")
+  const std::string mode_name = a.get("print-mode", "ascii");
+  if (mode_name != "ascii" && mode_name != "binary" && mode_name != "csv")
     throw std::runtime_error("Invalid value passed by user for argument print_mode");
+  const int mode = mode_name == "ascii" ? SBWT_COLOR_FORMAT_ASCII
+                   : mode_name == "binary" ? SBWT_COLOR_FORMAT_BINARY : SBWT_COLOR_FORMAT_CSV;
   const double t = std::stod(a.get("threshold", "1"));
   if (t < 0 || t > 1) {
     std::fprintf(stderr, "Invalid value for threshold, must be between 1 and 0 (both included)\n");
     std::exit(1);
   }
   fraction(a, "cpu-memory-percentage", "0.8", "Invalid value for cpu-memory-percentage. Must be between 0 and 1.");
-  fraction(a, "gpu-memory-percentage", "0.95", "Invalid value for gpu-memory-percentage. Must be between 0 and 1.");
+  const double gpu = fraction(a, "gpu-memory-percentage", "0.95",
+                              "Invalid value for gpu-memory-percentage. Must be between 0 and 1.");
   const auto in = sbwt_host::expand_list(q), out = sbwt_host::expand_list(o);
   if (in.size() != out.size()) throw std::runtime_error("Input and output file sizes differ");
   const u64 streams = std::max<u64>(1, std::min<u64>(in.size(), std::stoull(a.get("streams", "4"))));
@@ -456,13 +601,26 @@ int run_colors(const Args &a) {
   sbwt_color_index_info info;
   check(sbwt_color_index_get_info(ci, &info));
   sbwt_color_params params{t, a.flag("include-not-found"), a.flag("include-invalid"), 0};
+  // Batch bytes per stream from free device memory: body, results and ids
+  // (about 7 bytes per body byte) plus the printer output; reads per batch
+  // bounded by the per-read color bitmap.
+  u64 free_b = 0, total = 0;
+  check(sbwt_mem_info(&free_b, &total));
+  const u64 budget = static_cast<u64>(static_cast<double>(free_b) * gpu) / streams;
+  u64 max_bytes = std::min<u64>(budget / 16, 64ULL << 20);
+  if (!a.get("batch-bytes").empty())  // an extension: caps the bytes per batch
+    max_bytes = std::min<u64>(max_bytes, std::stoull(a.get("batch-bytes")));
+  if (max_bytes < 16) throw std::runtime_error("Not enough memory");
+  const u64 words = (info.num_colors + 63) / 64;
+  const u64 max_reads = std::max<u64>(1, std::min<u64>(max_bytes, (256ULL << 20) / (8 * std::max<u64>(words, 1))));
   const auto parts = sbwt_host::partition_by_size(in, streams);
   std::vector<std::thread> th;
   std::vector<std::string> errors(streams);
   for (u64 p = 0; p < streams; ++p)
     th.emplace_back([&, p] {
       try {
-        colors_stream(ci, info.num_colors, in, out, parts[p], mode, !a.flag("no-headers"), params);
+        colors_stream(ci, info.num_colors, in, out, parts[p], mode, !a.flag("no-headers"), params,
+                      max_bytes, max_reads);
       } catch (const std::exception &e) {
         errors[p] = e.what();
       }
@@ -471,6 +629,10 @@ int run_colors(const Args &a) {
   sbwt_color_index_destroy(ci);
   for (auto &e : errors)
     if (!e.empty()) throw std::runtime_error(e);
+  if (std::getenv("SBWT_SEARCH_TIMING"))
+    std::fprintf(stderr, "stage busy seconds (summed over %llu streams; stages overlap): read %.3f device %.3f write %.3f\n",
+                 static_cast<unsigned long long>(streams), g_times.parse_ns * 1e-9,
+                 g_times.search_ns * 1e-9, g_times.format_ns * 1e-9);
   return 0;
 }
 

@@ -92,6 +92,11 @@ def lib():
         L.orc_color_min_hits.argtypes = [C.c_uint64, C.c_uint64, C.c_uint64, C.c_double, C.c_int,
                                          C.c_int]
         L.orc_color_min_hits.restype = C.c_uint64
+        L.orc_parse_index_body.argtypes = [C.c_char_p, C.c_uint64, C.c_int, u64p, u64p, u64p]
+        L.orc_parse_index_body.restype = C.c_uint64
+        L.orc_color_lists_format.argtypes = [u64p, u32p, C.c_uint64, C.c_uint64, C.c_int,
+                                             C.c_char_p]
+        L.orc_color_lists_format.restype = C.c_uint64
         _lib = L
     return _lib
 
@@ -266,3 +271,25 @@ def ascii_index_output(results, read_ends, k, headers=True):
         i += nk
         start = e
     return bytes(out)
+
+
+def parse_index_body(body, fmt):
+    """Oracle parse of an index-result file body (0 ascii, 1 binary) ->
+    (results u64[], kmer_ends u64[] per read)."""
+    n = len(body)
+    res = np.zeros(n + 2, np.uint64)
+    ends = np.zeros(n + 2, np.uint64)
+    nres = np.zeros(1, np.uint64)
+    nr = lib().orc_parse_index_body(bytes(body), n, fmt, p64(res), p64(nres), p64(ends))
+    return res[:int(nres[0])].copy(), ends[:int(nr)].copy()
+
+
+def color_lists_format(offsets, ids, num_colors, mode):
+    """Oracle bytes of color id lists (0 ascii, 1 binary, 2 csv; no headers)."""
+    offsets = np.ascontiguousarray(offsets, np.uint64)
+    ids = np.ascontiguousarray(ids if len(ids) else np.zeros(1), np.uint32)
+    n = len(offsets) - 1
+    size = lib().orc_color_lists_format(p64(offsets), p32(ids), n, num_colors, mode, None)
+    buf = C.create_string_buffer(max(int(size), 1))
+    lib().orc_color_lists_format(p64(offsets), p32(ids), n, num_colors, mode, buf)
+    return buf.raw[:int(size)]

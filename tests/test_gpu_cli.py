@@ -89,3 +89,52 @@ def test_colors_cli_matches_oracle(tmp_path):
                           first_read_quirk=True)
     body = open(str(tmp_path / "col.txt"), "rb").read()[25:].decode().split("\n")[:-1]
     assert [[int(x) for x in line.split()] for line in body] == exp
+
+
+@pytest.mark.parametrize("in_mode", ["ascii", "binary"])
+def test_colors_cli_formats_batches_and_streams(tmp_path, in_mode):
+    """colors over a .list of index files (ascii or binary) in every output format,
+    on one stream with small batches (reads of a file spread over many batches) and
+    on one stream per file; every output file equals the oracle's."""
+    import struct
+    from colorlib import ColoredIndex, kmer_ends_of, read_stats, threshold_lists
+    ci = ColoredIndex(tmp_path, n_genomes=6, length=15_000, d=4)
+    ext_in = ".txt" if in_mode == "ascii" else ".bin"
+    files = []
+    for i in range(3):
+        chars, ends = ci.reads(150 + 60 * i, 120, seed=20 + i)
+        fa = tmp_path / f"r{i}.fna"
+        st = 0
+        with open(fa, "wb") as f:
+            for j, e in enumerate(ends):
+                f.write(b">r%d\n" % j + chars[st:int(e)] + b"\n")
+                st = int(e)
+        run(["index", "-q", str(fa), "-i", ci.sbwt_path, "-k", ci.colors_path, "-o",
+             str(tmp_path / f"idx{i}"), "-p", in_mode])
+        res = ci.index_results(chars, ends)
+        ke = kmer_ends_of(ends)
+        files.append((str(tmp_path / f"idx{i}") + ext_in, res, ke))
+    (tmp_path / "in.list").write_text("\n".join(f for f, _, _ in files) + "\n")
+    outs = [str(tmp_path / f"col{i}") for i in range(3)]
+    (tmp_path / "out.list").write_text("\n".join(outs) + "\n")
+    nc = ci.num_colors
+    # one stream takes its files largest first (the size-balanced partition)
+    first_on_one_stream = max(range(3), key=lambda i: os.path.getsize(files[i][0]))
+    for streams, extra in (("1", ["--batch-bytes", "3000"]), ("3", [])):
+        for mode, ext, m in (("ascii", ".txt", 0), ("binary", ".bin", 1), ("csv", ".csv", 2)):
+            run(["colors", "-q", str(tmp_path / "in.list"), "-k", ci.colors_path, "-o",
+                 str(tmp_path / "out.list"), "-p", mode, "-t", "0.7", "-s", streams] + extra)
+            for i, (_, res, ke) in enumerate(files):
+                # the carried u64 max reaches the first read of each stream only
+                quirk = i == first_on_one_stream or streams == "3"
+                lists = threshold_lists(ci.oracle_counts(res, ke), read_stats(res, ke), 0.7,
+                                        first_read_quirk=quirk)
+                offsets = np.cumsum([0] + [len(x) for x in lists]).astype(np.uint64)
+                ids = np.array([c for x in lists for c in x], np.uint32)
+                if m == 2:
+                    head = (",".join(str(c) for c in range(1, nc + 1)) + "\n").encode()
+                else:
+                    name = b"ascii" if m == 0 else b"binary"
+                    head = struct.pack("<Q", len(name)) + name + struct.pack("<Q", 4) + b"v1.0"
+                want = head + O.color_lists_format(offsets, ids, nc, m)
+                assert open(outs[i] + ext, "rb").read() == want, (streams, mode, i)

@@ -143,3 +143,143 @@ def test_warp_layout_on_reference_index_files(name):
         assert np.all(blk[len(reads[i]["found"]):] == O.NOT_FOUND)
     for i, r in enumerate(reads):
         assert (int(f[i]), int(nf[i]), int(inv[i])) == (len(r["found"]), r["nf"], r["inv"])
+
+
+# ---------------------------------------------------------------------------
+# Index-result file bodies (AsciiIndexFileParser.cpp:33-98,
+# BinaryIndexFileParser.cpp:33-70) and color printers (Ascii/Binary/Csv
+# ContinuousColorResultsPrinter.cpp).
+
+def _ascii_body(results, kmer_ends):
+    """Index results as the ascii printer writes them (pinned against the
+    reference's expected index files by test_oracle_pins)."""
+    import sys
+    out = []
+    st = 0
+    for e in kmer_ends:
+        toks = []
+        for v in results[st:int(e)]:
+            v = int(v)
+            toks.append(b"-2" if v == int(O.INVALID) else b"-1" if v == int(O.NOT_FOUND) else
+                        str(v).encode())
+        out.append(b" ".join(toks) + b"\n")
+        st = int(e)
+    return b"".join(out)
+
+
+def _binary_body(results, kmer_ends):
+    words = []
+    st = 0
+    for e in kmer_ends:
+        words += [int(v) for v in results[st:int(e)]] + [int(O.NEWLINE)]
+        st = int(e)
+    return np.asarray(words, np.uint64).tobytes()
+
+
+def _random_results(seed, n_reads):
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(0, 40, n_reads)
+    lens[rng.random(n_reads) < 0.2] = 0  # reads shorter than k: empty lines
+    total = int(lens.sum())
+    vals = rng.integers(0, 1 << 63, total, dtype=np.uint64) >> rng.integers(0, 63, total).astype(np.uint64)
+    pick = rng.random(total)
+    vals[pick < 0.15] = O.NOT_FOUND
+    vals[(pick >= 0.15) & (pick < 0.25)] = O.INVALID
+    vals[pick > 0.99] = np.uint64((1 << 64) - 4)  # the largest found index the formats allow
+    return vals, np.cumsum(lens).astype(np.uint64)
+
+
+@pytest.mark.parametrize("name", ["fasta1", "fasta2", "fasta3", "fasta4"])
+def test_parse_index_body_on_reference_files(name):
+    """The reference's own color-search inputs: as many reads as its expected colors
+    files have lines, and per read the same found / not-found / invalid tokens."""
+    import os
+    from conftest import FIXTURES
+    data = open(os.path.join(FIXTURES, "color_search", f"{name}.indexes.txt"), "rb").read()
+    res, ends = O.parse_index_body(data[25:], 0)
+    exp_lines = open(os.path.join(FIXTURES, "color_search", "expected", f"{name}.colors.txt"),
+                     "rb").read()[25:].split(b"\n")[:-1]
+    assert len(ends) == len(exp_lines)
+    st = 0
+    for line, e in zip(data[25:].split(b"\n")[:-1], ends):
+        toks = line.split()
+        got = [int(v) for v in res[st:int(e)]]
+        want = [int(O.NOT_FOUND) if t == b"-1" else int(O.INVALID) if t == b"-2" else int(t)
+                for t in toks]
+        assert got == want
+        st = int(e)
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_parse_index_body_round_trip(seed):
+    """Printer bytes -> parser -> the same results and read structure, ascii and
+    binary (the two formats must agree)."""
+    res, ke = _random_results(seed, 300)
+    for fmt, body in ((0, _ascii_body(res, ke)), (1, _binary_body(res, ke))):
+        r2, e2 = O.parse_index_body(body, fmt)
+        assert np.array_equal(r2, res) and np.array_equal(e2, ke), fmt
+
+
+def test_parse_index_body_known_answers():
+    """Inputs outside what the printers write, answered by hand from the reference's
+    char loop (AsciiIndexFileParser.cpp:46-63 with :85-98)."""
+    NF, INV = int(O.NOT_FOUND), int(O.INVALID)
+    cases = [
+        # '-' takes the next char, skips digits, consumes one more char ('-' here)
+        (b"-1-1 5-2\n--\n3\n-\n4 9\n", [NF, 1, 5, INV, 3, 9], [4, 4, 5, 6]),
+        (b"a\n", [ord("a") - ord("0")], [1]),          # any char >= '0' is a digit
+        (b"1\x802\n", [1, 2], [2]),                     # bytes >= 0x80 are negative chars
+        (b"1 2\n3", [1, 2, 3], [2]),                    # a tail with no read end
+        (b"-3 -12 -21\n", [NF, INV], [2]),              # only the char after '-' counts
+        (b"\n\n7\n", [7], [0, 0, 1]),
+        (b"-\n5\n", [], [0]),                           # '\n' after '-' is swallowed; 5 skipped
+        (b"18446744073709551616 99\n", [0, 99], [2]),  # u64 arithmetic wraps
+        (b"", [], []),
+    ]
+    for body, res, ends in cases:
+        r, e = O.parse_index_body(body, 0)
+        assert [int(x) for x in r] == res and [int(x) for x in e] == ends, body
+    words = np.array([5, int(O.NEWLINE), NF, INV, int(O.NEWLINE), 7], np.uint64).tobytes()
+    r, e = O.parse_index_body(words + b"\x01\x02", 1)  # a partial last word is not read
+    assert [int(x) for x in r] == [5, NF, INV, 7] and [int(x) for x in e] == [1, 3]
+
+
+@pytest.mark.parametrize("name", ["fasta1", "fasta2", "fasta3", "fasta4"])
+def test_color_lists_format_on_reference_files(name):
+    """Ascii color printer against the reference's expected colors files (lines
+    compared without trailing spaces, as the reference's own verifier does)."""
+    import os
+    from conftest import FIXTURES
+    exp = open(os.path.join(FIXTURES, "color_search", "expected", f"{name}.colors.txt"),
+               "rb").read()[25:]
+    lines = [[int(t) for t in ln.split()] for ln in exp.split(b"\n")[:-1]]
+    offsets = np.cumsum([0] + [len(x) for x in lines]).astype(np.uint64)
+    ids = np.array([c for x in lines for c in x] or [0], np.uint32)
+    got = O.color_lists_format(offsets, ids[:int(offsets[-1])], 3, 0)
+    assert [ln.rstrip(b" ") for ln in got.split(b"\n")] == \
+        [ln.rstrip(b" ") for ln in exp.split(b"\n")]
+
+
+def test_color_lists_format_binary_and_csv():
+    """Binary: u64 ids then u64 max per read; csv: the indicator row of each read."""
+    rng = np.random.default_rng(5)
+    nc = 13
+    lists = [sorted(rng.choice(nc, rng.integers(0, nc + 1), replace=False).tolist())
+             for _ in range(50)]
+    offsets = np.cumsum([0] + [len(x) for x in lists]).astype(np.uint64)
+    ids = np.array([c for x in lists for c in x], np.uint32)
+    words = np.frombuffer(O.color_lists_format(offsets, ids, nc, 1), np.uint64)
+    back, cur = [], []
+    for w in words:
+        if w == np.uint64(~np.uint64(0)):
+            back.append(cur)
+            cur = []
+        else:
+            cur.append(int(w))
+    assert back == lists
+    rows = O.color_lists_format(offsets, ids, nc, 2).split(b"\n")[:-1]
+    assert len(rows) == len(lists)
+    for row, x in zip(rows, lists):
+        assert row.split(b",") == [b"1" if c in x else b"0" for c in range(nc)]
+    asc = O.color_lists_format(offsets, ids, nc, 0).split(b"\n")[:-1]
+    assert [[int(t) for t in ln.split()] for ln in asc] == lists
