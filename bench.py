@@ -40,11 +40,12 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--config", default="c2", choices=["c1", "c2"])
     ap.add_argument("--mode", default="index",
-                    choices=["index", "batched", "serve", "colors", "cli"],
+                    choices=["index", "batched", "serve", "colors", "cli", "cli-colors"],
                     help="index: the headline line (default); batched: C4 plain vs batched "
                          "form; serve: Poisson request stream, served vs plain sequential; "
                          "colors: the color stage on the C3 recipe; cli: the sbwt_search "
-                         "command end to end on C2 written as FASTA")
+                         "command end to end on C2 written as FASTA; cli-colors: "
+                         "`sbwt_search colors` on C3 index-result files (ascii and binary)")
     ap.add_argument("--color-genomes", type=int, default=200)
     ap.add_argument("--color-roots", type=int, default=20)
     ap.add_argument("--color-min-len", type=int, default=2_000_000)
@@ -115,6 +116,8 @@ def main():
         return mode_cli(args, configs, S, torch, W)
     if args.mode == "colors":
         return mode_colors(args, configs, S, torch, W)
+    if args.mode == "cli-colors":
+        return mode_cli_colors(args, configs, S, torch, W)
     t0 = time.time()
     if args.config == "c1":
         wl = configs.c1(**({"n_reads": args.reads} if args.reads else {}))
@@ -450,6 +453,79 @@ def mode_cli(args, configs, S, torch, W):
         os.remove(out_file)
     os.remove(fasta)
     os.remove(sbwt_path)
+    os.rmdir(tmp)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def mode_cli_colors(args, configs, S, torch, W):
+    """C3 recipe through `sbwt_search colors`: index-result files (written by the
+    index stage with the key-k-mer move, ascii and binary) to color files."""
+    import subprocess
+    import tempfile
+    t0 = time.time()
+    wl = configs.c3(args.color_reads, args.color_genomes, args.color_roots, args.color_min_len,
+                    args.color_max_len)
+    built = W.BuiltSbwt(wl.genomes, wl.genome_ends, configs.K)
+    log(f"workload {wl.name}: {wl.n_reads} reads, SBWT {built.n} nodes ({time.time() - t0:.1f} s)")
+    tmp = tempfile.mkdtemp(prefix="sbwt_clicol_")
+    col = built.colors(np.arange(len(wl.genome_ends), dtype=np.uint32), d=20)
+    colors_path = os.path.join(tmp, "index.tcolors")
+    col.write(colors_path)
+    idx = S.Index.create([built.bits(c) for c in range(4)], built.n, 31, key_marks=col.key_marks(),
+                         suffix_group_starts=built.bits(4))
+    dev = torch.device("cuda:0")
+    s = idx.searcher(wl.n_chars, wl.n_reads, move_to_key_kmer=True)
+    d_chars = torch.from_numpy(wl.chars).to(dev)
+    d_ends = torch.from_numpy(wl.read_ends.view(np.int64)).to(dev)
+    d_res = torch.empty(wl.n_chars, dtype=torch.int64, device=dev)
+    s.search_reads_device(d_chars.data_ptr(), wl.n_chars, d_ends.data_ptr(), wl.n_reads,
+                          d_res.data_ptr())
+    s.sync()
+    lens = np.diff(np.concatenate([[0], wl.read_ends.astype(np.int64)]))
+    kends = np.cumsum(np.maximum(lens - configs.K + 1, 0)).astype(np.uint64)
+    n_kmers = int(kends[-1])
+    res = d_res[:n_kmers].cpu().numpy().view(np.uint64)
+    del d_chars, d_ends, d_res
+    exe = os.path.join(REPO, "sbwt-search_amd", "host", "sbwt_search")
+    out = {"metric": "sbwt_search colors end to end (index-result file to colors file)",
+           "mode": "cli-colors",
+           "config": {"workload": wl.name + " " + wl.description, "sbwt_nodes": int(built.n),
+                      "num_colors": int(col.num_colors), "kmers": n_kmers, "threshold": 0.7}}
+    fmt = S.Formatter(n_kmers, wl.n_reads)
+    closes = np.ones(wl.n_reads, np.uint8)
+    for in_mode, mode_id, ext in (("ascii", S.FORMAT_ASCII, ".txt"),
+                                  ("binary", S.FORMAT_BINARY, ".bin")):
+        body = fmt.format(mode_id, res, kends, closes)
+        in_path = os.path.join(tmp, "idx" + ext)
+        with open(in_path, "wb") as f:
+            name = in_mode.encode()
+            f.write(len(name).to_bytes(8, "little") + name + (4).to_bytes(8, "little") + b"v1.0")
+            f.write(body)
+        in_bytes = len(body) + 8 + len(in_mode) + 12
+        del body
+        for out_mode, out_ext in (("ascii", ".txt"), ("binary", ".bin"), ("csv", ".csv")):
+            if out_mode != "csv" and out_mode != in_mode:
+                continue
+            prefix = os.path.join(tmp, "col")
+            env = dict(os.environ, SBWT_SEARCH_TIMING="1")
+            t0 = time.perf_counter()
+            r = subprocess.run([exe, "colors", "-q", in_path, "-k", colors_path, "-o", prefix,
+                                "-p", out_mode, "-t", "0.7"], env=env, capture_output=True,
+                               text=True)
+            dt = time.perf_counter() - t0
+            if r.returncode != 0:
+                raise SystemExit(f"sbwt_search colors failed: {r.stderr}")
+            size = os.path.getsize(prefix + out_ext)
+            key = f"{in_mode}_to_{out_mode}"
+            log(f"cli colors {key}: {dt:.2f} s, {r.stderr.strip()}")
+            out[key] = {"seconds": dt, "reads_per_s": wl.n_reads / dt,
+                        "kmers_per_s": n_kmers / dt, "input_bytes": in_bytes,
+                        "input_MB_per_s": in_bytes / dt / 1e6, "output_bytes": size,
+                        "stages": r.stderr.strip()}
+            os.remove(prefix + out_ext)
+        os.remove(in_path)
+    os.remove(colors_path)
     os.rmdir(tmp)
     print(json.dumps(out), flush=True)
     return 0

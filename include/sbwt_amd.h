@@ -162,15 +162,20 @@ int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[16]);
 
 /* Search algorithm of the reads path: SBWT_ALGO_STREAMING steps each k-mer of
  * a read from the previous k-mer's node and searches from the presearch table
- * only after a miss (throughput: one line per found k-mer); SBWT_ALGO_PER_KMER
- * searches every k-mer independently, as the reference's d_search does
+ * only after a miss (throughput: about one line per found k-mer), in work
+ * items of the launch shape's chunk_kmers; SBWT_ALGO_PER_KMER searches every
+ * k-mer independently, as the reference's d_search does
  * (IndexSearcher.cuh:22-72) (latency: a small batch takes one search's
- * time); SBWT_ALGO_AUTO (default) takes streaming for batches of at least
- * 2^21 chars and per k-mer below. All return identical results. No
- * counterpart in the reference. */
+ * time); SBWT_ALGO_AUTO (default) searches per k-mer below 2^22 chars and
+ * streams above, with work items shrunk (down to 8 k-mers) until every
+ * persistent lane has one; SBWT_ALGO_STREAMING_ADAPTIVE streams with those
+ * shrunk work items from 2^16 chars (per k-mer below), for batches that
+ * share the GPU with others. All return identical results. No counterpart in
+ * the reference. */
 #define SBWT_ALGO_STREAMING 0
 #define SBWT_ALGO_PER_KMER 1
 #define SBWT_ALGO_AUTO 2
+#define SBWT_ALGO_STREAMING_ADAPTIVE 3
 int sbwt_searcher_set_algorithm(sbwt_searcher *searcher, int algorithm);
 
 /* Launch shape of the streaming search: k-mers per work item (chunk_kmers in

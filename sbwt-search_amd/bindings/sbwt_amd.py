@@ -16,7 +16,8 @@ HEADER = os.path.join(REPO, "include", "sbwt_amd.h")
 
 ALGO_STREAMING = 0
 ALGO_PER_KMER = 1
-ALGO_AUTO = 2  # default: streaming from 2^21 chars per batch, per k-mer below
+ALGO_AUTO = 2  # default: per k-mer below 2^22 chars per batch, streaming (adaptive work items) above
+ALGO_STREAMING_ADAPTIVE = 3  # adaptive work items from 2^16 chars: batches sharing the GPU
 NOT_FOUND = 0xFFFFFFFFFFFFFFFF
 INVALID = 0xFFFFFFFFFFFFFFFE
 WALK_STUCK = 0xFFFFFFFFFFFFFFFC
@@ -312,7 +313,8 @@ class Searcher:
                 + st["walk_steps"] + st["absence_checks"])
 
     def set_algorithm(self, algorithm):
-        """0 = streaming (default), 1 = per-k-mer (the reference's search shape)."""
+        """ALGO_STREAMING, ALGO_PER_KMER (the reference's search shape), ALGO_AUTO (default)
+        or ALGO_STREAMING_ADAPTIVE."""
         _check(lib().sbwt_searcher_set_algorithm(self.h, int(algorithm)))
 
     def set_launch(self, chunk_kmers, blocks_per_cu):

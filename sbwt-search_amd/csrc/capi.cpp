@@ -255,7 +255,7 @@ int sbwt_searcher_set_algorithm(sbwt_searcher *searcher, int algorithm) {
   return guarded([&] {
     require(searcher, "null argument");
     require(algorithm == SBWT_ALGO_STREAMING || algorithm == SBWT_ALGO_PER_KMER ||
-                algorithm == SBWT_ALGO_AUTO,
+                algorithm == SBWT_ALGO_AUTO || algorithm == SBWT_ALGO_STREAMING_ADAPTIVE,
             "unknown algorithm");
     searcher->impl->set_algorithm(algorithm);
   });

@@ -813,14 +813,19 @@ Searcher::Searcher(const DeviceIndex *index, u64 max_chars, u64 max_reads, bool 
   SBWT_HIP(hipMalloc(&d_scan_tmp_, scan_tmp_len_ * 8));
   for (auto &e : ev_) SBWT_HIP(hipEventCreate(&e));
   SBWT_HIP(hipMalloc(&d_stats_, kStatsWords * 8));
-  max_chunks_ = max_chars_ / kMinChunkKmers + max_reads_ + 1;
+  int dev = 0;
+  SBWT_HIP(hipGetDevice(&dev));
+  SBWT_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
+  // work items: at most chars / chunk + reads + 1; an explicit chunk is at
+  // least kMinChunkKmers, an automatic one gives no more items than lanes
+  const u64 max_lanes = u64{kMaxBlocksPerCu} * static_cast<u64>(std::max(1, cus_)) * kSearchThreads;
+  max_chunks_ = std::max<u64>(max_chars_ / kMinChunkKmers,
+                              std::min<u64>(max_chars_ / kAutoMinChunkKmers + 1, max_lanes)) +
+                max_reads_ + 1;
   SBWT_HIP(hipMalloc(&d_chunks_, max_chunks_ * 16));
   SBWT_HIP(hipMalloc(&d_chunk_counts_, std::max<u64>(max_reads_, 1) * 8 * 2));
   SBWT_HIP(hipMalloc(&d_chunk_scan_tmp_, scan_tmp_entries(max_reads_) * 8));
   SBWT_HIP(hipMalloc(&d_counter_, 8));
-  int dev = 0;
-  SBWT_HIP(hipGetDevice(&dev));
-  SBWT_HIP(hipDeviceGetAttribute(&cus_, hipDeviceAttributeMultiprocessorCount, dev));
   set_launch(kDefaultChunkKmers, kDefaultBlocksPerCu);
 }
 
@@ -832,6 +837,13 @@ void Searcher::set_launch(u32 chunk_kmers, u32 blocks_per_cu) {
   chunk_kmers_ = chunk_kmers;
   blocks_per_cu_ = blocks_per_cu;
   persistent_blocks_ = static_cast<u32>(std::max(1, cus_)) * blocks_per_cu;
+}
+
+u32 Searcher::stream_chunk(u64 n_chars) const {
+  if (algorithm_ != kAlgoAuto && algorithm_ != kAlgoStreamingAdaptive) return chunk_kmers_;
+  const u64 lanes = std::max<u64>(u64{persistent_blocks_} * kSearchThreads / lane_share_, 1);
+  const u64 c = (n_chars + lanes - 1) / lanes;
+  return static_cast<u32>(std::clamp<u64>(c, kAutoMinChunkKmers, chunk_kmers_));
 }
 
 LaunchShape Searcher::autotune(const char *d_chars, u64 n_chars, const u64 *d_read_ends, u64 n_reads,
@@ -963,21 +975,25 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
   if (timing_) SBWT_HIP(hipEventRecord(ev_[2], stream_));
   if (stats_) SBWT_HIP(hipMemsetAsync(d_stats_, 0, kStatsWords * 8, stream_));
   const bool streaming = algorithm_ == kAlgoStreaming ||
-                         (algorithm_ == kAlgoAuto && n_chars >= kAutoStreamingChars);
+                         (algorithm_ == kAlgoAuto && n_chars >= kAutoStreamingChars) ||
+                         (algorithm_ == kAlgoStreamingAdaptive && n_chars >= kAdaptiveStreamingChars);
   if (streaming) {
     // chunk table of (read, k-mer range) work items, then the persistent search
+    const u32 chunk = stream_chunk(n_chars);
+    if (n_chars / chunk + n_reads + 1 > max_chunks_)
+      throw std::runtime_error("work-item table too small for the batch");
     u64 *d_chunk_ends = d_chunk_counts_ + max_reads_;
     k_chunk_count<<<grid_for(n_reads, 256), 256, 0, stream_>>>(d_kmer_ends_, d_read_ends, n_reads,
-                                                               n_chars, chunk_kmers_, d_chunk_counts_);
+                                                               n_chars, chunk, d_chunk_counts_);
     scan_inclusive_u64(d_chunk_counts_, n_reads, d_chunk_ends, d_chunk_scan_tmp_, stream_);
     k_chunk_fill<<<grid_for(n_reads, 256), 256, 0, stream_>>>(d_kmer_ends_, d_read_ends, n_reads,
-                                                              d_chunk_ends, chunk_kmers_, d_chunks_);
+                                                              d_chunk_ends, chunk, d_chunks_);
     k_reset_counter<<<1, 1, 0, stream_>>>(d_counter_);
     SBWT_HIP(hipGetLastError());
     if (timing_) SBWT_HIP(hipEventRecord(ev_[2], stream_));
     // no more workgroups than there can be chunks for (a bound from the host:
     // reads plus chars per chunk)
-    const u64 chunk_bound = n_reads + n_chars / chunk_kmers_ + 1;
+    const u64 chunk_bound = n_reads + n_chars / chunk + 1;
     const u32 blocks = static_cast<u32>(
         std::min<u64>(persistent_blocks_, (chunk_bound + kSearchThreads - 1) / kSearchThreads));
     dispatch_search3(index_->wide, move_to_key_, stats_, [&]<typename Idx, bool KEY, bool STATS>() {

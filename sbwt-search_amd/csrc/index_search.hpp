@@ -12,12 +12,20 @@ namespace sbwt_amd {
 // independent search per k-mer (the reference's per-thread search).
 constexpr int kAlgoStreaming = 0;
 constexpr int kAlgoPerKmer = 1;
-// Auto: streaming for batches of at least kAutoStreamingChars chars (its
-// throughput needs many reads in flight), per k-mer below (every k-mer in
-// parallel: a small batch finishes in one search's latency instead of its
-// longest read's chain of steps).
+// Auto: per k-mer below kAutoStreamingChars chars, streaming above with work
+// items shrunk until every persistent lane has one: chunk =
+// clamp(ceil(chars / lanes), kAutoMinChunkKmers, the launch shape's chunk).
+// Alone on the GPU, a batch of a few million chars finishes sooner per k-mer
+// (every k-mer in flight at once, one search's chain of steps) although it
+// reads one line per letter; above, streaming's few lines per k-mer win.
+// Streaming-adaptive: the same shrunk work items from kAdaptiveStreamingChars
+// chars up, for batches that share the GPU with others (a server under load),
+// where the lines read, not one batch's chain of steps, bound the throughput.
 constexpr int kAlgoAuto = 2;
-constexpr u64 kAutoStreamingChars = 1ULL << 21;
+constexpr int kAlgoStreamingAdaptive = 3;
+constexpr u64 kAutoStreamingChars = 1ULL << 22;
+constexpr u64 kAdaptiveStreamingChars = 1ULL << 16;
+constexpr u32 kAutoMinChunkKmers = 8;
 
 struct SearchTimings {
   float pack_ms = 0, offsets_ms = 0, search_ms = 0;
@@ -63,6 +71,11 @@ class Searcher {
   // batch are in d_results, the same for every shape).
   void set_launch(u32 chunk_kmers, u32 blocks_per_cu);
   LaunchShape launch() const { return {chunk_kmers_, blocks_per_cu_, 0.0f}; }
+  // k-mers per work item of a streaming search over n_chars chars.
+  u32 stream_chunk(u64 n_chars) const;
+  // Adaptive work items assume 1 / share of the persistent lanes: a caller
+  // running `share` batches at once on the GPU sizes each for its part.
+  void set_lane_share(u32 share) { lane_share_ = share ? share : 1; }
   LaunchShape autotune(const char *d_chars, u64 n_chars, const u64 *d_read_ends, u64 n_reads,
                        u64 *d_results);
 
@@ -90,6 +103,7 @@ class Searcher {
   bool timing_ = true;
   bool stats_ = false;
   int algorithm_ = kAlgoAuto;
+  u32 lane_share_ = 1;
   u64 *d_stats_ = nullptr;
   u64 max_chunks_ = 0;
   ulonglong2 *d_chunks_ = nullptr;

@@ -119,7 +119,7 @@ void Server::run_slot(Slot &s) {
       if (queue_.empty()) return;  // stop_ and drained
       r = queue_.front();
       queue_.pop_front();
-      peak_in_flight_ = std::max(peak_in_flight_, ++in_flight_);
+      peak_in_flight_ = std::max<u64>(peak_in_flight_, ++in_flight_);
     }
     r->start_s = now();
     try {
@@ -177,6 +177,12 @@ void Server::process(Slot &s, Request &r) {
     while (rb < nr) rb <<= 1;
     const bool graph = cfg_.graphs && cb <= kGraphMaxChars && cb <= cfg_.max_chars_per_slot &&
                        rb <= cfg_.max_reads_per_slot && rb <= kGraphMaxChars;
+    // alone on the GPU a batch takes the search of least latency; sharing it
+    // with other requests, streaming (fewest lines read) with work items
+    // sized for its share of the chip
+    const u64 sharing = in_flight_.load();
+    s.searcher->set_algorithm(sharing > 1 ? kAlgoStreamingAdaptive : kAlgoAuto);
+    s.searcher->set_lane_share(static_cast<u32>(std::max<u64>(sharing, 1)));
     kmer_out += graph ? run_batch_graph(s, r, nc, nr, read0, kmer_out, cb, rb)
                       : run_batch(s, r, nc, nr, read0, kmer_out);
     read0 = read1;
@@ -242,11 +248,11 @@ u64 Server::run_batch_graph(Slot &s, Request &r, u64 nc, u64 nr, u64 read0, u64 
   return nk;
 }
 
-// Captures, once per bucket, the staging copies, the search, the color stage
+// Captures, once per bucket, search algorithm and work-item size, the staging copies, the search, the color stage
 // and the copies back on the slot's stream (timing and census are off in the
 // slots, so only kernels and copies are recorded).
 hipGraphExec_t Server::graph_for(Slot &s, u64 cb, u64 rb) {
-  const auto key = std::make_pair(cb, rb);
+  const auto key = std::make_tuple(cb, rb, s.searcher->algorithm(), s.searcher->stream_chunk(cb));
   auto it = s.graphs.find(key);
   if (it != s.graphs.end()) return it->second;
   hipGraph_t graph = nullptr;

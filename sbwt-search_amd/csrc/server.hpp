@@ -4,11 +4,13 @@
 // No counterpart in the reference, which processes files in fixed streams.
 #pragma once
 
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <cstdint>
 #include <deque>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -76,7 +78,8 @@ class Server {
     u64 ids_capacity = 0;
     u64 *h_results = nullptr;  // pinned, graph batches
     u64 *h_offsets = nullptr;
-    std::map<std::pair<u64, u64>, hipGraphExec_t> graphs;  // by (chars, reads) bucket
+    // by (chars, reads) bucket, search algorithm and work-item size
+    std::map<std::tuple<u64, u64, int, u32>, hipGraphExec_t> graphs;
     std::thread worker;
   };
   void run_slot(Slot &s);
@@ -99,7 +102,8 @@ class Server {
   std::deque<std::shared_ptr<Request>> queue_;
   std::map<u64, std::shared_ptr<Request>> requests_;
   u64 next_id_ = 1;
-  u64 in_flight_ = 0, peak_in_flight_ = 0;
+  std::atomic<u64> in_flight_{0};
+  u64 peak_in_flight_ = 0;
   bool stop_ = false;
   std::chrono::steady_clock::time_point t0_;
 };

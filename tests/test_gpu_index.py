@@ -367,17 +367,25 @@ def test_autotune_keeps_results(genome_index):
 
 
 def test_auto_algorithm_by_batch_size(genome_index):
-    """The default picks per-k-mer search for small batches and streaming for
-    batches of 2^21 chars and more; results are the same."""
+    """The default searches per k-mer below 2^22 chars and streams above (work
+    items shrunk when the batch is small for the chip); streaming-adaptive
+    streams from 2^16 chars. The results are the same."""
     g, b, path, orc = genome_index
     say("auto algorithm")
     idx = S.Index.load(path)
+    tiny_c, tiny_e = _reads(g, 300, 150, seed=70)
     small_c, small_e = _reads(g, 1000, 150, seed=71)
-    big_c, big_e = _reads(g, 15_000, 150, seed=72)
-    assert len(small_c) < (1 << 21) <= len(big_c)
+    big_c, big_e = _reads(g, 30_000, 150, seed=72)
+    assert len(tiny_c) < (1 << 16) <= len(small_c) < (1 << 22) <= len(big_c)
     s = idx.searcher(len(big_c), len(big_e))
     s.set_stats(True)
-    assert np.array_equal(s.search_reads(small_c, small_e), orc.index_reads(small_c, small_e))
-    assert s.last_stats()["streamed"] == 0
-    assert np.array_equal(s.search_reads(big_c, big_e), orc.index_reads(big_c, big_e))
-    assert s.last_stats()["streamed"] > 0
+    expect = {0: orc.index_reads(tiny_c, tiny_e), 1: orc.index_reads(small_c, small_e),
+              2: orc.index_reads(big_c, big_e)}
+    batches = {0: (tiny_c, tiny_e), 1: (small_c, small_e), 2: (big_c, big_e)}
+    for algo, streams in ((S.ALGO_AUTO, (False, False, True)),
+                          (S.ALGO_STREAMING_ADAPTIVE, (False, True, True))):
+        s.set_algorithm(algo)
+        for i in range(3):
+            c, e = batches[i]
+            assert np.array_equal(s.search_reads(c, e), expect[i]), (algo, i)
+            assert (s.last_stats()["streamed"] > 0) == streams[i], (algo, i)
