@@ -178,11 +178,14 @@ def main():
         step()
     torch.cuda.synchronize()
     s.sync()
-    search_ms = []
+    search_ms, pack_ms, offsets_ms = [], [], []
     t_start = time.perf_counter()
     for _ in range(args.steps):
         step()
-        search_ms.append(s.last_timings()["search_ms"])
+        tm = s.last_timings()
+        search_ms.append(tm["search_ms"])
+        pack_ms.append(tm["pack_ms"])
+        offsets_ms.append(tm["offsets_ms"])
     s.sync()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_start
@@ -232,6 +235,8 @@ def main():
             "algorithm": args.algorithm,
             "launch": launch,
             "search_kernel_ms": kern_ms,
+            "pack_ms": float(np.mean(pack_ms)),
+            "offsets_and_work_items_ms": float(np.mean(offsets_ms)),
             "reads_per_s": wl.n_reads / (elapsed / args.steps),
         },
         "roofline": {

@@ -22,17 +22,22 @@ constexpr u64 kMaxGrid = 1u << 20;  // workgroups per launch; kernels grid-strid
 // [63-2(j%32), 62-2(j%32)]; A/a 0, C/c 1, G/g 2, T/t 3; any other byte packs
 // as 0 and sets bit j%32 of invalid word j/32. One thread per output word.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void pack4(u32 bytes4, u32 j0, u64 &v, u32 &inv) {
-#pragma unroll
-  for (u32 q = 0; q < 4; ++q) {
-    const u32 ch = (bytes4 >> (8 * q)) & 0xFFu;
-    const u32 u = ch & 0xDFu;
-    const bool ok = (u == 0x41u) | (u == 0x43u) | (u == 0x47u) | (u == 0x54u);
-    const u32 code = ((u >> 1) ^ (u >> 2)) & 3u;
-    const u32 j = j0 + q;
-    v |= ok ? (static_cast<u64>(code) << (62 - 2 * j)) : 0ULL;
-    inv |= ok ? 0u : (1u << j);
-  }
+// Four chars at once (one per byte of bytes4): their four 2-bit codes,
+// first char in the top bits of the byte, and their invalid flags (bit q for
+// char q). Upper and lower case fold by clearing bit 5; the code of A C G T
+// is ((u >> 1) ^ (u >> 2)) & 3; a char is valid iff it equals the letter of
+// its code (byte permute from "ACGT"); invalid chars pack as 0.
+__device__ __forceinline__ void pack4(u32 bytes4, u32 &codes8, u32 &inv4) {
+  const u32 u = bytes4 & 0xDFDFDFDFu;
+  u32 code = ((u >> 1) ^ (u >> 2)) & 0x03030303u;
+  const u32 letter = __builtin_amdgcn_perm(0u, 0x54474341u, code);  // bytes 'A','C','G','T'
+  const u32 diff = letter ^ u;
+  const u32 bad = (((diff & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | diff) & 0x80808080u;  // nonzero bytes
+  code &= ((~bad >> 7) & 0x01010101u) * 3u;
+  // gather the byte fields (disjoint partial products, no carries):
+  // c0 << 6 | c1 << 4 | c2 << 2 | c3, and flag q at bit q
+  codes8 = (code * 0x40100401u) >> 24;
+  inv4 = ((((bad >> 7) & 0x01010101u) * 0x01020408u) >> 24) & 0xFu;
 }
 
 __global__ __launch_bounds__(256) void k_pack(const u8 *__restrict__ chars, u64 n, u64 n_words,
@@ -46,14 +51,17 @@ __global__ __launch_bounds__(256) void k_pack(const u8 *__restrict__ chars, u64 
     if (aligned && base + 32 <= n) {
       const uint4 a = *reinterpret_cast<const uint4 *>(chars + base);
       const uint4 b = *reinterpret_cast<const uint4 *>(chars + base + 16);
-      pack4(a.x, 0, v, inv);
-      pack4(a.y, 4, v, inv);
-      pack4(a.z, 8, v, inv);
-      pack4(a.w, 12, v, inv);
-      pack4(b.x, 16, v, inv);
-      pack4(b.y, 20, v, inv);
-      pack4(b.z, 24, v, inv);
-      pack4(b.w, 28, v, inv);
+      const u32 g[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+      u32 hi = 0, lo = 0;
+#pragma unroll
+      for (u32 q = 0; q < 8; ++q) {
+        u32 c8, i4;
+        pack4(g[q], c8, i4);
+        if (q < 4) hi |= c8 << (24 - 8 * q);
+        else lo |= c8 << (24 - 8 * (q - 4));
+        inv |= i4 << (4 * q);
+      }
+      v = (static_cast<u64>(hi) << 32) | lo;
     } else {
       for (u32 j = 0; j < 32 && base + j < n; ++j) {
         const u32 u = chars[base + j] & 0xDFu;
@@ -70,72 +78,130 @@ __global__ __launch_bounds__(256) void k_pack(const u8 *__restrict__ chars, u64 
 // ---------------------------------------------------------------------------
 // Cumulative k-mer counts per read (the positions of
 // src/PositionsBuilder/PositionsBuilder.cpp:15-60 made implicit): read r of
-// length len has max(0, len - k + 1) k-mers. Three-phase scan.
+// length len has max(0, len - k + 1) k-mers. For the streaming search the
+// same pass also accumulates the read's work items, ceil(k-mers / chunk).
+// Three-phase scan over blocks of
+// kScanChunk reads, each block reading row after row of 256 consecutive
+// reads (coalesced) and scanning a row with wave shuffles.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ u64 kmers_of_read(const u64 *ends, u64 r, u32 k) {
-  const u64 st = r ? ends[r - 1] : 0, len = ends[r] >= st ? ends[r] - st : 0;
+// (none for a read ending before its start or past the batch's chars)
+__device__ __forceinline__ u64 kmers_of_read(const u64 *ends, u64 r, u32 k, u64 n_chars) {
+  const u64 st = r ? ends[r - 1] : 0;
+  const u64 len = ends[r] >= st && ends[r] <= n_chars ? ends[r] - st : 0;
   return len >= k ? len - k + 1 : 0;
 }
 
-template <u32 N>
-__device__ __forceinline__ u64 block_inclusive_scan(u64 v, u64 *sh) {
-  const u32 t = threadIdx.x;
-  sh[t] = v;
-  __syncthreads();
-  for (u32 off = 1; off < N; off <<= 1) {
-    const u64 x = t >= off ? sh[t - off] : 0;
-    __syncthreads();
-    sh[t] += x;
-    __syncthreads();
+struct ReadCounts {
+  u64 kmers, items;
+};
+
+template <bool ITEMS>
+__device__ __forceinline__ ReadCounts read_counts(const u64 *ends, u64 r, u64 n, u32 k, u64 n_chars,
+                                                  u32 chunk) {
+  ReadCounts c{0, 0};
+  if (r >= n) return c;
+  c.kmers = kmers_of_read(ends, r, k, n_chars);
+  if constexpr (ITEMS) c.items = (c.kmers + chunk - 1) / chunk;
+  return c;
+}
+
+__device__ __forceinline__ u64 wave_inclusive_sum(u64 v) {
+  const u32 lane = threadIdx.x & 63;
+#pragma unroll
+  for (u32 d = 1; d < 64; d <<= 1) {
+    const u64 y = __shfl_up(v, d, 64);
+    if (lane >= d) v += y;
   }
-  const u64 r = sh[t];
+  return v;
+}
+
+// Exclusive prefix and total over the workgroup of one value per thread, in
+// thread order; sh holds kScanThreads / 64 entries.
+__device__ __forceinline__ u64 block_row_scan(u64 v, u64 *sh, u64 *total) {
+  const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const u64 incl = wave_inclusive_sum(v);
+  if (lane == 63) sh[wave] = incl;
   __syncthreads();
-  return r;
+  u64 before = 0, all = 0;
+#pragma unroll
+  for (u32 w = 0; w < kScanThreads / 64; ++w) {
+    before += w < wave ? sh[w] : 0;
+    all += sh[w];
+  }
+  __syncthreads();
+  *total = all;
+  return before + incl - v;
 }
 
+// sums[blk] = k-mers of the block's reads; with ITEMS, sums[cap + blk] = work items.
+template <bool ITEMS>
 __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const u64 *__restrict__ ends, u64 n,
-                                                               u32 k, u64 *__restrict__ sums) {
-  __shared__ u64 sh[kScanThreads];
-  const u64 first = blockIdx.x * kScanChunk + threadIdx.x * static_cast<u64>(kScanItems);
-  u64 s = 0;
-  for (u32 i = 0; i < kScanItems; ++i)
-    if (first + i < n) s += kmers_of_read(ends, first + i, k);
-  const u64 tot = block_inclusive_scan<kScanThreads>(s, sh);
-  if (threadIdx.x == kScanThreads - 1) sums[blockIdx.x] = tot;
+                                                               u32 k, u64 n_chars, u32 chunk,
+                                                               u64 *__restrict__ sums, u64 cap) {
+  __shared__ u64 sh[2][kScanThreads / 64];
+  const u64 base = blockIdx.x * kScanChunk;
+  u64 sk = 0, si = 0;
+  for (u32 i = 0; i < kScanItems; ++i) {
+    const ReadCounts c = read_counts<ITEMS>(ends, base + i * kScanThreads + threadIdx.x, n, k,
+                                            n_chars, chunk);
+    sk += c.kmers;
+    si += c.items;
+  }
+  u64 tk = 0, ti = 0;
+  block_row_scan(sk, sh[0], &tk);
+  if constexpr (ITEMS) block_row_scan(si, sh[1], &ti);
+  if (threadIdx.x == 0) {
+    sums[blockIdx.x] = tk;
+    if constexpr (ITEMS) sums[cap + blockIdx.x] = ti;
+  }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_chunks(u64 *__restrict__ sums, u64 n_chunks) {
-  __shared__ u64 sh[1024];
+// One workgroup per array (sums + blockIdx.x * cap): in place, exclusive.
+__global__ __launch_bounds__(1024) void k_scan_chunks(u64 *__restrict__ sums, u64 n_chunks, u64 cap) {
+  __shared__ u64 sh[1024 / 64];
+  u64 *a = sums + blockIdx.x * cap;
+  const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   u64 carry = 0;
   for (u64 base = 0; base < n_chunks; base += 1024) {
     const u64 i = base + threadIdx.x;
-    const u64 v = i < n_chunks ? sums[i] : 0;
-    const u64 inc = block_inclusive_scan<1024>(v, sh);
-    if (i < n_chunks) sums[i] = carry + inc - v;
-    __shared__ u64 tile_total;
-    if (threadIdx.x == 1023) tile_total = inc;
+    const u64 v = i < n_chunks ? a[i] : 0;
+    const u64 incl = wave_inclusive_sum(v);
+    if (lane == 63) sh[wave] = incl;
     __syncthreads();
-    carry += tile_total;
+    u64 before = 0, all = 0;
+    for (u32 w = 0; w < 1024 / 64; ++w) {
+      before += w < wave ? sh[w] : 0;
+      all += sh[w];
+    }
     __syncthreads();
+    if (i < n_chunks) a[i] = carry + before + incl - v;
+    carry += all;
   }
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_write(const u64 *__restrict__ ends, u64 n, u32 k,
-                                                              const u64 *__restrict__ offs,
-                                                              u64 *__restrict__ out) {
-  __shared__ u64 sh[kScanThreads];
-  const u64 first = blockIdx.x * kScanChunk + threadIdx.x * static_cast<u64>(kScanItems);
-  u64 v[kScanItems];
-  u64 s = 0;
+// Inclusive ends: kends[r] = k-mers of reads 0..r; with ITEMS, item_ends[r] =
+// work items of reads 0..r.
+template <bool ITEMS>
+__global__ __launch_bounds__(kScanThreads) void k_scan_write(const u64 *__restrict__ ends, u64 n,
+                                                              u32 k, u64 n_chars, u32 chunk,
+                                                              const u64 *__restrict__ offs, u64 cap,
+                                                              u64 *__restrict__ kends,
+                                                              u64 *__restrict__ item_ends) {
+  __shared__ u64 sh[2][kScanThreads / 64];
+  const u64 base = blockIdx.x * kScanChunk;
+  u64 ck = offs[blockIdx.x], ci = ITEMS ? offs[cap + blockIdx.x] : 0;
   for (u32 i = 0; i < kScanItems; ++i) {
-    v[i] = first + i < n ? kmers_of_read(ends, first + i, k) : 0;
-    s += v[i];
-  }
-  const u64 inc = block_inclusive_scan<kScanThreads>(s, sh);
-  u64 run = offs[blockIdx.x] + inc - s;
-  for (u32 i = 0; i < kScanItems; ++i) {
-    run += v[i];
-    if (first + i < n) out[first + i] = run;
+    const u64 r = base + i * kScanThreads + threadIdx.x;
+    const ReadCounts c = read_counts<ITEMS>(ends, r, n, k, n_chars, chunk);
+    u64 tk = 0, ti = 0;
+    const u64 bk = block_row_scan(c.kmers, sh[0], &tk);
+    if (r < n) kends[r] = ck + bk + c.kmers;
+    ck += tk;
+    if constexpr (ITEMS) {
+      const u64 bi = block_row_scan(c.items, sh[1], &ti);
+      if (r < n) item_ends[r] = ci + bi + c.items;
+      ci += ti;
+    }
   }
 }
 
@@ -287,32 +353,24 @@ __device__ __forceinline__ u32 lanes_below(u64 mask) {
                                    __builtin_amdgcn_mbcnt_lo(static_cast<u32>(mask), 0u));
 }
 
-// Per read: number of chunks (none for a read ending past n_chars, so malformed
-// read ends never send the search outside the batch).
-__global__ __launch_bounds__(256) void k_chunk_count(const u64 *__restrict__ kmer_ends,
-                                                      const u64 *__restrict__ char_ends, u64 n_reads,
-                                                      u64 n_chars, u32 chunk, u64 *__restrict__ counts) {
-  for (u64 r = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; r < n_reads;
-       r += static_cast<u64>(gridDim.x) * blockDim.x) {
-    const u64 nk = kmer_ends[r] - (r ? kmer_ends[r - 1] : 0);
-    counts[r] = char_ends[r] > n_chars ? 0 : (nk + chunk - 1) / chunk;
-  }
-}
-
-// Per read: its chunks {first k-mer (output slot), char position | count << 48}.
+// Per read: its chunks {first k-mer (output slot), char position | count << 48}
+// (a read ending past n_chars has none, so malformed read ends never send the
+// search outside the batch: the scan counted it no work items).
 __global__ __launch_bounds__(256) void k_chunk_fill(const u64 *__restrict__ kmer_ends,
                                                      const u64 *__restrict__ char_ends, u64 n_reads,
                                                      const u64 *__restrict__ chunk_ends, u32 chunk,
-                                                     ulonglong2 *__restrict__ chunks) {
+                                                     u64 n_out, ulonglong2 *__restrict__ chunks) {
   for (u64 r = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; r < n_reads;
        r += static_cast<u64>(gridDim.x) * blockDim.x) {
     const u64 ks = r ? kmer_ends[r - 1] : 0, nk = kmer_ends[r] - ks;
     const u64 cs = r ? char_ends[r - 1] : 0;
     u64 at = r ? chunk_ends[r - 1] : 0;
-    for (u64 off = 0; off < nk; off += chunk, ++at) {
+    for (u64 off = 0; at < chunk_ends[r]; off += chunk, ++at) {
       const u64 cnt = min(static_cast<u64>(chunk), nk - off);
       ulonglong2 ch;
-      ch.x = ks + off;
+      // output slots stay below n_out even for read ends that go back and
+      // forth (which break the read-ends contract; their results are void)
+      ch.x = min(ks + off, n_out - cnt);
       ch.y = (cs + off) | (cnt << 48);
       chunks[at] = ch;
     }
@@ -810,7 +868,7 @@ Searcher::Searcher(const DeviceIndex *index, u64 max_chars, u64 max_reads, bool 
   SBWT_HIP(hipMalloc(&d_invalid_, words * 4));
   SBWT_HIP(hipMalloc(&d_kmer_ends_, std::max<u64>(max_reads_, 1) * 8));
   scan_tmp_len_ = max_reads_ / kScanChunk + 2;
-  SBWT_HIP(hipMalloc(&d_scan_tmp_, scan_tmp_len_ * 8));
+  SBWT_HIP(hipMalloc(&d_scan_tmp_, 2 * scan_tmp_len_ * 8));  // k-mer and work-item block sums
   for (auto &e : ev_) SBWT_HIP(hipEventCreate(&e));
   SBWT_HIP(hipMalloc(&d_stats_, kStatsWords * 8));
   int dev = 0;
@@ -824,7 +882,6 @@ Searcher::Searcher(const DeviceIndex *index, u64 max_chars, u64 max_reads, bool 
                 max_reads_ + 1;
   SBWT_HIP(hipMalloc(&d_chunks_, max_chunks_ * 16));
   SBWT_HIP(hipMalloc(&d_chunk_counts_, std::max<u64>(max_reads_, 1) * 8 * 2));
-  SBWT_HIP(hipMalloc(&d_chunk_scan_tmp_, scan_tmp_entries(max_reads_) * 8));
   SBWT_HIP(hipMalloc(&d_counter_, 8));
   set_launch(kDefaultChunkKmers, kDefaultBlocksPerCu);
 }
@@ -883,7 +940,7 @@ Searcher::~Searcher() {
   for (auto &e : ev_) (void)hipEventDestroy(e);
   void *bufs[] = {d_packed_, d_invalid_, d_kmer_ends_, d_scan_tmp_, d_chars_, d_read_ends_,
                   d_results_, d_positions_, d_bits_, d_stats_, d_chunks_, d_chunk_counts_,
-                  d_chunk_scan_tmp_, d_counter_};
+                  d_counter_};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (own_stream_) (void)hipStreamDestroy(stream_);
@@ -965,29 +1022,37 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
   if (timing_) SBWT_HIP(hipEventRecord(ev_[0], stream_));
   launch_pack(d_chars, n_chars, d_packed_, d_invalid_, stream_);
   if (timing_) SBWT_HIP(hipEventRecord(ev_[1], stream_));
-  const u64 chunks = (n_reads + kScanChunk - 1) / kScanChunk;
-  k_scan_reduce<<<static_cast<u32>(chunks), kScanThreads, 0, stream_>>>(d_read_ends, n_reads, ix.k,
-                                                                         d_scan_tmp_);
-  k_scan_chunks<<<1, 1024, 0, stream_>>>(d_scan_tmp_, chunks);
-  k_scan_write<<<static_cast<u32>(chunks), kScanThreads, 0, stream_>>>(d_read_ends, n_reads, ix.k,
-                                                                        d_scan_tmp_, d_kmer_ends_);
-  SBWT_HIP(hipGetLastError());
-  if (timing_) SBWT_HIP(hipEventRecord(ev_[2], stream_));
-  if (stats_) SBWT_HIP(hipMemsetAsync(d_stats_, 0, kStatsWords * 8, stream_));
   const bool streaming = algorithm_ == kAlgoStreaming ||
                          (algorithm_ == kAlgoAuto && n_chars >= kAutoStreamingChars) ||
                          (algorithm_ == kAlgoStreamingAdaptive && n_chars >= kAdaptiveStreamingChars);
+  // work items of the streaming search: (read, up to `chunk` k-mers)
+  const u32 chunk = streaming ? stream_chunk(n_chars) : 1;
+  if (streaming && n_chars / chunk + n_reads + 1 > max_chunks_)
+    throw std::runtime_error("work-item table too small for the batch");
+  u64 *d_item_ends = d_chunk_counts_ + max_reads_;
+  const u64 chunks = (n_reads + kScanChunk - 1) / kScanChunk;
+  const u32 scan_grid = static_cast<u32>(chunks);
   if (streaming) {
-    // chunk table of (read, k-mer range) work items, then the persistent search
-    const u32 chunk = stream_chunk(n_chars);
-    if (n_chars / chunk + n_reads + 1 > max_chunks_)
-      throw std::runtime_error("work-item table too small for the batch");
-    u64 *d_chunk_ends = d_chunk_counts_ + max_reads_;
-    k_chunk_count<<<grid_for(n_reads, 256), 256, 0, stream_>>>(d_kmer_ends_, d_read_ends, n_reads,
-                                                               n_chars, chunk, d_chunk_counts_);
-    scan_inclusive_u64(d_chunk_counts_, n_reads, d_chunk_ends, d_chunk_scan_tmp_, stream_);
+    k_scan_reduce<true><<<scan_grid, kScanThreads, 0, stream_>>>(d_read_ends, n_reads, ix.k, n_chars,
+                                                                 chunk, d_scan_tmp_, scan_tmp_len_);
+    k_scan_chunks<<<2, 1024, 0, stream_>>>(d_scan_tmp_, chunks, scan_tmp_len_);
+    k_scan_write<true><<<scan_grid, kScanThreads, 0, stream_>>>(d_read_ends, n_reads, ix.k, n_chars,
+                                                                chunk, d_scan_tmp_, scan_tmp_len_,
+                                                                d_kmer_ends_, d_item_ends);
+  } else {
+    k_scan_reduce<false><<<scan_grid, kScanThreads, 0, stream_>>>(d_read_ends, n_reads, ix.k, n_chars,
+                                                                  chunk, d_scan_tmp_, scan_tmp_len_);
+    k_scan_chunks<<<1, 1024, 0, stream_>>>(d_scan_tmp_, chunks, scan_tmp_len_);
+    k_scan_write<false><<<scan_grid, kScanThreads, 0, stream_>>>(d_read_ends, n_reads, ix.k, n_chars,
+                                                                 chunk, d_scan_tmp_, scan_tmp_len_,
+                                                                 d_kmer_ends_, nullptr);
+  }
+  SBWT_HIP(hipGetLastError());
+  if (stats_) SBWT_HIP(hipMemsetAsync(d_stats_, 0, kStatsWords * 8, stream_));
+  if (streaming) {
+    // work-item table, then the persistent search
     k_chunk_fill<<<grid_for(n_reads, 256), 256, 0, stream_>>>(d_kmer_ends_, d_read_ends, n_reads,
-                                                              d_chunk_ends, chunk, d_chunks_);
+                                                              d_item_ends, chunk, n_chars, d_chunks_);
     k_reset_counter<<<1, 1, 0, stream_>>>(d_counter_);
     SBWT_HIP(hipGetLastError());
     if (timing_) SBWT_HIP(hipEventRecord(ev_[2], stream_));
@@ -998,10 +1063,11 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
         std::min<u64>(persistent_blocks_, (chunk_bound + kSearchThreads - 1) / kSearchThreads));
     dispatch_search3(index_->wide, move_to_key_, stats_, [&]<typename Idx, bool KEY, bool STATS>() {
       k_search_stream<Idx, KEY, STATS><<<blocks, kSearchThreads, 0, stream_>>>(
-          ix, d_packed_, d_invalid_, d_chunks_, d_chunk_ends, n_reads, d_counter_, d_results,
+          ix, d_packed_, d_invalid_, d_chunks_, d_item_ends, n_reads, d_counter_, d_results,
           d_stats_, absence_checks(*index_) ? 1u : 0u);
     });
   } else {
+    if (timing_) SBWT_HIP(hipEventRecord(ev_[2], stream_));
     // n_chars bounds the k-mer count; surplus workgroups leave at once
     dispatch_search3(index_->wide, move_to_key_, stats_, [&]<typename Idx, bool KEY, bool STATS>() {
       k_search_reads<Idx, KEY, STATS>

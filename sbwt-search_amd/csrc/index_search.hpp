@@ -107,8 +107,7 @@ class Searcher {
   u64 *d_stats_ = nullptr;
   u64 max_chunks_ = 0;
   ulonglong2 *d_chunks_ = nullptr;
-  u64 *d_chunk_counts_ = nullptr;  // [counts | inclusive chunk ends]
-  u64 *d_chunk_scan_tmp_ = nullptr;
+  u64 *d_chunk_counts_ = nullptr;  // [unused | inclusive work-item ends per read]
   u64 *d_counter_ = nullptr;
   u32 persistent_blocks_ = 0;
   int cus_ = 1;
