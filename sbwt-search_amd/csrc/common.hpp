@@ -62,6 +62,7 @@ constexpr u32 kFarGroup = 0xFFFFFFFFu;
 struct IndexView {
   const uint4 *blocks;        // [n_blocks][8] slots
   const u64 *hyper_base;      // [n_hyper][8] absolute counts at hyperblock start (slots 0-3, 5)
+  u32 hyper_shift;            // lines per hyperblock = 2^hyper_shift (kHyperShift)
   const void *presearch;      // [4^kp] of {left,right} (uint2 or ulonglong2)
   const void *key_dest;       // [n] key-k-mer move destination (u32 or u64), or null
   u64 c_map[5];

@@ -31,7 +31,7 @@ __device__ __forceinline__ Idx entry_base(const IndexView &ix, uint4 e, Idx b, u
   if constexpr (sizeof(Idx) == 4) {
     return e.x;
   } else {
-    return ix.hyper_base[(b >> kHyperShift) * kLineSlots + slot] + e.x;
+    return ix.hyper_base[(b >> ix.hyper_shift) * kLineSlots + slot] + e.x;
   }
 }
 

@@ -172,6 +172,15 @@ void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const
   // selects them for any index so the 64-bit kernels can be validated small.
   const char *force = std::getenv("SBWT_AMD_FORCE_WIDE");
   wide = n >= (1ULL << 32) || (force && force[0] == '1');
+  // SBWT_AMD_HYPER_SHIFT (1..25) shrinks the hyperblocks of a wide index so
+  // that small test indexes cross many hyperblock boundaries
+  hyper_shift = kHyperShift;
+  if (const char *hs = std::getenv("SBWT_AMD_HYPER_SHIFT"); wide && hs) {
+    const long v = std::strtol(hs, nullptr, 10);
+    if (v < 1 || v > static_cast<long>(kHyperShift))
+      throw std::runtime_error("SBWT_AMD_HYPER_SHIFT must be in [1, 25]");
+    hyper_shift = static_cast<u32>(v);
+  }
   if (kp == 0) {
     size_t free_b = 0, total_b = 0;
     SBWT_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -202,7 +211,7 @@ void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const
       host[b * kLineSlots + s] = e;
     }
   });
-  n_hyper = (n_blocks >> kHyperShift) + 1;
+  n_hyper = (n_blocks >> hyper_shift) + 1;
   std::vector<u64> hyper(n_hyper * kLineSlots, 0);
   u64 run[6] = {0, 0, 0, 0, 0, 0};
   u64 group = 0;  // start of the group of the current line's first node
@@ -220,8 +229,8 @@ void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const
         if (last_start[b]) group = first + last_start[b] - 1;
         continue;
       }
-      if ((b & ((1ULL << kHyperShift) - 1)) == 0) hyper[(b >> kHyperShift) * kLineSlots + s] = run[s];
-      e.x = static_cast<u32>(wide ? run[s] - hyper[(b >> kHyperShift) * kLineSlots + s] : run[s]);
+      if ((b & ((1ULL << hyper_shift) - 1)) == 0) hyper[(b >> hyper_shift) * kLineSlots + s] = run[s];
+      e.x = static_cast<u32>(wide ? run[s] - hyper[(b >> hyper_shift) * kLineSlots + s] : run[s]);
       run[s] += __builtin_popcount(e.y) + __builtin_popcount(e.z) + __builtin_popcount(e.w);
     }
   }
@@ -285,6 +294,7 @@ IndexView DeviceIndex::view() const {
   IndexView v{};
   v.blocks = blocks;
   v.hyper_base = hyper_base;
+  v.hyper_shift = hyper_shift;
   v.presearch = presearch;
   v.key_dest = key_dest;
   v.has_groups = has_groups;

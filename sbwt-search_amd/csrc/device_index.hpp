@@ -32,6 +32,7 @@ struct DeviceIndex {
   u64 n_blocks = 0;
   u64 *hyper_base = nullptr;
   u64 n_hyper = 0;
+  u32 hyper_shift = kHyperShift;
   void *presearch = nullptr;
   u64 presearch_bytes = 0;
   bool has_marks = false;   // key-k-mer marks are in the lines

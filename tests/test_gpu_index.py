@@ -139,11 +139,16 @@ def test_searcher_reuse_shrinking_batches(genome_index):
 
 
 @pytest.mark.parametrize("algo", ALGOS, ids=["streaming", "per-kmer"])
-def test_wide_index_kernels(genome_index, monkeypatch, algo):
-    """64-bit node-index kernels (needed from 2^32 nodes) on a small index."""
+@pytest.mark.parametrize("hyper_shift", [None, "3"], ids=["hyper25", "hyper3"])
+def test_wide_index_kernels(genome_index, monkeypatch, algo, hyper_shift):
+    """64-bit node-index kernels (needed from 2^32 nodes) on a small index; with
+    hyperblocks of 8 lines (768 nodes) the counts cross a hyperblock base on
+    most steps, as those of a 4 G-node index do every 2^25 lines."""
     g, b, path, orc = genome_index
-    say(f"forced 64-bit index algo={algo}")
+    say(f"forced 64-bit index algo={algo} hyper_shift={hyper_shift}")
     monkeypatch.setenv("SBWT_AMD_FORCE_WIDE", "1")
+    if hyper_shift:
+        monkeypatch.setenv("SBWT_AMD_HYPER_SHIFT", hyper_shift)
     idx = S.Index.load(path)
     assert idx.wide
     chars, ends = _reads(g, 2000, 150, seed=21)
@@ -234,6 +239,23 @@ def test_key_kmer_move_parity(d, algo, cap, tmp_path, monkeypatch):
     if cap is not None:
         monkeypatch.setenv("SBWT_AMD_KEY_TABLE_STEPS", cap)
     say(f"key k-mer walk d={d} algo={algo} cap={cap}")
+    _key_move_case(d, algo, tmp_path)
+
+
+@pytest.mark.parametrize("cap", [None, "0"], ids=["table", "walk-only"])
+@pytest.mark.parametrize("algo", ALGOS, ids=["streaming", "per-kmer"])
+def test_key_kmer_move_wide_small_hyperblocks(algo, cap, tmp_path, monkeypatch):
+    """The key move on 64-bit node indexes with hyperblocks of 8 lines: the mark
+    and letter counts of every line add a hyperblock base."""
+    monkeypatch.setenv("SBWT_AMD_FORCE_WIDE", "1")
+    monkeypatch.setenv("SBWT_AMD_HYPER_SHIFT", "3")
+    if cap is not None:
+        monkeypatch.setenv("SBWT_AMD_KEY_TABLE_STEPS", cap)
+    say(f"key k-mer walk wide hyper3 algo={algo} cap={cap}")
+    _key_move_case(3, algo, tmp_path)
+
+
+def _key_move_case(d, algo, tmp_path):
     root = W.genome_random(1, 30_000)
     gens = [root] + [W.genome_mutate(root, s, 0.02) for s in (2, 3, 4)]
     codes, ends = W.concat(gens)
