@@ -69,7 +69,9 @@ def parse():
     ap.add_argument("--presearch-letters", type=int, default=0,
                     help="presearch prefix letters (0: the library's choice for the index size; "
                          "the reference fixes 12)")
-    ap.add_argument("--algorithm", default="streaming", choices=["streaming", "per-kmer"])
+    ap.add_argument("--algorithm", default="streaming", choices=["streaming", "per-kmer", "auto"],
+                    help="auto: the library's default (per k-mer below 2^22 chars, streaming "
+                         "with work items sized to the batch above)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shape instead of autotuning it on the batch")
     ap.add_argument("--launch", default="",
@@ -100,7 +102,8 @@ def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU: the search path runs on a single MI355X here and "
-              f"--gpus {args.gpus} is not supported (see DESIGN.md for how it would shard).",
+              f"--gpus {args.gpus} is not supported (the path would shard by replicating the "
+              "index on every GPU and splitting the reads).",
               file=sys.stderr)
         return 2
     import configs
@@ -141,7 +144,8 @@ def main():
     d_out = torch.empty(wl.n_chars, dtype=torch.int64, device=dev)
     n_kmers = wl.n_kmers()
     s = idx.searcher(wl.n_chars, wl.n_reads)
-    s.set_algorithm(S.ALGO_STREAMING if args.algorithm == "streaming" else S.ALGO_PER_KMER)
+    s.set_algorithm({"streaming": S.ALGO_STREAMING, "per-kmer": S.ALGO_PER_KMER,
+                     "auto": S.ALGO_AUTO}[args.algorithm])
 
     def step():
         s.search_reads_device(d_chars.data_ptr(), wl.n_chars, d_ends.data_ptr(), wl.n_reads,

@@ -12,7 +12,7 @@ struct In {
   std::ifstream f;
   std::string path;
   explicit In(const std::string &p) : f(p, std::ios::in | std::ios::binary), path(p) {
-    if (!f) throw std::runtime_error("Unable to open file " + p);
+    if (!f) throw std::runtime_error("The input file " + p + " cannot be opened");  // IOUtils.cpp:16-21
   }
   void raw(void *dst, uint64_t n) {
     f.read(static_cast<char *>(dst), static_cast<std::streamsize>(n));

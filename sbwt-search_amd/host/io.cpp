@@ -13,6 +13,13 @@
 namespace sbwt_host {
 
 namespace {
+// ThrowingOfstream's message (src/Tools/IOUtils.cpp:27-37)
+std::string output_open_error(const std::string &path) {
+  return "The path " + path +
+         " cannot be opened. Check that all the folders in the path is correct and that you "
+         "have permission to create files in this path folder";
+}
+
 void write_all(FILE *f, const void *p, size_t n) {
   if (n && fwrite(p, 1, n, f) != n) throw std::runtime_error("write failed");
 }
@@ -27,7 +34,7 @@ void write_sized_string(FILE *f, const std::string &s) {
 // ------------------------------------------------------------------ reader
 SeqReader::SeqReader(const std::string &path) : buf_(1 << 20) {
   gzFile g = gzopen(path.c_str(), "rb");
-  if (!g) throw std::runtime_error("Unable to open file " + path);
+  if (!g) throw InputMissing("The input file " + path + " cannot be opened");
   gzbuffer(g, 1 << 20);
   gz_ = g;
 }
@@ -120,7 +127,7 @@ IndexFileOut::IndexFileOut(const std::string &path, int mode, bool headers) {
   static const char *ver[3] = {"v1.0", "v1.0", "v2.0"};
   if (mode < 0 || mode > 2) throw std::runtime_error("Invalid value passed by user for argument print_mode");
   f_ = fopen((path + ext[mode]).c_str(), "wb");
-  if (!f_) throw std::runtime_error("Unable to open file " + path + ext[mode]);
+  if (!f_) throw std::runtime_error(output_open_error(path + ext[mode]));
   if (headers) {
     write_sized_string(f_, fmt[mode]);
     write_sized_string(f_, ver[mode]);
@@ -136,7 +143,7 @@ ColorFileOut::ColorFileOut(const std::string &path, int mode, bool headers, u64 
   static const char *ext[3] = {".txt", ".bin", ".csv"};
   if (mode < 0 || mode > 2) throw std::runtime_error("Invalid value passed by user for argument print_mode");
   f_ = fopen((path + ext[mode]).c_str(), "wb");
-  if (!f_) throw std::runtime_error("Unable to open file " + path + ext[mode]);
+  if (!f_) throw std::runtime_error(output_open_error(path + ext[mode]));
   if (mode == 2) {
     std::string h;
     for (u64 i = 1; i <= num_colors; ++i) {
@@ -157,17 +164,17 @@ void ColorFileOut::write(const char *p, u64 n) { write_all(f_, p, n); }
 // ------------------------------------------------------ index file reading
 IndexFileIn::IndexFileIn(const std::string &path) {
   f_ = fopen(path.c_str(), "rb");
-  if (!f_) throw std::runtime_error("Unable to open file " + path);
+  if (!f_) throw InputMissing("The input file " + path + " cannot be opened");
   auto sized = [&]() {
     u64 n = 0;
     if (fread(&n, 8, 1, f_) != 1 || n > 64) {
       fclose(f_);
-      throw std::runtime_error("Invalid file format in file: " + path);
+      throw InputMissing("Invalid file format in file: " + path);
     }
     std::string s(n, '\0');
     if (n && fread(&s[0], 1, n, f_) != n) {
       fclose(f_);
-      throw std::runtime_error("Invalid file format in file: " + path);
+      throw InputMissing("Invalid file format in file: " + path);
     }
     return s;
   };
@@ -176,7 +183,7 @@ IndexFileIn::IndexFileIn(const std::string &path) {
   else if (fmt == "binary") format_ = 1;
   else {
     fclose(f_);
-    throw std::runtime_error("Invalid file format in file: " + path);
+    throw InputMissing("Invalid file format in file: " + path);
   }
   if (sized() != "v1.0") {
     fclose(f_);
@@ -203,7 +210,7 @@ std::vector<std::string> expand_list(const std::string &name) {
   const std::string ext = ".list";
   if (name.size() >= ext.size() && name.compare(name.size() - ext.size(), ext.size(), ext) == 0) {
     std::ifstream in(name);
-    if (!in) throw std::runtime_error("Unable to open file " + name);
+    if (!in) throw std::runtime_error("The input file " + name + " cannot be opened");
     std::vector<std::string> out;
     std::string line;
     while (std::getline(in, line)) out.push_back(line);

@@ -5,6 +5,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -12,6 +13,14 @@ namespace sbwt_host {
 
 using u32 = uint32_t;
 using u64 = uint64_t;
+
+// An input file that cannot be read as such: the reference logs it and goes
+// on with the next file, whose output file is still written (with headers
+// only) (ContinuousSequenceFileParser.cpp:113-132,
+// ContinuousIndexFileParser.cpp:64-100).
+struct InputMissing : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
 
 // FASTA/FASTQ reader (optionally gzipped). The reference delegates to
 // kseqpp_REad v1.6.0 (src/SequenceFileParser/ContinuousSequenceFileParser.cpp

@@ -298,7 +298,12 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
     std::vector<char> read;
     for (size_t fi = 0; fi < files.size(); ++fi) {
       const u64 f = files[fi];
-      sbwt_host::SeqReader reader(in[f]);
+      std::unique_ptr<sbwt_host::SeqReader> reader;
+      try {
+        reader = std::make_unique<sbwt_host::SeqReader>(in[f]);
+      } catch (const sbwt_host::InputMissing &e) {
+        std::fprintf(stderr, "%s\n", e.what());  // the file's output holds headers only
+      }
       InBatch b;
       bool ok = free_in.pop(b.chars);
       b.file = f;
@@ -316,7 +321,7 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
       };
       while (ok) {
         read.clear();
-        if (!reader.next(read)) break;
+        if (!reader || !reader->next(read)) break;
         // a read longer than the room left is cut into pieces overlapping by
         // k-1 chars (the carry-over of ContinuousSequenceFileParser.cpp:82-112)
         u64 st = 0;
@@ -516,17 +521,22 @@ void colors_stream(sbwt_color_index *ci, u64 num_colors, const std::vector<std::
     bool ok = true;
     for (size_t fi = 0; ok && fi < files.size(); ++fi) {
       const u64 f = files[fi];
-      sbwt_host::IndexFileIn file(in[f]);
+      std::unique_ptr<sbwt_host::IndexFileIn> file;
+      try {
+        file = std::make_unique<sbwt_host::IndexFileIn>(in[f]);
+      } catch (const sbwt_host::InputMissing &e) {
+        std::fprintf(stderr, "%s\n", e.what());  // the file's output holds headers only
+      }
       ColorInBatch b;
       ok = free_in.pop(b.body);
       b.file = f;
       b.first = true;
-      b.format = file.format();
-      bool eof = false;
+      b.format = file ? file->format() : SBWT_INDEX_FILE_ASCII;
+      bool eof = !file;
       Clock clk;
       while (ok) {
         if (!eof) {
-          const u64 got = file.read(b.body.p + b.n, max_bytes - b.n);
+          const u64 got = file->read(b.body.p + b.n, max_bytes - b.n);
           eof = b.n + got < max_bytes;
           b.n += got;
         }

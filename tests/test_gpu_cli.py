@@ -138,3 +138,48 @@ def test_colors_cli_formats_batches_and_streams(tmp_path, in_mode):
                     head = struct.pack("<Q", len(name)) + name + struct.pack("<Q", 4) + b"v1.0"
                 want = head + O.color_lists_format(offsets, ids, nc, m)
                 assert open(outs[i] + ext, "rb").read() == want, (streams, mode, i)
+
+
+def test_unreadable_inputs_are_skipped(tmp_path):
+    """A missing input file (or an index file of unknown format) is reported and
+    skipped; its output file holds the headers only and the other files are
+    processed (ContinuousSequenceFileParser.cpp:113-132,
+    ContinuousIndexFileParser.cpp:64-100)."""
+    import struct
+    sbwt = os.path.join(FIXTURES, "search_test_index.sbwt")
+    good = os.path.join(FIXTURES, "index_search", "fasta2.fna")
+    missing = str(tmp_path / "missing.fna")
+    (tmp_path / "in.list").write_text(f"{good}\n{missing}\n{good}\n")
+    outs = [str(tmp_path / f"o{i}") for i in range(3)]
+    (tmp_path / "out.list").write_text("\n".join(outs) + "\n")
+    r = run(["index", "-q", str(tmp_path / "in.list"), "-i", sbwt, "-o",
+             str(tmp_path / "out.list"), "-s", "1"])
+    assert "missing.fna cannot be opened" in r.stderr
+    exp = open(os.path.join(FIXTURES, "index_search", "expected", "fasta2.txt"), "rb").read()
+    head = struct.pack("<Q", 5) + b"ascii" + struct.pack("<Q", 4) + b"v1.0"
+    assert [open(o + ".txt", "rb").read() for o in outs] == [exp, head, exp]
+    # colors: a missing index file and one of unknown format next to a good one
+    from colorlib import ColoredIndex
+    ci = ColoredIndex(tmp_path, n_genomes=4, length=8_000, d=3)
+    chars, ends = ci.reads(40, 100, seed=3)
+    fa = tmp_path / "r.fna"
+    st = 0
+    with open(fa, "wb") as f:
+        for j, e in enumerate(ends):
+            f.write(b">r%d\n" % j + chars[st:int(e)] + b"\n")
+            st = int(e)
+    run(["index", "-q", str(fa), "-i", ci.sbwt_path, "-k", ci.colors_path, "-o",
+         str(tmp_path / "idx")])
+    bad = tmp_path / "bad.txt"
+    bad.write_bytes(struct.pack("<Q", 4) + b"text" + struct.pack("<Q", 4) + b"v1.0" + b"1 2\n")
+    ins = [str(tmp_path / "idx.txt"), str(tmp_path / "nope.txt"), str(bad)]
+    (tmp_path / "cin.list").write_text("\n".join(ins) + "\n")
+    couts = [str(tmp_path / f"c{i}") for i in range(3)]
+    (tmp_path / "cout.list").write_text("\n".join(couts) + "\n")
+    r = run(["colors", "-q", str(tmp_path / "cin.list"), "-k", ci.colors_path, "-o",
+             str(tmp_path / "cout.list"), "-s", "1", "-t", "0.7"])
+    assert "nope.txt cannot be opened" in r.stderr
+    assert "Invalid file format in file: " + str(bad) in r.stderr
+    col = [open(o + ".txt", "rb").read() for o in couts]
+    assert col[1] == col[2] == head
+    assert len(col[0]) > len(head) and col[0].count(b"\n") == len(ends)
