@@ -35,50 +35,86 @@ inline u64 host_intv(const HostIntVector &v, u64 i) {
 
 // ---------------------------------------------------------------------------
 // Color set of a found (key) SBWT index: key-mark rank -> color_set_idx ->
-// set table (ColorSearcher.cuh:99-131). Adds `m` to the LDS counter of every
-// color of the set (the per-color ballot/popcount of ColorSearcher.cuh:175-205
-// summed over the read by ColorPostProcessor.cuh:21-43).
+// set table (ColorSearcher.cuh:99-131): the payload's start (dense: first
+// bit; sparse: first id) and length (dense: bits; sparse: ids).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void add_color_set(const ColorView &cv, u64 s, u32 m, u32 *cnt) {
+struct SetRef {
+  u64 start = 0;
+  u32 len = 0;
+  bool dense = false;
+};
+
+__device__ __forceinline__ SetRef decode_set(const ColorView &cv, u64 s) {
+  SetRef r;
   const u64 j = rank_marks(cv.key_marks, s);
   const u64 cs = intv_get(cv.color_set_idxs, cv.csi_width, j);
-  if (cs >= cv.num_color_sets) return;
+  if (cs >= cv.num_color_sets) return r;  // len 0: no colors
   const uint4 e = cv.set_table[cs];
-  const u64 start = static_cast<u64>(e.x) | (static_cast<u64>(e.y) << 32);
-  const u32 len = e.z;
-  if (e.w & 1u) {
-    for (u32 b = 0; b < len; b += 64) {
-      const u64 bit = start + b, w = bit >> 6;
-      const u32 off = static_cast<u32>(bit & 63);
-      u64 x = cv.dense_arrays[w] >> off;
-      if (off) x |= cv.dense_arrays[w + 1] << (64 - off);
-      const u32 take = min(64u, len - b);
-      if (take < 64) x &= (1ULL << take) - 1;
-      while (x) {
-        const u32 c = b + static_cast<u32>(__builtin_ctzll(x));
-        if (c < cv.num_colors) atomicAdd(&cnt[c], m);
-        x &= x - 1;
-      }
-    }
-  } else {
-    for (u32 i = 0; i < len; ++i) {
-      const u64 c = intv_get(cv.sparse_arrays, cv.sparse_width, start + i);
-      if (c < cv.num_colors) atomicAdd(&cnt[c], m);
-    }
-  }
+  r.start = static_cast<u64>(e.x) | (static_cast<u64>(e.y) << 32);
+  r.len = e.z;
+  r.dense = (e.w & 1u) != 0;
+  return r;
 }
 
-// One wave-wide pass over 64 consecutive result slots of a read: lanes whose
-// index equals the previous lane's share one color-set decode (consecutive
-// k-mers of a read usually walk to the same key k-mer).
+// One wave-wide pass over 64 consecutive result slots of a read. Lanes whose
+// index equals the previous lane's share one decode (consecutive k-mers of a
+// read usually walk to the same key k-mer): the first lane of each run
+// decodes the set, and the whole wave then walks the payloads of all the
+// window's sets together, one unit per lane (a sparse id, or a bit of a
+// dense bitmap), adding the run length to the LDS counter of each color
+// (the per-color ballot/popcount of ColorSearcher.cuh:175-205 summed over the
+// read by ColorPostProcessor.cuh:21-43). All payload loads of a window are
+// in flight at once.
 __device__ __forceinline__ void wave_add(const ColorView &cv, u64 v, bool found, u32 lane, u32 *cnt) {
   const u64 prev = __shfl_up(v, 1, kWave);
   const bool same = found && lane > 0 && prev == v;
   const u64 same_mask = __ballot(same);
-  if (found && !same) {
+  const bool leader = found && !same;
+  u32 run = 0, units = 0;
+  SetRef set;
+  if (leader) {
     const u64 rest = lane == kWave - 1 ? 0 : (same_mask >> (lane + 1));
-    const u32 run = 1 + (rest == ~0ULL ? 64u : static_cast<u32>(__builtin_ctzll(~rest)));
-    add_color_set(cv, v, run, cnt);
+    run = 1 + (rest == ~0ULL ? 64u : static_cast<u32>(__builtin_ctzll(~rest)));
+    set = decode_set(cv, v);
+    units = set.len;
+  }
+  // inclusive prefix of the units over the lanes; a unit t belongs to the
+  // first lane whose prefix exceeds t
+  u32 incl = units;
+#pragma unroll
+  for (u32 d = 1; d < kWave; d <<= 1) {
+    const u32 y = __shfl_up(incl, d, kWave);
+    if (lane >= d) incl += y;
+  }
+  const u32 total = __shfl(incl, kWave - 1, kWave);
+  for (u32 t0 = 0; t0 < total; t0 += kWave) {
+    const u32 t = t0 + lane;
+    u32 lo = 0, hi = kWave - 1;
+#pragma unroll
+    for (u32 step = 0; step < 6; ++step) {  // 64 lanes: six halvings
+      const u32 mid = (lo + hi) >> 1;
+      if (__shfl(incl, static_cast<int>(mid), kWave) > t) hi = mid;
+      else lo = mid + 1;
+    }
+    const u32 owner = lo;
+    const u32 o_incl = __shfl(incl, static_cast<int>(owner), kWave);
+    const u32 o_units = __shfl(units, static_cast<int>(owner), kWave);
+    const u32 o_run = __shfl(run, static_cast<int>(owner), kWave);
+    const u32 o_lo = __shfl(static_cast<u32>(set.start), static_cast<int>(owner), kWave);
+    const u32 o_hi = __shfl(static_cast<u32>(set.start >> 32), static_cast<int>(owner), kWave);
+    const bool o_dense = __shfl(set.dense ? 1u : 0u, static_cast<int>(owner), kWave) != 0;
+    if (t < total) {
+      const u32 u = t - (o_incl - o_units);
+      const u64 start = static_cast<u64>(o_lo) | (static_cast<u64>(o_hi) << 32);
+      u64 c;
+      if (o_dense) {
+        const u64 bit = start + u;
+        c = ((cv.dense_arrays[bit >> 6] >> (bit & 63)) & 1ULL) ? u : ~0ULL;
+      } else {
+        c = intv_get(cv.sparse_arrays, cv.sparse_width, start + u);
+      }
+      if (c < cv.num_colors) atomicAdd(&cnt[c], o_run);
+    }
   }
 }
 
