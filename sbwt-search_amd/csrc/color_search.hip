@@ -35,8 +35,10 @@ inline u64 host_intv(const HostIntVector &v, u64 i) {
 
 // ---------------------------------------------------------------------------
 // Color set of a found (key) SBWT index: key-mark rank -> color_set_idx ->
-// set table (ColorSearcher.cuh:99-131): the payload's start (dense: first
-// bit; sparse: first id) and length (dense: bits; sparse: ids).
+// set table (ColorSearcher.cuh:99-131), the last two folded at load into one
+// table by key rank (two dependent loads: the mark block, the entry): the
+// payload's start (dense: first bit; sparse: first id) and length (dense:
+// bits; sparse: ids).
 // ---------------------------------------------------------------------------
 struct SetRef {
   u64 start = 0;
@@ -46,10 +48,7 @@ struct SetRef {
 
 __device__ __forceinline__ SetRef decode_set(const ColorView &cv, u64 s) {
   SetRef r;
-  const u64 j = rank_marks(cv.key_marks, s);
-  const u64 cs = intv_get(cv.color_set_idxs, cv.csi_width, j);
-  if (cs >= cv.num_color_sets) return r;  // len 0: no colors
-  const uint4 e = cv.set_table[cs];
+  const uint4 e = cv.key_sets[rank_marks(cv.key_marks, s)];
   r.start = static_cast<u64>(e.x) | (static_cast<u64>(e.y) << 32);
   r.len = e.z;
   r.dense = (e.w & 1u) != 0;
@@ -276,6 +275,20 @@ inline u32 waves_per_block(u64 nc) {
   return static_cast<u32>(w);
 }
 
+// key_sets[j] = set_table[color_set_idxs[j]], or an empty entry for an index
+// past the sets. j runs to the number of keys inclusive: a node past the last
+// key ranks there and reads the vector's zero padding word, as the
+// reference's lookup does.
+__global__ __launch_bounds__(256) void k_key_sets(const u64 *__restrict__ csi, u32 width,
+                                                   const uint4 *__restrict__ table, u64 n_sets,
+                                                   u64 n_keys, uint4 *__restrict__ out) {
+  for (u64 j = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; j < n_keys;
+       j += static_cast<u64>(gridDim.x) * blockDim.x) {
+    const u64 cs = intv_get(csi, width, j);
+    out[j] = cs < n_sets ? table[cs] : make_uint4(0, 0, 0, 0);
+  }
+}
+
 }  // namespace
 
 void DeviceColors::build(const HostColors &c) {
@@ -312,23 +325,30 @@ void DeviceColors::build(const HostColors &c) {
     table[cs] = e;
   }
   set_table = dev_upload(table.data(), table.size());
+  num_keys = c.color_set_idxs.size;
+  SBWT_HIP(hipMalloc(&key_sets, (num_keys + 1) * sizeof(uint4)));
+  k_key_sets<<<grid_for(num_keys + 1, 256), 256>>>(color_set_idxs, csi_width, set_table,
+                                                    num_color_sets, num_keys + 1, key_sets);
+  SBWT_HIP(hipGetLastError());
+  SBWT_HIP(hipDeviceSynchronize());
   std::vector<u64> dense = c.dense_arrays.words;
   dense.push_back(0);
   dense_arrays = dev_upload(dense.data(), dense.size());
   std::vector<u64> sparse = c.sparse_arrays.words;
   sparse.push_back(0);
   sparse_arrays = dev_upload(sparse.data(), sparse.size());
-  device_bytes = key_marks.n_blocks * 64 + csi.size() * 8 + table.size() * 16 + dense.size() * 8 +
-                 sparse.size() * 8;
+  device_bytes = key_marks.n_blocks * 64 + csi.size() * 8 + table.size() * 16 + (num_keys + 1) * 16 +
+                 dense.size() * 8 + sparse.size() * 8;
 }
 
 void DeviceColors::release() {
   key_marks.release();
-  void *bufs[] = {color_set_idxs, set_table, dense_arrays, sparse_arrays};
+  void *bufs[] = {color_set_idxs, set_table, key_sets, dense_arrays, sparse_arrays};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   color_set_idxs = nullptr;
   set_table = nullptr;
+  key_sets = nullptr;
   dense_arrays = nullptr;
   sparse_arrays = nullptr;
 }
@@ -338,6 +358,7 @@ ColorView DeviceColors::view() const {
   v.key_marks = key_marks.blocks;
   v.color_set_idxs = color_set_idxs;
   v.set_table = set_table;
+  v.key_sets = key_sets;
   v.dense_arrays = dense_arrays;
   v.sparse_arrays = sparse_arrays;
   v.num_nodes = num_nodes;

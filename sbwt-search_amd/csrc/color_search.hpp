@@ -16,6 +16,7 @@ struct ColorView {
   const u64 *key_marks;       // mark blocks of the key-k-mer marks
   const u64 *color_set_idxs;  // sdsl int_vector words (+ padding)
   const uint4 *set_table;
+  const uint4 *key_sets;      // set-table entry of every key k-mer, by key rank
   const u64 *dense_arrays;
   const u64 *sparse_arrays;
   u64 num_nodes;
@@ -29,6 +30,8 @@ struct DeviceColors {
   DeviceMarks key_marks;
   u64 *color_set_idxs = nullptr;
   uint4 *set_table = nullptr;
+  uint4 *key_sets = nullptr;  // [key rank, 0..num_keys] = set_table[color_set_idxs[key rank]] (len 0 if none)
+  u64 num_keys = 0;
   u64 *dense_arrays = nullptr;
   u64 *sparse_arrays = nullptr;
   u64 num_nodes = 0, num_color_sets = 0, num_colors = 0;
