@@ -59,7 +59,7 @@ __device__ __forceinline__ SetRef decode_set(const ColorView &cv, u64 s) {
 // index equals the previous lane's share one decode (consecutive k-mers of a
 // read usually walk to the same key k-mer): the first lane of each run
 // decodes the set, and the whole wave then walks the payloads of all the
-// window's sets together, one unit per lane (a sparse id, or a bit of a
+// window's sets together, one unit per lane (a sparse id, or 64 bits of a
 // dense bitmap), adding the run length to the LDS counter of each color
 // (the per-color ballot/popcount of ColorSearcher.cuh:175-205 summed over the
 // read by ColorPostProcessor.cuh:21-43). All payload loads of a window are
@@ -75,7 +75,7 @@ __device__ __forceinline__ void wave_add(const ColorView &cv, u64 v, bool found,
     const u64 rest = lane == kWave - 1 ? 0 : (same_mask >> (lane + 1));
     run = 1 + (rest == ~0ULL ? 64u : static_cast<u32>(__builtin_ctzll(~rest)));
     set = decode_set(cv, v);
-    units = set.len;
+    units = set.dense ? (set.len + 63) / 64 : set.len;
   }
   // inclusive prefix of the units over the lanes; a unit t belongs to the
   // first lane whose prefix exceeds t
@@ -101,18 +101,28 @@ __device__ __forceinline__ void wave_add(const ColorView &cv, u64 v, bool found,
     const u32 o_run = __shfl(run, static_cast<int>(owner), kWave);
     const u32 o_lo = __shfl(static_cast<u32>(set.start), static_cast<int>(owner), kWave);
     const u32 o_hi = __shfl(static_cast<u32>(set.start >> 32), static_cast<int>(owner), kWave);
+    const u32 o_len = __shfl(set.len, static_cast<int>(owner), kWave);
     const bool o_dense = __shfl(set.dense ? 1u : 0u, static_cast<int>(owner), kWave) != 0;
     if (t < total) {
       const u32 u = t - (o_incl - o_units);
       const u64 start = static_cast<u64>(o_lo) | (static_cast<u64>(o_hi) << 32);
-      u64 c;
       if (o_dense) {
-        const u64 bit = start + u;
-        c = ((cv.dense_arrays[bit >> 6] >> (bit & 63)) & 1ULL) ? u : ~0ULL;
+        // 64 bits of the bitmap per unit: colors 64u .. 64u + 63
+        const u64 bit = start + 64 * static_cast<u64>(u), w = bit >> 6;
+        const u32 off = static_cast<u32>(bit & 63);
+        u64 x = cv.dense_arrays[w] >> off;
+        if (off) x |= cv.dense_arrays[w + 1] << (64 - off);
+        const u32 take = min(64u, o_len - 64 * u);
+        if (take < 64) x &= (1ULL << take) - 1;
+        while (x) {
+          const u64 c = 64 * static_cast<u64>(u) + static_cast<u32>(__builtin_ctzll(x));
+          if (c < cv.num_colors) atomicAdd(&cnt[c], o_run);
+          x &= x - 1;
+        }
       } else {
-        c = intv_get(cv.sparse_arrays, cv.sparse_width, start + u);
+        const u64 c = intv_get(cv.sparse_arrays, cv.sparse_width, start + u);
+        if (c < cv.num_colors) atomicAdd(&cnt[c], o_run);
       }
-      if (c < cv.num_colors) atomicAdd(&cnt[c], o_run);
     }
   }
 }
