@@ -36,7 +36,7 @@ inline u64 host_intv(const HostIntVector &v, u64 i) {
 // ---------------------------------------------------------------------------
 // Color set of a found (key) SBWT index: key-mark rank -> color_set_idx ->
 // set table (ColorSearcher.cuh:99-131), the last two folded at load into one
-// table by key rank (two dependent loads: the mark block, the entry): the
+// table by key rank (two dependent loads: the mark entry, the set entry): the
 // payload's start (dense: first bit; sparse: first id) and length (dense:
 // bits; sparse: ids).
 // ---------------------------------------------------------------------------
@@ -46,37 +46,32 @@ struct SetRef {
   bool dense = false;
 };
 
-__device__ __forceinline__ SetRef decode_set(const ColorView &cv, u64 s) {
-  SetRef r;
-  const uint4 e = cv.key_sets[rank_marks(cv.key_marks, s)];
-  r.start = static_cast<u64>(e.x) | (static_cast<u64>(e.y) << 32);
-  r.len = e.z;
-  r.dense = (e.w & 1u) != 0;
-  return r;
-}
 
-// One wave-wide pass over 64 consecutive result slots of a read. Lanes whose
-// index equals the previous lane's share one decode (consecutive k-mers of a
-// read usually walk to the same key k-mer): the first lane of each run
-// decodes the set, and the whole wave then walks the payloads of all the
-// window's sets together, one unit per lane (a sparse id, or 64 bits of a
-// dense bitmap), adding the run length to the LDS counter of each color
-// (the per-color ballot/popcount of ColorSearcher.cuh:175-205 summed over the
-// read by ColorPostProcessor.cuh:21-43). All payload loads of a window are
-// in flight at once.
-__device__ __forceinline__ void wave_add(const ColorView &cv, u64 v, bool found, u32 lane, u32 *cnt) {
+// Lanes whose index equals the previous lane's share one decode (consecutive
+// k-mers of a read usually walk to the same key k-mer): the first lane of
+// each run (the leader) decodes the set, with `run` = the run's length
+// within the window of 64 slots.
+__device__ __forceinline__ bool wave_leader(u64 v, bool found, u32 lane, u32 &run) {
   const u64 prev = __shfl_up(v, 1, kWave);
   const bool same = found && lane > 0 && prev == v;
   const u64 same_mask = __ballot(same);
   const bool leader = found && !same;
-  u32 run = 0, units = 0;
-  SetRef set;
+  run = 0;
   if (leader) {
     const u64 rest = lane == kWave - 1 ? 0 : (same_mask >> (lane + 1));
     run = 1 + (rest == ~0ULL ? 64u : static_cast<u32>(__builtin_ctzll(~rest)));
-    set = decode_set(cv, v);
-    units = set.dense ? (set.len + 63) / 64 : set.len;
   }
+  return leader;
+}
+
+// The payloads of one window's decoded sets, walked by the whole wave: one
+// unit per lane (a sparse id, or 64 bits of a dense bitmap), adding the
+// run length to the LDS counter of each color (the per-color
+// ballot/popcount of ColorSearcher.cuh:175-205 summed over the read by
+// ColorPostProcessor.cuh:21-43). All payload loads of the window are in
+// flight at once.
+__device__ __forceinline__ void wave_payload(const ColorView &cv, const SetRef &set, u32 units,
+                                             u32 run, u32 lane, u32 *cnt) {
   // inclusive prefix of the units over the lanes; a unit t belongs to the
   // first lane whose prefix exceeds t
   u32 incl = units;
@@ -127,6 +122,35 @@ __device__ __forceinline__ void wave_add(const ColorView &cv, u64 v, bool found,
   }
 }
 
+// One window of 64 result slots starting at slot `base` of a read ending at
+// `ke`: counts found / not-found / invalid slots and adds every found
+// k-mer's colors to the LDS counters. A run's leader decodes its set with
+// two dependent loads (the key-mark entry, the key set); decoding several
+// windows together did not pay on gfx950 (more registers, fewer waves).
+__device__ __forceinline__ void window_add(const ColorView &cv, const u64 *__restrict__ results,
+                                           u64 base, u64 ke, u32 lane, u32 *cnt, u32 &f, u32 &nf,
+                                           u32 &inv) {
+  const u64 i = base + lane;
+  const bool in = i < ke;
+  const u64 v = in ? results[i] : kInvalid;
+  const bool found = in && v < cv.num_nodes;
+  f += found;
+  inv += in && v == kInvalid;
+  nf += in && !found && v != kInvalid;
+  u32 run = 0;
+  const bool lead = wave_leader(v, found, lane, run);
+  SetRef set;
+  u32 units = 0;
+  if (lead) {
+    const uint4 e = cv.key_sets[rank_marks(cv.key_marks, v)];
+    set.start = static_cast<u64>(e.x) | (static_cast<u64>(e.y) << 32);
+    set.len = e.z;
+    set.dense = (e.w & 1u) != 0;
+    units = set.dense ? (set.len + 63) / 64 : set.len;
+  }
+  wave_payload(cv, set, units, run, lane, cnt);
+}
+
 __device__ __forceinline__ u32 wave_sum(u32 x) {
   for (u32 o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
   return x;
@@ -162,16 +186,7 @@ __global__ __launch_bounds__(256) void k_color_reads(ColorView cv, const u64 *__
        r += static_cast<u64>(gridDim.x) * waves) {
     const u64 ks = r ? kmer_ends[r - 1] : 0, ke = kmer_ends[r];
     u32 f = 0, nf = 0, inv = 0;
-    for (u64 base = ks; base < ke; base += kWave) {
-      const u64 i = base + lane;
-      const bool in = i < ke;
-      const u64 v = in ? results[i] : kInvalid;
-      const bool found = in && v < cv.num_nodes;
-      f += found;
-      inv += in && v == kInvalid;
-      nf += in && !found && v != kInvalid;
-      wave_add(cv, v, found, lane, cnt);
-    }
+    for (u64 base = ks; base < ke; base += kWave) window_add(cv, results, base, ke, lane, cnt, f, nf, inv);
     f = wave_sum(f);
     nf = wave_sum(nf);
     inv = wave_sum(inv);
@@ -260,10 +275,9 @@ __global__ __launch_bounds__(256) void k_color_warped(ColorView cv, const u64 *_
   wave_lds_fence();
   for (u64 q = static_cast<u64>(blockIdx.x) * waves + wave; q < n_colored;
        q += static_cast<u64>(gridDim.x) * waves) {
-    for (u64 w = intervals[q]; w < intervals[q + 1]; ++w) {
-      const u64 v = warped[w * kWave + lane];
-      wave_add(cv, v, v < cv.num_nodes, lane, cnt);
-    }
+    u32 f = 0, nf = 0, inv = 0;  // padding reads as not found
+    const u64 wbeg = intervals[q] * kWave, wend = intervals[q + 1] * kWave;
+    for (u64 base = wbeg; base < wend; base += kWave) window_add(cv, warped, base, wend, lane, cnt, f, nf, inv);
     wave_lds_fence();
     for (u32 c = lane; c < nc; c += kWave) {
       counts[q * nc + c] = cnt[c];
@@ -347,7 +361,7 @@ void DeviceColors::build(const HostColors &c) {
   std::vector<u64> sparse = c.sparse_arrays.words;
   sparse.push_back(0);
   sparse_arrays = dev_upload(sparse.data(), sparse.size());
-  device_bytes = key_marks.n_blocks * 64 + csi.size() * 8 + table.size() * 16 + (num_keys + 1) * 16 +
+  device_bytes = key_marks.n_blocks * 16 + csi.size() * 8 + table.size() * 16 + (num_keys + 1) * 16 +
                  dense.size() * 8 + sparse.size() * 8;
 }
 

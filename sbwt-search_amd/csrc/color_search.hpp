@@ -13,7 +13,7 @@ namespace sbwt_amd {
 // This fuses the reference's is_dense bit, is_dense rank and the two interval
 // reads (src/ColorSearcher/ColorSearcher.cuh:111-131, 208-290) into one line.
 struct ColorView {
-  const u64 *key_marks;       // mark blocks of the key-k-mer marks
+  const u64 *key_marks;       // mark entries {count, 64 bits} of the key-k-mer marks
   const u64 *color_set_idxs;  // sdsl int_vector words (+ padding)
   const uint4 *set_table;
   const uint4 *key_sets;      // set-table entry of every key k-mer, by key rank

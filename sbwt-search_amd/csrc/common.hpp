@@ -48,15 +48,15 @@ constexpr u32 kKeyTableSteps = 4096;  // walk cap of the table build
 // Counts are relative to the line's hyperblock (2^25 lines) whose absolute
 // bases sit in a small table; indexes below 2^32 nodes have a single
 // hyperblock with base 0, so a rank is one 16-byte load and a popcount.
-// The color index keeps its marks in 64-byte "mark blocks": {u64 count,
-// 7 words} for 448 bits.
+// The color index keeps its key marks as 16-byte "mark entries": {u64 count,
+// 64 bits}.
 // ---------------------------------------------------------------------------
 constexpr u32 kBlockNodes = 96;
 constexpr u32 kLineSlots = 8;
 constexpr u32 kSlotGroups = 4;
 constexpr u32 kSlotMarks = 5;
 constexpr u32 kHyperShift = 25;  // lines per hyperblock = 2^25 (3.2 G nodes)
-constexpr u32 kMarkBits = 448;
+constexpr u32 kMarkBits = 64;  // bits per mark entry of the color index
 constexpr u32 kFarGroup = 0xFFFFFFFFu;
 
 struct IndexView {

@@ -70,26 +70,10 @@ __device__ __forceinline__ void sbwt_step(const IndexView &ix, u32 c, Idx &l, Id
   r = cm + entry_base(ix, er, br, c) + entry_popc(er, static_cast<u32>(r1 - br * kBlockNodes)) - 1;
 }
 
-// rank over [0, i) on mark blocks {u64 count, 7 words}.
+// rank over [0, i) on mark entries {u64 count, u64 bits} (one 16-byte load).
 __device__ __forceinline__ u64 rank_marks(const u64 *mb, u64 i) {
-  const u64 b = i / kMarkBits;
-  const u32 o = static_cast<u32>(i - b * kMarkBits);
-  const u64 *blk = mb + b * 8;
-  u64 r = blk[0];
-  const u32 full = o >> 6;
-#pragma unroll
-  for (u32 w = 0; w < 7; ++w) {
-    const u64 x = blk[1 + w];
-    if (w < full) r += __popcll(x);
-    else if (w == full && (o & 63)) r += __popcll(x & ((1ULL << (o & 63)) - 1));
-  }
-  return r;
-}
-
-__device__ __forceinline__ bool mark_bit(const u64 *mb, u64 i) {
-  const u64 b = i / kMarkBits;
-  const u32 o = static_cast<u32>(i - b * kMarkBits);
-  return (mb[b * 8 + 1 + (o >> 6)] >> (o & 63)) & 1ULL;
+  const ulonglong2 e = reinterpret_cast<const ulonglong2 *>(mb)[i >> 6];
+  return e.x + __popcll(e.y & ((1ULL << (i & 63)) - 1));
 }
 
 // Key-k-mer move (src/IndexSearcher/IndexSearcher.cuh:60-70): while node l

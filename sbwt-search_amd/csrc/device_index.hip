@@ -1,4 +1,4 @@
-// Device index construction: ACGT rank blocks and mark blocks (built on the
+// Device index construction: ACGT rank blocks and mark entries (built on the
 // host, uploaded once) and the presearch table (built on the device).
 #include <algorithm>
 #include <cstdlib>
@@ -144,14 +144,12 @@ void build_presearch(const IndexView &ix, u32 kp, Pair *out) {
 void DeviceMarks::build(const u64 *words, u64 bits) {
   n_bits = bits;
   n_blocks = bits / kMarkBits + 1;  // rank(n_bits) stays in range
-  std::vector<u64> host(n_blocks * 8, 0);
-  parallel_for(n_blocks, [&](u64 b) {
-    for (u32 w = 0; w < 7; ++w) host[b * 8 + 1 + w] = get_bits(words, bits, b * kMarkBits + 64 * w, 64);
-  });
+  std::vector<u64> host(n_blocks * 2, 0);
+  parallel_for(n_blocks, [&](u64 b) { host[b * 2 + 1] = get_bits(words, bits, b * kMarkBits, 64); });
   u64 run = 0;
   for (u64 b = 0; b < n_blocks; ++b) {
-    host[b * 8] = run;
-    for (u32 w = 0; w < 7; ++w) run += __builtin_popcountll(host[b * 8 + 1 + w]);
+    host[b * 2] = run;
+    run += __builtin_popcountll(host[b * 2 + 1]);
   }
   total = run;
   blocks = upload(host.data(), host.size());

@@ -12,10 +12,10 @@
 
 namespace sbwt_amd {
 
-// Mark blocks of a single bit vector: per 448 bits one 64-byte line
-// {u64 ones before the block, 7 words}. rank over [0, i) = one line.
+// Mark entries of a single bit vector: per 64 bits one 16-byte entry
+// {u64 ones before it, the 64 bits}. rank over [0, i) = one 16-byte load.
 struct DeviceMarks {
-  u64 *blocks = nullptr;  // [n_blocks][8]
+  u64 *blocks = nullptr;  // [n_blocks][2]
   u64 n_bits = 0, n_blocks = 0, total = 0;
   void build(const u64 *words, u64 n_bits);
   void release();
