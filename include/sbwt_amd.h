@@ -394,6 +394,48 @@ int sbwt_index_file_parser_parse(sbwt_index_file_parser *parser, int format, con
 uint64_t sbwt_index_file_batch_cut(int format, const char *body, uint64_t n_bytes,
                                    uint64_t max_reads);
 
+/* Sequence-file chunks parsed on the device, the input side of the index
+ * command: raw FASTA bytes that start at a record's header line and end at a
+ * record's end -> the concatenated sequence chars and the cumulative end of
+ * every read, the batch form that sbwt_searcher_search_reads takes. The
+ * grammar is the reference's record reader (kseq++ behind
+ * src/SequenceFileParser/ContinuousSequenceFileParser.cpp:72-112,148-156)
+ * on input with no line starting with '+': a line starting with '>' or '@'
+ * starts a record (its header is skipped), every other line is sequence whose
+ * bytes other than '\n' are chars. A chunk holding a line that starts with '+'
+ * (FASTQ quality) is rejected; such input stays on a host reader.
+ * max_bytes_per_chunk < 2^32. */
+typedef struct sbwt_fasta_parser sbwt_fasta_parser;
+int sbwt_fasta_parser_create(uint64_t max_bytes_per_chunk, uint64_t max_reads_per_chunk,
+                             void *hip_stream, sbwt_fasta_parser **out);
+void sbwt_fasta_parser_destroy(sbwt_fasta_parser *parser);
+/* Host bytes in, host chars and read ends out (capacities in entries);
+ * synchronous. Errors when raw does not start with '>' or '@', holds a line
+ * starting with '+', or more than max_reads_per_chunk records. */
+int sbwt_fasta_parser_parse(sbwt_fasta_parser *parser, const char *raw, uint64_t n_bytes,
+                            char *chars, uint64_t chars_capacity, uint64_t *read_ends,
+                            uint64_t reads_capacity, uint64_t *n_chars, uint64_t *n_reads);
+/* Longest prefix of raw[0, n_bytes) that a chunk may end with: it ends right
+ * before a line starting with '>' or '@' (or is all of raw when last != 0,
+ * the end of the file) and holds at most max_reads records. 0 if there is
+ * none (one record is longer than raw). Host only, no device work. */
+uint64_t sbwt_fasta_chunk_cut(const char *raw, uint64_t n_bytes, int last, uint64_t max_reads);
+/* 1 if raw[0, n_bytes) holds a line starting with '+' (its first byte
+ * counts when at_line_start != 0). Host only. */
+int sbwt_fasta_has_quality_line(const char *raw, uint64_t n_bytes, int at_line_start);
+/* One FASTA chunk -> device parse -> fused search -> formatted index output
+ * bytes (every read complete: the chunk ends at a record's end). *n_bytes and
+ * the copy-out follow sbwt_searcher_search_reads_format; *n_reads receives
+ * the records of the chunk. Synchronous. Replaces one batch of the
+ * reference's ContinuousSequenceFileParser + ContinuousIndexSearcher +
+ * ContinuousIndexResultsPrinter (ContinuousSequenceFileParser.cpp:72-112,
+ * src/IndexSearcher/ContinuousIndexSearcher.cpp:67-74,
+ * ContinuousIndexResultsPrinter.hpp:146-308). */
+int sbwt_searcher_search_fasta_format(sbwt_searcher *searcher, sbwt_fasta_parser *parser,
+                                      sbwt_formatter *formatter, int mode, const char *raw,
+                                      uint64_t n_raw, char *out, uint64_t capacity,
+                                      uint64_t *n_bytes, uint64_t *n_reads);
+
 /* Color id lists (the CSR of sbwt_color_searcher_pseudoalign_device) as the
  * reference's color printers write them, file headers excluded: ascii ids
  * joined by ' ' and a '\n' per read; binary a u64 per id and u64 max per read;

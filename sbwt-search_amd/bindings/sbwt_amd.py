@@ -125,6 +125,21 @@ def lib():
                                                    C.c_uint64, u64p, C.c_uint64, u64p, u64p]
         L.sbwt_index_file_batch_cut.argtypes = [C.c_int, C.c_char_p, C.c_uint64, C.c_uint64]
         L.sbwt_index_file_batch_cut.restype = C.c_uint64
+        L.sbwt_searcher_search_reads_format.argtypes = [vp, vp, C.c_int, C.c_char_p, C.c_uint64,
+                                                        u64p, C.c_char_p, C.c_uint64, C.c_int,
+                                                        C.c_char_p, C.c_uint64, u64p]
+        L.sbwt_formatter_copy_last.argtypes = [vp, C.c_char_p, C.c_uint64]
+        L.sbwt_fasta_parser_create.argtypes = [C.c_uint64, C.c_uint64, vp, C.POINTER(vp)]
+        L.sbwt_fasta_parser_destroy.argtypes = [vp]
+        L.sbwt_fasta_parser_destroy.restype = None
+        L.sbwt_fasta_parser_parse.argtypes = [vp, C.c_char_p, C.c_uint64, C.c_char_p, C.c_uint64,
+                                              u64p, C.c_uint64, u64p, u64p]
+        L.sbwt_fasta_chunk_cut.argtypes = [C.c_char_p, C.c_uint64, C.c_int, C.c_uint64]
+        L.sbwt_fasta_chunk_cut.restype = C.c_uint64
+        L.sbwt_fasta_has_quality_line.argtypes = [C.c_char_p, C.c_uint64, C.c_int]
+        L.sbwt_searcher_search_fasta_format.argtypes = [vp, vp, vp, C.c_int, C.c_char_p,
+                                                        C.c_uint64, C.c_char_p, C.c_uint64, u64p,
+                                                        u64p]
         L.sbwt_color_formatter_create.argtypes = [C.c_uint64, C.c_uint64, vp, C.POINTER(vp)]
         L.sbwt_color_formatter_destroy.argtypes = [vp]
         L.sbwt_color_formatter_destroy.restype = None
@@ -383,6 +398,72 @@ class Formatter:
     def close(self):
         if getattr(self, "h", None):
             lib().sbwt_formatter_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+def search_reads_format(searcher, formatter, mode, chars, read_ends, read_closes=None,
+                        line_open=False):
+    """Host reads -> search -> formatted index output bytes, in one C call."""
+    chars = bytes(chars)
+    ends = np.ascontiguousarray(read_ends, np.uint64)
+    closes = bytes(np.ones(len(ends), np.uint8) if read_closes is None
+                   else np.ascontiguousarray(read_closes, np.uint8))
+    n = C.c_uint64()
+    _check(lib().sbwt_searcher_search_reads_format(searcher.h, formatter.h, mode, chars, len(chars),
+                                                   _p64(ends), closes, len(ends), int(line_open),
+                                                   None, 0, C.byref(n)))
+    out = C.create_string_buffer(max(n.value, 1))
+    _check(lib().sbwt_formatter_copy_last(formatter.h, out, n.value))
+    return out.raw[:n.value]
+
+
+def fasta_chunk_cut(raw, last, max_reads):
+    """Longest prefix of FASTA bytes that a chunk may end with (0: none)."""
+    return int(lib().sbwt_fasta_chunk_cut(bytes(raw), len(raw), int(last), max_reads))
+
+
+def fasta_has_quality_line(raw, at_line_start=True):
+    return bool(lib().sbwt_fasta_has_quality_line(bytes(raw), len(raw), int(at_line_start)))
+
+
+class FastaParser:
+    """Device parser of FASTA chunks (raw bytes -> chars + read ends)."""
+
+    def __init__(self, max_bytes, max_reads, stream=None):
+        self.h = C.c_void_p()
+        _check(lib().sbwt_fasta_parser_create(max_bytes, max_reads,
+                                              C.c_void_p(stream) if stream else None,
+                                              C.byref(self.h)))
+        self.max_bytes = max_bytes
+        self.max_reads = max_reads
+
+    def parse(self, raw):
+        """Host bytes in -> (chars bytes, read_ends u64[])."""
+        raw = bytes(raw)
+        chars = C.create_string_buffer(max(len(raw), 1))
+        ends = np.zeros(self.max_reads, np.uint64)
+        nc, nr = C.c_uint64(), C.c_uint64()
+        _check(lib().sbwt_fasta_parser_parse(self.h, raw, len(raw), chars, len(raw), _p64(ends),
+                                             len(ends), C.byref(nc), C.byref(nr)))
+        return chars.raw[:nc.value], ends[:nr.value].copy()
+
+    def search_format(self, searcher, formatter, mode, raw):
+        """One chunk -> device parse -> search -> formatted bytes; (bytes, n_reads)."""
+        raw = bytes(raw)
+        n, nr = C.c_uint64(), C.c_uint64()
+        _check(lib().sbwt_searcher_search_fasta_format(searcher.h, self.h, formatter.h, mode, raw,
+                                                       len(raw), None, 0, C.byref(n),
+                                                       C.byref(nr)))
+        out = C.create_string_buffer(max(n.value, 1))
+        _check(lib().sbwt_formatter_copy_last(formatter.h, out, n.value))
+        return out.raw[:n.value], int(nr.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sbwt_fasta_parser_destroy(self.h)
             self.h = None
 
     def __del__(self):

@@ -12,6 +12,7 @@
 #include "index_parse.hpp"
 #include "index_search.hpp"
 #include "loaders.hpp"
+#include "seq_parse.hpp"
 #include "server.hpp"
 
 using sbwt_amd::DeviceIndex;
@@ -50,6 +51,14 @@ struct sbwt_index_file_parser {
 
 struct sbwt_color_formatter {
   std::unique_ptr<sbwt_amd::ColorFormatter> impl;
+};
+
+struct sbwt_fasta_parser {
+  std::unique_ptr<sbwt_amd::FastaParser> impl;
+  hipEvent_t parsed = nullptr;  // the chars and read ends of the last parse are written
+  ~sbwt_fasta_parser() {
+    if (parsed) (void)hipEventDestroy(parsed);
+  }
 };
 
 // The colors command's device stage: parser, color search and printer on the
@@ -560,6 +569,76 @@ uint64_t sbwt_index_file_batch_cut(int format, const char *body, uint64_t n_byte
                                    uint64_t max_reads) {
   if (!body || (format != SBWT_INDEX_FILE_ASCII && format != SBWT_INDEX_FILE_BINARY)) return 0;
   return sbwt_amd::IndexFileParser::batch_cut(format, body, n_bytes, max_reads);
+}
+
+int sbwt_fasta_parser_create(uint64_t max_bytes_per_chunk, uint64_t max_reads_per_chunk,
+                             void *hip_stream, sbwt_fasta_parser **out) {
+  return guarded([&] {
+    require(out, "null argument");
+    auto p = std::make_unique<sbwt_fasta_parser>();
+    p->impl = std::make_unique<sbwt_amd::FastaParser>(max_bytes_per_chunk, max_reads_per_chunk,
+                                                      static_cast<hipStream_t>(hip_stream));
+    SBWT_HIP(hipEventCreateWithFlags(&p->parsed, hipEventDisableTiming));
+    *out = p.release();
+  });
+}
+
+void sbwt_fasta_parser_destroy(sbwt_fasta_parser *parser) { delete parser; }
+
+int sbwt_fasta_parser_parse(sbwt_fasta_parser *parser, const char *raw, uint64_t n_bytes,
+                            char *chars, uint64_t chars_capacity, uint64_t *read_ends,
+                            uint64_t reads_capacity, uint64_t *n_chars, uint64_t *n_reads) {
+  return guarded([&] {
+    require(parser && n_chars && n_reads, "null argument");
+    require(n_bytes == 0 || raw, "null argument");
+    uint64_t nc = 0;
+    const uint64_t nr = parser->impl->parse_host(raw, n_bytes, &nc);
+    require(nc <= chars_capacity && nr <= reads_capacity, "output arrays too small for the chunk");
+    require((nc == 0 || chars) && (nr == 0 || read_ends), "null argument");
+    hipStream_t st = parser->impl->stream();
+    if (nc) SBWT_HIP(hipMemcpyAsync(chars, parser->impl->chars(), nc, hipMemcpyDeviceToHost, st));
+    if (nr)
+      SBWT_HIP(hipMemcpyAsync(read_ends, parser->impl->read_ends(), nr * 8, hipMemcpyDeviceToHost, st));
+    SBWT_HIP(hipStreamSynchronize(st));
+    *n_chars = nc;
+    *n_reads = nr;
+  });
+}
+
+uint64_t sbwt_fasta_chunk_cut(const char *raw, uint64_t n_bytes, int last, uint64_t max_reads) {
+  if (!raw) return 0;
+  return sbwt_amd::FastaParser::chunk_cut(raw, n_bytes, last != 0, max_reads);
+}
+
+int sbwt_fasta_has_quality_line(const char *raw, uint64_t n_bytes, int at_line_start) {
+  if (!raw) return 0;
+  return sbwt_amd::FastaParser::has_quality_line(raw, n_bytes, at_line_start != 0) ? 1 : 0;
+}
+
+int sbwt_searcher_search_fasta_format(sbwt_searcher *searcher, sbwt_fasta_parser *parser,
+                                      sbwt_formatter *formatter, int mode, const char *raw,
+                                      uint64_t n_raw, char *out, uint64_t capacity,
+                                      uint64_t *n_bytes, uint64_t *n_reads) {
+  return guarded([&] {
+    require(searcher && parser && formatter && n_bytes && n_reads, "null argument");
+    require(mode == SBWT_FORMAT_ASCII || mode == SBWT_FORMAT_BINARY || mode == SBWT_FORMAT_BOOL,
+            "unknown output format");
+    require(n_raw == 0 || raw, "null argument");
+    *n_bytes = 0;
+    *n_reads = 0;
+    uint64_t nc = 0;
+    const uint64_t nr = parser->impl->parse_host(raw, n_raw, &nc);
+    if (nr == 0) return;
+    Searcher &s = *searcher->impl;
+    SBWT_HIP(hipEventRecord(parser->parsed, parser->impl->stream()));
+    SBWT_HIP(hipStreamWaitEvent(s.stream(), parser->parsed, 0));
+    s.search_reads_device_staged(parser->impl->chars(), nc, parser->impl->read_ends(), nr);
+    SBWT_HIP(hipStreamSynchronize(s.stream()));
+    *n_bytes = formatter->impl->format_device_closes_to_host(
+        mode, s.results_device(), s.kmer_ends_device(), parser->impl->ones(), nr, false, out,
+        capacity);
+    *n_reads = nr;
+  });
 }
 
 int sbwt_color_formatter_create(uint64_t max_ids, uint64_t max_reads, void *hip_stream,

@@ -338,14 +338,24 @@ u64 Formatter::format_to_host(int mode, const u64 *d_results, const u64 *d_piece
   if (n_pieces > max_pieces_) throw std::runtime_error("batch exceeds the formatter's max_pieces");
   if (!d_closes_) SBWT_HIP(hipMalloc(&d_closes_, max_pieces_));
   SBWT_HIP(hipMemcpyAsync(d_closes_, host_closes, n_pieces, hipMemcpyHostToDevice, stream_));
-  const u64 bytes = format(mode, d_results, d_piece_kmer_ends, d_closes_, n_pieces, line_open,
+  return format_device_closes_to_host(mode, d_results, d_piece_kmer_ends, d_closes_, n_pieces,
+                                      line_open, host_out, capacity);
+}
+
+u64 Formatter::format_device_closes_to_host(int mode, const u64 *d_results,
+                                            const u64 *d_piece_kmer_ends, const u8 *d_closes,
+                                            u64 n_pieces, bool line_open, char *host_out,
+                                            u64 capacity) {
+  last_bytes_ = 0;
+  if (n_pieces == 0) return 0;
+  const u64 bytes = format(mode, d_results, d_piece_kmer_ends, d_closes, n_pieces, line_open,
                            nullptr, 0);
   if (bytes > out_capacity_) {
     if (d_out_) SBWT_HIP(hipFree(d_out_));
     out_capacity_ = bytes + bytes / 4 + 4096;
     SBWT_HIP(hipMalloc(&d_out_, out_capacity_));
   }
-  format(mode, d_results, d_piece_kmer_ends, d_closes_, n_pieces, line_open, d_out_, out_capacity_);
+  format(mode, d_results, d_piece_kmer_ends, d_closes, n_pieces, line_open, d_out_, out_capacity_);
   last_bytes_ = bytes;
   if (host_out && capacity >= bytes) copy_last(host_out, capacity);
   return bytes;

@@ -33,6 +33,10 @@ class Formatter {
   u64 format_to_host(int mode, const u64 *d_results, const u64 *d_piece_kmer_ends,
                      const u8 *host_closes, u64 n_pieces, bool line_open, char *host_out,
                      u64 capacity);
+  // The same with the closing flags already on the device.
+  u64 format_device_closes_to_host(int mode, const u64 *d_results, const u64 *d_piece_kmer_ends,
+                                   const u8 *d_closes, u64 n_pieces, bool line_open,
+                                   char *host_out, u64 capacity);
   void copy_last(char *host_out, u64 capacity);
 
  private:

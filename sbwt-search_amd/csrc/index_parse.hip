@@ -13,109 +13,36 @@
 #include <cstring>
 #include <stdexcept>
 
+#include "byte_automaton.hpp"
 #include "scan.hpp"
 
 namespace sbwt_amd {
 namespace {
 
-constexpr u32 kParseThreads = 256;
-constexpr u32 kSegBytes = 16;
-constexpr u64 kBlockBytes = u64{kParseThreads} * kSegBytes;   // ascii block: 4 KiB
-constexpr u64 kBinaryBlockBytes = u64{kParseThreads} * 8;      // binary block: 256 words
-constexpr u32 kStatesThreads = 1024;
+using automaton::block_prefix_sum;
+using automaton::kBlockBytes;
+using automaton::kSegBytes;
+using automaton::kStatesThreads;
+using automaton::load_segment;
+using automaton::next_state;
+constexpr u32 kParseThreads = automaton::kThreads;
+constexpr u64 kBinaryBlockBytes = u64{kParseThreads} * 8;  // binary block: 256 words
 
 // Automaton states: top level, inside a number, after '-', skipping the
 // digits after the char that follows '-'.
 constexpr u32 kTop = 0, kNum = 1, kDash = 2, kSkip = 3;
 // A state map packs the next state of start state s in bits [2s, 2s + 2).
-constexpr u32 kMapIdentity = 0 | 1 << 2 | 2 << 4 | 3 << 6;
 constexpr u32 kMapDigit = kNum | kNum << 2 | kSkip << 4 | kSkip << 6;
 constexpr u32 kMapMinus = kDash | kDash << 2 | kSkip << 4 | kTop << 6;
 constexpr u32 kMapOther = kTop | kTop << 2 | kSkip << 4 | kTop << 6;  // '\n' and the rest
 
 __device__ __forceinline__ bool is_digit(u8 b) { return static_cast<signed char>(b) >= '0'; }
 
-__device__ __forceinline__ u32 char_map(u8 b) {
-  return is_digit(b) ? kMapDigit : (b == '-' ? kMapMinus : kMapOther);
-}
-
-__device__ __forceinline__ u32 next_state(u32 map, u32 s) { return (map >> (2 * s)) & 3u; }
-
-// f, then g.
-__device__ __forceinline__ u32 compose(u32 f, u32 g) {
-  u32 h = 0;
-#pragma unroll
-  for (u32 s = 0; s < 4; ++s) h |= next_state(g, next_state(f, s)) << (2 * s);
-  return h;
-}
-
-// The bytes of segment `seg` (zero past n); returns how many are in range.
-__device__ __forceinline__ u32 load_segment(const u8 *body, u64 n, u64 seg, u8 (&b)[kSegBytes]) {
-  const u64 beg = seg * kSegBytes;
-  if (beg >= n) return 0;
-  const u32 len = static_cast<u32>(min<u64>(kSegBytes, n - beg));
-  const uint4 v = *reinterpret_cast<const uint4 *>(body + beg);
-  const u32 w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-  for (u32 i = 0; i < kSegBytes; ++i) b[i] = i < len ? static_cast<u8>(w[i / 4] >> (8 * (i % 4))) : 0;
-  return len;
-}
-
-__device__ __forceinline__ u32 segment_map(const u8 (&b)[kSegBytes], u32 len) {
-  u32 m = kMapIdentity;
-#pragma unroll
-  for (u32 i = 0; i < kSegBytes; ++i)
-    if (i < len) m = compose(m, char_map(b[i]));
-  return m;
-}
-
-// Exclusive prefix (in thread order) of the maps of a workgroup of WAVES
-// waves; *total gets the composition of all of them.
-template <u32 WAVES>
-__device__ __forceinline__ u32 block_scan_maps(u32 m, u32 *sh, u32 *total) {
-  const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  u32 incl = m;
-#pragma unroll
-  for (u32 d = 1; d < 64; d <<= 1) {
-    const u32 y = __shfl_up(incl, d, 64);
-    if (lane >= d) incl = compose(y, incl);
+struct AsciiMap {
+  __device__ __forceinline__ u32 operator()(u8 b) const {
+    return is_digit(b) ? kMapDigit : (b == '-' ? kMapMinus : kMapOther);
   }
-  if (lane == 63) sh[wave] = incl;
-  __syncthreads();
-  u32 before = kMapIdentity, all = kMapIdentity;
-#pragma unroll
-  for (u32 w = 0; w < WAVES; ++w) {
-    if (w < wave) before = compose(before, sh[w]);
-    all = compose(all, sh[w]);
-  }
-  __syncthreads();
-  u32 excl = __shfl_up(incl, 1, 64);
-  if (lane == 0) excl = kMapIdentity;
-  *total = all;
-  return compose(before, excl);
-}
-
-// Workgroup sum (every thread gets it) and exclusive prefix of x.
-__device__ __forceinline__ u32 block_prefix_sum(u32 x, u32 *sh, u32 *total) {
-  const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  u32 incl = x;
-#pragma unroll
-  for (u32 d = 1; d < 64; d <<= 1) {
-    const u32 y = __shfl_up(incl, d, 64);
-    if (lane >= d) incl += y;
-  }
-  if (lane == 63) sh[wave] = incl;
-  __syncthreads();
-  u32 before = 0, all = 0;
-#pragma unroll
-  for (u32 w = 0; w < kParseThreads / 64; ++w) {
-    if (w < wave) before += sh[w];
-    all += sh[w];
-  }
-  __syncthreads();
-  *total = all;
-  return before + incl - x;
-}
+};
 
 // u64 value of the digit run at body[pos] (reference parse_number: wrapping
 // arithmetic over every char >= '0').
@@ -152,48 +79,20 @@ __device__ __forceinline__ void walk_segment(const u8 *body, u64 n, u64 beg, con
       }
       r += (dash_result || number) ? 1u : 0u;
       nl += read_end ? 1u : 0u;
-      st = next_state(char_map(c), st);
+      st = next_state(AsciiMap{}(c), st);
     }
   }
 }
 
 __global__ __launch_bounds__(kParseThreads) void k_ascii_maps(const u8 *__restrict__ body, u64 n,
                                                               u64 n_blocks, u8 *__restrict__ maps) {
-  __shared__ u32 sh[kParseThreads / 64];
-  for (u64 blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
-    u8 b[kSegBytes];
-    const u32 len = load_segment(body, n, blk * kParseThreads + threadIdx.x, b);
-    u32 total;
-    block_scan_maps<kParseThreads / 64>(segment_map(b, len), sh, &total);
-    if (threadIdx.x == 0) maps[blk] = static_cast<u8>(total);
-  }
+  automaton::block_maps(body, n, n_blocks, maps, AsciiMap{});
 }
 
-// One workgroup: block maps -> block start states (in place is not used:
-// states go to their own array).
 __global__ __launch_bounds__(kStatesThreads) void k_ascii_states(const u8 *__restrict__ maps,
                                                                  u64 n_blocks,
                                                                  u8 *__restrict__ states) {
-  __shared__ u32 sh[kStatesThreads / 64];
-  const u64 per = (n_blocks + kStatesThreads - 1) / kStatesThreads;
-  const u64 beg = min<u64>(n_blocks, threadIdx.x * per), end = min<u64>(n_blocks, beg + per);
-  u32 m = kMapIdentity;
-  for (u64 i = beg; i < end; ++i) m = compose(m, maps[i]);
-  u32 total;
-  const u32 before = block_scan_maps<kStatesThreads / 64>(m, sh, &total);
-  u32 st = next_state(before, kTop);
-  for (u64 i = beg; i < end; ++i) {
-    states[i] = static_cast<u8>(st);
-    st = next_state(maps[i], st);
-  }
-}
-
-// Start state of this thread's segment, whose bytes are b[0, len).
-__device__ __forceinline__ u32 segment_start(u64 blk, const u8 *states, u32 *sh,
-                                             const u8 (&b)[kSegBytes], u32 len) {
-  u32 total;
-  const u32 before = block_scan_maps<kParseThreads / 64>(segment_map(b, len), sh, &total);
-  return next_state(before, states[blk]);
+  automaton::block_states(maps, n_blocks, kTop, states);
 }
 
 __global__ __launch_bounds__(kParseThreads) void k_ascii_count(const u8 *__restrict__ body, u64 n,
@@ -204,7 +103,7 @@ __global__ __launch_bounds__(kParseThreads) void k_ascii_count(const u8 *__restr
   for (u64 blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
     u8 b[kSegBytes];
     const u32 len = load_segment(body, n, blk * kParseThreads + threadIdx.x, b);
-    const u32 st = segment_start(blk, states, sh, b, len);
+    const u32 st = automaton::segment_start(blk, states, sh, b, len, AsciiMap{});
     u32 r = 0, nl = 0;
     const u64 beg = (blk * kParseThreads + threadIdx.x) * kSegBytes;
     walk_segment<false>(body, n, beg, b, len, st, r, nl, nullptr, 0, nullptr, 0);
@@ -225,7 +124,7 @@ __global__ __launch_bounds__(kParseThreads) void k_ascii_write(const u8 *__restr
   for (u64 blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
     u8 b[kSegBytes];
     const u32 len = load_segment(body, n, blk * kParseThreads + threadIdx.x, b);
-    const u32 st = segment_start(blk, states, sh, b, len);
+    const u32 st = automaton::segment_start(blk, states, sh, b, len, AsciiMap{});
     u32 r = 0, nl = 0;
     const u64 beg = (blk * kParseThreads + threadIdx.x) * kSegBytes;
     walk_segment<false>(body, n, beg, b, len, st, r, nl, nullptr, 0, nullptr, 0);

@@ -1111,6 +1111,13 @@ u64 Searcher::search_reads_host(const char *chars, u64 n_chars, const u64 *read_
   return nk;
 }
 
+void Searcher::search_reads_device_staged(const char *d_chars, u64 n_chars, const u64 *d_read_ends,
+                                          u64 n_reads) {
+  if (n_reads == 0) return;
+  ensure_staging(std::max<u64>(n_chars, 1), 0, 0);
+  search_reads_device(d_chars, n_chars, d_read_ends, n_reads, d_results_, nullptr);
+}
+
 SearchTimings Searcher::last_timings() {
   SearchTimings t;
   if (!events_recorded_) return t;

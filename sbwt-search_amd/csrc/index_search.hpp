@@ -62,6 +62,10 @@ class Searcher {
   // results may be null: they then stay on the device (results_device()).
   u64 search_reads_host(const char *chars, u64 n_chars, const u64 *read_ends, u64 n_reads,
                         u64 *results);
+  // Device reads in, results kept in the searcher's own buffer
+  // (results_device()); enqueued on the searcher's stream.
+  void search_reads_device_staged(const char *d_chars, u64 n_chars, const u64 *d_read_ends,
+                                  u64 n_reads);
   // Device positions form (explicit positions already on the device).
   void search_positions_device(const u64 *d_bit_seqs, const u64 *d_positions, u64 n,
                                u64 *d_results);

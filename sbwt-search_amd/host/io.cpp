@@ -39,6 +39,12 @@ SeqReader::SeqReader(const std::string &path) : buf_(1 << 20) {
   gz_ = g;
 }
 
+SeqReader::SeqReader(void *gz, const char *pending, u64 n)
+    : gz_(gz), buf_(std::max<u64>(n, 1 << 20)) {
+  if (n) std::memcpy(buf_.data(), pending, n);
+  len_ = static_cast<size_t>(n);
+}
+
 SeqReader::~SeqReader() {
   if (gz_) gzclose(static_cast<gzFile>(gz_));
 }
@@ -118,6 +124,35 @@ bool SeqReader::next(std::vector<char> &chars) {
     seq_len += take_line(&chars);
   }
   return true;
+}
+
+RawSeqFile::RawSeqFile(const std::string &path) {
+  gzFile g = gzopen(path.c_str(), "rb");
+  if (!g) throw InputMissing("The input file " + path + " cannot be opened");
+  gzbuffer(g, 1 << 20);
+  gz_ = g;
+}
+
+RawSeqFile::~RawSeqFile() {
+  if (gz_) gzclose(static_cast<gzFile>(gz_));
+}
+
+u64 RawSeqFile::read(char *dst, u64 n) {
+  u64 got = 0;
+  while (got < n && !eof_) {
+    const unsigned ask = static_cast<unsigned>(std::min<u64>(n - got, 1u << 30));
+    const int r = gzread(static_cast<gzFile>(gz_), dst + got, ask);
+    if (r < 0) throw std::runtime_error("read error in a sequence file");
+    if (r == 0) eof_ = true;
+    got += static_cast<u64>(r);
+  }
+  return got;
+}
+
+std::unique_ptr<SeqReader> RawSeqFile::to_reader(const char *pending, u64 n) {
+  auto r = std::make_unique<SeqReader>(gz_, pending, n);
+  gz_ = nullptr;
+  return r;
 }
 
 // ------------------------------------------------------------ index writer

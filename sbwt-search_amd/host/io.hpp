@@ -31,7 +31,12 @@ struct InputMissing : std::runtime_error {
 class SeqReader {
  public:
   explicit SeqReader(const std::string &path);
+  // Continues an open file (a gzFile, now owned by the reader) whose next
+  // bytes are pending[0, n) and then the rest of the file.
+  SeqReader(void *gz, const char *pending, u64 n);
   ~SeqReader();
+  SeqReader(const SeqReader &) = delete;
+  SeqReader &operator=(const SeqReader &) = delete;
   // Appends the next record's sequence to `chars`; false at end of file.
   bool next(std::vector<char> &chars);
 
@@ -46,6 +51,25 @@ class SeqReader {
   bool eof_ = false;
   bool started_ = false;
   int pending_ = -1;  // first char of the next header, already read
+};
+
+// A sequence file read as raw (decompressed) bytes, the input of the device
+// FASTA parser. to_reader() hands the open file to a SeqReader that goes on
+// from bytes already read, when the rest is for the host reader.
+class RawSeqFile {
+ public:
+  explicit RawSeqFile(const std::string &path);  // InputMissing when it cannot be opened
+  ~RawSeqFile();
+  RawSeqFile(const RawSeqFile &) = delete;
+  RawSeqFile &operator=(const RawSeqFile &) = delete;
+  // Reads up to n bytes into dst; fewer only at the end of the file.
+  u64 read(char *dst, u64 n);
+  bool at_end() const { return eof_; }
+  std::unique_ptr<SeqReader> to_reader(const char *pending, u64 n);
+
+ private:
+  void *gz_ = nullptr;
+  bool eof_ = false;
 };
 
 // Index output file (src/IndexResultsPrinter/*): the header (u64 length +

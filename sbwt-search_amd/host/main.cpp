@@ -1,7 +1,7 @@
 // sbwt_search: the drop-in command line of the MI355X engine.
 //   sbwt_search index  -q <fasta/q(.gz)|.list> -i <.sbwt|.tdbg> -o <prefix|.list>
 //                      [-k <.tcolors>] [-p ascii|binary|bool] [--no-headers] [-s streams]
-//                      [--batch-chars N]
+//                      [--batch-chars N] [--host-parse]
 //   sbwt_search colors -q <index results|.list> -k <.tcolors> -o <prefix|.list>
 //                      [-p ascii|binary|csv] [-t threshold] [--include-not-found]
 //                      [--include-invalid] [--no-headers] [-s streams] [--batch-bytes N]
@@ -50,7 +50,7 @@ const std::map<std::string, std::string> kShort = {
     {"r", "base-pairs-per-seq"},  {"g", "gpu-memory-percentage"}, {"s", "streams"},
     {"p", "print-mode"},          {"k", "colors-file"},     {"t", "threshold"},
     {"h", "help"}};
-const char *kFlags[] = {"no-headers", "include-not-found", "include-invalid", "help"};
+const char *kFlags[] = {"no-headers", "include-not-found", "include-invalid", "host-parse", "help"};
 
 Args parse(int argc, char **argv) {
   Args a;
@@ -191,9 +191,12 @@ struct Pinned {
 
 // One device batch: whole reads, or pieces of a long read (closes[e] == 0:
 // the next entry continues the same read, and so the same output line).
+// A raw batch instead holds FASTA bytes from a record start to a record end
+// (n_chars bytes), parsed on the device.
 struct InBatch {
   u64 file = 0;
   bool first = false, last = false;  // first / last batch of its file
+  bool raw = false;
   Pinned chars;
   u64 n_chars = 0;
   std::vector<u64> ends;
@@ -208,12 +211,16 @@ struct OutChunk {
 };
 
 // One stream of the index command, three stages on three threads: the reader
-// parses reads into batches; the device stage searches a batch and formats
+// cuts a file into batches; the device stage searches a batch and formats
 // its results into the output bytes on the device; the writer appends them
 // to the file. Parsing, device work and writing of different batches overlap.
+// A FASTA file goes to the device as raw chunks cut at record starts, parsed
+// there; from the first chunk holding a line that starts with '+' (FASTQ
+// quality) or a record longer than a batch, the rest of the file is parsed by
+// the host record reader into batches of reads.
 void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<std::string> &in,
                   const std::vector<std::string> &out, const std::vector<u64> &files, int mode,
-                  bool headers, u64 max_chars) {
+                  bool headers, u64 max_chars, bool device_parse) {
   const u64 max_reads = std::max<u64>(max_chars / 16, 1024);
   constexpr size_t kBuffers = 3;  // per direction: one filling, one in the queue, one in use
   Channel<InBatch> to_device(1);
@@ -241,6 +248,7 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
   std::thread device([&] {
     sbwt_searcher *s = nullptr;
     sbwt_formatter *fmt = nullptr;
+    sbwt_fasta_parser *fp = nullptr;
     try {
       check(sbwt_searcher_create(index, max_chars, max_reads, move_to_key, nullptr, &s));
       check(sbwt_formatter_create(max_chars, max_reads, nullptr, &fmt));
@@ -254,7 +262,17 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
         o.last = b.last;
         if (!free_out.pop(o.bytes)) break;
         if (b.first) line_open = false;
-        if (!b.ends.empty()) {
+        if (b.raw && b.n_chars) {
+          if (!fp) check(sbwt_fasta_parser_create(max_chars, max_reads, nullptr, &fp));
+          u64 nr = 0;
+          check(sbwt_searcher_search_fasta_format(s, fp, fmt, mode, b.chars.p, b.n_chars,
+                                                  o.bytes.p, o.bytes.cap, &o.n, &nr));
+          if (o.n > o.bytes.cap) {
+            o.bytes.ensure(o.n + o.n / 4);
+            check(sbwt_formatter_copy_last(fmt, o.bytes.p, o.bytes.cap));
+          }
+          line_open = false;
+        } else if (!b.raw && !b.ends.empty()) {
           check(sbwt_searcher_search_reads_format(s, fmt, mode, b.chars.p, b.n_chars,
                                                   b.ends.data(), b.closes.data(), b.ends.size(),
                                                   line_open, o.bytes.p, o.bytes.cap, &o.n));
@@ -272,6 +290,7 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
     } catch (const std::exception &e) {
       fail(e);
     }
+    if (fp) sbwt_fasta_parser_destroy(fp);
     if (fmt) sbwt_formatter_destroy(fmt);
     if (s) sbwt_searcher_destroy(s);
   });
@@ -299,8 +318,10 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
     for (size_t fi = 0; fi < files.size(); ++fi) {
       const u64 f = files[fi];
       std::unique_ptr<sbwt_host::SeqReader> reader;
+      std::unique_ptr<sbwt_host::RawSeqFile> raw;
       try {
-        reader = std::make_unique<sbwt_host::SeqReader>(in[f]);
+        if (device_parse) raw = std::make_unique<sbwt_host::RawSeqFile>(in[f]);
+        else reader = std::make_unique<sbwt_host::SeqReader>(in[f]);
       } catch (const sbwt_host::InputMissing &e) {
         std::fprintf(stderr, "%s\n", e.what());  // the file's output holds headers only
       }
@@ -309,6 +330,60 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
       b.file = f;
       b.first = true;
       Clock clk;
+      if (ok && raw) {
+        // Raw chunks: b.chars[0, n_chars) holds bytes from a record start on.
+        // Bytes before the first '>' or '@' are skipped, as the record reader
+        // does.
+        bool started = false;
+        while (ok) {
+          const u64 got = raw->read(b.chars.p + b.n_chars, max_chars - b.n_chars);
+          if (!started) {
+            const char *p = b.chars.p;
+            const char *h1 = static_cast<const char *>(std::memchr(p, '>', got));
+            const char *h2 = static_cast<const char *>(std::memchr(p, '@', got));
+            const char *h = h1 && h2 ? std::min(h1, h2) : (h1 ? h1 : h2);
+            if (!h) {
+              if (raw->at_end()) break;  // no record in the file
+              continue;                  // all of it skipped; read on
+            }
+            std::memmove(b.chars.p, h, static_cast<size_t>(p + got - h));
+            b.n_chars = static_cast<u64>(p + got - h);
+            started = true;
+            if (b.n_chars < max_chars && !raw->at_end()) continue;
+          } else {
+            b.n_chars += got;
+          }
+          const bool last = raw->at_end();
+          const u64 cut = sbwt_fasta_has_quality_line(b.chars.p, b.n_chars, 1)
+                              ? 0
+                              : sbwt_fasta_chunk_cut(b.chars.p, b.n_chars, last, max_reads);
+          if (cut == 0) {  // the rest of the file goes to the host reader
+            reader = raw->to_reader(b.chars.p, b.n_chars);
+            b.n_chars = 0;
+            break;
+          }
+          Pinned next;
+          if (!free_in.pop(next)) {
+            ok = false;
+            break;
+          }
+          const u64 tail = b.n_chars - cut;
+          if (tail) std::memcpy(next.p, b.chars.p + cut, tail);
+          b.n_chars = cut;
+          b.raw = true;
+          g_times.parse_ns += clk.lap();
+          if (!to_device.push(std::move(b))) {
+            next.release();
+            ok = false;
+            break;
+          }
+          b = InBatch();
+          b.file = f;
+          b.chars = next;
+          b.n_chars = tail;
+          if (last && tail == 0) break;
+        }
+      }
       // hands the batch to the device stage; a batch that is not the file's
       // last one is followed by a fresh buffer from the pool
       auto flush_batch = [&](bool last) {
@@ -385,6 +460,8 @@ int run_index(const Args &a) {
   // --batch-chars N (an extension) caps the chars per device batch
   if (!a.get("batch-chars").empty())
     max_chars = std::max<u64>(std::min<u64>(max_chars, std::stoull(a.get("batch-chars"))), info.kmer_size);
+  // --host-parse (an extension) keeps every sequence file on the host reader
+  const bool device_parse = !a.flag("host-parse");
   const auto parts = sbwt_host::partition_by_size(in, streams);
   std::vector<std::thread> th;
   std::vector<std::string> errors(streams);
@@ -392,7 +469,7 @@ int run_index(const Args &a) {
     th.emplace_back([&, p] {
       try {
         index_stream(index, info.kmer_size, !colors.empty(), in, out, parts[p], mode,
-                     !a.flag("no-headers"), max_chars);
+                     !a.flag("no-headers"), max_chars, device_parse);
       } catch (const std::exception &e) {
         errors[p] = e.what();
       }

@@ -65,3 +65,42 @@ def test_create_rejects_presearch_letters_out_of_range(kp, k):
     rc = S.lib().sbwt_index_create(acgt, 4, k, None, None, kp, ctypes.byref(h))
     assert rc != 0
     assert S.lib().sbwt_last_error() == b"presearch letters must be in [1, min(k, 16)]"
+
+
+def _fasta_bytes(seed, n):
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    alphabet = np.frombuffer(b"ACGT>@\n\n+x", np.uint8)
+    b = alphabet[rng.integers(0, len(alphabet), n)]
+    b[0] = ord(">")
+    return b.tobytes()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_fasta_chunk_cut_is_the_longest_record_prefix(seed):
+    """Host-only cut rule, checked by brute force over its definition: the cut
+    is the last header start (a '>'/'@' at a line start) before which at most
+    max_reads records begin; at the end of the input the whole tail when it
+    holds at most max_reads records."""
+    raw = _fasta_bytes(seed, 3000 + seed * 977)
+    n = len(raw)
+    starts = [0] + [i for i in range(1, n) if raw[i - 1] == 10 and raw[i] in b">@"]
+    for max_reads in (1, 2, 5, 40, 10_000):
+        before = [h for k, h in enumerate(starts) if k <= max_reads and h > 0]
+        want = max(before, default=0)
+        assert S.fasta_chunk_cut(raw, False, max_reads) == want, max_reads
+        want_last = n if len(starts) <= max_reads else want
+        assert S.fasta_chunk_cut(raw, True, max_reads) == want_last, max_reads
+
+
+def test_fasta_quality_line_detection():
+    import re
+    for seed in range(20):
+        raw = _fasta_bytes(100 + seed, 50 + seed * 13)
+        for at_start in (True, False):
+            want = bool(re.search(rb"\n\+", raw)) or (at_start and raw[:1] == b"+")
+            assert S.fasta_has_quality_line(raw, at_start) == want
+    assert S.fasta_has_quality_line(b"+AC", True) and not S.fasta_has_quality_line(b"+AC", False)
+    assert S.fasta_chunk_cut(b">a\nAC\n>b\nGT", False, 10) == 6
+    assert S.fasta_chunk_cut(b">a\nACGT", False, 10) == 0
+    assert S.fasta_chunk_cut(b">a\nACGT", True, 10) == 7

@@ -183,3 +183,76 @@ def test_unreadable_inputs_are_skipped(tmp_path):
     col = [open(o + ".txt", "rb").read() for o in couts]
     assert col[1] == col[2] == head
     assert len(col[0]) > len(head) and col[0].count(b"\n") == len(ends)
+
+
+@pytest.fixture(scope="module")
+def mixed_inputs(tmp_path_factory):
+    """A built index and sequence files that take every path of the index
+    command's reader: wrapped FASTA (device-parsed chunks), FASTA with a '+'
+    line start late in the file and with a record longer than a batch (the
+    host reader takes over mid-file), FASTQ, junk before the first header, an
+    empty file, and gzip."""
+    import gzip
+    import workloads as W
+    d = tmp_path_factory.mktemp("mixed")
+    g = W.genome_random(31, 40_000)
+    codes, ends = W.concat([g])
+    b = W.BuiltSbwt(codes, ends, 31)
+    sbwt = str(d / "m.sbwt")
+    b.write(sbwt)
+    lens = W.uniform_lengths(8, 1500, 0, 500)
+    chars, rends, _ = W.reads(12, g, np.array([len(g)], np.uint64), lens, 0.3, 0.02, 0.01)
+    seqs, st = [], 0
+    for e in rends:
+        seqs.append(chars[st:int(e)].tobytes())
+        st = int(e)
+
+    def fasta(recs, width=70):
+        out = bytearray()
+        for i, s in enumerate(recs):
+            out += b">r%d x\n" % i
+            for j in range(0, len(s), width):
+                out += s[j:j + width] + b"\n"
+        return bytes(out)
+
+    files = {
+        "wrapped.fna": fasta(seqs),
+        "oneline.fna": fasta(seqs[:700], 1 << 20),
+        "late_plus.fna": fasta(seqs[:900]) + b"@q\nACGTACGTTT\n+\nIIIIIIIIII\n" + fasta(seqs[900:1000]),
+        "long_record.fna": fasta(seqs[:300]) + b">long\n" + g[:9000].tobytes() + b"\n" + fasta(seqs[300:400]),
+        "reads.fnq": b"".join(b"@r%d\n%s\n+\n%s\n" % (i, s, b"I" * len(s)) for i, s in enumerate(seqs[:500])),
+        "junk_first.fna": b"junk line\nmore>" + fasta(seqs[:50])[1:],
+        "empty.fna": b"",
+    }
+    paths = []
+    for name, data in files.items():
+        p = d / name
+        p.write_bytes(data)
+        paths.append(str(p))
+    gz = d / "wrapped.fna.gz"
+    with gzip.open(str(gz), "wb") as f:
+        f.write(files["wrapped.fna"])
+    paths.append(str(gz))
+    return sbwt, paths
+
+
+@pytest.mark.parametrize("batch", [None, 6000, 700])
+@pytest.mark.parametrize("host_parse", [False, True])
+def test_index_cli_device_and_host_parsing_match_oracle(tmp_path, mixed_inputs, batch, host_parse):
+    sbwt, ins = mixed_inputs
+    orc = O.OracleSbwt(sbwt)
+    outs = [str(tmp_path / f"o{i}") for i in range(len(ins))]
+    (tmp_path / "in.list").write_text("\n".join(ins) + "\n")
+    (tmp_path / "out.list").write_text("\n".join(outs) + "\n")
+    extra = (["--batch-chars", str(batch)] if batch else []) + (["--host-parse"] if host_parse else [])
+    run(["index", "-q", str(tmp_path / "in.list"), "-i", sbwt, "-o", str(tmp_path / "out.list"),
+         "-s", "2"] + extra)
+    for i, o in zip(ins, outs):
+        src = i
+        if i.endswith(".gz"):
+            import gzip
+            src = o + ".plain"
+            with gzip.open(i, "rb") as f, open(src, "wb") as g:
+                g.write(f.read())
+        orc.index_file(src, o + ".orc", 0, True)
+        assert open(o + ".txt", "rb").read() == open(o + ".orc", "rb").read(), i
