@@ -190,36 +190,31 @@ u64 FastaParser::parse_host(const char *raw, u64 n_bytes, u64 *n_chars) {
 
 u64 FastaParser::chunk_cut(const char *raw, u64 n, bool last, u64 max_reads) {
   if (n == 0 || max_reads == 0) return 0;
-  const auto *p = reinterpret_cast<const unsigned char *>(raw);
-  // header starts after position 0: a '>' or '@' right after a '\n'
-  auto starts_in = [&](u64 from, u64 to) {  // counts i in [from, to) with i - 1 a '\n'
-    u64 c = 0;
-    for (u64 i = std::max<u64>(from, 1); i < to; ++i)
-      c += static_cast<u64>(p[i - 1] == '\n') & static_cast<u64>(p[i] == '>' || p[i] == '@');
-    return c;
-  };
-  u64 limit = n;  // cut no later than the start of record max_reads + 1
-  u64 seen = 1;   // the record starting at position 0
-  for (u64 at = 1; at < n;) {
-    const u64 span = std::min<u64>(n - at, 1 << 16);
-    const u64 c = starts_in(at, at + span);
-    if (seen + c <= max_reads) {
-      seen += c;
-      at += span;
-      continue;
-    }
-    for (u64 i = at; i < at + span; ++i)
-      if (p[i - 1] == '\n' && (p[i] == '>' || p[i] == '@') && ++seen > max_reads) {
-        limit = i;
+  // Header starts are position 0 and every '>' or '@' right after a '\n'; two
+  // of them are at least 2 bytes apart, so raw holds at most (n + 1) / 2.
+  u64 limit = n;  // the start of record max_reads + 1, if raw holds one
+  if ((n + 1) / 2 > max_reads) {
+    u64 seen = 1;
+    const char *p = raw, *end = raw + n;
+    while ((p = static_cast<const char *>(std::memchr(p, '\n', static_cast<size_t>(end - p)))) &&
+           ++p < end) {
+      if ((*p == '>' || *p == '@') && ++seen > max_reads) {
+        limit = static_cast<u64>(p - raw);
         break;
       }
-    break;
+    }
   }
   if (last && limit == n) return n;
-  // the last header start at or before the limit (a cut at n itself needs
-  // the next record's header to be the next byte, which only `last` knows)
-  for (u64 i = std::min(limit, n - 1); i >= 1; --i)
-    if (p[i - 1] == '\n' && (p[i] == '>' || p[i] == '@')) return i;
+  // the last header start in [1, min(limit, n - 1)]: a cut at n itself needs
+  // the next record's header to follow, which only `last` tells
+  u64 len = std::min(limit, n - 1);  // '\n' searched in raw[0, len)
+  while (len > 0) {
+    const char *nl = static_cast<const char *>(memrchr(raw, '\n', static_cast<size_t>(len)));
+    if (!nl) break;
+    const u64 at = static_cast<u64>(nl - raw) + 1;
+    if (raw[at] == '>' || raw[at] == '@') return at;
+    len = static_cast<u64>(nl - raw);
+  }
   return 0;
 }
 

@@ -20,8 +20,14 @@ std::string output_open_error(const std::string &path) {
          "have permission to create files in this path folder";
 }
 
+// Large batches go out in 16 MiB writes.
 void write_all(FILE *f, const void *p, size_t n) {
-  if (n && fwrite(p, 1, n, f) != n) throw std::runtime_error("write failed");
+  constexpr size_t kPiece = size_t{16} << 20;
+  const char *c = static_cast<const char *>(p);
+  for (size_t o = 0; o < n; o += kPiece) {
+    const size_t m = std::min(kPiece, n - o);
+    if (fwrite(c + o, 1, m, f) != m) throw std::runtime_error("write failed");
+  }
 }
 }  // namespace
 
