@@ -436,6 +436,40 @@ int sbwt_searcher_search_fasta_format(sbwt_searcher *searcher, sbwt_fasta_parser
                                       uint64_t n_raw, char *out, uint64_t capacity,
                                       uint64_t *n_bytes, uint64_t *n_reads);
 
+/* FASTQ chunks in four-line records (header, one sequence line, '+' line, one
+ * quality line) parsed on the device. The reference's record reader (same
+ * citation as above) reads such a record as its sequence line exactly when
+ * the header line starts with '@' or '>', the sequence line is not empty and
+ * does not start with '+', '@' or '>', the third line starts with '+' and the
+ * quality line is at least as long as the sequence line; the parser checks
+ * this per record. raw starts at a record's header line; last != 0 when raw
+ * ends the file (a last line without '
+' then counts). The parser takes the
+ * longest prefix of at most max_reads_per_chunk whole records that pass:
+ * *n_used receives its length in bytes (0: no such record, e.g. one record
+ * longer than raw or the first one failing), *n_chars / *n_reads its sequence
+ * chars and reads. What follows the prefix is the caller's: the next chunk,
+ * or the host reader when the prefix stopped at a failing record. chars /
+ * read_ends may be NULL (the parse then stays on the device for
+ * sbwt_searcher_search_fastq_format); otherwise capacities are in entries.
+ * Synchronous. max_bytes_per_chunk < 2^32, max_reads_per_chunk < 2^30. */
+typedef struct sbwt_fastq_parser sbwt_fastq_parser;
+int sbwt_fastq_parser_create(uint64_t max_bytes_per_chunk, uint64_t max_reads_per_chunk,
+                             void *hip_stream, sbwt_fastq_parser **out);
+void sbwt_fastq_parser_destroy(sbwt_fastq_parser *parser);
+int sbwt_fastq_parser_parse(sbwt_fastq_parser *parser, const char *raw, uint64_t n_bytes, int last,
+                            char *chars, uint64_t chars_capacity, uint64_t *read_ends,
+                            uint64_t reads_capacity, uint64_t *n_used, uint64_t *n_chars,
+                            uint64_t *n_reads);
+/* The parser's last parse -> fused search -> formatted index output bytes
+ * (every read complete), as sbwt_searcher_search_fasta_format. Synchronous.
+ * Replaces one batch of ContinuousIndexSearcher + ContinuousIndexResultsPrinter
+ * (src/IndexSearcher/ContinuousIndexSearcher.cpp:67-74,
+ * ContinuousIndexResultsPrinter.hpp:146-308) over a parsed FASTQ chunk. */
+int sbwt_searcher_search_fastq_format(sbwt_searcher *searcher, sbwt_fastq_parser *parser,
+                                      sbwt_formatter *formatter, int mode, char *out,
+                                      uint64_t capacity, uint64_t *n_bytes);
+
 /* Color id lists (the CSR of sbwt_color_searcher_pseudoalign_device) as the
  * reference's color printers write them, file headers excluded: ascii ids
  * joined by ' ' and a '\n' per read; binary a u64 per id and u64 max per read;

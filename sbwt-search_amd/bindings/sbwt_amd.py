@@ -140,6 +140,13 @@ def lib():
         L.sbwt_searcher_search_fasta_format.argtypes = [vp, vp, vp, C.c_int, C.c_char_p,
                                                         C.c_uint64, C.c_char_p, C.c_uint64, u64p,
                                                         u64p]
+        L.sbwt_fastq_parser_create.argtypes = [C.c_uint64, C.c_uint64, vp, C.POINTER(vp)]
+        L.sbwt_fastq_parser_destroy.argtypes = [vp]
+        L.sbwt_fastq_parser_destroy.restype = None
+        L.sbwt_fastq_parser_parse.argtypes = [vp, C.c_char_p, C.c_uint64, C.c_int, C.c_char_p,
+                                              C.c_uint64, u64p, C.c_uint64, u64p, u64p, u64p]
+        L.sbwt_searcher_search_fastq_format.argtypes = [vp, vp, vp, C.c_int, C.c_char_p,
+                                                        C.c_uint64, u64p]
         L.sbwt_color_formatter_create.argtypes = [C.c_uint64, C.c_uint64, vp, C.POINTER(vp)]
         L.sbwt_color_formatter_destroy.argtypes = [vp]
         L.sbwt_color_formatter_destroy.restype = None
@@ -464,6 +471,51 @@ class FastaParser:
     def close(self):
         if getattr(self, "h", None):
             lib().sbwt_fasta_parser_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+class FastqParser:
+    """Device parser of four-line FASTQ chunks (longest prefix of whole records
+    that the record reader reads the same way)."""
+
+    def __init__(self, max_bytes, max_reads, stream=None):
+        self.h = C.c_void_p()
+        _check(lib().sbwt_fastq_parser_create(max_bytes, max_reads,
+                                              C.c_void_p(stream) if stream else None,
+                                              C.byref(self.h)))
+        self.max_bytes = max_bytes
+        self.max_reads = max_reads
+
+    def parse(self, raw, last=True):
+        """Host bytes in -> (bytes used, chars bytes, read_ends u64[])."""
+        raw = bytes(raw)
+        chars = C.create_string_buffer(max(len(raw), 1))
+        ends = np.zeros(self.max_reads, np.uint64)
+        used, nc, nr = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(lib().sbwt_fastq_parser_parse(self.h, raw, len(raw), int(last), chars, len(raw),
+                                             _p64(ends), len(ends), C.byref(used), C.byref(nc),
+                                             C.byref(nr)))
+        return int(used.value), chars.raw[:nc.value], ends[:nr.value].copy()
+
+    def search_format(self, searcher, formatter, mode, raw, last=True):
+        """One chunk -> device parse -> search -> formatted bytes; (bytes, bytes used, n_reads)."""
+        raw = bytes(raw)
+        used, nc, nr = C.c_uint64(), C.c_uint64(), C.c_uint64()
+        _check(lib().sbwt_fastq_parser_parse(self.h, raw, len(raw), int(last), None, 0, None, 0,
+                                             C.byref(used), C.byref(nc), C.byref(nr)))
+        n = C.c_uint64()
+        _check(lib().sbwt_searcher_search_fastq_format(searcher.h, self.h, formatter.h, mode,
+                                                       None, 0, C.byref(n)))
+        out = C.create_string_buffer(max(n.value, 1))
+        _check(lib().sbwt_formatter_copy_last(formatter.h, out, n.value))
+        return out.raw[:n.value], int(used.value), int(nr.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sbwt_fastq_parser_destroy(self.h)
             self.h = None
 
     def __del__(self):

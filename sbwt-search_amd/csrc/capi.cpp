@@ -53,6 +53,11 @@ struct sbwt_color_formatter {
   std::unique_ptr<sbwt_amd::ColorFormatter> impl;
 };
 
+struct sbwt_fastq_parser {
+  std::unique_ptr<sbwt_amd::FastqParser> impl;
+  uint64_t n_chars = 0, n_reads = 0;  // of the last parse
+};
+
 struct sbwt_fasta_parser {
   std::unique_ptr<sbwt_amd::FastaParser> impl;
   hipEvent_t parsed = nullptr;  // the chars and read ends of the last parse are written
@@ -638,6 +643,67 @@ int sbwt_searcher_search_fasta_format(sbwt_searcher *searcher, sbwt_fasta_parser
         mode, s.results_device(), s.kmer_ends_device(), parser->impl->ones(), nr, false, out,
         capacity);
     *n_reads = nr;
+  });
+}
+
+int sbwt_fastq_parser_create(uint64_t max_bytes_per_chunk, uint64_t max_reads_per_chunk,
+                             void *hip_stream, sbwt_fastq_parser **out) {
+  return guarded([&] {
+    require(out, "null argument");
+    auto p = std::make_unique<sbwt_fastq_parser>();
+    p->impl = std::make_unique<sbwt_amd::FastqParser>(max_bytes_per_chunk, max_reads_per_chunk,
+                                                      static_cast<hipStream_t>(hip_stream));
+    *out = p.release();
+  });
+}
+
+void sbwt_fastq_parser_destroy(sbwt_fastq_parser *parser) { delete parser; }
+
+int sbwt_fastq_parser_parse(sbwt_fastq_parser *parser, const char *raw, uint64_t n_bytes, int last,
+                            char *chars, uint64_t chars_capacity, uint64_t *read_ends,
+                            uint64_t reads_capacity, uint64_t *n_used, uint64_t *n_chars,
+                            uint64_t *n_reads) {
+  return guarded([&] {
+    require(parser && n_used && n_chars && n_reads, "null argument");
+    require(n_bytes == 0 || raw, "null argument");
+    parser->n_chars = parser->n_reads = 0;
+    uint64_t nr = 0, nc = 0;
+    const uint64_t used = parser->impl->parse_host(raw, n_bytes, last != 0, &nr, &nc);
+    hipStream_t st = parser->impl->stream();
+    if (chars && nc) {
+      require(nc <= chars_capacity, "output arrays too small for the chunk");
+      SBWT_HIP(hipMemcpyAsync(chars, parser->impl->chars(), nc, hipMemcpyDeviceToHost, st));
+    }
+    if (read_ends && nr) {
+      require(nr <= reads_capacity, "output arrays too small for the chunk");
+      SBWT_HIP(hipMemcpyAsync(read_ends, parser->impl->read_ends(), nr * 8, hipMemcpyDeviceToHost, st));
+    }
+    SBWT_HIP(hipStreamSynchronize(st));
+    parser->n_chars = nc;
+    parser->n_reads = nr;
+    *n_used = used;
+    *n_chars = nc;
+    *n_reads = nr;
+  });
+}
+
+int sbwt_searcher_search_fastq_format(sbwt_searcher *searcher, sbwt_fastq_parser *parser,
+                                      sbwt_formatter *formatter, int mode, char *out,
+                                      uint64_t capacity, uint64_t *n_bytes) {
+  return guarded([&] {
+    require(searcher && parser && formatter && n_bytes, "null argument");
+    require(mode == SBWT_FORMAT_ASCII || mode == SBWT_FORMAT_BINARY || mode == SBWT_FORMAT_BOOL,
+            "unknown output format");
+    *n_bytes = 0;
+    if (parser->n_reads == 0) return;
+    Searcher &s = *searcher->impl;
+    // the parse ended with a synchronization of the parser's stream
+    s.search_reads_device_staged(parser->impl->chars(), parser->n_chars,
+                                 parser->impl->read_ends(), parser->n_reads);
+    SBWT_HIP(hipStreamSynchronize(s.stream()));
+    *n_bytes = formatter->impl->format_device_closes_to_host(
+        mode, s.results_device(), s.kmer_ends_device(), parser->impl->ones(), parser->n_reads,
+        false, out, capacity);
   });
 }
 

@@ -124,6 +124,138 @@ __global__ void k_fill_ones(u8 *p, u64 n) {
 
 u32 grid_of(u64 n_blocks) { return static_cast<u32>(std::min<u64>(n_blocks, 1u << 16)); }
 
+// ---- FASTQ: the state is the line's index within its record (mod 4)
+constexpr u32 kMapNextLine = 1u | 2u << 2 | 3u << 4 | 0u << 6;
+constexpr u32 kSeqLine = 1;
+
+struct FastqMap {
+  __device__ __forceinline__ u32 operator()(u8 b) const {
+    return b == '\n' ? kMapNextLine : automaton::kMapIdentity;
+  }
+};
+
+__global__ __launch_bounds__(kThreads) void k_fastq_maps(const u8 *__restrict__ raw, u64 n,
+                                                         u64 n_blocks, u8 *__restrict__ maps) {
+  automaton::block_maps(raw, n, n_blocks, maps, FastqMap{});
+}
+
+__global__ __launch_bounds__(kStatesThreads) void k_fastq_states(const u8 *__restrict__ maps,
+                                                                 u64 n_blocks,
+                                                                 u8 *__restrict__ states) {
+  automaton::block_states(maps, n_blocks, 0, states);
+}
+
+// Per block: sequence chars << 32 | newlines.
+__global__ __launch_bounds__(kThreads) void k_fastq_count(const u8 *__restrict__ raw, u64 n,
+                                                          u64 n_blocks,
+                                                          const u8 *__restrict__ states,
+                                                          u64 *__restrict__ counts) {
+  __shared__ u32 sh[kThreads / 64];
+  for (u64 blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+    u8 b[kSegBytes];
+    const u32 len = load_segment(raw, n, blk * kThreads + threadIdx.x, b);
+    u32 st = automaton::segment_start(blk, states, sh, b, len, FastqMap{});
+    u32 nc = 0, nl = 0;
+#pragma unroll
+    for (u32 i = 0; i < kSegBytes; ++i) {
+      if (i < len) {
+        const bool newline = b[i] == '\n';
+        nc += (st == kSeqLine && !newline) ? 1u : 0u;
+        nl += newline ? 1u : 0u;
+        st = newline ? (st + 1) & 3u : st;
+      }
+    }
+    u32 tc, tl;
+    block_prefix_sum(nc, sh, &tc);
+    block_prefix_sum(nl, sh, &tl);
+    if (threadIdx.x == 0) counts[blk] = (u64{tc} << 32) | tl;
+  }
+}
+
+// Copies the sequence chars; writes each line's '\n' offset (lines below
+// line_cap) and each record's read end (records below line_cap / 4).
+__global__ __launch_bounds__(kThreads) void k_fastq_write(const u8 *__restrict__ raw, u64 n,
+                                                          u64 n_blocks,
+                                                          const u8 *__restrict__ states,
+                                                          const u64 *__restrict__ incl,
+                                                          u64 line_cap, char *__restrict__ chars,
+                                                          u64 *__restrict__ ends,
+                                                          u32 *__restrict__ line_ends) {
+  __shared__ u32 sh[kThreads / 64];
+  for (u64 blk = blockIdx.x; blk < n_blocks; blk += gridDim.x) {
+    const u64 seg = blk * kThreads + threadIdx.x;
+    u8 b[kSegBytes];
+    const u32 len = load_segment(raw, n, seg, b);
+    const u32 st0 = automaton::segment_start(blk, states, sh, b, len, FastqMap{});
+    u32 st = st0, nc = 0, nl = 0;
+#pragma unroll
+    for (u32 i = 0; i < kSegBytes; ++i) {
+      if (i < len) {
+        const bool newline = b[i] == '\n';
+        nc += (st == kSeqLine && !newline) ? 1u : 0u;
+        nl += newline ? 1u : 0u;
+        st = newline ? (st + 1) & 3u : st;
+      }
+    }
+    u32 tc, tl;
+    const u32 c_before = block_prefix_sum(nc, sh, &tc);
+    const u32 l_before = block_prefix_sum(nl, sh, &tl);
+    const u64 base = blk ? incl[blk - 1] : 0;
+    const u64 cbase = (base >> 32) + c_before;
+    const u64 lbase = (base & 0xFFFFFFFFu) + l_before;
+    st = st0;
+    nc = 0;
+    nl = 0;
+#pragma unroll
+    for (u32 i = 0; i < kSegBytes; ++i) {
+      if (i < len) {
+        const bool newline = b[i] == '\n';
+        if (st == kSeqLine && !newline) chars[cbase + nc] = static_cast<char>(b[i]);
+        if (newline) {
+          const u64 line = lbase + nl;
+          if (line < line_cap) {
+            line_ends[line] = static_cast<u32>(seg * kSegBytes + i);
+            if (st == kSeqLine) ends[line >> 2] = cbase + nc;
+          }
+        }
+        nc += (st == kSeqLine && !newline) ? 1u : 0u;
+        nl += newline ? 1u : 0u;
+        st = newline ? (st + 1) & 3u : st;
+      }
+    }
+  }
+}
+
+__global__ void k_set_u32(u32 *p, u32 v) { *p = v; }
+
+__device__ __forceinline__ bool header_start(u8 c) { return c == '@' || c == '>'; }
+
+// Record r (below n_rec) reads as the record reader reads it: see seq_parse.hpp.
+__global__ void k_fastq_check(const u8 *__restrict__ raw, const u32 *__restrict__ line_ends,
+                              u64 n_rec, u32 *first_bad) {
+  for (u64 r = blockIdx.x * u64{blockDim.x} + threadIdx.x; r < n_rec;
+       r += u64{gridDim.x} * blockDim.x) {
+    const u64 s0 = r ? u64{line_ends[4 * r - 1]} + 1 : 0;
+    const u64 s1 = u64{line_ends[4 * r]} + 1;
+    const u64 s2 = u64{line_ends[4 * r + 1]} + 1;
+    const u64 s3 = u64{line_ends[4 * r + 2]} + 1;
+    const u64 len1 = s2 - 1 - s1, len3 = u64{line_ends[4 * r + 3]} - s3;
+    const u8 c1 = raw[s1];
+    const bool ok = header_start(raw[s0]) && len1 > 0 && c1 != '+' && !header_start(c1) &&
+                    raw[s2] == '+' && len3 >= len1;
+    if (!ok) atomicMin(first_bad, static_cast<u32>(r));
+  }
+}
+
+// out: records parsed, prefix bytes, chars.
+__global__ void k_fastq_finish(const u32 *first_bad, const u32 *line_ends, const u64 *ends, u64 n,
+                               u64 *out) {
+  const u64 r = *first_bad;
+  out[0] = r;
+  out[1] = r ? min<u64>(u64{line_ends[4 * r - 1]} + 1, n) : 0;
+  out[2] = r ? ends[r - 1] : 0;
+}
+
 }  // namespace
 
 FastaParser::FastaParser(u64 max_bytes, u64 max_reads, hipStream_t stream)
@@ -186,6 +318,78 @@ u64 FastaParser::parse_host(const char *raw, u64 n_bytes, u64 *n_chars) {
   SBWT_HIP(hipGetLastError());
   *n_chars = nc;
   return nh;
+}
+
+FastqParser::FastqParser(u64 max_bytes, u64 max_reads, hipStream_t stream)
+    : max_bytes_(max_bytes), max_reads_(max_reads), stream_(stream), own_stream_(stream == nullptr) {
+  if (max_bytes_ == 0 || max_bytes_ >= (1ULL << 32))
+    throw std::runtime_error("FASTQ chunks must hold between 1 byte and 4 GiB - 1");
+  if (max_reads_ == 0 || max_reads_ >= (1ULL << 30)) throw std::runtime_error("max_reads must be in [1, 2^30)");
+  if (own_stream_) SBWT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  block_cap_ = max_bytes_ / kBlockBytes + 2;
+  SBWT_HIP(hipMalloc(&d_raw_, max_bytes_ + 64));
+  SBWT_HIP(hipMalloc(&d_chars_, max_bytes_ + 64));
+  SBWT_HIP(hipMalloc(&d_ends_, max_reads_ * 8));
+  SBWT_HIP(hipMalloc(&d_ones_, max_reads_));
+  SBWT_HIP(hipMalloc(&d_line_ends_, max_reads_ * 4 * 4));
+  SBWT_HIP(hipMalloc(&d_block_maps_, block_cap_ * 2));
+  SBWT_HIP(hipMalloc(&d_block_counts_, block_cap_ * 2 * 8));
+  SBWT_HIP(hipMalloc(&d_scan_tmp_, scan_tmp_entries(block_cap_) * 8));
+  SBWT_HIP(hipMalloc(&d_first_bad_, 4));
+  SBWT_HIP(hipMalloc(&d_result_, 3 * 8));
+  k_fill_ones<<<static_cast<u32>(std::min<u64>((max_reads_ + 255) / 256, 1u << 16)), 256, 0, stream_>>>(
+      d_ones_, max_reads_);
+  SBWT_HIP(hipGetLastError());
+}
+
+FastqParser::~FastqParser() {
+  void *bufs[] = {d_raw_, d_chars_, d_ends_, d_ones_, d_line_ends_, d_block_maps_, d_block_counts_,
+                  d_scan_tmp_, d_first_bad_, d_result_};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  if (own_stream_ && stream_) (void)hipStreamDestroy(stream_);
+}
+
+u64 FastqParser::parse_host(const char *raw, u64 n, bool last, u64 *n_reads, u64 *n_chars) {
+  *n_reads = 0;
+  *n_chars = 0;
+  if (n == 0) return 0;
+  if (n > max_bytes_) throw std::runtime_error("FASTQ chunk exceeds the parser's max bytes");
+  SBWT_HIP(hipMemcpyAsync(d_raw_, raw, n, hipMemcpyHostToDevice, stream_));
+  const u8 *body = reinterpret_cast<const u8 *>(d_raw_);
+  const u64 n_blocks = (n + kBlockBytes - 1) / kBlockBytes;
+  u8 *states = d_block_maps_ + block_cap_;
+  u64 *incl = d_block_counts_ + block_cap_;
+  const u32 grid = grid_of(n_blocks);
+  k_fastq_maps<<<grid, kThreads, 0, stream_>>>(body, n, n_blocks, d_block_maps_);
+  k_fastq_states<<<1, kStatesThreads, 0, stream_>>>(d_block_maps_, n_blocks, states);
+  k_fastq_count<<<grid, kThreads, 0, stream_>>>(body, n, n_blocks, states, d_block_counts_);
+  SBWT_HIP(hipGetLastError());
+  scan_inclusive_u64(d_block_counts_, n_blocks, incl, d_scan_tmp_, stream_);
+  u64 tot = 0;
+  SBWT_HIP(hipMemcpyAsync(&tot, incl + n_blocks - 1, 8, hipMemcpyDeviceToHost, stream_));
+  SBWT_HIP(hipStreamSynchronize(stream_));
+  const u64 newlines = tot & 0xFFFFFFFFu;
+  const bool open_line = last && raw[n - 1] != '\n';  // a last line without '\n'
+  const u64 lines = newlines + (open_line ? 1 : 0);
+  const u64 r_cap = std::min<u64>(lines / 4, max_reads_);
+  if (r_cap == 0) return 0;
+  const u64 line_cap = 4 * r_cap;
+  k_fastq_write<<<grid, kThreads, 0, stream_>>>(body, n, n_blocks, states, incl, line_cap,
+                                                d_chars_, d_ends_, d_line_ends_);
+  if (open_line && newlines < line_cap)
+    k_set_u32<<<1, 1, 0, stream_>>>(d_line_ends_ + newlines, static_cast<u32>(n));
+  k_set_u32<<<1, 1, 0, stream_>>>(d_first_bad_, static_cast<u32>(r_cap));
+  k_fastq_check<<<static_cast<u32>(std::min<u64>((r_cap + 255) / 256, 1u << 16)), 256, 0, stream_>>>(
+      body, d_line_ends_, r_cap, d_first_bad_);
+  k_fastq_finish<<<1, 1, 0, stream_>>>(d_first_bad_, d_line_ends_, d_ends_, n, d_result_);
+  SBWT_HIP(hipGetLastError());
+  u64 res[3] = {0, 0, 0};
+  SBWT_HIP(hipMemcpyAsync(res, d_result_, sizeof(res), hipMemcpyDeviceToHost, stream_));
+  SBWT_HIP(hipStreamSynchronize(stream_));
+  *n_reads = res[0];
+  *n_chars = res[2];
+  return res[1];
 }
 
 u64 FastaParser::chunk_cut(const char *raw, u64 n, bool last, u64 max_reads) {

@@ -57,4 +57,52 @@ class FastaParser {
   u32 *d_flags_ = nullptr;         // a '+' line start was seen
 };
 
+// Device parser of FASTQ chunks in four-line records (header, one sequence
+// line, '+' line, one quality line), the common form of sequencer output.
+// The host reader (kseq's rules, as above) reads such a record as its
+// sequence line exactly when the header line starts with '@' or '>', the
+// sequence line is not empty and does not start with '+', '@' or '>', the
+// third line starts with '+', and the quality line is at least as long as
+// the sequence line (it then takes that one line as the whole quality and
+// the next record starts right after it). The parser checks this per record
+// and parses the longest prefix of whole records that passes; the caller
+// gives what follows to the host reader when the prefix stops short.
+class FastqParser {
+ public:
+  FastqParser(u64 max_bytes, u64 max_reads, hipStream_t stream);
+  ~FastqParser();
+  FastqParser(const FastqParser &) = delete;
+  FastqParser &operator=(const FastqParser &) = delete;
+
+  // raw[0, n) starts at a record's header line; `last` when it ends the file
+  // (a last line without '\n' then counts). Parses the longest prefix of at
+  // most max_reads whole four-line records that read as above and returns its
+  // length in bytes (0: no such record); *n_reads and *n_chars describe it.
+  // Synchronous.
+  u64 parse_host(const char *raw, u64 n, bool last, u64 *n_reads, u64 *n_chars);
+
+  const char *chars() const { return d_chars_; }
+  const u64 *read_ends() const { return d_ends_; }
+  const u8 *ones() const { return d_ones_; }
+  hipStream_t stream() const { return stream_; }
+  u64 max_bytes() const { return max_bytes_; }
+  u64 max_reads() const { return max_reads_; }
+
+ private:
+  u64 max_bytes_, max_reads_;
+  hipStream_t stream_;
+  bool own_stream_;
+  u64 block_cap_ = 0;
+  char *d_raw_ = nullptr;
+  char *d_chars_ = nullptr;
+  u64 *d_ends_ = nullptr;
+  u8 *d_ones_ = nullptr;
+  u32 *d_line_ends_ = nullptr;     // byte offset of each line's '\n' (n for a last line without)
+  u8 *d_block_maps_ = nullptr;     // per block: state map, then start state
+  u64 *d_block_counts_ = nullptr;  // per block: sequence chars << 32 | newlines, then the scan
+  u64 *d_scan_tmp_ = nullptr;
+  u32 *d_first_bad_ = nullptr;     // first record that fails the check
+  u64 *d_result_ = nullptr;        // records parsed, prefix bytes, chars
+};
+
 }  // namespace sbwt_amd

@@ -196,11 +196,13 @@ struct Pinned {
 // One device batch: whole reads, or pieces of a long read (closes[e] == 0:
 // the next entry continues the same read, and so the same output line).
 // A raw batch instead holds FASTA bytes from a record start to a record end
-// (n_chars bytes), parsed on the device.
+// (n_chars bytes), parsed on the device; a FASTQ batch was parsed on the
+// device by the reader already and is held by its parser (fq, no chars).
 struct InBatch {
   u64 file = 0;
   bool first = false, last = false;  // first / last batch of its file
   bool raw = false;
+  sbwt_fastq_parser *fq = nullptr;
   Pinned chars;
   u64 n_chars = 0;
   std::vector<u64> ends;
@@ -219,9 +221,13 @@ struct OutChunk {
 // its results into the output bytes on the device; the writer appends them
 // to the file. Parsing, device work and writing of different batches overlap.
 // A FASTA file goes to the device as raw chunks cut at record starts, parsed
-// there; from the first chunk holding a line that starts with '+' (FASTQ
-// quality) or a record longer than a batch, the rest of the file is parsed by
-// the host record reader into batches of reads.
+// there; from the first chunk holding a line that starts with '+' or a record
+// longer than a batch, the rest of the file is parsed by the host record
+// reader into batches of reads. A FASTQ file (its first chunk starts with '@'
+// and holds a '+' line start) is parsed on the device by the reader thread as
+// four-line records, chunk by chunk, up to the first record that the host
+// reader would read differently (or one longer than a batch); the host reader
+// takes the rest.
 void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<std::string> &in,
                   const std::vector<std::string> &out, const std::vector<u64> &files, int mode,
                   bool headers, u64 max_chars, bool device_parse) {
@@ -230,6 +236,8 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
   Channel<InBatch> to_device(1);
   Channel<OutChunk> to_writer(1);
   Channel<Pinned> free_in(kBuffers), free_out(kBuffers);
+  Channel<sbwt_fastq_parser *> free_fq(kBuffers);  // FASTQ parsers back from the device stage
+  size_t fq_made = 0;
   for (size_t i = 0; i < kBuffers; ++i) {
     Pinned in_buf;
     in_buf.ensure(max_chars + 64);
@@ -247,6 +255,7 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
     to_writer.abort();
     free_in.abort();
     free_out.abort();
+    free_fq.abort();
   };
 
   std::thread device([&] {
@@ -276,6 +285,19 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
             check(sbwt_formatter_copy_last(fmt, o.bytes.p, o.bytes.cap));
           }
           line_open = false;
+        } else if (b.fq) {
+          check(sbwt_searcher_search_fastq_format(s, b.fq, fmt, mode, o.bytes.p, o.bytes.cap, &o.n));
+          if (o.n > o.bytes.cap) {
+            o.bytes.ensure(o.n + o.n / 4);
+            check(sbwt_formatter_copy_last(fmt, o.bytes.p, o.bytes.cap));
+          }
+          line_open = false;
+          sbwt_fastq_parser *q = b.fq;
+          b.fq = nullptr;
+          if (!free_fq.push(q)) {
+            sbwt_fastq_parser_destroy(q);
+            break;
+          }
         } else if (!b.raw && !b.ends.empty()) {
           check(sbwt_searcher_search_reads_format(s, fmt, mode, b.chars.p, b.n_chars,
                                                   b.ends.data(), b.closes.data(), b.ends.size(),
@@ -286,7 +308,7 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
           }
           line_open = b.closes.back() == 0;
         }
-        if (!free_in.push(b.chars)) break;
+        if (b.chars.p && !free_in.push(b.chars)) break;
         g_times.search_ns += clk.lap();
         if (!to_writer.push(o)) break;
       }
@@ -339,6 +361,7 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
         // Bytes before the first '>' or '@' are skipped, as the record reader
         // does.
         bool started = false;
+        int kind = -1;  // 0 FASTA, 1 FASTQ: decided on the first chunk
         while (ok) {
           const u64 got = raw->read(b.chars.p + b.n_chars, max_chars - b.n_chars);
           if (!started) {
@@ -358,6 +381,45 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
             b.n_chars += got;
           }
           const bool last = raw->at_end();
+          if (kind < 0)
+            kind = b.chars.p[0] == '@' && sbwt_fasta_has_quality_line(b.chars.p, b.n_chars, 1) ? 1 : 0;
+          if (kind == 1) {
+            sbwt_fastq_parser *q = nullptr;
+            if (fq_made < kBuffers) {
+              check(sbwt_fastq_parser_create(max_chars, max_reads, nullptr, &q));
+              ++fq_made;
+            } else if (!free_fq.pop(q)) {
+              ok = false;
+              break;
+            }
+            u64 used = 0, nc = 0, nr = 0;
+            const int rc = sbwt_fastq_parser_parse(q, b.chars.p, b.n_chars, last ? 1 : 0, nullptr,
+                                                   0, nullptr, 0, &used, &nc, &nr);
+            if (rc != 0 || used == 0) {
+              free_fq.push(q);
+              check(rc);
+              // the rest of the file goes to the host reader
+              reader = raw->to_reader(b.chars.p, b.n_chars);
+              b.n_chars = 0;
+              break;
+            }
+            InBatch pb;
+            pb.file = f;
+            pb.first = b.first;
+            pb.fq = q;
+            b.first = false;
+            const u64 tail = b.n_chars - used;
+            if (tail) std::memmove(b.chars.p, b.chars.p + used, tail);
+            b.n_chars = tail;
+            g_times.parse_ns += clk.lap();
+            if (!to_device.push(std::move(pb))) {
+              sbwt_fastq_parser_destroy(q);
+              ok = false;
+              break;
+            }
+            if (last && tail == 0) break;
+            continue;
+          }
           const u64 cut = sbwt_fasta_has_quality_line(b.chars.p, b.n_chars, 1)
                               ? 0
                               : sbwt_fasta_chunk_cut(b.chars.p, b.n_chars, last, max_reads);
@@ -435,8 +497,12 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
   // every buffer is back in its pool (or stranded in a queue after a failure)
   for (auto &x : free_in.drain()) x.release();
   for (auto &x : free_out.drain()) x.release();
-  for (auto &x : to_device.drain()) x.chars.release();
+  for (auto &x : to_device.drain()) {
+    x.chars.release();
+    if (x.fq) sbwt_fastq_parser_destroy(x.fq);
+  }
   for (auto &x : to_writer.drain()) x.bytes.release();
+  for (auto *q : free_fq.drain()) sbwt_fastq_parser_destroy(q);
   if (!error.empty()) throw std::runtime_error(error);
 }
 

@@ -190,10 +190,13 @@ def mixed_inputs(tmp_path_factory):
     """A built index and sequence files that take every path of the index
     command's reader: wrapped FASTA (device-parsed chunks), FASTA with a '+'
     line start late in the file and with a record longer than a batch (the
-    host reader takes over mid-file), FASTQ, junk before the first header, an
-    empty file, and gzip."""
+    host reader takes over mid-file), four-line FASTQ (device-parsed), FASTQ
+    with records the host reader reads differently (it takes over there),
+    CRLF FASTQ without a last '\n', junk before the first header, an empty
+    file, and gzip."""
     import gzip
     import workloads as W
+    from test_gpu_fasta import fastq_records
     d = tmp_path_factory.mktemp("mixed")
     g = W.genome_random(31, 40_000)
     codes, ends = W.concat([g])
@@ -223,6 +226,9 @@ def mixed_inputs(tmp_path_factory):
         "reads.fnq": b"".join(b"@r%d\n%s\n+\n%s\n" % (i, s, b"I" * len(s)) for i, s in enumerate(seqs[:500])),
         "junk_first.fna": b"junk line\nmore>" + fasta(seqs[:50])[1:],
         "empty.fna": b"",
+        "odd_records.fnq": fastq_records(5, 3000, 0.01),
+        "crlf_no_eol.fnq": b"".join(b"@r%d\r\n%s\r\n+\r\n%s\r\n" % (i, s, b"I" * len(s))
+                                    for i, s in enumerate(seqs[:300]))[:-1],
     }
     paths = []
     for name, data in files.items():

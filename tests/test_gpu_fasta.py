@@ -157,3 +157,146 @@ def test_search_fasta_format_matches_oracle_file(small_index, tmp_path, mode, wi
         parts.append(p.search_format(s, f, mode, piece[:cut])[0])
         pos += cut
     assert b"".join(parts) == exp
+
+
+def fastq_records(seed, n_rec, odd=0.0):
+    """FASTQ text of n_rec records. With probability `odd` a record takes one
+    of the forms the host reader reads differently from four lines (empty
+    sequence, short quality, sequence starting with '+', '@' or '>', a
+    second sequence line, junk before the header) or a harmless variant
+    (CRLF endings, '>' header, '+name' line, quality longer than the
+    sequence, '@' / '>' / '+' inside the quality)."""
+    rng = np.random.default_rng(seed)
+    out = bytearray()
+    qual_chars = np.frombuffer(b"IIII#5@>+!?", np.uint8)
+    for i in range(n_rec):
+        L = int(rng.integers(1, 120))
+        seq = bytes(np.frombuffer(b"ACGTN", np.uint8)[rng.integers(0, 5, L)])
+        qual = bytes(qual_chars[rng.integers(0, len(qual_chars), L)])
+        head, plus, nl, extra = b"@r%d" % i, b"+", b"\n", b""
+        if rng.random() < odd:
+            kind = int(rng.integers(0, 11))
+            if kind == 0:
+                seq, qual = b"", b"II@II"
+            elif kind == 1:
+                qual = qual[:-1]
+            elif kind == 2:
+                seq = b"+@>"[int(rng.integers(0, 3))].to_bytes(1, "little") + seq
+                qual += b"I"
+            elif kind == 3:
+                extra = b"ACGT\n"
+                qual += b"IIII"
+            elif kind == 4:
+                head = b"junk\n" + head
+            elif kind == 5:
+                nl = b"\r\n"
+            elif kind == 6:
+                head = b">r%d" % i
+            elif kind == 7:
+                plus = b"+r%d" % i
+            elif kind == 8:
+                qual += b"@@>"
+            elif kind == 9:
+                qual = b"@" + qual[1:]
+            else:
+                seq, qual = b"", b""
+        out += head + nl + seq + nl + extra + plus + nl + qual + nl
+    return bytes(out)
+
+
+def parse_in_chunks(parser, raw, window, tmp_path):
+    """The command line's protocol: parse whole-record prefixes chunk by chunk
+    and give the rest to the host reader (here the oracle) once a chunk yields
+    nothing."""
+    chars, ends, pos, device_reads = [], [], 0, 0
+    while pos < len(raw):
+        piece = raw[pos:pos + window]
+        used, c, e = parser.parse(piece, pos + window >= len(raw))
+        if used == 0:
+            c, e = oracle_parse(tmp_path, raw[pos:])
+            pos = len(raw)
+        else:
+            device_reads += len(e)
+            pos += used
+        base = sum(len(x) for x in chars)
+        ends.append(np.asarray(e, np.uint64) + np.uint64(base))
+        chars.append(c)
+    return b"".join(chars), np.concatenate(ends) if ends else np.zeros(0, np.uint64), device_reads
+
+
+@pytest.mark.parametrize("odd,window,max_reads", [(0.0, 1 << 21, 1 << 18), (0.0, 3000, 1 << 18),
+                                                  (0.0, 1 << 21, 5), (0.02, 1 << 21, 1 << 18),
+                                                  (0.3, 5000, 1 << 18)])
+def test_fastq_chunks_match_oracle(tmp_path, odd, window, max_reads):
+    """Four-line records parse on the device up to the first record the host
+    reader reads differently; the host (oracle) parse of the rest completes
+    the file: together they equal the oracle's parse of the whole file."""
+    say(f"fastq chunks odd={odd} window={window} max_reads={max_reads}")
+    raw = fastq_records(int(odd * 100) + window % 97, 4000, odd)
+    p = S.FastqParser(1 << 21, max_reads)
+    chars, ends, dev = parse_in_chunks(p, raw, window, tmp_path)
+    c0, e0 = oracle_parse(tmp_path, raw)
+    assert chars == c0 and np.array_equal(ends, e0)
+    if odd == 0.0:
+        assert dev == 4000  # every record parsed on the device
+    else:
+        assert dev > 0
+
+
+def test_fastq_record_forms(tmp_path):
+    """Each form alone: the device takes exactly the records the host reader
+    reads as four lines; a last record without '\\n' and trailing empty lines
+    are handled as the host reader handles them."""
+    say("fastq record forms")
+    p = S.FastqParser(1 << 16, 1 << 10)
+    good = b"@a\nACGT\n+\nIIII\n"
+    cases = [
+        (good, 1), (good * 3, 3), (good + b"@b\nAC\n+\nII", 2), (good + b"\n\n", 1),
+        (good + b"@b\n\n+\n\n", 1), (good + b"@b\nACG\n+\nII\n" + good, 1),
+        (good + b"@b\n+AC\n+\nIII\n", 1), (good + b"@b\nAC\nGT\n+\nIIII\n", 1),
+        (good + b"@b\nACGT\n+\nIIIIII@@\n" + good, 3), (b">a\nACGT\n+x\nI@>+\n", 1),
+        (b"@a\r\nACGT\r\n+\r\nIIII\r\n", 1), (b"@a\nACGT\n-\nIIII\n", 0),
+    ]
+    for raw, device_records in cases:
+        used, c, e = p.parse(raw, True)
+        assert len(e) == device_records, raw
+        chars, ends, _ = parse_in_chunks(p, raw, 1 << 16, tmp_path)
+        c0, e0 = oracle_parse(tmp_path, raw)
+        assert chars == c0 and np.array_equal(ends, e0), raw
+    # not the end of the file: a last line without '\n' may continue
+    used, c, e = p.parse(good + b"@b\nAC\n+\nII", False)
+    assert used == len(good) and len(e) == 1
+
+
+@pytest.mark.parametrize("mode", [S.FORMAT_ASCII, S.FORMAT_BOOL], ids=["ascii", "bool"])
+def test_search_fastq_format_matches_oracle_file(small_index, tmp_path, mode):
+    g, path, orc = small_index
+    say(f"search fastq format mode={mode}")
+    lens = np.concatenate([np.array([10, 31, 30, 1, 32], np.uint32), W.uniform_lengths(7, 3000, 1, 400)])
+    chars, ends, _ = W.reads(13, g, np.array([len(g)], np.uint64), lens, 0.3, 0.02, 0.01)
+    fq = tmp_path / "r.fq"
+    st = 0
+    with open(fq, "wb") as f:
+        for i, e in enumerate(ends):
+            s = chars[st:int(e)].tobytes()
+            f.write(b"@r%d\n%s\n+\n%s\n" % (i, s, b"I" * len(s)))
+            st = int(e)
+    out = tmp_path / "o"
+    orc.index_file(str(fq), str(out), mode, True)
+    exp = open(str(out), "rb").read()[HEADER[mode]:]
+    raw = fq.read_bytes()
+    idx = S.Index.load(path)
+    s = idx.searcher(len(raw), len(ends))
+    f = S.Formatter(len(raw), len(ends))
+    p = S.FastqParser(len(raw), len(ends))
+    got, used, n = p.search_format(s, f, mode, raw)
+    assert used == len(raw) and n == len(ends)
+    assert got == exp
+    parts, pos = [], 0
+    while pos < len(raw):
+        piece = raw[pos:pos + 30_000]
+        b, used, _ = p.search_format(s, f, mode, piece, pos + 30_000 >= len(raw))
+        assert used > 0
+        parts.append(b)
+        pos += used
+    assert b"".join(parts) == exp
