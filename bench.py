@@ -46,10 +46,11 @@ def parse():
                          "colors: the color stage on the C3 recipe; cli: the sbwt_search "
                          "command end to end on C2 written as FASTA and FASTQ; cli-colors: "
                          "`sbwt_search colors` on C3 index-result files (ascii and binary)")
-    ap.add_argument("--color-genomes", type=int, default=200)
+    # the C3 recipe: 1000 colors of 200-500 kbp genomes (index build fits one command)
+    ap.add_argument("--color-genomes", type=int, default=1000)
     ap.add_argument("--color-roots", type=int, default=20)
-    ap.add_argument("--color-min-len", type=int, default=2_000_000)
-    ap.add_argument("--color-max-len", type=int, default=5_000_000)
+    ap.add_argument("--color-min-len", type=int, default=200_000)
+    ap.add_argument("--color-max-len", type=int, default=500_000)
     ap.add_argument("--color-reads", type=int, default=5_000_000)
     ap.add_argument("--batch-chars", type=int, default=600_000_000)
     ap.add_argument("--long-reads", type=int, default=1_000_000)

@@ -14,6 +14,7 @@
 #include <stdexcept>
 
 #include "byte_automaton.hpp"
+#include "host_scan.hpp"
 #include "scan.hpp"
 
 namespace sbwt_amd {
@@ -259,38 +260,68 @@ u64 IndexFileParser::parse_host(int format, const char *body, u64 n_bytes, u64 *
 }
 
 u64 IndexFileParser::batch_cut(int format, const char *body, u64 n, u64 max_reads) {
+  if (max_reads == 0) return 0;
   if (format == kIndexFileBinary) {
-    u64 cut = 0, reads = 0;
-    for (u64 w = 0; w + 8 <= n && reads < max_reads; w += 8) {
-      u64 v;
-      std::memcpy(&v, body + w, 8);
-      if (v == kNewline) {
-        ++reads;
-        cut = w + 8;
+    // read ends are u64 max - 2 words: stop after the max_reads-th one, then
+    // step back to the last one
+    const u64 words = n / 8;
+    u64 limit = words;
+    if (words > max_reads) {
+      u64 seen = 0;
+      for (u64 w = 0; w < words;) {
+        const u64 span = std::min<u64>(words - w, 1 << 16);
+        const u64 c = count_word(body + 8 * w, span, kNewline);
+        if (seen + c < max_reads) {
+          seen += c;
+          w += span;
+          continue;
+        }
+        for (u64 q = w; q < w + span; ++q) {
+          u64 v;
+          std::memcpy(&v, body + 8 * q, 8);
+          if (v == kNewline && ++seen == max_reads) {
+            limit = q + 1;
+            break;
+          }
+        }
+        break;
       }
     }
-    return cut;
+    for (u64 w = limit; w > 0; --w) {
+      u64 v;
+      std::memcpy(&v, body + 8 * (w - 1), 8);
+      if (v == kNewline) return 8 * w;
+    }
+    return 0;
   }
   // A read end is at most one per '\n' byte: stop at the max_reads-th one,
   // then step back to a '\n' that is not the char consumed after a '-'.
-  u64 limit = n, seen = 0;
-  for (u64 p = 0; p < n;) {
-    const u64 span = std::min<u64>(n - p, 1 << 16);
-    const u64 c = static_cast<u64>(std::count(body + p, body + p + span, '\n'));
-    if (seen + c <= max_reads) {
-      seen += c;
-      p += span;
-      continue;
-    }
-    for (u64 q = p; q < p + span; ++q)
-      if (body[q] == '\n' && ++seen == max_reads) {
-        limit = q + 1;
-        break;
+  u64 limit = n;
+  if (n > max_reads) {
+    u64 seen = 0;
+    for (u64 p = 0; p < n;) {
+      const u64 span = std::min<u64>(n - p, 1 << 16);
+      const u64 c = count_byte(body + p, span, '\n');
+      if (seen + c < max_reads) {
+        seen += c;
+        p += span;
+        continue;
       }
-    break;
+      for (u64 q = p; q < p + span; ++q)
+        if (body[q] == '\n' && ++seen == max_reads) {
+          limit = q + 1;
+          break;
+        }
+      break;
+    }
   }
-  for (u64 c = limit; c > 0; --c)
-    if (body[c - 1] == '\n' && (c < 2 || body[c - 2] != '-')) return c;
+  for (u64 c = limit; c > 0;) {
+    const void *nl = memrchr(body, '\n', static_cast<size_t>(c));
+    if (!nl) break;
+    c = static_cast<u64>(static_cast<const char *>(nl) - body) + 1;
+    if (c < 2 || body[c - 2] != '-') return c;
+    --c;
+  }
   return 0;
 }
 

@@ -104,3 +104,31 @@ def test_fasta_quality_line_detection():
     assert S.fasta_chunk_cut(b">a\nAC\n>b\nGT", False, 10) == 6
     assert S.fasta_chunk_cut(b">a\nACGT", False, 10) == 0
     assert S.fasta_chunk_cut(b">a\nACGT", True, 10) == 7
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_index_file_batch_cut_by_brute_force(seed):
+    """Host-only cut rule of index-file batches, checked over its definition:
+    ascii cuts right after the last '\\n' not preceded by '-' among the first
+    max_reads '\\n' bytes; binary right after the last u64 max - 2 word among
+    the first max_reads of them. Spans of 64 KiB are crossed."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    n = 150_000 + seed * 33_333
+    alphabet = np.frombuffer(b"0123456789 --\n", np.uint8)
+    body = alphabet[rng.integers(0, len(alphabet), n)].tobytes()
+    nls = [i for i in range(n) if body[i] == 10]
+    for max_reads in (1, 2, 7, 1000, len(nls) - 1, len(nls), len(nls) + 5, 10 * n):
+        limit = nls[max_reads - 1] + 1 if max_reads <= len(nls) else n
+        want = max((i + 1 for i in nls if i + 1 <= limit and (i < 1 or body[i - 1] != ord("-"))),
+                   default=0)
+        assert S.index_file_batch_cut(0, body, max_reads) == want, max_reads
+    words = rng.integers(0, 6, n // 8).astype(np.uint64)
+    words[words == 5] = np.uint64(2**64 - 3)
+    raw = words.tobytes() + b"\x01\x02\x03"  # a partial last word is not read
+    ends = np.flatnonzero(words == np.uint64(2**64 - 3))
+    for max_reads in (1, 3, 100, len(ends) - 1, len(ends), len(ends) + 1):
+        want = 8 * (int(ends[min(max_reads, len(ends)) - 1]) + 1) if len(ends) else 0
+        assert S.index_file_batch_cut(1, raw, max_reads) == want, max_reads
+    assert S.index_file_batch_cut(0, b"12 -\n", 5) == 0
+    assert S.index_file_batch_cut(0, b"12\n-\n", 5) == 3
