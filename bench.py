@@ -63,6 +63,9 @@ def parse():
                     help="Poisson arrival rates (requests/s) to run, comma separated")
     ap.add_argument("--serve-max-reads", type=int, default=100_000)
     ap.add_argument("--reads", type=int, default=0, help="override the number of reads")
+    ap.add_argument("--cli-runs", default="",
+                    help="cli mode: comma list of input:output:parsing runs (e.g. "
+                         "fastq:bool:device); default all")
     ap.add_argument("--cpu-sample-reads", type=int, default=1_500_000,
                     help="reads of the batch timed on the CPU oracle (about 10 s of host work)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -463,33 +466,73 @@ def mode_cli(args, configs, S, torch, W):
     subprocess.run([exe, "index", "-q", inputs["fasta"][0], "-i", sbwt_path, "-o", warm],
                    check=True, capture_output=True)
     os.remove(warm + ".txt")
-    runs = [("fasta", "ascii"), ("fasta", "binary"), ("fastq", "ascii"), ("fastq", "bool")]
-    for kind, fmt in runs:
+    runs = [(kind, fmt, parse)
+            for kind, fmt in (("fasta", "ascii"), ("fasta", "binary"), ("fastq", "ascii"),
+                              ("fastq", "bool"))
+            for parse in ("device", "host")]
+    if args.cli_runs:
+        runs = [tuple(r.split(":")) for r in args.cli_runs.split(",")]
+    for kind, fmt, parse in runs:
         path, in_bytes = inputs[kind]
-        for parse in ("device", "host"):
-            prefix = os.path.join(tmp, "out_" + fmt)
-            env = dict(os.environ, SBWT_SEARCH_TIMING="1")
-            extra = ["--host-parse"] if parse == "host" else []
-            t0 = time.perf_counter()
-            r = subprocess.run([exe, "index", "-q", path, "-i", sbwt_path, "-o", prefix, "-p", fmt]
-                               + extra, env=env, capture_output=True, text=True)
-            dt = time.perf_counter() - t0
-            if r.returncode != 0:
-                raise SystemExit(f"sbwt_search failed: {r.stderr}")
-            out_file = prefix + ext[fmt]
-            size = os.path.getsize(out_file)
-            log(f"cli {kind} -> {fmt} ({parse} parsing): {dt:.2f} s, {r.stderr.strip()}")
-            out[f"{kind}_{fmt}_{parse}_parse"] = {
-                "seconds": dt, "reads_per_s": n / dt, "kmers_per_s": wl.n_kmers() / dt,
-                "input_MB_per_s": in_bytes / dt / 1e6, "output_bytes": size,
-                "stages": r.stderr.strip()}
-            os.remove(out_file)
+        prefix = os.path.join(tmp, "out_" + fmt)
+        env = dict(os.environ, SBWT_SEARCH_TIMING="1")
+        extra = ["--host-parse"] if parse == "host" else []
+        t0 = time.perf_counter()
+        r = subprocess.run([exe, "index", "-q", path, "-i", sbwt_path, "-o", prefix, "-p", fmt]
+                           + extra, env=env, capture_output=True, text=True)
+        dt = time.perf_counter() - t0
+        if r.returncode != 0:
+            raise SystemExit(f"sbwt_search failed: {r.stderr}")
+        out_file = prefix + ext[fmt]
+        size = os.path.getsize(out_file)
+        log(f"cli {kind} -> {fmt} ({parse} parsing): {dt:.2f} s, {r.stderr.strip()}")
+        out[f"{kind}_{fmt}_{parse}_parse"] = {
+            "seconds": dt, "reads_per_s": n / dt, "kmers_per_s": wl.n_kmers() / dt,
+            "input_MB_per_s": in_bytes / dt / 1e6, "output_bytes": size,
+            "stages": r.stderr.strip()}
+        os.remove(out_file)
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cli_cpu_baseline(args, wl, exe, sbwt_path, inputs["fasta"][0], L, tmp)
     for path, _ in inputs.values():
         os.remove(path)
     os.remove(sbwt_path)
     os.rmdir(tmp)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def cli_cpu_baseline(args, wl, exe, sbwt_path, fasta, L, tmp):
+    """The oracle's FASTA -> ascii file (C/OpenMP restatement of the reference's
+    index command: read, presearch-12 search, classify, print) on the first
+    reads of the same FASTA file; the command line runs on the same sample and
+    its file must be byte-identical."""
+    import subprocess
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    import oracle_py as O
+    n_sample = min(args.cpu_sample_reads, wl.n_reads)
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    sample = os.path.join(tmp, "sample.fna")
+    with open(fasta, "rb") as f, open(sample, "wb") as g:
+        g.write(f.read(n_sample * (L + 4)))
+    orc = O.OracleSbwt(sbwt_path)
+    t0 = time.perf_counter()
+    orc.index_file(sample, os.path.join(tmp, "cpu.txt"), 0, True, threads=threads)
+    dt = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    subprocess.run([exe, "index", "-q", sample, "-i", sbwt_path, "-o", os.path.join(tmp, "gpu")],
+                   check=True, capture_output=True)
+    dt_gpu = time.perf_counter() - t0
+    same = open(os.path.join(tmp, "cpu.txt"), "rb").read() == open(os.path.join(tmp, "gpu.txt"), "rb").read()
+    for name in ("cpu.txt", "gpu.txt", "sample.fna"):
+        os.remove(os.path.join(tmp, name))
+    if not same:
+        raise SystemExit("sbwt_search output differs from the CPU oracle's file on the sample")
+    log(f"cli cpu baseline: {n_sample} reads FASTA -> ascii in {dt:.2f} s on {threads} threads "
+        f"(sbwt_search on the same sample: {dt_gpu:.2f} s, byte-identical)")
+    return {"value": n_sample / dt, "unit": "reads/s", "cores": threads, "kind": "port",
+            "sample": f"first {n_sample} reads of the FASTA file, FASTA -> ascii file, "
+                      "including the oracle's index load and presearch",
+            "gpu_same_sample_seconds": dt_gpu}
 
 
 def mode_cli_colors(args, configs, S, torch, W):

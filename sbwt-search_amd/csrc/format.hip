@@ -348,14 +348,18 @@ u64 Formatter::format_device_closes_to_host(int mode, const u64 *d_results,
                                             u64 capacity) {
   last_bytes_ = 0;
   if (n_pieces == 0) return 0;
+  // written at once when the device buffer holds it; else grown and formatted again
   const u64 bytes = format(mode, d_results, d_piece_kmer_ends, d_closes, n_pieces, line_open,
-                           nullptr, 0);
+                           d_out_, out_capacity_);
   if (bytes > out_capacity_) {
     if (d_out_) SBWT_HIP(hipFree(d_out_));
-    out_capacity_ = bytes + bytes / 4 + 4096;
-    SBWT_HIP(hipMalloc(&d_out_, out_capacity_));
+    d_out_ = nullptr;
+    out_capacity_ = 0;
+    const u64 cap = bytes + bytes / 4 + 4096;
+    SBWT_HIP(hipMalloc(&d_out_, cap));
+    out_capacity_ = cap;
+    format(mode, d_results, d_piece_kmer_ends, d_closes, n_pieces, line_open, d_out_, out_capacity_);
   }
-  format(mode, d_results, d_piece_kmer_ends, d_closes, n_pieces, line_open, d_out_, out_capacity_);
   last_bytes_ = bytes;
   if (host_out && capacity >= bytes) copy_last(host_out, capacity);
   return bytes;
@@ -438,7 +442,8 @@ u64 ColorFormatter::format(int mode, const u64 *d_offsets, const u32 *d_ids, u64
 u64 ColorFormatter::format_to_host(int mode, const u64 *d_offsets, const u32 *d_ids, u64 n_reads,
                                    u64 num_colors, char *host_out, u64 capacity) {
   last_bytes_ = 0;
-  const u64 bytes = format(mode, d_offsets, d_ids, n_reads, num_colors, nullptr, 0);
+  // written at once when the device buffer holds it; else grown and formatted again
+  const u64 bytes = format(mode, d_offsets, d_ids, n_reads, num_colors, d_out_, out_capacity_);
   if (bytes > out_capacity_) {
     if (d_out_) SBWT_HIP(hipFree(d_out_));
     d_out_ = nullptr;
@@ -446,8 +451,8 @@ u64 ColorFormatter::format_to_host(int mode, const u64 *d_offsets, const u32 *d_
     const u64 cap = bytes + bytes / 4 + 4096;
     SBWT_HIP(hipMalloc(&d_out_, cap));
     out_capacity_ = cap;
+    format(mode, d_offsets, d_ids, n_reads, num_colors, d_out_, out_capacity_);
   }
-  format(mode, d_offsets, d_ids, n_reads, num_colors, d_out_, out_capacity_);
   last_bytes_ = bytes;
   if (host_out && capacity >= bytes) copy_last(host_out, capacity);
   return bytes;
