@@ -94,8 +94,10 @@ double fraction(const Args &a, const std::string &key, const std::string &def, c
 }
 
 // chars per index-search batch from free device memory: input, packed
-// sequence, invalid mask, results and offsets (about 13 bytes per char) plus
-// the formatted output (up to 21 bytes per k-mer in ascii), at most 2^24.
+// sequence, invalid mask, results and offsets (about 13 bytes per char), the
+// formatted output (up to 21 bytes per k-mer in ascii) and the device parsers
+// (one FASTA parser, about 3 bytes per char, and three FASTQ parsers, about 4
+// each), at most 2^24.
 // The cap is measured, not a memory limit: file to file, the command is bound
 // by writing its output, and with 2^27-char batches (output buffers near
 // 1 GB) the writes ran at half the speed they reach with 2^24-char batches,
@@ -104,7 +106,7 @@ u64 chars_per_batch(double gpu_fraction, u64 streams) {
   u64 free_b = 0, total = 0;
   check(sbwt_mem_info(&free_b, &total));
   const u64 budget = static_cast<u64>(static_cast<double>(free_b) * gpu_fraction) / std::max<u64>(streams, 1);
-  const u64 n = std::min<u64>(budget / 34, 1ULL << 24);
+  const u64 n = std::min<u64>(budget / 49, 1ULL << 24);
   if (n == 0) throw std::runtime_error("Not enough memory");
   return n;
 }
