@@ -44,7 +44,11 @@ constexpr u32 kKeyTableSteps = 4096;  // walk cap of the table build
 //                  from the line's first node back to the start of its suffix
 //                  group (kFarGroup if unknown or too far)
 //   slot 5         key-k-mer marks: {count, bits[0..95]}
-//   slots 6, 7     zero
+//   slots 6, 7     key-walk codes of nodes 0..47 and 48..95 (indexes with key
+//                  marks; zero otherwise): one of six values per node, three
+//                  nodes per byte (c0 + 6 c1 + 36 c2): 0..3 the node is not
+//                  marked and its key walk takes its first edge, that letter;
+//                  4 the node is marked; 5 neither (no edge: the walk is stuck)
 // Counts are relative to the line's hyperblock (2^25 lines) whose absolute
 // bases sit in a small table; indexes below 2^32 nodes have a single
 // hyperblock with base 0, so a rank is one 16-byte load and a popcount.
@@ -55,6 +59,15 @@ constexpr u32 kBlockNodes = 96;
 constexpr u32 kLineSlots = 8;
 constexpr u32 kSlotGroups = 4;
 constexpr u32 kSlotMarks = 5;
+constexpr u32 kSlotWalk = 6;            // slots 6 and 7
+constexpr u32 kWalkNodesPerSlot = 48;
+constexpr u32 kWalkMarked = 4, kWalkNoEdge = 5;
+// What the streaming search leaves in bits 61-62 of a found k-mer's node for
+// the key pass, about the k-mer before it in the read (found, same work item):
+// nothing known, same key as this k-mer (its walk's first step is this
+// k-mer's node), marked (its own key), or stuck walk.
+constexpr u32 kPredShift = 61;
+constexpr u32 kPredLookup = 0, kPredSameKey = 1, kPredMarked = 2, kPredStuck = 3;
 constexpr u32 kHyperShift = 25;  // lines per hyperblock = 2^25 (3.2 G nodes)
 constexpr u32 kMarkBits = 64;  // bits per mark entry of the color index
 constexpr u32 kFarGroup = 0xFFFFFFFFu;

@@ -208,6 +208,26 @@ void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const
       }
       host[b * kLineSlots + s] = e;
     }
+    if (km_words) {  // key-walk codes
+      unsigned char code_bytes[32] = {};
+      for (u32 o = 0; o < kBlockNodes; ++o) {
+        const u64 node = b * kBlockNodes + o;
+        u32 code = kWalkNoEdge;
+        if (node < n) {
+          if (node < km_bits && ((km_words[node >> 6] >> (node & 63)) & 1)) {
+            code = kWalkMarked;
+          } else {
+            for (u32 c = 0; c < 4; ++c)
+              if ((acgt[c][node >> 6] >> (node & 63)) & 1) {
+                code = c;
+                break;
+              }
+          }
+        }
+        code_bytes[o / 3] = static_cast<unsigned char>(code_bytes[o / 3] + code * (o % 3 == 0 ? 1 : (o % 3 == 1 ? 6 : 36)));
+      }
+      std::memcpy(&host[b * kLineSlots + kSlotWalk], code_bytes, 32);
+    }
   });
   n_hyper = (n_blocks >> hyper_shift) + 1;
   std::vector<u64> hyper(n_hyper * kLineSlots, 0);

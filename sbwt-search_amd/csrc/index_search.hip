@@ -407,7 +407,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     IndexView ix, const u64 *__restrict__ packed, const u32 *__restrict__ invalid,
     const ulonglong2 *__restrict__ chunks, const u64 *__restrict__ chunk_ends, u64 n_reads,
     u64 *__restrict__ counter, u64 *__restrict__ out, u64 *__restrict__ stats, u32 absence) {
-  enum : u32 { kChunk, kSeq0, kSeq1, kStream, kGsEntry, kPresearch, kTail, kStep, kKey, kDone };
+  enum : u32 { kChunk, kSeq0, kSeq1, kStream, kGsEntry, kPresearch, kTail, kStep, kDone };
   enum : u32 { kHintNone, kHintMiss, kHintPre };  // what the last miss left known
   const u64 n_chunks = chunk_ends[n_reads - 1];
   const u32 lane = threadIdx.x & 63;
@@ -437,6 +437,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
   u64 miss_at = 0, tq = 0;            // char that ended the last miss; checked kp-mer start
   u32 hint = kHintNone;
   u32 left = 0, j = 0, c = 0;
+  u32 pred = kPredLookup;  // KEY: what the next found result says about v (set by kStream)
   Idx l = 0, r = 0, v = 0, gs = 0;
   bool prev = false;
   __shared__ u64 obuf[8][kSearchThreads];  // staged results, obuf[slot][thread]
@@ -500,10 +501,6 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         a1 = e1;
         a2 = line_slot(ix, b2, c);
         break;
-      case kKey:  // the 16 bytes of the key-destination table holding v's entry
-        a1 = a2 = static_cast<const uint4 *>(ix.key_dest) +
-                  (sizeof(Idx) == 4 ? (static_cast<u64>(v) >> 2) : (static_cast<u64>(v) >> 1));
-        break;
       default:
         break;
     }
@@ -513,6 +510,13 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     // would add a dependent round trip
     asm volatile("" : "+v"(d1.x), "+v"(d1.y), "+v"(d1.z), "+v"(d1.w), "+v"(d2.x), "+v"(d2.y),
                  "+v"(d2.z), "+v"(d2.w));
+    // KEY: v's key-walk code, from the line the stream step reads anyway
+    uint4 d3 = make_uint4(0, 0, 0, 0);
+    if constexpr (KEY) {
+      d3 = *(mode == kStream ? line_slot(ix, b1, kSlotWalk + (o1 >= kWalkNodesPerSlot ? 1u : 0u))
+                             : dummy);
+      asm volatile("" : "+v"(d3.x), "+v"(d3.y), "+v"(d3.z), "+v"(d3.w));
+    }
     if (refill) {
       tbase = dyn0 + __shfl(tbase, 0, 64);
       if (tbase >= n_chunks) {
@@ -528,8 +532,6 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     const bool bit1 = entry_bit(d1, o1);
     // ---- advance the lane's state
     bool found = false, notfound = false, decide = false, resolve = false;
-    bool key_done = false;  // kKey: the table gave the found k-mer's key node
-    u64 key_res = 0;
     Idx found_at = x1;
     switch (mode) {
       case kChunk:
@@ -562,6 +564,18 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         break;
       case kStream:
         if (STATS) ++st_stream;
+        if constexpr (KEY) {
+          // v's code: the byte of its three-node group, then its base-6 digit
+          const u32 o3 = o1 >= kWalkNodesPerSlot ? o1 - kWalkNodesPerSlot : o1;
+          const u32 q3 = o3 / 3;
+          const u32 word = (q3 >> 2) == 0 ? d3.x : ((q3 >> 2) == 1 ? d3.y : ((q3 >> 2) == 2 ? d3.z : d3.w));
+          const u32 byte = (word >> (8 * (q3 & 3))) & 255u;
+          const u32 dig = o3 - 3 * q3;
+          const u32 wcode = (dig == 0 ? byte : (dig == 1 ? byte / 6 : byte / 36)) % 6;
+          pred = wcode == kWalkMarked ? kPredMarked
+                 : (wcode == kWalkNoEdge ? kPredStuck
+                                         : (bit1 && wcode == c ? kPredSameKey : kPredLookup));
+        }
         if (bit1) {
           found = true;
         } else if (!ix.has_groups) {
@@ -636,24 +650,6 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         if (STATS) ++st_steps;
         resolve = true;
         break;
-      case kKey:
-        if constexpr (KEY) {
-          u64 dest;
-          if constexpr (sizeof(Idx) == 4) {
-            const u32 q = static_cast<u32>(v) & 3;
-            const u32 e = q == 0 ? d1.x : (q == 1 ? d1.y : (q == 2 ? d1.z : d1.w));
-            dest = e == kKeyLong32 ? kKeyLong64 : (e == kKeyStuck32 ? kWalkStuck : static_cast<u64>(e));
-          } else {
-            dest = (static_cast<u64>(v) & 1) ? (static_cast<u64>(d1.z) | (static_cast<u64>(d1.w) << 32))
-                                              : (static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32));
-          }
-          u32 w = 1;
-          if (dest == kKeyLong64) dest = key_walk<Idx>(ix, v, ix.num_nodes, w);
-          if (STATS) st_walk += w;
-          key_res = dest;
-          key_done = true;
-        }
-        break;
       default:
         break;
     }
@@ -670,31 +666,29 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         c = static_cast<u32>(window >> (62 - 2 * j)) & 3u;
       }
     }
-    u64 res = key_done ? key_res : kNotFound;
-    bool emitted = notfound || key_done;
+    u64 res = kNotFound;
+    bool emitted = notfound;
     if (notfound && (mode == kStream || mode == kGsEntry)) {
       // the k-mer's last char made the miss: its predecessor was found
       hint = kHintMiss;
       miss_at = p + k - 1;
     }
-    if (notfound) prev = false;
+    if (notfound) {
+      prev = false;
+      pred = kPredLookup;
+    }
     if (found) {
       v = found_at;
       prev = true;
       hint = kHintNone;
+      res = static_cast<u64>(v);
       if constexpr (KEY) {
-        if (ix.key_dest) {
-          mode = kKey;  // emitted when the table entry arrives
-        } else {
-          u32 w = 0;
-          res = key_walk<Idx>(ix, v, ix.num_nodes, w);
-          if (STATS) st_walk += w;
-          emitted = true;
-        }
-      } else {
-        res = static_cast<u64>(v);
-        emitted = true;
+        // the node and what it says about the k-mer before it; k_key_runs
+        // replaces it with the key node
+        res |= static_cast<u64>(pred) << kPredShift;
+        pred = kPredLookup;
       }
+      emitted = true;
     }
     // ---- emit, then set up the next k-mer (invalid k-mers resolve here)
     // stage one result in LDS; a complete 64-byte group (or the chunk's end)
@@ -809,6 +803,77 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     atomicAdd(reinterpret_cast<unsigned long long *>(stats + 8), static_cast<u64>(st_tail));
     atomicAdd(reinterpret_cast<unsigned long long *>(stats + 9), static_cast<u64>(st_tail_hit));
     atomicAdd(reinterpret_cast<unsigned long long *>(stats + 10), static_cast<u64>(st_skip));
+  }
+}
+
+// Key-k-mer move of the streaming search's results (move_to_key_kmer), one
+// wave per work item, right to left in blocks of 64 k-mers. A found k-mer's
+// result holds its node and, in bits 61-62, what the search learnt about the
+// k-mer before it from that k-mer's node line (kPred*). The key of k-mer i is
+// then: the key of k-mer i + 1 when that result says "same key" (k-mer i's
+// node is unmarked and its walk's first step is k-mer i + 1's node), its own
+// node when marked, kWalkStuck when its walk has no edge, else the table (or
+// the walk, key_move). Chains of "same key" are resolved with one ballot and a
+// shuffle per block; the block's first k-mer hands its key and flag to the
+// block on its left. Only chain ends read the key table, instead of every
+// found k-mer. Work items never share a chain: each starts with a full search.
+template <typename Idx, bool STATS>
+__global__ __launch_bounds__(kSearchThreads) void k_key_runs(IndexView ix,
+                                                            const ulonglong2 *__restrict__ chunks,
+                                                            const u64 *__restrict__ chunk_ends,
+                                                            u64 n_reads, u64 *__restrict__ out,
+                                                            u64 *__restrict__ stats) {
+  const u64 n_chunks = chunk_ends[n_reads - 1];
+  const u32 lane = threadIdx.x & 63;
+  const u64 n_waves = static_cast<u64>(gridDim.x) * (kSearchThreads / 64);
+  const u64 wave = static_cast<u64>(blockIdx.x) * (kSearchThreads / 64) + threadIdx.x / 64;
+  constexpr u64 kNodeMask = (1ULL << kPredShift) - 1;
+  u32 st_walk = 0;
+  for (u64 ci = wave; ci < n_chunks; ci += n_waves) {
+    const ulonglong2 item = chunks[ci];
+    const u64 g0 = item.x;
+    const u32 len = static_cast<u32>(item.y >> 48);
+    u32 carry_flag = kPredLookup;  // flag of the k-mer right of the block
+    u64 carry_key = 0;             // and its key
+    for (u32 hi = len; hi > 0;) {
+      const u32 lo = hi > 64 ? hi - 64 : 0;
+      // lane 63 holds k-mer hi - 1, lane q k-mer hi - 64 + q; none below lo
+      const bool valid = lane + hi >= lo + 64;
+      const u64 pos = g0 + (valid ? hi + lane - 64 : 0);
+      const u64 rv = valid ? out[pos] : kNotFound;
+      const bool found = (rv >> 63) == 0;
+      const u32 flag = found ? static_cast<u32>(rv >> kPredShift) & 3u : kPredLookup;
+      const u64 node = rv & kNodeMask;
+      u32 succ = static_cast<u32>(__shfl_down(static_cast<int>(flag), 1, 64));
+      if (lane == 63) succ = carry_flag;
+      const bool same = found && succ == kPredSameKey;
+      const bool brk = found && !same;
+      u64 base = 0;
+      if (brk) {
+        if (succ == kPredMarked) {
+          base = node;
+        } else if (succ == kPredStuck) {
+          base = kWalkStuck;
+        } else {
+          u32 w = 0;
+          base = key_move<Idx>(ix, static_cast<Idx>(node), w);
+          if (STATS) st_walk += w;
+        }
+      }
+      // a chain takes the key of the first break to its right, else the carry
+      const u64 breaks = __ballot(brk);
+      const u64 right = lane == 63 ? 0 : breaks & (~0ULL << (lane + 1));
+      const int src = right ? __ffsll(static_cast<long long>(right)) - 1 : 0;
+      const u64 from = __shfl(base, src, 64);
+      const u64 key = brk ? base : (right ? from : carry_key);
+      if (valid && found) out[pos] = key;
+      carry_flag = static_cast<u32>(__shfl(static_cast<int>(flag), 0, 64));
+      carry_key = __shfl(key, 0, 64);
+      hi = lo;
+    }
+  }
+  if constexpr (STATS) {
+    if (st_walk) atomicAdd(reinterpret_cast<unsigned long long *>(stats + 2), static_cast<u64>(st_walk));
   }
 }
 
@@ -1065,6 +1130,11 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
       k_search_stream<Idx, KEY, STATS><<<blocks, kSearchThreads, 0, stream_>>>(
           ix, d_packed_, d_invalid_, d_chunks_, d_item_ends, n_reads, d_counter_, d_results,
           d_stats_, absence_checks(*index_) ? 1u : 0u);
+      if constexpr (KEY) {
+        // one wave per work item
+        k_key_runs<Idx, STATS><<<grid_for(chunk_bound, kSearchThreads / 64), kSearchThreads, 0,
+                                 stream_>>>(ix, d_chunks_, d_item_ends, n_reads, d_results, d_stats_);
+      }
     });
   } else {
     if (timing_) SBWT_HIP(hipEventRecord(ev_[2], stream_));
