@@ -835,12 +835,23 @@ __global__ __launch_bounds__(kSearchThreads) void k_key_runs(IndexView ix,
     const u32 len = static_cast<u32>(item.y >> 48);
     u32 carry_flag = kPredLookup;  // flag of the k-mer right of the block
     u64 carry_key = 0;             // and its key
+    // lane 63 holds k-mer hi - 1, lane q k-mer hi - 64 + q; none below lo
+    auto block_pos = [&](u32 hi_, bool &valid_) {
+      const u32 lo_ = hi_ > 64 ? hi_ - 64 : 0;
+      valid_ = hi_ > 0 && lane + hi_ >= lo_ + 64;
+      return g0 + (valid_ ? hi_ + lane - 64 : 0);
+    };
+    bool valid_next = false;
+    u64 pos_next = block_pos(len, valid_next);
+    u64 rv_next = valid_next ? out[pos_next] : kNotFound;
     for (u32 hi = len; hi > 0;) {
       const u32 lo = hi > 64 ? hi - 64 : 0;
-      // lane 63 holds k-mer hi - 1, lane q k-mer hi - 64 + q; none below lo
-      const bool valid = lane + hi >= lo + 64;
-      const u64 pos = g0 + (valid ? hi + lane - 64 : 0);
-      const u64 rv = valid ? out[pos] : kNotFound;
+      const bool valid = valid_next;
+      const u64 pos = pos_next;
+      const u64 rv = rv_next;
+      // the block to the left is loaded while this one is resolved
+      pos_next = block_pos(lo, valid_next);
+      rv_next = valid_next ? out[pos_next] : kNotFound;
       const bool found = (rv >> 63) == 0;
       const u32 flag = found ? static_cast<u32>(rv >> kPredShift) & 3u : kPredLookup;
       const u64 node = rv & kNodeMask;
