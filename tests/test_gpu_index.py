@@ -411,3 +411,33 @@ def test_auto_algorithm_by_batch_size(genome_index):
             c, e = batches[i]
             assert np.array_equal(s.search_reads(c, e), expect[i]), (algo, i)
             assert (s.last_stats()["streamed"] > 0) == streams[i], (algo, i)
+
+
+@pytest.mark.parametrize("algo,launch", [(S.ALGO_STREAMING, (64, 4)),
+                                         (S.ALGO_STREAMING_ADAPTIVE, None)],
+                         ids=["items-64", "adaptive-items"])
+def test_key_kmer_move_across_work_items(algo, launch, tmp_path):
+    """The key pass resolves chains of same-key k-mers per work item; reads of
+    up to 3 kbp split into many items (64 k-mers, or the adaptive 8+ of a
+    small batch) cut chains at every item start, where the search restarts."""
+    say(f"key move across work items algo={algo} launch={launch}")
+    root = W.genome_random(41, 40_000)
+    gens = [root] + [W.genome_mutate(root, s, 0.01) for s in (42, 43)]
+    codes, ends = W.concat(gens)
+    b = W.BuiltSbwt(codes, ends, 31)
+    col = b.colors(np.arange(len(gens), dtype=np.uint32), d=7)
+    sp, cp = str(tmp_path / "w.sbwt"), str(tmp_path / "w.tcolors")
+    b.write(sp)
+    col.write(cp)
+    orc = O.OracleSbwt(sp)
+    km = np.concatenate([col.key_marks(), np.zeros(8, np.uint64)])
+    idx = S.Index.load(sp, cp)
+    lens = W.uniform_lengths(44, 300, 31, 3000)
+    chars, rends, _ = W.reads(45, codes, ends, lens, 0.1, 0.003, 0.0)
+    chars = chars.tobytes()
+    s = idx.searcher(len(chars), len(rends), move_to_key_kmer=True)
+    s.set_algorithm(algo)
+    if launch:
+        s.set_launch(*launch)
+    res = s.search_reads(chars, rends)
+    assert np.array_equal(res, orc.index_reads(chars, rends, key_marks=km))
