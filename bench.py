@@ -70,6 +70,9 @@ def parse():
                     help="reads of the batch timed on the CPU oracle (about 10 s of host work)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-kmer-table", action="store_true",
+                    help="search without the index's k-mer table (fresh searches from the "
+                         "presearch table and SBWT steps)")
     ap.add_argument("--presearch-letters", type=int, default=0,
                     help="presearch prefix letters (0: the library's choice for the index size; "
                          "the reference fixes 12)")
@@ -86,7 +89,7 @@ def parse():
     return ap.parse_args()
 
 
-def traffic_from_profile(config, algorithm, presearch_letters):
+def traffic_from_profile(config, algorithm, presearch_letters, kmer_table):
     """HBM bytes per search launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a
     copy of known size) from the committed summary of the same workload, or None."""
     path = os.path.join(REPO, "profiles", f"search_{algorithm}_{config}_pmc.json")
@@ -96,14 +99,16 @@ def traffic_from_profile(config, algorithm, presearch_letters):
         d = json.load(open(path))
     except (ValueError, OSError):
         return None
-    if (d.get("config"), d.get("algorithm"), d.get("presearch_letters")) != \
-            (config, algorithm, presearch_letters):
+    if (d.get("config"), d.get("algorithm"), d.get("presearch_letters"),
+            d.get("kmer_table", False)) != (config, algorithm, presearch_letters, kmer_table):
         return None
     return d.get("hbm_bytes_per_launch")
 
 
 def main():
+    global _ARGS
     args = parse()
+    _ARGS = args
     if args.gpus != 1:
         print("bench.py measures one GPU: the search path runs on a single MI355X here and "
               f"--gpus {args.gpus} is not supported (the path would shard by replicating the "
@@ -141,6 +146,7 @@ def main():
     log(f"device index: {idx.device_bytes / 2**20:.1f} MiB, wide={idx.wide}, "
         f"presearch {idx.presearch_letters} letters "
         f"({time.time() - t0:.1f} s)")
+    table = _kmer_table(args, idx)
 
     dev = torch.device("cuda:0")
     d_chars = torch.from_numpy(wl.chars).to(dev)
@@ -240,6 +246,7 @@ def main():
             "chars_per_step": wl.n_chars,
             "found_fraction": found / n_kmers,
             "presearch_letters": idx.presearch_letters,
+            "kmer_table": table,
             "algorithm": args.algorithm,
             "launch": launch,
             "search_kernel_ms": kern_ms,
@@ -254,7 +261,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic_from_profile(wl.name.lower(), args.algorithm.replace("-", "_"),
-                                            idx.presearch_letters),
+                                            idx.presearch_letters, table["built"]),
             "algorithmic_bytes_per_launch": algo_bytes,
             # the reference algorithm's cost model (one 64-byte line per presearch and
             # per SBWT step of every valid k-mer, u64 out): what an independent search
@@ -309,6 +316,21 @@ def cpu_baseline(args, wl, bits, built, gpu_res):
             "host": f"{model}, {os.cpu_count()} logical cpus"}
 
 
+_ARGS = None
+
+
+def _kmer_table(args, idx):
+    """Builds the index's k-mer table (once per index, before any timing) unless
+    --no-kmer-table; returns {"built": bool, "build_ms": float}."""
+    if args is not None and args.no_kmer_table:
+        return {"built": False, "build_ms": 0.0}
+    t0 = time.perf_counter()
+    built = idx.build_kmer_table()
+    ms = (time.perf_counter() - t0) * 1e3
+    log(f"k-mer table: {'built' if built else 'not built'} ({ms:.0f} ms)")
+    return {"built": built, "build_ms": ms}
+
+
 def _colored_index(W, S, built, color_of, d, tag):
     """Color index of a built SBWT, written to a temporary .tcolors and loaded."""
     import tempfile
@@ -320,6 +342,7 @@ def _colored_index(W, S, built, color_of, d, tag):
     km = col.key_marks()
     idx = S.Index.create([built.bits(c) for c in range(4)], built.n, 31, key_marks=km,
                          suffix_group_starts=built.bits(4))
+    _kmer_table(_ARGS, idx)
     cidx = S.ColorIndex(path)
     log(f"colors: {col.num_colors} colors, {col.num_sets} sets ({col.num_dense} dense), "
         f"{time.time() - t0:.1f} s")

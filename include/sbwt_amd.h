@@ -77,6 +77,19 @@ int sbwt_index_create(const uint64_t *const acgt[4], uint64_t num_bits, uint32_t
 int sbwt_index_load(const char *sbwt_path, const char *colors_path, uint32_t presearch_letters,
                     sbwt_index **out);
 
+/* Builds the index's k-mer table: every node whose label is a whole k-mer,
+ * by its 2k-bit code, so that a search that cannot step from the previous
+ * k-mer (a read start, a k-mer after a miss) reads about one 32-byte bucket
+ * pair instead of the presearch entry and k - presearch_letters SBWT steps.
+ * Results do not change. One-time cost (about a second for 10^8 nodes) and
+ * up to 32 bytes per node of device memory, so it pays for an index that
+ * serves many batches; the command line does not build it. *built (nullable)
+ * receives 0 when the index cannot have one (nodes >= 2^32, k > 32, or the
+ * table would take more than a sixteenth of device memory). No counterpart in
+ * the reference (its searches always start from the presearch table,
+ * src/IndexSearcher/IndexSearcher.cuh:36-55). */
+int sbwt_index_build_kmer_table(sbwt_index *index, int *built);
+
 void sbwt_index_destroy(sbwt_index *index);
 
 typedef struct {

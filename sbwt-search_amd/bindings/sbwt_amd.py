@@ -78,6 +78,7 @@ def lib():
         L.sbwt_index_create.argtypes = [u64p * 4, C.c_uint64, C.c_uint32, u64p, u64p,
                                         C.c_uint32, C.POINTER(vp)]
         L.sbwt_index_load.argtypes = [C.c_char_p, C.c_char_p, C.c_uint32, C.POINTER(vp)]
+        L.sbwt_index_build_kmer_table.argtypes = [vp, C.POINTER(C.c_int)]
         L.sbwt_index_destroy.argtypes = [vp]
         L.sbwt_index_destroy.restype = None
         L.sbwt_index_get_info.argtypes = [vp, C.POINTER(IndexInfo)]
@@ -250,6 +251,13 @@ class Index:
         except Exception:
             pass
 
+    def build_kmer_table(self):
+        """Builds the k-mer table (fresh searches read a bucket pair instead of
+        the presearch entry and SBWT steps); False when the index cannot have one."""
+        built = C.c_int()
+        _check(lib().sbwt_index_build_kmer_table(self.h, C.byref(built)))
+        return bool(built.value)
+
     def searcher(self, max_chars, max_reads, move_to_key_kmer=False, stream=None):
         return Searcher(self, max_chars, max_reads, move_to_key_kmer, stream)
 
@@ -320,7 +328,7 @@ class Searcher:
                 "invalid": int(out[3]), "streamed": int(out[4]), "stream_extra_lines": int(out[5]),
                 "stream_fallbacks": int(out[6]), "group_steps": int(out[7]),
                 "absence_checks": int(out[8]), "absence_hits": int(out[9]),
-                "absent_skipped": int(out[10])}
+                "absent_skipped": int(out[10]), "hash_lines": int(out[11])}
 
     @staticmethod
     def resolved(st):
@@ -330,9 +338,10 @@ class Searcher:
 
     @staticmethod
     def lines(st):
-        """128-byte lines a census reads: presearch and node lines."""
+        """128-byte lines a census reads: presearch (or a first k-mer table pair),
+        node lines and further k-mer table pairs."""
         return (st["full_searches"] + st["steps"] + st["streamed"] + st["stream_extra_lines"]
-                + st["walk_steps"] + st["absence_checks"])
+                + st["walk_steps"] + st["absence_checks"] + st.get("hash_lines", 0))
 
     def set_algorithm(self, algorithm):
         """ALGO_STREAMING, ALGO_PER_KMER (the reference's search shape), ALGO_AUTO (default)

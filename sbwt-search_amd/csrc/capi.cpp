@@ -172,6 +172,14 @@ int sbwt_index_load(const char *sbwt_path, const char *colors_path, uint32_t pre
   });
 }
 
+int sbwt_index_build_kmer_table(sbwt_index *index, int *built) {
+  return guarded([&] {
+    require(index, "null argument");
+    const bool ok = index->dev.build_kmer_hash();
+    if (built) *built = ok ? 1 : 0;
+  });
+}
+
 void sbwt_index_destroy(sbwt_index *index) { delete index; }
 
 int sbwt_index_get_info(const sbwt_index *index, sbwt_index_info *info) {

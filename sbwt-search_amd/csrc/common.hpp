@@ -78,6 +78,8 @@ struct IndexView {
   u32 hyper_shift;            // lines per hyperblock = 2^hyper_shift (kHyperShift)
   const void *presearch;      // [4^kp] of {left,right} (uint2 or ulonglong2)
   const void *key_dest;       // [n] key-k-mer move destination (u32 or u64), or null
+  const uint4 *kmer_hash;     // [2^kmer_hash_bits] {k-mer code + 1 (u64), node (u32), 0}, or null
+  u32 kmer_hash_bits;
   u64 c_map[5];
   u64 num_nodes;
   u32 k;
@@ -85,6 +87,16 @@ struct IndexView {
   u32 has_groups;             // slot 4 holds the suffix-group starts
   u32 has_marks;              // slot 5 holds the key-k-mer marks
 };
+
+// k-mer table: every node whose label is a whole k-mer (k <= 32, fewer than
+// 2^32 nodes), by its 2k-bit code (first letter highest), in open addressing
+// with linear probing from the code's multiplicative hash; a lookup reads
+// buckets in aligned pairs (one 32-byte load each) and gives up after
+// kHashPairs pairs (the full search then answers).
+constexpr u32 kHashPairs = 4;
+__host__ __device__ __forceinline__ u64 kmer_hash_slot(u64 code, u32 bits) {
+  return (code * 0x9E3779B97F4A7C15ULL) >> (64 - bits);
+}
 
 // Words of the search census (sbwt_searcher_last_stats).
 constexpr u32 kStatsWords = 16;

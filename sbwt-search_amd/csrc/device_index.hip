@@ -118,6 +118,81 @@ __global__ __launch_bounds__(256) void k_key_dest(IndexView ix, Idx *out, u64 n,
   }
 }
 
+// ---- k-mer table: the non-empty intervals of every prefix length from kp
+// up to k, level by level (each level has at most one entry per node: the
+// intervals of distinct strings of one length are disjoint), then the k-letter
+// intervals (single nodes) inserted by code.
+struct PrefixEnt {
+  u64 code;
+  u32 l, r;
+};
+
+// Appends the lanes' entries where keep is set (one atomic per wave); never
+// past cap (a level holds at most n entries, cap is n + 1).
+__device__ __forceinline__ void wave_append(bool keep, const PrefixEnt &e, PrefixEnt *out, u64 *n_out,
+                                            u64 cap) {
+  const u64 m = __ballot(keep);
+  if (m == 0) return;
+  const u32 lane = threadIdx.x & 63;
+  const int leader = __ffsll(static_cast<long long>(m)) - 1;
+  u64 base = 0;
+  if (static_cast<int>(lane) == leader)
+    base = atomicAdd(reinterpret_cast<unsigned long long *>(n_out), static_cast<u64>(__popcll(m)));
+  base = __shfl(base, leader, 64);
+  const u64 at = base + __popcll(m & ((1ULL << lane) - 1));
+  if (keep && at < cap) out[at] = e;
+}
+
+__global__ __launch_bounds__(256) void k_hash_seed(const uint2 *__restrict__ pre, u64 count,
+                                                   PrefixEnt *out, u64 *n_out, u64 cap) {
+  const u64 stride = static_cast<u64>(gridDim.x) * blockDim.x;
+  for (u64 w = static_cast<u64>(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u); w < count; w += stride) {
+    const u64 x = w + (threadIdx.x & 63);
+    const uint2 e = x < count ? pre[x] : make_uint2(1, 0);
+    wave_append(e.x <= e.y, PrefixEnt{x, e.x, e.y}, out, n_out, cap);
+  }
+}
+
+// One letter more for every entry of the level; at the last level (k
+// letters) the single-node intervals go into the table.
+__global__ __launch_bounds__(256) void k_hash_extend(IndexView ix, const PrefixEnt *__restrict__ in,
+                                                     const u64 *n_in, PrefixEnt *out, u64 *n_out,
+                                                     u64 cap, u32 last, uint4 *table, u32 bits) {
+  const u64 n = min(*n_in, cap);
+  const u64 stride = static_cast<u64>(gridDim.x) * blockDim.x;
+  const u64 mask = (1ULL << bits) - 1;
+  for (u64 w = static_cast<u64>(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u); w < n; w += stride) {
+    const u64 i = w + (threadIdx.x & 63);
+    PrefixEnt e{0, 1, 0};
+    if (i < n) e = in[i];
+#pragma unroll
+    for (u32 c = 0; c < 4; ++c) {
+      u32 l = e.l, r = e.r;
+      const bool live = l <= r;
+      if (live) sbwt_step<u32>(ix, c, l, r);
+      const bool keep = live && l <= r;
+      const PrefixEnt ch{(e.code << 2) | c, l, r};
+      if (!last) {
+        wave_append(keep, ch, out, n_out, cap);
+      } else if (keep) {
+        // a k-letter interval is one node; insert (code + 1 keeps 0 for empty)
+        const u64 key = ch.code + 1;
+        u64 h = kmer_hash_slot(ch.code, bits);
+        for (u64 probe = 0; probe <= mask; ++probe, h = (h + 1) & mask) {
+          unsigned long long *slot = reinterpret_cast<unsigned long long *>(table + h);
+          const unsigned long long prev = atomicCAS(slot, 0ULL, static_cast<unsigned long long>(key));
+          if (prev == 0 || prev == key) {
+            table[h].z = l;
+            break;
+          }
+        }
+      }
+    }
+  }
+}
+
+__global__ void k_zero_u64(u64 *p) { *p = 0; }
+
 template <typename Idx, typename Pair>
 void build_presearch(const IndexView &ix, u32 kp, Pair *out) {
   constexpr u32 kBase = 12;
@@ -297,7 +372,54 @@ void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const
   }
 }
 
+// The k-mer table, on request (narrow indexes with k <= 32 and kp < k): two
+// level lists of n + 1 entries during the build, then 2^bits buckets of 16
+// bytes with 2^bits >= 2 (n + 1), built when it takes at most a sixteenth of
+// device memory. Returns whether the index has one.
+bool DeviceIndex::build_kmer_hash() {
+  if (kmer_hash) return true;
+  if (wide || k > 32 || kp >= k) return false;
+  u32 bits = 1;
+  while ((1ULL << bits) < 2 * (n + 1)) ++bits;
+  const u64 table_bytes = (1ULL << bits) * sizeof(uint4);
+  const u64 list_bytes = (n + 1) * sizeof(PrefixEnt);
+  size_t free_b = 0, total_b = 0;
+  SBWT_HIP(hipMemGetInfo(&free_b, &total_b));
+  if (table_bytes > total_b / 16 || table_bytes + 2 * list_bytes > free_b / 2) return false;
+  PrefixEnt *lists[2] = {nullptr, nullptr};
+  u64 *counts = nullptr;
+  SBWT_HIP(hipMalloc(&lists[0], list_bytes));
+  SBWT_HIP(hipMalloc(&lists[1], list_bytes));
+  SBWT_HIP(hipMalloc(&counts, 2 * sizeof(u64)));
+  SBWT_HIP(hipMalloc(&kmer_hash, table_bytes));
+  SBWT_HIP(hipMemset(kmer_hash, 0, table_bytes));
+  SBWT_HIP(hipMemset(counts, 0, 2 * sizeof(u64)));
+  kmer_hash_bits = bits;
+  const IndexView ix = view();
+  const u64 count = 1ULL << (2 * kp);
+  k_hash_seed<<<static_cast<u32>(std::min<u64>((count + 255) / 256, 1u << 20)), 256>>>(
+      static_cast<const uint2 *>(presearch), count, lists[0], counts, n + 1);
+  SBWT_HIP(hipGetLastError());
+  const u32 grid = static_cast<u32>(std::min<u64>((n + 1 + 255) / 256, 1u << 20));
+  for (u32 len = kp; len < k; ++len) {
+    const u32 from = (len - kp) & 1, to = from ^ 1;
+    k_zero_u64<<<1, 1>>>(counts + to);
+    k_hash_extend<<<grid, 256>>>(ix, lists[from], counts + from, lists[to], counts + to, n + 1,
+                                 len + 1 == k ? 1u : 0u, kmer_hash, bits);
+    SBWT_HIP(hipGetLastError());
+  }
+  SBWT_HIP(hipDeviceSynchronize());
+  (void)hipFree(lists[0]);
+  (void)hipFree(lists[1]);
+  (void)hipFree(counts);
+  device_bytes += table_bytes;
+  return true;
+}
+
 void DeviceIndex::release() {
+  if (kmer_hash) (void)hipFree(kmer_hash);
+  kmer_hash = nullptr;
+  kmer_hash_bits = 0;
   if (blocks) (void)hipFree(blocks);
   if (hyper_base) (void)hipFree(hyper_base);
   if (presearch) (void)hipFree(presearch);
@@ -315,6 +437,8 @@ IndexView DeviceIndex::view() const {
   v.hyper_shift = hyper_shift;
   v.presearch = presearch;
   v.key_dest = key_dest;
+  v.kmer_hash = kmer_hash;
+  v.kmer_hash_bits = kmer_hash_bits;
   v.has_groups = has_groups;
   v.has_marks = has_marks;
   for (int i = 0; i < 5; ++i) v.c_map[i] = c_map[i];

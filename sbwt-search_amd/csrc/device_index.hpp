@@ -37,6 +37,8 @@ struct DeviceIndex {
   u64 presearch_bytes = 0;
   bool has_marks = false;   // key-k-mer marks are in the lines
   void *key_dest = nullptr; // per node: its key-k-mer move destination (u32, or u64 if wide)
+  uint4 *kmer_hash = nullptr;  // k-mer code -> node table (narrow indexes), or null
+  u32 kmer_hash_bits = 0;
   bool has_groups = false;  // suffix-group starts are in the lines
   u64 device_bytes = 0;
 
@@ -47,6 +49,11 @@ struct DeviceIndex {
              u64 key_marks_bits, const u64 *sgs_words);
   void release();
   IndexView view() const;
+  // The k-mer table that lets a fresh search read about one bucket pair
+  // instead of the presearch entry and k - kp steps; built on request (about
+  // a second for 10^8 nodes), used by every later search. False when the
+  // index cannot have one (64-bit nodes, k > 32, not enough memory).
+  bool build_kmer_hash();
 };
 
 // Presearch prefix length used when the caller does not choose one. The

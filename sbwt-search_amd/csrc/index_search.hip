@@ -216,6 +216,38 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_write(const u64 *__restri
 template <typename Idx, bool KEY>
 __device__ __forceinline__ u64 search_window(const IndexView &ix, u64 window, u32 &steps,
                                              u32 &walked) {
+  if (ix.kmer_hash) {  // the k-mer table answers in a pair or two of buckets
+    const u64 code = window >> (64 - 2 * ix.k);
+    const u64 key = code + 1, mask = (1ULL << ix.kmer_hash_bits) - 1;
+    u64 h = kmer_hash_slot(code, ix.kmer_hash_bits);
+    // `steps` counts the pairs read beyond the first, which stands in for the
+    // presearch line the caller counts
+    for (u32 pr = 0; pr < kHashPairs; ++pr) {
+      const uint4 *pair = ix.kmer_hash + (h & ~1ULL);
+      const uint4 b0 = pair[0], b1 = pair[1];
+      if (pr) ++steps;
+      const u64 k0 = static_cast<u64>(b0.x) | (static_cast<u64>(b0.y) << 32);
+      const u64 k1 = static_cast<u64>(b1.x) | (static_cast<u64>(b1.y) << 32);
+      const bool use0 = (h & 1) == 0;  // an odd start skips the pair's first bucket
+      u64 node = kNotFound;
+      bool done = false;
+      if (use0 && (k0 == key || k0 == 0)) {
+        node = k0 == key ? static_cast<u64>(b0.z) : kNotFound;
+        done = true;
+      } else if (k1 == key || k1 == 0) {
+        node = k1 == key ? static_cast<u64>(b1.z) : kNotFound;
+        done = true;
+      }
+      if (done) {
+        if constexpr (KEY) {
+          if (node != kNotFound) return key_move<Idx>(ix, static_cast<Idx>(node), walked);
+        }
+        return node;
+      }
+      h = ((h | 1) + 1) & mask;
+    }
+    ++steps;  // all pairs were extra: the presearch line follows
+  }
   const u64 pre = window >> (64 - 2 * ix.kp);
   Idx l, r;
   if constexpr (sizeof(Idx) == 4) {
@@ -407,7 +439,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     IndexView ix, const u64 *__restrict__ packed, const u32 *__restrict__ invalid,
     const ulonglong2 *__restrict__ chunks, const u64 *__restrict__ chunk_ends, u64 n_reads,
     u64 *__restrict__ counter, u64 *__restrict__ out, u64 *__restrict__ stats, u32 absence) {
-  enum : u32 { kChunk, kSeq0, kSeq1, kStream, kGsEntry, kPresearch, kTail, kStep, kDone };
+  enum : u32 { kChunk, kSeq0, kSeq1, kStream, kGsEntry, kPresearch, kTail, kStep, kHash, kDone };
   enum : u32 { kHintNone, kHintMiss, kHintPre };  // what the last miss left known
   const u64 n_chunks = chunk_ends[n_reads - 1];
   const u32 lane = threadIdx.x & 63;
@@ -445,7 +477,20 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
   const u64 phase = (reinterpret_cast<uintptr_t>(out) >> 3) & 7;  // out's offset in a 64-byte group
   u64 gbeg = 0;  // first staged output slot
   u32 st_full = 0, st_steps = 0, st_walk = 0, st_inv = 0, st_stream = 0, st_extra = 0, st_fb = 0;
-  u32 st_group = 0, st_tail = 0, st_tail_hit = 0, st_skip = 0;
+  u32 st_group = 0, st_tail = 0, st_tail_hit = 0, st_skip = 0, st_hash = 0;
+  const u64 hash_mask = ix.kmer_hash ? (1ULL << ix.kmer_hash_bits) - 1 : 0;
+  // a fresh search of the current window: the k-mer table when there is one
+  // (pre_idx = bucket, j = pairs read), else the presearch entry and steps
+  auto start_search = [&]() {
+    if (ix.kmer_hash) {
+      pre_idx = kmer_hash_slot(window >> (64 - 2 * k), ix.kmer_hash_bits);
+      j = 0;
+      mode = kHash;
+    } else {
+      pre_idx = window >> (64 - 2 * kp);
+      mode = kPresearch;
+    }
+  };
   for (;;) {
     // hand the wave's chunks to lanes that wait for one
     const u64 need = __ballot(want);
@@ -500,6 +545,10 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
       case kStep:
         a1 = e1;
         a2 = line_slot(ix, b2, c);
+        break;
+      case kHash:  // an aligned pair of buckets
+        a1 = ix.kmer_hash + (pre_idx & ~1ULL);
+        a2 = a1 + 1;
         break;
       default:
         break;
@@ -580,8 +629,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
           found = true;
         } else if (!ix.has_groups) {
           if (STATS) ++st_fb;
-          pre_idx = window >> (64 - 2 * kp);
-          mode = kPresearch;  // the group start is unknown: search afresh
+          start_search();  // the group start is unknown: search afresh
         } else {
           // v's group start: the last group-start bit at or before v in this
           // line, else delta nodes before the line
@@ -602,8 +650,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
             }
           } else if (d2.x == kFarGroup) {
             if (STATS) ++st_fb;
-            pre_idx = window >> (64 - 2 * kp);
-            mode = kPresearch;
+            start_search();
           } else {
             gs = static_cast<Idx>(static_cast<u64>(b1) * kBlockNodes - d2.x);
             if (STATS) ++st_group;
@@ -633,8 +680,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
             dead_end = tq + 1;
             notfound = true;
           } else {
-            pre_idx = window >> (64 - 2 * kp);
-            mode = kPresearch;
+            start_search();
           }
           break;
         }
@@ -650,6 +696,44 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         if (STATS) ++st_steps;
         resolve = true;
         break;
+      case kHash: {
+        if (STATS) ++st_hash;
+        const u64 key = (window >> (64 - 2 * k)) + 1;
+        const u64 k0 = static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32);
+        const u64 k1 = static_cast<u64>(d2.x) | (static_cast<u64>(d2.y) << 32);
+        const bool use0 = (pre_idx & 1) == 0;  // an odd start skips the pair's first bucket
+        bool settled = true;
+        if (use0 && k0 == key) {
+          found_at = static_cast<Idx>(d1.z);
+          found = true;
+        } else if (use0 && k0 == 0) {
+          notfound = true;
+        } else if (k1 == key) {
+          found_at = static_cast<Idx>(d2.z);
+          found = true;
+        } else if (k1 == 0) {
+          notfound = true;
+        } else {
+          settled = false;
+          if (++j < kHashPairs) {
+            pre_idx = ((pre_idx | 1) + 1) & hash_mask;
+          } else {  // a long probe run: the full search answers
+            pre_idx = window >> (64 - 2 * kp);
+            mode = kPresearch;
+          }
+        }
+        if (settled) {
+          if (STATS) {
+            ++st_full;
+            --st_hash;  // the first pair stands in for the presearch line of a full search
+          }
+          if (notfound) {  // where the k-mer fails is unknown: check its last kp-mer next
+            hint = kHintMiss;
+            miss_at = p + k - 1;
+          }
+        }
+        break;
+      }
       default:
         break;
     }
@@ -775,7 +859,9 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         // in the presearch prefix the next window's prefix likely holds the
         // same bad char, so it is searched directly.
         u64 t = p;
-        if (absence && hint == kHintNone) t = p + k - kp;
+        // with the k-mer table a search costs about what a check does: only a
+        // known miss is worth checking around
+        if (absence && hint == kHintNone && !ix.kmer_hash) t = p + k - kp;
         if (absence && hint == kHintMiss && miss_at >= p) t = min(miss_at, p + k - kp);
         if (t > p) {
           tq = t;
@@ -785,8 +871,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
           pre_idx = (tsh ? ((ta << tsh) | (tb >> (64 - tsh))) : ta) >> (64 - 2 * kp);
           mode = kTail;
         } else {
-          pre_idx = window >> (64 - 2 * kp);
-          mode = kPresearch;
+          start_search();
         }
       }
     }
@@ -803,6 +888,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     atomicAdd(reinterpret_cast<unsigned long long *>(stats + 8), static_cast<u64>(st_tail));
     atomicAdd(reinterpret_cast<unsigned long long *>(stats + 9), static_cast<u64>(st_tail_hit));
     atomicAdd(reinterpret_cast<unsigned long long *>(stats + 10), static_cast<u64>(st_skip));
+    atomicAdd(reinterpret_cast<unsigned long long *>(stats + 11), static_cast<u64>(st_hash));
   }
 }
 

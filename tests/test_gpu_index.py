@@ -268,6 +268,8 @@ def _key_move_case(d, algo, tmp_path):
     km = col.key_marks()
     idx = S.Index.load(sp, cp)
     assert idx.has_key_marks
+    if algo == S.ALGO_STREAMING:  # fresh searches from the k-mer table (none when wide)
+        assert idx.build_kmer_table() != idx.wide
     chars, ends2, _ = W.reads(5, codes, ends, np.full(2000, 120, np.uint32), 0.2, 0.01, 0.001)
     chars = chars.tobytes()
     s = idx.searcher(len(chars), len(ends2), move_to_key_kmer=True)
@@ -441,3 +443,26 @@ def test_key_kmer_move_across_work_items(algo, launch, tmp_path):
         s.set_launch(*launch)
     res = s.search_reads(chars, rends)
     assert np.array_equal(res, orc.index_reads(chars, rends, key_marks=km))
+
+
+@pytest.mark.parametrize("table", ["1", "0"], ids=["kmer-table", "no-table"])
+@pytest.mark.parametrize("algo", ALGOS + [S.ALGO_STREAMING_ADAPTIVE],
+                         ids=["streaming", "per-kmer", "adaptive"])
+def test_kmer_table_and_full_search_agree(genome_index, algo, table, monkeypatch):
+    """Fresh searches start from the k-mer table (every whole-k-mer node by its
+    code) when the index has one, else from the presearch interval and SBWT
+    steps; both give the oracle's results, with reads that miss often."""
+    g, b, path, orc = genome_index
+    say(f"k-mer table={table} algo={algo}")
+    idx = S.Index.load(path)
+    if table == "1":
+        assert idx.build_kmer_table()
+    chars, ends = _reads(g, 3000, W.uniform_lengths(52, 3000, 20, 300), seed=51, random_frac=0.2,
+                         err=0.02, n_rate=0.005)
+    s = idx.searcher(len(chars), len(ends))
+    s.set_algorithm(algo)
+    s.set_stats(True)
+    res = s.search_reads(chars, ends)
+    assert np.array_equal(res, orc.index_reads(chars, ends))
+    st = s.last_stats()
+    assert S.Searcher.resolved(st) == len(res) or algo == S.ALGO_PER_KMER
