@@ -85,7 +85,7 @@ int sbwt_index_load(const char *sbwt_path, const char *colors_path, uint32_t pre
  * up to 32 bytes per node of device memory, so it pays for an index that
  * serves many batches; the command line does not build it. *built (nullable)
  * receives 0 when the index cannot have one (nodes >= 2^32, k > 32, or the
- * table would take more than a sixteenth of device memory). No counterpart in
+ * table would take more than a quarter of device memory). No counterpart in
  * the reference (its searches always start from the presearch table,
  * src/IndexSearcher/IndexSearcher.cuh:36-55). */
 int sbwt_index_build_kmer_table(sbwt_index *index, int *built);

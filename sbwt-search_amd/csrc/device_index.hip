@@ -374,7 +374,7 @@ void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const
 
 // The k-mer table, on request (narrow indexes with k <= 32 and kp < k): two
 // level lists of n + 1 entries during the build, then 2^bits buckets of 16
-// bytes with 2^bits >= 2 (n + 1), built when it takes at most a sixteenth of
+// bytes with 2^bits >= 2 (n + 1), built when it takes at most a quarter of
 // device memory. Returns whether the index has one.
 bool DeviceIndex::build_kmer_hash() {
   if (kmer_hash) return true;
@@ -385,7 +385,7 @@ bool DeviceIndex::build_kmer_hash() {
   const u64 list_bytes = (n + 1) * sizeof(PrefixEnt);
   size_t free_b = 0, total_b = 0;
   SBWT_HIP(hipMemGetInfo(&free_b, &total_b));
-  if (table_bytes > total_b / 16 || table_bytes + 2 * list_bytes > free_b / 2) return false;
+  if (table_bytes > total_b / 4 || table_bytes + 2 * list_bytes > free_b / 2) return false;
   PrefixEnt *lists[2] = {nullptr, nullptr};
   u64 *counts = nullptr;
   SBWT_HIP(hipMalloc(&lists[0], list_bytes));
