@@ -81,7 +81,7 @@ int sbwt_index_load(const char *sbwt_path, const char *colors_path, uint32_t pre
  * by its 2k-bit code, so that a search that cannot step from the previous
  * k-mer (a read start, a k-mer after a miss) reads about one 32-byte bucket
  * pair instead of the presearch entry and k - presearch_letters SBWT steps.
- * Results do not change. One-time cost (about a second for 10^8 nodes) and
+ * Results do not change. One-time cost (about 0.1 s for 10^8 nodes) and
  * up to 32 bytes per node of device memory, so it pays for an index that
  * serves many batches; the command line does not build it. *built (nullable)
  * receives 0 when the index cannot have one (nodes >= 2^32, k > 32, or the

@@ -127,29 +127,59 @@ struct PrefixEnt {
   u32 l, r;
 };
 
-// Appends the lanes' entries where keep is set (one atomic per wave); never
-// past cap (a level holds at most n entries, cap is n + 1).
-__device__ __forceinline__ void wave_append(bool keep, const PrefixEnt &e, PrefixEnt *out, u64 *n_out,
-                                            u64 cap) {
-  const u64 m = __ballot(keep);
-  if (m == 0) return;
-  const u32 lane = threadIdx.x & 63;
-  const int leader = __ffsll(static_cast<long long>(m)) - 1;
-  u64 base = 0;
-  if (static_cast<int>(lane) == leader)
-    base = atomicAdd(reinterpret_cast<unsigned long long *>(n_out), static_cast<u64>(__popcll(m)));
-  base = __shfl(base, leader, 64);
-  const u64 at = base + __popcll(m & ((1ULL << lane) - 1));
-  if (keep && at < cap) out[at] = e;
+// Reserves cnt output slots for each thread of the workgroup, in thread
+// order, with one atomic per workgroup (a single counter takes ~10^8 atomics
+// per second, so per-wave appends would bound the build). Every thread of the
+// workgroup calls it.
+__device__ __forceinline__ u64 block_reserve(u32 cnt, u64 *n_out) {
+  __shared__ u32 wave_sums[4];
+  __shared__ u64 block_base;
+  const u32 lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  u32 incl = cnt;
+#pragma unroll
+  for (u32 d = 1; d < 64; d <<= 1) {
+    const u32 y = __shfl_up(incl, d, 64);
+    if (lane >= d) incl += y;
+  }
+  if (lane == 63) wave_sums[wave] = incl;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const u32 total = wave_sums[0] + wave_sums[1] + wave_sums[2] + wave_sums[3];
+    block_base = total ? atomicAdd(reinterpret_cast<unsigned long long *>(n_out), static_cast<u64>(total)) : 0;
+  }
+  __syncthreads();
+  u32 before = 0;
+  for (u32 w = 0; w < wave; ++w) before += wave_sums[w];
+  const u64 base = block_base + before + incl - cnt;
+  __syncthreads();  // the shared words are reused by the next call
+  return base;
 }
 
+constexpr u32 kSeedItems = 16;  // presearch entries per thread
+
+// The non-empty presearch intervals (cap: a level holds at most n entries,
+// cap is n + 1; nothing is written past it).
 __global__ __launch_bounds__(256) void k_hash_seed(const uint2 *__restrict__ pre, u64 count,
                                                    PrefixEnt *out, u64 *n_out, u64 cap) {
-  const u64 stride = static_cast<u64>(gridDim.x) * blockDim.x;
-  for (u64 w = static_cast<u64>(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u); w < count; w += stride) {
-    const u64 x = w + (threadIdx.x & 63);
-    const uint2 e = x < count ? pre[x] : make_uint2(1, 0);
-    wave_append(e.x <= e.y, PrefixEnt{x, e.x, e.y}, out, n_out, cap);
+  const u64 tile = 256ULL * kSeedItems;
+  for (u64 b = static_cast<u64>(blockIdx.x) * tile; b < count; b += static_cast<u64>(gridDim.x) * tile) {
+    const u64 x0 = b + threadIdx.x * kSeedItems;
+    u32 keep = 0;
+    for (u32 q = 0; q < kSeedItems; ++q)
+      if (x0 + q < count) {
+        const uint2 e = pre[x0 + q];
+        keep += e.x <= e.y ? 1u : 0u;
+      }
+    u64 at = block_reserve(keep, n_out);
+    for (u32 q = 0; q < kSeedItems && keep; ++q)
+      if (x0 + q < count) {
+        const uint2 e = pre[x0 + q];
+        if (e.x <= e.y) {
+          if (at < cap) out[at] = PrefixEnt{x0 + q, e.x, e.y};
+          ++at;
+          --keep;
+        }
+      }
   }
 }
 
@@ -159,30 +189,40 @@ __global__ __launch_bounds__(256) void k_hash_extend(IndexView ix, const PrefixE
                                                      const u64 *n_in, PrefixEnt *out, u64 *n_out,
                                                      u64 cap, u32 last, uint4 *table, u32 bits) {
   const u64 n = min(*n_in, cap);
-  const u64 stride = static_cast<u64>(gridDim.x) * blockDim.x;
   const u64 mask = (1ULL << bits) - 1;
-  for (u64 w = static_cast<u64>(blockIdx.x) * blockDim.x + (threadIdx.x & ~63u); w < n; w += stride) {
-    const u64 i = w + (threadIdx.x & 63);
+  for (u64 b = static_cast<u64>(blockIdx.x) * 256; b < n; b += static_cast<u64>(gridDim.x) * 256) {
+    const u64 i = b + threadIdx.x;
     PrefixEnt e{0, 1, 0};
     if (i < n) e = in[i];
+    PrefixEnt ch[4];
+    u32 keep = 0;
 #pragma unroll
     for (u32 c = 0; c < 4; ++c) {
       u32 l = e.l, r = e.r;
-      const bool live = l <= r;
-      if (live) sbwt_step<u32>(ix, c, l, r);
-      const bool keep = live && l <= r;
-      const PrefixEnt ch{(e.code << 2) | c, l, r};
-      if (!last) {
-        wave_append(keep, ch, out, n_out, cap);
-      } else if (keep) {
+      if (l <= r) sbwt_step<u32>(ix, c, l, r);
+      ch[c] = PrefixEnt{(e.code << 2) | c, l, r};
+      keep |= (e.l <= e.r && l <= r ? 1u : 0u) << c;
+    }
+    if (!last) {
+      u64 at = block_reserve(static_cast<u32>(__builtin_popcount(keep)), n_out);
+#pragma unroll
+      for (u32 c = 0; c < 4; ++c)
+        if ((keep >> c) & 1) {
+          if (at < cap) out[at] = ch[c];
+          ++at;
+        }
+    } else {
+#pragma unroll
+      for (u32 c = 0; c < 4; ++c) {
+        if (!((keep >> c) & 1)) continue;
         // a k-letter interval is one node; insert (code + 1 keeps 0 for empty)
-        const u64 key = ch.code + 1;
-        u64 h = kmer_hash_slot(ch.code, bits);
+        const u64 key = ch[c].code + 1;
+        u64 h = kmer_hash_slot(ch[c].code, bits);
         for (u64 probe = 0; probe <= mask; ++probe, h = (h + 1) & mask) {
           unsigned long long *slot = reinterpret_cast<unsigned long long *>(table + h);
           const unsigned long long prev = atomicCAS(slot, 0ULL, static_cast<unsigned long long>(key));
           if (prev == 0 || prev == key) {
-            table[h].z = l;
+            table[h].z = ch[c].l;
             break;
           }
         }
@@ -397,7 +437,7 @@ bool DeviceIndex::build_kmer_hash() {
   kmer_hash_bits = bits;
   const IndexView ix = view();
   const u64 count = 1ULL << (2 * kp);
-  k_hash_seed<<<static_cast<u32>(std::min<u64>((count + 255) / 256, 1u << 20)), 256>>>(
+  k_hash_seed<<<static_cast<u32>(std::min<u64>((count + 256 * kSeedItems - 1) / (256 * kSeedItems), 1u << 20)), 256>>>(
       static_cast<const uint2 *>(presearch), count, lists[0], counts, n + 1);
   SBWT_HIP(hipGetLastError());
   const u32 grid = static_cast<u32>(std::min<u64>((n + 1 + 255) / 256, 1u << 20));

@@ -51,7 +51,7 @@ struct DeviceIndex {
   IndexView view() const;
   // The k-mer table that lets a fresh search read about one bucket pair
   // instead of the presearch entry and k - kp steps; built on request (about
-  // a second for 10^8 nodes), used by every later search. False when the
+  // 0.1 s for 10^8 nodes), used by every later search. False when the
   // index cannot have one (64-bit nodes, k > 32, not enough memory).
   bool build_kmer_hash();
 };
