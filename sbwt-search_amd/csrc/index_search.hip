@@ -1185,7 +1185,8 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
   launch_pack(d_chars, n_chars, d_packed_, d_invalid_, stream_);
   if (timing_) SBWT_HIP(hipEventRecord(ev_[1], stream_));
   const bool streaming = algorithm_ == kAlgoStreaming ||
-                         (algorithm_ == kAlgoAuto && n_chars >= kAutoStreamingChars) ||
+                         (algorithm_ == kAlgoAuto &&
+                          n_chars >= (ix.kmer_hash ? kAutoStreamingCharsTable : kAutoStreamingChars)) ||
                          (algorithm_ == kAlgoStreamingAdaptive && n_chars >= kAdaptiveStreamingChars);
   // work items of the streaming search: (read, up to `chunk` k-mers)
   const u32 chunk = streaming ? stream_chunk(n_chars) : 1;
