@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstring>
 #include <stdexcept>
+#include <thread>
 
 #include "byte_automaton.hpp"
 #include "host_scan.hpp"
@@ -297,9 +298,35 @@ u64 IndexFileParser::batch_cut(int format, const char *body, u64 n, u64 max_read
   // A read end is at most one per '\n' byte: stop at the max_reads-th one,
   // then step back to a '\n' that is not the char consumed after a '-'.
   u64 limit = n;
-  if (n > max_reads) {
-    u64 seen = 0;
-    for (u64 p = 0; p < n;) {
+  // A large batch is counted by four threads first: the scan below then
+  // starts in the quarter that holds the max_reads-th '\n', or is skipped when
+  // the batch has fewer.
+  constexpr u64 kParallelBytes = 8ULL << 20;
+  bool capped = n > max_reads;
+  u64 start = 0, seen = 0;
+  if (capped && n >= kParallelBytes) {
+    constexpr u32 kParts = 4;
+    u64 part_counts[kParts] = {};
+    std::thread th[kParts];
+    const u64 step = (n + kParts - 1) / kParts;
+    for (u32 t = 0; t < kParts; ++t)
+      th[t] = std::thread([&, t] {
+        const u64 b = std::min(n, t * step), e = std::min(n, b + step);
+        part_counts[t] = count_byte(body + b, e - b, '\n');
+      });
+    for (u32 t = 0; t < kParts; ++t) th[t].join();
+    capped = false;
+    for (u32 t = 0; t < kParts; ++t) {
+      if (seen + part_counts[t] >= max_reads) {
+        capped = true;
+        start = std::min(n, t * step);
+        break;
+      }
+      seen += part_counts[t];
+    }
+  }
+  if (capped) {
+    for (u64 p = start; p < n;) {
       const u64 span = std::min<u64>(n - p, 1 << 16);
       const u64 c = count_byte(body + p, span, '\n');
       if (seen + c < max_reads) {

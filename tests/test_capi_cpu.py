@@ -132,3 +132,27 @@ def test_index_file_batch_cut_by_brute_force(seed):
         assert S.index_file_batch_cut(1, raw, max_reads) == want, max_reads
     assert S.index_file_batch_cut(0, b"12 -\n", 5) == 0
     assert S.index_file_batch_cut(0, b"12\n-\n", 5) == 3
+
+
+def test_index_file_batch_cut_large_body():
+    """Bodies of 8 MiB and more are counted in four parts first; the cut is the
+    same rule: the max_reads-th '\\n' is searched for in every quarter, at the
+    quarter boundaries and past the last one."""
+    import numpy as np
+    rng = np.random.default_rng(11)
+    n = (12 << 20) + 5
+    alphabet = np.frombuffer(b"0123456789 -\n", np.uint8)
+    arr = alphabet[rng.integers(0, len(alphabet), n)]
+    body = arr.tobytes()
+    nls = np.flatnonzero(arr == 10)
+    ok = nls[(nls < 1) | (arr[np.maximum(nls - 1, 0)] != ord("-"))]
+    step = (n + 3) // 4
+    probes = {1, 2, len(nls) - 1, len(nls), len(nls) + 1, 4 * n}
+    for t in range(1, 4):
+        before = int(np.searchsorted(nls, t * step))  # '\n' bytes in the first t quarters
+        probes |= {before - 1, before, before + 1}
+    for max_reads in sorted(p for p in probes if p >= 1):
+        limit = int(nls[max_reads - 1]) + 1 if max_reads <= len(nls) else n
+        below = ok[ok + 1 <= limit]
+        want = int(below[-1]) + 1 if len(below) else 0
+        assert S.index_file_batch_cut(0, body, max_reads) == want, max_reads
