@@ -260,16 +260,54 @@ u64 IndexFileParser::parse_host(int format, const char *body, u64 n_bytes, u64 *
   return parse_device(format, d_body_, n_bytes, n_results);
 }
 
+namespace {
+
+// Start of the scan for the max_reads-th read end among n units (bytes or
+// words): a batch of at least kParallelUnits units is counted by four threads
+// first (count(b, e) = read ends in units [b, e)). Returns false when the
+// batch holds fewer than max_reads read ends, so that the cap cannot bind;
+// otherwise *start is the first unit of the quarter holding the max_reads-th
+// one and *seen the read ends before it (both 0 for a small batch).
+constexpr u64 kParallelUnits = 1ULL << 20;
+
+template <class Count>
+bool cap_scan_start(u64 n, u64 max_reads, Count count, u64 *start, u64 *seen) {
+  *start = 0;
+  *seen = 0;
+  if (n <= max_reads) return false;
+  if (n < kParallelUnits) return true;
+  constexpr u32 kParts = 4;
+  u64 part_counts[kParts] = {};
+  std::thread th[kParts];
+  const u64 step = (n + kParts - 1) / kParts;
+  for (u32 t = 0; t < kParts; ++t)
+    th[t] = std::thread([&, t] {
+      const u64 b = std::min(n, t * step);
+      part_counts[t] = count(b, std::min(n, b + step));
+    });
+  for (u32 t = 0; t < kParts; ++t) th[t].join();
+  for (u32 t = 0; t < kParts; ++t) {
+    if (*seen + part_counts[t] >= max_reads) {
+      *start = std::min(n, t * step);
+      return true;
+    }
+    *seen += part_counts[t];
+  }
+  return false;
+}
+
+}  // namespace
+
 u64 IndexFileParser::batch_cut(int format, const char *body, u64 n, u64 max_reads) {
   if (max_reads == 0) return 0;
   if (format == kIndexFileBinary) {
     // read ends are u64 max - 2 words: stop after the max_reads-th one, then
     // step back to the last one
     const u64 words = n / 8;
-    u64 limit = words;
-    if (words > max_reads) {
-      u64 seen = 0;
-      for (u64 w = 0; w < words;) {
+    u64 limit = words, start, seen;
+    const auto count = [&](u64 b, u64 e) { return count_word(body + 8 * b, e - b, kNewline); };
+    if (cap_scan_start(words, max_reads, count, &start, &seen)) {
+      for (u64 w = start; w < words;) {
         const u64 span = std::min<u64>(words - w, 1 << 16);
         const u64 c = count_word(body + 8 * w, span, kNewline);
         if (seen + c < max_reads) {
@@ -298,34 +336,9 @@ u64 IndexFileParser::batch_cut(int format, const char *body, u64 n, u64 max_read
   // A read end is at most one per '\n' byte: stop at the max_reads-th one,
   // then step back to a '\n' that is not the char consumed after a '-'.
   u64 limit = n;
-  // A large batch is counted by four threads first: the scan below then
-  // starts in the quarter that holds the max_reads-th '\n', or is skipped when
-  // the batch has fewer.
-  constexpr u64 kParallelBytes = 8ULL << 20;
-  bool capped = n > max_reads;
-  u64 start = 0, seen = 0;
-  if (capped && n >= kParallelBytes) {
-    constexpr u32 kParts = 4;
-    u64 part_counts[kParts] = {};
-    std::thread th[kParts];
-    const u64 step = (n + kParts - 1) / kParts;
-    for (u32 t = 0; t < kParts; ++t)
-      th[t] = std::thread([&, t] {
-        const u64 b = std::min(n, t * step), e = std::min(n, b + step);
-        part_counts[t] = count_byte(body + b, e - b, '\n');
-      });
-    for (u32 t = 0; t < kParts; ++t) th[t].join();
-    capped = false;
-    for (u32 t = 0; t < kParts; ++t) {
-      if (seen + part_counts[t] >= max_reads) {
-        capped = true;
-        start = std::min(n, t * step);
-        break;
-      }
-      seen += part_counts[t];
-    }
-  }
-  if (capped) {
+  u64 start, seen;
+  const auto count = [&](u64 b, u64 e) { return count_byte(body + b, e - b, '\n'); };
+  if (cap_scan_start(n, max_reads, count, &start, &seen)) {
     for (u64 p = start; p < n;) {
       const u64 span = std::min<u64>(n - p, 1 << 16);
       const u64 c = count_byte(body + p, span, '\n');

@@ -156,3 +156,14 @@ def test_index_file_batch_cut_large_body():
         below = ok[ok + 1 <= limit]
         want = int(below[-1]) + 1 if len(below) else 0
         assert S.index_file_batch_cut(0, body, max_reads) == want, max_reads
+    words = rng.integers(0, 6, (1 << 21) + 3).astype(np.uint64)
+    words[words == 5] = np.uint64(2**64 - 3)
+    ends = np.flatnonzero(words == np.uint64(2**64 - 3))
+    wstep = (len(words) + 3) // 4
+    probes = {1, len(ends) - 1, len(ends), len(ends) + 1}
+    for t in range(1, 4):
+        before = int(np.searchsorted(ends, t * wstep))
+        probes |= {before - 1, before, before + 1}
+    for max_reads in sorted(probes):
+        want = 8 * (int(ends[min(max_reads, len(ends)) - 1]) + 1)
+        assert S.index_file_batch_cut(1, words.tobytes(), max_reads) == want, max_reads
