@@ -263,11 +263,12 @@ u64 IndexFileParser::parse_host(int format, const char *body, u64 n_bytes, u64 *
 namespace {
 
 // Start of the scan for the max_reads-th read end among n units (bytes or
-// words): a batch of at least kParallelUnits units is counted by four threads
-// first (count(b, e) = read ends in units [b, e)). Returns false when the
-// batch holds fewer than max_reads read ends, so that the cap cannot bind;
-// otherwise *start is the first unit of the quarter holding the max_reads-th
-// one and *seen the read ends before it (both 0 for a small batch).
+// words): a batch of at least kParallelUnits units is counted first in 64
+// parts by four threads (count(b, e) = read ends in units [b, e)). Returns
+// false when the batch holds fewer than max_reads read ends, so that the cap
+// cannot bind; otherwise *start is the first unit of the part holding the
+// max_reads-th one and *seen the read ends before it (both 0 for a small
+// batch).
 constexpr u64 kParallelUnits = 1ULL << 20;
 
 template <class Count>
@@ -276,16 +277,18 @@ bool cap_scan_start(u64 n, u64 max_reads, Count count, u64 *start, u64 *seen) {
   *seen = 0;
   if (n <= max_reads) return false;
   if (n < kParallelUnits) return true;
-  constexpr u32 kParts = 4;
+  constexpr u32 kThreads = 4, kParts = 64, kPartsPerThread = kParts / kThreads;
   u64 part_counts[kParts] = {};
-  std::thread th[kParts];
+  std::thread th[kThreads];
   const u64 step = (n + kParts - 1) / kParts;
-  for (u32 t = 0; t < kParts; ++t)
+  for (u32 t = 0; t < kThreads; ++t)
     th[t] = std::thread([&, t] {
-      const u64 b = std::min(n, t * step);
-      part_counts[t] = count(b, std::min(n, b + step));
+      for (u32 q = t * kPartsPerThread; q < (t + 1) * kPartsPerThread; ++q) {
+        const u64 b = std::min(n, q * step);
+        part_counts[q] = count(b, std::min(n, b + step));
+      }
     });
-  for (u32 t = 0; t < kParts; ++t) th[t].join();
+  for (u32 t = 0; t < kThreads; ++t) th[t].join();
   for (u32 t = 0; t < kParts; ++t) {
     if (*seen + part_counts[t] >= max_reads) {
       *start = std::min(n, t * step);

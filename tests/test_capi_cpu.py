@@ -135,9 +135,10 @@ def test_index_file_batch_cut_by_brute_force(seed):
 
 
 def test_index_file_batch_cut_large_body():
-    """Bodies of 8 MiB and more are counted in four parts first; the cut is the
-    same rule: the max_reads-th '\\n' is searched for in every quarter, at the
-    quarter boundaries and past the last one."""
+    """Batches of 1 M units and more are counted in 64 parts first; the cut is
+    the same rule: the max_reads-th read end is searched for around part
+    boundaries (one thread's first and last part among them) and past the
+    last one."""
     import numpy as np
     rng = np.random.default_rng(11)
     n = (12 << 20) + 5
@@ -146,10 +147,10 @@ def test_index_file_batch_cut_large_body():
     body = arr.tobytes()
     nls = np.flatnonzero(arr == 10)
     ok = nls[(nls < 1) | (arr[np.maximum(nls - 1, 0)] != ord("-"))]
-    step = (n + 3) // 4
+    step = (n + 63) // 64
     probes = {1, 2, len(nls) - 1, len(nls), len(nls) + 1, 4 * n}
-    for t in range(1, 4):
-        before = int(np.searchsorted(nls, t * step))  # '\n' bytes in the first t quarters
+    for t in (1, 15, 16, 17, 40, 63):
+        before = int(np.searchsorted(nls, t * step))  # '\n' bytes in the first t parts
         probes |= {before - 1, before, before + 1}
     for max_reads in sorted(p for p in probes if p >= 1):
         limit = int(nls[max_reads - 1]) + 1 if max_reads <= len(nls) else n
@@ -159,9 +160,9 @@ def test_index_file_batch_cut_large_body():
     words = rng.integers(0, 6, (1 << 21) + 3).astype(np.uint64)
     words[words == 5] = np.uint64(2**64 - 3)
     ends = np.flatnonzero(words == np.uint64(2**64 - 3))
-    wstep = (len(words) + 3) // 4
+    wstep = (len(words) + 63) // 64
     probes = {1, len(ends) - 1, len(ends), len(ends) + 1}
-    for t in range(1, 4):
+    for t in (1, 16, 31, 63):
         before = int(np.searchsorted(ends, t * wstep))
         probes |= {before - 1, before, before + 1}
     for max_reads in sorted(probes):
