@@ -77,6 +77,19 @@ int sbwt_index_create(const uint64_t *const acgt[4], uint64_t num_bits, uint32_t
 int sbwt_index_load(const char *sbwt_path, const char *colors_path, uint32_t presearch_letters,
                     sbwt_index **out);
 
+/* 1 if suffix_group_starts fits the four ACGT vectors of an SBWT of num_bits
+ * nodes, else 0: every node not marked as a group start has an empty row (in
+ * an SBWT only the first node of a suffix group carries the group's edges,
+ * which the reference's interval step C[c] + rank_c relies on,
+ * src/IndexSearcher/IndexSearcher.cuh:48-55). sbwt_index_create and
+ * sbwt_index_load run this check; when it fails the rows are not an SBWT's
+ * and every search of that index runs per k-mer (the reference's interval
+ * search on the file's rows), whatever algorithm a searcher asks for. Host
+ * only, no device work. No counterpart in the reference, which never reads
+ * the vector. */
+int sbwt_suffix_groups_consistent(const uint64_t *const acgt[4], const uint64_t *suffix_group_starts,
+                                  uint64_t num_bits);
+
 /* Builds the index's k-mer table: every node whose label is a whole k-mer,
  * by its 2k-bit code, so that a search that cannot step from the previous
  * k-mer (a read start, a k-mer after a miss) reads about one 32-byte bucket
@@ -100,6 +113,8 @@ typedef struct {
   uint64_t device_bytes;
   int has_key_kmer_marks;
   int wide;                   /* node indexes need 64 bits */
+  int has_suffix_groups;      /* the streaming search steps through group starts */
+  int suffix_groups_rejected; /* starts were given but failed sbwt_suffix_groups_consistent */
 } sbwt_index_info;
 int sbwt_index_get_info(const sbwt_index *index, sbwt_index_info *info);
 
@@ -406,6 +421,14 @@ int sbwt_index_file_parser_parse(sbwt_index_file_parser *parser, int format, con
  * (ContinuousColorResultsPrinter.hpp:175-185). */
 uint64_t sbwt_index_file_batch_cut(int format, const char *body, uint64_t n_bytes,
                                    uint64_t max_reads);
+/* Longest prefix of body[0, n_bytes) (starting at a read start) that ends
+ * right after the read end of a read with at least one found index; 0 if
+ * none. Cutting a batch there keeps each read without a found index in one
+ * batch with the next read that has one: with --include-not-found or
+ * --include-invalid the reference prints such a read with that read's counts
+ * (colored_seq_id, src/IndexFileParser/IndexFileParser.cpp:65-74,
+ * ContinuousColorResultsPrinter.hpp:201-209). Host only, no device work. */
+uint64_t sbwt_index_file_colored_cut(int format, const char *body, uint64_t n_bytes);
 
 /* Sequence-file chunks parsed on the device, the input side of the index
  * command: raw FASTA bytes that start at a record's header line and end at a

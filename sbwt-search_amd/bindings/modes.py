@@ -1,10 +1,9 @@
 """Measurement modes beyond the headline index search (used by bench.py --mode ...).
 
-batched: the uneven-length workload (C4) through the plain form -- one launch per
-  batch of the per-k-mer search, then the color search, batch after batch -- and the
-  batched form -- the streaming persistent-lane search over a device work queue of
-  (read, k-mer range) chunks plus the color search, with batches alternating over two
-  streams so one batch's tail overlaps the next one's start. Outputs are checked equal.
+batched: the uneven-length workload (C4) through the plain form -- one launch of the
+  streaming search and of the color stage per batch, batch after batch on one stream --
+  and the batched form -- the same kernels with batches alternating over two streams so
+  one batch's tail overlaps the next one's start. Outputs are compared in full.
 colors: the color stage alone on a colored workload (C3 recipe), reads/s.
 serve: a Poisson stream of requests (N log-uniform in [1, max] reads of U[75, 250] bp)
   through the serving loop vs plain sequential processing; p50/p99 latency, reads/s.
@@ -29,7 +28,18 @@ def _split_batches(read_ends, max_chars):
     return out
 
 
-def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print):
+def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect=False):
+    """C4: the same streaming search + color stage in two forms, on the same batches.
+
+    plain: one launch of each stage per batch on one stream, batch after batch,
+      each batch's color ids copied back before the next starts (the reference's
+      one-launch-per-batch form, with this library's kernels).
+    batched: the batches spread over two streams, so one batch's tail overlaps
+      the next one's start.
+
+    Both forms run the same algorithm (streaming search with key-k-mer chains,
+    then the color stage); their outputs (per batch: CSR offsets and color ids)
+    are compared in full. With collect, the outputs are returned too."""
     dev = torch.device("cuda:0")
     d_chars = torch.from_numpy(wl.chars).to(dev)
     ends64 = wl.read_ends.astype(np.int64)
@@ -44,10 +54,10 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print):
     nc = col.num_colors
     params = S.color_params(tau)
 
-    def make_lane(algo):
+    def make_lane():
         stream = torch.cuda.Stream()
         s = idx.searcher(max_chars, max_reads, move_to_key_kmer=True, stream=stream.cuda_stream)
-        s.set_algorithm(algo)
+        s.set_algorithm(S.ALGO_STREAMING)
         cs = col.searcher(max_chars, max_reads, stream=stream.cuda_stream)
         res = torch.empty(max_chars, dtype=torch.int64, device=dev)
         off = torch.empty(max_reads + 1, dtype=torch.int64, device=dev)
@@ -61,38 +71,35 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print):
         r0, r1 = batches[b]
         lane["s"].search_reads_device(d_chars.data_ptr() + c0, n, d_ends.data_ptr(), r1 - r0,
                                       lane["res"].data_ptr())
-        return lane["cs"].pseudoalign_device(lane["res"].data_ptr(), lane["s"].kmer_ends_device(),
-                                             r1 - r0, params, lane["off"].data_ptr(),
-                                             lane["ids"].data_ptr(), lane["cap"])
+        n_ids = lane["cs"].pseudoalign_device(lane["res"].data_ptr(),
+                                              lane["s"].kmer_ends_device(), r1 - r0, params,
+                                              lane["off"].data_ptr(), lane["ids"].data_ptr(),
+                                              lane["cap"])
+        # the batch's output back on the host (the same copy in both forms)
+        with torch.cuda.stream(lane["stream"]):
+            off = lane["off"][:r1 - r0 + 1].to("cpu", non_blocking=False)
+            ids = lane["ids"][:n_ids].to("cpu", non_blocking=False)
+        return off.numpy().view(np.uint64), ids.numpy().view(np.uint32)
 
-    def checksum(lane, b, n_ids):
-        ids = lane["ids"][:n_ids].to(torch.int64)
-        return (int(n_ids), int(ids.sum().item()) if n_ids else 0)
-
-    # plain: one launch per batch, per-k-mer search, batch after batch
-    plain = make_lane(S.ALGO_PER_KMER)
-    sums_plain = []
+    # plain: one stream, batch after batch
+    plain = make_lane()
+    out_plain = [None] * len(batches)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for b in range(len(batches)):
-        n_ids = run_batch(plain, b)
-        plain["stream"].synchronize()
-        sums_plain.append(checksum(plain, b, n_ids))
+        out_plain[b] = run_batch(plain, b)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
-    log(f"plain: {len(batches)} batches in {t_plain:.2f} s")
+    log(f"plain: {len(batches)} batches in {t_plain:.3f} s")
 
-    # batched: streaming persistent search, two streams in flight
-    lanes = [make_lane(S.ALGO_STREAMING), make_lane(S.ALGO_STREAMING)]
+    # batched: two streams in flight
+    lanes = [make_lane(), make_lane()]
     import threading
-    sums_b = [None] * len(batches)
+    out_b = [None] * len(batches)
 
     def worker(li):
-        lane = lanes[li]
         for b in range(li, len(batches), 2):
-            n_ids = run_batch(lane, b)
-            lane["stream"].synchronize()
-            sums_b[b] = checksum(lane, b, n_ids)
+            out_b[b] = run_batch(lanes[li], b)
 
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -103,14 +110,21 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print):
         t.join()
     torch.cuda.synchronize()
     t_batched = time.perf_counter() - t0
-    log(f"batched: {len(batches)} batches in {t_batched:.2f} s")
-    if sums_b != sums_plain:
-        raise SystemExit("batched and plain color outputs differ")
-    return {"reads": wl.n_reads, "batches": len(batches), "batch_chars": batch_chars,
-            "plain_s": t_plain, "batched_s": t_batched,
-            "plain_reads_per_s": wl.n_reads / t_plain,
-            "batched_reads_per_s": wl.n_reads / t_batched,
-            "speedup": t_plain / t_batched}
+    log(f"batched: {len(batches)} batches in {t_batched:.3f} s")
+    for b in range(len(batches)):
+        if not (np.array_equal(out_plain[b][0], out_b[b][0])
+                and np.array_equal(out_plain[b][1], out_b[b][1])):
+            raise SystemExit(f"batched and plain color outputs differ in batch {b}")
+    n_ids = sum(len(o[1]) for o in out_plain)
+    r = {"reads": wl.n_reads, "batches": len(batches), "batch_chars": batch_chars,
+         "algorithm": "streaming (both forms)", "color_ids": n_ids,
+         "plain_s": t_plain, "batched_s": t_batched,
+         "plain_reads_per_s": wl.n_reads / t_plain,
+         "batched_reads_per_s": wl.n_reads / t_batched,
+         "speedup": t_plain / t_batched}
+    if collect:
+        r["outputs"] = {"batches": batches, "plain": out_plain, "batched": out_b}
+    return r
 
 
 def run_serve(S, idx, col, wl_reads, rate_per_s, n_requests, max_reads, seed, params,

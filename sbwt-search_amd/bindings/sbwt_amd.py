@@ -29,7 +29,8 @@ u32p = C.POINTER(C.c_uint32)
 class IndexInfo(C.Structure):
     _fields_ = [("num_nodes", C.c_uint64), ("kmer_size", C.c_uint32),
                 ("presearch_letters", C.c_uint32), ("c_map", C.c_uint64 * 5),
-                ("device_bytes", C.c_uint64), ("has_key_kmer_marks", C.c_int), ("wide", C.c_int)]
+                ("device_bytes", C.c_uint64), ("has_key_kmer_marks", C.c_int), ("wide", C.c_int),
+                ("has_suffix_groups", C.c_int), ("suffix_groups_rejected", C.c_int)]
 
 
 class ColorIndexInfo(C.Structure):
@@ -79,6 +80,7 @@ def lib():
                                         C.c_uint32, C.POINTER(vp)]
         L.sbwt_index_load.argtypes = [C.c_char_p, C.c_char_p, C.c_uint32, C.POINTER(vp)]
         L.sbwt_index_build_kmer_table.argtypes = [vp, C.POINTER(C.c_int)]
+        L.sbwt_suffix_groups_consistent.argtypes = [u64p * 4, u64p, C.c_uint64]
         L.sbwt_index_destroy.argtypes = [vp]
         L.sbwt_index_destroy.restype = None
         L.sbwt_index_get_info.argtypes = [vp, C.POINTER(IndexInfo)]
@@ -126,6 +128,8 @@ def lib():
                                                    C.c_uint64, u64p, C.c_uint64, u64p, u64p]
         L.sbwt_index_file_batch_cut.argtypes = [C.c_int, C.c_char_p, C.c_uint64, C.c_uint64]
         L.sbwt_index_file_batch_cut.restype = C.c_uint64
+        L.sbwt_index_file_colored_cut.argtypes = [C.c_int, C.c_char_p, C.c_uint64]
+        L.sbwt_index_file_colored_cut.restype = C.c_uint64
         L.sbwt_searcher_search_reads_format.argtypes = [vp, vp, C.c_int, C.c_char_p, C.c_uint64,
                                                         u64p, C.c_char_p, C.c_uint64, C.c_int,
                                                         C.c_char_p, C.c_uint64, u64p]
@@ -203,6 +207,18 @@ def mem_info():
     return f.value, t.value
 
 
+def suffix_groups_consistent(acgt, suffix_group_starts, num_bits):
+    """Host-only check that every node outside the suffix-group starts has an
+    empty row (sbwt_suffix_groups_consistent); no device work."""
+    keep = [np.ascontiguousarray(a, np.uint64) for a in acgt]
+    arr = (u64p * 4)(*[_p64(a) for a in keep])
+    sg = np.ascontiguousarray(suffix_group_starts, np.uint64)
+    rc = lib().sbwt_suffix_groups_consistent(arr, _p64(sg), num_bits)
+    if rc < 0:
+        raise SbwtError(lib().sbwt_last_error().decode())
+    return bool(rc)
+
+
 class Index:
     """Device-resident SBWT index (rank blocks + presearch table + key marks)."""
 
@@ -217,6 +233,8 @@ class Index:
         self.device_bytes = info.device_bytes
         self.has_key_marks = bool(info.has_key_kmer_marks)
         self.wide = bool(info.wide)
+        self.has_suffix_groups = bool(info.has_suffix_groups)
+        self.suffix_groups_rejected = bool(info.suffix_groups_rejected)
 
     @classmethod
     def load(cls, sbwt_path, colors_path=None, presearch_letters=0):
@@ -701,6 +719,11 @@ COLOR_FORMAT_ASCII, COLOR_FORMAT_BINARY, COLOR_FORMAT_CSV = 0, 1, 2
 def index_file_batch_cut(fmt, body, max_reads):
     """Longest prefix of an index-file body that a batch may end with."""
     return int(lib().sbwt_index_file_batch_cut(fmt, bytes(body), len(body), max_reads))
+
+
+def index_file_colored_cut(fmt, body):
+    """Longest prefix ending after a read that has a found index (0 if none)."""
+    return int(lib().sbwt_index_file_colored_cut(fmt, bytes(body), len(body)))
 
 
 class IndexFileParser:

@@ -192,7 +192,20 @@ int sbwt_index_get_info(const sbwt_index *index, sbwt_index_info *info) {
     info->device_bytes = index->dev.device_bytes;
     info->has_key_kmer_marks = index->dev.has_marks;
     info->wide = index->dev.wide;
+    info->has_suffix_groups = index->dev.has_groups;
+    info->suffix_groups_rejected = index->dev.groups_rejected;
   });
+}
+
+int sbwt_suffix_groups_consistent(const uint64_t *const acgt[4], const uint64_t *suffix_group_starts,
+                                  uint64_t num_bits) {
+  int ok = 0;
+  const int rc = guarded([&] {
+    require(acgt != nullptr && suffix_group_starts != nullptr, "null argument");
+    for (int c = 0; c < 4; ++c) require(acgt[c] != nullptr, "null bit vector");
+    ok = sbwt_amd::suffix_groups_consistent(acgt, suffix_group_starts, num_bits) ? 1 : 0;
+  });
+  return rc < 0 ? rc : ok;
 }
 
 int sbwt_searcher_create(sbwt_index *index, uint64_t max_chars_per_batch,
@@ -582,6 +595,11 @@ uint64_t sbwt_index_file_batch_cut(int format, const char *body, uint64_t n_byte
                                    uint64_t max_reads) {
   if (!body || (format != SBWT_INDEX_FILE_ASCII && format != SBWT_INDEX_FILE_BINARY)) return 0;
   return sbwt_amd::IndexFileParser::batch_cut(format, body, n_bytes, max_reads);
+}
+
+uint64_t sbwt_index_file_colored_cut(int format, const char *body, uint64_t n_bytes) {
+  if (!body || (format != SBWT_INDEX_FILE_ASCII && format != SBWT_INDEX_FILE_BINARY)) return 0;
+  return sbwt_amd::IndexFileParser::colored_cut(format, body, n_bytes);
 }
 
 int sbwt_fasta_parser_create(uint64_t max_bytes_per_chunk, uint64_t max_reads_per_chunk,

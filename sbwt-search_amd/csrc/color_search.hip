@@ -151,6 +151,12 @@ __device__ __forceinline__ void window_add(const ColorView &cv, const u64 *__res
   wave_payload(cv, set, units, run, lane, cnt);
 }
 
+// All k-mer results [ks, ke) of one read into the wave's LDS counters.
+__device__ __forceinline__ void read_add(const ColorView &cv, const u64 *__restrict__ results, u64 ks,
+                                         u64 ke, u32 lane, u32 *cnt, u32 &f, u32 &nf, u32 &inv) {
+  for (u64 base = ks; base < ke; base += kWave) window_add(cv, results, base, ke, lane, cnt, f, nf, inv);
+}
+
 __device__ __forceinline__ u32 wave_sum(u32 x) {
   for (u32 o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
   return x;
@@ -167,13 +173,18 @@ struct KernelParams {
 
 // Fused color search: one wave per read (grid-stride). COUNTS writes the raw
 // u32 counts; otherwise the read's colors with count >= ceil(seq_size * tau)
-// (ContinuousColorResultsPrinter.hpp:235-263) become a bitmap + count.
+// (ContinuousColorResultsPrinter.hpp:235-263) become a bitmap + count. With
+// blank_min (quirk 2, below) each read also leaves colored[r] = (found > 0)
+// and, when it found nothing but its threshold is positive, blank_min[r] =
+// that threshold (else 0).
 template <bool COUNTS>
 __global__ __launch_bounds__(256) void k_color_reads(ColorView cv, const u64 *__restrict__ results,
                                                       const u64 *__restrict__ kmer_ends, u64 n_reads,
                                                       KernelParams p, u64 *__restrict__ bitmap,
                                                       u64 *__restrict__ nout, u32 *__restrict__ stats,
-                                                      u32 *__restrict__ counts_out) {
+                                                      u32 *__restrict__ counts_out,
+                                                      u64 *__restrict__ blank_min,
+                                                      u64 *__restrict__ colored) {
   extern __shared__ u32 lds[];
   const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   const u32 waves = blockDim.x / kWave;
@@ -186,7 +197,7 @@ __global__ __launch_bounds__(256) void k_color_reads(ColorView cv, const u64 *__
        r += static_cast<u64>(gridDim.x) * waves) {
     const u64 ks = r ? kmer_ends[r - 1] : 0, ke = kmer_ends[r];
     u32 f = 0, nf = 0, inv = 0;
-    for (u64 base = ks; base < ke; base += kWave) window_add(cv, results, base, ke, lane, cnt, f, nf, inv);
+    read_add(cv, results, ks, ke, lane, cnt, f, nf, inv);
     f = wave_sum(f);
     nf = wave_sum(nf);
     inv = wave_sum(inv);
@@ -216,6 +227,10 @@ __global__ __launch_bounds__(256) void k_color_reads(ColorView cv, const u64 *__
       }
       if (lane == 0) {
         nout[r] = total;
+        if (blank_min) {
+          colored[r] = f > 0 ? 1 : 0;
+          blank_min[r] = f == 0 ? min_hits : 0;
+        }
         if (stats) {
           stats[3 * r] = f;
           stats[3 * r + 1] = nf;
@@ -223,6 +238,66 @@ __global__ __launch_bounds__(256) void k_color_reads(ColorView cv, const u64 *__
         }
       }
     }
+    wave_lds_fence();
+  }
+}
+
+// Quirk 2 of the reference's colors stage: a read with no found index gets
+// the colored-read slot of the next read that has one (colored_seq_id,
+// src/IndexFileParser/IndexFileParser.cpp:65-74) and is printed with that
+// slot's counts against its own threshold (ContinuousColorResultsPrinter.hpp
+// :201-209). That only shows when its threshold is positive
+// (--include-not-found / --include-invalid). colored_incl[r] = colored reads
+// among 0..r; slot_read[q] = the read of colored slot q.
+__global__ __launch_bounds__(256) void k_colored_slots(const u64 *__restrict__ colored_incl,
+                                                        u64 n_reads, u64 *__restrict__ slot_read) {
+  for (u64 r = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; r < n_reads;
+       r += static_cast<u64>(gridDim.x) * blockDim.x) {
+    const u64 c = colored_incl[r], before = r ? colored_incl[r - 1] : 0;
+    if (c != before) slot_read[c - 1] = r;
+  }
+}
+
+// One wave per blank read with a positive threshold: the counts of the next
+// colored read of the batch, thresholded with the blank read's minimum. A
+// blank read with no colored read after it in the batch keeps its zero counts:
+// the reference reads past its batch's counts there (slot = number of colored
+// reads), so it has no defined output.
+__global__ __launch_bounds__(256) void k_color_quirk2(ColorView cv, const u64 *__restrict__ results,
+                                                       const u64 *__restrict__ kmer_ends,
+                                                       u64 n_reads, const u64 *__restrict__ blank_min,
+                                                       const u64 *__restrict__ colored_incl,
+                                                       const u64 *__restrict__ slot_read,
+                                                       u64 *__restrict__ bitmap, u64 *__restrict__ nout) {
+  extern __shared__ u32 lds[];
+  const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const u32 waves = blockDim.x / kWave;
+  const u32 nc = cv.num_colors;
+  u32 *cnt = lds + static_cast<u64>(wave) * nc;
+  for (u32 c = lane; c < nc; c += kWave) cnt[c] = 0;
+  wave_lds_fence();
+  const u32 words = (nc + 63) / 64;
+  const u64 n_colored = colored_incl[n_reads - 1];
+  for (u64 r = static_cast<u64>(blockIdx.x) * waves + wave; r < n_reads;
+       r += static_cast<u64>(gridDim.x) * waves) {
+    const u64 min_hits = blank_min[r];
+    const u64 q = colored_incl[r];  // r found nothing: the colored reads before it
+    if (min_hits == 0 || q >= n_colored) continue;
+    const u64 rc = slot_read[q];
+    const u64 ks = rc ? kmer_ends[rc - 1] : 0, ke = kmer_ends[rc];
+    u32 f = 0, nf = 0, inv = 0;
+    read_add(cv, results, ks, ke, lane, cnt, f, nf, inv);
+    wave_lds_fence();
+    u64 total = 0;
+    for (u32 w = 0; w < words; ++w) {
+      const u32 c = w * 64 + lane;
+      const bool hit = c < nc && cnt[c] >= min_hits;
+      const u64 m = __ballot(hit);
+      if (lane == 0) bitmap[r * words + w] = m;
+      total += __popcll(m);
+      if (c < nc) cnt[c] = 0;
+    }
+    if (lane == 0) nout[r] = total;
     wave_lds_fence();
   }
 }
@@ -406,12 +481,13 @@ ColorSearcher::ColorSearcher(const DeviceColors *colors, u64 max_indexes, u64 ma
   SBWT_HIP(hipMalloc(&d_bitmap_, std::max<u64>(max_reads_ * words, 1) * 8));
   SBWT_HIP(hipMalloc(&d_nout_, std::max<u64>(max_reads_, 1) * 8 * 2));
   SBWT_HIP(hipMalloc(&d_scan_tmp_, scan_tmp_entries(max_reads_) * 8));
+  SBWT_HIP(hipMalloc(&d_quirk_, std::max<u64>(max_reads_, 1) * 8 * 4));
   for (auto &e : ev_) SBWT_HIP(hipEventCreate(&e));
 }
 
 ColorSearcher::~ColorSearcher() {
   for (auto &e : ev_) (void)hipEventDestroy(e);
-  void *bufs[] = {d_bitmap_, d_nout_, d_scan_tmp_, d_warped_, d_intervals_, d_counts_};
+  void *bufs[] = {d_bitmap_, d_nout_, d_scan_tmp_, d_quirk_, d_warped_, d_intervals_, d_counts_};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (own_stream_) (void)hipStreamDestroy(stream_);
@@ -454,16 +530,7 @@ u64 ColorSearcher::pseudoalign_device(const u64 *d_results, const u64 *d_kmer_en
   if (n_reads == 0) return 0;
   if (n_reads > max_reads_) throw std::runtime_error("batch exceeds the color searcher's max reads");
   const u32 nc = static_cast<u32>(colors_->num_colors);
-  const u32 waves = waves_per_block(nc);
-  KernelParams kp{p.threshold, p.include_not_found, p.include_invalid, p.first_read_quirk};
-  if (timing_) SBWT_HIP(hipEventRecord(ev_[0], stream_));
-  k_color_reads<false><<<grid_for(n_reads, waves), waves * kWave, waves * nc * 4, stream_>>>(
-      colors_->view(), d_results, d_kmer_ends, n_reads, kp, d_bitmap_, d_nout_, d_stats, nullptr);
-  SBWT_HIP(hipGetLastError());
-  if (timing_) {
-    SBWT_HIP(hipEventRecord(ev_[1], stream_));
-    recorded_ = true;
-  }
+  launch_threshold(d_results, d_kmer_ends, n_reads, p, d_stats);
   u64 *d_incl = d_nout_ + max_reads_;
   scan_inclusive_u64(d_nout_, n_reads, d_incl, d_scan_tmp_, stream_);
   u64 total = 0;
@@ -484,21 +551,42 @@ void ColorSearcher::pseudoalign_device_async(const u64 *d_results, const u64 *d_
   if (n_reads == 0) return;
   if (n_reads > max_reads_) throw std::runtime_error("batch exceeds the color searcher's max reads");
   const u32 nc = static_cast<u32>(colors_->num_colors);
-  const u32 waves = waves_per_block(nc);
-  KernelParams kp{p.threshold, p.include_not_found, p.include_invalid, p.first_read_quirk};
-  if (timing_) SBWT_HIP(hipEventRecord(ev_[0], stream_));
-  k_color_reads<false><<<grid_for(n_reads, waves), waves * kWave, waves * nc * 4, stream_>>>(
-      colors_->view(), d_results, d_kmer_ends, n_reads, kp, d_bitmap_, d_nout_, d_stats, nullptr);
-  SBWT_HIP(hipGetLastError());
-  if (timing_) {
-    SBWT_HIP(hipEventRecord(ev_[1], stream_));
-    recorded_ = true;
-  }
+  launch_threshold(d_results, d_kmer_ends, n_reads, p, d_stats);
   u64 *d_incl = d_nout_ + max_reads_;
   scan_inclusive_u64(d_nout_, n_reads, d_incl, d_scan_tmp_, stream_);
   k_color_compact<<<grid_for(n_reads, 4), 4 * kWave, 0, stream_>>>(d_bitmap_, d_incl, n_reads, nc,
                                                                     d_ids, ids_capacity, d_offsets);
   SBWT_HIP(hipGetLastError());
+}
+
+void ColorSearcher::launch_threshold(const u64 *d_results, const u64 *d_kmer_ends, u64 n_reads,
+                                     const ColorParams &p, u32 *d_stats) {
+  const u32 nc = static_cast<u32>(colors_->num_colors);
+  const u32 waves = waves_per_block(nc);
+  KernelParams kp{p.threshold, p.include_not_found, p.include_invalid, p.first_read_quirk};
+  // quirk 2 needs a positive threshold on a read that found nothing, which
+  // only the include flags give
+  const bool quirk2 = p.include_not_found || p.include_invalid;
+  u64 *blank_min = d_quirk_, *colored = d_quirk_ + max_reads_;
+  u64 *colored_incl = d_quirk_ + 2 * max_reads_, *slot_read = d_quirk_ + 3 * max_reads_;
+  if (timing_) SBWT_HIP(hipEventRecord(ev_[0], stream_));
+  const u32 grid = grid_for(n_reads, waves);
+  k_color_reads<false><<<grid, waves * kWave, waves * nc * 4, stream_>>>(
+      colors_->view(), d_results, d_kmer_ends, n_reads, kp, d_bitmap_, d_nout_, d_stats, nullptr,
+      quirk2 ? blank_min : nullptr, quirk2 ? colored : nullptr);
+  SBWT_HIP(hipGetLastError());
+  if (quirk2) {
+    scan_inclusive_u64(colored, n_reads, colored_incl, d_scan_tmp_, stream_);
+    k_colored_slots<<<grid_for(n_reads, 256), 256, 0, stream_>>>(colored_incl, n_reads, slot_read);
+    k_color_quirk2<<<grid, waves * kWave, waves * nc * 4, stream_>>>(
+        colors_->view(), d_results, d_kmer_ends, n_reads, blank_min, colored_incl, slot_read,
+        d_bitmap_, d_nout_);
+    SBWT_HIP(hipGetLastError());
+  }
+  if (timing_) {
+    SBWT_HIP(hipEventRecord(ev_[1], stream_));
+    recorded_ = true;
+  }
 }
 
 void ColorSearcher::counts_device(const u64 *d_results, const u64 *d_kmer_ends, u64 n_reads,
@@ -509,7 +597,8 @@ void ColorSearcher::counts_device(const u64 *d_results, const u64 *d_kmer_ends, 
   KernelParams kp{1.0, 0, 0, 0};
   SBWT_HIP(hipEventRecord(ev_[0], stream_));
   k_color_reads<true><<<grid_for(n_reads, waves), waves * kWave, waves * nc * 4, stream_>>>(
-      colors_->view(), d_results, d_kmer_ends, n_reads, kp, nullptr, nullptr, nullptr, d_counts);
+      colors_->view(), d_results, d_kmer_ends, n_reads, kp, nullptr, nullptr, nullptr, d_counts,
+      nullptr, nullptr);
   SBWT_HIP(hipGetLastError());
   SBWT_HIP(hipEventRecord(ev_[1], stream_));
   recorded_ = true;

@@ -50,6 +50,9 @@ struct ColorParams {
   // (ContinuousColorResultsPrinter.hpp:62-64,178-180), so the first read of a
   // stream is thresholded with found-1, not_found-1, invalid-1 (mod 2^64).
   bool first_read_quirk = false;
+  // Quirk 2 needs no flag: with include_not_found or include_invalid, a read
+  // with no found index is thresholded against the counts of the next read of
+  // the batch that has one (IndexFileParser.cpp:65-74).
 };
 
 class ColorSearcher {
@@ -85,6 +88,10 @@ class ColorSearcher {
   void set_timing(bool on) { timing_ = on; }
 
  private:
+  // k_color_reads (bitmap + count per read) and, with an include flag, the
+  // quirk-2 pass over reads that found nothing
+  void launch_threshold(const u64 *d_results, const u64 *d_kmer_ends, u64 n_reads,
+                        const ColorParams &p, u32 *d_stats);
   const DeviceColors *colors_;
   hipStream_t stream_;
   bool own_stream_;
@@ -92,6 +99,7 @@ class ColorSearcher {
   u64 *d_bitmap_ = nullptr;   // [max_reads][words per read]
   u64 *d_nout_ = nullptr;     // per read
   u64 *d_scan_tmp_ = nullptr;
+  u64 *d_quirk_ = nullptr;    // [4][max_reads]: blank_min, colored, colored_incl, slot_read
   // staging for the reference interface
   u64 *d_warped_ = nullptr, *d_intervals_ = nullptr, *d_counts_ = nullptr;
   hipEvent_t ev_[2];

@@ -8,6 +8,7 @@
 
 #include "device_fns.hpp"
 #include "device_index.hpp"
+#include "loaders.hpp"
 
 namespace sbwt_amd {
 
@@ -215,14 +216,16 @@ __global__ __launch_bounds__(256) void k_hash_extend(IndexView ix, const PrefixE
 #pragma unroll
       for (u32 c = 0; c < 4; ++c) {
         if (!((keep >> c) & 1)) continue;
-        // a k-letter interval is one node; insert (code + 1 keeps 0 for empty)
-        const u64 key = ch[c].code + 1;
+        // a k-letter interval is one node. A bucket is claimed by setting its
+        // {node, occupied} word from zero; the code goes in after the claim
+        // (every code is inserted once: the level's strings are distinct, and
+        // lookups run in later launches)
+        const unsigned long long claim = static_cast<unsigned long long>(ch[c].l) | (1ULL << 32);
         u64 h = kmer_hash_slot(ch[c].code, bits);
         for (u64 probe = 0; probe <= mask; ++probe, h = (h + 1) & mask) {
-          unsigned long long *slot = reinterpret_cast<unsigned long long *>(table + h);
-          const unsigned long long prev = atomicCAS(slot, 0ULL, static_cast<unsigned long long>(key));
-          if (prev == 0 || prev == key) {
-            table[h].z = ch[c].l;
+          unsigned long long *half = reinterpret_cast<unsigned long long *>(table + h) + 1;
+          if (atomicCAS(half, 0ULL, claim) == 0ULL) {
+            *reinterpret_cast<unsigned long long *>(table + h) = ch[c].code;
             break;
           }
         }
@@ -301,6 +304,13 @@ void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const
   }
   n_blocks = n / kBlockNodes + 1;  // the line of node n exists: rank(n) is defined
   has_marks = km_words != nullptr;
+  // the suffix-group starts are used only when every node outside them has an
+  // empty row, as in every SBWT. Rows that break this break the SBWT property
+  // the streaming search rests on (a k-mer's successor is one step from a
+  // single node), so such an index is searched per k-mer, which is the
+  // reference's interval search on whatever rows the file holds
+  groups_rejected = sgs_words != nullptr && !suffix_groups_consistent(acgt, sgs_words, n);
+  if (groups_rejected) sgs_words = nullptr;
   has_groups = sgs_words != nullptr;
   // line bits; counts are filled by the prefix pass below
   std::vector<uint4> host(n_blocks * kLineSlots);

@@ -40,6 +40,9 @@ struct DeviceIndex {
   uint4 *kmer_hash = nullptr;  // k-mer code -> node table (narrow indexes), or null
   u32 kmer_hash_bits = 0;
   bool has_groups = false;  // suffix-group starts are in the lines
+  // starts were given but some non-start row has an edge: not an SBWT, so
+  // every search runs per k-mer (the streaming search needs the SBWT property)
+  bool groups_rejected = false;
   u64 device_bytes = 0;
 
   // acgt: four host bit vectors of ceil(n/64) words (bit i of word w = node

@@ -301,6 +301,58 @@ bool cap_scan_start(u64 n, u64 max_reads, Count count, u64 *start, u64 *seen) {
 
 }  // namespace
 
+u64 IndexFileParser::colored_cut(int format, const char *body, u64 n) {
+  if (format == kIndexFileBinary) {
+    // a word below u64 max - 2 is a found index
+    u64 end = 0;  // words of the candidate read end (exclusive), 0: none yet
+    bool found = false;
+    for (u64 w = n / 8; w > 0; --w) {
+      u64 v;
+      std::memcpy(&v, body + 8 * (w - 1), 8);
+      if (v == kNewline) {
+        if (end && found) return 8 * end;
+        end = w;
+        found = false;
+      } else if (end && v < kNewline) {
+        found = true;
+      }
+    }
+    return end && found ? 8 * end : 0;
+  }
+  // ascii: read ends are '\n' not preceded by '-' (as batch_cut); a read
+  // has a found index when the reference's loop meets a char >= '0' outside
+  // the skip that follows a '-'
+  const auto line_found = [&](u64 s, u64 e) {
+    for (u64 i = s; i < e;) {
+      const signed char c = static_cast<signed char>(body[i++]);
+      if (c == '-') {
+        ++i;  // the char after '-' (1: not found, 2: invalid, else nothing)
+        while (i < e && static_cast<signed char>(body[i]) >= '0') ++i;
+        ++i;  // the char that ended the skip
+        continue;
+      }
+      if (c >= '0') return true;
+    }
+    return false;
+  };
+  u64 c = n;
+  u64 end = 0;  // candidate: offset right after a read end
+  while (c > 0) {
+    const void *nl = memrchr(body, '\n', static_cast<size_t>(c));
+    u64 at = nl ? static_cast<u64>(static_cast<const char *>(nl) - body) : 0;
+    if (nl && at > 0 && body[at - 1] == '-') {  // consumed after a '-': not a read end
+      c = at;
+      continue;
+    }
+    const u64 line_start = nl ? at + 1 : 0;
+    if (end && line_found(line_start, end - 1)) return end;
+    if (!nl) break;
+    end = at + 1;
+    c = at;
+  }
+  return 0;
+}
+
 u64 IndexFileParser::batch_cut(int format, const char *body, u64 n, u64 max_reads) {
   if (max_reads == 0) return 0;
   if (format == kIndexFileBinary) {

@@ -58,6 +58,13 @@ class IndexFileParser {
   // '-'), or after a u64 max - 2 word in binary, holding at most max_reads
   // read ends; 0 if there is none.
   static u64 batch_cut(int format, const char *body, u64 n, u64 max_reads);
+  // Longest prefix of body[0, n) (which starts at a read start) ending right
+  // after the read end of a read with a found index, or 0. A batch cut there
+  // keeps every read without a found index in the batch of the next read that
+  // has one, whose counts the reference prints it with (colored_seq_id,
+  // IndexFileParser.cpp:65-74). Read ends as in batch_cut; a read's tokens
+  // follow the reference's char loop (AsciiIndexFileParser.cpp:49-66).
+  static u64 colored_cut(int format, const char *body, u64 n);
 
  private:
   u64 max_bytes_, max_reads_, max_results_;

@@ -218,7 +218,7 @@ __device__ __forceinline__ u64 search_window(const IndexView &ix, u64 window, u3
                                              u32 &walked) {
   if (ix.kmer_hash) {  // the k-mer table answers in a pair or two of buckets
     const u64 code = window >> (64 - 2 * ix.k);
-    const u64 key = code + 1, mask = (1ULL << ix.kmer_hash_bits) - 1;
+    const u64 mask = (1ULL << ix.kmer_hash_bits) - 1;
     u64 h = kmer_hash_slot(code, ix.kmer_hash_bits);
     // `steps` counts the pairs read beyond the first, which stands in for the
     // presearch line the caller counts
@@ -226,16 +226,15 @@ __device__ __forceinline__ u64 search_window(const IndexView &ix, u64 window, u3
       const uint4 *pair = ix.kmer_hash + (h & ~1ULL);
       const uint4 b0 = pair[0], b1 = pair[1];
       if (pr) ++steps;
-      const u64 k0 = static_cast<u64>(b0.x) | (static_cast<u64>(b0.y) << 32);
-      const u64 k1 = static_cast<u64>(b1.x) | (static_cast<u64>(b1.y) << 32);
-      const bool use0 = (h & 1) == 0;  // an odd start skips the pair's first bucket
+      const int m0 = (h & 1) == 0 ? kmer_bucket_match(b0, code) : -1;  // an odd start skips b0
+      const int m1 = kmer_bucket_match(b1, code);
       u64 node = kNotFound;
       bool done = false;
-      if (use0 && (k0 == key || k0 == 0)) {
-        node = k0 == key ? static_cast<u64>(b0.z) : kNotFound;
+      if (m0 >= 0) {
+        node = m0 ? static_cast<u64>(b0.z) : kNotFound;
         done = true;
-      } else if (k1 == key || k1 == 0) {
-        node = k1 == key ? static_cast<u64>(b1.z) : kNotFound;
+      } else if (m1 >= 0) {
+        node = m1 ? static_cast<u64>(b1.z) : kNotFound;
         done = true;
       }
       if (done) {
@@ -698,20 +697,20 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         break;
       case kHash: {
         if (STATS) ++st_hash;
-        const u64 key = (window >> (64 - 2 * k)) + 1;
-        const u64 k0 = static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32);
-        const u64 k1 = static_cast<u64>(d2.x) | (static_cast<u64>(d2.y) << 32);
-        const bool use0 = (pre_idx & 1) == 0;  // an odd start skips the pair's first bucket
+        const u64 code = window >> (64 - 2 * k);
+        // an odd start skips the pair's first bucket
+        const int m0 = (pre_idx & 1) == 0 ? kmer_bucket_match(d1, code) : -1;
+        const int m1 = kmer_bucket_match(d2, code);
         bool settled = true;
-        if (use0 && k0 == key) {
+        if (m0 == 1) {
           found_at = static_cast<Idx>(d1.z);
           found = true;
-        } else if (use0 && k0 == 0) {
+        } else if (m0 == 0) {
           notfound = true;
-        } else if (k1 == key) {
+        } else if (m1 == 1) {
           found_at = static_cast<Idx>(d2.z);
           found = true;
-        } else if (k1 == 0) {
+        } else if (m1 == 0) {
           notfound = true;
         } else {
           settled = false;
@@ -1184,10 +1183,11 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
   if (timing_) SBWT_HIP(hipEventRecord(ev_[0], stream_));
   launch_pack(d_chars, n_chars, d_packed_, d_invalid_, stream_);
   if (timing_) SBWT_HIP(hipEventRecord(ev_[1], stream_));
-  const bool streaming = algorithm_ == kAlgoStreaming ||
-                         (algorithm_ == kAlgoAuto &&
-                          n_chars >= (ix.kmer_hash ? kAutoStreamingCharsTable : kAutoStreamingChars)) ||
-                         (algorithm_ == kAlgoStreamingAdaptive && n_chars >= kAdaptiveStreamingChars);
+  const bool streaming = !index_->groups_rejected &&
+                         (algorithm_ == kAlgoStreaming ||
+                          (algorithm_ == kAlgoAuto &&
+                           n_chars >= (ix.kmer_hash ? kAutoStreamingCharsTable : kAutoStreamingChars)) ||
+                          (algorithm_ == kAlgoStreamingAdaptive && n_chars >= kAdaptiveStreamingChars));
   // work items of the streaming search: (read, up to `chunk` k-mers)
   const u32 chunk = streaming ? stream_chunk(n_chars) : 1;
   if (streaming && n_chars / chunk + n_reads + 1 > max_chunks_)

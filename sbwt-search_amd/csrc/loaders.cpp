@@ -95,6 +95,18 @@ HostSbwt load_sbwt(const std::string &path) {
   return s;
 }
 
+bool suffix_groups_consistent(const uint64_t *const acgt[4], const uint64_t *sgs, uint64_t n) {
+  const uint64_t words = (n + 63) / 64;
+  for (uint64_t w = 0; w < words; ++w) {
+    uint64_t edges = acgt[0][w] | acgt[1][w] | acgt[2][w] | acgt[3][w];
+    uint64_t starts = sgs[w];
+    if (w == 0) starts |= 1;
+    if (w + 1 == words && (n & 63)) edges &= (1ULL << (n & 63)) - 1;
+    if (edges & ~starts) return false;
+  }
+  return true;
+}
+
 HostBitVector load_key_kmer_marks(const std::string &path) {
   In in(path);
   check_colors_magic(in);  // SbwtBuilder.cpp:147-165, skip per :167-182

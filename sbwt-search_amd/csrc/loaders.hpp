@@ -42,6 +42,17 @@ struct HostColors {
 };
 
 HostSbwt load_sbwt(const std::string &path);
+
+// Whether a suffix-group-starts vector fits the SBWT it came with: in an SBWT
+// only the first node of each suffix group (the nodes sharing their last k - 1
+// letters) carries the group's outgoing edges, so every node that is not
+// marked as a group start has an empty row in all four letter vectors. This is
+// what the reference's interval step C[c] + rank_c relies on to count each
+// edge once (src/IndexSearcher/IndexSearcher.cuh:48-55), and what a search
+// stepping from a single node through its group start needs. Bits past
+// num_bits are ignored; node 0 (the root) always starts a group.
+bool suffix_groups_consistent(const uint64_t *const acgt[4], const uint64_t *suffix_group_starts,
+                              uint64_t num_bits);
 HostBitVector load_key_kmer_marks(const std::string &tcolors_path);
 HostColors load_colors(const std::string &tcolors_path);
 

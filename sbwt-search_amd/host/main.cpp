@@ -2,6 +2,7 @@
 //   sbwt_search index  -q <fasta/q(.gz)|.list> -i <.sbwt|.tdbg> -o <prefix|.list>
 //                      [-k <.tcolors>] [-p ascii|binary|bool] [--no-headers] [-s streams]
 //                      [--batch-chars N] [--host-parse]
+//                      [--search-algorithm auto|streaming|per-kmer|adaptive] [--kmer-table]
 //   sbwt_search colors -q <index results|.list> -k <.tcolors> -o <prefix|.list>
 //                      [-p ascii|binary|csv] [-t threshold] [--include-not-found]
 //                      [--include-invalid] [--no-headers] [-s streams] [--batch-bytes N]
@@ -50,7 +51,8 @@ const std::map<std::string, std::string> kShort = {
     {"r", "base-pairs-per-seq"},  {"g", "gpu-memory-percentage"}, {"s", "streams"},
     {"p", "print-mode"},          {"k", "colors-file"},     {"t", "threshold"},
     {"h", "help"}};
-const char *kFlags[] = {"no-headers", "include-not-found", "include-invalid", "host-parse", "help"};
+const char *kFlags[] = {"no-headers", "include-not-found", "include-invalid", "host-parse",
+                        "kmer-table", "help"};
 
 Args parse(int argc, char **argv) {
   Args a;
@@ -232,7 +234,7 @@ struct OutChunk {
 // takes the rest.
 void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<std::string> &in,
                   const std::vector<std::string> &out, const std::vector<u64> &files, int mode,
-                  bool headers, u64 max_chars, bool device_parse) {
+                  bool headers, u64 max_chars, bool device_parse, int algorithm) {
   const u64 max_reads = std::max<u64>(max_chars / 16, 1024);
   constexpr size_t kBuffers = 3;  // per direction: one filling, one in the queue, one in use
   Channel<InBatch> to_device(1);
@@ -266,6 +268,7 @@ void index_stream(sbwt_index *index, u32 k, bool move_to_key, const std::vector<
     sbwt_fasta_parser *fp = nullptr;
     try {
       check(sbwt_searcher_create(index, max_chars, max_reads, move_to_key, nullptr, &s));
+      check(sbwt_searcher_set_algorithm(s, algorithm));
       check(sbwt_formatter_create(max_chars, max_reads, nullptr, &fmt));
       bool line_open = false;
       InBatch b;
@@ -534,6 +537,20 @@ int run_index(const Args &a) {
     max_chars = std::max<u64>(std::min<u64>(max_chars, std::stoull(a.get("batch-chars"))), info.kmer_size);
   // --host-parse (an extension) keeps every sequence file on the host reader
   const bool device_parse = !a.flag("host-parse");
+  // --search-algorithm (an extension): the search kernel of every batch; the
+  // default picks by batch size. Results are identical for all four.
+  const std::string algo_name = a.get("search-algorithm", "auto");
+  const std::map<std::string, int> algos = {{"auto", SBWT_ALGO_AUTO},
+                                            {"streaming", SBWT_ALGO_STREAMING},
+                                            {"per-kmer", SBWT_ALGO_PER_KMER},
+                                            {"adaptive", SBWT_ALGO_STREAMING_ADAPTIVE}};
+  if (!algos.count(algo_name)) throw std::runtime_error("Invalid value passed by user for argument search-algorithm");
+  const int algorithm = algos.at(algo_name);
+  // --kmer-table (an extension): build the index's k-mer table before searching
+  if (a.flag("kmer-table")) {
+    int built = 0;
+    check(sbwt_index_build_kmer_table(index, &built));
+  }
   const auto parts = sbwt_host::partition_by_size(in, streams);
   std::vector<std::thread> th;
   std::vector<std::string> errors(streams);
@@ -541,7 +558,7 @@ int run_index(const Args &a) {
     th.emplace_back([&, p] {
       try {
         index_stream(index, info.kmer_size, !colors.empty(), in, out, parts[p], mode,
-                     !a.flag("no-headers"), max_chars, device_parse);
+                     !a.flag("no-headers"), max_chars, device_parse, algorithm);
       } catch (const std::exception &e) {
         errors[p] = e.what();
       }
@@ -704,6 +721,10 @@ void colors_stream(sbwt_color_index *ci, u64 num_colors, const std::vector<std::
         if (cut == 0)
           throw std::runtime_error("no read end within one batch (" + std::to_string(max_bytes) +
                                    " bytes) of index file " + in[f] + "; raise --batch-bytes");
+        // end the batch after a read with a found index, so that reads with
+        // none (printed with the next such read's counts under the include
+        // flags) stay with it; a batch with no such read keeps its cut
+        if (const u64 cc = sbwt_index_file_colored_cut(b.format, b.body.p, cut)) cut = cc;
         ColorInBatch next;
         if (!free_in.pop(next.body)) {
           ok = false;

@@ -95,8 +95,36 @@ def read_stats(results, kmer_ends):
     return out
 
 
-def threshold_lists(counts, stats, tau, inc_nf=False, inc_inv=False, first_read_quirk=False):
-    """Sorted color ids per read: count >= ceil(seq_size * tau), u64 wrap for the quirk."""
+def colored_seq_ids(stats):
+    """colored_seq_id of every read (IndexFileParser.cpp:65-74 through the oracle's
+    orc_warp_layout): a read's colored slot, or for a read with no found index the
+    slot of the next read that has one (= the number of colored reads before it)."""
+    n = len(stats)
+    res, ends = [], []
+    for f, nf, inv in stats:
+        res += [0] * f + [int(O.NOT_FOUND)] * nf + [int(O.INVALID)] * inv
+        ends.append(len(res))
+    res = np.array(res + [0], np.uint64)
+    ends = np.array(ends + [0], np.uint64)
+    warped = np.zeros(len(res) + 64 * (n + 1), np.uint64)
+    wi = np.zeros(n + 2, np.uint64)
+    ni = np.zeros(1, np.uint64)
+    cs = np.zeros(n + 1, np.uint64)
+    f_, nf_, inv_ = (np.zeros(n + 1, np.uint64) for _ in range(3))
+    O.lib().orc_warp_layout(O.p64(res), O.p64(ends), n, 64, O.p64(warped), O.p64(wi), O.p64(ni),
+                            O.p64(cs), O.p64(f_), O.p64(nf_), O.p64(inv_))
+    return [int(x) for x in cs[:n]]
+
+
+def threshold_lists(counts, stats, tau, inc_nf=False, inc_inv=False, first_read_quirk=False,
+                    quirk2=True):
+    """Sorted color ids per read: count >= ceil(seq_size * tau), u64 wrap for the quirk.
+    Quirk 2 (ContinuousColorResultsPrinter.hpp:201-209): each read is printed with the
+    counts of its colored_seq_id slot; a read with no found index therefore takes the
+    counts of the next read that has one, and none after the batch's last such read
+    (the reference reads past its counts there; zero counts are used here)."""
+    colored = [r for r, st in enumerate(stats) if st[0] > 0]
+    slots = colored_seq_ids(stats) if quirk2 else None
     lists = []
     for r, (f, nf, inv) in enumerate(stats):
         if first_read_quirk and r == 0:
@@ -104,5 +132,12 @@ def threshold_lists(counts, stats, tau, inc_nf=False, inc_inv=False, first_read_
         seq = (f + (nf if inc_nf else 0) + (inv if inc_inv else 0)) % (1 << 64)
         mn = O.lib().orc_color_min_hits(f, nf, inv, tau, int(inc_nf), int(inc_inv))
         assert seq >= 0
-        lists.append([c for c in range(counts.shape[1]) if mn > 0 and counts[r, c] >= mn])
+        row = r
+        if quirk2:
+            q = slots[r]
+            row = colored[q] if q < len(colored) else None
+        if row is None:
+            lists.append([])
+            continue
+        lists.append([c for c in range(counts.shape[1]) if mn > 0 and counts[row, c] >= mn])
     return lists

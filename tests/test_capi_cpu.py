@@ -168,3 +168,39 @@ def test_index_file_batch_cut_large_body():
     for max_reads in sorted(probes):
         want = 8 * (int(ends[min(max_reads, len(ends)) - 1]) + 1)
         assert S.index_file_batch_cut(1, words.tobytes(), max_reads) == want, max_reads
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_index_file_colored_cut_by_oracle_parse(seed):
+    """The cut after the last read with a found index, checked against the oracle's
+    parse of the same body (the reference's char loop): ascii lines of tokens
+    including malformed ones ('-1-1' is a not-found then a found 1; '-\\n5' swallows
+    the newline and the 5), and binary words."""
+    import numpy as np
+    import oracle_py as O
+    rng = np.random.default_rng(seed)
+    toks = [b"123", b"-1", b"-2", b"7", b"-1-1", b"-\n5", b"0", b"-12"]
+    for trial in range(40):
+        lines = []
+        for _ in range(int(rng.integers(1, 12))):
+            k = int(rng.integers(0, 5))
+            pick = [toks[int(i)] for i in rng.integers(0, len(toks), k)]
+            if rng.random() < 0.5:  # many reads with no found index
+                pick = [t for t in pick if t in (b"-1", b"-2", b"-12")]
+            lines.append(b" ".join(pick) + b"\n")
+        body = b"".join(lines)
+        res, kends = O.parse_index_body(body, 0)
+        ends_b = [i + 1 for i in range(len(body)) if body[i] == 10 and (i < 1 or body[i - 1] != 45)]
+        assert len(ends_b) == len(kends)
+        want, st = 0, 0
+        for r, e in enumerate(kends):
+            if np.any(res[st:int(e)] < O.INVALID - np.uint64(1)):
+                want = ends_b[r]
+            st = int(e)
+        assert S.index_file_colored_cut(0, body) == want, body
+    nl, nf, inv = 2**64 - 3, 2**64 - 1, 2**64 - 2
+    words = [5, nf, nl, nf, inv, nl, nl, 9, nl, nf, nl]
+    raw = np.array(words, np.uint64).tobytes()
+    assert S.index_file_colored_cut(1, raw) == 8 * 9
+    assert S.index_file_colored_cut(1, raw[:8 * 8]) == 8 * 3
+    assert S.index_file_colored_cut(1, np.array([nf, nl, nl], np.uint64).tobytes()) == 0

@@ -22,14 +22,19 @@ def run(args):
     return r
 
 
+@pytest.mark.parametrize("search", [[], ["--search-algorithm", "streaming"],
+                                    ["--search-algorithm", "streaming", "--kmer-table"],
+                                    ["--search-algorithm", "adaptive", "--kmer-table"],
+                                    ["--search-algorithm", "per-kmer"]],
+                         ids=["auto", "streaming", "streaming-table", "adaptive-table", "per-kmer"])
 @pytest.mark.parametrize("batch", [None, 40, 31])
-def test_index_cli_matches_reference_expected(tmp_path, batch):
+def test_index_cli_matches_reference_expected(tmp_path, batch, search):
     sbwt = os.path.join(FIXTURES, "search_test_index.sbwt")
     for name in NAMES:
         out = str(tmp_path / name)
         extra = ["--batch-chars", str(batch)] if batch else []
         run(["index", "-q", os.path.join(FIXTURES, "index_search", name), "-i", sbwt, "-o", out]
-            + extra)
+            + extra + search)
         exp = open(os.path.join(FIXTURES, "index_search", "expected",
                                 name.split(".")[0] + ".txt"), "rb").read()
         assert open(out + ".txt", "rb").read() == exp, name

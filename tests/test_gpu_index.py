@@ -27,18 +27,177 @@ def fixture_pair():
     return S.Index.load(path), O.OracleSbwt(path)
 
 
+@pytest.fixture(scope="module")
+def fixture_table_index():
+    """The fixture index with its k-mer table (fresh searches read buckets)."""
+    idx = S.Index.load(os.path.join(FIXTURES, "search_test_index.sbwt"))
+    assert idx.build_kmer_table()
+    return idx
+
+
+FIXTURE_ALGOS = [(S.ALGO_AUTO, "auto"), (S.ALGO_STREAMING, "streaming"),
+                 (S.ALGO_PER_KMER, "per-kmer")]
+
+
+@pytest.mark.parametrize("table", [False, True], ids=["no-table", "kmer-table"])
+@pytest.mark.parametrize("algo", [a for a, _ in FIXTURE_ALGOS], ids=[n for _, n in FIXTURE_ALGOS])
 @pytest.mark.parametrize("name", INDEX_FIXTURES)
-def test_reference_fixtures_byte_identical(fixture_pair, name):
-    say(f"fixture {name}")
+def test_reference_fixtures_byte_identical(fixture_pair, fixture_table_index, name, algo, table):
+    """The reference's own index and expected files, through every search kernel:
+    the streaming kernel (the bench's and the large-batch kernel) included."""
+    say(f"fixture {name} algo={algo} table={table}")
     idx, orc = fixture_pair
+    if table:
+        idx = fixture_table_index
     chars, ends = O.read_seqfile(os.path.join(FIXTURES, "index_search", name))
     s = idx.searcher(max(len(chars), 1), max(len(ends), 1))
+    s.set_algorithm(algo)
+    s.set_stats(True)
     res = s.search_reads(chars, ends)
     assert np.array_equal(res, orc.index_reads(chars, ends))
     out = O.ascii_index_output(res, ends, idx.k)
     exp = open(os.path.join(FIXTURES, "index_search", "expected",
                             name.split(".")[0] + ".txt"), "rb").read()
     assert out == exp
+    st = s.last_stats()
+    if algo == S.ALGO_STREAMING:
+        assert S.Searcher.resolved(st) == len(res)
+        if np.count_nonzero(res[1:] < O.INVALID) and len(res) > 1:
+            # consecutive found k-mers step from their predecessor's node
+            both = (res[1:] < O.INVALID) & (res[:-1] < O.INVALID)
+            assert st["streamed"] > 0 or not both.any()
+
+
+@pytest.mark.parametrize("table", [False, True], ids=["no-table", "kmer-table"])
+@pytest.mark.parametrize("algo,launch", [(S.ALGO_STREAMING_ADAPTIVE, None),
+                                         (S.ALGO_STREAMING, (64, 2))],
+                         ids=["adaptive", "streaming-64"])
+def test_reference_fixtures_repeated_streaming(fixture_pair, fixture_table_index, algo, launch,
+                                               table):
+    """Every fixture's reads repeated past 2^16 chars, so that the adaptive form
+    streams too (with work items shrunk to 8+ k-mers); results are the tiled
+    oracle results and the reference's expected file repeated."""
+    idx, orc = fixture_pair
+    if table:
+        idx = fixture_table_index
+    for name in INDEX_FIXTURES:
+        say(f"fixture {name} repeated algo={algo} launch={launch} table={table}")
+        chars, ends = O.read_seqfile(os.path.join(FIXTURES, "index_search", name))
+        reps = (1 << 16) // max(len(chars), 1) + 2
+        big = bytes(chars) * reps
+        big_ends = np.concatenate([ends + np.uint64(i * len(chars)) for i in range(reps)])
+        assert len(big) >= (1 << 16)
+        s = idx.searcher(len(big), len(big_ends))
+        s.set_algorithm(algo)
+        if launch:
+            s.set_launch(*launch)
+        s.set_stats(True)
+        res = s.search_reads(big, big_ends)
+        one = orc.index_reads(chars, ends)
+        assert np.array_equal(res, np.tile(one, reps))
+        st = s.last_stats()
+        assert S.Searcher.resolved(st) == len(res)
+        if np.count_nonzero(one < O.INVALID) > 1:
+            assert st["streamed"] + st["full_searches"] > 0
+        exp = open(os.path.join(FIXTURES, "index_search", "expected",
+                                name.split(".")[0] + ".txt"), "rb").read()
+        body = O.ascii_index_output(res, big_ends, idx.k, headers=False)
+        one_body = O.ascii_index_output(one, ends, idx.k, headers=False)
+        assert body == one_body * reps
+        assert exp.endswith(one_body)
+
+
+def test_rejected_suffix_groups_search_per_kmer(tmp_path):
+    """A file whose rows break the SBWT property (an edge on a node that does not
+    start its suffix group) is searched per k-mer whatever the algorithm: the
+    results are the reference's interval search on the file's rows (the oracle)."""
+    from sbwt_file import PlainMatrix
+    say("rejected suffix-group starts: reference fixture with a start cleared")
+    pm = PlainMatrix(os.path.join(FIXTURES, "search_test_index.sbwt"))
+    edges = pm.vector(0) | pm.vector(1) | pm.vector(2) | pm.vector(3)
+    node = next(i for i in range(1, pm.num_bits) if (int(edges[i // 64]) >> (i % 64)) & 1)
+    pm.set_bit(4, node, 0)
+    bad = str(tmp_path / "bad_starts.sbwt")
+    pm.write(bad)
+    orc = O.OracleSbwt(bad)
+    idx = S.Index.load(bad)
+    assert idx.suffix_groups_rejected and not idx.has_suffix_groups
+    for name in INDEX_FIXTURES:
+        chars, ends = O.read_seqfile(os.path.join(FIXTURES, "index_search", name))
+        s = idx.searcher(max(len(chars), 1), max(len(ends), 1))
+        s.set_algorithm(S.ALGO_STREAMING)
+        s.set_stats(True)
+        res = s.search_reads(chars, ends)
+        assert np.array_equal(res, orc.index_reads(chars, ends))
+        assert s.last_stats()["streamed"] == 0
+    say("rejected suffix-group starts: builder index with an edge on a non-start row")
+    root = W.genome_random(12, 20_000)
+    gens = [root] + [W.genome_mutate(root, s_, 0.03) for s_ in (1, 2, 3)]
+    codes, gends = W.concat(gens)
+    b = W.BuiltSbwt(codes, gends, 31)
+    good = str(tmp_path / "good.sbwt")
+    b.write(good)
+    pm = PlainMatrix(good)
+    starts = pm.vector(4)
+    non_start = [i for i in range(1, b.n) if not (int(starts[i // 64]) >> (i % 64)) & 1]
+    for v in non_start[::max(1, len(non_start) // 50)]:
+        pm.set_bit(v % 4, v, 1)
+    bad2 = str(tmp_path / "bad_rows.sbwt")
+    pm.write(bad2)
+    assert S.Index.load(good).has_suffix_groups
+    idx2 = S.Index.load(bad2)
+    assert idx2.suffix_groups_rejected
+    orc2 = O.OracleSbwt(bad2)
+    chars, rends, _ = W.reads(7, codes, gends, np.full(2000, 150, np.uint32), 0.05, 0.005, 0.0)
+    chars = chars.tobytes()
+    for algo in (S.ALGO_STREAMING, S.ALGO_STREAMING_ADAPTIVE, S.ALGO_AUTO):
+        s = idx2.searcher(len(chars), len(rends))
+        s.set_algorithm(algo)
+        assert np.array_equal(s.search_reads(chars, rends), orc2.index_reads(chars, rends)), algo
+
+
+@pytest.mark.parametrize("algo", [S.ALGO_STREAMING, S.ALGO_PER_KMER, S.ALGO_STREAMING_ADAPTIVE],
+                         ids=["streaming", "per-kmer", "adaptive"])
+@pytest.mark.parametrize("present", [True, False], ids=["polyT-present", "polyT-absent"])
+def test_kmer_table_k32_poly_t(tmp_path, present, algo):
+    """k = 32 with the k-mer table: poly-T's code is 2^64 - 1 and poly-A's is 0,
+    so bucket occupancy cannot be a code value. Reads full of T and A runs are
+    searched with the index holding those k-mers and without them."""
+    say(f"k=32 k-mer table poly-T present={present} algo={algo}")
+    g = W.genome_random(321, 30_000)
+    if present:
+        g = np.concatenate([g[:10_000], np.full(40, 3, np.uint8), g[10_000:20_000],
+                            np.full(35, 0, np.uint8), g[20_000:]])
+    else:  # break every run of T or A longer than 20
+        for c in (0, 3):
+            run = 0
+            for i in range(len(g)):
+                run = run + 1 if g[i] == c else 0
+                if run > 20:
+                    g[i] = 1
+                    run = 0
+    codes, ends = W.concat([g])
+    b = W.BuiltSbwt(codes, ends, 32)
+    path = str(tmp_path / "k32.sbwt")
+    b.write(path)
+    orc = O.OracleSbwt(path)
+    idx = S.Index.load(path)
+    assert idx.k == 32 and idx.build_kmer_table()
+    rd, rends, _ = W.reads(9, codes, ends, np.full(3000, 120, np.uint32), 0.1, 0.01, 0.0)
+    polys = [b"T" * 120, b"A" * 100, b"C" * 10 + b"T" * 60 + b"G" * 50, b"T" * 32, b"A" * 33,
+             b"T" * 31 + b"A" * 40]
+    chars = rd.tobytes() + b"".join(polys)
+    rends = np.concatenate([rends, rends[-1] + np.cumsum([len(p) for p in polys]).astype(np.uint64)])
+    s = idx.searcher(len(chars), len(rends))
+    s.set_algorithm(algo)
+    res = s.search_reads(chars, rends)
+    exp = orc.index_reads(chars, rends)
+    assert np.array_equal(res, exp)
+    tail = exp[-(sum(max(0, len(p) - 31) for p in polys)):]
+    if present:
+        assert np.count_nonzero(tail < O.INVALID) > 0
+    else:
+        assert np.all(tail == O.NOT_FOUND)
 
 
 @pytest.fixture(scope="module")

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle_py as O
-from colorlib import ColoredIndex, kmer_ends_of, read_stats, threshold_lists, K
+from colorlib import colored_seq_ids, ColoredIndex, kmer_ends_of, read_stats, threshold_lists, K
 
 
 @pytest.fixture(scope="module")
@@ -81,9 +81,72 @@ def test_first_read_quirk_changes_threshold():
     assert threshold_lists(counts, [(11, 0, 0)], 0.7) == [[1, 2]]
     assert threshold_lists(counts, [(11, 0, 0)], 0.7, first_read_quirk=True) == [[0, 1, 2]]
     # a first read with no found k-mers wraps to 2^64 - 1 and reports nothing ...
-    assert threshold_lists(np.array([[5]], np.uint64), [(0, 5, 0)], 0.5, False, False, True) == [[]]
-    # ... unless not-found k-mers are included: (2^64 - 1) + 4 wraps to 3, minimum 2
-    assert threshold_lists(np.array([[5]], np.uint64), [(0, 5, 0)], 0.5, True, False, True) == [[0]]
+    counts = np.array([[0], [5]], np.uint64)
+    stats = [(0, 5, 0), (6, 0, 0)]
+    assert threshold_lists(counts, stats, 0.5, False, False, True) == [[], [0]]
+    # ... unless not-found k-mers are included: (2^64 - 1) + 4 wraps to 3, minimum 2,
+    # met by the next colored read's count 5 (quirk 2)
+    assert threshold_lists(counts, stats, 0.5, True, False, True) == [[0], [0]]
+
+
+def test_warp_layout_matches_reference_unit_vectors():
+    """orc_warp_layout (IndexFileParser.cpp:36-74) against the expected vectors of the
+    reference's own parser tests (BinaryIndexFileParser_test.cpp), including
+    colored_seq_id: a read with no found index gets the slot of the next read that
+    has one; the entry after the last read is the batch-end slot."""
+    import json
+    import os
+    from conftest import GOLDEN
+    vecs = json.load(open(os.path.join(GOLDEN, "reference_unit_vectors.json")))["warp_layout"]
+    for v in vecs:
+        reads = v["reads"]
+        n = len(reads)
+        res, ends = [], []
+        for r in reads:
+            res += [int(O.NOT_FOUND) if x == -1 else int(O.INVALID) if x == -2 else x for x in r]
+            ends.append(len(res))
+        res = np.array(res + [0], np.uint64)
+        ends = np.array(ends, np.uint64)
+        w = v["warp"]
+        warped = np.zeros(len(res) + w * (n + 1), np.uint64)
+        wi = np.zeros(n + 2, np.uint64)
+        ni = np.zeros(1, np.uint64)
+        cs = np.zeros(n + 1, np.uint64)
+        f, nf, inv = (np.zeros(n + 1, np.uint64) for _ in range(3))
+        nw = O.lib().orc_warp_layout(O.p64(res), O.p64(ends), n, w, O.p64(warped), O.p64(wi),
+                                     O.p64(ni), O.p64(cs), O.p64(f), O.p64(nf), O.p64(inv))
+        exp_cs = v["colored_seq_id"]
+        assert [int(x) for x in cs[:n]] == exp_cs[:n], v["case"]
+        # a whole-file vector holds one more entry: the open sequence after the last
+        # read end, whose slot is the batch-end slot (number of colored reads); a
+        # first-batch vector ends inside its last read, whose counts are partial
+        whole = len(exp_cs) == n + 1
+        if whole:
+            assert int(ni[0]) - 1 == exp_cs[n], v["case"]
+        m = n if whole else n - 1
+        assert [int(x) for x in f[:m]] == v["found"][:m]
+        assert [int(x) for x in nf[:m]] == v["not_found"][:m]
+        assert [int(x) for x in inv[:m]] == v["invalid"][:m]
+        if "warped" in v:
+            exp_w = [int(O.NOT_FOUND) if x == -1 else x for x in v["warped"]]
+            assert [int(x) for x in warped[:nw]] == exp_w
+            assert [int(x) for x in wi[:int(ni[0])]] == v["warp_intervals"]
+
+
+def test_quirk2_known_answers():
+    """Reads with no found index are printed with the next colored read's counts
+    against their own threshold (hand-derived answers, tau = 0.5)."""
+    counts = np.array([[3, 1, 0], [0, 0, 0], [0, 0, 0], [0, 4, 2], [0, 0, 0]], np.uint64)
+    stats = [(4, 0, 0), (0, 5, 0), (0, 0, 3), (4, 1, 0), (0, 2, 0)]
+    # without include flags a blank read's threshold is 0: nothing printed
+    assert threshold_lists(counts, stats, 0.5) == [[0], [], [], [1, 2], []]
+    # with not-found included: read 1 (min 3) takes read 3's counts -> [1];
+    # read 2 (min 0, invalid not included) prints nothing; read 3 (min 3) -> [1];
+    # read 4 (min 1) has no colored read after it: batch end, nothing
+    assert threshold_lists(counts, stats, 0.5, True) == [[0], [1], [], [1], []]
+    # with both flags: read 2 (min 2) takes read 3's counts -> [1, 2]
+    assert threshold_lists(counts, stats, 0.5, True, True) == [[0], [1], [1, 2], [1], []]
+    assert colored_seq_ids(stats) == [0, 1, 1, 1, 2]
 
 
 def _parse_ascii_indexes(data):

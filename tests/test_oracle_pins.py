@@ -171,6 +171,34 @@ def test_search_equals_brute_force_colex_rank(seed, glen, k, tmp_path):
         assert got == exp, q
 
 
+def test_search_k32_poly_runs_equals_brute_force(tmp_path):
+    """k = 32, where every 64-bit value is a k-mer code: poly-T (2^64 - 1) and
+    poly-A (0) k-mers, present and absent, against the explicit colex rank."""
+    k = 32
+    g = W.genome_random(8, 700)
+    g = np.concatenate([g[:300], np.full(40, 3, np.uint8), g[300:]]).astype(np.uint8)
+    codes, ends = W.concat([g])
+    b = W.BuiltSbwt(codes, ends, k)
+    nodes = _node_strings(g, k)
+    assert len(nodes) == b.n
+    path = str(tmp_path / "t.sbwt")
+    b.write(path)
+    s = O.OracleSbwt(path)
+    rank = {x: i for i, x in enumerate(nodes)}
+    gs = "".join("ACGT"[c] for c in g)
+    queries = ["T" * 32, "A" * 32, "C" * 32, "G" * 32, "T" * 31 + "A", "A" * 31 + "T"]
+    queries += [gs[i:i + k] for i in range(0, len(gs) - k + 1, 7)]
+    for q in queries:
+        got = s.search_codes(np.array(["ACGT".index(c) for c in q], np.uint8))
+        assert got == rank.get(q, int(O.NOT_FOUND)), q
+    assert rank["T" * 32] != int(O.NOT_FOUND) and rank["A" * 32] != int(O.NOT_FOUND)
+    # and through the reads path (packing, windows of 32 chars)
+    reads = ("T" * 50 + "A" * 40 + "C" * 33 + gs[100:200]).encode()
+    res = s.index_reads(reads, np.array([len(reads)], np.uint64))
+    for i in range(len(reads) - k + 1):
+        assert int(res[i]) == rank.get(reads[i:i + k].decode(), int(O.NOT_FOUND)), i
+
+
 def test_presearch_width_does_not_change_results(tmp_path):
     g, b = _tiny_sbwt(7, 5000, 31)
     path = str(tmp_path / "t.sbwt")
