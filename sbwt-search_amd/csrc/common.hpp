@@ -78,7 +78,7 @@ struct IndexView {
   u32 hyper_shift;            // lines per hyperblock = 2^hyper_shift (kHyperShift)
   const void *presearch;      // [4^kp] of {left,right} (uint2 or ulonglong2)
   const void *key_dest;       // [n] key-k-mer move destination (u32 or u64), or null
-  const uint4 *kmer_hash;     // [2^kmer_hash_bits] {k-mer code (u64), node (u32), 1}; empty: all 0; or null
+  const uint4 *kmer_hash;     // [2^kmer_hash_bits] {k-mer code + 1 (u64), node (u32), 0}, or null
   u32 kmer_hash_bits;
   u64 c_map[5];
   u64 num_nodes;
@@ -92,16 +92,13 @@ struct IndexView {
 // 2^32 nodes), by its 2k-bit code (first letter highest), in open addressing
 // with linear probing from the code's multiplicative hash; a lookup reads
 // buckets in aligned pairs (one 32-byte load each) and gives up after
-// kHashPairs pairs (the full search then answers). A bucket's last word marks
-// it occupied: at k = 32 every 64-bit value is a k-mer code (poly-T is
-// 2^64 - 1), so no code value can stand for "empty".
+// kHashPairs pairs (the full search then answers). A bucket holds code + 1,
+// so 0 marks it empty; the one code whose key would wrap to 0, poly-T at
+// k = 32 (2^64 - 1), is never inserted and always searched in full
+// (kmer_hash_key).
 constexpr u32 kHashPairs = 4;
-// What a bucket says about a k-mer code: 1 it holds the code (node in .z),
-// 0 the code is absent (the bucket is empty), -1 another code (probe on).
-__device__ __forceinline__ int kmer_bucket_match(const uint4 b, u64 code) {
-  if (b.w == 0) return 0;
-  return (static_cast<u64>(b.x) | (static_cast<u64>(b.y) << 32)) == code ? 1 : -1;
-}
+// Table key of a k-mer code: code + 1, 0 for the code that has no key.
+__host__ __device__ __forceinline__ u64 kmer_hash_key(u64 code) { return code + 1; }
 __host__ __device__ __forceinline__ u64 kmer_hash_slot(u64 code, u32 bits) {
   return (code * 0x9E3779B97F4A7C15ULL) >> (64 - bits);
 }

@@ -216,16 +216,17 @@ __global__ __launch_bounds__(256) void k_hash_extend(IndexView ix, const PrefixE
 #pragma unroll
       for (u32 c = 0; c < 4; ++c) {
         if (!((keep >> c) & 1)) continue;
-        // a k-letter interval is one node. A bucket is claimed by setting its
-        // {node, occupied} word from zero; the code goes in after the claim
-        // (every code is inserted once: the level's strings are distinct, and
-        // lookups run in later launches)
-        const unsigned long long claim = static_cast<unsigned long long>(ch[c].l) | (1ULL << 32);
+        // a k-letter interval is one node; its bucket holds code + 1 (0 marks
+        // an empty bucket; the code without a key, poly-T at k = 32, stays out
+        // and is always searched in full)
+        const u64 key = kmer_hash_key(ch[c].code);
+        if (key == 0) continue;
         u64 h = kmer_hash_slot(ch[c].code, bits);
         for (u64 probe = 0; probe <= mask; ++probe, h = (h + 1) & mask) {
-          unsigned long long *half = reinterpret_cast<unsigned long long *>(table + h) + 1;
-          if (atomicCAS(half, 0ULL, claim) == 0ULL) {
-            *reinterpret_cast<unsigned long long *>(table + h) = ch[c].code;
+          unsigned long long *slot = reinterpret_cast<unsigned long long *>(table + h);
+          const unsigned long long prev = atomicCAS(slot, 0ULL, static_cast<unsigned long long>(key));
+          if (prev == 0 || prev == key) {
+            table[h].z = ch[c].l;
             break;
           }
         }

@@ -216,8 +216,9 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_write(const u64 *__restri
 template <typename Idx, bool KEY>
 __device__ __forceinline__ u64 search_window(const IndexView &ix, u64 window, u32 &steps,
                                              u32 &walked) {
-  if (ix.kmer_hash) {  // the k-mer table answers in a pair or two of buckets
-    const u64 code = window >> (64 - 2 * ix.k);
+  const u64 code = window >> (64 - 2 * ix.k);
+  const u64 key = kmer_hash_key(code);
+  if (ix.kmer_hash && key != 0) {  // the k-mer table answers in a pair or two of buckets
     const u64 mask = (1ULL << ix.kmer_hash_bits) - 1;
     u64 h = kmer_hash_slot(code, ix.kmer_hash_bits);
     // `steps` counts the pairs read beyond the first, which stands in for the
@@ -226,15 +227,16 @@ __device__ __forceinline__ u64 search_window(const IndexView &ix, u64 window, u3
       const uint4 *pair = ix.kmer_hash + (h & ~1ULL);
       const uint4 b0 = pair[0], b1 = pair[1];
       if (pr) ++steps;
-      const int m0 = (h & 1) == 0 ? kmer_bucket_match(b0, code) : -1;  // an odd start skips b0
-      const int m1 = kmer_bucket_match(b1, code);
+      const u64 k0 = static_cast<u64>(b0.x) | (static_cast<u64>(b0.y) << 32);
+      const u64 k1 = static_cast<u64>(b1.x) | (static_cast<u64>(b1.y) << 32);
+      const bool use0 = (h & 1) == 0;  // an odd start skips the pair's first bucket
       u64 node = kNotFound;
       bool done = false;
-      if (m0 >= 0) {
-        node = m0 ? static_cast<u64>(b0.z) : kNotFound;
+      if (use0 && (k0 == key || k0 == 0)) {
+        node = k0 == key ? static_cast<u64>(b0.z) : kNotFound;
         done = true;
-      } else if (m1 >= 0) {
-        node = m1 ? static_cast<u64>(b1.z) : kNotFound;
+      } else if (k1 == key || k1 == 0) {
+        node = k1 == key ? static_cast<u64>(b1.z) : kNotFound;
         done = true;
       }
       if (done) {
@@ -481,7 +483,8 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
   // a fresh search of the current window: the k-mer table when there is one
   // (pre_idx = bucket, j = pairs read), else the presearch entry and steps
   auto start_search = [&]() {
-    if (ix.kmer_hash) {
+    // (the code without a table key, poly-T at k = 32, is searched in full)
+    if (ix.kmer_hash && kmer_hash_key(window >> (64 - 2 * k)) != 0) {
       pre_idx = kmer_hash_slot(window >> (64 - 2 * k), ix.kmer_hash_bits);
       j = 0;
       mode = kHash;
@@ -697,20 +700,20 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         break;
       case kHash: {
         if (STATS) ++st_hash;
-        const u64 code = window >> (64 - 2 * k);
-        // an odd start skips the pair's first bucket
-        const int m0 = (pre_idx & 1) == 0 ? kmer_bucket_match(d1, code) : -1;
-        const int m1 = kmer_bucket_match(d2, code);
+        const u64 key = kmer_hash_key(window >> (64 - 2 * k));
+        const u64 k0 = static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32);
+        const u64 k1 = static_cast<u64>(d2.x) | (static_cast<u64>(d2.y) << 32);
+        const bool use0 = (pre_idx & 1) == 0;  // an odd start skips the pair's first bucket
         bool settled = true;
-        if (m0 == 1) {
+        if (use0 && k0 == key) {
           found_at = static_cast<Idx>(d1.z);
           found = true;
-        } else if (m0 == 0) {
+        } else if (use0 && k0 == 0) {
           notfound = true;
-        } else if (m1 == 1) {
+        } else if (k1 == key) {
           found_at = static_cast<Idx>(d2.z);
           found = true;
-        } else if (m1 == 0) {
+        } else if (k1 == 0) {
           notfound = true;
         } else {
           settled = false;

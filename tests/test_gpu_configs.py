@@ -214,7 +214,7 @@ def test_c4_shape_batched_equals_plain_and_oracle(tmp_path):
     say("C4 shape: 12 genomes x 200 kbp, 3,000 log-normal + 12,000 short reads")
     wl = configs.c4(n_long=3000, n_short=12_000, n_genomes=12, length=200_000)
     lens = np.diff(np.concatenate([[0], wl.read_ends.astype(np.int64)]))
-    assert lens.max() > 20_000 and np.count_nonzero(lens == 150) == 12_000
+    assert lens.max() > 20_000 and np.count_nonzero(lens == 150) >= 12_000
     ci = Colored(tmp_path, wl.genomes, wl.genome_ends, d=20)
     idx = S.Index.load(ci.sbwt_path, ci.colors_path)
     assert idx.build_kmer_table()

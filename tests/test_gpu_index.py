@@ -83,7 +83,9 @@ def test_reference_fixtures_repeated_streaming(fixture_pair, fixture_table_index
     for name in INDEX_FIXTURES:
         say(f"fixture {name} repeated algo={algo} launch={launch} table={table}")
         chars, ends = O.read_seqfile(os.path.join(FIXTURES, "index_search", name))
-        reps = (1 << 16) // max(len(chars), 1) + 2
+        if len(chars) == 0:  # a file of empty reads has nothing to repeat
+            continue
+        reps = (1 << 16) // len(chars) + 2
         big = bytes(chars) * reps
         big_ends = np.concatenate([ends + np.uint64(i * len(chars)) for i in range(reps)])
         assert len(big) >= (1 << 16)
