@@ -165,6 +165,28 @@ int sbwt_searcher_search_reads_device(sbwt_searcher *searcher, const char *d_cha
                                       uint64_t n_chars, const uint64_t *d_read_ends,
                                       uint64_t n_reads, uint64_t *d_results, uint64_t *n_kmers);
 
+/* Allocation-free fused search: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The per-k-mer fused search
+ * of sbwt_searcher_search_reads_device (same kernels, same results) of one
+ * batch of at most SBWT_NOALLOC_MAX_CHARS chars and SBWT_NOALLOC_MAX_READS
+ * reads, without a searcher: the packed sequence, invalid masks, k-mer ends
+ * and scan sums live in a static device array of the library (about 3.7 MB
+ * per GPU, held until the process ends) instead of a searcher's buffers, so
+ * the call makes no hipMalloc, hipFree or other allocation on the device.
+ * d_results receives one result per k-mer, in read order (as many as the
+ * reads have k-mers: sum of max(0, length - k + 1)). Enqueued on hip_stream
+ * (null: the null stream); the index's device must be current. Every call on
+ * a GPU shares that GPU's array and it keeps the last call's contents: calls
+ * that overlap on one GPU (other streams or threads) race, so serialize them.
+ * Larger batches are rejected with an error. */
+#define SBWT_NOALLOC_MAX_CHARS (1ULL << 22)
+#define SBWT_NOALLOC_MAX_READS (1ULL << 18)
+int sbwt_index_search_reads_noalloc(const sbwt_index *index, int move_to_key_kmer,
+                                    const char *d_chars, uint64_t n_chars,
+                                    const uint64_t *d_read_ends, uint64_t n_reads,
+                                    uint64_t *d_results, void *hip_stream);
+
 /* Device time of the last call's kernels (ms): packing, k-mer offsets,
  * search. The reference logs the same with GpuEvent pairs
  * (src/IndexSearcher/IndexSearcher.cu:22,62,66-70). */

@@ -18,6 +18,8 @@ ALGO_STREAMING = 0
 ALGO_PER_KMER = 1
 ALGO_AUTO = 2  # default: per k-mer below 2^22 chars per batch, streaming (adaptive work items) above
 ALGO_STREAMING_ADAPTIVE = 3  # adaptive work items from 2^16 chars: batches sharing the GPU
+NOALLOC_MAX_CHARS = 1 << 22
+NOALLOC_MAX_READS = 1 << 18
 NOT_FOUND = 0xFFFFFFFFFFFFFFFF
 INVALID = 0xFFFFFFFFFFFFFFFE
 WALK_STUCK = 0xFFFFFFFFFFFFFFFC
@@ -81,6 +83,8 @@ def lib():
         L.sbwt_index_load.argtypes = [C.c_char_p, C.c_char_p, C.c_uint32, C.POINTER(vp)]
         L.sbwt_index_build_kmer_table.argtypes = [vp, C.POINTER(C.c_int)]
         L.sbwt_suffix_groups_consistent.argtypes = [u64p * 4, u64p, C.c_uint64]
+        L.sbwt_index_search_reads_noalloc.argtypes = [vp, C.c_int, vp, C.c_uint64, vp,
+                                                      C.c_uint64, vp, vp]
         L.sbwt_index_destroy.argtypes = [vp]
         L.sbwt_index_destroy.restype = None
         L.sbwt_index_get_info.argtypes = [vp, C.POINTER(IndexInfo)]
@@ -278,6 +282,15 @@ class Index:
 
     def searcher(self, max_chars, max_reads, move_to_key_kmer=False, stream=None):
         return Searcher(self, max_chars, max_reads, move_to_key_kmer, stream)
+
+    def search_reads_noalloc(self, d_chars, n_chars, d_read_ends, n_reads, d_results,
+                             move_to_key_kmer=False, stream=None):
+        """Allocation-free fused search (sbwt_index_search_reads_noalloc): device
+        buffers in and out, enqueued on `stream` (a hipStream_t, or None)."""
+        _check(lib().sbwt_index_search_reads_noalloc(
+            self.h, int(move_to_key_kmer), C.c_void_p(d_chars), n_chars,
+            C.c_void_p(d_read_ends), n_reads, C.c_void_p(d_results),
+            C.c_void_p(stream) if stream else None))
 
 
 class Searcher:

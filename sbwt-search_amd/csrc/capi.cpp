@@ -147,6 +147,7 @@ int sbwt_index_create(const uint64_t *const acgt[4], uint64_t num_bits, uint32_t
     auto idx = std::make_unique<sbwt_index>();
     idx->dev.build(acgt, num_bits, kmer_size, kp, key_kmer_marks, key_kmer_marks ? num_bits : 0,
                    suffix_group_starts);
+    (void)sbwt_amd::noalloc_scratch(idx->dev.device);  // resolve the device's scratch at set-up
     *out = idx.release();
   });
 }
@@ -168,7 +169,22 @@ int sbwt_index_load(const char *sbwt_path, const char *colors_path, uint32_t pre
     auto idx = std::make_unique<sbwt_index>();
     idx->dev.build(acgt, s.num_bits, k, kp, colors_path ? km.words.data() : nullptr,
                    colors_path ? km.bits : 0, s.suffix_group_starts.data());
+    (void)sbwt_amd::noalloc_scratch(idx->dev.device);  // resolve the device's scratch at set-up
     *out = idx.release();
+  });
+}
+
+int sbwt_index_search_reads_noalloc(const sbwt_index *index, int move_to_key_kmer,
+                                    const char *d_chars, uint64_t n_chars,
+                                    const uint64_t *d_read_ends, uint64_t n_reads,
+                                    uint64_t *d_results, void *hip_stream) {
+  return guarded([&] {
+    require(index != nullptr, "null argument");
+    require(n_reads == 0 || (d_read_ends && d_results && (n_chars == 0 || d_chars)),
+            "null argument");
+    sbwt_amd::search_reads_noalloc(index->dev, move_to_key_kmer != 0, d_chars, n_chars,
+                                   d_read_ends, n_reads, d_results,
+                                   static_cast<hipStream_t>(hip_stream));
   });
 }
 

@@ -1,6 +1,7 @@
 // Index search kernels for gfx950 and the Searcher that drives them.
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "device_fns.hpp"
@@ -1007,6 +1008,70 @@ void dispatch_search3(bool wide, bool key, bool stats, F f) {
 }
 
 }  // namespace
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free search, in 256-byte aligned parts (u64
+// words): packed sequence (n/32 + 8 words), invalid masks (as many u32),
+// inclusive k-mer ends per read, and the block sums of their scan.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr u64 align_words(u64 w) { return (w + 31) / 32 * 32; }
+constexpr u64 kNaPackedWords = kNoAllocMaxChars / 32 + 8;
+constexpr u64 kNaScanLen = kNoAllocMaxReads / kScanChunk + 2;
+constexpr u64 kNaPacked = 0;
+constexpr u64 kNaInvalid = kNaPacked + align_words(kNaPackedWords);
+constexpr u64 kNaKmerEnds = kNaInvalid + align_words((kNaPackedWords + 1) / 2);
+constexpr u64 kNaScanTmp = kNaKmerEnds + align_words(kNoAllocMaxReads);
+constexpr u64 kNaWords = kNaScanTmp + align_words(2 * kNaScanLen);
+}  // namespace
+static_assert(kNaWords * 8 < (4u << 20), "the allocation-free scratch stays below 4 MiB");
+
+static __device__ __align__(256) u64 scratch[kNaWords];
+
+u64 *noalloc_scratch(int device) {
+  static std::mutex mu;
+  static u64 *ptrs[256] = {};
+  if (device < 0 || device >= 256) throw std::runtime_error("device ordinal out of range");
+  std::lock_guard<std::mutex> g(mu);
+  if (!ptrs[device]) {
+    void *p = nullptr;
+    SBWT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ptrs[device] = static_cast<u64 *>(p);
+  }
+  return ptrs[device];
+}
+
+void search_reads_noalloc(const DeviceIndex &index, bool move_to_key_kmer, const char *d_chars,
+                          u64 n_chars, const u64 *d_read_ends, u64 n_reads, u64 *d_results,
+                          hipStream_t stream) {
+  if (n_chars > kNoAllocMaxChars || n_reads > kNoAllocMaxReads)
+    throw std::runtime_error("batch exceeds the allocation-free search's maximum of 4194304 chars "
+                             "and 262144 reads");
+  if (move_to_key_kmer && !index.has_marks)
+    throw std::runtime_error("move to key k-mer requested but the index has no key-k-mer marks");
+  int dev = 0;
+  SBWT_HIP(hipGetDevice(&dev));
+  if (dev != index.device) throw std::runtime_error("the index's device is not the current device");
+  if (n_reads == 0) return;
+  u64 *base = noalloc_scratch(dev);
+  u64 *packed = base + kNaPacked;
+  u32 *invalid = reinterpret_cast<u32 *>(base + kNaInvalid);
+  u64 *kmer_ends = base + kNaKmerEnds, *scan_tmp = base + kNaScanTmp;
+  const IndexView ix = index.view();
+  launch_pack(d_chars, n_chars, packed, invalid, stream);
+  const u64 chunks = (n_reads + kScanChunk - 1) / kScanChunk;
+  const u32 scan_grid = static_cast<u32>(chunks);
+  k_scan_reduce<false><<<scan_grid, kScanThreads, 0, stream>>>(d_read_ends, n_reads, ix.k, n_chars, 1,
+                                                               scan_tmp, kNaScanLen);
+  k_scan_chunks<<<1, 1024, 0, stream>>>(scan_tmp, chunks, kNaScanLen);
+  k_scan_write<false><<<scan_grid, kScanThreads, 0, stream>>>(d_read_ends, n_reads, ix.k, n_chars, 1,
+                                                              scan_tmp, kNaScanLen, kmer_ends, nullptr);
+  dispatch_search(index.wide, move_to_key_kmer, [&]<typename Idx, bool KEY>() {
+    k_search_reads<Idx, KEY, false><<<grid_for(n_chars, kSearchThreads), kSearchThreads, 0, stream>>>(
+        ix, packed, invalid, d_read_ends, kmer_ends, n_reads, n_chars, d_results, nullptr);
+  });
+  SBWT_HIP(hipGetLastError());
+}
 
 void launch_pack(const char *d_chars, u64 n_chars, u64 *d_packed, u32 *d_invalid, hipStream_t s) {
   const u64 n_words = (n_chars + 31) / 32 + 4;  // zero words of padding past the end

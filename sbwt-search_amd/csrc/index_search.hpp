@@ -141,4 +141,20 @@ class Searcher {
 // Device kernels exposed for reuse by the color stage and tests.
 void launch_pack(const char *d_chars, u64 n_chars, u64 *d_packed, u32 *d_invalid, hipStream_t s);
 
+// The fused per-k-mer search of one batch (the kernels of
+// Searcher::search_reads_device with the per-k-mer algorithm) that allocates
+// nothing: its packed sequence, invalid masks, k-mer ends and scan sums live
+// in a file-scope device array of kNoAllocScratchBytes, one copy per GPU,
+// whose address is resolved once per device (noalloc_scratch). Batches are
+// limited to kNoAllocMaxChars chars and kNoAllocMaxReads reads. Every call on
+// one GPU shares that GPU's array, so calls that overlap on a GPU race.
+constexpr u64 kNoAllocMaxChars = 1ULL << 22;
+constexpr u64 kNoAllocMaxReads = 1ULL << 18;
+// Device address of this GPU's scratch array (the index's device must be
+// current); resolved by hipGetSymbolAddress on the first call per device.
+u64 *noalloc_scratch(int device);
+void search_reads_noalloc(const DeviceIndex &index, bool move_to_key_kmer, const char *d_chars,
+                          u64 n_chars, const u64 *d_read_ends, u64 n_reads, u64 *d_results,
+                          hipStream_t stream);
+
 }  // namespace sbwt_amd
