@@ -379,7 +379,7 @@ def mode_serve(args, configs, S, torch, W):
     def reads(seed, n):
         lens = W.uniform_lengths(seed, n, 75, 250)
         c, e, _ = W.reads(seed, codes, ends, lens, random_frac=0.0, err_rate=0.01)
-        return c.tobytes(), e
+        return c, e
 
     runs = []
     for rate in (float(x) for x in args.serve_rates.split(",")):

@@ -613,6 +613,21 @@ int sbwt_server_submit(sbwt_server *server, const char *chars, uint64_t n_chars,
  * are unaffected. */
 int sbwt_server_wait(sbwt_server *server, uint64_t request_id, sbwt_server_result *out);
 int sbwt_server_release(sbwt_server *server, uint64_t request_id);
+/* Load generator over the serving loop, for measurement (no counterpart in
+ * the reference): request i is reads [first_read[i], first_read[i+1]) of one
+ * concatenated batch (chars and their cumulative read_ends; first_read has
+ * n_requests + 1 entries, from 0), submitted without copying from the calling
+ * thread at arrival_s[i] seconds (non-decreasing) after the call starts, while
+ * a second thread waits for the requests in order and releases them. Per
+ * request: latency_s[i] = its end minus its scheduled arrival (time spent
+ * waiting behind a late submitter counts, so the schedule is never
+ * coordinated away), lag_s[i] = its submit minus its scheduled arrival, and
+ * digest[i] (nullable) a 64-bit hash of its k-mer results and color lists.
+ * *wall_s receives the call's duration. Returns the first failed request's
+ * error after all requests have ended. */
+int sbwt_server_replay(sbwt_server *server, const char *chars, const uint64_t *read_ends,
+                       const uint64_t *first_read, uint64_t n_requests, const double *arrival_s,
+                       double *latency_s, double *lag_s, uint64_t *digest, double *wall_s);
 int sbwt_server_get_slots(sbwt_server *server, uint32_t *slots, uint64_t *peak_in_flight);
 void sbwt_server_destroy(sbwt_server *server);
 

@@ -6,7 +6,8 @@ batched: the uneven-length workload (C4) through the plain form -- one launch of
   one batch's tail overlaps the next one's start. Outputs are compared in full.
 colors: the color stage alone on a colored workload (C3 recipe), reads/s.
 serve: a Poisson stream of requests (N log-uniform in [1, max] reads of U[75, 250] bp)
-  through the serving loop vs plain sequential processing; p50/p99 latency, reads/s.
+  replayed by the library's C++ load generator through the serving loop and through
+  plain sequential processing; p50/p99 latency from the scheduled arrival, reads/s.
 """
 import time
 
@@ -129,82 +130,70 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect
 
 def run_serve(S, idx, col, wl_reads, rate_per_s, n_requests, max_reads, seed, params,
               slots=4, log=print):
-    """wl_reads(seed, n_reads) -> (chars, ends). Poisson arrivals at rate_per_s.
-    The same request stream goes through the server with HIP graphs, the server
-    without them, and plain sequential processing on one searcher."""
-    import threading
+    """wl_reads(seed, n_reads) -> (chars uint8 array, ends). Poisson arrivals at
+    rate_per_s, N log-uniform in [1, max_reads] reads per request. The same request
+    stream is replayed by the library's load generator (sbwt_server_replay: a C++
+    submitter on the schedule and a C++ collector, no Python in the loop) through
+      served: the serving loop with `slots` slots and HIP graphs,
+      served_no_graphs: the same without graphs,
+      plain: one slot with eager launches, i.e. one request at a time in arrival
+        order, each one search + color stage (the plain sequential form).
+    Latency runs from each request's scheduled arrival to its end on the server's
+    clock, so time spent behind a late submitter counts. Every request's outputs
+    (k-mer results and color lists) are compared across the three forms by digest."""
     rng = np.random.default_rng(seed)
     sizes = np.exp(rng.uniform(0, np.log(max_reads), n_requests)).astype(np.int64)
     sizes = np.maximum(sizes, 1)
-    gaps = rng.exponential(1.0 / rate_per_s, n_requests)
-    reqs = [wl_reads(seed * 100_000 + i, int(sizes[i])) for i in range(n_requests)]
+    arrival = np.cumsum(rng.exponential(1.0 / rate_per_s, n_requests))
+    arrival -= arrival[0]
+    parts, ends, first = [], [], [0]
+    total_chars = 0
+    for i in range(n_requests):
+        c, e = wl_reads(seed * 100_000 + i, int(sizes[i]))
+        parts.append(np.frombuffer(c, np.uint8) if isinstance(c, bytes) else c)
+        ends.append(e.astype(np.uint64) + np.uint64(total_chars))
+        total_chars += len(parts[-1])
+        first.append(first[-1] + len(e))
+    chars = np.concatenate(parts)
+    read_ends = np.concatenate(ends)
+    first = np.array(first, np.uint64)
     total_reads = int(sizes.sum())
-    max_chars = max(len(c) for c, _ in reqs)
+    req_chars = np.diff(np.concatenate([[0], read_ends[first[1:] - 1].astype(np.int64)]))
+    slot_chars = max(int(req_chars.max()), 1 << 20)
     out = {}
+    digests = {}
 
-    def served(graphs):
-        # submit on the arrival schedule; slots refill as they free up
-        srv = S.Server(idx, col, slots=slots, max_chars_per_slot=max(max_chars, 1 << 20),
+    def replay(name, n_slots, graphs):
+        srv = S.Server(idx, col, slots=n_slots, max_chars_per_slot=slot_chars,
                        max_reads_per_slot=max(max_reads, 1), move_to_key_kmer=col is not None,
                        params=params, graphs=graphs)
-        lat = [None] * n_requests
-        ids = [None] * n_requests
-        done = threading.Event()
-
-        def collector():
-            for i in range(n_requests):
-                while ids[i] is None:
-                    time.sleep(0.0005)
-                r = srv.wait(ids[i])
-                lat[i] = r["end_s"] - r["submit_s"]
-            done.set()
-
-        th = threading.Thread(target=collector)
-        th.start()
-        t0 = time.perf_counter()
-        arrive = t0
-        for i, (c, e) in enumerate(reqs):
-            arrive += gaps[i]
-            while time.perf_counter() < arrive:
-                time.sleep(0.0002)
-            ids[i] = srv.submit(c, e)
-        done.wait()
-        th.join()
-        wall = time.perf_counter() - t0
-        r = {"slots": srv.slots, "peak_in_flight": srv.peak_in_flight(),
-             "p50_ms": 1000 * float(np.percentile(lat, 50)),
-             "p99_ms": 1000 * float(np.percentile(lat, 99)),
-             "reads_per_s": total_reads / wall, "wall_s": wall}
+        # one untimed pass so that first-touch costs (graph capture, page faults)
+        # stay out of the measured replay
+        srv.replay(chars, read_ends, first, np.zeros(n_requests))
+        r = srv.replay(chars, read_ends, first, arrival)
+        lat, lag = r["latency_s"], r["lag_s"]
+        digests[name] = r["digest"]
+        res = {"slots": srv.slots, "peak_in_flight": srv.peak_in_flight(),
+               "p50_ms": 1000 * float(np.percentile(lat, 50)),
+               "p99_ms": 1000 * float(np.percentile(lat, 99)),
+               "max_ms": 1000 * float(lat.max()),
+               "submit_lag_p99_ms": 1000 * float(np.percentile(lag, 99)),
+               "submit_lag_max_ms": 1000 * float(lag.max()),
+               "reads_per_s": total_reads / r["wall_s"], "wall_s": r["wall_s"],
+               "schedule_s": float(arrival[-1])}
         srv.close()
-        return r
+        log(f"{name}: {res}")
+        return res
 
-    out["served"] = served(True)
-    log(f"served: {out['served']}")
-    out["served_no_graphs"] = served(False)
-    log(f"served without graphs: {out['served_no_graphs']}")
-
-    # plain: one request at a time on one searcher, same arrival schedule
-    s = idx.searcher(max(max_chars, 1 << 20), max(max_reads, 1), move_to_key_kmer=col is not None)
-    cs = col.searcher(max(max_chars, 1 << 20), max(max_reads, 1)) if col is not None else None
-    lat2 = []
-    t0 = time.perf_counter()
-    arrive = t0
-    for i, (c, e) in enumerate(reqs):
-        arrive += gaps[i]
-        while time.perf_counter() < arrive:
-            time.sleep(0.0002)
-        res = s.search_reads(c, e)
-        if cs is not None:
-            lens = np.diff(np.concatenate([[0], e.astype(np.int64)]))
-            ke = np.cumsum(np.maximum(lens - idx.k + 1, 0)).astype(np.uint64)
-            cs.pseudoalign(res, ke, params, ids_capacity=max(len(res), 1))
-        lat2.append(time.perf_counter() - arrive)
-    t_plain = time.perf_counter() - t0
-    out["plain"] = {"p50_ms": 1000 * float(np.percentile(lat2, 50)),
-                    "p99_ms": 1000 * float(np.percentile(lat2, 99)),
-                    "reads_per_s": total_reads / t_plain, "wall_s": t_plain}
-    log(f"plain: {out['plain']}")
+    out["served"] = replay("served", slots, True)
+    out["served_no_graphs"] = replay("served_no_graphs", slots, False)
+    out["plain"] = replay("plain", 1, False)
+    for name in ("served", "served_no_graphs"):
+        if not np.array_equal(digests[name], digests["plain"]):
+            raise SystemExit(f"{name} outputs differ from the plain form's")
+    out["outputs_equal"] = True
     out["requests"] = n_requests
     out["reads"] = total_reads
     out["rate_per_s"] = rate_per_s
+    out["latency_from"] = "scheduled arrival"
     return out

@@ -173,6 +173,9 @@ def lib():
         L.sbwt_server_submit.argtypes = [vp, C.c_char_p, C.c_uint64, u64p, C.c_uint64, u64p]
         L.sbwt_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(ServerResult)]
         L.sbwt_server_release.argtypes = [vp, C.c_uint64]
+        f64p = C.POINTER(C.c_double)
+        L.sbwt_server_replay.argtypes = [vp, C.c_char_p, u64p, u64p, C.c_uint64, f64p, f64p,
+                                         f64p, u64p, f64p]
         L.sbwt_server_get_slots.argtypes = [vp, C.POINTER(C.c_uint32), u64p]
         L.sbwt_server_destroy.argtypes = [vp]
         L.sbwt_server_destroy.restype = None
@@ -707,6 +710,28 @@ class Server:
                     "start_s": r.start_s, "end_s": r.end_s}
         finally:
             lib().sbwt_server_release(self.h, rid)
+
+    def replay(self, chars, read_ends, first_read, arrival_s):
+        """Load generator (sbwt_server_replay): request i = reads
+        [first_read[i], first_read[i+1]) of the concatenated batch, submitted at
+        arrival_s[i] seconds; returns per-request latency (end - scheduled
+        arrival), submit lag and output digest, and the wall time."""
+        chars = np.ascontiguousarray(chars, np.uint8)
+        ends = np.ascontiguousarray(read_ends, np.uint64)
+        first = np.ascontiguousarray(first_read, np.uint64)
+        arr = np.ascontiguousarray(arrival_s, np.float64)
+        n = len(arr)
+        assert len(first) == n + 1
+        lat = np.zeros(n, np.float64)
+        lag = np.zeros(n, np.float64)
+        dig = np.zeros(n, np.uint64)
+        wall = C.c_double()
+        f64p = C.POINTER(C.c_double)
+        _check(lib().sbwt_server_replay(
+            self.h, chars.ctypes.data_as(C.c_char_p), _p64(ends) if len(ends) else None,
+            _p64(first), n, arr.ctypes.data_as(f64p), lat.ctypes.data_as(f64p),
+            lag.ctypes.data_as(f64p), _p64(dig), C.byref(wall)))
+        return {"latency_s": lat, "lag_s": lag, "digest": dig, "wall_s": wall.value}
 
     def peak_in_flight(self):
         n, p = C.c_uint32(), C.c_uint64()

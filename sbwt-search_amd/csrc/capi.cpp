@@ -868,13 +868,29 @@ int sbwt_server_wait(sbwt_server *server, uint64_t request_id, sbwt_server_resul
     auto r = server->impl->wait(request_id);
     out->n_kmers = r->kmer_results.size();
     out->kmer_results = r->kmer_results.data();
-    out->n_reads = r->read_ends.size();
+    out->n_reads = r->n_reads;
     out->color_offsets = r->color_offsets.empty() ? nullptr : r->color_offsets.data();
     out->color_ids = r->color_ids.data();
     out->submit_s = r->submit_s;
     out->start_s = r->start_s;
     out->end_s = r->end_s;
     if (!r->error.empty()) throw std::runtime_error(r->error);
+  });
+}
+
+int sbwt_server_replay(sbwt_server *server, const char *chars, const uint64_t *read_ends,
+                       const uint64_t *first_read, uint64_t n_requests, const double *arrival_s,
+                       double *latency_s, double *lag_s, uint64_t *digest, double *wall_s) {
+  return guarded([&] {
+    require(server && first_read && arrival_s && latency_s && lag_s && wall_s, "null argument");
+    require(first_read[0] == 0, "first_read must start at 0");
+    for (uint64_t i = 0; i < n_requests; ++i) {
+      require(first_read[i + 1] >= first_read[i], "first_read must be non-decreasing");
+      require(i == 0 || arrival_s[i] >= arrival_s[i - 1], "arrival times must be non-decreasing");
+    }
+    require(first_read[n_requests] == 0 || (chars && read_ends), "null argument");
+    server->impl->replay(chars, read_ends, first_read, n_requests, arrival_s, latency_s, lag_s,
+                         digest, wall_s);
   });
 }
 

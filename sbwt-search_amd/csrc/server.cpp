@@ -36,11 +36,13 @@ Server::Server(const DeviceIndex *index, const DeviceColors *colors, const Serve
     }
     SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_chars), C + 64, hipHostMallocDefault));
     SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_ends), R * 8, hipHostMallocDefault));
-    if (cfg_.graphs) {
-      const u64 gc = std::min<u64>(C, kGraphMaxChars), gr = std::min<u64>(R, kGraphMaxChars);
-      SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_results), gc * 8 + 8, hipHostMallocDefault));
-      SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_offsets), (gr + 1) * 8, hipHostMallocDefault));
-    }
+    // pinned landing buffers for everything that comes back (copies from
+    // device memory into pageable memory would go through a bounce buffer)
+    SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_results), C * 8 + 8, hipHostMallocDefault));
+    SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_offsets), (R + 1) * 8, hipHostMallocDefault));
+    if (colors_)
+      SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_ids), std::max<u64>(ids_cap, 1) * 4,
+                             hipHostMallocDefault));
     SBWT_HIP(hipMalloc(&s->d_chars, C + 64));
     SBWT_HIP(hipMalloc(&s->d_ends, R * 8));
     SBWT_HIP(hipMalloc(&s->d_results, (C + 1) * 8));
@@ -75,6 +77,7 @@ Server::~Server() {
     if (s->h_ends) (void)hipHostFree(s->h_ends);
     if (s->h_results) (void)hipHostFree(s->h_results);
     if (s->h_offsets) (void)hipHostFree(s->h_offsets);
+    if (s->h_ids) (void)hipHostFree(s->h_ids);
     if (s->stream) (void)hipStreamDestroy(s->stream);
   }
 }
@@ -87,6 +90,25 @@ u64 Server::submit(const char *chars, u64 n_chars, const u64 *read_ends, u64 n_r
   auto r = std::make_shared<Request>();
   r->chars.assign(chars, chars + n_chars);
   r->read_ends.assign(read_ends, read_ends + n_reads);
+  r->in_chars = r->chars.data();
+  r->in_ends = r->read_ends.data();
+  r->n_chars = n_chars;
+  r->n_reads = n_reads;
+  return enqueue(std::move(r));
+}
+
+u64 Server::submit_borrowed(const char *chars, u64 n_chars, const u64 *read_ends, u64 n_reads,
+                            u64 ends_base) {
+  auto r = std::make_shared<Request>();
+  r->in_chars = chars;
+  r->in_ends = read_ends;
+  r->n_chars = n_chars;
+  r->n_reads = n_reads;
+  r->ends_base = ends_base;
+  return enqueue(std::move(r));
+}
+
+u64 Server::enqueue(std::shared_ptr<Request> r) {
   std::lock_guard<std::mutex> g(mu_);
   r->id = next_id_++;
   r->submit_s = now();
@@ -141,20 +163,22 @@ void Server::run_slot(Slot &s) {
 // One request through one slot: batches of whole reads up to the slot's
 // capacity, each packed, searched (and color-searched) on the slot's stream.
 void Server::process(Slot &s, Request &r) {
-  const u64 n_reads = r.read_ends.size();
-  const u64 n_chars = r.chars.size();
+  const u64 n_reads = r.n_reads;
+  const u64 n_chars = r.n_chars;
   const u64 k = index_->k;
+  // read end i within the request's chars (validated below before any use)
+  const auto end_of = [&](u64 i) { return r.in_ends[i] - r.ends_base; };
   u64 prev = 0, total_kmers = 0;
   for (u64 i = 0; i < n_reads; ++i) {
-    if (r.read_ends[i] < prev || r.read_ends[i] > n_chars)
+    if (r.in_ends[i] < r.ends_base || end_of(i) < prev || end_of(i) > n_chars)
       throw std::runtime_error("read_ends must be non-decreasing and end at n_chars");
-    if (r.read_ends[i] - prev > cfg_.max_chars_per_slot)
+    if (end_of(i) - prev > cfg_.max_chars_per_slot)
       throw std::runtime_error("a read is longer than the slot capacity");
-    const u64 len = r.read_ends[i] - prev;
+    const u64 len = end_of(i) - prev;
     total_kmers += len >= k ? len - k + 1 : 0;
-    prev = r.read_ends[i];
+    prev = end_of(i);
   }
-  if (n_reads && r.read_ends[n_reads - 1] != n_chars) throw std::runtime_error("read_ends must end at n_chars");
+  if (n_reads && end_of(n_reads - 1) != n_chars) throw std::runtime_error("read_ends must end at n_chars");
   r.kmer_results.resize(total_kmers);
   if (colors_) {
     r.color_offsets.assign(n_reads + 1, 0);
@@ -163,14 +187,14 @@ void Server::process(Slot &s, Request &r) {
   u64 read0 = 0, kmer_out = 0;
   while (read0 < n_reads) {
     // whole reads up to the slot capacity
-    const u64 c0 = read0 ? r.read_ends[read0 - 1] : 0;
+    const u64 c0 = read0 ? end_of(read0 - 1) : 0;
     u64 read1 = read0;
     while (read1 < n_reads && read1 - read0 < cfg_.max_reads_per_slot &&
-           r.read_ends[read1] - c0 <= cfg_.max_chars_per_slot)
+           end_of(read1) - c0 <= cfg_.max_chars_per_slot)
       ++read1;
-    const u64 nr = read1 - read0, nc = r.read_ends[read1 - 1] - c0;
-    std::memcpy(s.h_chars, r.chars.data() + c0, nc);
-    for (u64 i = 0; i < nr; ++i) s.h_ends[i] = r.read_ends[read0 + i] - c0;
+    const u64 nr = read1 - read0, nc = end_of(read1 - 1) - c0;
+    std::memcpy(s.h_chars, r.in_chars + c0, nc);
+    for (u64 i = 0; i < nr; ++i) s.h_ends[i] = end_of(read0 + i) - c0;
     // power-of-two bucket of the batch: small batches replay a captured graph
     u64 cb = kGraphMinChars, rb = kGraphMinReads;
     while (cb < nc) cb <<= 1;
@@ -195,23 +219,22 @@ u64 Server::run_batch(Slot &s, Request &r, u64 nc, u64 nr, u64 read0, u64 kmer_o
   u64 nk = 0;
   s.searcher->search_reads_device(s.d_chars, nc, s.d_ends, nr, s.d_results, &nk);
   if (nk)
-    SBWT_HIP(hipMemcpyAsync(r.kmer_results.data() + kmer_out, s.d_results, nk * 8,
-                            hipMemcpyDeviceToHost, s.stream));
+    SBWT_HIP(hipMemcpyAsync(s.h_results, s.d_results, nk * 8, hipMemcpyDeviceToHost, s.stream));
+  u64 n_ids = 0;
   if (colors_) {
-    const u64 n_ids = s.colors->pseudoalign_device(s.d_results, s.searcher->kmer_ends_device(), nr,
-                                                   cfg_.color, s.d_offsets, s.d_ids,
-                                                   s.ids_capacity, nullptr);
-    std::vector<u64> off(nr + 1);
+    n_ids = s.colors->pseudoalign_device(s.d_results, s.searcher->kmer_ends_device(), nr, cfg_.color,
+                                         s.d_offsets, s.d_ids, s.ids_capacity, nullptr);
+    SBWT_HIP(hipMemcpyAsync(s.h_offsets, s.d_offsets, (nr + 1) * 8, hipMemcpyDeviceToHost, s.stream));
+    if (n_ids)
+      SBWT_HIP(hipMemcpyAsync(s.h_ids, s.d_ids, n_ids * 4, hipMemcpyDeviceToHost, s.stream));
+  }
+  SBWT_HIP(hipStreamSynchronize(s.stream));
+  if (nk) std::memcpy(r.kmer_results.data() + kmer_out, s.h_results, nk * 8);
+  if (colors_) {
     const u64 base = r.color_ids.size();
     r.color_ids.resize(base + n_ids);
-    SBWT_HIP(hipMemcpyAsync(off.data(), s.d_offsets, (nr + 1) * 8, hipMemcpyDeviceToHost, s.stream));
-    if (n_ids)
-      SBWT_HIP(hipMemcpyAsync(r.color_ids.data() + base, s.d_ids, n_ids * 4, hipMemcpyDeviceToHost,
-                              s.stream));
-    SBWT_HIP(hipStreamSynchronize(s.stream));
-    for (u64 i = 0; i < nr; ++i) r.color_offsets[read0 + i + 1] = base + off[i + 1];
-  } else {
-    SBWT_HIP(hipStreamSynchronize(s.stream));
+    if (n_ids) std::memcpy(r.color_ids.data() + base, s.h_ids, n_ids * 4);
+    for (u64 i = 0; i < nr; ++i) r.color_offsets[read0 + i + 1] = base + s.h_offsets[i + 1];
   }
   return nk;
 }
@@ -239,13 +262,104 @@ u64 Server::run_batch_graph(Slot &s, Request &r, u64 nc, u64 nr, u64 read0, u64 
     const u64 base = r.color_ids.size();
     r.color_ids.resize(base + n_ids);
     if (n_ids) {
-      SBWT_HIP(hipMemcpyAsync(r.color_ids.data() + base, s.d_ids, n_ids * 4, hipMemcpyDeviceToHost,
-                              s.stream));
+      SBWT_HIP(hipMemcpyAsync(s.h_ids, s.d_ids, n_ids * 4, hipMemcpyDeviceToHost, s.stream));
       SBWT_HIP(hipStreamSynchronize(s.stream));
+      std::memcpy(r.color_ids.data() + base, s.h_ids, n_ids * 4);
     }
     for (u64 i = 0; i < nr; ++i) r.color_offsets[read0 + i + 1] = base + s.h_offsets[i + 1];
   }
   return nk;
+}
+
+namespace {
+
+// Word-wise 64-bit hash of a request's outputs (multiply-xor per word, then
+// a final avalanche); equal outputs give equal digests.
+struct Digest {
+  u64 h = 0x9E3779B97F4A7C15ULL;
+  void add(u64 w) {
+    h ^= w;
+    h *= 0xFF51AFD7ED558CCDULL;
+    h ^= h >> 29;
+  }
+  u64 done() const {
+    u64 x = h ^ (h >> 33);
+    x *= 0xC4CEB9FE1A85EC53ULL;
+    return x ^ (x >> 33);
+  }
+};
+
+}  // namespace
+
+void Server::replay(const char *chars, const u64 *read_ends, const u64 *first_read, u64 n_requests,
+                    const double *arrival_s, double *latency_s, double *lag_s, u64 *digest,
+                    double *wall_s) {
+  using clock = std::chrono::steady_clock;
+  std::vector<u64> ids(n_requests, 0);
+  u64 submitted = 0, limit = n_requests;  // limit: requests that will be submitted
+  std::mutex m;
+  std::condition_variable cv;
+  std::string first_error;
+  const clock::time_point t0 = clock::now();
+  const auto at = [&](double sec) {
+    return t0 + std::chrono::duration_cast<clock::duration>(std::chrono::duration<double>(sec));
+  };
+  std::thread waiter([&] {
+    for (u64 i = 0; i < n_requests; ++i) {
+      u64 id = 0;
+      {
+        std::unique_lock<std::mutex> g(m);
+        cv.wait(g, [&] { return submitted > i || limit <= i; });
+        if (submitted <= i) break;
+        id = ids[i];
+      }
+      auto r = wait(id);
+      const clock::time_point end =
+          t0_ + std::chrono::duration_cast<clock::duration>(std::chrono::duration<double>(r->end_s));
+      latency_s[i] = std::chrono::duration<double>(end - at(arrival_s[i])).count();
+      if (!r->error.empty() && first_error.empty()) first_error = r->error;
+      if (digest) {
+        Digest d;
+        for (u64 v : r->kmer_results) d.add(v);
+        for (u64 v : r->color_offsets) d.add(v);
+        for (u32 v : r->color_ids) d.add(v);
+        digest[i] = d.done();
+      }
+      release(id);
+    }
+  });
+  std::string submit_error;
+  try {
+    for (u64 i = 0; i < n_requests; ++i) {
+      const clock::time_point target = at(arrival_s[i]);
+      // sleep to just before the arrival, then spin to it
+      std::this_thread::sleep_until(target - std::chrono::microseconds(200));
+      while (clock::now() < target) {
+      }
+      const u64 a = first_read[i], b = first_read[i + 1];
+      const u64 base = a ? read_ends[a - 1] : 0;
+      const u64 nc = (b > a ? read_ends[b - 1] : base) - base;
+      const u64 id = submit_borrowed(chars + base, nc, read_ends + a, b - a, base);
+      lag_s[i] = std::chrono::duration<double>(clock::now() - target).count();
+      {
+        std::lock_guard<std::mutex> g(m);
+        ids[i] = id;
+        submitted = i + 1;
+      }
+      cv.notify_one();
+    }
+  } catch (const std::exception &e) {
+    submit_error = e.what();
+  }
+  {
+    std::lock_guard<std::mutex> g(m);
+    limit = submitted;
+  }
+  cv.notify_one();
+  waiter.join();
+  if (!submit_error.empty()) throw std::runtime_error(submit_error);
+  *wall_s = std::chrono::duration<double>(clock::now() - t0).count();
+  if (!first_error.empty()) throw std::runtime_error(first_error);
 }
 
 // Captures, once per bucket, search algorithm and work-item size, the staging copies, the search, the color stage

@@ -41,6 +41,12 @@ constexpr u64 kGraphMinChars = 1 << 12, kGraphMinReads = 1 << 6;
 
 struct Request {
   u64 id = 0;
+  // the request's reads: in_chars[0, n_chars), read ends in_ends[i] - ends_base
+  // (either the request's own copies below, or caller memory that outlives the
+  // request, see submit_borrowed)
+  const char *in_chars = nullptr;
+  const u64 *in_ends = nullptr;
+  u64 n_chars = 0, n_reads = 0, ends_base = 0;
   std::vector<char> chars;
   std::vector<u64> read_ends;
   // results
@@ -57,6 +63,23 @@ class Server {
   Server(const DeviceIndex *index, const DeviceColors *colors, const ServerConfig &cfg);
   ~Server();
   u64 submit(const char *chars, u64 n_chars, const u64 *read_ends, u64 n_reads);
+  // The same without copying: chars and read_ends (cumulative, each minus
+  // ends_base is the end within chars) stay valid until the request is released.
+  u64 submit_borrowed(const char *chars, u64 n_chars, const u64 *read_ends, u64 n_reads,
+                      u64 ends_base);
+  // The clock origin of submit_s / start_s / end_s.
+  std::chrono::steady_clock::time_point clock_origin() const { return t0_; }
+  // Load generator (measurement): request i = reads [first_read[i],
+  // first_read[i + 1]) of one concatenated batch (chars, cumulative read_ends),
+  // submitted without copies from this thread at arrival_s[i] seconds after the
+  // call starts while another thread waits for the requests in order. Per
+  // request: latency_s = end - scheduled arrival (time spent behind a late
+  // submitter counts), lag_s = submit - scheduled arrival, digest (nullable) =
+  // a 64-bit hash of its k-mer results and color lists. wall_s = the call's
+  // duration. Throws the first failed request's error after all have ended.
+  void replay(const char *chars, const u64 *read_ends, const u64 *first_read, u64 n_requests,
+              const double *arrival_s, double *latency_s, double *lag_s, u64 *digest,
+              double *wall_s);
   // Blocks until request id is done; the request stays until release().
   std::shared_ptr<Request> wait(u64 id);
   void release(u64 id);
@@ -76,8 +99,9 @@ class Server {
     u64 *d_offsets = nullptr;
     u32 *d_ids = nullptr;
     u64 ids_capacity = 0;
-    u64 *h_results = nullptr;  // pinned, graph batches
+    u64 *h_results = nullptr;  // pinned: results, color offsets and ids come back here
     u64 *h_offsets = nullptr;
+    u32 *h_ids = nullptr;
     // by (chars, reads) bucket, search algorithm and work-item size
     std::map<std::tuple<u64, u64, int, u32>, hipGraphExec_t> graphs;
     std::thread worker;
@@ -92,6 +116,7 @@ class Server {
                       u64 rb);
   hipGraphExec_t graph_for(Slot &s, u64 cb, u64 rb);
   double now() const;
+  u64 enqueue(std::shared_ptr<Request> r);
 
   const DeviceIndex *index_;
   const DeviceColors *colors_;

@@ -94,3 +94,38 @@ def test_graph_replays_equal_eager(served):
             assert g["color_lists"] == e["color_lists"], rep
             assert np.array_equal(g["kmers"], ci.index_results(chars, ends))
             other.search_reads(chars, ends)  # eager device work between replays
+
+
+def test_replay_load_generator(served):
+    """sbwt_server_replay: requests submitted on a schedule from C++; outputs of the
+    served form equal the one-slot plain form's request by request (digests), and
+    latency runs from the scheduled arrival."""
+    import modes
+    ci, idx, col = served
+    say("serving: replay load generator, served vs plain")
+    import workloads as W
+
+    def reads(seed, n):
+        lens = W.uniform_lengths(seed, n, 75, 250)
+        c, e, _ = W.reads(seed, ci.codes, ci.ends, lens, random_frac=0.05, err_rate=0.01)
+        return c, e
+
+    out = modes.run_serve(S, idx, col, reads, 400.0, 40, 3000, 3, S.color_params(0.7),
+                          slots=4, log=say)
+    assert out["outputs_equal"]
+    for form in ("served", "served_no_graphs", "plain"):
+        r = out[form]
+        assert 0 <= r["p50_ms"] <= r["p99_ms"] <= r["max_ms"]
+        assert r["submit_lag_max_ms"] >= 0 and r["reads_per_s"] > 0
+    assert out["plain"]["slots"] == 1 and out["served"]["slots"] >= 2
+    # one request's digest matches its own search alone
+    srv = S.Server(idx, col, slots=2, max_chars_per_slot=1 << 20, max_reads_per_slot=3000,
+                   move_to_key_kmer=True, params=S.color_params(0.7))
+    c, e = reads(77, 500)
+    r1 = srv.replay(c, e, np.array([0, len(e)], np.uint64), np.zeros(1))
+    r2 = srv.replay(c, e, np.array([0, len(e)], np.uint64), np.zeros(1))
+    assert r1["digest"][0] == r2["digest"][0] != 0
+    rid = srv.submit(c.tobytes(), e)
+    g = srv.wait(rid)
+    assert np.array_equal(g["kmers"], ci.index_results(c.tobytes(), e))
+    srv.close()
