@@ -383,8 +383,10 @@ def mode_serve(args, configs, S, torch, W):
 
     runs = []
     for rate in (float(x) for x in args.serve_rates.split(",")):
-        runs.append(modes.run_serve(S, idx, cidx, reads, rate, args.serve_requests,
-                                    args.serve_max_reads, 5, S.color_params(0.7), log=log))
+        for kr in (False, True):
+            runs.append(modes.run_serve(S, idx, cidx, reads, rate, args.serve_requests,
+                                        args.serve_max_reads, 5, S.color_params(0.7), log=log,
+                                        kmer_results=kr))
     print(json.dumps({"metric": "p50/p99 request latency + reads/s, served vs plain",
                       "mode": "serve", "config": {
                           "index": f"{args.serve_genomes} genomes x {args.serve_length} bp from "

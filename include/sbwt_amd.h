@@ -592,6 +592,8 @@ typedef struct {
   int move_to_key_kmer;
   sbwt_color_params color;   /* used when a color index is attached */
   int no_graphs;
+  int no_kmer_results;  /* with a color index: return only color lists (n_kmers = 0),
+                           not the 8 bytes per k-mer of index results */
 } sbwt_server_config;
 
 typedef struct {

@@ -129,7 +129,7 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect
 
 
 def run_serve(S, idx, col, wl_reads, rate_per_s, n_requests, max_reads, seed, params,
-              slots=4, log=print):
+              slots=4, log=print, kmer_results=True):
     """wl_reads(seed, n_reads) -> (chars uint8 array, ends). Poisson arrivals at
     rate_per_s, N log-uniform in [1, max_reads] reads per request. The same request
     stream is replayed by the library's load generator (sbwt_server_replay: a C++
@@ -138,7 +138,8 @@ def run_serve(S, idx, col, wl_reads, rate_per_s, n_requests, max_reads, seed, pa
       served_no_graphs: the same without graphs,
       plain: one slot with eager launches, i.e. one request at a time in arrival
         order, each one search + color stage (the plain sequential form).
-    Latency runs from each request's scheduled arrival to its end on the server's
+    With kmer_results off (and a color index) requests return their color lists
+    only. Latency runs from each request's scheduled arrival to its end on the server's
     clock, so time spent behind a late submitter counts. Every request's outputs
     (k-mer results and color lists) are compared across the three forms by digest."""
     rng = np.random.default_rng(seed)
@@ -166,7 +167,7 @@ def run_serve(S, idx, col, wl_reads, rate_per_s, n_requests, max_reads, seed, pa
     def replay(name, n_slots, graphs):
         srv = S.Server(idx, col, slots=n_slots, max_chars_per_slot=slot_chars,
                        max_reads_per_slot=max(max_reads, 1), move_to_key_kmer=col is not None,
-                       params=params, graphs=graphs)
+                       params=params, graphs=graphs, kmer_results=kmer_results)
         # one untimed pass so that first-touch costs (graph capture, page faults)
         # stay out of the measured replay
         srv.replay(chars, read_ends, first, np.zeros(n_requests))
@@ -196,4 +197,6 @@ def run_serve(S, idx, col, wl_reads, rate_per_s, n_requests, max_reads, seed, pa
     out["reads"] = total_reads
     out["rate_per_s"] = rate_per_s
     out["latency_from"] = "scheduled arrival"
+    out["returns"] = "k-mer results and color lists" if kmer_results or col is None else \
+        "color lists"
     return out

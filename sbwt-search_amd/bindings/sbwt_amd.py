@@ -48,7 +48,8 @@ class ColorParams(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("slots", C.c_uint32), ("max_chars_per_slot", C.c_uint64),
                 ("max_reads_per_slot", C.c_uint64), ("memory_fraction", C.c_double),
-                ("move_to_key_kmer", C.c_int), ("color", ColorParams), ("no_graphs", C.c_int)]
+                ("move_to_key_kmer", C.c_int), ("color", ColorParams), ("no_graphs", C.c_int),
+                ("no_kmer_results", C.c_int)]
 
 
 class ServerResult(C.Structure):
@@ -671,10 +672,11 @@ class Server:
 
     def __init__(self, index, colors=None, slots=4, max_chars_per_slot=1 << 25,
                  max_reads_per_slot=1 << 20, memory_fraction=0.9, move_to_key_kmer=False,
-                 params=None, graphs=True):
+                 params=None, graphs=True, kmer_results=True):
         self.index, self.colors = index, colors
         cfg = ServerConfig(slots, max_chars_per_slot, max_reads_per_slot, memory_fraction,
-                           int(move_to_key_kmer), params or color_params(), int(not graphs))
+                           int(move_to_key_kmer), params or color_params(), int(not graphs),
+                           int(not kmer_results))
         self.h = C.c_void_p()
         _check(lib().sbwt_server_create(index.h, colors.h if colors else None, C.byref(cfg),
                                         C.byref(self.h)))

@@ -846,6 +846,7 @@ int sbwt_server_create(sbwt_index *index, sbwt_color_index *colors,
     c.memory_fraction = config->memory_fraction;
     c.move_to_key_kmer = config->move_to_key_kmer != 0;
     c.graphs = config->no_graphs == 0;
+    c.kmer_results = config->no_kmer_results == 0 || colors == nullptr;
     if (colors) c.color = to_params(&config->color);
     auto s = std::make_unique<sbwt_server>();
     s->impl = std::make_unique<sbwt_amd::Server>(&index->dev, colors ? &colors->dev : nullptr, c);

@@ -38,7 +38,8 @@ Server::Server(const DeviceIndex *index, const DeviceColors *colors, const Serve
     SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_ends), R * 8, hipHostMallocDefault));
     // pinned landing buffers for everything that comes back (copies from
     // device memory into pageable memory would go through a bounce buffer)
-    SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_results), C * 8 + 8, hipHostMallocDefault));
+    if (cfg_.kmer_results)
+      SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_results), C * 8 + 8, hipHostMallocDefault));
     SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_offsets), (R + 1) * 8, hipHostMallocDefault));
     if (colors_)
       SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_ids), std::max<u64>(ids_cap, 1) * 4,
@@ -179,7 +180,7 @@ void Server::process(Slot &s, Request &r) {
     prev = end_of(i);
   }
   if (n_reads && end_of(n_reads - 1) != n_chars) throw std::runtime_error("read_ends must end at n_chars");
-  r.kmer_results.resize(total_kmers);
+  if (cfg_.kmer_results) r.kmer_results.resize(total_kmers);
   if (colors_) {
     r.color_offsets.assign(n_reads + 1, 0);
     r.color_ids.clear();
@@ -218,7 +219,7 @@ u64 Server::run_batch(Slot &s, Request &r, u64 nc, u64 nr, u64 read0, u64 kmer_o
   SBWT_HIP(hipMemcpyAsync(s.d_ends, s.h_ends, nr * 8, hipMemcpyHostToDevice, s.stream));
   u64 nk = 0;
   s.searcher->search_reads_device(s.d_chars, nc, s.d_ends, nr, s.d_results, &nk);
-  if (nk)
+  if (nk && cfg_.kmer_results)
     SBWT_HIP(hipMemcpyAsync(s.h_results, s.d_results, nk * 8, hipMemcpyDeviceToHost, s.stream));
   u64 n_ids = 0;
   if (colors_) {
@@ -229,7 +230,7 @@ u64 Server::run_batch(Slot &s, Request &r, u64 nc, u64 nr, u64 read0, u64 kmer_o
       SBWT_HIP(hipMemcpyAsync(s.h_ids, s.d_ids, n_ids * 4, hipMemcpyDeviceToHost, s.stream));
   }
   SBWT_HIP(hipStreamSynchronize(s.stream));
-  if (nk) std::memcpy(r.kmer_results.data() + kmer_out, s.h_results, nk * 8);
+  if (nk && cfg_.kmer_results) std::memcpy(r.kmer_results.data() + kmer_out, s.h_results, nk * 8);
   if (colors_) {
     const u64 base = r.color_ids.size();
     r.color_ids.resize(base + n_ids);
@@ -253,7 +254,7 @@ u64 Server::run_batch_graph(Slot &s, Request &r, u64 nc, u64 nr, u64 read0, u64 
   }
   SBWT_HIP(hipGraphLaunch(graph_for(s, cb, rb), s.stream));
   SBWT_HIP(hipStreamSynchronize(s.stream));
-  if (nk) std::memcpy(r.kmer_results.data() + kmer_out, s.h_results, nk * 8);
+  if (nk && cfg_.kmer_results) std::memcpy(r.kmer_results.data() + kmer_out, s.h_results, nk * 8);
   if (colors_) {
     const u64 n_ids = s.h_offsets[nr];
     if (n_ids > s.ids_capacity)
@@ -375,7 +376,8 @@ hipGraphExec_t Server::graph_for(Slot &s, u64 cb, u64 rb) {
     SBWT_HIP(hipMemcpyAsync(s.d_chars, s.h_chars, cb, hipMemcpyHostToDevice, s.stream));
     SBWT_HIP(hipMemcpyAsync(s.d_ends, s.h_ends, rb * 8, hipMemcpyHostToDevice, s.stream));
     s.searcher->search_reads_device(s.d_chars, cb, s.d_ends, rb, s.d_results, nullptr);
-    SBWT_HIP(hipMemcpyAsync(s.h_results, s.d_results, cb * 8, hipMemcpyDeviceToHost, s.stream));
+    if (cfg_.kmer_results)
+      SBWT_HIP(hipMemcpyAsync(s.h_results, s.d_results, cb * 8, hipMemcpyDeviceToHost, s.stream));
     if (colors_) {
       s.colors->pseudoalign_device_async(s.d_results, s.searcher->kmer_ends_device(), rb, cfg_.color,
                                          s.d_offsets, s.d_ids, s.ids_capacity, nullptr);

@@ -34,6 +34,9 @@ struct ServerConfig {
   // (chars, reads) power-of-two bucket and slot: copies in, search, color
   // stage and copies out in one launch.
   bool graphs = true;
+  // Copy every k-mer's index result back to the request (8 bytes per k-mer);
+  // off, a request with a color index returns only its color lists.
+  bool kmer_results = true;
 };
 
 constexpr u64 kGraphMaxChars = 1 << 20;

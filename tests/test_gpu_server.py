@@ -118,6 +118,10 @@ def test_replay_load_generator(served):
         assert 0 <= r["p50_ms"] <= r["p99_ms"] <= r["max_ms"]
         assert r["submit_lag_max_ms"] >= 0 and r["reads_per_s"] > 0
     assert out["plain"]["slots"] == 1 and out["served"]["slots"] >= 2
+    # color lists only: the forms still agree request by request
+    out2 = modes.run_serve(S, idx, col, reads, 400.0, 20, 2000, 4, S.color_params(0.7),
+                           slots=4, log=say, kmer_results=False)
+    assert out2["outputs_equal"] and out2["returns"] == "color lists"
     # one request's digest matches its own search alone
     srv = S.Server(idx, col, slots=2, max_chars_per_slot=1 << 20, max_reads_per_slot=3000,
                    move_to_key_kmer=True, params=S.color_params(0.7))
