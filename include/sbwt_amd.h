@@ -397,6 +397,16 @@ int sbwt_color_searcher_counts_device(sbwt_color_searcher *searcher,
                                       const uint64_t *d_read_kmer_ends, uint64_t n_reads,
                                       uint32_t *d_counts);
 
+/* Work distribution of the fused color search (sbwt_color_searcher_pseudoalign*):
+ * item_kmers = 0 (default) gives one wave per read; otherwise the batch runs
+ * over a device queue of (read, k-mer range) work items of item_kmers k-mers
+ * (at least 64; lengthened when the partial-count slots of long reads would
+ * take more than a 32nd of device memory), so reads of very uneven length
+ * spread over many waves. Results are identical. No counterpart in the
+ * reference (its color kernel takes one thread per index,
+ * src/ColorSearcher/ColorSearcher.cuh:96-205). */
+int sbwt_color_searcher_set_work_items(sbwt_color_searcher *searcher, uint32_t item_kmers);
+
 /* Device time (ms) of the last call's color kernel. */
 int sbwt_color_searcher_last_time(sbwt_color_searcher *searcher, float *ms);
 

@@ -29,18 +29,22 @@ def _split_batches(read_ends, max_chars):
     return out
 
 
-def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect=False):
+def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect=False,
+                item_kmers=1024):
     """C4: the same streaming search + color stage in two forms, on the same batches.
 
     plain: one launch of each stage per batch on one stream, batch after batch,
-      each batch's color ids copied back before the next starts (the reference's
-      one-launch-per-batch form, with this library's kernels).
-    batched: the batches spread over two streams, so one batch's tail overlaps
-      the next one's start.
+      each batch's color ids copied back before the next starts, the color stage
+      one wave per read (the reference's one-launch-per-batch form, with this
+      library's kernels).
+    batched: the color stage over a device queue of (read, k-mer range) work
+      items of item_kmers k-mers, so long reads spread over many waves, and the
+      batches spread over two streams, so one batch's tail overlaps the next's
+      start.
 
-    Both forms run the same algorithm (streaming search with key-k-mer chains,
-    then the color stage); their outputs (per batch: CSR offsets and color ids)
-    are compared in full. With collect, the outputs are returned too."""
+    Both forms run the same search algorithm (streaming with key-k-mer chains)
+    and the same color decode; their outputs (per batch: CSR offsets and color
+    ids) are compared in full. With collect, the outputs are returned too."""
     dev = torch.device("cuda:0")
     d_chars = torch.from_numpy(wl.chars).to(dev)
     ends64 = wl.read_ends.astype(np.int64)
@@ -55,11 +59,12 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect
     nc = col.num_colors
     params = S.color_params(tau)
 
-    def make_lane():
+    def make_lane(items):
         stream = torch.cuda.Stream()
         s = idx.searcher(max_chars, max_reads, move_to_key_kmer=True, stream=stream.cuda_stream)
         s.set_algorithm(S.ALGO_STREAMING)
         cs = col.searcher(max_chars, max_reads, stream=stream.cuda_stream)
+        cs.set_work_items(items)
         res = torch.empty(max_chars, dtype=torch.int64, device=dev)
         off = torch.empty(max_reads + 1, dtype=torch.int64, device=dev)
         cap = max_reads * nc
@@ -83,7 +88,7 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect
         return off.numpy().view(np.uint64), ids.numpy().view(np.uint32)
 
     # plain: one stream, batch after batch
-    plain = make_lane()
+    plain = make_lane(0)
     out_plain = [None] * len(batches)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -94,7 +99,7 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect
     log(f"plain: {len(batches)} batches in {t_plain:.3f} s")
 
     # batched: two streams in flight
-    lanes = [make_lane(), make_lane()]
+    lanes = [make_lane(item_kmers), make_lane(item_kmers)]
     import threading
     out_b = [None] * len(batches)
 
@@ -119,6 +124,7 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect
     n_ids = sum(len(o[1]) for o in out_plain)
     r = {"reads": wl.n_reads, "batches": len(batches), "batch_chars": batch_chars,
          "algorithm": "streaming (both forms)", "color_ids": n_ids,
+         "color_work_items": {"plain": "one wave per read", "batched": f"{item_kmers} k-mers"},
          "plain_s": t_plain, "batched_s": t_batched,
          "plain_reads_per_s": wl.n_reads / t_plain,
          "batched_reads_per_s": wl.n_reads / t_batched,

@@ -123,6 +123,7 @@ def lib():
                                                              C.POINTER(ColorParams), vp, vp,
                                                              C.c_uint64, vp, u64p]
         L.sbwt_color_searcher_counts_device.argtypes = [vp, vp, vp, C.c_uint64, vp]
+        L.sbwt_color_searcher_set_work_items.argtypes = [vp, C.c_uint32]
         L.sbwt_color_searcher_last_time.argtypes = [vp, C.POINTER(C.c_float)]
         L.sbwt_index_file_parser_create.argtypes = [C.c_uint64, C.c_uint64, vp, C.POINTER(vp)]
         L.sbwt_index_file_parser_destroy.argtypes = [vp]
@@ -655,6 +656,10 @@ class ColorSearcher:
             C.c_void_p(d_offsets), C.c_void_p(d_ids), ids_capacity,
             C.c_void_p(d_stats) if d_stats else None, C.byref(nids)))
         return nids.value
+
+    def set_work_items(self, item_kmers):
+        """0: one wave per read; else a queue of (read, k-mer range) items."""
+        _check(lib().sbwt_color_searcher_set_work_items(self.h, item_kmers))
 
     def counts_device(self, d_results, d_kmer_ends, n_reads, d_counts):
         _check(lib().sbwt_color_searcher_counts_device(self.h, C.c_void_p(d_results),

@@ -540,6 +540,13 @@ int sbwt_color_searcher_pseudoalign(sbwt_color_searcher *searcher, const uint64_
   });
 }
 
+int sbwt_color_searcher_set_work_items(sbwt_color_searcher *searcher, uint32_t item_kmers) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    searcher->impl->set_work_items(item_kmers);
+  });
+}
+
 int sbwt_color_searcher_counts_device(sbwt_color_searcher *searcher,
                                       const uint64_t *d_kmer_results,
                                       const uint64_t *d_read_kmer_ends, uint64_t n_reads,

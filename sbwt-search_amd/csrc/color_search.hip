@@ -12,7 +12,7 @@ namespace sbwt_amd {
 namespace {
 
 constexpr u32 kWave = 64;
-constexpr u32 kLdsBytes = 64 * 1024;
+constexpr u64 kLdsColors = 1024;  // counters in LDS up to this many colors, else global
 constexpr u64 kMaxGrid = 1u << 20;
 
 template <typename T>
@@ -164,6 +164,42 @@ __device__ __forceinline__ u32 wave_sum(u32 x) {
 
 __device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
+// ---------------------------------------------------------------------------
+// Per-wave color counters: nc u32 in LDS when nc <= kLdsColors (16 KiB per
+// workgroup of four waves at most, so LDS never limits occupancy), else a
+// slice of a global array per wave of a bounded grid (any number of colors;
+// the payload atomics go to L2). LDS selects the storage at compile time.
+// Global counters are read with L2 loads and are zero between reads.
+// ---------------------------------------------------------------------------
+template <bool LDS>
+__device__ __forceinline__ u32 cnt_get(const u32 *p) {
+  if constexpr (LDS) {
+    return *p;
+  } else {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// The wave's counter updates are complete and visible to its own reads.
+template <bool LDS>
+__device__ __forceinline__ void cnt_fence() {
+  if constexpr (LDS) {
+    wave_lds_fence();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+  }
+}
+
+template <bool LDS>
+__device__ __forceinline__ u32 *wave_counters(u32 *lds, u32 *global_cnt, u32 nc) {
+  const u32 wave = threadIdx.x / kWave;
+  if constexpr (LDS) {
+    return lds + static_cast<u64>(wave) * nc;
+  } else {
+    return global_cnt + (static_cast<u64>(blockIdx.x) * (blockDim.x / kWave) + wave) * nc;
+  }
+}
+
 struct KernelParams {
   double threshold;
   u32 include_not_found;
@@ -171,28 +207,73 @@ struct KernelParams {
   u32 first_read_quirk;
 };
 
-// Fused color search: one wave per read (grid-stride). COUNTS writes the raw
-// u32 counts; otherwise the read's colors with count >= ceil(seq_size * tau)
-// (ContinuousColorResultsPrinter.hpp:235-263) become a bitmap + count. With
-// blank_min (quirk 2, below) each read also leaves colored[r] = (found > 0)
-// and, when it found nothing but its threshold is positive, blank_min[r] =
-// that threshold (else 0).
-template <bool COUNTS>
+// Outputs of the threshold stage per read: the hit bitmap (words per read),
+// the number of hits, optional {found, not found, invalid} and the quirk-2
+// inputs (colored flag, threshold of a read that found nothing).
+struct ReadOut {
+  u64 *bitmap;
+  u64 *nout;
+  u32 *stats;
+  u64 *blank_min;  // null: no quirk-2 pass follows
+  u64 *colored;
+};
+
+// Read r's colors with count >= ceil(seq_size * tau)
+// (ContinuousColorResultsPrinter.hpp:235-263) into its bitmap row, from the
+// counters cnt (zeroed on the way); f/nf/inv are the read's whole counts.
+template <bool LDS>
+__device__ __forceinline__ void finish_read(u32 *cnt, u32 nc, u64 r, u32 f, u32 nf, u32 inv,
+                                            const KernelParams &p, const ReadOut &o, u32 lane) {
+  u64 F = f, NF = nf, INV = inv;
+  if (p.first_read_quirk && r == 0) {
+    F -= 1;
+    NF -= 1;
+    INV -= 1;
+  }
+  const u64 seq = F + (p.include_not_found ? NF : 0) + (p.include_invalid ? INV : 0);
+  const u64 min_hits = static_cast<u64>(ceil(static_cast<double>(seq) * p.threshold));
+  const u32 words = (nc + 63) / 64;
+  u64 total = 0;
+  for (u32 w = 0; w < words; ++w) {
+    const u32 c = w * 64 + lane;
+    const u32 v = c < nc ? cnt_get<LDS>(cnt + c) : 0u;
+    const bool hit = min_hits > 0 && c < nc && v >= min_hits;
+    const u64 m = __ballot(hit);
+    if (lane == 0) o.bitmap[r * words + w] = m;
+    total += __popcll(m);
+    if (c < nc && v) cnt[c] = 0;
+  }
+  if (lane == 0) {
+    o.nout[r] = total;
+    if (o.blank_min) {
+      o.colored[r] = f > 0 ? 1 : 0;
+      o.blank_min[r] = f == 0 ? min_hits : 0;
+    }
+    if (o.stats) {
+      o.stats[3 * r] = f;
+      o.stats[3 * r + 1] = nf;
+      o.stats[3 * r + 2] = inv;
+    }
+  }
+}
+
+// Fused color search, one wave per read (grid-stride): the plain form.
+// COUNTS writes the raw u32 counts; otherwise finish_read thresholds them.
+template <bool COUNTS, bool LDS>
 __global__ __launch_bounds__(256) void k_color_reads(ColorView cv, const u64 *__restrict__ results,
                                                       const u64 *__restrict__ kmer_ends, u64 n_reads,
-                                                      KernelParams p, u64 *__restrict__ bitmap,
-                                                      u64 *__restrict__ nout, u32 *__restrict__ stats,
+                                                      KernelParams p, ReadOut o,
                                                       u32 *__restrict__ counts_out,
-                                                      u64 *__restrict__ blank_min,
-                                                      u64 *__restrict__ colored) {
+                                                      u32 *__restrict__ global_cnt) {
   extern __shared__ u32 lds[];
   const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   const u32 waves = blockDim.x / kWave;
   const u32 nc = cv.num_colors;
-  u32 *cnt = lds + static_cast<u64>(wave) * nc;
-  for (u32 c = lane; c < nc; c += kWave) cnt[c] = 0;
-  wave_lds_fence();
-  const u32 words = (nc + 63) / 64;
+  u32 *cnt = wave_counters<LDS>(lds, global_cnt, nc);
+  if constexpr (LDS) {
+    for (u32 c = lane; c < nc; c += kWave) cnt[c] = 0;
+  }
+  cnt_fence<LDS>();
   for (u64 r = static_cast<u64>(blockIdx.x) * waves + wave; r < n_reads;
        r += static_cast<u64>(gridDim.x) * waves) {
     const u64 ks = r ? kmer_ends[r - 1] : 0, ke = kmer_ends[r];
@@ -201,44 +282,110 @@ __global__ __launch_bounds__(256) void k_color_reads(ColorView cv, const u64 *__
     f = wave_sum(f);
     nf = wave_sum(nf);
     inv = wave_sum(inv);
-    wave_lds_fence();
+    cnt_fence<LDS>();
     if constexpr (COUNTS) {
       for (u32 c = lane; c < nc; c += kWave) {
-        counts_out[r * nc + c] = cnt[c];
-        cnt[c] = 0;
+        const u32 v = cnt_get<LDS>(cnt + c);
+        counts_out[r * nc + c] = v;
+        if (v) cnt[c] = 0;
       }
     } else {
-      u64 F = f, NF = nf, INV = inv;
-      if (p.first_read_quirk && r == 0) {
-        F -= 1;
-        NF -= 1;
-        INV -= 1;
-      }
-      const u64 seq = F + (p.include_not_found ? NF : 0) + (p.include_invalid ? INV : 0);
-      const u64 min_hits = static_cast<u64>(ceil(static_cast<double>(seq) * p.threshold));
-      u64 total = 0;
-      for (u32 w = 0; w < words; ++w) {
-        const u32 c = w * 64 + lane;
-        const bool hit = min_hits > 0 && c < nc && cnt[c] >= min_hits;
-        const u64 m = __ballot(hit);
-        if (lane == 0) bitmap[r * words + w] = m;
-        total += __popcll(m);
-        if (c < nc) cnt[c] = 0;
+      finish_read<LDS>(cnt, nc, r, f, nf, inv, p, o, lane);
+    }
+    cnt_fence<LDS>();
+  }
+}
+
+// Work items of the batched form: read r is split into
+// items[r] = max(1, ceil(k-mers / item_kmers)) items of consecutive k-mers;
+// multi[r] = 1 when it has more than one (its partial counts meet in a slot).
+__global__ __launch_bounds__(256) void k_color_item_counts(const u64 *__restrict__ kmer_ends,
+                                                            u64 n_reads, u32 item_kmers,
+                                                            u64 *__restrict__ items,
+                                                            u64 *__restrict__ multi) {
+  for (u64 r = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; r < n_reads;
+       r += static_cast<u64>(gridDim.x) * blockDim.x) {
+    const u64 nk = kmer_ends[r] - (r ? kmer_ends[r - 1] : 0);
+    const u64 n = nk ? (nk + item_kmers - 1) / item_kmers : 1;
+    items[r] = n;
+    multi[r] = n > 1 ? 1 : 0;
+  }
+}
+
+// The batched form's color search: one wave per work item (a read, or up to
+// item_kmers k-mers of a long one), over a device queue of (read, k-mer
+// range) items, so a read of 100 kbp is spread over ~100 waves instead of one
+// wave's chain of windows. A read of one item is thresholded by its wave. The
+// items of a longer read add their counts into the read's slot (long_incl[r]
+// - 1) of long_cnt, and their {found, not found, invalid} and a finished-item
+// count into long_meta; the wave that finishes the read's last item
+// thresholds the slot and zeroes it for the next batch.
+template <bool LDS>
+__global__ __launch_bounds__(256) void k_color_items(
+    ColorView cv, const u64 *__restrict__ results, const u64 *__restrict__ kmer_ends, u64 n_reads,
+    const u64 *__restrict__ item_ends, u32 item_kmers, const u64 *__restrict__ long_incl,
+    KernelParams p, ReadOut o, u32 *__restrict__ global_cnt, u32 *__restrict__ long_cnt,
+    u32 *__restrict__ long_meta) {
+  extern __shared__ u32 lds[];
+  const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const u32 waves = blockDim.x / kWave;
+  const u32 nc = cv.num_colors;
+  u32 *cnt = wave_counters<LDS>(lds, global_cnt, nc);
+  if constexpr (LDS) {
+    for (u32 c = lane; c < nc; c += kWave) cnt[c] = 0;
+  }
+  cnt_fence<LDS>();
+  const u64 n_items = item_ends[n_reads - 1];
+  for (u64 it = static_cast<u64>(blockIdx.x) * waves + wave; it < n_items;
+       it += static_cast<u64>(gridDim.x) * waves) {
+    u64 lo = 0, hi = n_reads;  // the item's read: the first whose item end exceeds it
+    while (lo < hi) {
+      const u64 mid = (lo + hi) >> 1;
+      if (item_ends[mid] > it) hi = mid;
+      else lo = mid + 1;
+    }
+    const u64 r = lo;
+    const u64 i0 = r ? item_ends[r - 1] : 0, ni = item_ends[r] - i0;
+    const u64 rs = r ? kmer_ends[r - 1] : 0, re = kmer_ends[r];
+    const u64 ks = min(re, rs + (it - i0) * item_kmers), ke = min(re, ks + item_kmers);
+    u32 f = 0, nf = 0, inv = 0;
+    read_add(cv, results, ks, ke, lane, cnt, f, nf, inv);
+    f = wave_sum(f);
+    nf = wave_sum(nf);
+    inv = wave_sum(inv);
+    cnt_fence<LDS>();
+    if (ni == 1) {
+      finish_read<LDS>(cnt, nc, r, f, nf, inv, p, o, lane);
+    } else {
+      const u64 slot = long_incl[r] - 1;
+      u32 *lc = long_cnt + slot * nc;
+      u32 *meta = long_meta + slot * 4;
+      for (u32 c = lane; c < nc; c += kWave) {
+        const u32 v = cnt_get<LDS>(cnt + c);
+        if (v) {
+          atomicAdd(lc + c, v);
+          cnt[c] = 0;
+        }
       }
       if (lane == 0) {
-        nout[r] = total;
-        if (blank_min) {
-          colored[r] = f > 0 ? 1 : 0;
-          blank_min[r] = f == 0 ? min_hits : 0;
-        }
-        if (stats) {
-          stats[3 * r] = f;
-          stats[3 * r + 1] = nf;
-          stats[3 * r + 2] = inv;
-        }
+        atomicAdd(meta + 1, f);
+        atomicAdd(meta + 2, nf);
+        atomicAdd(meta + 3, inv);
+      }
+      __threadfence();  // this item's counts land before its finished count
+      u32 done = 0;
+      if (lane == 0) done = atomicAdd(meta, 1u);
+      done = __shfl(done, 0, kWave);
+      if (done + 1 == ni) {  // the read's last item: every other item's counts are in
+        __threadfence();
+        const u32 F = __hip_atomic_load(meta + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const u32 NF = __hip_atomic_load(meta + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const u32 INV = __hip_atomic_load(meta + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        finish_read<false>(lc, nc, r, F, NF, INV, p, o, lane);
+        if (lane < 4) meta[lane] = 0;
       }
     }
-    wave_lds_fence();
+    cnt_fence<LDS>();
   }
 }
 
@@ -263,19 +410,23 @@ __global__ __launch_bounds__(256) void k_colored_slots(const u64 *__restrict__ c
 // blank read with no colored read after it in the batch keeps its zero counts:
 // the reference reads past its batch's counts there (slot = number of colored
 // reads), so it has no defined output.
+template <bool LDS>
 __global__ __launch_bounds__(256) void k_color_quirk2(ColorView cv, const u64 *__restrict__ results,
                                                        const u64 *__restrict__ kmer_ends,
                                                        u64 n_reads, const u64 *__restrict__ blank_min,
                                                        const u64 *__restrict__ colored_incl,
                                                        const u64 *__restrict__ slot_read,
-                                                       u64 *__restrict__ bitmap, u64 *__restrict__ nout) {
+                                                       u64 *__restrict__ bitmap, u64 *__restrict__ nout,
+                                                       u32 *__restrict__ global_cnt) {
   extern __shared__ u32 lds[];
   const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   const u32 waves = blockDim.x / kWave;
   const u32 nc = cv.num_colors;
-  u32 *cnt = lds + static_cast<u64>(wave) * nc;
-  for (u32 c = lane; c < nc; c += kWave) cnt[c] = 0;
-  wave_lds_fence();
+  u32 *cnt = wave_counters<LDS>(lds, global_cnt, nc);
+  if constexpr (LDS) {
+    for (u32 c = lane; c < nc; c += kWave) cnt[c] = 0;
+  }
+  cnt_fence<LDS>();
   const u32 words = (nc + 63) / 64;
   const u64 n_colored = colored_incl[n_reads - 1];
   for (u64 r = static_cast<u64>(blockIdx.x) * waves + wave; r < n_reads;
@@ -287,18 +438,19 @@ __global__ __launch_bounds__(256) void k_color_quirk2(ColorView cv, const u64 *_
     const u64 ks = rc ? kmer_ends[rc - 1] : 0, ke = kmer_ends[rc];
     u32 f = 0, nf = 0, inv = 0;
     read_add(cv, results, ks, ke, lane, cnt, f, nf, inv);
-    wave_lds_fence();
+    cnt_fence<LDS>();
     u64 total = 0;
     for (u32 w = 0; w < words; ++w) {
       const u32 c = w * 64 + lane;
-      const bool hit = c < nc && cnt[c] >= min_hits;
+      const u32 v = c < nc ? cnt_get<LDS>(cnt + c) : 0u;
+      const bool hit = c < nc && v >= min_hits;
       const u64 m = __ballot(hit);
       if (lane == 0) bitmap[r * words + w] = m;
       total += __popcll(m);
-      if (c < nc) cnt[c] = 0;
+      if (c < nc && v) cnt[c] = 0;
     }
     if (lane == 0) nout[r] = total;
-    wave_lds_fence();
+    cnt_fence<LDS>();
   }
 }
 
@@ -339,26 +491,31 @@ __global__ __launch_bounds__(256) void k_color_compact(const u64 *__restrict__ b
 
 // Reference interface: one wave per colored read over its warps of 64
 // warp-padded indexes (u64 max = padding), u64 counts per (read, color).
+template <bool LDS>
 __global__ __launch_bounds__(256) void k_color_warped(ColorView cv, const u64 *__restrict__ warped,
                                                        const u64 *__restrict__ intervals,
-                                                       u64 n_colored, u64 *__restrict__ counts) {
+                                                       u64 n_colored, u64 *__restrict__ counts,
+                                                       u32 *__restrict__ global_cnt) {
   extern __shared__ u32 lds[];
   const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave, waves = blockDim.x / kWave;
   const u32 nc = cv.num_colors;
-  u32 *cnt = lds + static_cast<u64>(wave) * nc;
-  for (u32 c = lane; c < nc; c += kWave) cnt[c] = 0;
-  wave_lds_fence();
+  u32 *cnt = wave_counters<LDS>(lds, global_cnt, nc);
+  if constexpr (LDS) {
+    for (u32 c = lane; c < nc; c += kWave) cnt[c] = 0;
+  }
+  cnt_fence<LDS>();
   for (u64 q = static_cast<u64>(blockIdx.x) * waves + wave; q < n_colored;
        q += static_cast<u64>(gridDim.x) * waves) {
     u32 f = 0, nf = 0, inv = 0;  // padding reads as not found
     const u64 wbeg = intervals[q] * kWave, wend = intervals[q + 1] * kWave;
-    for (u64 base = wbeg; base < wend; base += kWave) window_add(cv, warped, base, wend, lane, cnt, f, nf, inv);
-    wave_lds_fence();
+    read_add(cv, warped, wbeg, wend, lane, cnt, f, nf, inv);
+    cnt_fence<LDS>();
     for (u32 c = lane; c < nc; c += kWave) {
-      counts[q * nc + c] = cnt[c];
-      cnt[c] = 0;
+      const u32 v = cnt_get<LDS>(cnt + c);
+      counts[q * nc + c] = v;
+      if (v) cnt[c] = 0;
     }
-    wave_lds_fence();
+    cnt_fence<LDS>();
   }
 }
 
@@ -367,11 +524,26 @@ inline u32 grid_for(u64 items, u32 per_block) {
   return static_cast<u32>(std::max<u64>(1, std::min<u64>(g, kMaxGrid)));
 }
 
-inline u32 waves_per_block(u64 nc) {
-  const u64 per_wave = std::max<u64>(nc, 1) * 4;
-  const u64 w = std::min<u64>(4, kLdsBytes / per_wave);
-  if (w == 0) throw std::runtime_error("more than 16384 colors is not supported");
-  return static_cast<u32>(w);
+// Workgroups of four waves; with global counters the grid is bounded so that
+// each wave of it owns one slice of kGlobalCntBlocks * 4 slices.
+constexpr u32 kColorWaves = 4;
+constexpr u32 kGlobalCntBlocks = 1024;
+
+inline bool lds_counters(u64 nc) { return nc <= kLdsColors; }
+
+inline u32 color_grid(u64 work_waves, u64 nc) {
+  const u32 g = grid_for(work_waves, kColorWaves);
+  return lds_counters(nc) ? g : std::min<u32>(g, kGlobalCntBlocks);
+}
+
+inline u32 color_lds_bytes(u64 nc) {
+  return lds_counters(nc) ? static_cast<u32>(kColorWaves * nc * 4) : 0u;
+}
+
+template <typename F>
+void with_counters(u64 nc, F f) {
+  if (lds_counters(nc)) f.template operator()<true>();
+  else f.template operator()<false>();
 }
 
 // key_sets[j] = set_table[color_set_idxs[j]], or an empty entry for an index
@@ -475,19 +647,55 @@ ColorSearcher::ColorSearcher(const DeviceColors *colors, u64 max_indexes, u64 ma
       own_stream_(stream == nullptr),
       max_indexes_(max_indexes),
       max_reads_(max_reads) {
-  (void)waves_per_block(colors_->num_colors);
   if (own_stream_) SBWT_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
-  const u64 words = (colors_->num_colors + 63) / 64;
+  const u64 nc = colors_->num_colors;
+  const u64 words = (nc + 63) / 64;
   SBWT_HIP(hipMalloc(&d_bitmap_, std::max<u64>(max_reads_ * words, 1) * 8));
   SBWT_HIP(hipMalloc(&d_nout_, std::max<u64>(max_reads_, 1) * 8 * 2));
   SBWT_HIP(hipMalloc(&d_scan_tmp_, scan_tmp_entries(max_reads_) * 8));
   SBWT_HIP(hipMalloc(&d_quirk_, std::max<u64>(max_reads_, 1) * 8 * 4));
+  if (!lds_counters(nc)) {  // one zeroed slice of counters per wave of the bounded grid
+    const u64 bytes = u64{kGlobalCntBlocks} * kColorWaves * nc * 4;
+    SBWT_HIP(hipMalloc(&d_global_cnt_, bytes));
+    SBWT_HIP(hipMemset(d_global_cnt_, 0, bytes));
+  }
   for (auto &e : ev_) SBWT_HIP(hipEventCreate(&e));
+}
+
+void ColorSearcher::set_work_items(u32 item_kmers) {
+  if (item_kmers == 0) {
+    item_kmers_ = 0;
+    return;
+  }
+  if (item_kmers < kWave) throw std::runtime_error("work items hold at least 64 k-mers");
+  const u64 nc = std::max<u64>(colors_->num_colors, 1);
+  // a read of more than one item has more than item_kmers k-mers, so at most
+  // max_indexes / item_kmers of them share a batch; their count slots (nc u32
+  // each) are kept within a 32nd of device memory by lengthening the items
+  size_t free_b = 0, total_b = 0;
+  SBWT_HIP(hipMemGetInfo(&free_b, &total_b));
+  const u64 budget = std::max<u64>(total_b / 32, 1 << 20);
+  u64 ik = item_kmers;
+  while ((max_indexes_ / ik + 1) * nc * 4 > budget && ik < (1ULL << 31)) ik *= 2;
+  const u64 slots = max_indexes_ / ik + 1;
+  if (slots > long_slots_ || d_items_ == nullptr) {
+    for (void *b : {static_cast<void *>(d_items_), static_cast<void *>(d_long_cnt_),
+                    static_cast<void *>(d_long_meta_)})
+      if (b) SBWT_HIP(hipFree(b));
+    SBWT_HIP(hipMalloc(&d_items_, std::max<u64>(max_reads_, 1) * 8 * 4));
+    SBWT_HIP(hipMalloc(&d_long_cnt_, slots * nc * 4));
+    SBWT_HIP(hipMalloc(&d_long_meta_, slots * 4 * 4));
+    SBWT_HIP(hipMemset(d_long_cnt_, 0, slots * nc * 4));
+    SBWT_HIP(hipMemset(d_long_meta_, 0, slots * 4 * 4));
+    long_slots_ = slots;
+  }
+  item_kmers_ = static_cast<u32>(ik);
 }
 
 ColorSearcher::~ColorSearcher() {
   for (auto &e : ev_) (void)hipEventDestroy(e);
-  void *bufs[] = {d_bitmap_, d_nout_, d_scan_tmp_, d_quirk_, d_warped_, d_intervals_, d_counts_};
+  void *bufs[] = {d_bitmap_, d_nout_,      d_scan_tmp_, d_quirk_,    d_warped_,
+                  d_intervals_, d_counts_, d_global_cnt_, d_items_, d_long_cnt_, d_long_meta_};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (own_stream_) (void)hipStreamDestroy(stream_);
@@ -513,10 +721,12 @@ void ColorSearcher::search_warped_host(const u64 *warped, u64 n_idxs, const u64 
   SBWT_HIP(hipMalloc(&d_counts_, std::max<u64>(n_colored * nc, 1) * 8));
   SBWT_HIP(hipMemcpyAsync(d_warped_, warped, n_idxs * 8, hipMemcpyHostToDevice, stream_));
   SBWT_HIP(hipMemcpyAsync(d_intervals_, warp_intervals, n_intervals * 8, hipMemcpyHostToDevice, stream_));
-  const u32 waves = waves_per_block(nc);
   SBWT_HIP(hipEventRecord(ev_[0], stream_));
-  k_color_warped<<<grid_for(n_colored, waves), waves * kWave, waves * nc * 4, stream_>>>(
-      colors_->view(), d_warped_, d_intervals_, n_colored, d_counts_);
+  with_counters(nc, [&]<bool LDS>() {
+    k_color_warped<LDS><<<color_grid(n_colored, nc), kColorWaves * kWave, color_lds_bytes(nc),
+                          stream_>>>(colors_->view(), d_warped_, d_intervals_, n_colored, d_counts_,
+                                     d_global_cnt_);
+  });
   SBWT_HIP(hipGetLastError());
   SBWT_HIP(hipEventRecord(ev_[1], stream_));
   recorded_ = true;
@@ -561,26 +771,48 @@ void ColorSearcher::pseudoalign_device_async(const u64 *d_results, const u64 *d_
 
 void ColorSearcher::launch_threshold(const u64 *d_results, const u64 *d_kmer_ends, u64 n_reads,
                                      const ColorParams &p, u32 *d_stats) {
-  const u32 nc = static_cast<u32>(colors_->num_colors);
-  const u32 waves = waves_per_block(nc);
+  const u64 nc = colors_->num_colors;
   KernelParams kp{p.threshold, p.include_not_found, p.include_invalid, p.first_read_quirk};
   // quirk 2 needs a positive threshold on a read that found nothing, which
   // only the include flags give
   const bool quirk2 = p.include_not_found || p.include_invalid;
   u64 *blank_min = d_quirk_, *colored = d_quirk_ + max_reads_;
   u64 *colored_incl = d_quirk_ + 2 * max_reads_, *slot_read = d_quirk_ + 3 * max_reads_;
+  const ReadOut out{d_bitmap_, d_nout_, d_stats, quirk2 ? blank_min : nullptr,
+                    quirk2 ? colored : nullptr};
   if (timing_) SBWT_HIP(hipEventRecord(ev_[0], stream_));
-  const u32 grid = grid_for(n_reads, waves);
-  k_color_reads<false><<<grid, waves * kWave, waves * nc * 4, stream_>>>(
-      colors_->view(), d_results, d_kmer_ends, n_reads, kp, d_bitmap_, d_nout_, d_stats, nullptr,
-      quirk2 ? blank_min : nullptr, quirk2 ? colored : nullptr);
+  if (item_kmers_) {
+    // the batched form: a device queue of (read, k-mer range) items
+    u64 *items = d_items_, *item_ends = d_items_ + max_reads_;
+    u64 *multi = d_items_ + 2 * max_reads_, *long_incl = d_items_ + 3 * max_reads_;
+    k_color_item_counts<<<grid_for(n_reads, 256), 256, 0, stream_>>>(d_kmer_ends, n_reads,
+                                                                      item_kmers_, items, multi);
+    scan_inclusive_u64(items, n_reads, item_ends, d_scan_tmp_, stream_);
+    scan_inclusive_u64(multi, n_reads, long_incl, d_scan_tmp_, stream_);
+    // items never outnumber reads plus k-mers / item_kmers; surplus waves leave
+    const u64 bound = n_reads + max_indexes_ / item_kmers_ + 1;
+    with_counters(nc, [&]<bool LDS>() {
+      k_color_items<LDS><<<color_grid(bound, nc), kColorWaves * kWave, color_lds_bytes(nc),
+                           stream_>>>(colors_->view(), d_results, d_kmer_ends, n_reads, item_ends,
+                                      item_kmers_, long_incl, kp, out, d_global_cnt_, d_long_cnt_,
+                                      d_long_meta_);
+    });
+  } else {
+    with_counters(nc, [&]<bool LDS>() {
+      k_color_reads<false, LDS><<<color_grid(n_reads, nc), kColorWaves * kWave,
+                                  color_lds_bytes(nc), stream_>>>(
+          colors_->view(), d_results, d_kmer_ends, n_reads, kp, out, nullptr, d_global_cnt_);
+    });
+  }
   SBWT_HIP(hipGetLastError());
   if (quirk2) {
     scan_inclusive_u64(colored, n_reads, colored_incl, d_scan_tmp_, stream_);
     k_colored_slots<<<grid_for(n_reads, 256), 256, 0, stream_>>>(colored_incl, n_reads, slot_read);
-    k_color_quirk2<<<grid, waves * kWave, waves * nc * 4, stream_>>>(
-        colors_->view(), d_results, d_kmer_ends, n_reads, blank_min, colored_incl, slot_read,
-        d_bitmap_, d_nout_);
+    with_counters(nc, [&]<bool LDS>() {
+      k_color_quirk2<LDS><<<color_grid(n_reads, nc), kColorWaves * kWave, color_lds_bytes(nc),
+                            stream_>>>(colors_->view(), d_results, d_kmer_ends, n_reads, blank_min,
+                                       colored_incl, slot_read, d_bitmap_, d_nout_, d_global_cnt_);
+    });
     SBWT_HIP(hipGetLastError());
   }
   if (timing_) {
@@ -592,13 +824,15 @@ void ColorSearcher::launch_threshold(const u64 *d_results, const u64 *d_kmer_end
 void ColorSearcher::counts_device(const u64 *d_results, const u64 *d_kmer_ends, u64 n_reads,
                                   u32 *d_counts) {
   if (n_reads == 0) return;
-  const u32 nc = static_cast<u32>(colors_->num_colors);
-  const u32 waves = waves_per_block(nc);
+  const u64 nc = colors_->num_colors;
   KernelParams kp{1.0, 0, 0, 0};
+  const ReadOut out{nullptr, nullptr, nullptr, nullptr, nullptr};
   SBWT_HIP(hipEventRecord(ev_[0], stream_));
-  k_color_reads<true><<<grid_for(n_reads, waves), waves * kWave, waves * nc * 4, stream_>>>(
-      colors_->view(), d_results, d_kmer_ends, n_reads, kp, nullptr, nullptr, nullptr, d_counts,
-      nullptr, nullptr);
+  with_counters(nc, [&]<bool LDS>() {
+    k_color_reads<true, LDS><<<color_grid(n_reads, nc), kColorWaves * kWave, color_lds_bytes(nc),
+                               stream_>>>(colors_->view(), d_results, d_kmer_ends, n_reads, kp, out,
+                                          d_counts, d_global_cnt_);
+  });
   SBWT_HIP(hipGetLastError());
   SBWT_HIP(hipEventRecord(ev_[1], stream_));
   recorded_ = true;

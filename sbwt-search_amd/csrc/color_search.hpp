@@ -86,6 +86,13 @@ class ColorSearcher {
   hipStream_t stream() const { return stream_; }
   float last_search_ms();
   void set_timing(bool on) { timing_ = on; }
+  // Work distribution of the fused color search: 0 (default) one wave per
+  // read; otherwise a device queue of (read, k-mer range) items of item_kmers
+  // k-mers (lengthened when the partial-count slots of long reads would take
+  // more than a 32nd of device memory), the batched form for reads of uneven
+  // length. Results are identical.
+  void set_work_items(u32 item_kmers);
+  u32 work_items() const { return item_kmers_; }
 
  private:
   // k_color_reads (bitmap + count per read) and, with an include flag, the
@@ -100,6 +107,12 @@ class ColorSearcher {
   u64 *d_nout_ = nullptr;     // per read
   u64 *d_scan_tmp_ = nullptr;
   u64 *d_quirk_ = nullptr;    // [4][max_reads]: blank_min, colored, colored_incl, slot_read
+  u32 *d_global_cnt_ = nullptr;  // per-wave counters when the colors do not fit LDS
+  u32 item_kmers_ = 0;
+  u64 *d_items_ = nullptr;     // [4][max_reads]: items, item ends, multi flags, their scan
+  u32 *d_long_cnt_ = nullptr;  // [slots][num_colors] partial counts of multi-item reads
+  u32 *d_long_meta_ = nullptr; // [slots][4]: finished items, found, not found, invalid
+  u64 long_slots_ = 0;
   // staging for the reference interface
   u64 *d_warped_ = nullptr, *d_intervals_ = nullptr, *d_counts_ = nullptr;
   hipEvent_t ev_[2];

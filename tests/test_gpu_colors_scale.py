@@ -1,0 +1,139 @@
+"""Color search at scale on the MI355X: the batched form's work queue of
+(read, k-mer range) items, and color counts beyond the LDS (20,000 and 70,000
+colors, counted in per-wave global counters), against the oracle.
+
+The reference sizes its color results as reads x num_colors with no cap
+(src/ColorSearcher/ColorSearcher.cpp:25-33,51); so does this library.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_py as O
+import sbwt_amd as S
+import workloads as W
+from colorlib import ColoredIndex, colored_seq_ids, kmer_ends_of, read_stats
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+
+
+def expected_lists(ci, res, ke, tau, inc_nf=False, inc_inv=False, first_read_quirk=False):
+    """Oracle counts thresholded per read (quirk 2 included), vectorized."""
+    n = len(ke)
+    nc = ci.num_colors
+    counts = np.zeros(max(n, 1) * nc, np.uint64)
+    O.lib().orc_color_counts(C.byref(ci.oc), O.p64(np.ascontiguousarray(res, np.uint64)),
+                             O.p64(np.ascontiguousarray(ke, np.uint64)), n, O.p64(counts), 8)
+    counts = counts[:n * nc].reshape(n, nc)
+    stats = read_stats(res, ke)
+    slots = colored_seq_ids(stats)
+    colored = [r for r, st in enumerate(stats) if st[0] > 0]
+    out = []
+    for r, (f, nf, inv) in enumerate(stats):
+        if first_read_quirk and r == 0:
+            f, nf, inv = [(x - 1) % (1 << 64) for x in (f, nf, inv)]
+        mn = O.lib().orc_color_min_hits(f, nf, inv, tau, int(inc_nf), int(inc_inv))
+        q = slots[r]
+        out.append([] if mn == 0 or q >= len(colored) else
+                   np.flatnonzero(counts[colored[q]] >= mn).tolist())
+    return out, counts
+
+
+@pytest.fixture(scope="module")
+def long_reads_index(tmp_path_factory):
+    ci = ColoredIndex(tmp_path_factory.mktemp("items"), n_genomes=10, length=60_000, d=12,
+                      sub=0.02)
+    lens = np.concatenate([W.lognormal_lengths(31, 120, 3000, 1.2, 31, 40_000),
+                           np.full(300, 150, np.uint32), np.array([0, 10, 31, 32], np.uint32)])
+    lens = lens[W.permutation(32, len(lens)).astype(np.int64)]
+    chars, ends = ci.reads(len(lens), lens, seed=33, random_frac=0.1, err=0.01)
+    res = ci.index_results(chars, ends)
+    return ci, res, kmer_ends_of(ends)
+
+
+@pytest.mark.parametrize("items", [64, 1024, 4096])
+@pytest.mark.parametrize("tau,inc_nf,inc_inv,quirk", [(0.7, False, False, True),
+                                                      (0.5, True, True, False)])
+def test_work_items_equal_per_read_and_oracle(long_reads_index, items, tau, inc_nf, inc_inv,
+                                              quirk):
+    """Reads up to 40 kbp split into items of 64 to 4096 k-mers: the partial counts
+    of a long read meet in its slot and the last item thresholds them."""
+    ci, res, ke = long_reads_index
+    say(f"work items {items} tau={tau} nf={inc_nf} inv={inc_inv} quirk={quirk}")
+    col = S.ColorIndex(ci.colors_path)
+    p = S.color_params(tau, inc_nf, inc_inv, quirk)
+    exp, _ = expected_lists(ci, res, ke, tau, inc_nf, inc_inv, quirk)
+    per_read = col.searcher(len(res) + 64, len(ke))
+    lists0, st0, _, _ = per_read.pseudoalign(res, ke, p)
+    assert lists0 == exp
+    cs = col.searcher(len(res) + 64, len(ke))
+    cs.set_work_items(items)
+    for rep in range(2):  # the long-read slots are clean again after a batch
+        lists, st, off, ids = cs.pseudoalign(res, ke, p)
+        assert lists == exp, rep
+        assert np.array_equal(st, st0)
+    assert max(int(ke[i]) - (int(ke[i - 1]) if i else 0) for i in range(len(ke))) > 8 * items
+
+
+def _many_colors_index(tmp, n_genomes, length, n_colors, seed):
+    # genome g gets color g * stride: the ids spread over [0, n_colors)
+    stride = (n_colors - 1) // (n_genomes - 1)
+    colors = np.arange(n_genomes, dtype=np.uint32) * np.uint32(stride)
+    colors[-1] = n_colors - 1
+    return ColoredIndex(tmp, n_genomes=n_genomes, length=length, sub=0.004, d=10, seed=seed,
+                        colors=colors)
+
+
+@pytest.mark.parametrize("n_colors,n_genomes,length", [(20_000, 2000, 2000),
+                                                        (70_000, 5000, 1000)],
+                         ids=["20k-colors", "70k-colors"])
+def test_many_colors_equal_oracle(tmp_path, n_colors, n_genomes, length):
+    say(f"{n_colors} colors over {n_genomes} genomes x {length} bp")
+    ci = _many_colors_index(tmp_path, n_genomes, length, n_colors, seed=7)
+    assert ci.num_colors == n_colors
+    assert ci.colors.num_dense > 0 and ci.colors.num_sets > ci.colors.num_dense
+    chars, ends = ci.reads(300, 120, seed=8, random_frac=0.1, err=0.005)
+    res = ci.index_results(chars, ends)
+    ke = kmer_ends_of(ends)
+    col = S.ColorIndex(ci.colors_path)
+    assert col.num_colors == n_colors
+    for items in (0, 128):
+        for tau, inc_nf in ((0.7, False), (0.3, True)):
+            say(f"{n_colors} colors items={items} tau={tau} nf={inc_nf}")
+            exp, counts = expected_lists(ci, res, ke, tau, inc_nf)
+            cs = col.searcher(len(res) + 64, len(ke))
+            cs.set_work_items(items)
+            lists, _, _, _ = cs.pseudoalign(res, ke, S.color_params(tau, inc_nf))
+            assert lists == exp
+            assert sum(len(x) for x in lists) > 0
+    # raw per-read counts through the device counts interface
+    import torch
+    say(f"{n_colors} colors: raw counts")
+    d_res = torch.from_numpy(res.view(np.int64)).cuda()
+    d_ke = torch.from_numpy(ke.view(np.int64)).cuda()
+    d_cnt = torch.zeros(len(ke) * n_colors, dtype=torch.int32, device="cuda")
+    cs = col.searcher(len(res) + 64, len(ke))
+    cs.counts_device(d_res.data_ptr(), d_ke.data_ptr(), len(ke), d_cnt.data_ptr())
+    torch.cuda.synchronize()
+    got = d_cnt.cpu().numpy().view(np.uint32).reshape(len(ke), n_colors)
+    assert np.array_equal(got.astype(np.uint64), counts)
+    # the reference's warp-padded interface (u64 counts per colored read)
+    say(f"{n_colors} colors: warped interface")
+    n = len(ke)
+    warped = np.zeros(len(res) + 64 * n + 64, np.uint64)
+    wi = np.zeros(n + 2, np.uint64)
+    ni = np.zeros(1, np.uint64)
+    csid = np.zeros(n + 1, np.uint64)
+    f, nf, inv = (np.zeros(n + 1, np.uint64) for _ in range(3))
+    nw = O.lib().orc_warp_layout(O.p64(res), O.p64(ke), n, 64, O.p64(warped), O.p64(wi),
+                                 O.p64(ni), O.p64(csid), O.p64(f), O.p64(nf), O.p64(inv))
+    n_int = int(ni[0])
+    cw = col.searcher(int(nw) + 64, n_int)
+    wcounts = cw.search(warped[:nw], wi[:n_int])
+    colored = [r for r in range(n) if int(f[r]) > 0]
+    assert np.array_equal(wcounts.reshape(n_int - 1, n_colors), counts[colored])
