@@ -360,6 +360,10 @@ def mode_batched(args, configs, S, torch, W):
     built = W.BuiltSbwt(wl.genomes, wl.genome_ends, configs.K)
     log(f"SBWT: {built.n} nodes")
     idx, cidx = _colored_index(W, S, built, np.arange(len(wl.genome_ends)), 20, "c4")
+    t0 = time.perf_counter()
+    kind = idx.build_color_table(cidx)
+    log(f"color table: {'masks' if kind == 1 else 'key ranks'} "
+        f"({1000 * (time.perf_counter() - t0):.0f} ms)")
     r = modes.run_batched(S, torch, wl, idx, cidx, args.batch_chars, log=log)
     print(json.dumps({"metric": "reads/s (index + color), batched vs plain", "mode": "batched",
                       "config": {"workload": wl.name + " " + wl.description,

@@ -407,6 +407,24 @@ int sbwt_color_searcher_counts_device(sbwt_color_searcher *searcher,
  * src/ColorSearcher/ColorSearcher.cuh:96-205). */
 int sbwt_color_searcher_set_work_items(sbwt_color_searcher *searcher, uint32_t item_kmers);
 
+/* Fused index + color path (no counterpart in the reference, whose stages
+ * meet through index files): builds the index's color table from a color
+ * index over the same SBWT (the index needs its key-k-mer marks): per node,
+ * its key k-mer's color set as a 64-bit mask when the index has at most 61
+ * colors (*kind = 1; 8 bytes per node), else its key's rank among the key
+ * k-mers (*kind = 2; 4 bytes per node). A searcher with color output
+ * (sbwt_searcher_set_color_output) then writes each found k-mer's table entry
+ * instead of its key node (one table line per run of k-mers sharing a key,
+ * replacing the key move), and a color searcher whose input is that index
+ * (sbwt_color_searcher_set_input) counts masks by ballots with no lookups, or
+ * ranks with one key-set lookup per run instead of the key-mark rank and the
+ * key set. Results are identical to the two-stage path. kind nullable. */
+int sbwt_index_build_color_table(sbwt_index *index, const sbwt_color_index *colors, int *kind);
+int sbwt_searcher_set_color_output(sbwt_searcher *searcher, int on);
+/* index: the index whose color-table entries the results are (null: key nodes,
+ * the default). */
+int sbwt_color_searcher_set_input(sbwt_color_searcher *searcher, const sbwt_index *index);
+
 /* Device time (ms) of the last call's color kernel. */
 int sbwt_color_searcher_last_time(sbwt_color_searcher *searcher, float *ms);
 

@@ -31,20 +31,24 @@ def _split_batches(read_ends, max_chars):
 
 def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect=False,
                 item_kmers=1024):
-    """C4: the same streaming search + color stage in two forms, on the same batches.
+    """C4: the same streaming search, key-k-mer chains and color decode in three forms,
+    on the same batches.
 
     plain: one launch of each stage per batch on one stream, batch after batch,
-      each batch's color ids copied back before the next starts, the color stage
-      one wave per read (the reference's one-launch-per-batch form, with this
-      library's kernels).
-    batched: the color stage over a device queue of (read, k-mer range) work
-      items of item_kmers k-mers, so long reads spread over many waves, and the
-      batches spread over two streams, so one batch's tail overlaps the next's
-      start.
+      each batch's color ids copied back before the next starts; the index stage
+      writes key nodes and the color stage runs one wave per read, ranking each
+      key and reading its set (the reference's two-stage, one-launch-per-batch
+      form, with this library's kernels).
+    batched_unfused: the color stage over a device queue of (read, k-mer range)
+      work items of item_kmers k-mers, the batches spread over two streams so one
+      batch's tail overlaps the next one's start.
+    batched: as batched_unfused, with index and color search fused through the
+      index's color table (idx.build_color_table(col) must have run): the key
+      pass writes each k-mer's color entry (a mask, or a key rank) and the color
+      stage counts masks by ballots (ranks: one key-set lookup per run).
 
-    Both forms run the same search algorithm (streaming with key-k-mer chains)
-    and the same color decode; their outputs (per batch: CSR offsets and color
-    ids) are compared in full. With collect, the outputs are returned too."""
+    Outputs (per batch: CSR offsets and color ids) are compared in full with the
+    plain form's. With collect, the outputs are returned too."""
     dev = torch.device("cuda:0")
     d_chars = torch.from_numpy(wl.chars).to(dev)
     ends64 = wl.read_ends.astype(np.int64)
@@ -59,12 +63,15 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect
     nc = col.num_colors
     params = S.color_params(tau)
 
-    def make_lane(items):
+    def make_lane(items, fused):
         stream = torch.cuda.Stream()
         s = idx.searcher(max_chars, max_reads, move_to_key_kmer=True, stream=stream.cuda_stream)
         s.set_algorithm(S.ALGO_STREAMING)
         cs = col.searcher(max_chars, max_reads, stream=stream.cuda_stream)
         cs.set_work_items(items)
+        if fused:
+            s.set_color_output(True)
+            cs.set_input(idx)
         res = torch.empty(max_chars, dtype=torch.int64, device=dev)
         off = torch.empty(max_reads + 1, dtype=torch.int64, device=dev)
         cap = max_reads * nc
@@ -81,56 +88,62 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect
                                               lane["s"].kmer_ends_device(), r1 - r0, params,
                                               lane["off"].data_ptr(), lane["ids"].data_ptr(),
                                               lane["cap"])
-        # the batch's output back on the host (the same copy in both forms)
+        # the batch's output back on the host (the same copy in every form)
         with torch.cuda.stream(lane["stream"]):
             off = lane["off"][:r1 - r0 + 1].to("cpu", non_blocking=False)
             ids = lane["ids"][:n_ids].to("cpu", non_blocking=False)
         return off.numpy().view(np.uint64), ids.numpy().view(np.uint32)
 
-    # plain: one stream, batch after batch
-    plain = make_lane(0)
-    out_plain = [None] * len(batches)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for b in range(len(batches)):
-        out_plain[b] = run_batch(plain, b)
-    torch.cuda.synchronize()
-    t_plain = time.perf_counter() - t0
-    log(f"plain: {len(batches)} batches in {t_plain:.3f} s")
-
-    # batched: two streams in flight
-    lanes = [make_lane(item_kmers), make_lane(item_kmers)]
     import threading
-    out_b = [None] * len(batches)
 
-    def worker(li):
-        for b in range(li, len(batches), 2):
-            out_b[b] = run_batch(lanes[li], b)
+    def run_form(n_lanes, items, fused):
+        lanes = [make_lane(items, fused) for _ in range(n_lanes)]
+        out = [None] * len(batches)
 
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    th = [threading.Thread(target=worker, args=(i,)) for i in range(2)]
-    for t in th:
-        t.start()
-    for t in th:
-        t.join()
-    torch.cuda.synchronize()
-    t_batched = time.perf_counter() - t0
+        def worker(li):
+            for b in range(li, len(batches), n_lanes):
+                out[b] = run_batch(lanes[li], b)
+
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if n_lanes == 1:
+            worker(0)
+        else:
+            th = [threading.Thread(target=worker, args=(i,)) for i in range(n_lanes)]
+            for t in th:
+                t.start()
+            for t in th:
+                t.join()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, out
+
+    t_plain, out_plain = run_form(1, 0, False)
+    log(f"plain: {len(batches)} batches in {t_plain:.3f} s")
+    t_unfused, out_unfused = run_form(2, item_kmers, False)
+    log(f"batched without fusion: {len(batches)} batches in {t_unfused:.3f} s")
+    t_batched, out_b = run_form(2, item_kmers, True)
     log(f"batched: {len(batches)} batches in {t_batched:.3f} s")
-    for b in range(len(batches)):
-        if not (np.array_equal(out_plain[b][0], out_b[b][0])
-                and np.array_equal(out_plain[b][1], out_b[b][1])):
-            raise SystemExit(f"batched and plain color outputs differ in batch {b}")
+    for name, other in (("batched", out_b), ("batched without fusion", out_unfused)):
+        for b in range(len(batches)):
+            if not (np.array_equal(out_plain[b][0], other[b][0])
+                    and np.array_equal(out_plain[b][1], other[b][1])):
+                raise SystemExit(f"{name} and plain color outputs differ in batch {b}")
     n_ids = sum(len(o[1]) for o in out_plain)
     r = {"reads": wl.n_reads, "batches": len(batches), "batch_chars": batch_chars,
-         "algorithm": "streaming (both forms)", "color_ids": n_ids,
-         "color_work_items": {"plain": "one wave per read", "batched": f"{item_kmers} k-mers"},
-         "plain_s": t_plain, "batched_s": t_batched,
+         "algorithm": "streaming search with key-k-mer chains (every form)",
+         "color_ids": n_ids, "outputs_equal": True,
+         "forms": {"plain": "one stream; key nodes; color stage one wave per read",
+                   "batched_unfused": f"two streams; key nodes; {item_kmers}-k-mer work items",
+                   "batched": f"two streams; color-table entries (fused); {item_kmers}-k-mer "
+                              "work items"},
+         "plain_s": t_plain, "batched_unfused_s": t_unfused, "batched_s": t_batched,
          "plain_reads_per_s": wl.n_reads / t_plain,
+         "batched_unfused_reads_per_s": wl.n_reads / t_unfused,
          "batched_reads_per_s": wl.n_reads / t_batched,
-         "speedup": t_plain / t_batched}
+         "speedup": t_plain / t_batched, "speedup_unfused": t_plain / t_unfused}
     if collect:
-        r["outputs"] = {"batches": batches, "plain": out_plain, "batched": out_b}
+        r["outputs"] = {"batches": batches, "plain": out_plain, "batched": out_b,
+                        "batched_unfused": out_unfused}
     return r
 
 

@@ -124,6 +124,9 @@ def lib():
                                                              C.c_uint64, vp, u64p]
         L.sbwt_color_searcher_counts_device.argtypes = [vp, vp, vp, C.c_uint64, vp]
         L.sbwt_color_searcher_set_work_items.argtypes = [vp, C.c_uint32]
+        L.sbwt_index_build_color_table.argtypes = [vp, vp, C.POINTER(C.c_int)]
+        L.sbwt_searcher_set_color_output.argtypes = [vp, C.c_int]
+        L.sbwt_color_searcher_set_input.argtypes = [vp, vp]
         L.sbwt_color_searcher_last_time.argtypes = [vp, C.POINTER(C.c_float)]
         L.sbwt_index_file_parser_create.argtypes = [C.c_uint64, C.c_uint64, vp, C.POINTER(vp)]
         L.sbwt_index_file_parser_destroy.argtypes = [vp]
@@ -278,6 +281,13 @@ class Index:
         except Exception:
             pass
 
+    def build_color_table(self, colors):
+        """Per-node key color table from a ColorIndex (fused path); returns its
+        kind: 1 masks, 2 key ranks."""
+        kind = C.c_int()
+        _check(lib().sbwt_index_build_color_table(self.h, colors.h, C.byref(kind)))
+        return kind.value
+
     def build_kmer_table(self):
         """Builds the k-mer table (fresh searches read a bucket pair instead of
         the presearch entry and SBWT steps); False when the index cannot have one."""
@@ -356,6 +366,10 @@ class Searcher:
 
     def set_stats(self, enable):
         _check(lib().sbwt_searcher_set_stats(self.h, int(enable)))
+
+    def set_color_output(self, on):
+        """Write each found k-mer's color-table entry instead of its key node."""
+        _check(lib().sbwt_searcher_set_color_output(self.h, int(on)))
 
     def last_stats(self):
         out = np.zeros(16, np.uint64)
@@ -656,6 +670,10 @@ class ColorSearcher:
             C.c_void_p(d_offsets), C.c_void_p(d_ids), ids_capacity,
             C.c_void_p(d_stats) if d_stats else None, C.byref(nids)))
         return nids.value
+
+    def set_input(self, index):
+        """None: results are key nodes; an Index: its color-table entries."""
+        _check(lib().sbwt_color_searcher_set_input(self.h, index.h if index is not None else None))
 
     def set_work_items(self, item_kmers):
         """0: one wave per read; else a queue of (read, k-mer range) items."""

@@ -540,6 +540,28 @@ int sbwt_color_searcher_pseudoalign(sbwt_color_searcher *searcher, const uint64_
   });
 }
 
+int sbwt_index_build_color_table(sbwt_index *index, const sbwt_color_index *colors, int *kind) {
+  return guarded([&] {
+    require(index && colors, "null argument");
+    sbwt_amd::build_color_table(index->dev, colors->dev);
+    if (kind) *kind = static_cast<int>(index->dev.color_table_kind);
+  });
+}
+
+int sbwt_searcher_set_color_output(sbwt_searcher *searcher, int on) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    searcher->impl->set_color_output(on != 0);
+  });
+}
+
+int sbwt_color_searcher_set_input(sbwt_color_searcher *searcher, const sbwt_index *index) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    searcher->impl->set_input(index ? &index->dev : nullptr);
+  });
+}
+
 int sbwt_color_searcher_set_work_items(sbwt_color_searcher *searcher, uint32_t item_kmers) {
   return guarded([&] {
     require(searcher, "null argument");

@@ -64,85 +64,142 @@ __device__ __forceinline__ bool wave_leader(u64 v, bool found, u32 lane, u32 &ru
   return leader;
 }
 
-// The payloads of one window's decoded sets, walked by the whole wave: one
-// unit per lane (a sparse id, or 64 bits of a dense bitmap), adding the
-// run length to the LDS counter of each color (the per-color
-// ballot/popcount of ColorSearcher.cuh:175-205 summed over the read by
-// ColorPostProcessor.cuh:21-43). All payload loads of the window are in
-// flight at once.
-__device__ __forceinline__ void wave_payload(const ColorView &cv, const SetRef &set, u32 units,
-                                             u32 run, u32 lane, u32 *cnt) {
-  // inclusive prefix of the units over the lanes; a unit t belongs to the
-  // first lane whose prefix exceeds t
-  u32 incl = units;
+// Counters belong to one wave (LDS, or its own slice of a global array), so
+// their atomics need only workgroup scope: device-scope ordering across the
+// MI355X's eight XCD-local L2s would write back and invalidate L2.
+__device__ __forceinline__ void cnt_add(u32 *p, u32 v) {
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Units of the window's sets spread over the lanes: unit t of `total` (the
+// sum of `units` over the lanes) belongs to the first lane whose inclusive
+// prefix `incl` exceeds it; returns that lane and t's index in its set.
+__device__ __forceinline__ u32 unit_owner(u32 incl, u32 units, u32 t, u32 &index) {
+  u32 lo = 0, hi = kWave - 1;
+#pragma unroll
+  for (u32 step = 0; step < 6; ++step) {  // 64 lanes: six halvings
+    const u32 mid = (lo + hi) >> 1;
+    if (__shfl(incl, static_cast<int>(mid), kWave) > t) hi = mid;
+    else lo = mid + 1;
+  }
+  index = t - (__shfl(incl, static_cast<int>(lo), kWave) - __shfl(units, static_cast<int>(lo), kWave));
+  return lo;
+}
+
+__device__ __forceinline__ u32 wave_prefix(u32 v, u32 lane) {
+  u32 incl = v;
 #pragma unroll
   for (u32 d = 1; d < kWave; d <<= 1) {
     const u32 y = __shfl_up(incl, d, kWave);
     if (lane >= d) incl += y;
   }
-  const u32 total = __shfl(incl, kWave - 1, kWave);
-  for (u32 t0 = 0; t0 < total; t0 += kWave) {
+  return incl;
+}
+
+// The payloads of one window's decoded sets, walked by the whole wave, adding
+// each set's run length to the counter of every color it holds (the per-color
+// ballot/popcount of ColorSearcher.cuh:175-205 summed over the read by
+// ColorPostProcessor.cuh:21-43). All payload loads of a pass are in flight at
+// once.
+//  - dense sets: their 64-bit bitmap words are loaded one per lane; then the
+//    wave steps through the words, each broadcast from its lane (readlane),
+//    and lane i adds the run to color 64 u + i when its bit is set: no bit
+//    loop and no two lanes on one counter;
+//  - sparse sets: one color id per lane, one counter add each.
+__device__ __forceinline__ void wave_payload(const ColorView &cv, const SetRef &set, u32 units,
+                                             u32 run, u32 lane, u32 *cnt) {
+  const u32 nc = cv.num_colors;
+  const u32 d_units = set.dense ? units : 0u, s_units = set.dense ? 0u : units;
+  const u32 d_incl = wave_prefix(d_units, lane);
+  const u32 d_total = __shfl(d_incl, kWave - 1, kWave);
+  for (u32 t0 = 0; t0 < d_total; t0 += kWave) {
     const u32 t = t0 + lane;
-    u32 lo = 0, hi = kWave - 1;
-#pragma unroll
-    for (u32 step = 0; step < 6; ++step) {  // 64 lanes: six halvings
-      const u32 mid = (lo + hi) >> 1;
-      if (__shfl(incl, static_cast<int>(mid), kWave) > t) hi = mid;
-      else lo = mid + 1;
-    }
-    const u32 owner = lo;
-    const u32 o_incl = __shfl(incl, static_cast<int>(owner), kWave);
-    const u32 o_units = __shfl(units, static_cast<int>(owner), kWave);
+    u32 u = 0;
+    const u32 owner = unit_owner(d_incl, d_units, t, u);
     const u32 o_run = __shfl(run, static_cast<int>(owner), kWave);
     const u32 o_lo = __shfl(static_cast<u32>(set.start), static_cast<int>(owner), kWave);
     const u32 o_hi = __shfl(static_cast<u32>(set.start >> 32), static_cast<int>(owner), kWave);
     const u32 o_len = __shfl(set.len, static_cast<int>(owner), kWave);
-    const bool o_dense = __shfl(set.dense ? 1u : 0u, static_cast<int>(owner), kWave) != 0;
-    if (t < total) {
-      const u32 u = t - (o_incl - o_units);
+    u64 x = 0;
+    if (t < d_total) {
+      // 64 bits of the bitmap: colors 64u .. 64u + 63
+      const u64 bit = (static_cast<u64>(o_lo) | (static_cast<u64>(o_hi) << 32)) + 64 * static_cast<u64>(u);
+      const u64 w = bit >> 6;
+      const u32 off = static_cast<u32>(bit & 63);
+      x = cv.dense_arrays[w] >> off;
+      if (off) x |= cv.dense_arrays[w + 1] << (64 - off);
+      const u32 take = min(64u, o_len - 64 * u);
+      if (take < 64) x &= (1ULL << take) - 1;
+    }
+    const u32 n_here = min(kWave, d_total - t0);
+    for (u32 q = 0; q < n_here; ++q) {
+      const u32 xl = __builtin_amdgcn_readlane(static_cast<u32>(x), q);
+      const u32 xh = __builtin_amdgcn_readlane(static_cast<u32>(x >> 32), q);
+      if ((xl | xh) == 0) continue;
+      const u32 rq = __builtin_amdgcn_readlane(o_run, q);
+      const u32 uq = __builtin_amdgcn_readlane(u, q);
+      const u32 bitv = lane < 32 ? (xl >> lane) : (xh >> (lane - 32));
+      const u32 c = 64 * uq + lane;
+      if ((bitv & 1u) && c < nc) cnt_add(cnt + c, rq);
+    }
+  }
+  const u32 s_incl = wave_prefix(s_units, lane);
+  const u32 s_total = __shfl(s_incl, kWave - 1, kWave);
+  for (u32 t0 = 0; t0 < s_total; t0 += kWave) {
+    const u32 t = t0 + lane;
+    u32 u = 0;
+    const u32 owner = unit_owner(s_incl, s_units, t, u);
+    const u32 o_run = __shfl(run, static_cast<int>(owner), kWave);
+    const u32 o_lo = __shfl(static_cast<u32>(set.start), static_cast<int>(owner), kWave);
+    const u32 o_hi = __shfl(static_cast<u32>(set.start >> 32), static_cast<int>(owner), kWave);
+    if (t < s_total) {
       const u64 start = static_cast<u64>(o_lo) | (static_cast<u64>(o_hi) << 32);
-      if (o_dense) {
-        // 64 bits of the bitmap per unit: colors 64u .. 64u + 63
-        const u64 bit = start + 64 * static_cast<u64>(u), w = bit >> 6;
-        const u32 off = static_cast<u32>(bit & 63);
-        u64 x = cv.dense_arrays[w] >> off;
-        if (off) x |= cv.dense_arrays[w + 1] << (64 - off);
-        const u32 take = min(64u, o_len - 64 * u);
-        if (take < 64) x &= (1ULL << take) - 1;
-        while (x) {
-          const u64 c = 64 * static_cast<u64>(u) + static_cast<u32>(__builtin_ctzll(x));
-          if (c < cv.num_colors) atomicAdd(&cnt[c], o_run);
-          x &= x - 1;
-        }
-      } else {
-        const u64 c = intv_get(cv.sparse_arrays, cv.sparse_width, start + u);
-        if (c < cv.num_colors) atomicAdd(&cnt[c], o_run);
-      }
+      const u64 c = intv_get(cv.sparse_arrays, cv.sparse_width, start + u);
+      if (c < nc) cnt_add(cnt + c, o_run);
     }
   }
 }
 
+// Inputs of the color stage: each k-mer's key node (the reference's index
+// output), or its key's entry of the index's color table: a color mask (the
+// counts then come from ballots, with no lookups) or the key's rank.
+constexpr int kInKeys = 0, kInMasks = 1, kInRanks = 2;
+
 // One window of 64 result slots starting at slot `base` of a read ending at
 // `ke`: counts found / not-found / invalid slots and adds every found
-// k-mer's colors to the LDS counters. A run's leader decodes its set with
-// two dependent loads (the key-mark entry, the key set); decoding several
-// windows together did not pay on gfx950 (more registers, fewer waves).
+// k-mer's colors to the wave's counters. Key nodes and ranks: a run's leader
+// decodes its set (the key-mark entry then the key set, or the key set
+// alone) and the wave walks the payloads. Masks: color c gains the number of
+// lanes whose mask holds it, lane c adding it.
+template <int IN>
 __device__ __forceinline__ void window_add(const ColorView &cv, const u64 *__restrict__ results,
                                            u64 base, u64 ke, u32 lane, u32 *cnt, u32 &f, u32 &nf,
                                            u32 &inv) {
   const u64 i = base + lane;
   const bool in = i < ke;
   const u64 v = in ? results[i] : kInvalid;
-  const bool found = in && v < cv.num_nodes;
+  const bool found = in && (IN == kInKeys ? v < cv.num_nodes : (v >> 63) == 0);
   f += found;
   inv += in && v == kInvalid;
   nf += in && !found && v != kInvalid;
+  if constexpr (IN == kInMasks) {
+    const u64 m = found ? v : 0;
+    if (__ballot(m != 0) == 0) return;
+    u32 mine = 0;
+    for (u32 c = 0; c < cv.num_colors; ++c) {
+      const u32 n = static_cast<u32>(__popcll(__ballot((m >> c) & 1)));
+      if (lane == c) mine = n;
+    }
+    if (mine) cnt_add(cnt + lane, mine);
+    return;
+  }
   u32 run = 0;
   const bool lead = wave_leader(v, found, lane, run);
   SetRef set;
   u32 units = 0;
   if (lead) {
-    const uint4 e = cv.key_sets[rank_marks(cv.key_marks, v)];
+    const u64 rank = IN == kInKeys ? rank_marks(cv.key_marks, v) : v;
+    const uint4 e = cv.key_sets[rank];
     set.start = static_cast<u64>(e.x) | (static_cast<u64>(e.y) << 32);
     set.len = e.z;
     set.dense = (e.w & 1u) != 0;
@@ -151,10 +208,12 @@ __device__ __forceinline__ void window_add(const ColorView &cv, const u64 *__res
   wave_payload(cv, set, units, run, lane, cnt);
 }
 
-// All k-mer results [ks, ke) of one read into the wave's LDS counters.
+// All k-mer results [ks, ke) of one read into the wave's counters.
+template <int IN>
 __device__ __forceinline__ void read_add(const ColorView &cv, const u64 *__restrict__ results, u64 ks,
                                          u64 ke, u32 lane, u32 *cnt, u32 &f, u32 &nf, u32 &inv) {
-  for (u64 base = ks; base < ke; base += kWave) window_add(cv, results, base, ke, lane, cnt, f, nf, inv);
+  for (u64 base = ks; base < ke; base += kWave)
+    window_add<IN>(cv, results, base, ke, lane, cnt, f, nf, inv);
 }
 
 __device__ __forceinline__ u32 wave_sum(u32 x) {
@@ -169,14 +228,15 @@ __device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOM
 // workgroup of four waves at most, so LDS never limits occupancy), else a
 // slice of a global array per wave of a bounded grid (any number of colors;
 // the payload atomics go to L2). LDS selects the storage at compile time.
-// Global counters are read with L2 loads and are zero between reads.
+// Global counters are read past L1 (workgroup scope: only their wave touches
+// them) and are zero between reads.
 // ---------------------------------------------------------------------------
 template <bool LDS>
 __device__ __forceinline__ u32 cnt_get(const u32 *p) {
   if constexpr (LDS) {
     return *p;
   } else {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
   }
 }
 
@@ -186,7 +246,7 @@ __device__ __forceinline__ void cnt_fence() {
   if constexpr (LDS) {
     wave_lds_fence();
   } else {
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "agent");
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
   }
 }
 
@@ -259,7 +319,7 @@ __device__ __forceinline__ void finish_read(u32 *cnt, u32 nc, u64 r, u32 f, u32 
 
 // Fused color search, one wave per read (grid-stride): the plain form.
 // COUNTS writes the raw u32 counts; otherwise finish_read thresholds them.
-template <bool COUNTS, bool LDS>
+template <bool COUNTS, bool LDS, int IN>
 __global__ __launch_bounds__(256) void k_color_reads(ColorView cv, const u64 *__restrict__ results,
                                                       const u64 *__restrict__ kmer_ends, u64 n_reads,
                                                       KernelParams p, ReadOut o,
@@ -278,7 +338,7 @@ __global__ __launch_bounds__(256) void k_color_reads(ColorView cv, const u64 *__
        r += static_cast<u64>(gridDim.x) * waves) {
     const u64 ks = r ? kmer_ends[r - 1] : 0, ke = kmer_ends[r];
     u32 f = 0, nf = 0, inv = 0;
-    read_add(cv, results, ks, ke, lane, cnt, f, nf, inv);
+    read_add<IN>(cv, results, ks, ke, lane, cnt, f, nf, inv);
     f = wave_sum(f);
     nf = wave_sum(nf);
     inv = wave_sum(inv);
@@ -318,9 +378,10 @@ __global__ __launch_bounds__(256) void k_color_item_counts(const u64 *__restrict
 // wave's chain of windows. A read of one item is thresholded by its wave. The
 // items of a longer read add their counts into the read's slot (long_incl[r]
 // - 1) of long_cnt, and their {found, not found, invalid} and a finished-item
-// count into long_meta; the wave that finishes the read's last item
-// thresholds the slot and zeroes it for the next batch.
-template <bool LDS>
+// count into long_meta, and k_color_long_finish thresholds the slot after
+// the launch (no fences in this kernel: a device-scope release per item would
+// write back the XCD's L2).
+template <bool LDS, int IN>
 __global__ __launch_bounds__(256) void k_color_items(
     ColorView cv, const u64 *__restrict__ results, const u64 *__restrict__ kmer_ends, u64 n_reads,
     const u64 *__restrict__ item_ends, u32 item_kmers, const u64 *__restrict__ long_incl,
@@ -349,7 +410,7 @@ __global__ __launch_bounds__(256) void k_color_items(
     const u64 rs = r ? kmer_ends[r - 1] : 0, re = kmer_ends[r];
     const u64 ks = min(re, rs + (it - i0) * item_kmers), ke = min(re, ks + item_kmers);
     u32 f = 0, nf = 0, inv = 0;
-    read_add(cv, results, ks, ke, lane, cnt, f, nf, inv);
+    read_add<IN>(cv, results, ks, ke, lane, cnt, f, nf, inv);
     f = wave_sum(f);
     nf = wave_sum(nf);
     inv = wave_sum(inv);
@@ -372,20 +433,30 @@ __global__ __launch_bounds__(256) void k_color_items(
         atomicAdd(meta + 2, nf);
         atomicAdd(meta + 3, inv);
       }
-      __threadfence();  // this item's counts land before its finished count
-      u32 done = 0;
-      if (lane == 0) done = atomicAdd(meta, 1u);
-      done = __shfl(done, 0, kWave);
-      if (done + 1 == ni) {  // the read's last item: every other item's counts are in
-        __threadfence();
-        const u32 F = __hip_atomic_load(meta + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const u32 NF = __hip_atomic_load(meta + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const u32 INV = __hip_atomic_load(meta + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        finish_read<false>(lc, nc, r, F, NF, INV, p, o, lane);
-        if (lane < 4) meta[lane] = 0;
-      }
     }
     cnt_fence<LDS>();
+  }
+}
+
+// Reads of more than one work item, after k_color_items: one wave per such
+// read thresholds its slot (every item's counts are in: the launch boundary
+// orders them) and zeroes the slot for the next batch.
+__global__ __launch_bounds__(256) void k_color_long_finish(u32 nc, u64 n_reads,
+                                                            const u64 *__restrict__ multi,
+                                                            const u64 *__restrict__ long_incl,
+                                                            KernelParams p, ReadOut o,
+                                                            u32 *__restrict__ long_cnt,
+                                                            u32 *__restrict__ long_meta) {
+  const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+  const u32 waves = blockDim.x / kWave;
+  for (u64 r = static_cast<u64>(blockIdx.x) * waves + wave; r < n_reads;
+       r += static_cast<u64>(gridDim.x) * waves) {
+    if (!multi[r]) continue;
+    const u64 slot = long_incl[r] - 1;
+    u32 *meta = long_meta + slot * 4;
+    const u32 f = meta[1], nf = meta[2], inv = meta[3];
+    finish_read<false>(long_cnt + slot * nc, nc, r, f, nf, inv, p, o, lane);
+    if (lane < 4) meta[lane] = 0;
   }
 }
 
@@ -410,7 +481,7 @@ __global__ __launch_bounds__(256) void k_colored_slots(const u64 *__restrict__ c
 // blank read with no colored read after it in the batch keeps its zero counts:
 // the reference reads past its batch's counts there (slot = number of colored
 // reads), so it has no defined output.
-template <bool LDS>
+template <bool LDS, int IN>
 __global__ __launch_bounds__(256) void k_color_quirk2(ColorView cv, const u64 *__restrict__ results,
                                                        const u64 *__restrict__ kmer_ends,
                                                        u64 n_reads, const u64 *__restrict__ blank_min,
@@ -437,7 +508,7 @@ __global__ __launch_bounds__(256) void k_color_quirk2(ColorView cv, const u64 *_
     const u64 rc = slot_read[q];
     const u64 ks = rc ? kmer_ends[rc - 1] : 0, ke = kmer_ends[rc];
     u32 f = 0, nf = 0, inv = 0;
-    read_add(cv, results, ks, ke, lane, cnt, f, nf, inv);
+    read_add<IN>(cv, results, ks, ke, lane, cnt, f, nf, inv);
     cnt_fence<LDS>();
     u64 total = 0;
     for (u32 w = 0; w < words; ++w) {
@@ -508,7 +579,7 @@ __global__ __launch_bounds__(256) void k_color_warped(ColorView cv, const u64 *_
        q += static_cast<u64>(gridDim.x) * waves) {
     u32 f = 0, nf = 0, inv = 0;  // padding reads as not found
     const u64 wbeg = intervals[q] * kWave, wend = intervals[q + 1] * kWave;
-    read_add(cv, warped, wbeg, wend, lane, cnt, f, nf, inv);
+    read_add<kInKeys>(cv, warped, wbeg, wend, lane, cnt, f, nf, inv);
     cnt_fence<LDS>();
     for (u32 c = lane; c < nc; c += kWave) {
       const u32 v = cnt_get<LDS>(cnt + c);
@@ -544,6 +615,56 @@ template <typename F>
 void with_counters(u64 nc, F f) {
   if (lds_counters(nc)) f.template operator()<true>();
   else f.template operator()<false>();
+}
+
+// The counter storage and the input kind as template arguments.
+template <typename F>
+void with_counters_input(u64 nc, int input, F f) {
+  with_counters(nc, [&]<bool LDS>() {
+    if (input == kInMasks) f.template operator()<LDS, kInMasks>();
+    else if (input == kInRanks) f.template operator()<LDS, kInRanks>();
+    else f.template operator()<LDS, kInKeys>();
+  });
+}
+
+// The color table of an index: per node, the color entry of its key k-mer
+// (the key walk of IndexSearcher.cuh:60-70 through the index's key
+// destinations), as the fused color stage reads it. Ranks: the key's rank
+// among the key k-mers (ColorSearcher.cuh:99-105); masks (at most
+// kColorMaskMaxColors colors): the key's color set as a bit mask.
+template <typename Idx, bool MASKS>
+__global__ __launch_bounds__(256) void k_color_table(IndexView ix, ColorView cv, u64 n, void *out) {
+  for (u64 v = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
+       v += static_cast<u64>(gridDim.x) * blockDim.x) {
+    u32 walked = 0;
+    const u64 key = key_move<Idx>(ix, static_cast<Idx>(v), walked);
+    const bool stuck = key == kWalkStuck;
+    const u64 rank = stuck ? 0 : rank_marks(cv.key_marks, key);
+    if constexpr (!MASKS) {
+      static_cast<u32 *>(out)[v] = stuck ? kNoKeyRank : static_cast<u32>(rank);
+    } else {
+      u64 m = kWalkStuck;
+      if (!stuck) {
+        const uint4 e = cv.key_sets[rank];
+        const u64 start = static_cast<u64>(e.x) | (static_cast<u64>(e.y) << 32);
+        m = 0;
+        if (e.w & 1u) {  // dense: bits [start, start + len), len <= num_colors < 64
+          const u64 w = start >> 6;
+          const u32 off = static_cast<u32>(start & 63);
+          u64 x = cv.dense_arrays[w] >> off;
+          if (off) x |= cv.dense_arrays[w + 1] << (64 - off);
+          m = e.z >= 64 ? x : (x & ((1ULL << e.z) - 1));
+        } else {
+          for (u32 q = 0; q < e.z; ++q) {
+            const u64 c = intv_get(cv.sparse_arrays, cv.sparse_width, start + q);
+            if (c < cv.num_colors) m |= 1ULL << c;
+          }
+        }
+        m &= (1ULL << cv.num_colors) - 1;
+      }
+      static_cast<u64 *>(out)[v] = m;
+    }
+  }
 }
 
 // key_sets[j] = set_table[color_set_idxs[j]], or an empty entry for an index
@@ -660,6 +781,43 @@ ColorSearcher::ColorSearcher(const DeviceColors *colors, u64 max_indexes, u64 ma
     SBWT_HIP(hipMemset(d_global_cnt_, 0, bytes));
   }
   for (auto &e : ev_) SBWT_HIP(hipEventCreate(&e));
+}
+
+void build_color_table(DeviceIndex &index, const DeviceColors &colors) {
+  if (!index.has_marks) throw std::runtime_error("the index has no key-k-mer marks");
+  if (colors.num_nodes != index.n) throw std::runtime_error("color index and SBWT differ in length");
+  if (index.color_table) {
+    SBWT_HIP(hipFree(index.color_table));
+    index.color_table = nullptr;
+  }
+  const bool masks = colors.num_colors <= kColorMaskMaxColors;
+  if (!masks && colors.num_keys >= kNoKeyRank) throw std::runtime_error("too many key k-mers for a rank table");
+  const u64 bytes = index.n * (masks ? 8 : 4) + 16;
+  SBWT_HIP(hipMalloc(&index.color_table, bytes));
+  const IndexView ix = index.view();
+  const u32 grid = grid_for(index.n, 256);
+  if (index.wide) {
+    if (masks) k_color_table<u64, true><<<grid, 256>>>(ix, colors.view(), index.n, index.color_table);
+    else k_color_table<u64, false><<<grid, 256>>>(ix, colors.view(), index.n, index.color_table);
+  } else {
+    if (masks) k_color_table<u32, true><<<grid, 256>>>(ix, colors.view(), index.n, index.color_table);
+    else k_color_table<u32, false><<<grid, 256>>>(ix, colors.view(), index.n, index.color_table);
+  }
+  SBWT_HIP(hipGetLastError());
+  SBWT_HIP(hipDeviceSynchronize());
+  index.color_table_kind = masks ? kColorTableMasks : kColorTableRanks;
+  index.color_table_source = &colors;
+  index.device_bytes += bytes;
+}
+
+void ColorSearcher::set_input(const DeviceIndex *index) {
+  if (!index) {
+    input_ = kInKeys;
+    return;
+  }
+  if (!index->color_table || index->color_table_source != colors_)
+    throw std::runtime_error("the index's color table was not built from this color index");
+  input_ = index->color_table_kind == kColorTableMasks ? kInMasks : kInRanks;
 }
 
 void ColorSearcher::set_work_items(u32 item_kmers) {
@@ -791,16 +949,19 @@ void ColorSearcher::launch_threshold(const u64 *d_results, const u64 *d_kmer_end
     scan_inclusive_u64(multi, n_reads, long_incl, d_scan_tmp_, stream_);
     // items never outnumber reads plus k-mers / item_kmers; surplus waves leave
     const u64 bound = n_reads + max_indexes_ / item_kmers_ + 1;
-    with_counters(nc, [&]<bool LDS>() {
-      k_color_items<LDS><<<color_grid(bound, nc), kColorWaves * kWave, color_lds_bytes(nc),
-                           stream_>>>(colors_->view(), d_results, d_kmer_ends, n_reads, item_ends,
+    with_counters_input(nc, input_, [&]<bool LDS, int IN>() {
+      k_color_items<LDS, IN><<<color_grid(bound, nc), kColorWaves * kWave, color_lds_bytes(nc),
+                               stream_>>>(colors_->view(), d_results, d_kmer_ends, n_reads, item_ends,
                                       item_kmers_, long_incl, kp, out, d_global_cnt_, d_long_cnt_,
                                       d_long_meta_);
     });
+    SBWT_HIP(hipGetLastError());
+    k_color_long_finish<<<grid_for(n_reads, kColorWaves), kColorWaves * kWave, 0, stream_>>>(
+        static_cast<u32>(nc), n_reads, multi, long_incl, kp, out, d_long_cnt_, d_long_meta_);
   } else {
-    with_counters(nc, [&]<bool LDS>() {
-      k_color_reads<false, LDS><<<color_grid(n_reads, nc), kColorWaves * kWave,
-                                  color_lds_bytes(nc), stream_>>>(
+    with_counters_input(nc, input_, [&]<bool LDS, int IN>() {
+      k_color_reads<false, LDS, IN><<<color_grid(n_reads, nc), kColorWaves * kWave,
+                                      color_lds_bytes(nc), stream_>>>(
           colors_->view(), d_results, d_kmer_ends, n_reads, kp, out, nullptr, d_global_cnt_);
     });
   }
@@ -808,9 +969,9 @@ void ColorSearcher::launch_threshold(const u64 *d_results, const u64 *d_kmer_end
   if (quirk2) {
     scan_inclusive_u64(colored, n_reads, colored_incl, d_scan_tmp_, stream_);
     k_colored_slots<<<grid_for(n_reads, 256), 256, 0, stream_>>>(colored_incl, n_reads, slot_read);
-    with_counters(nc, [&]<bool LDS>() {
-      k_color_quirk2<LDS><<<color_grid(n_reads, nc), kColorWaves * kWave, color_lds_bytes(nc),
-                            stream_>>>(colors_->view(), d_results, d_kmer_ends, n_reads, blank_min,
+    with_counters_input(nc, input_, [&]<bool LDS, int IN>() {
+      k_color_quirk2<LDS, IN><<<color_grid(n_reads, nc), kColorWaves * kWave, color_lds_bytes(nc),
+                                stream_>>>(colors_->view(), d_results, d_kmer_ends, n_reads, blank_min,
                                        colored_incl, slot_read, d_bitmap_, d_nout_, d_global_cnt_);
     });
     SBWT_HIP(hipGetLastError());
@@ -828,9 +989,9 @@ void ColorSearcher::counts_device(const u64 *d_results, const u64 *d_kmer_ends, 
   KernelParams kp{1.0, 0, 0, 0};
   const ReadOut out{nullptr, nullptr, nullptr, nullptr, nullptr};
   SBWT_HIP(hipEventRecord(ev_[0], stream_));
-  with_counters(nc, [&]<bool LDS>() {
-    k_color_reads<true, LDS><<<color_grid(n_reads, nc), kColorWaves * kWave, color_lds_bytes(nc),
-                               stream_>>>(colors_->view(), d_results, d_kmer_ends, n_reads, kp, out,
+  with_counters_input(nc, input_, [&]<bool LDS, int IN>() {
+    k_color_reads<true, LDS, IN><<<color_grid(n_reads, nc), kColorWaves * kWave,
+                                   color_lds_bytes(nc), stream_>>>(colors_->view(), d_results, d_kmer_ends, n_reads, kp, out,
                                           d_counts, d_global_cnt_);
   });
   SBWT_HIP(hipGetLastError());

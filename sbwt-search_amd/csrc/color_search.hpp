@@ -55,6 +55,11 @@ struct ColorParams {
   // the batch that has one (IndexFileParser.cpp:65-74).
 };
 
+// The index's color table from a color index over the same SBWT: per node,
+// its key k-mer's color set as a mask (at most 61 colors) or its key's rank
+// among the key k-mers (more colors). Replaces any table built before.
+void build_color_table(DeviceIndex &index, const DeviceColors &colors);
+
 class ColorSearcher {
  public:
   ColorSearcher(const DeviceColors *colors, u64 max_indexes, u64 max_reads, hipStream_t stream);
@@ -93,6 +98,10 @@ class ColorSearcher {
   // length. Results are identical.
   void set_work_items(u32 item_kmers);
   u32 work_items() const { return item_kmers_; }
+  // Input of the fused color stage: the key nodes of the index search (null,
+  // default), or the color-table entries that a searcher with color output
+  // writes over index (whose table must come from this color index).
+  void set_input(const DeviceIndex *index);
 
  private:
   // k_color_reads (bitmap + count per read) and, with an include flag, the
@@ -109,6 +118,7 @@ class ColorSearcher {
   u64 *d_quirk_ = nullptr;    // [4][max_reads]: blank_min, colored, colored_incl, slot_read
   u32 *d_global_cnt_ = nullptr;  // per-wave counters when the colors do not fit LDS
   u32 item_kmers_ = 0;
+  int input_ = 0;             // key nodes, masks or ranks (color_search.hip)
   u64 *d_items_ = nullptr;     // [4][max_reads]: items, item ends, multi flags, their scan
   u32 *d_long_cnt_ = nullptr;  // [slots][num_colors] partial counts of multi-item reads
   u32 *d_long_meta_ = nullptr; // [slots][4]: finished items, found, not found, invalid

@@ -86,7 +86,17 @@ struct IndexView {
   u32 kp;
   u32 has_groups;             // slot 4 holds the suffix-group starts
   u32 has_marks;              // slot 5 holds the key-k-mer marks
+  // With the key-k-mer move, results may be the color-table entry of each
+  // k-mer's key instead of the key's node: per node, its key's color mask
+  // (kColorTableMasks) or its key's rank among the key k-mers
+  // (kColorTableRanks); null for key nodes.
+  const void *color_table;
+  u32 color_table_kind;
 };
+
+constexpr u32 kColorTableNone = 0, kColorTableMasks = 1, kColorTableRanks = 2;
+constexpr u32 kColorMaskMaxColors = 61;  // masks leave bits 61-63 to the sentinels
+constexpr u32 kNoKeyRank = 0xFFFFFFFFu;  // rank table entry of a node whose key walk is stuck
 
 // k-mer table: every node whose label is a whole k-mer (k <= 32, fewer than
 // 2^32 nodes), by its 2k-bit code (first letter highest), in open addressing

@@ -148,6 +148,26 @@ __device__ __forceinline__ u64 key_move(const IndexView &ix, Idx l, u32 &walked)
   return key_walk<Idx>(ix, l, ix.num_nodes, walked);
 }
 
+// The color-table entry of node l's key (IndexView::color_table): its key's
+// color mask or key rank; kWalkStuck when the key walk has no end.
+template <typename Idx>
+__device__ __forceinline__ u64 color_table_entry(const IndexView &ix, Idx l) {
+  if (ix.color_table_kind == kColorTableMasks) return static_cast<const u64 *>(ix.color_table)[l];
+  const u32 r = static_cast<const u32 *>(ix.color_table)[l];
+  return r == kNoKeyRank ? kWalkStuck : static_cast<u64>(r);
+}
+
+// The key move of a found node l, or its key's color-table entry when the
+// view carries a color table (one line either way for most nodes).
+template <typename Idx>
+__device__ __forceinline__ u64 key_result(const IndexView &ix, Idx l, u32 &walked) {
+  if (ix.color_table) {
+    ++walked;
+    return color_table_entry<Idx>(ix, l);
+  }
+  return key_move<Idx>(ix, l, walked);
+}
+
 // sdsl int_vector<> element i of width w (LSB-first, may straddle two words):
 // src/UtilityKernels/VariableLengthIntegerIndex.cuh:23-36.
 __device__ __forceinline__ u64 intv_get(const u64 *v, u32 width, u64 i) {

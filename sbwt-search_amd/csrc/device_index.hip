@@ -476,6 +476,9 @@ void DeviceIndex::release() {
   if (presearch) (void)hipFree(presearch);
   if (key_dest) (void)hipFree(key_dest);
   key_dest = nullptr;
+  if (color_table) (void)hipFree(color_table);
+  color_table = nullptr;
+  color_table_kind = kColorTableNone;
   blocks = nullptr;
   hyper_base = nullptr;
   presearch = nullptr;
@@ -492,6 +495,8 @@ IndexView DeviceIndex::view() const {
   v.kmer_hash_bits = kmer_hash_bits;
   v.has_groups = has_groups;
   v.has_marks = has_marks;
+  v.color_table = nullptr;  // searchers that output color-table entries set it
+  v.color_table_kind = kColorTableNone;
   for (int i = 0; i < 5; ++i) v.c_map[i] = c_map[i];
   v.num_nodes = n;
   v.k = k;

@@ -39,6 +39,12 @@ struct DeviceIndex {
   void *key_dest = nullptr; // per node: its key-k-mer move destination (u32, or u64 if wide)
   uint4 *kmer_hash = nullptr;  // k-mer code -> node table (narrow indexes), or null
   u32 kmer_hash_bits = 0;
+  // per node: its key's color entry (masks or key ranks), built from a color
+  // index by build_color_table (color_search.hip); searchers use it only when
+  // asked to output color-table entries
+  void *color_table = nullptr;
+  u32 color_table_kind = kColorTableNone;
+  const void *color_table_source = nullptr;  // the DeviceColors it was built from
   bool has_groups = false;  // suffix-group starts are in the lines
   // starts were given but some non-start row has an edge: not an SBWT, so
   // every search runs per k-mer (the streaming search needs the SBWT property)

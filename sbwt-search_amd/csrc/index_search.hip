@@ -242,7 +242,7 @@ __device__ __forceinline__ u64 search_window(const IndexView &ix, u64 window, u3
       }
       if (done) {
         if constexpr (KEY) {
-          if (node != kNotFound) return key_move<Idx>(ix, static_cast<Idx>(node), walked);
+          if (node != kNotFound) return key_result<Idx>(ix, static_cast<Idx>(node), walked);
         }
         return node;
       }
@@ -268,7 +268,7 @@ __device__ __forceinline__ u64 search_window(const IndexView &ix, u64 window, u3
   }
   if (l > r) return kNotFound;
   if constexpr (KEY) {
-    return key_move<Idx>(ix, l, walked);
+    return key_result<Idx>(ix, l, walked);
   } else {
     return static_cast<u64>(l);
   }
@@ -902,7 +902,8 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
 // then: the key of k-mer i + 1 when that result says "same key" (k-mer i's
 // node is unmarked and its walk's first step is k-mer i + 1's node), its own
 // node when marked, kWalkStuck when its walk has no edge, else the table (or
-// the walk, key_move). Chains of "same key" are resolved with one ballot and a
+// the walk, key_move); with a color table in the view, every chain end reads
+// its key's color entry instead. Chains of "same key" are resolved with one ballot and a
 // shuffle per block; the block's first k-mer hands its key and flag to the
 // block on its left. Only chain ends read the key table, instead of every
 // found k-mer. Work items never share a chain: each starts with a full search.
@@ -950,10 +951,13 @@ __global__ __launch_bounds__(kSearchThreads) void k_key_runs(IndexView ix,
       const bool brk = found && !same;
       u64 base = 0;
       if (brk) {
-        if (succ == kPredMarked) {
-          base = node;
-        } else if (succ == kPredStuck) {
+        if (succ == kPredStuck) {
           base = kWalkStuck;
+        } else if (ix.color_table) {  // the key's color entry (for a marked node, its own)
+          base = color_table_entry<Idx>(ix, static_cast<Idx>(node));
+          if (STATS) ++st_walk;
+        } else if (succ == kPredMarked) {
+          base = node;
         } else {
           u32 w = 0;
           base = key_move<Idx>(ix, static_cast<Idx>(node), w);
@@ -1115,6 +1119,21 @@ Searcher::Searcher(const DeviceIndex *index, u64 max_chars, u64 max_reads, bool 
   set_launch(kDefaultChunkKmers, kDefaultBlocksPerCu);
 }
 
+void Searcher::set_color_output(bool on) {
+  if (on && (!move_to_key_ || !index_->color_table))
+    throw std::runtime_error("color-table output needs the key-k-mer move and the index's color table");
+  color_output_ = on;
+}
+
+IndexView Searcher::reads_view() const {
+  IndexView v = index_->view();
+  if (color_output_) {
+    v.color_table = index_->color_table;
+    v.color_table_kind = index_->color_table_kind;
+  }
+  return v;
+}
+
 void Searcher::set_launch(u32 chunk_kmers, u32 blocks_per_cu) {
   if (chunk_kmers < kMinChunkKmers || chunk_kmers > kMaxChunkKmers)
     throw std::runtime_error("chunk_kmers must be in [64, 65535]");
@@ -1247,7 +1266,7 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
   }
   if (n_chars > max_chars_ || n_reads > max_reads_)
     throw std::runtime_error("batch exceeds the searcher's max_chars_per_batch / max_reads_per_batch");
-  const IndexView ix = index_->view();
+  const IndexView ix = reads_view();
   if (timing_) SBWT_HIP(hipEventRecord(ev_[0], stream_));
   launch_pack(d_chars, n_chars, d_packed_, d_invalid_, stream_);
   if (timing_) SBWT_HIP(hipEventRecord(ev_[1], stream_));

@@ -95,6 +95,12 @@ class Searcher {
   // (synchronous).
   void set_stats(bool on) { stats_ = on; }
   void set_algorithm(int a) { algorithm_ = a; }
+  // Results of the reads path with the key-k-mer move: the key's node (the
+  // reference's output) or, with color_output, the key's entry in the
+  // index's color table (built from a color index), the input of the fused
+  // color stage. Needs move_to_key_kmer and a built table.
+  void set_color_output(bool on);
+  bool color_output() const { return color_output_; }
   int algorithm() const { return algorithm_; }
   void last_stats(u64 out[kStatsWords]);
   const u64 *kmer_ends_device() const { return d_kmer_ends_; }
@@ -111,6 +117,8 @@ class Searcher {
   bool timing_ = true;
   bool stats_ = false;
   int algorithm_ = kAlgoAuto;
+  bool color_output_ = false;
+  IndexView reads_view() const;
   u32 lane_share_ = 1;
   u64 *d_stats_ = nullptr;
   u64 max_chunks_ = 0;

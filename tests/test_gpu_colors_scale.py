@@ -137,3 +137,54 @@ def test_many_colors_equal_oracle(tmp_path, n_colors, n_genomes, length):
     wcounts = cw.search(warped[:nw], wi[:n_int])
     colored = [r for r in range(n) if int(f[r]) > 0]
     assert np.array_equal(wcounts.reshape(n_int - 1, n_colors), counts[colored])
+
+
+@pytest.mark.parametrize("algo", [S.ALGO_STREAMING, S.ALGO_PER_KMER], ids=["streaming", "per-kmer"])
+@pytest.mark.parametrize("n_genomes", [8, 80], ids=["masks-8-colors", "ranks-80-colors"])
+def test_fused_color_table_equals_oracle(tmp_path, n_genomes, algo):
+    """The fused path: the index's color table gives each k-mer its key's color mask
+    (at most 61 colors) or key rank; the index stage writes those entries and the
+    color stage counts them. Entries and color lists equal the oracle's."""
+    say(f"fused color table: {n_genomes} colors algo={algo}")
+    ci = ColoredIndex(tmp_path, n_genomes=n_genomes, length=12_000, sub=0.02, d=9, seed=5)
+    idx = S.Index.load(ci.sbwt_path, ci.colors_path)
+    col = S.ColorIndex(ci.colors_path)
+    kind = idx.build_color_table(col)
+    assert kind == (1 if n_genomes <= 61 else 2)
+    lens = np.concatenate([np.full(600, 120, np.uint32), W.uniform_lengths(61, 60, 31, 6000)])
+    chars, ends = ci.reads(len(lens), lens, seed=62, random_frac=0.1, err=0.01)
+    keys = ci.index_results(chars, ends)
+    ke = kmer_ends_of(ends)
+    # expected table entries from the oracle's key nodes
+    km = ci.key_marks
+    bits = np.unpackbits(km.view(np.uint8), bitorder="little").astype(np.int64)
+    ranks = np.concatenate([[0], np.cumsum(bits)])
+    exp_entries = keys.copy()
+    found = keys < np.uint64(ci.sbwt.n)
+    if kind == 2:
+        exp_entries[found] = ranks[keys[found].astype(np.int64)].astype(np.uint64)
+    else:
+        for i in np.flatnonzero(found):
+            m = 0
+            for c in ci.color_set(int(keys[i])):
+                m |= 1 << c
+            exp_entries[i] = np.uint64(m)
+    s = idx.searcher(len(chars), len(ends), move_to_key_kmer=True)
+    s.set_algorithm(algo)
+    s.set_color_output(True)
+    got = s.search_reads(chars, ends)
+    assert np.array_equal(got, exp_entries)
+    s.set_color_output(False)
+    assert np.array_equal(s.search_reads(chars, ends), keys)
+    for items in (0, 64):
+        for tau, inc_nf, quirk in ((0.7, False, True), (0.4, True, False)):
+            exp, _ = expected_lists(ci, keys, ke, tau, inc_nf, False, quirk)
+            cs = col.searcher(len(got) + 64, len(ke))
+            cs.set_work_items(items)
+            cs.set_input(idx)
+            lists, _, _, _ = cs.pseudoalign(got, ke, S.color_params(tau, inc_nf, False, quirk))
+            assert lists == exp, (items, tau)
+    # a color searcher of another color index refuses this index's entries
+    other = S.ColorIndex(ci.colors_path)
+    with pytest.raises(S.SbwtError):
+        other.searcher(100, 10).set_input(idx)

@@ -219,6 +219,7 @@ def test_c4_shape_batched_equals_plain_and_oracle(tmp_path):
     idx = S.Index.load(ci.sbwt_path, ci.colors_path)
     assert idx.build_kmer_table()
     col = S.ColorIndex(ci.colors_path)
+    assert idx.build_color_table(col) == 1  # 12 colors: masks
     batch_chars = wl.n_chars // 5 + 1
     r = modes.run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=say, collect=True)
     assert r["batches"] >= 5
@@ -235,8 +236,9 @@ def test_c4_shape_batched_equals_plain_and_oracle(tmp_path):
         off, ids = outs["plain"][b]
         got = [ids[int(off[i]):int(off[i + 1])].tolist() for i in range(r1 - r0)]
         assert got == exp, b
-        assert np.array_equal(outs["batched"][b][0], off)
-        assert np.array_equal(outs["batched"][b][1], ids)
+        for form in ("batched", "batched_unfused"):
+            assert np.array_equal(outs[form][b][0], off)
+            assert np.array_equal(outs[form][b][1], ids)
 
 
 # ---------------------------------------------------------------- C5
