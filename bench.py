@@ -38,14 +38,18 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--config", default="c2", choices=["c1", "c2", "c5"],
+                    help="index mode workload: c2 (default, the headline), c1, or c5 (the "
+                         "serving index, HBM-resident node lines)")
     ap.add_argument("--mode", default="index",
-                    choices=["index", "batched", "serve", "colors", "cli", "cli-colors"],
+                    choices=["index", "batched", "serve", "colors", "cli", "cli-colors",
+                             "gather-probe"],
                     help="index: the headline line (default); batched: C4 plain vs batched "
                          "form; serve: Poisson request stream, served vs plain sequential; "
                          "colors: the color stage on the C3 recipe; cli: the sbwt_search "
                          "command end to end on C2 written as FASTA and FASTQ; cli-colors: "
-                         "`sbwt_search colors` on C3 index-result files (ascii and binary)")
+                         "`sbwt_search colors` on C3 index-result files (ascii and binary); "
+                         "gather-probe: the random 128-byte line ceiling over working-set sizes")
     # the C3 recipe: 1000 colors of 200-500 kbp genomes (index build fits one command)
     ap.add_argument("--color-genomes", type=int, default=1000)
     ap.add_argument("--color-roots", type=int, default=20)
@@ -130,9 +134,13 @@ def main():
         return mode_colors(args, configs, S, torch, W)
     if args.mode == "cli-colors":
         return mode_cli_colors(args, configs, S, torch, W)
+    if args.mode == "gather-probe":
+        return mode_gather_probe(args, S)
     t0 = time.time()
     if args.config == "c1":
         wl = configs.c1(**({"n_reads": args.reads} if args.reads else {}))
+    elif args.config == "c5":
+        wl = configs.c5_index(**({"n_reads": args.reads} if args.reads else {}))
     else:
         wl = configs.c2(**({"n_reads": args.reads} if args.reads else {}))
     log(f"workload {wl.name}: {wl.n_reads} reads, {wl.n_chars} chars ({time.time() - t0:.1f} s)")
@@ -211,6 +219,16 @@ def main():
     found = int(np.count_nonzero(res < np.uint64(S.INVALID)))
     log(f"{ms_per_step:.2f} ms/step, search kernel {kern_ms:.2f} ms, {value / 1e9:.3f} G k-mers/s,"
         f" found {found / n_kmers:.3f}")
+    # the random-line ceiling: at the node lines' working set (what most of the
+    # search's lines hit) and at an HBM-resident size
+    node_line_bytes = (int(built.n) // 96 + 1) * 128
+    probes = {}
+    for name, ws in (("node_lines", node_line_bytes), ("hbm_32GiB", 32 << 30)):
+        best = max((S.random_line_probe(ws, bpc) for bpc in (4, 8)), key=lambda r: r["gbs"])
+        probes[name] = {"working_set_bytes": best["table_bytes"], "gbs": best["gbs"],
+                        "lines_per_s": best["lines_per_s"]}
+        log(f"random-line probe {name}: {best['gbs']:.0f} GB/s "
+            f"({best['lines_per_s'] / 1e9:.1f} G lines/s over {best['table_bytes'] >> 20} MiB)")
 
     if args.calibrate_copy_mib:
         n = args.calibrate_copy_mib << 20
@@ -260,6 +278,16 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
+            # the lines the search reads are mostly node lines (a 128 B line per
+            # random access) of a structure that fits the 256 MB Infinity Cache:
+            # the measured ceiling of random lines at that size is the relevant one
+            "regime": ("cache-assisted: node lines of %d MiB fit the 256 MB Infinity Cache"
+                       if node_line_bytes <= (256 << 20) else
+                       "HBM: node lines of %d MiB exceed the 256 MB Infinity Cache")
+                      % (node_line_bytes >> 20),
+            "random_line_ceiling": probes,
+            "frac_of_random_line_ceiling": achieved / probes["node_lines"]["gbs"],
+            "frac_of_hbm_random_line_ceiling": achieved / probes["hbm_32GiB"]["gbs"],
             "traffic": traffic_from_profile(wl.name.lower(), args.algorithm.replace("-", "_"),
                                             idx.presearch_letters, table["built"]),
             "algorithmic_bytes_per_launch": algo_bytes,
@@ -281,7 +309,7 @@ def cpu_baseline(args, wl, bits, built, gpu_res):
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_py as O
     n_sample = min(args.cpu_sample_reads, wl.n_reads)
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads or host_threads()
     end_char = int(wl.read_ends[n_sample - 1])
     chars = wl.chars[:end_char].tobytes()
     ends = wl.read_ends[:n_sample].copy()
@@ -317,6 +345,14 @@ def cpu_baseline(args, wl, bits, built, gpu_res):
 
 
 _ARGS = None
+
+
+def host_threads():
+    """Every CPU this process may run on (all the host cores it is given)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        return os.cpu_count() or 1
 
 
 def _kmer_table(args, idx):
@@ -539,7 +575,7 @@ def cli_cpu_baseline(args, wl, exe, sbwt_path, fasta, L, tmp):
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import oracle_py as O
     n_sample = min(args.cpu_sample_reads, wl.n_reads)
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads or host_threads()
     sample = os.path.join(tmp, "sample.fna")
     with open(fasta, "rb") as f, open(sample, "wb") as g:
         g.write(f.read(n_sample * (L + 4)))
@@ -634,6 +670,21 @@ def mode_cli_colors(args, configs, S, torch, W):
     os.remove(colors_path)
     os.rmdir(tmp)
     print(json.dumps(out), flush=True)
+    return 0
+
+
+def mode_gather_probe(args, S):
+    """The random-line probe over working-set sizes (the ceiling of dependent random
+    128-byte line reads, cache-resident to HBM-resident)."""
+    rows = []
+    for mib in (16, 64, 128, 256, 512, 1024, 4096, 16384, 65536):
+        for bpc in (2, 4, 8):
+            r = S.random_line_probe(mib << 20, bpc)
+            rows.append({"working_set_mib": r["table_bytes"] >> 20, "blocks_per_cu": bpc,
+                         "lines_per_s": r["lines_per_s"], "gbs": r["gbs"], "ms": r["ms"]})
+            log(f"{mib} MiB x {bpc}/CU: {r['lines_per_s'] / 1e9:.1f} G lines/s, {r['gbs']:.0f} GB/s")
+    print(json.dumps({"metric": "random 128-byte line reads/s by working set", "mode":
+                      "gather-probe", "rows": rows}), flush=True)
     return 0
 
 

@@ -45,6 +45,16 @@ const char *sbwt_last_error(void);
  * Replaces gpu_utils::get_free_gpu_memory (src/Tools/GpuUtils.cu:7-12). */
 int sbwt_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
 
+/* Random-line probe, the ceiling the search kernels are measured against (no
+ * counterpart in the reference): dependent random 16-byte loads, one per
+ * 128-byte line, over a table of the smallest power-of-two number of lines
+ * that holds working_set_bytes (allocated and freed by the call), from
+ * blocks_per_cu 256-lane workgroups per CU each following `steps` lines.
+ * *ms = one timed pass after an untimed one; *lines = lines read in it;
+ * *table_bytes (nullable) = the working set used. Synchronous. */
+int sbwt_random_line_probe(uint64_t working_set_bytes, uint32_t blocks_per_cu, uint32_t steps,
+                           float *ms, uint64_t *lines, uint64_t *table_bytes);
+
 /* ---- SBWT index -------------------------------------------------------- */
 
 /* Device index from the four ACGT bit vectors of an SBWT with num_bits nodes

@@ -103,6 +103,17 @@ def c5_genomes(n_genomes=200, n_roots=10, length=5_000_000, rate=0.03):
     return W.concat(gens)
 
 
+def c5_index(n_reads=10_000_000, n_genomes=200, n_roots=10, length=5_000_000):
+    """The serving index (C5 recipe) searched with a batch of reads of U[75, 250] bp
+    (1% error): an index whose node lines are far larger than the Infinity Cache."""
+    codes, ends = c5_genomes(n_genomes, n_roots, length)
+    lens = W.uniform_lengths(5, n_reads, 75, 250)
+    chars, rends, _ = W.reads(5, codes, ends, lens, random_frac=0.0, err_rate=0.01)
+    return Workload("C5", f"index search on the serving index: {n_genomes} genomes x {length} bp "
+                    f"from {n_roots} seeded roots (3% substitutions), SBWT k=31; {n_reads} reads "
+                    "of U[75,250] bp with 1% error", codes, ends, chars, rends)
+
+
 def c4(n_long=1_000_000, n_short=4_000_000, n_genomes=50, length=2_000_000):
     codes, ends = c2_genomes(n_genomes, length)
     long_lens = W.lognormal_lengths(4, n_long, 2000.0, 1.2, 31, 100_000)

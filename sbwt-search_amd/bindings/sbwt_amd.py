@@ -79,6 +79,8 @@ def lib():
         vp = C.c_void_p
         L.sbwt_last_error.restype = C.c_char_p
         L.sbwt_mem_info.argtypes = [u64p, u64p]
+        L.sbwt_random_line_probe.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32,
+                                             C.POINTER(C.c_float), u64p, u64p]
         L.sbwt_index_create.argtypes = [u64p * 4, C.c_uint64, C.c_uint32, u64p, u64p,
                                         C.c_uint32, C.POINTER(vp)]
         L.sbwt_index_load.argtypes = [C.c_char_p, C.c_char_p, C.c_uint32, C.POINTER(vp)]
@@ -229,6 +231,17 @@ def suffix_groups_consistent(acgt, suffix_group_starts, num_bits):
     if rc < 0:
         raise SbwtError(lib().sbwt_last_error().decode())
     return bool(rc)
+
+
+def random_line_probe(working_set_bytes, blocks_per_cu=8, steps=256):
+    """Dependent random 128-byte line reads at a working-set size: returns
+    {"ms", "lines", "table_bytes", "lines_per_s", "gbs"} (gbs counts 128 B/line)."""
+    ms, lines, tb = C.c_float(), C.c_uint64(), C.c_uint64()
+    _check(lib().sbwt_random_line_probe(working_set_bytes, blocks_per_cu, steps, C.byref(ms),
+                                        C.byref(lines), C.byref(tb)))
+    lps = lines.value / (ms.value / 1e3)
+    return {"ms": ms.value, "lines": lines.value, "table_bytes": tb.value, "lines_per_s": lps,
+            "gbs": lps * 128 / 1e9}
 
 
 class Index:

@@ -12,6 +12,7 @@
 #include "index_parse.hpp"
 #include "index_search.hpp"
 #include "loaders.hpp"
+#include "probe.hpp"
 #include "seq_parse.hpp"
 #include "server.hpp"
 
@@ -124,6 +125,17 @@ sbwt_amd::ColorParams to_params(const sbwt_color_params *p) {
 extern "C" {
 
 const char *sbwt_last_error(void) { return g_last_error.c_str(); }
+
+int sbwt_random_line_probe(uint64_t working_set_bytes, uint32_t blocks_per_cu, uint32_t steps,
+                           float *ms, uint64_t *lines, uint64_t *table_bytes) {
+  return guarded([&] {
+    require(ms && lines && steps > 0, "null argument");
+    const sbwt_amd::ProbeResult r = sbwt_amd::random_line_probe(working_set_bytes, blocks_per_cu, steps);
+    *ms = r.ms;
+    *lines = r.lines;
+    if (table_bytes) *table_bytes = r.table_bytes;
+  });
+}
 
 int sbwt_mem_info(uint64_t *free_bytes, uint64_t *total_bytes) {
   return guarded([&] {
