@@ -55,8 +55,23 @@ def lib():
             getattr(L, f).restype = C.c_uint64
         L.bld_colors_key_marks.argtypes = [C.c_void_p, u64p]
         L.bld_colors_write.argtypes = [C.c_void_p, C.c_char_p]
+        L.bld_sbwt_lookup.restype = C.c_int
+        L.bld_gpu_build.argtypes = [u8p, u64p, C.c_uint64, C.c_uint32, C.c_int, u32p, C.c_uint32,
+                                    C.c_int, C.POINTER(C.c_void_p), C.POINTER(GpuBuildStats),
+                                    C.c_char_p, C.c_uint64]
+        L.bld_gpu_build.restype = C.c_void_p
         _lib = L
     return _lib
+
+
+class GpuBuildStats(C.Structure):
+    _fields_ = [(f, C.c_uint64) for f in ("occurrences", "pairs", "kmers", "dummies", "nodes",
+                                           "sets", "key_nodes", "cycle_nodes",
+                                           "peak_device_bytes")] + \
+               [(f, C.c_double) for f in ("ms_keys", "ms_sort", "ms_sbwt", "ms_colors", "ms_total")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
 
 
 def _p(a, t):
@@ -123,12 +138,14 @@ def kmer_key(codes):
 
 
 class BuiltSbwt:
-    def __init__(self, codes, ends, k, add_rc=True, threads=0):
+    def __init__(self, codes, ends, k, add_rc=True, threads=0, handle=None):
         self.codes = np.ascontiguousarray(codes, np.uint8)
         self.ends = np.ascontiguousarray(ends, np.uint64)
         self.k = k
-        self.h = lib().bld_sbwt_build(_p(self.codes, u8p), _p(self.ends, u64p), len(self.ends), k,
-                                      int(add_rc), threads)
+        if handle is None:
+            handle = lib().bld_sbwt_build(_p(self.codes, u8p), _p(self.ends, u64p),
+                                          len(self.ends), k, int(add_rc), threads)
+        self.h = handle
         if not self.h:
             raise RuntimeError("SBWT construction failed")
         self.n = lib().bld_sbwt_num_nodes(self.h)
@@ -146,7 +163,8 @@ class BuiltSbwt:
     def lookup(self, keys):
         keys = np.ascontiguousarray(keys, np.uint64)
         out = np.empty(len(keys), np.uint64)
-        lib().bld_sbwt_lookup(self.h, _p(keys, u64p), len(keys), _p(out, u64p))
+        if lib().bld_sbwt_lookup(self.h, _p(keys, u64p), len(keys), _p(out, u64p)):
+            raise RuntimeError("this SBWT was built without its k-mer list")
         return out
 
     def colors(self, color_of, d=20, threads=0):
@@ -159,11 +177,13 @@ class BuiltSbwt:
 
 
 class BuiltColors:
-    def __init__(self, sbwt, color_of, d, threads):
+    def __init__(self, sbwt, color_of, d, threads, handle=None):
         self.sbwt = sbwt
-        co = np.ascontiguousarray(color_of, np.uint32)
-        self.h = lib().bld_colors_build(sbwt.h, _p(sbwt.codes, u8p), _p(sbwt.ends, u64p),
-                                        len(sbwt.ends), _p(co, u32p), d, threads)
+        if handle is None:
+            co = np.ascontiguousarray(color_of, np.uint32)
+            handle = lib().bld_colors_build(sbwt.h, _p(sbwt.codes, u8p), _p(sbwt.ends, u64p),
+                                            len(sbwt.ends), _p(co, u32p), d, threads)
+        self.h = handle
         if not self.h:
             raise RuntimeError("color index construction failed")
         self.num_colors = lib().bld_colors_num_colors(self.h)
@@ -189,3 +209,23 @@ def permutation(seed, n):
     out = np.empty(n, np.uint64)
     lib().bld_permutation(seed, n, _p(out, u64p))
     return out
+
+
+def gpu_build(codes, ends, k, add_rc=True, color_of=None, d=20, keep_kmers=False):
+    """The SBWT (and with color_of the color index) built on the current GPU:
+    bit for bit the host builder's. Returns (BuiltSbwt, BuiltColors or None,
+    stats dict)."""
+    codes = np.ascontiguousarray(codes, np.uint8)
+    ends = np.ascontiguousarray(ends, np.uint64)
+    co = None if color_of is None else np.ascontiguousarray(color_of, np.uint32)
+    col_h = C.c_void_p()
+    st = GpuBuildStats()
+    err = C.create_string_buffer(512)
+    h = lib().bld_gpu_build(_p(codes, u8p), _p(ends, u64p), len(ends), k, int(add_rc),
+                            None if co is None else _p(co, u32p), d, int(keep_kmers),
+                            C.byref(col_h), C.byref(st), err, len(err))
+    if not h:
+        raise RuntimeError("GPU construction failed: " + err.value.decode())
+    sb = BuiltSbwt(codes, ends, k, add_rc, handle=h)
+    colors = BuiltColors(sb, None, d, 0, handle=col_h.value) if co is not None else None
+    return sb, colors, st.as_dict()

@@ -11,10 +11,13 @@
 #include <string>
 #include <vector>
 
+#include "builder_impl.hpp"
 #include "radix.hpp"
 
 using sbwt_build::radix_sort;
 using sbwt_build::radix_sort_pairs;
+using sbwt_build::bits_for;
+using sbwt_build::low_mask;
 using std::vector;
 
 namespace {
@@ -34,14 +37,6 @@ struct Rng {
   uint64_t next() { return mix64(base + i++); }
   double uniform() { return unit(next()); }
 };
-
-inline uint64_t low_mask(uint32_t digits) {  // low 2*digits bits set
-  return digits >= 32 ? ~0ULL : ((1ULL << (2 * digits)) - 1);
-}
-
-inline int bits_for(uint64_t max_value) {  // sdsl bits::hi(x)+1, at least 1
-  return max_value == 0 ? 1 : 64 - __builtin_clzll(max_value);
-}
 
 inline int threads_or_default(int t) { return t > 0 ? t : omp_get_max_threads(); }
 
@@ -287,28 +282,6 @@ extern "C" uint64_t bld_reads(uint64_t seed, const uint8_t *genomes, const uint6
 }
 
 // ===================================================================== SBWT
-// Nodes are the k-mers (real) plus the dummy nodes $^(k-j) p[0..j-1] needed
-// for every k-mer with no predecessor, in colexicographic order ($ < A < C <
-// G < T). A node is (primary, len): primary holds char i of the node string at
-// bits [2i, 2i+1] with $ as 0, len = number of non-$ chars; colex order is
-// (primary, len) ascending.
-struct bld_sbwt {
-  uint64_t n = 0, n_kmers = 0;
-  uint32_t k = 0;
-  bool add_rc = true;
-  vector<uint64_t> kmers;       // sorted colex keys of the real k-mers
-  vector<uint64_t> dummy_key;   // sorted dummies
-  vector<uint8_t> dummy_len;
-  vector<uint64_t> bits[5];     // A, C, G, T, suffix-group starts
-  uint64_t c_map[5] = {0, 0, 0, 0, 0};
-
-  // number of dummies ordered before real k-mer key x
-  uint64_t dummies_before(uint64_t x) const {
-    return std::upper_bound(dummy_key.begin(), dummy_key.end(), x) - dummy_key.begin();
-  }
-  uint64_t node_of_kmer_index(uint64_t i) const { return i + dummies_before(kmers[i]); }
-};
-
 extern "C" bld_sbwt *bld_sbwt_build(const uint8_t *codes, const uint64_t *seq_ends, uint64_t n_seqs,
                                     uint32_t k, int add_rc, int threads) {
   if (k < 2 || k > 32) return nullptr;
@@ -427,13 +400,15 @@ extern "C" void bld_sbwt_bits(const bld_sbwt *s, int c, uint64_t *out) {
   std::memcpy(out, s->bits[c].data(), (s->n + 63) / 64 * 8);
 }
 
-extern "C" void bld_sbwt_lookup(const bld_sbwt *s, const uint64_t *keys, uint64_t n, uint64_t *out) {
+extern "C" int bld_sbwt_lookup(const bld_sbwt *s, const uint64_t *keys, uint64_t n, uint64_t *out) {
+  if (!s->has_kmers()) return -1;
 #pragma omp parallel for schedule(static)
   for (uint64_t i = 0; i < n; ++i) {
     auto it = std::lower_bound(s->kmers.begin(), s->kmers.end(), keys[i]);
     out[i] = (it != s->kmers.end() && *it == keys[i]) ? s->node_of_kmer_index(it - s->kmers.begin())
                                                       : UINT64_MAX;
   }
+  return 0;
 }
 
 // plain-matrix v0.1: the fields SbwtBuilder.cpp:58-104 reads or skips, in order.
@@ -476,21 +451,11 @@ extern "C" int bld_sbwt_write(const bld_sbwt *s, const char *path) {
 }
 
 // ===================================================================== colors
-struct bld_colors {
-  uint64_t n = 0, num_colors = 0;
-  vector<uint64_t> key_marks;
-  vector<uint64_t> set_offsets;  // set s = set_ids[set_offsets[s] .. set_offsets[s+1])
-  vector<uint32_t> set_ids;
-  vector<uint64_t> color_set_idxs;  // one per key node, in node order
-  uint64_t n_dense = 0;
-  vector<uint8_t> dense;
-};
-
-
 extern "C" bld_colors *bld_colors_build(const bld_sbwt *s, const uint8_t *codes,
                                         const uint64_t *seq_ends, uint64_t n_seqs,
                                         const uint32_t *color_of, uint32_t d, int threads) {
   const int T = threads_or_default(threads);
+  if (!s->has_kmers()) return nullptr;
   const uint32_t k = s->k;
   const uint64_t n = s->n;
   auto *C = new bld_colors();
@@ -676,17 +641,7 @@ extern "C" bld_colors *bld_colors_build(const bld_sbwt *s, const uint8_t *codes,
   // 4. color_set_idxs over key nodes; dense/sparse choice per set
   for (uint64_t v = 0; v < n; ++v)
     if (bit(C->key_marks, v)) C->color_set_idxs.push_back(node_set[v] == UINT32_MAX ? 0 : node_set[v]);
-  const uint64_t nsets = C->set_offsets.size() - 1;
-  const int sparse_w = bits_for(C->num_colors - 1);
-  C->dense.assign(nsets, 0);
-  for (uint64_t i = 0; i < nsets; ++i) {
-    const uint64_t sz = C->set_offsets[i + 1] - C->set_offsets[i];
-    const uint64_t mx = sz ? C->set_ids[C->set_offsets[i + 1] - 1] : 0;
-    if (sz && mx + 1 < sz * static_cast<uint64_t>(sparse_w)) {
-      C->dense[i] = 1;
-      C->n_dense++;
-    }
-  }
+  sbwt_build::choose_dense(C);
   return C;
 }
 

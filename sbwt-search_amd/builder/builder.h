@@ -3,8 +3,9 @@
  * search path: seeded genomes and reads, an SBWT (plain-matrix v0.1) builder
  * and writer, and a color index (.tcolors, sdsl-hybrid-v4) builder and writer.
  *
- * These produce the inputs the search path consumes; nothing here runs on the
- * GPU or sits inside a timed region.
+ * These produce the inputs the search path consumes; nothing here sits inside a
+ * timed region. bld_gpu_build constructs the same SBWT and color index on the
+ * GPU for collections of billions of bases (gpu_build.hip).
  *
  * Random numbers: counter-based splitmix64 (Steele, Lea, Flood 2014),
  *   mix64(z) = splitmix64 finalizer of z + 0x9E3779B97F4A7C15,
@@ -62,8 +63,10 @@ uint64_t bld_sbwt_num_kmers(const bld_sbwt *s);
 /* Copies ceil(n/64) words of bit vector c (0..3 = A,C,G,T; 4 = suffix-group starts). */
 void bld_sbwt_bits(const bld_sbwt *s, int c, uint64_t *out);
 int bld_sbwt_write(const bld_sbwt *s, const char *path);
-/* Node index of each k-mer (colex key, first char in the low bits) or UINT64_MAX. */
-void bld_sbwt_lookup(const bld_sbwt *s, const uint64_t *keys, uint64_t n, uint64_t *out);
+/* Node index of each k-mer (colex key, first char in the low bits) or UINT64_MAX.
+ * Returns -1 (and writes nothing) when the structure holds no k-mer list (a GPU
+ * build without keep_kmers). */
+int bld_sbwt_lookup(const bld_sbwt *s, const uint64_t *keys, uint64_t n, uint64_t *out);
 
 /* ---- color index ---- */
 /* Colors: genome g of the sequences given to bld_sbwt_build (with add_rc a
@@ -83,6 +86,28 @@ uint64_t bld_colors_num_dense(const bld_colors *c);
 /* Copies the key-k-mer marks (ceil(n/64) words). */
 void bld_colors_key_marks(const bld_colors *c, uint64_t *out);
 int bld_colors_write(const bld_colors *c, const char *path);
+
+/* ---- GPU construction (current HIP device) ----
+ * The SBWT of bld_sbwt_build and, when color_of is given, the color index of
+ * bld_colors_build with the same d, bit for bit: same nodes, bit vectors,
+ * C map, color sets, set ids, key marks and set-id vector. k in [2, 31];
+ * color ids below 2^31. Device memory: about 16 bytes per k-mer occurrence for
+ * the sort (24 with colors) plus one byte per occurrence for flags; 1000
+ * genomes of 2-5 Mbp with reverse complements (7 G occurrences) take about
+ * 190 GB. keep_kmers != 0 keeps the sorted k-mer keys on the host (8 bytes per
+ * k-mer), which bld_sbwt_lookup and bld_colors_build need.
+ * Two distinct color lists with one 64-bit list hash are not supported (an
+ * error, at about m^2 / 2^65 odds for m sets). Returns NULL on failure with
+ * the reason in err. */
+typedef struct bld_gpu_stats {
+  uint64_t occurrences, pairs, kmers, dummies, nodes, sets, key_nodes, cycle_nodes;
+  uint64_t peak_device_bytes;
+  double ms_keys, ms_sort, ms_sbwt, ms_colors, ms_total;
+} bld_gpu_stats;
+bld_sbwt *bld_gpu_build(const uint8_t *codes, const uint64_t *seq_ends, uint64_t n_seqs,
+                        uint32_t k, int add_rc, const uint32_t *color_of, uint32_t d,
+                        int keep_kmers, bld_colors **colors, bld_gpu_stats *stats, char *err,
+                        uint64_t err_len);
 
 #ifdef __cplusplus
 }
