@@ -50,11 +50,11 @@ def parse():
                          "command end to end on C2 written as FASTA and FASTQ; cli-colors: "
                          "`sbwt_search colors` on C3 index-result files (ascii and binary); "
                          "gather-probe: the random 128-byte line ceiling over working-set sizes")
-    # the C3 recipe: 1000 colors of 200-500 kbp genomes (index build fits one command)
+    # the C3 recipe: 1000 colors of 2-5 Mbp genomes from 20 roots (4.09 G nodes, built on the GPU)
     ap.add_argument("--color-genomes", type=int, default=1000)
     ap.add_argument("--color-roots", type=int, default=20)
-    ap.add_argument("--color-min-len", type=int, default=200_000)
-    ap.add_argument("--color-max-len", type=int, default=500_000)
+    ap.add_argument("--color-min-len", type=int, default=2_000_000)
+    ap.add_argument("--color-max-len", type=int, default=5_000_000)
     ap.add_argument("--color-reads", type=int, default=5_000_000)
     ap.add_argument("--batch-chars", type=int, default=600_000_000)
     ap.add_argument("--long-reads", type=int, default=1_000_000)
@@ -144,9 +144,7 @@ def main():
     else:
         wl = configs.c2(**({"n_reads": args.reads} if args.reads else {}))
     log(f"workload {wl.name}: {wl.n_reads} reads, {wl.n_chars} chars ({time.time() - t0:.1f} s)")
-    t0 = time.time()
-    built = W.BuiltSbwt(wl.genomes, wl.genome_ends, configs.K)
-    log(f"SBWT: {built.n} nodes, {built.n_kmers} k-mers ({time.time() - t0:.1f} s)")
+    built, _ = _gpu_built(W, wl.genomes, wl.genome_ends, configs.K)
     bits = [built.bits(c) for c in range(4)]
     t0 = time.time()
     idx = S.Index.create(bits, built.n, configs.K, presearch_letters=args.presearch_letters,
@@ -367,11 +365,23 @@ def _kmer_table(args, idx):
     return {"built": built, "build_ms": ms}
 
 
-def _colored_index(W, S, built, color_of, d, tag):
-    """Color index of a built SBWT, written to a temporary .tcolors and loaded."""
+def _gpu_built(W, codes, ends, k, color_of=None, d=20):
+    """SBWT (and color index) of the genomes from the GPU builder (identical to
+    the host builder's); construction is set-up, outside every timed region."""
+    t0 = time.time()
+    built, col, st = W.gpu_build(codes, ends, k, True,
+                                 None if color_of is None else np.asarray(color_of, np.uint32), d)
+    log(f"GPU build: {st['nodes']} nodes, {st['kmers']} k-mers, {st['sets']} color sets, "
+        f"{st['ms_total']:.0f} ms on the device, peak {st['peak_device_bytes'] / 2**30:.1f} GiB "
+        f"({time.time() - t0:.1f} s)")
+    return built, col
+
+
+def _colored_index(S, built, col, tag):
+    """Device index + color index of a built SBWT and color index (the colors
+    written to a temporary .tcolors and loaded)."""
     import tempfile
     t0 = time.time()
-    col = built.colors(np.asarray(color_of, np.uint32), d=d)
     tmp = tempfile.mkdtemp(prefix=f"sbwt_{tag}_")
     path = os.path.join(tmp, "index.tcolors")
     col.write(path)
@@ -393,9 +403,9 @@ def mode_batched(args, configs, S, torch, W):
     t0 = time.time()
     wl = configs.c4(args.long_reads, args.short_reads)
     log(f"workload {wl.name}: {wl.n_reads} reads, {wl.n_chars} chars ({time.time() - t0:.1f} s)")
-    built = W.BuiltSbwt(wl.genomes, wl.genome_ends, configs.K)
-    log(f"SBWT: {built.n} nodes")
-    idx, cidx = _colored_index(W, S, built, np.arange(len(wl.genome_ends)), 20, "c4")
+    built, col = _gpu_built(W, wl.genomes, wl.genome_ends, configs.K,
+                            np.arange(len(wl.genome_ends)), 20)
+    idx, cidx = _colored_index(S, built, col, "c4")
     t0 = time.perf_counter()
     kind = idx.build_color_table(cidx)
     log(f"color table: {'masks' if kind == 1 else 'key ranks'} "
@@ -412,9 +422,9 @@ def mode_serve(args, configs, S, torch, W):
     import modes
     t0 = time.time()
     codes, ends = configs.c5_genomes(args.serve_genomes, args.serve_roots, args.serve_length)
-    built = W.BuiltSbwt(codes, ends, configs.K)
+    built, col = _gpu_built(W, codes, ends, configs.K, np.arange(len(ends)), 20)
     log(f"serve index: {built.n} nodes ({time.time() - t0:.1f} s)")
-    idx, cidx = _colored_index(W, S, built, np.arange(len(ends)), 20, "serve")
+    idx, cidx = _colored_index(S, built, col, "serve")
 
     def reads(seed, n):
         lens = W.uniform_lengths(seed, n, 75, 250)
@@ -443,10 +453,9 @@ def mode_colors(args, configs, S, torch, W):
     wl = configs.c3(args.color_reads, args.color_genomes, args.color_roots, args.color_min_len,
                     args.color_max_len)
     log(f"workload {wl.name}: {wl.n_reads} reads ({time.time() - t0:.1f} s)")
-    t0 = time.time()
-    built = W.BuiltSbwt(wl.genomes, wl.genome_ends, configs.K)
-    log(f"SBWT: {built.n} nodes ({time.time() - t0:.1f} s)")
-    idx, cidx = _colored_index(W, S, built, np.arange(len(wl.genome_ends)), 20, "c3")
+    built, col = _gpu_built(W, wl.genomes, wl.genome_ends, configs.K,
+                            np.arange(len(wl.genome_ends)), 20)
+    idx, cidx = _colored_index(S, built, col, "c3")
     dev = torch.device("cuda:0")
     stream = torch.cuda.Stream()
     s = idx.searcher(wl.n_chars, wl.n_reads, move_to_key_kmer=True, stream=stream.cuda_stream)
@@ -498,7 +507,7 @@ def mode_cli(args, configs, S, torch, W):
     lens = np.diff(np.concatenate([[0], wl.read_ends.astype(np.int64)]))
     if not np.all(lens == lens[0]):
         raise SystemExit("cli mode writes fixed-length reads")
-    built = W.BuiltSbwt(wl.genomes, wl.genome_ends, configs.K)
+    built, _ = _gpu_built(W, wl.genomes, wl.genome_ends, configs.K)
     tmp = tempfile.mkdtemp(prefix="sbwt_cli_")
     sbwt_path = os.path.join(tmp, "index.sbwt")
     built.write(sbwt_path)
@@ -608,10 +617,10 @@ def mode_cli_colors(args, configs, S, torch, W):
     t0 = time.time()
     wl = configs.c3(args.color_reads, args.color_genomes, args.color_roots, args.color_min_len,
                     args.color_max_len)
-    built = W.BuiltSbwt(wl.genomes, wl.genome_ends, configs.K)
+    built, col = _gpu_built(W, wl.genomes, wl.genome_ends, configs.K,
+                            np.arange(len(wl.genome_ends)), 20)
     log(f"workload {wl.name}: {wl.n_reads} reads, SBWT {built.n} nodes ({time.time() - t0:.1f} s)")
     tmp = tempfile.mkdtemp(prefix="sbwt_clicol_")
-    col = built.colors(np.arange(len(wl.genome_ends), dtype=np.uint32), d=20)
     colors_path = os.path.join(tmp, "index.tcolors")
     col.write(colors_path)
     idx = S.Index.create([built.bits(c) for c in range(4)], built.n, 31, key_marks=col.key_marks(),

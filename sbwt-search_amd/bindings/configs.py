@@ -22,6 +22,9 @@ Each builder returns plain numpy data; nothing here touches the GPU.
       substitutions, one color per genome; requests of N log-uniform in [1, 10^5]
       reads of length U[75, 250].
 """
+import os
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 import workloads as W
@@ -79,11 +82,14 @@ def c2(n_reads=10_000_000, n_genomes=50, length=2_000_000):
 
 def c3_genomes(n_genomes=1000, n_roots=20, min_len=2_000_000, max_len=5_000_000):
     roots = [W.genome_random(100 + i, max_len) for i in range(n_roots)]
-    gens = []
-    for g in range(n_genomes):
+
+    def genome(g):  # seeded per genome, so the threads' order does not matter
         length = int(min_len + W.lib().bld_draw(3, 11, g) % (max_len - min_len + 1))
         rate = 0.01 + 0.04 * (W.lib().bld_draw(3, 12, g) >> 11) * 2.0 ** -53
-        gens.append(W.genome_mutate(roots[g % n_roots][:length], 1000 + g, rate))
+        return W.genome_mutate(roots[g % n_roots][:length], 1000 + g, rate)
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        gens = list(ex.map(genome, range(n_genomes)))
     return W.concat(gens)
 
 

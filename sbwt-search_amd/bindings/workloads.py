@@ -65,7 +65,8 @@ def lib():
 
 
 class GpuBuildStats(C.Structure):
-    _fields_ = [(f, C.c_uint64) for f in ("occurrences", "pairs", "kmers", "dummies", "nodes",
+    _fields_ = [(f, C.c_uint64) for f in ("occurrences", "pairs", "kmers", "missing_prefixes",
+                                           "dummies", "nodes",
                                            "sets", "key_nodes", "cycle_nodes",
                                            "peak_device_bytes")] + \
                [(f, C.c_double) for f in ("ms_keys", "ms_sort", "ms_sbwt", "ms_colors", "ms_total")]
@@ -98,7 +99,7 @@ def genome_mutate(root, seed, sub_rate, indel_rate=0.0, max_indel=0):
 
 def concat(seqs):
     ends = np.cumsum([len(s) for s in seqs]).astype(np.uint64)
-    return np.concatenate(seqs).astype(np.uint8), ends
+    return np.concatenate(seqs).astype(np.uint8, copy=False), ends
 
 
 def reads(seed, genomes, genome_ends, lens, random_frac=0.0, err_rate=0.0, n_rate=0.0,
@@ -225,7 +226,8 @@ def gpu_build(codes, ends, k, add_rc=True, color_of=None, d=20, keep_kmers=False
                             None if co is None else _p(co, u32p), d, int(keep_kmers),
                             C.byref(col_h), C.byref(st), err, len(err))
     if not h:
-        raise RuntimeError("GPU construction failed: " + err.value.decode())
+        raise RuntimeError(f"GPU construction failed: {err.value.decode()}; counts so far: "
+                           f"{st.as_dict()}")
     sb = BuiltSbwt(codes, ends, k, add_rc, handle=h)
     colors = BuiltColors(sb, None, d, 0, handle=col_h.value) if co is not None else None
     return sb, colors, st.as_dict()

@@ -98,9 +98,10 @@ int bld_colors_write(const bld_colors *c, const char *path);
  * k-mer), which bld_sbwt_lookup and bld_colors_build need.
  * Two distinct color lists with one 64-bit list hash are not supported (an
  * error, at about m^2 / 2^65 odds for m sets). Returns NULL on failure with
- * the reason in err. */
+ * the reason in err; the stats hold what was counted up to the failure. */
 typedef struct bld_gpu_stats {
-  uint64_t occurrences, pairs, kmers, dummies, nodes, sets, key_nodes, cycle_nodes;
+  uint64_t occurrences, pairs, kmers, missing_prefixes, dummies, nodes, sets, key_nodes,
+      cycle_nodes;
   uint64_t peak_device_bytes;
   double ms_keys, ms_sort, ms_sbwt, ms_colors, ms_total;
 } bld_gpu_stats;

@@ -71,10 +71,14 @@ __host__ __device__ inline u64 mask_digits(u32 digits) {
   return digits >= 32 ? ~0ULL : ((1ULL << (2 * digits)) - 1);
 }
 
+// Grid-stride loops throughout: a launch's grid is counted in work-items in a
+// 32-bit field, so one lane per element would cut off launches beyond 2^32.
+__device__ inline u64 lane0() { return static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; }
+__device__ inline u64 lanes() { return static_cast<u64>(gridDim.x) * blockDim.x; }
+
 inline u32 grid_for(u64 threads) {
   const u64 g = (threads + kBlock - 1) / kBlock;
-  if (g > 0x7FFFFFFFULL) throw std::runtime_error("grid too large");
-  return static_cast<u32>(g ? g : 1);
+  return static_cast<u32>(std::min<u64>(std::max<u64>(g, 1), 1u << 20));
 }
 
 // Device memory with a running total, so the build reports its peak.
@@ -133,8 +137,7 @@ __global__ __launch_bounds__(kBlock) void k_kmer_keys(const u8 *__restrict__ cod
                                                       const u32 *__restrict__ color_of, u32 k,
                                                       int add_rc, u64 *__restrict__ keys,
                                                       u32 *__restrict__ cols) {
-  const u64 p0 = (static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x) * kKeyChunk;
-  if (p0 >= L) return;
+  for (u64 p0 = lane0() * kKeyChunk; p0 < L; p0 += lanes() * kKeyChunk) {
   const u64 p1 = p0 + kKeyChunk < L ? p0 + kKeyChunk : L;
   u64 lo = 0, hi = n_seqs;  // first sequence ending after p0
   while (lo < hi) {
@@ -178,16 +181,49 @@ __global__ __launch_bounds__(kBlock) void k_kmer_keys(const u8 *__restrict__ cod
       }
     }
   }
+  }
 }
 
 // Sorted by key, and by color within a key when colors ride along.
 __global__ __launch_bounds__(kBlock) void k_check_sorted(const u64 *__restrict__ keys,
                                                          const u32 *__restrict__ cols, u64 n,
                                                          u32 *__restrict__ err) {
-  const u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x + 1;
-  if (i >= n) return;
-  const u64 a = keys[i - 1], b = keys[i];
-  if (a > b || (cols && a == b && cols[i - 1] > cols[i])) atomicOr(err, 1u);
+  const auto body = [&](const u64 i) {
+    if (i >= n) return;
+    const u64 a = keys[i - 1], b = keys[i];
+    if (a > b || (cols && a == b && cols[i - 1] > cols[i])) atomicOr(err, 1u);
+  };
+  for (u64 t = lane0() + 1; t < n; t += lanes()) body(t);
+}
+
+// Multiset fingerprints: the wrapping sum of a hash of every element (of the
+// keys, the (key, color) pairs or the colors; of flagged elements only when
+// flags are given). A sort or a compaction that loses or repeats an element
+// changes it.
+__device__ inline u64 element_print(const u64 *keys, const u32 *cols, u64 i) {
+  if (!keys) return mix64((cols[i] & ~kTag) + 0x51ED27ULL);
+  if (!cols) return mix64(keys[i]);
+  return mix64(keys[i] ^ mix64((cols[i] & ~kTag) + 0x51ED27ULL));
+}
+
+__global__ __launch_bounds__(kBlock) void k_fingerprint(const u64 *__restrict__ keys,
+                                                        const u32 *__restrict__ cols,
+                                                        const u8 *__restrict__ flags, u64 n,
+                                                        unsigned long long *__restrict__ out) {
+  u64 acc = 0;
+  for (u64 i = lane0(); i < n; i += lanes())
+    if (!flags || flags[i]) acc += element_print(keys, cols, i);
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(static_cast<unsigned long long>(acc), o);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, static_cast<unsigned long long>(acc));
+}
+
+// Strictly increasing keys (the distinct k-mers).
+__global__ __launch_bounds__(kBlock) void k_check_increasing(const u64 *__restrict__ keys, u64 n,
+                                                             u32 *__restrict__ err) {
+  const auto body = [&](const u64 i) {
+    if (i < n && keys[i - 1] >= keys[i]) atomicOr(err, 1u);
+  };
+  for (u64 t = lane0() + 1; t < n; t += lanes()) body(t);
 }
 
 // flag_k: first occurrence of a real k-mer key. With colors, flag_p: first
@@ -197,18 +233,20 @@ __global__ __launch_bounds__(kBlock) void k_distinct_flags(const u64 *__restrict
                                                            u32 *__restrict__ cols, u64 n,
                                                            u8 *__restrict__ flag_k,
                                                            u8 *__restrict__ flag_p) {
-  const u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const u64 key = keys[i];
-  const bool valid = key != kMax;
-  const bool new_key = valid && (i == 0 || keys[i - 1] != key);
-  if (flag_k) flag_k[i] = new_key;
-  if (flag_p) {
-    const u32 c = cols[i] & ~kTag;
-    const bool new_pair = valid && (new_key || (cols[i - 1] & ~kTag) != c);
-    flag_p[i] = new_pair;
-    if (new_key) cols[i] = c | kTag;
-  }
+  const auto body = [&](const u64 i) {
+    if (i >= n) return;
+    const u64 key = keys[i];
+    const bool valid = key != kMax;
+    const bool new_key = valid && (i == 0 || keys[i - 1] != key);
+    if (flag_k) flag_k[i] = new_key;
+    if (flag_p) {
+      const u32 c = cols[i] & ~kTag;
+      const bool new_pair = valid && (new_key || (cols[i - 1] & ~kTag) != c);
+      flag_p[i] = new_pair;
+      if (new_key) cols[i] = c | kTag;
+    }
+  };
+  for (u64 t = lane0(); t < n; t += lanes()) body(t);
 }
 
 struct TagFlag {
@@ -216,8 +254,10 @@ struct TagFlag {
 };
 
 __global__ __launch_bounds__(kBlock) void k_strip_tags(u32 *__restrict__ cols, u64 n) {
-  const u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) cols[i] &= ~kTag;
+  const auto body = [&](const u64 i) {
+    if (i < n) cols[i] &= ~kTag;
+  };
+  for (u64 t = lane0(); t < n; t += lanes()) body(t);
 }
 
 // ---------------------------------------------------------------- 4-5. SBWT
@@ -241,28 +281,32 @@ struct SuffixTable {  // bucket b = top `bits` bits of a (k-1)-mer
 __global__ __launch_bounds__(kBlock) void k_bucket_table(const u64 *__restrict__ K, u64 nk,
                                                          u32 shift, u64 buckets,
                                                          u64 *__restrict__ T) {
-  const u64 b = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (b > buckets) return;
-  u64 lo = 0, hi = nk;
-  while (lo < hi) {
-    const u64 m = (lo + hi) >> 1;
-    if (((K[m] >> 2) >> shift) < b) lo = m + 1;
-    else hi = m;
-  }
-  T[b] = lo;
+  const auto body = [&](const u64 b) {
+    if (b > buckets) return;
+    u64 lo = 0, hi = nk;
+    while (lo < hi) {
+      const u64 m = (lo + hi) >> 1;
+      if (((K[m] >> 2) >> shift) < b) lo = m + 1;
+      else hi = m;
+    }
+    T[b] = lo;
+  };
+  for (u64 t = lane0(); t < buckets + 1; t += lanes()) body(t);
 }
 
 // (k-1)-prefixes of k-mers that are no k-mer's (k-1)-suffix (with repeats).
 __global__ __launch_bounds__(kBlock) void k_missing_prefixes(SuffixTable st, u64 pmask,
                                                              u64 *__restrict__ out, u64 cap,
                                                              unsigned long long *__restrict__ count) {
-  const u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= st.nk) return;
-  const u64 q = st.K[i] & pmask;
-  const u64 j = st.lower(q);
-  if (j < st.nk && (st.K[j] >> 2) == q) return;
-  const u64 slot = atomicAdd(count, 1ULL);
-  if (slot < cap) out[slot] = q;
+  const auto body = [&](const u64 i) {
+    if (i >= st.nk) return;
+    const u64 q = st.K[i] & pmask;
+    const u64 j = st.lower(q);
+    if (j < st.nk && (st.K[j] >> 2) == q) return;
+    const u64 slot = atomicAdd(count, 1ULL);
+    if (slot < cap) out[slot] = q;
+  };
+  for (u64 t = lane0(); t < st.nk; t += lanes()) body(t);
 }
 
 __device__ inline u64 upper_u64(const u64 *a, u64 n, u64 x) {  // # a[j] <= x
@@ -288,8 +332,10 @@ __device__ inline u64 lower_u64(const u64 *a, u64 n, u64 x) {  // # a[j] < x
 __global__ __launch_bounds__(kBlock) void k_real_before(const u64 *__restrict__ K, u64 nk,
                                                         const u64 *__restrict__ dkey, u64 nd,
                                                         u64 *__restrict__ out) {
-  const u64 j = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (j < nd) out[j] = lower_u64(K, nk, dkey[j]);
+  const auto body = [&](const u64 j) {
+    if (j < nd) out[j] = lower_u64(K, nk, dkey[j]);
+  };
+  for (u64 t = lane0(); t < nd; t += lanes()) body(t);
 }
 
 struct Nodes {  // the merged node order: real k-mers and dummies
@@ -306,39 +352,41 @@ struct Nodes {  // the merged node order: real k-mers and dummies
 // key is (primary >> 2, min(len, k - 1)); a start differs from its predecessor.
 __global__ __launch_bounds__(kBlock) void k_group_starts(Nodes N, u64 words,
                                                          u64 *__restrict__ sgs) {
-  const u64 w = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (w >= words) return;
-  const u64 v0 = w * 64;
-  u64 t = lower_u64(N.dpos, N.nd, v0);  // dummies before v0
-  u64 pk = kMax;
-  u32 pl = 0xFFFFFFFFu;
-  if (v0 > 0) {  // the node before v0
-    if (t > 0 && N.dpos[t - 1] == v0 - 1) {
-      pk = N.dkey[t - 1] >> 2;
-      pl = min(static_cast<u32>(N.dlen[t - 1]), N.k - 1);
-    } else {
-      pk = N.K[v0 - 1 - t] >> 2;
-      pl = N.k - 1;
+  const auto body = [&](const u64 w) {
+    if (w >= words) return;
+    const u64 v0 = w * 64;
+    u64 t = lower_u64(N.dpos, N.nd, v0);  // dummies before v0
+    u64 pk = kMax;
+    u32 pl = 0xFFFFFFFFu;
+    if (v0 > 0) {  // the node before v0
+      if (t > 0 && N.dpos[t - 1] == v0 - 1) {
+        pk = N.dkey[t - 1] >> 2;
+        pl = min(static_cast<u32>(N.dlen[t - 1]), N.k - 1);
+      } else {
+        pk = N.K[v0 - 1 - t] >> 2;
+        pl = N.k - 1;
+      }
     }
-  }
-  u64 word = 0;
-  const u64 v1 = v0 + 64 < N.n ? v0 + 64 : N.n;
-  for (u64 v = v0; v < v1; ++v) {
-    u64 ck;
-    u32 cl;
-    if (t < N.nd && N.dpos[t] == v) {
-      ck = N.dkey[t] >> 2;
-      cl = min(static_cast<u32>(N.dlen[t]), N.k - 1);
-      ++t;
-    } else {
-      ck = N.K[v - t] >> 2;
-      cl = N.k - 1;
+    u64 word = 0;
+    const u64 v1 = v0 + 64 < N.n ? v0 + 64 : N.n;
+    for (u64 v = v0; v < v1; ++v) {
+      u64 ck;
+      u32 cl;
+      if (t < N.nd && N.dpos[t] == v) {
+        ck = N.dkey[t] >> 2;
+        cl = min(static_cast<u32>(N.dlen[t]), N.k - 1);
+        ++t;
+      } else {
+        ck = N.K[v - t] >> 2;
+        cl = N.k - 1;
+      }
+      if (v == 0 || ck != pk || cl != pl) word |= 1ULL << (v & 63);
+      pk = ck;
+      pl = cl;
     }
-    if (v == 0 || ck != pk || cl != pl) word |= 1ULL << (v & 63);
-    pk = ck;
-    pl = cl;
-  }
-  sgs[w] = word;
+    sgs[w] = word;
+  };
+  for (u64 t = lane0(); t < words; t += lanes()) body(t);
 }
 
 // Edges into real k-mers: k-mer v = x[0..k-1] hangs off the first node of the
@@ -348,56 +396,62 @@ __global__ __launch_bounds__(kBlock) void k_real_edges(SuffixTable st, Nodes N, 
                                                        u64 *__restrict__ b0, u64 *__restrict__ b1,
                                                        u64 *__restrict__ b2, u64 *__restrict__ b3,
                                                        u32 *__restrict__ err) {
-  const u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= N.nk) return;
-  const u64 key = N.K[i];
-  const u64 q = key & pmask;
-  const u32 c = static_cast<u32>(key >> (2 * (N.k - 1))) & 3;
-  const u64 j = st.lower(q);
-  u64 s;
-  if (j < N.nk && (N.K[j] >> 2) == q) {
-    s = j + upper_u64(N.dkey, N.nd, N.K[j]);
-  } else {  // the dummy (q << 2, k - 1)
-    const u64 dk = q << 2;
-    u64 lo = 0, hi = N.nd;
-    while (lo < hi) {
-      const u64 m = (lo + hi) >> 1;
-      if (N.dkey[m] < dk || (N.dkey[m] == dk && N.dlen[m] < N.k - 1)) lo = m + 1;
-      else hi = m;
+  const auto body = [&](const u64 i) {
+    if (i >= N.nk) return;
+    const u64 key = N.K[i];
+    const u64 q = key & pmask;
+    const u32 c = static_cast<u32>(key >> (2 * (N.k - 1))) & 3;
+    const u64 j = st.lower(q);
+    u64 s;
+    if (j < N.nk && (N.K[j] >> 2) == q) {
+      s = j + upper_u64(N.dkey, N.nd, N.K[j]);
+    } else {  // the dummy (q << 2, k - 1)
+      const u64 dk = q << 2;
+      u64 lo = 0, hi = N.nd;
+      while (lo < hi) {
+        const u64 m = (lo + hi) >> 1;
+        if (N.dkey[m] < dk || (N.dkey[m] == dk && N.dlen[m] < N.k - 1)) lo = m + 1;
+        else hi = m;
+      }
+      if (lo >= N.nd || N.dkey[lo] != dk || N.dlen[lo] != N.k - 1) {
+        atomicOr(err, 2u);
+        return;
+      }
+      s = N.dpos[lo];
     }
-    if (lo >= N.nd || N.dkey[lo] != dk || N.dlen[lo] != N.k - 1) {
-      atomicOr(err, 2u);
-      return;
-    }
-    s = N.dpos[lo];
-  }
-  u64 *b = c == 0 ? b0 : c == 1 ? b1 : c == 2 ? b2 : b3;
-  atomicOr(reinterpret_cast<unsigned long long *>(b + (s >> 6)), 1ULL << (s & 63));
+    u64 *b = c == 0 ? b0 : c == 1 ? b1 : c == 2 ? b2 : b3;
+    atomicOr(reinterpret_cast<unsigned long long *>(b + (s >> 6)), 1ULL << (s & 63));
+  };
+  for (u64 t = lane0(); t < N.nk; t += lanes()) body(t);
 }
 
 __global__ __launch_bounds__(kBlock) void k_set_bits(const u64 *__restrict__ list, u64 n,
                                                      u64 *__restrict__ b0, u64 *__restrict__ b1,
                                                      u64 *__restrict__ b2, u64 *__restrict__ b3) {
-  const u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const u64 e = list[i];  // node << 2 | letter
-  const u64 s = e >> 2;
-  const u32 c = e & 3;
-  u64 *b = c == 0 ? b0 : c == 1 ? b1 : c == 2 ? b2 : b3;
-  atomicOr(reinterpret_cast<unsigned long long *>(b + (s >> 6)), 1ULL << (s & 63));
+  const auto body = [&](const u64 i) {
+    if (i >= n) return;
+    const u64 e = list[i];  // node << 2 | letter
+    const u64 s = e >> 2;
+    const u32 c = e & 3;
+    u64 *b = c == 0 ? b0 : c == 1 ? b1 : c == 2 ? b2 : b3;
+    atomicOr(reinterpret_cast<unsigned long long *>(b + (s >> 6)), 1ULL << (s & 63));
+  };
+  for (u64 t = lane0(); t < n; t += lanes()) body(t);
 }
 
 // ---------------------------------------------------------------- 6. color sets
 __global__ __launch_bounds__(kBlock) void k_list_hash(const u64 *__restrict__ run,
                                                       const u32 *__restrict__ cols, u64 nk,
                                                       u64 *__restrict__ h, u32 *__restrict__ err) {
-  const u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= nk) return;
-  const u64 a = run[i], b = run[i + 1];
-  u64 x = 0x12345678ULL + (b - a);
-  for (u64 j = a; j < b; ++j) x = mix64(x ^ cols[j]);
-  if (x == kMax) atomicOr(err, 4u);  // kMax marks an empty table slot
-  h[i] = x;
+  const auto body = [&](const u64 i) {
+    if (i >= nk) return;
+    const u64 a = run[i], b = run[i + 1];
+    u64 x = 0x12345678ULL + (b - a);
+    for (u64 j = a; j < b; ++j) x = mix64(x ^ cols[j]);
+    if (x == kMax) atomicOr(err, 4u);  // kMax marks an empty table slot
+    h[i] = x;
+  };
+  for (u64 t = lane0(); t < nk; t += lanes()) body(t);
 }
 
 struct HashTable {
@@ -421,21 +475,23 @@ __global__ __launch_bounds__(kBlock) void k_table_insert(HashTable t, const u64 
                                                          u64 nk,
                                                          unsigned long long *__restrict__ distinct,
                                                          u32 *__restrict__ err) {
-  const u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= nk) return;
-  const u64 x = h[i];
-  u64 s = x & t.mask;
-  for (u64 probes = 0; probes <= t.mask; ++probes) {
-    const unsigned long long prev =
-        atomicCAS(reinterpret_cast<unsigned long long *>(t.keys + s), kMax, x);
-    if (prev == kMax || prev == x) {
-      if (prev == kMax) atomicAdd(distinct, 1ULL);
-      atomicMin(reinterpret_cast<unsigned long long *>(t.reps + s), i);
-      return;
+  const auto body = [&](const u64 i) {
+    if (i >= nk) return;
+    const u64 x = h[i];
+    u64 s = x & t.mask;
+    for (u64 probes = 0; probes <= t.mask; ++probes) {
+      const unsigned long long prev =
+          atomicCAS(reinterpret_cast<unsigned long long *>(t.keys + s), kMax, x);
+      if (prev == kMax || prev == x) {
+        if (prev == kMax) atomicAdd(distinct, 1ULL);
+        atomicMin(reinterpret_cast<unsigned long long *>(t.reps + s), i);
+        return;
+      }
+      s = (s + 1) & t.mask;
     }
-    s = (s + 1) & t.mask;
-  }
-  atomicOr(err, 8u);  // table full
+    atomicOr(err, 8u);  // table full
+  };
+  for (u64 t = lane0(); t < nk; t += lanes()) body(t);
 }
 
 struct Occupied {
@@ -444,17 +500,21 @@ struct Occupied {
 
 __global__ __launch_bounds__(kBlock) void k_table_ids(HashTable t, const u64 *__restrict__ hs,
                                                       u64 m) {
-  const u64 id = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (id >= m) return;
-  const u64 s = t.slot_of(hs[id]);
-  if (s != kMax) t.ids[s] = static_cast<u32>(id);
+  const auto body = [&](const u64 id) {
+    if (id >= m) return;
+    const u64 s = t.slot_of(hs[id]);
+    if (s != kMax) t.ids[s] = static_cast<u32>(id);
+  };
+  for (u64 t = lane0(); t < m; t += lanes()) body(t);
 }
 
 __global__ __launch_bounds__(kBlock) void k_set_sizes(const u64 *__restrict__ reps,
                                                       const u64 *__restrict__ run, u64 m,
                                                       u64 *__restrict__ sizes) {
-  const u64 id = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (id < m) sizes[id] = run[reps[id] + 1] - run[reps[id]];
+  const auto body = [&](const u64 id) {
+    if (id < m) sizes[id] = run[reps[id] + 1] - run[reps[id]];
+  };
+  for (u64 t = lane0(); t < m; t += lanes()) body(t);
 }
 
 __global__ __launch_bounds__(kBlock) void k_set_lists(const u64 *__restrict__ reps,
@@ -462,11 +522,13 @@ __global__ __launch_bounds__(kBlock) void k_set_lists(const u64 *__restrict__ re
                                                       const u64 *__restrict__ offs,
                                                       const u32 *__restrict__ cols, u64 m,
                                                       u32 *__restrict__ ids) {
-  const u64 id = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (id >= m) return;
-  const u64 a = run[reps[id]], b = run[reps[id] + 1];
-  u64 o = offs[id];
-  for (u64 j = a; j < b; ++j) ids[o++] = cols[j];
+  const auto body = [&](const u64 id) {
+    if (id >= m) return;
+    const u64 a = run[reps[id]], b = run[reps[id] + 1];
+    u64 o = offs[id];
+    for (u64 j = a; j < b; ++j) ids[o++] = cols[j];
+  };
+  for (u64 t = lane0(); t < m; t += lanes()) body(t);
 }
 
 // node_set[node of k-mer i] = its set id; the k-mer's list must equal the
@@ -477,21 +539,23 @@ __global__ __launch_bounds__(kBlock) void k_node_sets(HashTable t, const u64 *__
                                                       const u64 *__restrict__ real_before, u64 nd,
                                                       u32 *__restrict__ node_set,
                                                       u32 *__restrict__ err) {
-  const u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= nk) return;
-  const u64 s = t.slot_of(h[i]);
-  if (s == kMax) {
-    atomicOr(err, 16u);
-    return;
-  }
-  const u64 rep = t.reps[s];
-  if (rep != i) {
-    const u64 a = run[i], b = run[i + 1], ra = run[rep], rb = run[rep + 1];
-    bool same = b - a == rb - ra;
-    for (u64 j = 0; same && j < b - a; ++j) same = cols[a + j] == cols[ra + j];
-    if (!same) atomicOr(err, 32u);
-  }
-  node_set[i + upper_u64(real_before, nd, i)] = t.ids[s];
+  const auto body = [&](const u64 i) {
+    if (i >= nk) return;
+    const u64 s = t.slot_of(h[i]);
+    if (s == kMax) {
+      atomicOr(err, 16u);
+      return;
+    }
+    const u64 rep = t.reps[s];
+    if (rep != i) {
+      const u64 a = run[i], b = run[i + 1], ra = run[rep], rb = run[rep + 1];
+      bool same = b - a == rb - ra;
+      for (u64 j = 0; same && j < b - a; ++j) same = cols[a + j] == cols[ra + j];
+      if (!same) atomicOr(err, 32u);
+    }
+    node_set[i + upper_u64(real_before, nd, i)] = t.ids[s];
+  };
+  for (u64 t = lane0(); t < nk; t += lanes()) body(t);
 }
 
 // ---------------------------------------------------------------- 7. key marks
@@ -529,8 +593,10 @@ __device__ inline void or_bit(u64 *b, u64 v) {
 
 __global__ __launch_bounds__(kBlock) void k_popcounts(const u64 *__restrict__ b, u64 words,
                                                       u64 *__restrict__ out) {
-  const u64 w = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (w < words) out[w] = __popcll(b[w]);
+  const auto body = [&](const u64 w) {
+    if (w < words) out[w] = __popcll(b[w]);
+  };
+  for (u64 t = lane0(); t < words; t += lanes()) body(t);
 }
 
 // Marks of nodes that end a chain by themselves: a real node that shares its
@@ -538,14 +604,15 @@ __global__ __launch_bounds__(kBlock) void k_popcounts(const u64 *__restrict__ b,
 // ballot per 64 nodes (the wave covers one word).
 __global__ __launch_bounds__(kBlock) void k_marks_local(Graph g, u64 words,
                                                         u64 *__restrict__ marks) {
-  const u64 v = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  bool m = false;
-  if (v < g.n && g.node_set[v] != kNone) {
-    const u64 w = g.single_successor(v);
-    m = w == kMax || g.node_set[w] != g.node_set[v];
+  for (u64 v = lane0(); v < words * 64; v += lanes()) {  // whole waves: the stride is 64-aligned
+    bool m = false;
+    if (v < g.n && g.node_set[v] != kNone) {
+      const u64 w = g.single_successor(v);
+      m = w == kMax || g.node_set[w] != g.node_set[v];
+    }
+    const u64 word = __ballot(m);
+    if ((threadIdx.x & 63) == 0) marks[v >> 6] = word;
   }
-  const u64 word = __ballot(m);
-  if ((threadIdx.x & 63) == 0 && (v >> 6) < words) marks[v >> 6] = word;
 }
 
 __device__ inline bool chain_node(const Graph &g, const u64 *marks1, u64 v) {
@@ -554,10 +621,12 @@ __device__ inline bool chain_node(const Graph &g, const u64 *marks1, u64 v) {
 
 __global__ __launch_bounds__(kBlock) void k_has_pred(Graph g, const u64 *__restrict__ marks1,
                                                      u64 *__restrict__ has_pred) {
-  const u64 v = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (v >= g.n || !chain_node(g, marks1, v)) return;
-  const u64 w = g.single_successor(v);
-  if (!test_bit(marks1, w)) or_bit(has_pred, w);
+  const auto body = [&](const u64 v) {
+    if (v >= g.n || !chain_node(g, marks1, v)) return;
+    const u64 w = g.single_successor(v);
+    if (!test_bit(marks1, w)) or_bit(has_pred, w);
+  };
+  for (u64 t = lane0(); t < g.n; t += lanes()) body(t);
 }
 
 // One lane per chain: from a chain's first node, mark every d-th node until a
@@ -567,37 +636,43 @@ __global__ __launch_bounds__(kBlock) void k_walk_chains(Graph g, const u64 *__re
                                                         u64 *__restrict__ marks,
                                                         u64 *__restrict__ seen,
                                                         u32 *__restrict__ err) {
-  const u64 v = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (v >= g.n || !chain_node(g, marks1, v) || test_bit(has_pred, v)) return;
-  u64 u = v, steps = 0;
-  while (!test_bit(marks1, u)) {
-    if (steps >= g.n) {  // a chain holds at most n nodes
-      atomicOr(err, 64u);
-      return;
+  const auto body = [&](const u64 v) {
+    if (v >= g.n || !chain_node(g, marks1, v) || test_bit(has_pred, v)) return;
+    u64 u = v, steps = 0;
+    while (!test_bit(marks1, u)) {
+      if (steps >= g.n) {  // a chain holds at most n nodes
+        atomicOr(err, 64u);
+        return;
+      }
+      or_bit(seen, u);
+      if (++steps % d == 0) or_bit(marks, u);
+      u = g.single_successor(u);
     }
-    or_bit(seen, u);
-    if (++steps % d == 0) or_bit(marks, u);
-    u = g.single_successor(u);
-  }
+  };
+  for (u64 t = lane0(); t < g.n; t += lanes()) body(t);
 }
 
 __global__ __launch_bounds__(kBlock) void k_cycle_nodes(Graph g, const u64 *__restrict__ marks1,
                                                         const u64 *__restrict__ seen,
                                                         u64 *__restrict__ out, u64 cap,
                                                         unsigned long long *__restrict__ count) {
-  const u64 v = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (v >= g.n || !chain_node(g, marks1, v) || test_bit(seen, v)) return;
-  const u64 slot = atomicAdd(count, 1ULL);
-  if (slot < cap) {
-    out[2 * slot] = v;
-    out[2 * slot + 1] = g.single_successor(v);
-  }
+  const auto body = [&](const u64 v) {
+    if (v >= g.n || !chain_node(g, marks1, v) || test_bit(seen, v)) return;
+    const u64 slot = atomicAdd(count, 1ULL);
+    if (slot < cap) {
+      out[2 * slot] = v;
+      out[2 * slot + 1] = g.single_successor(v);
+    }
+  };
+  for (u64 t = lane0(); t < g.n; t += lanes()) body(t);
 }
 
 __global__ __launch_bounds__(kBlock) void k_mark_list(const u64 *__restrict__ list, u64 n,
                                                       u64 *__restrict__ marks) {
-  const u64 i = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) or_bit(marks, list[i]);
+  const auto body = [&](const u64 i) {
+    if (i < n) or_bit(marks, list[i]);
+  };
+  for (u64 t = lane0(); t < n; t += lanes()) body(t);
 }
 
 struct MarkFlag {
@@ -625,6 +700,16 @@ u64 read_u64(const u64 *p) {
   u64 v = 0;
   GB_HIP(hipMemcpy(&v, p, 8, hipMemcpyDeviceToHost));
   return v;
+}
+
+u64 fingerprint(const u64 *keys, const u32 *cols, const u8 *flags, u64 n, DBuf<u64> &acc) {
+  GB_HIP(hipMemset(acc.p, 0, 8));
+  if (n) {
+    k_fingerprint<<<static_cast<u32>(std::min<u64>((n + kBlock - 1) / kBlock, 8192)), kBlock>>>(
+        keys, cols, flags, n, reinterpret_cast<unsigned long long *>(acc.p));
+    GB_HIP(hipGetLastError());
+  }
+  return read_u64(acc.p);
 }
 
 template <class In, class Flags, class Out>
@@ -671,11 +756,15 @@ struct Built {
   bld_colors *C = nullptr;
 };
 
+thread_local const char *g_stage = "start";  // named in error messages
+
 Built build(const u8 *codes_in, const u64 *ends_in, u64 n_seqs, u32 k, bool add_rc,
             const u32 *color_of_in, u32 d, bool keep_kmers, bld_gpu_stats *stats) {
   if (k < 2 || k > 31) throw std::invalid_argument("GPU build supports k in [2, 31]");
   const auto t_all = Clock::now();
-  bld_gpu_stats st{};
+  bld_gpu_stats dummy_stats{};
+  bld_gpu_stats &st = stats ? *stats : dummy_stats;
+  st = bld_gpu_stats{};
   Arena ar;
   Temp tmp(ar);
   const bool colors = color_of_in != nullptr;
@@ -705,6 +794,7 @@ Built build(const u8 *codes_in, const u64 *ends_in, u64 n_seqs, u32 k, bool add_
   const u64 N = L * (add_rc ? 2 : 1);
   st.occurrences = N;
 
+  g_stage = "keys";
   auto t0 = Clock::now();
   DBuf<u64> count;
   count.alloc(ar, 1);
@@ -741,7 +831,9 @@ Built build(const u8 *codes_in, const u64 *ends_in, u64 n_seqs, u32 k, bool add_
     GB_HIP(hipDeviceSynchronize());
     st.ms_keys = ms_since(t0);
   }
+  g_stage = "sort";
   t0 = Clock::now();
+  const u64 print_in = fingerprint(ka.p, colors ? ca.p : nullptr, nullptr, N, count);
   if (N > 1) {
     rocprim::double_buffer<u64> kbuf(ka.p, kb.p);
     size_t bytes = 0;
@@ -759,9 +851,12 @@ Built build(const u8 *codes_in, const u64 *ends_in, u64 n_seqs, u32 k, bool add_
     GB_HIP(hipGetLastError());
     if (read_err(err)) throw std::runtime_error("device sort produced an unsorted sequence");
   }
+  if (fingerprint(ka.p, colors ? ca.p : nullptr, nullptr, N, count) != print_in)
+    throw std::runtime_error("device sort lost or repeated elements");
   GB_HIP(hipDeviceSynchronize());
   st.ms_sort = ms_since(t0);
   t0 = Clock::now();
+  g_stage = "distinct";
   // ---- 3. distinct k-mers (into kb) and (k-mer, color) pairs (into cb)
   u64 nk = 0, P = 0;
   DBuf<u64> run;
@@ -775,7 +870,19 @@ Built build(const u8 *codes_in, const u64 *ends_in, u64 n_seqs, u32 k, bool add_
       GB_HIP(hipGetLastError());
     }
     nk = select_flagged(tmp, ka.p, fk.p, kb.p, N, count);
-    if (colors) P = select_flagged(tmp, ca.p, fp.p, cb.p, N, count);
+    if (fingerprint(ka.p, nullptr, fk.p, N, count) != fingerprint(kb.p, nullptr, nullptr, nk, count))
+      throw std::runtime_error("distinct k-mer compaction lost or repeated elements");
+    if (nk > 1) {
+      k_check_increasing<<<grid_for(nk), kBlock>>>(kb.p, nk, err.p);
+      GB_HIP(hipGetLastError());
+      if (read_err(err)) throw std::runtime_error("distinct k-mers are not strictly increasing");
+    }
+    if (colors) {
+      P = select_flagged(tmp, ca.p, fp.p, cb.p, N, count);
+      if (fingerprint(nullptr, ca.p, fp.p, N, count) !=
+          fingerprint(nullptr, cb.p, nullptr, P, count))
+        throw std::runtime_error("distinct pair compaction lost or repeated elements");
+    }
   }
   ka.release();
   ca.release();
@@ -795,6 +902,7 @@ Built build(const u8 *codes_in, const u64 *ends_in, u64 n_seqs, u32 k, bool add_
   st.kmers = nk;
   st.pairs = P;
 
+  g_stage = "dummies";
   // ---- 4. dummies
   auto *S = new bld_sbwt();
   std::unique_ptr<bld_sbwt> S_guard(S);
@@ -833,6 +941,7 @@ Built build(const u8 *codes_in, const u64 *ends_in, u64 n_seqs, u32 k, bool add_
   }
   std::sort(missing.begin(), missing.end());
   missing.erase(std::unique(missing.begin(), missing.end()), missing.end());
+  st.missing_prefixes = missing.size();
   {
     vector<std::pair<u64, u8>> dm;
     dm.reserve(1 + missing.size() * (k - 1));
@@ -869,6 +978,7 @@ Built build(const u8 *codes_in, const u64 *ends_in, u64 n_seqs, u32 k, bool add_
   for (u64 j = 0; j < nd; ++j) pos[j] = j + rb[j];
   GB_HIP(hipMemcpy(dpos.p, pos.data(), nd * 8, hipMemcpyHostToDevice));
 
+  g_stage = "edges";
   // ---- 5. C map, group starts, edges
   {
     u64 cnt[4] = {0, 0, 0, 0};
@@ -946,6 +1056,7 @@ Built build(const u8 *codes_in, const u64 *ends_in, u64 n_seqs, u32 k, bool add_
     std::unique_ptr<bld_colors> C_guard(C);
     C->n = n;
     C->num_colors = static_cast<u64>(max_color) + 1;
+  g_stage = "color sets";
     // ---- 6. color sets
     DBuf<u64> h;
     h.alloc(ar, nk);
@@ -1048,6 +1159,7 @@ Built build(const u8 *codes_in, const u64 *ends_in, u64 n_seqs, u32 k, bool add_
     tid.release();
     run.release();
     cb.release();
+  g_stage = "key marks";
     // ---- 7. key k-mer marks
     DBuf<u64> pre[4];
     Graph g{};
@@ -1155,7 +1267,6 @@ Built build(const u8 *codes_in, const u64 *ends_in, u64 n_seqs, u32 k, bool add_
   }
   st.peak_device_bytes = ar.peak;
   st.ms_total = ms_since(t_all);
-  if (stats) *stats = st;
   out.S = S_guard.release();
   return out;
 }
@@ -1168,13 +1279,14 @@ extern "C" bld_sbwt *bld_gpu_build(const uint8_t *codes, const uint64_t *seq_end
                                    bld_colors **colors, bld_gpu_stats *stats, char *err,
                                    uint64_t err_len) {
   if (colors) *colors = nullptr;
+  g_stage = "start";
   try {
     if (color_of && !colors) throw std::invalid_argument("colors requested without an output");
     Built b = build(codes, seq_ends, n_seqs, k, add_rc != 0, color_of, d, keep_kmers != 0, stats);
     if (colors) *colors = b.C;
     return b.S;
   } catch (const std::exception &e) {
-    if (err && err_len) std::snprintf(err, err_len, "%s", e.what());
+    if (err && err_len) std::snprintf(err, err_len, "%s (stage: %s)", e.what(), g_stage);
     return nullptr;
   }
 }
