@@ -423,7 +423,8 @@ extern "C" int bld_sbwt_write(const bld_sbwt *s, const char *path) {
   o.bit_vector(s->bits[4], s->n);
   o.u64(4 * 8);  // C array (sigma entries)
   for (int c = 0; c < 4; ++c) o.u64(s->c_map[c]);
-  // precomputed intervals of all 8-mers (first char in the top bits), (-1,-1) if empty
+  // precomputed intervals of all 8-mers, (-1,-1) if empty; 8-mer x has its first
+  // char in the low bits (the order the reference's fixture index holds them in)
   const uint32_t pk = 8;
   WordRank wr[4];
   for (int c = 0; c < 4; ++c) wr[c].build(s->bits[c]);
@@ -431,7 +432,7 @@ extern "C" int bld_sbwt_write(const bld_sbwt *s, const char *path) {
   for (uint64_t x = 0; x < (1ULL << (2 * pk)); ++x) {
     uint64_t l = 0, r = s->n - 1;
     bool empty = false;
-    for (int sh = 2 * pk - 2; sh >= 0 && !empty; sh -= 2) {
+    for (uint32_t sh = 0; sh < 2 * pk && !empty; sh += 2) {
       const int c = (x >> sh) & 3;
       l = s->c_map[c] + wr[c].rank(l);
       r = s->c_map[c] + wr[c].rank(r + 1) - 1;
