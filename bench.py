@@ -485,10 +485,34 @@ def mode_colors(args, configs, S, torch, W):
             times["color"].append(t2 - t1)
             times["total"].append(t2 - t0)
     med = {k: float(np.median(v)) for k, v in times.items()}
+    search_ms = s.last_timings()["search_ms"]
+    # census of the index stage (untimed): lines read by the search and the key move
+    s.set_stats(True)
+    s.search_reads_device(d_chars.data_ptr(), wl.n_chars, d_ends.data_ptr(), wl.n_reads,
+                          d_res.data_ptr())
+    s.sync()
+    stats = s.last_stats()
+    s.set_stats(False)
+    n_kmers = wl.n_kmers()
+    algo_bytes = LINE * S.Searcher.lines(stats) + 8 * n_kmers + 0.375 * wl.n_chars
+    node_line_bytes = (int(built.n) // 96 + 1) * 128
+    probe = max((S.random_line_probe(min(node_line_bytes, 32 << 30), bpc) for bpc in (4, 8)),
+                key=lambda r: r["gbs"])
+    achieved = algo_bytes / (search_ms / 1e3) / 1e9
+    log(f"index stage census: {S.Searcher.lines(stats) / n_kmers:.2f} lines per k-mer, "
+        f"{achieved:.0f} GB/s over {search_ms:.2f} ms; random-line ceiling at "
+        f"{probe['table_bytes'] >> 20} MiB: {probe['gbs']:.0f} GB/s")
     out = {"metric": "reads/s (color)", "mode": "colors",
            "value": wl.n_reads / med["total"], "unit": "reads/s",
            "color_stage_reads_per_s": wl.n_reads / med["color"],
-           "index_stage_kmers_per_s": wl.n_kmers() / med["index"],
+           "index_stage_kmers_per_s": n_kmers / med["index"],
+           "index_roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0,
+                              "unit": "GB/s", "frac": achieved / 8000.0,
+                              "search_ms": search_ms, "algorithmic_bytes": algo_bytes,
+                              "node_line_bytes": node_line_bytes,
+                              "random_line_ceiling_gbs": probe["gbs"],
+                              "frac_of_random_line_ceiling": achieved / probe["gbs"],
+                              "census": stats},
            "color_kernel_ms": cs.last_time(), "ids_per_step": n_ids,
            "config": {"workload": wl.name + " " + wl.description, "sbwt_nodes": int(built.n),
                       "num_colors": int(cidx.num_colors),

@@ -158,10 +158,11 @@ __device__ __forceinline__ u64 block_reserve(u32 cnt, u64 *n_out) {
 
 constexpr u32 kSeedItems = 16;  // presearch entries per thread
 
-// The non-empty presearch intervals (cap: a level holds at most n entries,
-// cap is n + 1; nothing is written past it).
-__global__ __launch_bounds__(256) void k_hash_seed(const uint2 *__restrict__ pre, u64 count,
-                                                   PrefixEnt *out, u64 *n_out, u64 cap) {
+// The non-empty presearch intervals of entries [first, first + count)
+// (nothing is written past cap; the count goes on).
+__global__ __launch_bounds__(256) void k_hash_seed(const uint2 *__restrict__ pre, u64 first,
+                                                   u64 count, PrefixEnt *out, u64 *n_out, u64 cap) {
+  pre += first;
   const u64 tile = 256ULL * kSeedItems;
   for (u64 b = static_cast<u64>(blockIdx.x) * tile; b < count; b += static_cast<u64>(gridDim.x) * tile) {
     const u64 x0 = b + threadIdx.x * kSeedItems;
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(256) void k_hash_seed(const uint2 *__restrict__ pre
       if (x0 + q < count) {
         const uint2 e = pre[x0 + q];
         if (e.x <= e.y) {
-          if (at < cap) out[at] = PrefixEnt{x0 + q, e.x, e.y};
+          if (at < cap) out[at] = PrefixEnt{first + x0 + q, e.x, e.y};
           ++at;
           --keep;
         }
@@ -234,8 +235,6 @@ __global__ __launch_bounds__(256) void k_hash_extend(IndexView ix, const PrefixE
     }
   }
 }
-
-__global__ void k_zero_u64(u64 *p) { *p = 0; }
 
 template <typename Idx, typename Pair>
 void build_presearch(const IndexView &ix, u32 kp, Pair *out) {
@@ -423,41 +422,88 @@ void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const
   }
 }
 
-// The k-mer table, on request (narrow indexes with k <= 32 and kp < k): two
-// level lists of n + 1 entries during the build, then 2^bits buckets of 16
-// bytes with 2^bits >= 2 (n + 1), built when it takes at most a quarter of
-// device memory. Returns whether the index has one.
+// The k-mer table, on request (narrow indexes with k <= 32 and kp < k):
+// 2^bits buckets of 16 bytes with 2^bits >= 2 (n + 1), built when it takes at
+// most half of device memory and leaves an eighth free. The build grows the
+// strings of each slice of presearch entries (a range of kp-letter prefixes)
+// one letter per level up to k in two level lists; slices are sized so their
+// lists fit beside the table, and a slice whose level overflows its list is
+// done again as four slices (inserting a k-mer twice leaves one bucket).
+// Returns whether the index has one.
 bool DeviceIndex::build_kmer_hash() {
   if (kmer_hash) return true;
   if (wide || k > 32 || kp >= k) return false;
   u32 bits = 1;
   while ((1ULL << bits) < 2 * (n + 1)) ++bits;
   const u64 table_bytes = (1ULL << bits) * sizeof(uint4);
-  const u64 list_bytes = (n + 1) * sizeof(PrefixEnt);
   size_t free_b = 0, total_b = 0;
   SBWT_HIP(hipMemGetInfo(&free_b, &total_b));
-  if (table_bytes > total_b / 4 || table_bytes + 2 * list_bytes > free_b / 2) return false;
+  const u64 reserve = total_b / 8;
+  if (table_bytes > total_b / 2 || table_bytes + reserve >= free_b) return false;
+  const u64 count = 1ULL << (2 * kp);
+  // list capacity: a whole level when it fits, else what the memory left allows
+  const u64 room = (free_b - reserve - table_bytes) / 2 / sizeof(PrefixEnt);
+  u64 cap = std::min<u64>(n + 1, room);
+  if (cap < std::min<u64>(n + 1, 1u << 16)) return false;  // too little memory left for lists
+  u64 first_len = count;  // initial slice: a power of 4 of entries whose share of n fits
+  while (first_len > 1 && (n + 1) / (count / first_len) * 3 / 2 > cap) first_len >>= 2;
+  // SBWT_AMD_KMER_TABLE_LIST_CAP caps the level lists and starts from one slice,
+  // so that slicing and the split of overflowing slices run on small indexes
+  if (const char *env = std::getenv("SBWT_AMD_KMER_TABLE_LIST_CAP")) {
+    cap = std::max<u64>(1024, std::min<u64>(cap, std::strtoull(env, nullptr, 10)));
+    first_len = count;
+  }
+  const u32 levels = k - kp + 1;
   PrefixEnt *lists[2] = {nullptr, nullptr};
   u64 *counts = nullptr;
-  SBWT_HIP(hipMalloc(&lists[0], list_bytes));
-  SBWT_HIP(hipMalloc(&lists[1], list_bytes));
-  SBWT_HIP(hipMalloc(&counts, 2 * sizeof(u64)));
+  SBWT_HIP(hipMalloc(&lists[0], cap * sizeof(PrefixEnt)));
+  SBWT_HIP(hipMalloc(&lists[1], cap * sizeof(PrefixEnt)));
+  SBWT_HIP(hipMalloc(&counts, levels * sizeof(u64)));
   SBWT_HIP(hipMalloc(&kmer_hash, table_bytes));
   SBWT_HIP(hipMemset(kmer_hash, 0, table_bytes));
-  SBWT_HIP(hipMemset(counts, 0, 2 * sizeof(u64)));
   kmer_hash_bits = bits;
   const IndexView ix = view();
-  const u64 count = 1ULL << (2 * kp);
-  k_hash_seed<<<static_cast<u32>(std::min<u64>((count + 256 * kSeedItems - 1) / (256 * kSeedItems), 1u << 20)), 256>>>(
-      static_cast<const uint2 *>(presearch), count, lists[0], counts, n + 1);
-  SBWT_HIP(hipGetLastError());
-  const u32 grid = static_cast<u32>(std::min<u64>((n + 1 + 255) / 256, 1u << 20));
-  for (u32 len = kp; len < k; ++len) {
-    const u32 from = (len - kp) & 1, to = from ^ 1;
-    k_zero_u64<<<1, 1>>>(counts + to);
-    k_hash_extend<<<grid, 256>>>(ix, lists[from], counts + from, lists[to], counts + to, n + 1,
-                                 len + 1 == k ? 1u : 0u, kmer_hash, bits);
-    SBWT_HIP(hipGetLastError());
+  std::vector<std::pair<u64, u64>> todo;  // (first entry, entries), done last first
+  for (u64 f = count; f > 0;) {
+    const u64 len = std::min(first_len, f);
+    f -= len;
+    todo.emplace_back(f, len);
+  }
+  std::vector<u64> level_counts(levels);
+  try {
+    while (!todo.empty()) {
+      const auto [first, len] = todo.back();
+      todo.pop_back();
+      SBWT_HIP(hipMemset(counts, 0, levels * sizeof(u64)));
+      k_hash_seed<<<static_cast<u32>(std::min<u64>((len + 256 * kSeedItems - 1) / (256 * kSeedItems), 1u << 20)), 256>>>(
+          static_cast<const uint2 *>(presearch), first, len, lists[0], counts, cap);
+      SBWT_HIP(hipGetLastError());
+      const u32 grid = static_cast<u32>(std::min<u64>((cap + 255) / 256, 1u << 20));
+      for (u32 l = kp; l < k; ++l) {
+        const u32 from = (l - kp) & 1, to = from ^ 1;
+        k_hash_extend<<<grid, 256>>>(ix, lists[from], counts + (l - kp), lists[to],
+                                     counts + (l - kp + 1), cap, l + 1 == k ? 1u : 0u, kmer_hash,
+                                     bits);
+        SBWT_HIP(hipGetLastError());
+      }
+      SBWT_HIP(hipMemcpy(level_counts.data(), counts, levels * sizeof(u64), hipMemcpyDeviceToHost));
+      // the last level's entries went to the table; an earlier level that
+      // overflowed lost strings, so the slice is done again in quarters
+      bool lost = false;
+      for (u32 l = 0; l + 1 < levels; ++l) lost |= level_counts[l] > cap;
+      if (lost) {
+        if (len < 4) throw std::runtime_error("k-mer table: one prefix overflows the level lists");
+        for (u32 q = 4; q-- > 0;) todo.emplace_back(first + q * (len / 4), len / 4);
+      }
+    }
+  } catch (...) {
+    (void)hipFree(lists[0]);
+    (void)hipFree(lists[1]);
+    (void)hipFree(counts);
+    (void)hipFree(kmer_hash);
+    kmer_hash = nullptr;
+    kmer_hash_bits = 0;
+    throw;
   }
   SBWT_HIP(hipDeviceSynchronize());
   (void)hipFree(lists[0]);

@@ -606,17 +606,23 @@ def test_key_kmer_move_across_work_items(algo, launch, tmp_path):
     assert np.array_equal(res, orc.index_reads(chars, rends, key_marks=km))
 
 
-@pytest.mark.parametrize("table", ["1", "0"], ids=["kmer-table", "no-table"])
+@pytest.mark.parametrize("table", ["1", "0", "sliced"], ids=["kmer-table", "no-table",
+                                                              "sliced-build"])
 @pytest.mark.parametrize("algo", ALGOS + [S.ALGO_STREAMING_ADAPTIVE],
                          ids=["streaming", "per-kmer", "adaptive"])
 def test_kmer_table_and_full_search_agree(genome_index, algo, table, monkeypatch):
     """Fresh searches start from the k-mer table (every whole-k-mer node by its
     code) when the index has one, else from the presearch interval and SBWT
-    steps; both give the oracle's results, with reads that miss often."""
+    steps; both give the oracle's results, with reads that miss often. The
+    sliced build (level lists of 4096 entries against ~0.4 M nodes: the first
+    slice overflows and is split into quarters down to slices that fit) gives
+    the same table."""
     g, b, path, orc = genome_index
     say(f"k-mer table={table} algo={algo}")
+    if table == "sliced":
+        monkeypatch.setenv("SBWT_AMD_KMER_TABLE_LIST_CAP", "4096")
     idx = S.Index.load(path)
-    if table == "1":
+    if table != "0":
         assert idx.build_kmer_table()
     chars, ends = _reads(g, 3000, W.uniform_lengths(52, 3000, 20, 300), seed=51, random_frac=0.2,
                          err=0.02, n_rate=0.005)
