@@ -73,6 +73,11 @@ def parse():
     ap.add_argument("--cpu-sample-reads", type=int, default=1_500_000,
                     help="reads of the batch timed on the CPU oracle (about 10 s of host work)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--leg-timeout", type=int, default=300)
+    ap.add_argument("--legs", default="colors,batched,serve",
+                    help="index mode: the other legs of the metric (colors: C3 reads/s; batched: "
+                         "C4 batched vs plain; serve: C5 p50/p99), each run first as a child "
+                         "process and summarized under \"legs\" in the one JSON line; empty: none")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kmer-table", action="store_true",
                     help="search without the index's k-mer table (fresh searches from the "
@@ -136,6 +141,7 @@ def main():
         return mode_cli_colors(args, configs, S, torch, W)
     if args.mode == "gather-probe":
         return mode_gather_probe(args, S)
+    legs = run_legs(args) if args.legs else None
     t0 = time.time()
     if args.config == "c1":
         wl = configs.c1(**({"n_reads": args.reads} if args.reads else {}))
@@ -298,6 +304,8 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if legs is not None:
+        out["legs"] = legs
     print(json.dumps(out), flush=True)
     return 0
 
@@ -351,6 +359,64 @@ def host_threads():
         return len(os.sched_getaffinity(0))
     except (AttributeError, OSError):
         return os.cpu_count() or 1
+
+
+def _leg_summary(leg, r):
+    """The figures of a leg's JSON line that the headline line carries."""
+    if leg == "colors":
+        return {"metric": "reads/s (color)", "value": r["value"], "unit": "reads/s",
+                "config": r["config"]["workload"], "sbwt_nodes": r["config"]["sbwt_nodes"],
+                "form": r["form"], "color_stage_reads_per_s": r["color_stage_reads_per_s"],
+                "index_stage_kmers_per_s": r["index_stage_kmers_per_s"],
+                "key_node_form_reads_per_s": r["key_node_form"]["reads_per_s"],
+                "outputs_identical": r["outputs_identical"],
+                "index_stage_roofline_frac": r["index_roofline"]["frac"],
+                "index_stage_frac_of_random_line_ceiling":
+                    r["index_roofline"]["frac_of_random_line_ceiling"],
+                "seconds_per_step": r["seconds"]}
+    if leg == "batched":
+        keep = ("reads", "batches", "plain_s", "batched_unfused_s", "batched_s", "speedup",
+                "speedup_unfused", "plain_reads_per_s", "batched_reads_per_s")
+        return {"metric": "reads/s (index + color), batched vs plain",
+                "config": r["config"]["workload"],
+                **{k: r[k] for k in keep if k in r}}
+    if leg == "serve":
+        runs = []
+        for run in r["runs"]:
+            runs.append({"rate_per_s": run["rate_per_s"], "returns": run["returns"],
+                         "requests": run["requests"], "reads": run["reads"],
+                         **{form: {k: run[form][k] for k in ("p50_ms", "p99_ms", "reads_per_s")}
+                            for form in ("served", "served_no_graphs", "plain")},
+                         "outputs_equal": run["outputs_equal"]})
+        return {"metric": "p50/p99 request latency + reads/s, served vs plain",
+                "config": r["config"], "latency_from": "scheduled arrival", "runs": runs}
+    return r
+
+
+def run_legs(args):
+    """The metric's other legs (C3 color search, C4 batched vs plain, C5 serving),
+    each as a child process on the same GPU before the headline measurement;
+    their stderr streams through, their JSON lines are summarized."""
+    import subprocess
+    out = {}
+    for leg in [x for x in args.legs.split(",") if x]:
+        cmd = [sys.executable, "-u", os.path.abspath(__file__), "--mode", leg, "--steps", "3",
+               "--warmup", "1"]
+        log(f"leg {leg}: {' '.join(cmd[2:])}")
+        t0 = time.time()
+        try:
+            p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, text=True,
+                               timeout=args.leg_timeout)
+            lines = [x for x in p.stdout.splitlines() if x.startswith("{")]
+            if p.returncode != 0 or not lines:
+                out[leg] = {"error": f"exit status {p.returncode}"}
+            else:
+                out[leg] = _leg_summary(leg, json.loads(lines[-1]))
+        except subprocess.TimeoutExpired:
+            out[leg] = {"error": f"no result within {args.leg_timeout} s"}
+        out[leg]["wall_s"] = time.time() - t0
+        log(f"leg {leg}: done in {out[leg]['wall_s']:.0f} s")
+    return out
 
 
 def _kmer_table(args, idx):
@@ -467,26 +533,48 @@ def mode_colors(args, configs, S, torch, W):
     cap = min(wl.n_reads * cidx.num_colors, wl.n_chars)
     d_ids = torch.empty(cap, dtype=torch.int32, device=dev)
     params = S.color_params(0.7)
-    times = {"index": [], "color": [], "total": []}
-    n_ids = 0
-    for it in range(args.warmup + args.steps):
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        s.search_reads_device(d_chars.data_ptr(), wl.n_chars, d_ends.data_ptr(), wl.n_reads,
-                              d_res.data_ptr())
-        stream.synchronize()
-        t1 = time.perf_counter()
-        n_ids = cs.pseudoalign_device(d_res.data_ptr(), s.kmer_ends_device(), wl.n_reads, params,
-                                      d_off.data_ptr(), d_ids.data_ptr(), cap)
-        stream.synchronize()
-        t2 = time.perf_counter()
-        if it >= args.warmup:
-            times["index"].append(t1 - t0)
-            times["color"].append(t2 - t1)
-            times["total"].append(t2 - t0)
-    med = {k: float(np.median(v)) for k, v in times.items()}
-    search_ms = s.last_timings()["search_ms"]
-    # census of the index stage (untimed): lines read by the search and the key move
+
+    def run(fused):
+        """W + K passes of both stages; the key-node form (fused False) or the
+        color-table form (the index stage writes key ranks or masks)."""
+        s.set_color_output(fused)
+        cs.set_input(idx if fused else None)
+        times = {"index": [], "color": [], "total": []}
+        n_ids = 0
+        for it in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            s.search_reads_device(d_chars.data_ptr(), wl.n_chars, d_ends.data_ptr(), wl.n_reads,
+                                  d_res.data_ptr())
+            stream.synchronize()
+            t1 = time.perf_counter()
+            n_ids = cs.pseudoalign_device(d_res.data_ptr(), s.kmer_ends_device(), wl.n_reads,
+                                          params, d_off.data_ptr(), d_ids.data_ptr(), cap)
+            stream.synchronize()
+            t2 = time.perf_counter()
+            if it >= args.warmup:
+                times["index"].append(t1 - t0)
+                times["color"].append(t2 - t1)
+                times["total"].append(t2 - t0)
+        med = {k: float(np.median(v)) for k, v in times.items()}
+        lists = (d_off.cpu().numpy().copy(), d_ids[:n_ids].cpu().numpy().copy())
+        return med, n_ids, lists, s.last_timings()["search_ms"], cs.last_time()
+
+    t0 = time.perf_counter()
+    kind = idx.build_color_table(cidx)
+    table_ms = 1000 * (time.perf_counter() - t0)
+    log(f"color table: {'masks' if kind == 1 else 'key ranks'} ({table_ms:.0f} ms)")
+    med_f, n_ids_f, lists_f, search_ms_f, color_ms_f = run(True)
+    med, n_ids, lists, search_ms, color_ms = run(False)
+    same = n_ids == n_ids_f and all(np.array_equal(a, b) for a, b in zip(lists, lists_f))
+    log(f"key nodes: {wl.n_reads / med['total'] / 1e6:.1f} M reads/s; color table: "
+        f"{wl.n_reads / med_f['total'] / 1e6:.1f} M reads/s; lists identical: {same}")
+    if not same:
+        raise SystemExit("the color-table form's color lists differ from the key-node form's")
+    # census of the index stage in the color-table form (untimed): lines read by
+    # the search and the key move
+    search_ms = search_ms_f
+    s.set_color_output(True)
     s.set_stats(True)
     s.search_reads_device(d_chars.data_ptr(), wl.n_chars, d_ends.data_ptr(), wl.n_reads,
                           d_res.data_ptr())
@@ -503,9 +591,16 @@ def mode_colors(args, configs, S, torch, W):
         f"{achieved:.0f} GB/s over {search_ms:.2f} ms; random-line ceiling at "
         f"{probe['table_bytes'] >> 20} MiB: {probe['gbs']:.0f} GB/s")
     out = {"metric": "reads/s (color)", "mode": "colors",
-           "value": wl.n_reads / med["total"], "unit": "reads/s",
-           "color_stage_reads_per_s": wl.n_reads / med["color"],
-           "index_stage_kmers_per_s": n_kmers / med["index"],
+           "value": wl.n_reads / med_f["total"], "unit": "reads/s",
+           "form": "color table (the index stage writes each found k-mer's key rank or mask)",
+           "color_stage_reads_per_s": wl.n_reads / med_f["color"],
+           "index_stage_kmers_per_s": n_kmers / med_f["index"],
+           "color_table": {"kind": "masks" if kind == 1 else "key ranks", "build_ms": table_ms},
+           "key_node_form": {"reads_per_s": wl.n_reads / med["total"],
+                             "color_stage_reads_per_s": wl.n_reads / med["color"],
+                             "index_stage_kmers_per_s": n_kmers / med["index"],
+                             "color_kernel_ms": color_ms, "seconds": med},
+           "outputs_identical": same,
            "index_roofline": {"bound": "hbm", "achieved": achieved, "peak": 8000.0,
                               "unit": "GB/s", "frac": achieved / 8000.0,
                               "search_ms": search_ms, "algorithmic_bytes": algo_bytes,
@@ -513,11 +608,11 @@ def mode_colors(args, configs, S, torch, W):
                               "random_line_ceiling_gbs": probe["gbs"],
                               "frac_of_random_line_ceiling": achieved / probe["gbs"],
                               "census": stats},
-           "color_kernel_ms": cs.last_time(), "ids_per_step": n_ids,
+           "color_kernel_ms": color_ms_f, "ids_per_step": n_ids,
            "config": {"workload": wl.name + " " + wl.description, "sbwt_nodes": int(built.n),
                       "num_colors": int(cidx.num_colors),
                       "color_sets": int(cidx.num_color_sets), "threshold": 0.7},
-           "seconds": med}
+           "seconds": med_f}
     print(json.dumps(out), flush=True)
     return 0
 
