@@ -2,6 +2,7 @@
 #include "server.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <stdexcept>
 
@@ -13,6 +14,8 @@ Server::Server(const DeviceIndex *index, const DeviceColors *colors, const Serve
     throw std::runtime_error("Not enough memory");
   if (cfg_.move_to_key_kmer && !index_->has_marks)
     throw std::runtime_error("move to key k-mer requested but the index has no key-k-mer marks");
+  if (cfg_.max_chars_per_slot < index_->k)
+    throw std::runtime_error("a slot must hold at least k chars");
   const u64 nc = colors_ ? colors_->num_colors : 0;
   const u64 C = cfg_.max_chars_per_slot, R = cfg_.max_reads_per_slot;
   const u64 ids_cap = colors_ ? std::min<u64>(C, R * nc) : 0;
@@ -51,6 +54,9 @@ Server::Server(const DeviceIndex *index, const DeviceColors *colors, const Serve
       s->ids_capacity = std::max<u64>(ids_cap, 1);
       SBWT_HIP(hipMalloc(&s->d_offsets, (R + 1) * 8));
       SBWT_HIP(hipMalloc(&s->d_ids, s->ids_capacity * 4));
+      SBWT_HIP(hipMalloc(&s->d_piece_counts, (nc + 1) * 4));
+      SBWT_HIP(hipMalloc(&s->d_piece_stats, 16));
+      SBWT_HIP(hipHostMalloc(reinterpret_cast<void **>(&s->h_piece), (nc + 4) * 4, hipHostMallocDefault));
     }
     slots_.push_back(std::move(s));
   }
@@ -71,9 +77,11 @@ Server::~Server() {
     for (auto &g : s->graphs) (void)hipGraphExecDestroy(g.second);
     s->colors.reset();
     s->searcher.reset();
-    void *dev[] = {s->d_chars, s->d_ends, s->d_results, s->d_offsets, s->d_ids};
+    void *dev[] = {s->d_chars, s->d_ends, s->d_results, s->d_offsets, s->d_ids,
+                   s->d_piece_counts, s->d_piece_stats};
     for (void *p : dev)
       if (p) (void)hipFree(p);
+    if (s->h_piece) (void)hipHostFree(s->h_piece);
     if (s->h_chars) (void)hipHostFree(s->h_chars);
     if (s->h_ends) (void)hipHostFree(s->h_ends);
     if (s->h_results) (void)hipHostFree(s->h_results);
@@ -173,8 +181,6 @@ void Server::process(Slot &s, Request &r) {
   for (u64 i = 0; i < n_reads; ++i) {
     if (r.in_ends[i] < r.ends_base || end_of(i) < prev || end_of(i) > n_chars)
       throw std::runtime_error("read_ends must be non-decreasing and end at n_chars");
-    if (end_of(i) - prev > cfg_.max_chars_per_slot)
-      throw std::runtime_error("a read is longer than the slot capacity");
     const u64 len = end_of(i) - prev;
     total_kmers += len >= k ? len - k + 1 : 0;
     prev = end_of(i);
@@ -189,6 +195,11 @@ void Server::process(Slot &s, Request &r) {
   while (read0 < n_reads) {
     // whole reads up to the slot capacity
     const u64 c0 = read0 ? end_of(read0 - 1) : 0;
+    if (end_of(read0) - c0 > cfg_.max_chars_per_slot) {
+      kmer_out += run_long_read(s, r, read0, kmer_out);
+      ++read0;
+      continue;
+    }
     u64 read1 = read0;
     while (read1 < n_reads && read1 - read0 < cfg_.max_reads_per_slot &&
            end_of(read1) - c0 <= cfg_.max_chars_per_slot)
@@ -238,6 +249,69 @@ u64 Server::run_batch(Slot &s, Request &r, u64 nc, u64 nr, u64 read0, u64 kmer_o
     for (u64 i = 0; i < nr; ++i) r.color_offsets[read0 + i + 1] = base + s.h_offsets[i + 1];
   }
   return nk;
+}
+
+// A read longer than the slot: pieces of at most max_chars_per_slot chars that
+// overlap by k - 1, so every k-mer lies in exactly one piece, in order (the
+// reference carries the last k - 1 chars of a batch into the next,
+// ContinuousSequenceFileParser.cpp:82-112). The k-mer results are the pieces'
+// in turn. The color counts and the read's found / not-found / invalid counts
+// add up over the pieces and are thresholded once, as a read alone in its
+// batch is (ContinuousColorResultsPrinter.hpp:235-263).
+u64 Server::run_long_read(Slot &s, Request &r, u64 i, u64 kmer_out) {
+  const u64 k = index_->k, C = cfg_.max_chars_per_slot;
+  const u64 a0 = i ? r.in_ends[i - 1] - r.ends_base : 0, end = r.in_ends[i] - r.ends_base;
+  const u64 nc_colors = colors_ ? colors_->num_colors : 0;
+  std::vector<u64> counts(nc_colors, 0);
+  u64 F = 0, NF = 0, INV = 0, out = kmer_out;
+  s.searcher->set_algorithm(kAlgoAuto);
+  s.searcher->set_lane_share(1);
+  for (u64 a = a0;;) {
+    const u64 b = std::min(end, a + C), nc = b - a;
+    std::memcpy(s.h_chars, r.in_chars + a, nc);
+    s.h_ends[0] = nc;
+    SBWT_HIP(hipMemcpyAsync(s.d_chars, s.h_chars, nc, hipMemcpyHostToDevice, s.stream));
+    SBWT_HIP(hipMemcpyAsync(s.d_ends, s.h_ends, 8, hipMemcpyHostToDevice, s.stream));
+    u64 nk = 0;
+    s.searcher->search_reads_device(s.d_chars, nc, s.d_ends, 1, s.d_results, &nk);
+    if (nk && cfg_.kmer_results)
+      SBWT_HIP(hipMemcpyAsync(s.h_results, s.d_results, nk * 8, hipMemcpyDeviceToHost, s.stream));
+    if (colors_ && nk) {
+      s.colors->counts_device(s.d_results, s.searcher->kmer_ends_device(), 1, s.d_piece_counts);
+      (void)s.colors->pseudoalign_device(s.d_results, s.searcher->kmer_ends_device(), 1, cfg_.color,
+                                         s.d_offsets, s.d_ids, s.ids_capacity, s.d_piece_stats);
+      SBWT_HIP(hipMemcpyAsync(s.h_piece, s.d_piece_counts, nc_colors * 4, hipMemcpyDeviceToHost,
+                              s.stream));
+      SBWT_HIP(hipMemcpyAsync(s.h_piece + nc_colors, s.d_piece_stats, 12, hipMemcpyDeviceToHost,
+                              s.stream));
+    }
+    SBWT_HIP(hipStreamSynchronize(s.stream));
+    if (nk && cfg_.kmer_results) std::memcpy(r.kmer_results.data() + out, s.h_results, nk * 8);
+    out += nk;
+    if (colors_ && nk) {
+      for (u64 c = 0; c < nc_colors; ++c) counts[c] += s.h_piece[c];
+      F += s.h_piece[nc_colors];
+      NF += s.h_piece[nc_colors + 1];
+      INV += s.h_piece[nc_colors + 2];
+    }
+    if (b == end) break;
+    a = b - (k - 1);
+  }
+  if (colors_) {
+    const ColorParams &p = cfg_.color;
+    if (p.first_read_quirk) {  // the first read of its batch (u64 wrap, as on the device)
+      F -= 1;
+      NF -= 1;
+      INV -= 1;
+    }
+    const u64 seq = F + (p.include_not_found ? NF : 0) + (p.include_invalid ? INV : 0);
+    const u64 min_hits = static_cast<u64>(std::ceil(static_cast<double>(seq) * p.threshold));
+    if (min_hits > 0)
+      for (u64 c = 0; c < nc_colors; ++c)
+        if (counts[c] >= min_hits) r.color_ids.push_back(static_cast<u32>(c));
+    r.color_offsets[i + 1] = r.color_ids.size();
+  }
+  return out - kmer_out;
 }
 
 // The batch is padded to its bucket with empty reads (they have no k-mers and

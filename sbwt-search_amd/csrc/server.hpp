@@ -25,6 +25,8 @@ namespace sbwt_amd {
 
 struct ServerConfig {
   u32 slots = 4;                    // requests in flight on the device at once
+  // chars per batch; a longer read is searched in pieces of this size that
+  // overlap by k - 1 chars (at least k chars per slot)
   u64 max_chars_per_slot = 1 << 25;
   u64 max_reads_per_slot = 1 << 20;
   double memory_fraction = 0.9;     // of free device memory the slots may take
@@ -105,6 +107,9 @@ class Server {
     u64 *h_results = nullptr;  // pinned: results, color offsets and ids come back here
     u64 *h_offsets = nullptr;
     u32 *h_ids = nullptr;
+    u32 *d_piece_counts = nullptr;  // a long read's piece: color counts, {found, not found, invalid}
+    u32 *d_piece_stats = nullptr;
+    u32 *h_piece = nullptr;         // pinned: the piece's counts then its stats
     // by (chars, reads) bucket, search algorithm and work-item size
     std::map<std::tuple<u64, u64, int, u32>, hipGraphExec_t> graphs;
     std::thread worker;
@@ -115,6 +120,8 @@ class Server {
   // reads, the request's reads from read0); its results go to the request at
   // k-mer kmer_out (and its color lists). Returns the batch's k-mers.
   u64 run_batch(Slot &s, Request &r, u64 nc, u64 nr, u64 read0, u64 kmer_out);
+  // Read i, longer than a slot, in pieces; returns its k-mers.
+  u64 run_long_read(Slot &s, Request &r, u64 i, u64 kmer_out);
   u64 run_batch_graph(Slot &s, Request &r, u64 nc, u64 nr, u64 read0, u64 kmer_out, u64 cb,
                       u64 rb);
   hipGraphExec_t graph_for(Slot &s, u64 cb, u64 rb);

@@ -52,6 +52,38 @@ def test_served_results_equal_alone(served):
         assert g["submit_s"] <= g["start_s"] <= g["end_s"]
 
 
+@pytest.mark.parametrize("kmer_results", [True, False], ids=["kmer-results", "lists-only"])
+def test_reads_longer_than_a_slot_are_searched_in_pieces(served, kmer_results):
+    """Slots of 1000 chars: reads of up to 12 kbp are searched in pieces that
+    overlap by k - 1 chars, their color counts summed over the pieces; k-mer
+    results and color lists equal the oracle's for the whole reads."""
+    ci, idx, col = served
+    say(f"serving: reads longer than a slot, kmer_results={kmer_results}")
+    rng = np.random.default_rng(21)
+    reqs = []
+    for i in range(6):
+        n = int(rng.integers(3, 40))
+        lens = rng.integers(20, 300, n)
+        lens[rng.integers(0, n, 3)] = rng.integers(1001, 12_000, 3)
+        reqs.append(ci.reads(n, lens, seed=500 + i))
+    params = S.color_params(0.7)
+    srv = S.Server(idx, col, slots=2, max_chars_per_slot=1000, max_reads_per_slot=64,
+                   move_to_key_kmer=True, params=params, kmer_results=kmer_results)
+    ids = [srv.submit(c, e) for c, e in reqs]
+    n_long = 0
+    for (chars, ends), rid in zip(reqs, ids):
+        g = srv.wait(rid)
+        res = ci.index_results(chars, ends)
+        if kmer_results:
+            assert np.array_equal(g["kmers"], res)
+        ke = kmer_ends_of(ends)
+        assert g["color_lists"] == threshold_lists(ci.oracle_counts(res, ke), read_stats(res, ke),
+                                                   0.7)
+        n_long += int(np.count_nonzero(np.diff(np.concatenate([[0], ends.astype(np.int64)]))
+                                       > 1000))
+    assert n_long >= 6
+
+
 def test_malformed_request_does_not_disturb_others(served):
     ci, idx, col = served
     say("serving: malformed request in the mix")
