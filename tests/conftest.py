@@ -1,6 +1,12 @@
 import os
 import sys
 
+# torch first: its bundled HIP runtime (soname libamdhip64.so.7) is then the one
+# the libraries under test bind to. Loaded after them, torch brings in its own
+# copy beside the system one, and two HIP runtimes in one process can leave the
+# second without a visible GPU.
+import torch  # noqa: F401,E402
+
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 for p in (os.path.join(REPO, "tests"), os.path.join(REPO, "sbwt-search_amd", "bindings")):
     if p not in sys.path:
