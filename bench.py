@@ -373,6 +373,10 @@ def _leg_summary(leg, r):
                 "index_stage_roofline_frac": r["index_roofline"]["frac"],
                 "index_stage_frac_of_random_line_ceiling":
                     r["index_roofline"]["frac_of_random_line_ceiling"],
+                "color_kernel_roofline_frac": r["color_roofline"]["color_table"]["frac"],
+                "color_kernel_frac_of_random_line_ceiling":
+                    r["color_roofline"]["color_table"]["frac_of_random_line_ceiling"],
+                "color_kernel_ms": r["color_kernel_ms"],
                 "seconds_per_step": r["seconds"]}
     if leg == "batched":
         keep = ("reads", "batches", "plain_s", "batched_unfused_s", "batched_s", "speedup",
@@ -564,6 +568,14 @@ def mode_colors(args, configs, S, torch, W):
     kind = idx.build_color_table(cidx)
     table_ms = 1000 * (time.perf_counter() - t0)
     log(f"color table: {'masks' if kind == 1 else 'key ranks'} ({table_ms:.0f} ms)")
+    # launch shape of the streaming search with the key move and color output
+    s.set_color_output(True)
+    launch = None
+    if not args.no_autotune:
+        ch, bpc, tms = s.autotune(d_chars.data_ptr(), wl.n_chars, d_ends.data_ptr(), wl.n_reads,
+                                  d_res.data_ptr())
+        launch = {"chunk_kmers": ch, "blocks_per_cu": bpc}
+        log(f"autotune: {ch} k-mers per work item, {bpc} workgroups per CU ({tms:.2f} ms)")
     med_f, n_ids_f, lists_f, search_ms_f, color_ms_f = run(True)
     med, n_ids, lists, search_ms, color_ms = run(False)
     same = n_ids == n_ids_f and all(np.array_equal(a, b) for a, b in zip(lists, lists_f))
@@ -590,6 +602,32 @@ def mode_colors(args, configs, S, torch, W):
     log(f"index stage census: {S.Searcher.lines(stats) / n_kmers:.2f} lines per k-mer, "
         f"{achieved:.0f} GB/s over {search_ms:.2f} ms; random-line ceiling at "
         f"{probe['table_bytes'] >> 20} MiB: {probe['gbs']:.0f} GB/s")
+    # census of the color stage in both forms (untimed): key-set, key-mark and
+    # payload lines of the color kernel, against its hipEvent time
+    words = (int(cidx.num_colors) + 63) // 64
+    color_roofline = {}
+    for fused, kernel_ms in ((True, color_ms_f), (False, color_ms)):
+        s.set_color_output(fused)
+        cs.set_input(idx if fused else None)
+        s.search_reads_device(d_chars.data_ptr(), wl.n_chars, d_ends.data_ptr(), wl.n_reads,
+                              d_res.data_ptr())
+        s.sync()
+        cs.set_census(True)
+        cs.pseudoalign_device(d_res.data_ptr(), s.kmer_ends_device(), wl.n_reads, params,
+                              d_off.data_ptr(), d_ids.data_ptr(), cap)
+        cen = cs.last_census()
+        cs.set_census(False)
+        c_bytes = (LINE * (cen["set_lines"] + cen["mark_lines"] + cen["payload_lines"])
+                   + 8 * cen["kmers"] + wl.n_reads * (8 + 8 + 8 * words))
+        c_gbs = c_bytes / (kernel_ms / 1e3) / 1e9
+        color_roofline["color_table" if fused else "key_nodes"] = {
+            "bound": "hbm", "achieved": c_gbs, "peak": 8000.0, "unit": "GB/s",
+            "frac": c_gbs / 8000.0, "kernel_ms": kernel_ms, "algorithmic_bytes": c_bytes,
+            "frac_of_random_line_ceiling": c_gbs / probe["gbs"], "census": cen}
+        log(f"color stage census ({'color table' if fused else 'key nodes'}): "
+            f"{cen['set_lines'] / wl.n_reads:.2f} set + {cen['mark_lines'] / wl.n_reads:.2f} mark + "
+            f"{cen['payload_lines'] / wl.n_reads:.2f} payload lines per read, {c_gbs:.0f} GB/s "
+            f"over {kernel_ms:.2f} ms")
     out = {"metric": "reads/s (color)", "mode": "colors",
            "value": wl.n_reads / med_f["total"], "unit": "reads/s",
            "form": "color table (the index stage writes each found k-mer's key rank or mask)",
@@ -608,9 +646,10 @@ def mode_colors(args, configs, S, torch, W):
                               "random_line_ceiling_gbs": probe["gbs"],
                               "frac_of_random_line_ceiling": achieved / probe["gbs"],
                               "census": stats},
-           "color_kernel_ms": color_ms_f, "ids_per_step": n_ids,
+           "color_kernel_ms": color_ms_f, "color_roofline": color_roofline,
+           "ids_per_step": n_ids,
            "config": {"workload": wl.name + " " + wl.description, "sbwt_nodes": int(built.n),
-                      "num_colors": int(cidx.num_colors),
+                      "launch": launch, "num_colors": int(cidx.num_colors),
                       "color_sets": int(cidx.num_color_sets), "threshold": 0.7},
            "seconds": med_f}
     print(json.dumps(out), flush=True)

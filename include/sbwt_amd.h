@@ -438,6 +438,17 @@ int sbwt_color_searcher_set_input(sbwt_color_searcher *searcher, const sbwt_inde
 /* Device time (ms) of the last call's color kernel. */
 int sbwt_color_searcher_last_time(sbwt_color_searcher *searcher, float *ms);
 
+/* Census of the fused color stage (measurement; results unchanged; no
+ * counterpart in the reference). With enable set, each
+ * sbwt_color_searcher_pseudoalign* call counts its exact algorithmic work:
+ * census[0] key-set entries read (one 128-byte line each: a run of k-mers
+ * sharing a key reads one), census[1] key-mark entries read (key-node input:
+ * one line each), census[2] 128-byte lines spanned by the payloads of the
+ * decoded sets, census[3] k-mer results read (8 bytes each). last_census
+ * synchronizes the searcher's stream. */
+int sbwt_color_searcher_set_census(sbwt_color_searcher *searcher, int enable);
+int sbwt_color_searcher_last_census(sbwt_color_searcher *searcher, uint64_t census[4]);
+
 /* ---- colors stage from index-result files --------------------------------- */
 
 /* Bodies of the index stage's output files (after the two sized header

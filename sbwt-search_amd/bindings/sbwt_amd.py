@@ -702,6 +702,15 @@ class ColorSearcher:
         _check(lib().sbwt_color_searcher_last_time(self.h, C.byref(ms)))
         return ms.value
 
+    def set_census(self, on):
+        _check(lib().sbwt_color_searcher_set_census(self.h, 1 if on else 0))
+
+    def last_census(self):
+        """{set_lines, mark_lines, payload_lines, kmers} of the last call."""
+        out = (C.c_uint64 * 4)()
+        _check(lib().sbwt_color_searcher_last_census(self.h, out))
+        return dict(zip(("set_lines", "mark_lines", "payload_lines", "kmers"), list(out)))
+
 
 class Server:
     """Serving loop over resident indexes (sbwt_server_*)."""

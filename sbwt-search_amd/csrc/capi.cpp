@@ -598,6 +598,20 @@ int sbwt_color_searcher_last_time(sbwt_color_searcher *searcher, float *ms) {
   });
 }
 
+int sbwt_color_searcher_set_census(sbwt_color_searcher *searcher, int enable) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    searcher->impl->set_census(enable != 0);
+  });
+}
+
+int sbwt_color_searcher_last_census(sbwt_color_searcher *searcher, uint64_t census[4]) {
+  return guarded([&] {
+    require(searcher && census, "null argument");
+    searcher->impl->last_census(census);
+  });
+}
+
 int sbwt_index_file_parser_create(uint64_t max_bytes_per_batch, uint64_t max_reads_per_batch,
                                   void *hip_stream, sbwt_index_file_parser **out) {
   return guarded([&] {

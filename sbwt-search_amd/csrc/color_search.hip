@@ -47,21 +47,51 @@ struct SetRef {
 };
 
 
-// Lanes whose index equals the previous lane's share one decode (consecutive
-// k-mers of a read usually walk to the same key k-mer): the first lane of
-// each run (the leader) decodes the set, with `run` = the run's length
-// within the window of 64 slots.
-__device__ __forceinline__ bool wave_leader(u64 v, bool found, u32 lane, u32 &run) {
-  const u64 prev = __shfl_up(v, 1, kWave);
-  const bool same = found && lane > 0 && prev == v;
-  const u64 same_mask = __ballot(same);
-  const bool leader = found && !same;
-  run = 0;
-  if (leader) {
-    const u64 rest = lane == kWave - 1 ? 0 : (same_mask >> (lane + 1));
-    run = 1 + (rest == ~0ULL ? 64u : static_cast<u32>(__builtin_ctzll(~rest)));
-  }
-  return leader;
+// Per-wave census of the color stage's algorithmic units, counted only when
+// the kernel gets a census array: key-set entries read (one 128-byte line
+// each), key-mark entries read (key-node input; one line each) and the
+// 128-byte lines spanned by the decoded sets' payloads.
+struct Census {
+  bool on = false;
+  u32 set_lines = 0, mark_lines = 0, payload_lines = 0;
+};
+
+constexpr u32 kCensusWords = 4;  // set lines, mark lines, payload lines, k-mers
+
+// Windows of 64 result slots decoded together: their result loads, the set
+// lookups of all their runs and the payload loads of those sets are each in
+// flight at once, so a read of up to 256 k-mers costs about three memory
+// round trips rather than three per window.
+constexpr u32 kGroup = 4;
+// Consecutive sparse ids per lane in the payload walk (loaded together
+// before their counter adds), and dense 64-bit words per lane in flight.
+constexpr u32 kPerLane = 8;
+constexpr u32 kDenseUnroll = 2;
+// The wave's LDS table: 64 entries of 16 bytes.
+constexpr u32 kTabWords = kWave * 4;
+constexpr u32 kNoColor = 0xFFFFFFFFu;
+
+__device__ __forceinline__ u32 wave_sum(u32 x) {
+  for (u32 o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
+  return x;
+}
+
+__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
+
+__device__ __forceinline__ u32 wave_or(u32 x) {
+  for (u32 o = kWave / 2; o > 0; o >>= 1) x |= __shfl_xor(x, o, kWave);
+  return x;
+}
+
+__device__ __forceinline__ u64 read_lane64(u64 x, u32 l) {
+  const u32 lo = __builtin_amdgcn_readlane(static_cast<u32>(x), l);
+  const u32 hi = __builtin_amdgcn_readlane(static_cast<u32>(x >> 32), l);
+  return static_cast<u64>(lo) | (static_cast<u64>(hi) << 32);
+}
+
+// 128-byte lines spanned by bits [b0, b1).
+__device__ __forceinline__ u32 lines_spanned(u64 b0, u64 b1) {
+  return b1 > b0 ? static_cast<u32>(((b1 - 1) >> 10) - (b0 >> 10) + 1) : 0u;
 }
 
 // Counters belong to one wave (LDS, or its own slice of a global array), so
@@ -71,9 +101,9 @@ __device__ __forceinline__ void cnt_add(u32 *p, u32 v) {
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// Units of the window's sets spread over the lanes: unit t of `total` (the
-// sum of `units` over the lanes) belongs to the first lane whose inclusive
-// prefix `incl` exceeds it; returns that lane and t's index in its set.
+// Units of the sets spread over the lanes: unit t of `total` (the sum of
+// `units` over the lanes) belongs to the first lane whose inclusive prefix
+// `incl` exceeds it; returns that lane and t's index in its set.
 __device__ __forceinline__ u32 unit_owner(u32 incl, u32 units, u32 t, u32 &index) {
   u32 lo = 0, hi = kWave - 1;
 #pragma unroll
@@ -86,78 +116,149 @@ __device__ __forceinline__ u32 unit_owner(u32 incl, u32 units, u32 t, u32 &index
   return lo;
 }
 
-__device__ __forceinline__ u32 wave_prefix(u32 v, u32 lane) {
-  u32 incl = v;
-#pragma unroll
-  for (u32 d = 1; d < kWave; d <<= 1) {
-    const u32 y = __shfl_up(incl, d, kWave);
-    if (lane >= d) incl += y;
-  }
-  return incl;
+// Inclusive prefix sum over the wave's lanes by DPP: shifts within each row
+// of 16 lanes (row_shr 1, 2, 4, 8), then the row totals carried by the
+// row_bcast15 / row_bcast31 broadcasts into the rows above them.
+__device__ __forceinline__ u32 wave_prefix(u32 v) {
+  u32 x = v;
+  x += static_cast<u32>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x111, 0xF, 0xF, true));
+  x += static_cast<u32>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x112, 0xF, 0xF, true));
+  x += static_cast<u32>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x114, 0xF, 0xF, true));
+  x += static_cast<u32>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x118, 0xF, 0xF, true));
+  x += static_cast<u32>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x142, 0xA, 0xF, false));
+  x += static_cast<u32>(__builtin_amdgcn_update_dpp(0, static_cast<int>(x), 0x143, 0xC, 0xF, false));
+  return x;
 }
 
-// The payloads of one window's decoded sets, walked by the whole wave, adding
-// each set's run length to the counter of every color it holds (the per-color
-// ballot/popcount of ColorSearcher.cuh:175-205 summed over the read by
-// ColorPostProcessor.cuh:21-43). All payload loads of a pass are in flight at
-// once.
-//  - dense sets: their 64-bit bitmap words are loaded one per lane; then the
-//    wave steps through the words, each broadcast from its lane (readlane),
-//    and lane i adds the run to color 64 u + i when its bit is set: no bit
-//    loop and no two lanes on one counter;
-//  - sparse sets: one color id per lane, one counter add each.
+// Records that color c's counter word (64 colors) was written; only the
+// touched words are scanned at the threshold (LDS counters: at most 16).
+__device__ __forceinline__ void touch(u32 &touched, u32 c) { touched |= 1u << ((c >> 6) & 31); }
+
+// The payloads of up to 64 decoded sets (one per lane), walked by the whole
+// wave, adding each set's run length to the counter of every color it holds
+// (the per-color ballot/popcount of ColorSearcher.cuh:175-205 summed over
+// the read by ColorPostProcessor.cuh:21-43). `tab` is the wave's LDS table
+// of 64 entries.
+//  - dense sets: their 64-bit bitmap words are loaded one per lane, two per
+//    lane in flight; then the wave steps through the words, each broadcast
+//    from its lane (readlane), and lane i adds the run to color 64 u + i when
+//    its bit is set: no bit loop and no two lanes on one counter;
+//  - sparse sets: the ids of all sets, concatenated, are cut into ranges of
+//    kPerLane consecutive ids, one range per lane; a lane finds the set of its
+//    first id once (a search over the lanes' unit counts) and steps on through
+//    the table of set ranges. All loads of a range are issued before any id
+//    is used (one memory round trip per range), then the ids are extracted
+//    and added.
 __device__ __forceinline__ void wave_payload(const ColorView &cv, const SetRef &set, u32 units,
-                                             u32 run, u32 lane, u32 *cnt) {
+                                             u32 run, u32 lane, u32 *cnt, uint4 *tab,
+                                             u32 &touched) {
   const u32 nc = cv.num_colors;
   const u32 d_units = set.dense ? units : 0u, s_units = set.dense ? 0u : units;
-  const u32 d_incl = wave_prefix(d_units, lane);
-  const u32 d_total = __shfl(d_incl, kWave - 1, kWave);
-  for (u32 t0 = 0; t0 < d_total; t0 += kWave) {
-    const u32 t = t0 + lane;
-    u32 u = 0;
-    const u32 owner = unit_owner(d_incl, d_units, t, u);
-    const u32 o_run = __shfl(run, static_cast<int>(owner), kWave);
-    const u32 o_lo = __shfl(static_cast<u32>(set.start), static_cast<int>(owner), kWave);
-    const u32 o_hi = __shfl(static_cast<u32>(set.start >> 32), static_cast<int>(owner), kWave);
-    const u32 o_len = __shfl(set.len, static_cast<int>(owner), kWave);
-    u64 x = 0;
-    if (t < d_total) {
-      // 64 bits of the bitmap: colors 64u .. 64u + 63
-      const u64 bit = (static_cast<u64>(o_lo) | (static_cast<u64>(o_hi) << 32)) + 64 * static_cast<u64>(u);
-      const u64 w = bit >> 6;
-      const u32 off = static_cast<u32>(bit & 63);
-      x = cv.dense_arrays[w] >> off;
-      if (off) x |= cv.dense_arrays[w + 1] << (64 - off);
-      const u32 take = min(64u, o_len - 64 * u);
-      if (take < 64) x &= (1ULL << take) - 1;
-    }
-    const u32 n_here = min(kWave, d_total - t0);
-    for (u32 q = 0; q < n_here; ++q) {
-      const u32 xl = __builtin_amdgcn_readlane(static_cast<u32>(x), q);
-      const u32 xh = __builtin_amdgcn_readlane(static_cast<u32>(x >> 32), q);
-      if ((xl | xh) == 0) continue;
-      const u32 rq = __builtin_amdgcn_readlane(o_run, q);
-      const u32 uq = __builtin_amdgcn_readlane(u, q);
-      const u32 bitv = lane < 32 ? (xl >> lane) : (xh >> (lane - 32));
-      const u32 c = 64 * uq + lane;
-      if ((bitv & 1u) && c < nc) cnt_add(cnt + c, rq);
-    }
-  }
-  const u32 s_incl = wave_prefix(s_units, lane);
-  const u32 s_total = __shfl(s_incl, kWave - 1, kWave);
-  for (u32 t0 = 0; t0 < s_total; t0 += kWave) {
-    const u32 t = t0 + lane;
-    u32 u = 0;
-    const u32 owner = unit_owner(s_incl, s_units, t, u);
-    const u32 o_run = __shfl(run, static_cast<int>(owner), kWave);
-    const u32 o_lo = __shfl(static_cast<u32>(set.start), static_cast<int>(owner), kWave);
-    const u32 o_hi = __shfl(static_cast<u32>(set.start >> 32), static_cast<int>(owner), kWave);
-    if (t < s_total) {
-      const u64 start = static_cast<u64>(o_lo) | (static_cast<u64>(o_hi) << 32);
-      const u64 c = intv_get(cv.sparse_arrays, cv.sparse_width, start + u);
-      if (c < nc) cnt_add(cnt + c, o_run);
+  if (__ballot(d_units != 0)) {
+    const u32 d_incl = wave_prefix(d_units);
+    const u32 d_total = __builtin_amdgcn_readlane(d_incl, kWave - 1);
+    for (u32 t0 = 0; t0 < d_total; t0 += kDenseUnroll * kWave) {
+      // loads first (both words of every unit, all in flight), then the bits
+      u64 x0[kDenseUnroll], x1[kDenseUnroll];
+      u32 rq[kDenseUnroll], uq[kDenseUnroll], off[kDenseUnroll], take[kDenseUnroll];
+#pragma unroll
+      for (u32 q = 0; q < kDenseUnroll; ++q) {
+        const u32 t = min(t0 + q * kWave + lane, d_total - 1);
+        u32 u = 0;
+        const u32 owner = unit_owner(d_incl, d_units, t, u);
+        rq[q] = __shfl(run, static_cast<int>(owner), kWave);
+        const u32 o_lo = __shfl(static_cast<u32>(set.start), static_cast<int>(owner), kWave);
+        const u32 o_hi = __shfl(static_cast<u32>(set.start >> 32), static_cast<int>(owner), kWave);
+        const u32 o_len = __shfl(set.len, static_cast<int>(owner), kWave);
+        uq[q] = u;
+        // 64 bits of the bitmap: colors 64u .. 64u + 63
+        const u64 bit = (static_cast<u64>(o_lo) | (static_cast<u64>(o_hi) << 32)) + 64 * static_cast<u64>(u);
+        off[q] = static_cast<u32>(bit & 63);
+        take[q] = t0 + q * kWave + lane < d_total ? min(64u, o_len - 64 * u) : 0u;
+        x0[q] = cv.dense_arrays[bit >> 6];
+        x1[q] = cv.dense_arrays[(bit >> 6) + 1];
+      }
+#pragma unroll
+      for (u32 q = 0; q < kDenseUnroll; ++q) {
+        if (t0 + q * kWave >= d_total) continue;  // wave-uniform
+        u64 x = off[q] ? ((x0[q] >> off[q]) | (x1[q] << (64 - off[q]))) : x0[q];
+        if (take[q] < 64) x &= (1ULL << take[q]) - 1;
+        const u32 n_here = min(kWave, d_total - t0 - q * kWave);
+        for (u32 l = 0; l < n_here; ++l) {
+          const u32 xl = __builtin_amdgcn_readlane(static_cast<u32>(x), l);
+          const u32 xh = __builtin_amdgcn_readlane(static_cast<u32>(x >> 32), l);
+          if ((xl | xh) == 0) continue;
+          const u32 r = __builtin_amdgcn_readlane(rq[q], l);
+          const u32 u = __builtin_amdgcn_readlane(uq[q], l);
+          const u32 bitv = lane < 32 ? (xl >> lane) : (xh >> (lane - 32));
+          const u32 c = 64 * u + lane;
+          if ((bitv & 1u) && c < nc) {
+            cnt_add(cnt + c, r);
+            touch(touched, c);
+          }
+        }
+      }
     }
   }
+  if (__ballot(s_units != 0) == 0) return;
+  const u32 s_incl = wave_prefix(s_units);
+  const u32 s_total = __builtin_amdgcn_readlane(s_incl, kWave - 1);
+  // the set ranges: {end of the set's ids in the concatenation, start lo,
+  // start hi, run}, lane order
+  tab[lane] = make_uint4(s_incl, static_cast<u32>(set.start), static_cast<u32>(set.start >> 32), run);
+  wave_lds_fence();
+  const u32 width = cv.sparse_width;
+  for (u32 b0 = 0; b0 < s_total; b0 += kPerLane * kWave) {
+    const u32 t = b0 + lane * kPerLane;  // this lane's first id
+    const u32 t_first = min(t, s_total - 1);
+    u32 u = 0;
+    u32 o = unit_owner(s_incl, s_units, t_first, u);
+    uint4 e = tab[o];
+    u32 excl = t_first - u;  // the set's first id in the concatenation
+    if (width <= 32) {
+      // phase 1: each id's set (stepping on through the table), its two
+      // dwords loaded (all in flight) and {bit offset, run, valid} kept
+      uint2 w[kPerLane];
+      u32 meta[kPerLane];
+#pragma unroll
+      for (u32 k = 0; k < kPerLane; ++k) {
+        const u32 tk = min(t + k, s_total - 1);
+        while (tk >= e.x && o < kWave - 1) {  // on to the next set with ids (the ends rise)
+          excl = e.x;
+          e = tab[++o];
+        }
+        const u64 bit = ((static_cast<u64>(e.y) | (static_cast<u64>(e.z) << 32)) + (tk - excl)) * width;
+        w[k] = *reinterpret_cast<const uint2 *>(reinterpret_cast<const u32 *>(cv.sparse_arrays) + (bit >> 5));
+        meta[k] = (static_cast<u32>(bit) & 31u) | (min(e.w, 1023u) << 5) | (t + k < s_total ? 1u << 15 : 0u);
+      }
+      // phase 2: the ids and their counter adds
+      const u32 wmask = width == 32 ? ~0u : ((1u << width) - 1);
+#pragma unroll
+      for (u32 k = 0; k < kPerLane; ++k) {
+        const u32 id = __builtin_amdgcn_alignbit(w[k].y, w[k].x, meta[k] & 31u) & wmask;
+        if ((meta[k] >> 15) && id < nc) {
+          cnt_add(cnt + id, (meta[k] >> 5) & 1023u);
+          touch(touched, id);
+        }
+      }
+    } else {
+      for (u32 k = 0; k < kPerLane; ++k) {
+        const u32 tk = t + k;
+        if (tk >= s_total) break;
+        while (tk >= e.x && o < kWave - 1) {
+          excl = e.x;
+          e = tab[++o];
+        }
+        const u64 id = intv_get(cv.sparse_arrays, width,
+                                (static_cast<u64>(e.y) | (static_cast<u64>(e.z) << 32)) + (tk - excl));
+        if (id < nc) {
+          cnt_add(cnt + id, e.w);
+          touch(touched, static_cast<u32>(id));
+        }
+      }
+    }
+  }
+  wave_lds_fence();  // the table is written again by the next chunk
 }
 
 // Inputs of the color stage: each k-mer's key node (the reference's index
@@ -165,63 +266,128 @@ __device__ __forceinline__ void wave_payload(const ColorView &cv, const SetRef &
 // counts then come from ballots, with no lookups) or the key's rank.
 constexpr int kInKeys = 0, kInMasks = 1, kInRanks = 2;
 
-// One window of 64 result slots starting at slot `base` of a read ending at
-// `ke`: counts found / not-found / invalid slots and adds every found
-// k-mer's colors to the wave's counters. Key nodes and ranks: a run's leader
-// decodes its set (the key-mark entry then the key set, or the key set
-// alone) and the wave walks the payloads. Masks: color c gains the number of
-// lanes whose mask holds it, lane c adding it.
+// Up to kGroup windows of 64 result slots from slot `base` of a read ending
+// at `ke`: counts found / not-found / invalid slots and adds every found
+// k-mer's colors to the wave's counters.
+// Key nodes and ranks: consecutive equal found results (consecutive k-mers
+// of a read usually reach the same key k-mer) form runs, also across the
+// windows; the first slot of each run (its leader) writes {result, run} to
+// the wave's LDS table at its rank among the leaders, 64 leaders at a time,
+// lane j takes entry j and decodes its set (the key-mark entry then the key
+// set, or the key set alone) and the wave walks the payloads, adding the
+// run's length. Masks: color c gains the number of slots whose mask holds
+// it, lane c adding it.
 template <int IN>
-__device__ __forceinline__ void window_add(const ColorView &cv, const u64 *__restrict__ results,
-                                           u64 base, u64 ke, u32 lane, u32 *cnt, u32 &f, u32 &nf,
-                                           u32 &inv) {
-  const u64 i = base + lane;
-  const bool in = i < ke;
-  const u64 v = in ? results[i] : kInvalid;
-  const bool found = in && (IN == kInKeys ? v < cv.num_nodes : (v >> 63) == 0);
-  f += found;
-  inv += in && v == kInvalid;
-  nf += in && !found && v != kInvalid;
+__device__ __forceinline__ void group_add(const ColorView &cv, const u64 *__restrict__ results,
+                                          u64 base, u64 ke, u32 lane, u32 *cnt, uint4 *tab,
+                                          u32 &f, u32 &nf, u32 &inv, u32 &touched, Census &cs) {
+  u64 v[kGroup];
+  bool fd[kGroup];
+#pragma unroll
+  for (u32 g = 0; g < kGroup; ++g) {
+    const u64 i = base + g * kWave + lane;
+    v[g] = i < ke ? results[i] : kInvalid;
+  }
+#pragma unroll
+  for (u32 g = 0; g < kGroup; ++g) {
+    const bool in = base + g * kWave + lane < ke;
+    fd[g] = in && (IN == kInKeys ? v[g] < cv.num_nodes : (v[g] >> 63) == 0);
+    f += static_cast<u32>(__popcll(__ballot(fd[g])));
+    inv += static_cast<u32>(__popcll(__ballot(in && v[g] == kInvalid)));
+    nf += static_cast<u32>(__popcll(__ballot(in && !fd[g] && v[g] != kInvalid)));
+  }
   if constexpr (IN == kInMasks) {
-    const u64 m = found ? v : 0;
-    if (__ballot(m != 0) == 0) return;
     u32 mine = 0;
-    for (u32 c = 0; c < cv.num_colors; ++c) {
-      const u32 n = static_cast<u32>(__popcll(__ballot((m >> c) & 1)));
-      if (lane == c) mine = n;
+#pragma unroll
+    for (u32 g = 0; g < kGroup; ++g) {
+      const u64 m = fd[g] ? v[g] : 0;
+      if (__ballot(m != 0) == 0) continue;
+      for (u32 c = 0; c < cv.num_colors; ++c) {
+        const u32 n = static_cast<u32>(__popcll(__ballot((m >> c) & 1)));
+        if (lane == c) mine += n;
+      }
     }
-    if (mine) cnt_add(cnt + lane, mine);
+    if (mine) {
+      cnt_add(cnt + lane, mine);
+      touch(touched, lane);
+    }
     return;
   }
-  u32 run = 0;
-  const bool lead = wave_leader(v, found, lane, run);
-  SetRef set;
-  u32 units = 0;
-  if (lead) {
-    const u64 rank = IN == kInKeys ? rank_marks(cv.key_marks, v) : v;
-    const uint4 e = cv.key_sets[rank];
-    set.start = static_cast<u64>(e.x) | (static_cast<u64>(e.y) << 32);
-    set.len = e.z;
-    set.dense = (e.w & 1u) != 0;
-    units = set.dense ? (set.len + 63) / 64 : set.len;
+  // same_m[g]: slots equal to the slot before them (lane 0 of a window
+  // follows lane 63 of the window before); lead_m[g]: the runs' leaders
+  u64 same_m[kGroup], lead_m[kGroup];
+#pragma unroll
+  for (u32 g = 0; g < kGroup; ++g) {
+    const u64 last = g ? read_lane64(v[g - 1], kWave - 1) : 0;
+    u64 prev = __shfl_up(v[g], 1, kWave);
+    if (lane == 0) prev = g ? last : ~v[g];
+    const bool same = fd[g] && prev == v[g];
+    same_m[g] = __ballot(same);
+    lead_m[g] = __ballot(fd[g] && !same);
   }
-  wave_payload(cv, set, units, run, lane, cnt);
+  // cont[g]: slots from window g's start that continue the run before it;
+  // cum[g]: leaders before window g
+  u32 cont[kGroup + 1], cum[kGroup + 1];
+  cont[kGroup] = 0;
+#pragma unroll
+  for (int g = kGroup - 1; g >= 0; --g) {
+    const u32 ones = same_m[g] == ~0ULL ? 64u : static_cast<u32>(__builtin_ctzll(~same_m[g]));
+    cont[g] = ones + (ones == 64 ? cont[g + 1] : 0u);
+  }
+  cum[0] = 0;
+#pragma unroll
+  for (u32 g = 0; g < kGroup; ++g) cum[g + 1] = cum[g] + static_cast<u32>(__popcll(lead_m[g]));
+  const u32 n_lead = cum[kGroup];
+  for (u32 j0 = 0; j0 < n_lead; j0 += kWave) {
+#pragma unroll
+    for (u32 g = 0; g < kGroup; ++g) {
+      if (cum[g + 1] <= j0 || cum[g] >= j0 + kWave) continue;  // no leader of this chunk here
+      const u64 lm = lead_m[g];
+      const u32 below = __builtin_amdgcn_mbcnt_hi(static_cast<u32>(lm >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo(static_cast<u32>(lm), 0u));
+      const u32 rank = cum[g] + below;
+      if (((lm >> lane) & 1) && rank >= j0 && rank < j0 + kWave) {
+        // the run: the leader, the equal slots after it in its window and,
+        // when they reach the window's end, the slots continuing it later
+        const u64 rest = lane == kWave - 1 ? 0 : (same_m[g] >> (lane + 1));
+        const u32 ones = static_cast<u32>(__builtin_ctzll(~rest));
+        const u32 run = 1 + ones + (ones == kWave - 1 - lane ? cont[g + 1] : 0u);
+        tab[rank - j0] = make_uint4(static_cast<u32>(v[g]), static_cast<u32>(v[g] >> 32), run, 0u);
+      }
+    }
+    wave_lds_fence();
+    const bool has = j0 + lane < n_lead;
+    const uint4 le = has ? tab[lane] : make_uint4(0u, 0u, 0u, 0u);
+    wave_lds_fence();
+    const u64 x = static_cast<u64>(le.x) | (static_cast<u64>(le.y) << 32);
+    SetRef set;
+    u32 units = 0;
+    if (has) {
+      const u64 rank = IN == kInKeys ? rank_marks(cv.key_marks, x) : x;
+      const uint4 e = cv.key_sets[rank];
+      set.start = static_cast<u64>(e.x) | (static_cast<u64>(e.y) << 32);
+      set.len = e.z;
+      set.dense = (e.w & 1u) != 0;
+      units = set.dense ? (set.len + 63) / 64 : set.len;
+      if (cs.on) {
+        ++cs.set_lines;
+        cs.mark_lines += IN == kInKeys ? 1u : 0u;
+        const u64 w = set.dense ? 1 : cv.sparse_width;
+        cs.payload_lines += lines_spanned(set.start * w, (set.start + set.len) * w);
+      }
+    }
+    wave_payload(cv, set, units, le.z, lane, cnt, tab, touched);
+  }
 }
 
 // All k-mer results [ks, ke) of one read into the wave's counters.
 template <int IN>
 __device__ __forceinline__ void read_add(const ColorView &cv, const u64 *__restrict__ results, u64 ks,
-                                         u64 ke, u32 lane, u32 *cnt, u32 &f, u32 &nf, u32 &inv) {
-  for (u64 base = ks; base < ke; base += kWave)
-    window_add<IN>(cv, results, base, ke, lane, cnt, f, nf, inv);
+                                         u64 ke, u32 lane, u32 *cnt, uint4 *tab, u32 &f, u32 &nf,
+                                         u32 &inv, u32 &touched, Census &cs) {
+  for (u64 base = ks; base < ke; base += kGroup * kWave)
+    group_add<IN>(cv, results, base, ke, lane, cnt, tab, f, nf, inv, touched, cs);
 }
-
-__device__ __forceinline__ u32 wave_sum(u32 x) {
-  for (u32 o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
-  return x;
-}
-
-__device__ __forceinline__ void wave_lds_fence() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront"); }
 
 // ---------------------------------------------------------------------------
 // Per-wave color counters: nc u32 in LDS when nc <= kLdsColors (16 KiB per
@@ -250,13 +416,34 @@ __device__ __forceinline__ void cnt_fence() {
   }
 }
 
+// The workgroup's LDS: each wave's table (kTabWords u32), then with LDS
+// counters each wave's nc counters.
+__device__ __forceinline__ uint4 *wave_table(u32 *lds) {
+  return reinterpret_cast<uint4 *>(lds) + (threadIdx.x / kWave) * kWave;
+}
+
 template <bool LDS>
 __device__ __forceinline__ u32 *wave_counters(u32 *lds, u32 *global_cnt, u32 nc) {
   const u32 wave = threadIdx.x / kWave;
   if constexpr (LDS) {
-    return lds + static_cast<u64>(wave) * nc;
+    return lds + (blockDim.x / kWave) * kTabWords + static_cast<u64>(wave) * nc;
   } else {
     return global_cnt + (static_cast<u64>(blockIdx.x) * (blockDim.x / kWave) + wave) * nc;
+  }
+}
+
+// The wave's census into census[0..3] (set lines, mark lines, payload
+// lines, k-mers), one atomic each.
+__device__ __forceinline__ void census_flush(const Census &cs, u64 kmers, u32 lane,
+                                             u64 *census) {
+  if (!cs.on) return;
+  const u32 sl = wave_sum(cs.set_lines), ml = wave_sum(cs.mark_lines);
+  const u32 pl = wave_sum(cs.payload_lines);
+  if (lane == 0) {
+    atomicAdd(reinterpret_cast<unsigned long long *>(census), static_cast<unsigned long long>(sl));
+    atomicAdd(reinterpret_cast<unsigned long long *>(census + 1), static_cast<unsigned long long>(ml));
+    atomicAdd(reinterpret_cast<unsigned long long *>(census + 2), static_cast<unsigned long long>(pl));
+    atomicAdd(reinterpret_cast<unsigned long long *>(census + 3), static_cast<unsigned long long>(kmers));
   }
 }
 
@@ -281,9 +468,13 @@ struct ReadOut {
 // Read r's colors with count >= ceil(seq_size * tau)
 // (ContinuousColorResultsPrinter.hpp:235-263) into its bitmap row, from the
 // counters cnt (zeroed on the way); f/nf/inv are the read's whole counts.
+// With LDS counters only the words in `touched` (this lane's record of the
+// counter words it wrote; at most 16 words) are scanned: every other counter
+// is zero, so no color there reaches a positive threshold.
 template <bool LDS>
 __device__ __forceinline__ void finish_read(u32 *cnt, u32 nc, u64 r, u32 f, u32 nf, u32 inv,
-                                            const KernelParams &p, const ReadOut &o, u32 lane) {
+                                            const KernelParams &p, const ReadOut &o, u32 lane,
+                                            u32 touched) {
   u64 F = f, NF = nf, INV = inv;
   if (p.first_read_quirk && r == 0) {
     F -= 1;
@@ -294,14 +485,28 @@ __device__ __forceinline__ void finish_read(u32 *cnt, u32 nc, u64 r, u32 f, u32 
   const u64 min_hits = static_cast<u64>(ceil(static_cast<double>(seq) * p.threshold));
   const u32 words = (nc + 63) / 64;
   u64 total = 0;
-  for (u32 w = 0; w < words; ++w) {
-    const u32 c = w * 64 + lane;
-    const u32 v = c < nc ? cnt_get<LDS>(cnt + c) : 0u;
-    const bool hit = min_hits > 0 && c < nc && v >= min_hits;
-    const u64 m = __ballot(hit);
-    if (lane == 0) o.bitmap[r * words + w] = m;
-    total += __popcll(m);
-    if (c < nc && v) cnt[c] = 0;
+  if constexpr (LDS) {
+    u64 mine = 0;  // lane w: bitmap word w
+    for (u32 tw = wave_or(touched); tw; tw &= tw - 1) {
+      const u32 w = static_cast<u32>(__builtin_ctz(tw));
+      const u32 c = w * 64 + lane;
+      const u32 v = c < nc ? cnt[c] : 0u;
+      const u64 m = __ballot(min_hits > 0 && c < nc && v >= min_hits);
+      if (lane == w) mine = m;
+      total += __popcll(m);
+      if (v) cnt[c] = 0;
+    }
+    if (lane < words) o.bitmap[r * words + lane] = mine;
+  } else {
+    for (u32 w = 0; w < words; ++w) {
+      const u32 c = w * 64 + lane;
+      const u32 v = c < nc ? cnt_get<LDS>(cnt + c) : 0u;
+      const bool hit = min_hits > 0 && c < nc && v >= min_hits;
+      const u64 m = __ballot(hit);
+      if (lane == 0) o.bitmap[r * words + w] = m;
+      total += __popcll(m);
+      if (c < nc && v) cnt[c] = 0;
+    }
   }
   if (lane == 0) {
     o.nout[r] = total;
@@ -324,8 +529,10 @@ __global__ __launch_bounds__(256) void k_color_reads(ColorView cv, const u64 *__
                                                       const u64 *__restrict__ kmer_ends, u64 n_reads,
                                                       KernelParams p, ReadOut o,
                                                       u32 *__restrict__ counts_out,
-                                                      u32 *__restrict__ global_cnt) {
-  extern __shared__ u32 lds[];
+                                                      u32 *__restrict__ global_cnt,
+                                                      u64 *__restrict__ census) {
+  extern __shared__ __align__(16) u32 lds[];
+  uint4 *tab = wave_table(lds);
   const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   const u32 waves = blockDim.x / kWave;
   const u32 nc = cv.num_colors;
@@ -334,14 +541,15 @@ __global__ __launch_bounds__(256) void k_color_reads(ColorView cv, const u64 *__
     for (u32 c = lane; c < nc; c += kWave) cnt[c] = 0;
   }
   cnt_fence<LDS>();
+  Census cs;
+  cs.on = census != nullptr;
+  u64 kmers = 0;
   for (u64 r = static_cast<u64>(blockIdx.x) * waves + wave; r < n_reads;
        r += static_cast<u64>(gridDim.x) * waves) {
     const u64 ks = r ? kmer_ends[r - 1] : 0, ke = kmer_ends[r];
-    u32 f = 0, nf = 0, inv = 0;
-    read_add<IN>(cv, results, ks, ke, lane, cnt, f, nf, inv);
-    f = wave_sum(f);
-    nf = wave_sum(nf);
-    inv = wave_sum(inv);
+    u32 f = 0, nf = 0, inv = 0, touched = 0;
+    read_add<IN>(cv, results, ks, ke, lane, cnt, tab, f, nf, inv, touched, cs);
+    kmers += ke - ks;
     cnt_fence<LDS>();
     if constexpr (COUNTS) {
       for (u32 c = lane; c < nc; c += kWave) {
@@ -350,10 +558,11 @@ __global__ __launch_bounds__(256) void k_color_reads(ColorView cv, const u64 *__
         if (v) cnt[c] = 0;
       }
     } else {
-      finish_read<LDS>(cnt, nc, r, f, nf, inv, p, o, lane);
+      finish_read<LDS>(cnt, nc, r, f, nf, inv, p, o, lane, touched);
     }
     cnt_fence<LDS>();
   }
+  census_flush(cs, kmers, lane, census);
 }
 
 // Work items of the batched form: read r is split into
@@ -386,8 +595,9 @@ __global__ __launch_bounds__(256) void k_color_items(
     ColorView cv, const u64 *__restrict__ results, const u64 *__restrict__ kmer_ends, u64 n_reads,
     const u64 *__restrict__ item_ends, u32 item_kmers, const u64 *__restrict__ long_incl,
     KernelParams p, ReadOut o, u32 *__restrict__ global_cnt, u32 *__restrict__ long_cnt,
-    u32 *__restrict__ long_meta) {
-  extern __shared__ u32 lds[];
+    u32 *__restrict__ long_meta, u64 *__restrict__ census) {
+  extern __shared__ __align__(16) u32 lds[];
+  uint4 *tab = wave_table(lds);
   const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   const u32 waves = blockDim.x / kWave;
   const u32 nc = cv.num_colors;
@@ -396,6 +606,9 @@ __global__ __launch_bounds__(256) void k_color_items(
     for (u32 c = lane; c < nc; c += kWave) cnt[c] = 0;
   }
   cnt_fence<LDS>();
+  Census cs;
+  cs.on = census != nullptr;
+  u64 kmers = 0;
   const u64 n_items = item_ends[n_reads - 1];
   for (u64 it = static_cast<u64>(blockIdx.x) * waves + wave; it < n_items;
        it += static_cast<u64>(gridDim.x) * waves) {
@@ -409,19 +622,20 @@ __global__ __launch_bounds__(256) void k_color_items(
     const u64 i0 = r ? item_ends[r - 1] : 0, ni = item_ends[r] - i0;
     const u64 rs = r ? kmer_ends[r - 1] : 0, re = kmer_ends[r];
     const u64 ks = min(re, rs + (it - i0) * item_kmers), ke = min(re, ks + item_kmers);
-    u32 f = 0, nf = 0, inv = 0;
-    read_add<IN>(cv, results, ks, ke, lane, cnt, f, nf, inv);
-    f = wave_sum(f);
-    nf = wave_sum(nf);
-    inv = wave_sum(inv);
+    u32 f = 0, nf = 0, inv = 0, touched = 0;
+    read_add<IN>(cv, results, ks, ke, lane, cnt, tab, f, nf, inv, touched, cs);
+    kmers += ke - ks;
     cnt_fence<LDS>();
     if (ni == 1) {
-      finish_read<LDS>(cnt, nc, r, f, nf, inv, p, o, lane);
+      finish_read<LDS>(cnt, nc, r, f, nf, inv, p, o, lane, touched);
     } else {
       const u64 slot = long_incl[r] - 1;
       u32 *lc = long_cnt + slot * nc;
       u32 *meta = long_meta + slot * 4;
+      // the counter words this item wrote (all of them with global counters)
+      const u32 tw = LDS ? wave_or(touched) : 0xFFFFFFFFu;
       for (u32 c = lane; c < nc; c += kWave) {
+        if (LDS && !((tw >> (c / kWave)) & 1u)) continue;
         const u32 v = cnt_get<LDS>(cnt + c);
         if (v) {
           atomicAdd(lc + c, v);
@@ -436,6 +650,7 @@ __global__ __launch_bounds__(256) void k_color_items(
     }
     cnt_fence<LDS>();
   }
+  census_flush(cs, kmers, lane, census);
 }
 
 // Reads of more than one work item, after k_color_items: one wave per such
@@ -455,7 +670,7 @@ __global__ __launch_bounds__(256) void k_color_long_finish(u32 nc, u64 n_reads,
     const u64 slot = long_incl[r] - 1;
     u32 *meta = long_meta + slot * 4;
     const u32 f = meta[1], nf = meta[2], inv = meta[3];
-    finish_read<false>(long_cnt + slot * nc, nc, r, f, nf, inv, p, o, lane);
+    finish_read<false>(long_cnt + slot * nc, nc, r, f, nf, inv, p, o, lane, 0u);
     if (lane < 4) meta[lane] = 0;
   }
 }
@@ -489,7 +704,8 @@ __global__ __launch_bounds__(256) void k_color_quirk2(ColorView cv, const u64 *_
                                                        const u64 *__restrict__ slot_read,
                                                        u64 *__restrict__ bitmap, u64 *__restrict__ nout,
                                                        u32 *__restrict__ global_cnt) {
-  extern __shared__ u32 lds[];
+  extern __shared__ __align__(16) u32 lds[];
+  uint4 *tab = wave_table(lds);
   const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
   const u32 waves = blockDim.x / kWave;
   const u32 nc = cv.num_colors;
@@ -508,7 +724,9 @@ __global__ __launch_bounds__(256) void k_color_quirk2(ColorView cv, const u64 *_
     const u64 rc = slot_read[q];
     const u64 ks = rc ? kmer_ends[rc - 1] : 0, ke = kmer_ends[rc];
     u32 f = 0, nf = 0, inv = 0;
-    read_add<IN>(cv, results, ks, ke, lane, cnt, f, nf, inv);
+    Census off;
+    u32 touched = 0;
+    read_add<IN>(cv, results, ks, ke, lane, cnt, tab, f, nf, inv, touched, off);
     cnt_fence<LDS>();
     u64 total = 0;
     for (u32 w = 0; w < words; ++w) {
@@ -567,7 +785,8 @@ __global__ __launch_bounds__(256) void k_color_warped(ColorView cv, const u64 *_
                                                        const u64 *__restrict__ intervals,
                                                        u64 n_colored, u64 *__restrict__ counts,
                                                        u32 *__restrict__ global_cnt) {
-  extern __shared__ u32 lds[];
+  extern __shared__ __align__(16) u32 lds[];
+  uint4 *tab = wave_table(lds);
   const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave, waves = blockDim.x / kWave;
   const u32 nc = cv.num_colors;
   u32 *cnt = wave_counters<LDS>(lds, global_cnt, nc);
@@ -579,7 +798,9 @@ __global__ __launch_bounds__(256) void k_color_warped(ColorView cv, const u64 *_
        q += static_cast<u64>(gridDim.x) * waves) {
     u32 f = 0, nf = 0, inv = 0;  // padding reads as not found
     const u64 wbeg = intervals[q] * kWave, wend = intervals[q + 1] * kWave;
-    read_add<kInKeys>(cv, warped, wbeg, wend, lane, cnt, f, nf, inv);
+    Census off;
+    u32 touched = 0;
+    read_add<kInKeys>(cv, warped, wbeg, wend, lane, cnt, tab, f, nf, inv, touched, off);
     cnt_fence<LDS>();
     for (u32 c = lane; c < nc; c += kWave) {
       const u32 v = cnt_get<LDS>(cnt + c);
@@ -602,13 +823,19 @@ constexpr u32 kGlobalCntBlocks = 1024;
 
 inline bool lds_counters(u64 nc) { return nc <= kLdsColors; }
 
+// Waves loop over their reads or items, so the grid is bounded: a few
+// rounds of the chip's resident workgroups (fewer wave launches, counter
+// clears and kernel prologues than one wave per read).
+constexpr u32 kColorMaxBlocks = 16384;
+
 inline u32 color_grid(u64 work_waves, u64 nc) {
-  const u32 g = grid_for(work_waves, kColorWaves);
+  const u32 g = std::min<u32>(grid_for(work_waves, kColorWaves), kColorMaxBlocks);
   return lds_counters(nc) ? g : std::min<u32>(g, kGlobalCntBlocks);
 }
 
+// Each wave's table, then (nc <= kLdsColors) each wave's counters.
 inline u32 color_lds_bytes(u64 nc) {
-  return lds_counters(nc) ? static_cast<u32>(kColorWaves * nc * 4) : 0u;
+  return static_cast<u32>(kColorWaves * (kTabWords + (lds_counters(nc) ? nc : 0)) * 4);
 }
 
 template <typename F>
@@ -780,7 +1007,13 @@ ColorSearcher::ColorSearcher(const DeviceColors *colors, u64 max_indexes, u64 ma
     SBWT_HIP(hipMalloc(&d_global_cnt_, bytes));
     SBWT_HIP(hipMemset(d_global_cnt_, 0, bytes));
   }
+  SBWT_HIP(hipMalloc(&d_census_, kCensusWords * 8));
   for (auto &e : ev_) SBWT_HIP(hipEventCreate(&e));
+}
+
+void ColorSearcher::last_census(u64 out[4]) {
+  SBWT_HIP(hipMemcpyAsync(out, d_census_, kCensusWords * 8, hipMemcpyDeviceToHost, stream_));
+  SBWT_HIP(hipStreamSynchronize(stream_));
 }
 
 void build_color_table(DeviceIndex &index, const DeviceColors &colors) {
@@ -853,7 +1086,8 @@ void ColorSearcher::set_work_items(u32 item_kmers) {
 ColorSearcher::~ColorSearcher() {
   for (auto &e : ev_) (void)hipEventDestroy(e);
   void *bufs[] = {d_bitmap_, d_nout_,      d_scan_tmp_, d_quirk_,    d_warped_,
-                  d_intervals_, d_counts_, d_global_cnt_, d_items_, d_long_cnt_, d_long_meta_};
+                  d_intervals_, d_counts_, d_global_cnt_, d_items_, d_long_cnt_, d_long_meta_,
+                  d_census_};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (own_stream_) (void)hipStreamDestroy(stream_);
@@ -938,6 +1172,11 @@ void ColorSearcher::launch_threshold(const u64 *d_results, const u64 *d_kmer_end
   u64 *colored_incl = d_quirk_ + 2 * max_reads_, *slot_read = d_quirk_ + 3 * max_reads_;
   const ReadOut out{d_bitmap_, d_nout_, d_stats, quirk2 ? blank_min : nullptr,
                     quirk2 ? colored : nullptr};
+  u64 *census = nullptr;
+  if (census_on_) {  // measurement only: never on in a captured graph
+    census = d_census_;
+    SBWT_HIP(hipMemsetAsync(census, 0, kCensusWords * 8, stream_));
+  }
   if (timing_) SBWT_HIP(hipEventRecord(ev_[0], stream_));
   if (item_kmers_) {
     // the batched form: a device queue of (read, k-mer range) items
@@ -953,7 +1192,7 @@ void ColorSearcher::launch_threshold(const u64 *d_results, const u64 *d_kmer_end
       k_color_items<LDS, IN><<<color_grid(bound, nc), kColorWaves * kWave, color_lds_bytes(nc),
                                stream_>>>(colors_->view(), d_results, d_kmer_ends, n_reads, item_ends,
                                       item_kmers_, long_incl, kp, out, d_global_cnt_, d_long_cnt_,
-                                      d_long_meta_);
+                                      d_long_meta_, census);
     });
     SBWT_HIP(hipGetLastError());
     k_color_long_finish<<<grid_for(n_reads, kColorWaves), kColorWaves * kWave, 0, stream_>>>(
@@ -962,7 +1201,7 @@ void ColorSearcher::launch_threshold(const u64 *d_results, const u64 *d_kmer_end
     with_counters_input(nc, input_, [&]<bool LDS, int IN>() {
       k_color_reads<false, LDS, IN><<<color_grid(n_reads, nc), kColorWaves * kWave,
                                       color_lds_bytes(nc), stream_>>>(
-          colors_->view(), d_results, d_kmer_ends, n_reads, kp, out, nullptr, d_global_cnt_);
+          colors_->view(), d_results, d_kmer_ends, n_reads, kp, out, nullptr, d_global_cnt_, census);
     });
   }
   SBWT_HIP(hipGetLastError());
@@ -992,7 +1231,7 @@ void ColorSearcher::counts_device(const u64 *d_results, const u64 *d_kmer_ends, 
   with_counters_input(nc, input_, [&]<bool LDS, int IN>() {
     k_color_reads<true, LDS, IN><<<color_grid(n_reads, nc), kColorWaves * kWave,
                                    color_lds_bytes(nc), stream_>>>(colors_->view(), d_results, d_kmer_ends, n_reads, kp, out,
-                                          d_counts, d_global_cnt_);
+                                          d_counts, d_global_cnt_, nullptr);
   });
   SBWT_HIP(hipGetLastError());
   SBWT_HIP(hipEventRecord(ev_[1], stream_));

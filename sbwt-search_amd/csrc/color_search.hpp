@@ -102,6 +102,12 @@ class ColorSearcher {
   // default), or the color-table entries that a searcher with color output
   // writes over index (whose table must come from this color index).
   void set_input(const DeviceIndex *index);
+  // Census of the fused color stage (measurement; leaves results unchanged):
+  // when on, each pseudoalign call counts {key-set entries read, key-mark
+  // entries read, 128-byte payload lines spanned by the decoded sets, k-mers},
+  // read back by last_census (synchronizes the stream).
+  void set_census(bool on) { census_on_ = on; }
+  void last_census(u64 out[4]);
 
  private:
   // k_color_reads (bitmap + count per read) and, with an include flag, the
@@ -125,6 +131,8 @@ class ColorSearcher {
   u64 long_slots_ = 0;
   // staging for the reference interface
   u64 *d_warped_ = nullptr, *d_intervals_ = nullptr, *d_counts_ = nullptr;
+  u64 *d_census_ = nullptr;    // [4] census of the last call
+  bool census_on_ = false;
   hipEvent_t ev_[2];
   bool recorded_ = false;
   bool timing_ = true;
