@@ -461,7 +461,7 @@ def _colored_index(S, built, col, tag):
     _kmer_table(_ARGS, idx)
     cidx = S.ColorIndex(path)
     log(f"colors: {col.num_colors} colors, {col.num_sets} sets ({col.num_dense} dense), "
-        f"{time.time() - t0:.1f} s")
+        f"{cidx.device_bytes / 2**30:.1f} GiB on the device, {time.time() - t0:.1f} s")
     os.remove(path)
     os.rmdir(tmp)
     return idx, cidx

@@ -130,6 +130,8 @@ def lib():
         L.sbwt_searcher_set_color_output.argtypes = [vp, C.c_int]
         L.sbwt_color_searcher_set_input.argtypes = [vp, vp]
         L.sbwt_color_searcher_last_time.argtypes = [vp, C.POINTER(C.c_float)]
+        L.sbwt_color_searcher_set_census.argtypes = [vp, C.c_int]
+        L.sbwt_color_searcher_last_census.argtypes = [vp, u64p]
         L.sbwt_index_file_parser_create.argtypes = [C.c_uint64, C.c_uint64, vp, C.POINTER(vp)]
         L.sbwt_index_file_parser_destroy.argtypes = [vp]
         L.sbwt_index_file_parser_destroy.restype = None

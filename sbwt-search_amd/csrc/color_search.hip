@@ -69,7 +69,6 @@ constexpr u32 kPerLane = 8;
 constexpr u32 kDenseUnroll = 2;
 // The wave's LDS table: 64 entries of 16 bytes.
 constexpr u32 kTabWords = kWave * 4;
-constexpr u32 kNoColor = 0xFFFFFFFFu;
 
 __device__ __forceinline__ u32 wave_sum(u32 x) {
   for (u32 o = kWave / 2; o > 0; o >>= 1) x += __shfl_xor(x, o, kWave);
