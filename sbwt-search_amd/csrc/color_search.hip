@@ -215,20 +215,30 @@ __device__ __forceinline__ void wave_payload(const ColorView &cv, const SetRef &
     uint4 e = tab[o];
     u32 excl = t_first - u;  // the set's first id in the concatenation
     if (width <= 32) {
-      // phase 1: each id's set (stepping on through the table), its two
-      // dwords loaded (all in flight) and {bit offset, run, valid} kept
+      // phase 1: a bit cursor over the lane's ids (one element on per id; the
+      // next set's first element when the range crosses into it), each id's
+      // two dwords loaded (all in flight) and {bit offset, run, valid} kept
       uint2 w[kPerLane];
       u32 meta[kPerLane];
+      u64 bit = ((static_cast<u64>(e.y) | (static_cast<u64>(e.z) << 32)) + (t_first - excl)) * width;
+      u32 end = e.x;                    // the current set's end in the concatenation
+      u32 rs = min(e.w, 1023u) << 5;    // its run, placed for meta
+      const u32 *sparse32 = reinterpret_cast<const u32 *>(cv.sparse_arrays);
 #pragma unroll
       for (u32 k = 0; k < kPerLane; ++k) {
-        const u32 tk = min(t + k, s_total - 1);
-        while (tk >= e.x && o < kWave - 1) {  // on to the next set with ids (the ends rise)
-          excl = e.x;
-          e = tab[++o];
+        const u32 tk = t + k;
+        if (tk >= end && tk < s_total) {
+          // on to the next set with ids (the ends rise); tk is its first id
+          do {
+            e = tab[++o];
+          } while (e.x <= tk && o < kWave - 1);
+          bit = (static_cast<u64>(e.y) | (static_cast<u64>(e.z) << 32)) * width;
+          end = e.x;
+          rs = min(e.w, 1023u) << 5;
         }
-        const u64 bit = ((static_cast<u64>(e.y) | (static_cast<u64>(e.z) << 32)) + (tk - excl)) * width;
-        w[k] = *reinterpret_cast<const uint2 *>(reinterpret_cast<const u32 *>(cv.sparse_arrays) + (bit >> 5));
-        meta[k] = (static_cast<u32>(bit) & 31u) | (min(e.w, 1023u) << 5) | (t + k < s_total ? 1u << 15 : 0u);
+        w[k] = *reinterpret_cast<const uint2 *>(sparse32 + (bit >> 5));
+        meta[k] = (static_cast<u32>(bit) & 31u) | rs | (tk < s_total ? 1u << 15 : 0u);
+        if (tk + 1 < s_total) bit += width;  // past the last id the cursor stays in the arrays
       }
       // phase 2: the ids and their counter adds
       const u32 wmask = width == 32 ? ~0u : ((1u << width) - 1);
