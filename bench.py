@@ -377,6 +377,7 @@ def _leg_summary(leg, r):
                 "color_kernel_frac_of_random_line_ceiling":
                     r["color_roofline"]["color_table"]["frac_of_random_line_ceiling"],
                 "color_kernel_ms": r["color_kernel_ms"],
+                "key_runs_roofline_frac": r["key_runs_roofline"]["frac"],
                 "seconds_per_step": r["seconds"]}
     if leg == "batched":
         keep = ("reads", "batches", "plain_s", "batched_unfused_s", "batched_s", "speedup",
@@ -562,7 +563,8 @@ def mode_colors(args, configs, S, torch, W):
                 times["total"].append(t2 - t0)
         med = {k: float(np.median(v)) for k, v in times.items()}
         lists = (d_off.cpu().numpy().copy(), d_ids[:n_ids].cpu().numpy().copy())
-        return med, n_ids, lists, s.last_timings()["search_ms"], cs.last_time()
+        tm = s.last_timings()
+        return med, n_ids, lists, tm, cs.last_time()
 
     t0 = time.perf_counter()
     kind = idx.build_color_table(cidx)
@@ -576,8 +578,8 @@ def mode_colors(args, configs, S, torch, W):
                                   d_res.data_ptr())
         launch = {"chunk_kmers": ch, "blocks_per_cu": bpc}
         log(f"autotune: {ch} k-mers per work item, {bpc} workgroups per CU ({tms:.2f} ms)")
-    med_f, n_ids_f, lists_f, search_ms_f, color_ms_f = run(True)
-    med, n_ids, lists, search_ms, color_ms = run(False)
+    med_f, n_ids_f, lists_f, tm_f, color_ms_f = run(True)
+    med, n_ids, lists, tm, color_ms = run(False)
     same = n_ids == n_ids_f and all(np.array_equal(a, b) for a, b in zip(lists, lists_f))
     log(f"key nodes: {wl.n_reads / med['total'] / 1e6:.1f} M reads/s; color table: "
         f"{wl.n_reads / med_f['total'] / 1e6:.1f} M reads/s; lists identical: {same}")
@@ -585,7 +587,7 @@ def mode_colors(args, configs, S, torch, W):
         raise SystemExit("the color-table form's color lists differ from the key-node form's")
     # census of the index stage in the color-table form (untimed): lines read by
     # the search and the key move
-    search_ms = search_ms_f
+    search_ms = tm_f["search_ms"]
     s.set_color_output(True)
     s.set_stats(True)
     s.search_reads_device(d_chars.data_ptr(), wl.n_chars, d_ends.data_ptr(), wl.n_reads,
@@ -602,6 +604,14 @@ def mode_colors(args, configs, S, torch, W):
     log(f"index stage census: {S.Searcher.lines(stats) / n_kmers:.2f} lines per k-mer, "
         f"{achieved:.0f} GB/s over {search_ms:.2f} ms; random-line ceiling at "
         f"{probe['table_bytes'] >> 20} MiB: {probe['gbs']:.0f} GB/s")
+    # the key-runs pass on its own: each k-mer's result read and rewritten (16 B)
+    # and one color-table line per chain end (the census' walk steps)
+    kr_ms = tm_f["key_runs_ms"]
+    kr_bytes = 16 * n_kmers + LINE * stats["walk_steps"]
+    kr_gbs = kr_bytes / (kr_ms / 1e3) / 1e9 if kr_ms > 0 else None
+    if kr_gbs:
+        log(f"key runs: {kr_ms:.2f} ms, {stats['walk_steps'] / n_kmers:.2f} chain-end lines per "
+            f"k-mer, {kr_gbs:.0f} GB/s by census")
     # census of the color stage in both forms (untimed): key-set, key-mark and
     # payload lines of the color kernel, against its hipEvent time
     words = (int(cidx.num_colors) + 63) // 64
@@ -646,6 +656,9 @@ def mode_colors(args, configs, S, torch, W):
                               "random_line_ceiling_gbs": probe["gbs"],
                               "frac_of_random_line_ceiling": achieved / probe["gbs"],
                               "census": stats},
+           "key_runs_roofline": {"bound": "hbm", "achieved": kr_gbs, "peak": 8000.0,
+                                 "unit": "GB/s", "frac": kr_gbs / 8000.0 if kr_gbs else None,
+                                 "kernel_ms": kr_ms, "algorithmic_bytes": kr_bytes},
            "color_kernel_ms": color_ms_f, "color_roofline": color_roofline,
            "ids_per_step": n_ids,
            "config": {"workload": wl.name + " " + wl.description, "sbwt_nodes": int(built.n),

@@ -202,6 +202,10 @@ int sbwt_index_search_reads_noalloc(const sbwt_index *index, int move_to_key_kme
  * (src/IndexSearcher/IndexSearcher.cu:22,62,66-70). */
 int sbwt_searcher_last_timings(sbwt_searcher *searcher, float *pack_ms, float *offsets_ms,
                                float *search_ms);
+/* Of the last call's search time, the key-k-mer runs pass (streaming search
+ * with move_to_key_kmer; 0 otherwise). No counterpart in the reference, whose
+ * key move sits inside its search kernel (IndexSearcher.cuh:60-70). */
+int sbwt_searcher_last_key_runs_time(sbwt_searcher *searcher, float *ms);
 
 /* Census mode for sbwt_searcher_search_reads(_device): when enabled, the next
  * calls also count the exact algorithmic work of the batch: stats[0] full

@@ -131,6 +131,7 @@ def lib():
         L.sbwt_color_searcher_set_input.argtypes = [vp, vp]
         L.sbwt_color_searcher_last_time.argtypes = [vp, C.POINTER(C.c_float)]
         L.sbwt_color_searcher_set_census.argtypes = [vp, C.c_int]
+        L.sbwt_searcher_last_key_runs_time.argtypes = [vp, C.POINTER(C.c_float)]
         L.sbwt_color_searcher_last_census.argtypes = [vp, u64p]
         L.sbwt_index_file_parser_create.argtypes = [C.c_uint64, C.c_uint64, vp, C.POINTER(vp)]
         L.sbwt_index_file_parser_destroy.argtypes = [vp]
@@ -432,9 +433,11 @@ class Searcher:
         _check(lib().sbwt_searcher_sync(self.h))
 
     def last_timings(self):
-        a, b, c = C.c_float(), C.c_float(), C.c_float()
+        a, b, c, d = C.c_float(), C.c_float(), C.c_float(), C.c_float()
         _check(lib().sbwt_searcher_last_timings(self.h, C.byref(a), C.byref(b), C.byref(c)))
-        return {"pack_ms": a.value, "offsets_ms": b.value, "search_ms": c.value}
+        _check(lib().sbwt_searcher_last_key_runs_time(self.h, C.byref(d)))
+        return {"pack_ms": a.value, "offsets_ms": b.value, "search_ms": c.value,
+                "key_runs_ms": d.value}
 
 
 def pack_2bit_device(d_chars, n_chars, d_words, d_invalid, stream=None):

@@ -296,6 +296,13 @@ int sbwt_searcher_search_reads_device(sbwt_searcher *searcher, const char *d_cha
   });
 }
 
+int sbwt_searcher_last_key_runs_time(sbwt_searcher *searcher, float *ms) {
+  return guarded([&] {
+    require(searcher && ms, "null argument");
+    *ms = searcher->impl->last_timings().key_runs_ms;
+  });
+}
+
 int sbwt_searcher_last_timings(sbwt_searcher *searcher, float *pack_ms, float *offsets_ms,
                                float *search_ms) {
   return guarded([&] {

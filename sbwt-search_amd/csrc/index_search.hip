@@ -1239,6 +1239,7 @@ void Searcher::search_positions_device(const u64 *d_bit_seqs, const u64 *d_posit
   if (timing_) {
     SBWT_HIP(hipEventRecord(ev_[3], stream_));
     events_recorded_ = true;
+    key_runs_timed_ = false;
   }
 }
 
@@ -1316,6 +1317,7 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
           ix, d_packed_, d_invalid_, d_chunks_, d_item_ends, n_reads, d_counter_, d_results,
           d_stats_, absence_checks(*index_) ? 1u : 0u);
       if constexpr (KEY) {
+        if (timing_) SBWT_HIP(hipEventRecord(ev_[4], stream_));
         // one wave per work item
         k_key_runs<Idx, STATS><<<grid_for(chunk_bound, kSearchThreads / 64), kSearchThreads, 0,
                                  stream_>>>(ix, d_chunks_, d_item_ends, n_reads, d_results, d_stats_);
@@ -1335,6 +1337,7 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
   if (timing_) {
     SBWT_HIP(hipEventRecord(ev_[3], stream_));
     events_recorded_ = true;
+    key_runs_timed_ = streaming && move_to_key_;
   }
   u64 nk = 0;
   if (n_kmers_host) {
@@ -1380,6 +1383,7 @@ SearchTimings Searcher::last_timings() {
   SBWT_HIP(hipEventElapsedTime(&t.pack_ms, ev_[0], ev_[1]));
   SBWT_HIP(hipEventElapsedTime(&t.offsets_ms, ev_[1], ev_[2]));
   SBWT_HIP(hipEventElapsedTime(&t.search_ms, ev_[2], ev_[3]));
+  if (key_runs_timed_) SBWT_HIP(hipEventElapsedTime(&t.key_runs_ms, ev_[4], ev_[3]));
   return t;
 }
 

@@ -33,6 +33,7 @@ constexpr u32 kAutoMinChunkKmers = 8;
 
 struct SearchTimings {
   float pack_ms = 0, offsets_ms = 0, search_ms = 0;
+  float key_runs_ms = 0;  // of search_ms: the key-k-mer runs after the streaming search
 };
 
 // Launch shape of the streaming search (work item = up to chunk_kmers
@@ -142,7 +143,8 @@ class Searcher {
   u64 *d_positions_ = nullptr;
   u64 *d_bits_ = nullptr;
   u64 staged_chars_ = 0, staged_reads_ = 0, staged_positions_ = 0;
-  hipEvent_t ev_[4];
+  hipEvent_t ev_[5];  // pack, offsets, search, end; [4] between the search and the key runs
+  bool key_runs_timed_ = false;
   bool events_recorded_ = false;
 };
 
