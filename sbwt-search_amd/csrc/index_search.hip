@@ -907,6 +907,8 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
 // shuffle per block; the block's first k-mer hands its key and flag to the
 // block on its left. Only chain ends read the key table, instead of every
 // found k-mer. Work items never share a chain: each starts with a full search.
+constexpr u32 kRunBlocks = 4;  // blocks of 64 k-mers whose chain ends are looked up together
+
 template <typename Idx, bool STATS>
 __global__ __launch_bounds__(kSearchThreads) void k_key_runs(IndexView ix,
                                                             const ulonglong2 *__restrict__ chunks,
@@ -925,55 +927,63 @@ __global__ __launch_bounds__(kSearchThreads) void k_key_runs(IndexView ix,
     const u32 len = static_cast<u32>(item.y >> 48);
     u32 carry_flag = kPredLookup;  // flag of the k-mer right of the block
     u64 carry_key = 0;             // and its key
-    // lane 63 holds k-mer hi - 1, lane q k-mer hi - 64 + q; none below lo
+    // the block ending at hi_: lane 63 holds k-mer hi_ - 1, lane q k-mer
+    // hi_ - 64 + q; none below hi_ - 64 or 0
     auto block_pos = [&](u32 hi_, bool &valid_) {
       const u32 lo_ = hi_ > 64 ? hi_ - 64 : 0;
       valid_ = hi_ > 0 && lane + hi_ >= lo_ + 64;
       return g0 + (valid_ ? hi_ + lane - 64 : 0);
     };
-    bool valid_next = false;
-    u64 pos_next = block_pos(len, valid_next);
-    u64 rv_next = valid_next ? out[pos_next] : kNotFound;
-    for (u32 hi = len; hi > 0;) {
-      const u32 lo = hi > 64 ? hi - 64 : 0;
-      const bool valid = valid_next;
-      const u64 pos = pos_next;
-      const u64 rv = rv_next;
-      // the block to the left is loaded while this one is resolved
-      pos_next = block_pos(lo, valid_next);
-      rv_next = valid_next ? out[pos_next] : kNotFound;
-      const bool found = (rv >> 63) == 0;
-      const u32 flag = found ? static_cast<u32>(rv >> kPredShift) & 3u : kPredLookup;
-      const u64 node = rv & kNodeMask;
-      u32 succ = static_cast<u32>(__shfl_down(static_cast<int>(flag), 1, 64));
-      if (lane == 63) succ = carry_flag;
-      const bool same = found && succ == kPredSameKey;
-      const bool brk = found && !same;
-      u64 base = 0;
-      if (brk) {
-        if (succ == kPredStuck) {
-          base = kWalkStuck;
-        } else if (ix.color_table) {  // the key's color entry (for a marked node, its own)
-          base = color_table_entry<Idx>(ix, static_cast<Idx>(node));
-          if (STATS) ++st_walk;
-        } else if (succ == kPredMarked) {
-          base = node;
-        } else {
-          u32 w = 0;
-          base = key_move<Idx>(ix, static_cast<Idx>(node), w);
-          if (STATS) st_walk += w;
-        }
+    // groups of kRunBlocks blocks, right to left: their results are loaded
+    // together, then every chain end's lookup of the group is issued before
+    // any is used, then the chains are resolved block by block
+    for (u32 hi = len; hi > 0; hi = hi > 64 * kRunBlocks ? hi - 64 * kRunBlocks : 0) {
+      u64 rv[kRunBlocks], pos[kRunBlocks], base[kRunBlocks];
+      bool valid[kRunBlocks], brk[kRunBlocks];
+      u32 flag[kRunBlocks];
+#pragma unroll
+      for (u32 b = 0; b < kRunBlocks; ++b) {
+        pos[b] = block_pos(hi > 64 * b ? hi - 64 * b : 0, valid[b]);
+        rv[b] = valid[b] ? out[pos[b]] : kNotFound;
       }
-      // a chain takes the key of the first break to its right, else the carry
-      const u64 breaks = __ballot(brk);
-      const u64 right = lane == 63 ? 0 : breaks & (~0ULL << (lane + 1));
-      const int src = right ? __ffsll(static_cast<long long>(right)) - 1 : 0;
-      const u64 from = __shfl(base, src, 64);
-      const u64 key = brk ? base : (right ? from : carry_key);
-      if (valid && found) out[pos] = key;
-      carry_flag = static_cast<u32>(__shfl(static_cast<int>(flag), 0, 64));
-      carry_key = __shfl(key, 0, 64);
-      hi = lo;
+      u32 right_flag = carry_flag;
+#pragma unroll
+      for (u32 b = 0; b < kRunBlocks; ++b) {
+        const bool found = (rv[b] >> 63) == 0;
+        flag[b] = found ? static_cast<u32>(rv[b] >> kPredShift) & 3u : kPredLookup;
+        const u64 node = rv[b] & kNodeMask;
+        u32 succ = static_cast<u32>(__shfl_down(static_cast<int>(flag[b]), 1, 64));
+        if (lane == 63) succ = right_flag;
+        brk[b] = found && succ != kPredSameKey;
+        base[b] = 0;
+        if (brk[b]) {
+          if (succ == kPredStuck) {
+            base[b] = kWalkStuck;
+          } else if (ix.color_table) {  // the key's color entry (for a marked node, its own)
+            base[b] = color_table_entry<Idx>(ix, static_cast<Idx>(node));
+            if (STATS) ++st_walk;
+          } else if (succ == kPredMarked) {
+            base[b] = node;
+          } else {
+            u32 w = 0;
+            base[b] = key_move<Idx>(ix, static_cast<Idx>(node), w);
+            if (STATS) st_walk += w;
+          }
+        }
+        right_flag = static_cast<u32>(__shfl(static_cast<int>(flag[b]), 0, 64));
+      }
+#pragma unroll
+      for (u32 b = 0; b < kRunBlocks; ++b) {
+        // a chain takes the key of the first break to its right, else the carry
+        const u64 breaks = __ballot(brk[b]);
+        const u64 right = lane == 63 ? 0 : breaks & (~0ULL << (lane + 1));
+        const int src = right ? __ffsll(static_cast<long long>(right)) - 1 : 0;
+        const u64 from = __shfl(base[b], src, 64);
+        const u64 key = brk[b] ? base[b] : (right ? from : carry_key);
+        if (valid[b] && (rv[b] >> 63) == 0) out[pos[b]] = key;
+        carry_flag = static_cast<u32>(__shfl(static_cast<int>(flag[b]), 0, 64));
+        carry_key = __shfl(key, 0, 64);
+      }
     }
   }
   if constexpr (STATS) {
