@@ -801,17 +801,20 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
       ++p;
       --left;
     };
-    while (emitted || decide) {
-      if (emitted) {
-        stage(res);
-        emitted = false;
+    // emit res_ (when emitted_), then set up the lane's next k-mer: its window,
+    // and the state that finds it
+    auto advance = [&](u64 res_, bool emitted_, bool decide_) {
+    while (emitted_ || decide_) {
+      if (emitted_) {
+        stage(res_);
+        emitted_ = false;
         if (left == 0) {  // chunk done: wait for the next one
           want = true;
           mode = kChunk;
           break;
         }
       }
-      decide = false;
+      decide_ = false;
       // windows settled by an earlier absence check: a tight loop, since a
       // run (up to k - kp + 1 windows) would otherwise hold the whole wave in
       // this loop's full body once per window
@@ -843,10 +846,10 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
       window = sh ? ((wa << sh) | (wb >> (64 - sh))) : wa;
       const u64 iv = q ? ((i0 >> q) | (i1 << (64 - q))) : i0;
       if (iv & inv_mask) {
-        res = kInvalid;
+        res_ = kInvalid;
         prev = false;
         hint = kHintNone;
-        emitted = true;
+        emitted_ = true;
         if (STATS) ++st_inv;
         continue;
       }
@@ -877,6 +880,31 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
           start_search();
         }
       }
+    }
+    };
+    // Then a second stream step in the same iteration, for the common case
+    // only: a lane whose next k-mer is v's edge c in v's own letter slot. Every
+    // other state's code then runs once per two streamed k-mers of most lanes,
+    // not per one; any other outcome (no edge c at v: its suffix group decides)
+    // leaves the lane in kStream for the next iteration's full step. One copy
+    // of advance() serves both passes (two copies cost registers, then spills).
+#pragma clang loop unroll(disable)
+    for (u32 pass = 0; pass < (KEY ? 1u : 2u); ++pass) {
+      advance(res, emitted, decide);
+      emitted = false;
+      decide = false;
+      if (!KEY && pass == 0 && mode == kStream) {
+        const Idx bs = block_of(v);
+        const u32 os = static_cast<u32>(v - bs * kBlockNodes);
+        const uint4 ds = *line_slot(ix, bs, c);
+        if (entry_bit(ds, os)) {
+          if (STATS) ++st_stream;
+          v = cmap_of<Idx>(ix, c) + entry_base(ix, ds, bs, c) + entry_popc(ds, os);
+          res = static_cast<u64>(v);
+          emitted = true;
+        }
+      }
+      if (__ballot(emitted) == 0) break;
     }
   }
   if constexpr (STATS) {
@@ -1164,7 +1192,7 @@ u32 Searcher::stream_chunk(u64 n_chars) const {
 LaunchShape Searcher::autotune(const char *d_chars, u64 n_chars, const u64 *d_read_ends, u64 n_reads,
                                u64 *d_results) {
   static constexpr u32 kChunks[] = {128, 256, 512, 1024};
-  static constexpr u32 kBlocks[] = {2, 3, 4, 6, 8};
+  static constexpr u32 kBlocks[] = {2, 3, 4, 5, 6, 7, 8};
   const bool timing = timing_;
   const int algorithm = algorithm_;
   timing_ = true;

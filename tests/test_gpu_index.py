@@ -544,7 +544,7 @@ def test_autotune_keeps_results(genome_index):
     dr = torch.empty(len(chars), dtype=torch.int64, device="cuda")
     s = idx.searcher(len(chars), len(ends))
     ch, bpc, ms = s.autotune(dc.data_ptr(), len(chars), de.data_ptr(), len(ends), dr.data_ptr())
-    assert s.launch() == (ch, bpc) and ch in (128, 256, 512, 1024) and bpc in (2, 3, 4, 6, 8)
+    assert s.launch() == (ch, bpc) and ch in (128, 256, 512, 1024) and bpc in (2, 3, 4, 5, 6, 7, 8)
     assert ms > 0
     exp = orc.index_reads(chars, ends)
     assert np.array_equal(dr[:len(exp)].cpu().numpy().view(np.uint64), exp)
