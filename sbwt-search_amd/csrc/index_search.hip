@@ -436,6 +436,12 @@ __global__ void k_reset_counter(u64 *c) { *c = 0; }
 // loads. Device-scope atomics on one address serialize at the memory side, so
 // a counter touched once per lane refill stalls every wave; per 64 chunks it
 // only balances the tail.
+// Stream steps per loop iteration at most (the first through the full state
+// machine, the second only for a found edge of the lane's node). On C2, 2 /
+// 3 / 4 passes: search kernel 23.6 / 24.0 / 24.6 ms (each pass adds a
+// dependent round trip to the iteration).
+constexpr u32 kStreamPasses = 2;
+
 template <typename Idx, bool KEY, bool STATS>
 __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     IndexView ix, const u64 *__restrict__ packed, const u32 *__restrict__ invalid,
@@ -889,11 +895,11 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     // leaves the lane in kStream for the next iteration's full step. One copy
     // of advance() serves both passes (two copies cost registers, then spills).
 #pragma clang loop unroll(disable)
-    for (u32 pass = 0; pass < (KEY ? 1u : 2u); ++pass) {
+    for (u32 pass = 0; pass < (KEY ? 1u : kStreamPasses); ++pass) {
       advance(res, emitted, decide);
       emitted = false;
       decide = false;
-      if (!KEY && pass == 0 && mode == kStream) {
+      if (!KEY && pass + 1 < kStreamPasses && mode == kStream) {
         const Idx bs = block_of(v);
         const u32 os = static_cast<u32>(v - bs * kBlockNodes);
         const uint4 ds = *line_slot(ix, bs, c);
