@@ -442,6 +442,21 @@ __global__ void k_reset_counter(u64 *c) { *c = 0; }
 // dependent round trip to the iteration).
 constexpr u32 kStreamPasses = 2;
 
+// What v's key-walk code (from the walk slot d3 of v's line; o = v's offset in
+// the line) says about the k-mer found on v's edge c: v marked, v with no edge
+// (stuck), v's first edge is c (the two k-mers share a key), or unknown.
+__device__ __forceinline__ u32 walk_pred(const uint4 d3, u32 o, bool has_c, u32 c) {
+  // v's code: the byte of its three-node group, then its base-6 digit
+  const u32 o3 = o >= kWalkNodesPerSlot ? o - kWalkNodesPerSlot : o;
+  const u32 q3 = o3 / 3;
+  const u32 word = (q3 >> 2) == 0 ? d3.x : ((q3 >> 2) == 1 ? d3.y : ((q3 >> 2) == 2 ? d3.z : d3.w));
+  const u32 byte = (word >> (8 * (q3 & 3))) & 255u;
+  const u32 dig = o3 - 3 * q3;
+  const u32 wcode = (dig == 0 ? byte : (dig == 1 ? byte / 6 : byte / 36)) % 6;
+  return wcode == kWalkMarked ? kPredMarked
+         : (wcode == kWalkNoEdge ? kPredStuck : (has_c && wcode == c ? kPredSameKey : kPredLookup));
+}
+
 template <typename Idx, bool KEY, bool STATS>
 __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     IndexView ix, const u64 *__restrict__ packed, const u32 *__restrict__ invalid,
@@ -622,18 +637,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         break;
       case kStream:
         if (STATS) ++st_stream;
-        if constexpr (KEY) {
-          // v's code: the byte of its three-node group, then its base-6 digit
-          const u32 o3 = o1 >= kWalkNodesPerSlot ? o1 - kWalkNodesPerSlot : o1;
-          const u32 q3 = o3 / 3;
-          const u32 word = (q3 >> 2) == 0 ? d3.x : ((q3 >> 2) == 1 ? d3.y : ((q3 >> 2) == 2 ? d3.z : d3.w));
-          const u32 byte = (word >> (8 * (q3 & 3))) & 255u;
-          const u32 dig = o3 - 3 * q3;
-          const u32 wcode = (dig == 0 ? byte : (dig == 1 ? byte / 6 : byte / 36)) % 6;
-          pred = wcode == kWalkMarked ? kPredMarked
-                 : (wcode == kWalkNoEdge ? kPredStuck
-                                         : (bit1 && wcode == c ? kPredSameKey : kPredLookup));
-        }
+        if constexpr (KEY) pred = walk_pred(d3, o1, bit1, c);
         if (bit1) {
           found = true;
         } else if (!ix.has_groups) {
@@ -895,18 +899,22 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     // leaves the lane in kStream for the next iteration's full step. One copy
     // of advance() serves both passes (two copies cost registers, then spills).
 #pragma clang loop unroll(disable)
-    for (u32 pass = 0; pass < (KEY ? 1u : kStreamPasses); ++pass) {
+    for (u32 pass = 0; pass < kStreamPasses; ++pass) {
       advance(res, emitted, decide);
       emitted = false;
       decide = false;
-      if (!KEY && pass + 1 < kStreamPasses && mode == kStream) {
+      if (pass + 1 < kStreamPasses && mode == kStream) {
         const Idx bs = block_of(v);
         const u32 os = static_cast<u32>(v - bs * kBlockNodes);
         const uint4 ds = *line_slot(ix, bs, c);
+        // KEY: v's key-walk code from the same line, as in the full step
+        uint4 dw = make_uint4(0, 0, 0, 0);
+        if constexpr (KEY) dw = *line_slot(ix, bs, kSlotWalk + (os >= kWalkNodesPerSlot ? 1u : 0u));
         if (entry_bit(ds, os)) {
           if (STATS) ++st_stream;
           v = cmap_of<Idx>(ix, c) + entry_base(ix, ds, bs, c) + entry_popc(ds, os);
           res = static_cast<u64>(v);
+          if constexpr (KEY) res |= static_cast<u64>(walk_pred(dw, os, true, c)) << kPredShift;
           emitted = true;
         }
       }
