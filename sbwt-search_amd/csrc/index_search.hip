@@ -442,6 +442,19 @@ __global__ void k_reset_counter(u64 *c) { *c = 0; }
 // dependent round trip to the iteration).
 constexpr u32 kStreamPasses = 2;
 
+// v's suffix-group start when its line holds it: the last group-start bit at
+// or before offset o in the line's group slot dg (pos = its offset); false
+// when the group began in an earlier line.
+__device__ __forceinline__ bool group_start_in_line(const uint4 dg, u32 o, u32 &pos) {
+  const u32 q = o + 1;
+  const u64 glo = (static_cast<u64>(dg.y) | (static_cast<u64>(dg.z) << 32)) &
+                  (q >= 64 ? ~0ULL : ((1ULL << q) - 1));
+  const u32 ghi = q > 64 ? (dg.w & ((q - 64 >= 32) ? ~0u : ((1u << (q - 64)) - 1))) : 0u;
+  if ((ghi | glo) == 0) return false;
+  pos = ghi ? 95 - static_cast<u32>(__builtin_clz(ghi)) : 63 - static_cast<u32>(__builtin_clzll(glo));
+  return true;
+}
+
 // What v's key-walk code (from the walk slot d3 of v's line; o = v's offset in
 // the line) says about the k-mer found on v's edge c: v marked, v with no edge
 // (stuck), v's first edge is c (the two k-mers share a key), or unknown.
@@ -646,13 +659,8 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         } else {
           // v's group start: the last group-start bit at or before v in this
           // line, else delta nodes before the line
-          const u32 q = o1 + 1;
-          const u64 glo = (static_cast<u64>(d2.y) | (static_cast<u64>(d2.z) << 32)) &
-                          (q >= 64 ? ~0ULL : ((1ULL << q) - 1));
-          const u32 ghi = q > 64 ? (d2.w & ((q - 64 >= 32) ? ~0u : ((1u << (q - 64)) - 1))) : 0u;
-          if (ghi | glo) {
-            const u32 pos = ghi ? 95 - static_cast<u32>(__builtin_clz(ghi))
-                                : 63 - static_cast<u32>(__builtin_clzll(glo));
+          u32 pos = 0;
+          if (group_start_in_line(d2, o1, pos)) {
             if (STATS && pos != o1) ++st_group;
             // the group's edges sit at its start (v itself when pos == o1)
             if (pos != o1 && entry_bit(d1, pos)) {
@@ -907,15 +915,36 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         const Idx bs = block_of(v);
         const u32 os = static_cast<u32>(v - bs * kBlockNodes);
         const uint4 ds = *line_slot(ix, bs, c);
-        // KEY: v's key-walk code from the same line, as in the full step
+        // the group slot of the same line (v's suffix-group start), and, KEY,
+        // v's key-walk code, as in the full step
+        const uint4 dg = *line_slot(ix, bs, ix.has_groups ? kSlotGroups : c);
         uint4 dw = make_uint4(0, 0, 0, 0);
         if constexpr (KEY) dw = *line_slot(ix, bs, kSlotWalk + (os >= kWalkNodesPerSlot ? 1u : 0u));
-        if (entry_bit(ds, os)) {
+        const bool has_c = entry_bit(ds, os);
+        u32 at = os;  // where edge c is read: v, or its group start in this line
+        bool hit = has_c, miss = false;
+        if (!has_c && ix.has_groups) {
+          u32 pos = 0;
+          if (group_start_in_line(dg, os, pos)) {
+            if (STATS && pos != os) ++st_group;
+            hit = pos != os && entry_bit(ds, pos);
+            miss = !hit;
+            at = pos;
+          }
+        }
+        if (hit || miss) {  // else the full step of the next iteration decides
           if (STATS) ++st_stream;
-          v = cmap_of<Idx>(ix, c) + entry_base(ix, ds, bs, c) + entry_popc(ds, os);
-          res = static_cast<u64>(v);
-          if constexpr (KEY) res |= static_cast<u64>(walk_pred(dw, os, true, c)) << kPredShift;
           emitted = true;
+          if (hit) {
+            v = cmap_of<Idx>(ix, c) + entry_base(ix, ds, bs, c) + entry_popc(ds, at);
+            res = static_cast<u64>(v);
+            if constexpr (KEY) res |= static_cast<u64>(walk_pred(dw, os, has_c, c)) << kPredShift;
+          } else {  // the k-mer's last char made the miss: its predecessor was found
+            res = kNotFound;
+            prev = false;
+            hint = kHintMiss;
+            miss_at = p + k - 1;
+          }
         }
       }
       if (__ballot(emitted) == 0) break;
