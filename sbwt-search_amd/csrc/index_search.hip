@@ -436,11 +436,12 @@ __global__ void k_reset_counter(u64 *c) { *c = 0; }
 // loads. Device-scope atomics on one address serialize at the memory side, so
 // a counter touched once per lane refill stalls every wave; per 64 chunks it
 // only balances the tail.
-// Stream steps per loop iteration at most (the first through the full state
-// machine, the second only for a found edge of the lane's node). On C2, 2 /
-// 3 / 4 passes: search kernel 23.6 / 24.0 / 24.6 ms (each pass adds a
-// dependent round trip to the iteration).
-constexpr u32 kStreamPasses = 2;
+// Stream steps per loop iteration at most: the first through the full state
+// machine, the others settled from the lane's node line alone (an edge of the
+// node, or of its suffix-group start in the same line, or a miss). Each pass
+// adds a dependent round trip to the iteration; on C2, 2 / 3 / 4 passes:
+// search kernel 23.3 / 22.8 / 22.9 ms.
+constexpr u32 kStreamPasses = 3;
 
 // v's suffix-group start when its line holds it: the last group-start bit at
 // or before offset o in the line's group slot dg (pos = its offset); false
@@ -900,12 +901,13 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
       }
     }
     };
-    // Then a second stream step in the same iteration, for the common case
-    // only: a lane whose next k-mer is v's edge c in v's own letter slot. Every
-    // other state's code then runs once per two streamed k-mers of most lanes,
-    // not per one; any other outcome (no edge c at v: its suffix group decides)
-    // leaves the lane in kStream for the next iteration's full step. One copy
-    // of advance() serves both passes (two copies cost registers, then spills).
+    // Then further stream steps in the same iteration (kStreamPasses in all),
+    // for the cases v's own line settles: the next k-mer on v's edge c, on the
+    // edge c of v's suffix-group start in the same line, or a miss. Every other
+    // state's code then runs once per several streamed k-mers of most lanes, not
+    // per one; any other outcome (a group that began in an earlier line) leaves
+    // the lane in kStream for the next iteration's full step. One copy of
+    // advance() serves every pass (two copies cost registers, then spills).
 #pragma clang loop unroll(disable)
     for (u32 pass = 0; pass < kStreamPasses; ++pass) {
       advance(res, emitted, decide);
