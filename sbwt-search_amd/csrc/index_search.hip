@@ -443,6 +443,26 @@ __global__ void k_reset_counter(u64 *c) { *c = 0; }
 // search kernel 23.3 / 22.8 / 22.9 ms.
 constexpr u32 kStreamPasses = 3;
 
+// One probe of the k-mer table: the aligned bucket pair {b0, b1} read for the
+// probe at bucket h (an odd h skips the pair's first bucket). 1: the key's
+// bucket (node set), 0: an empty bucket first (the k-mer is absent), -1: both
+// buckets hold other keys (probe the next pair).
+__device__ __forceinline__ int probe_pair(const uint4 b0, const uint4 b1, u64 h, u64 key, u32 &node) {
+  const u64 k0 = static_cast<u64>(b0.x) | (static_cast<u64>(b0.y) << 32);
+  const u64 k1 = static_cast<u64>(b1.x) | (static_cast<u64>(b1.y) << 32);
+  const bool use0 = (h & 1) == 0;
+  if (use0 && k0 == key) {
+    node = b0.z;
+    return 1;
+  }
+  if (use0 && k0 == 0) return 0;
+  if (k1 == key) {
+    node = b1.z;
+    return 1;
+  }
+  return k1 == 0 ? 0 : -1;
+}
+
 // v's suffix-group start when its line holds it: the last group-start bit at
 // or before offset o in the line's group slot dg (pos = its offset); false
 // when the group began in an earlier line.
@@ -720,23 +740,15 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         break;
       case kHash: {
         if (STATS) ++st_hash;
-        const u64 key = kmer_hash_key(window >> (64 - 2 * k));
-        const u64 k0 = static_cast<u64>(d1.x) | (static_cast<u64>(d1.y) << 32);
-        const u64 k1 = static_cast<u64>(d2.x) | (static_cast<u64>(d2.y) << 32);
-        const bool use0 = (pre_idx & 1) == 0;  // an odd start skips the pair's first bucket
-        bool settled = true;
-        if (use0 && k0 == key) {
-          found_at = static_cast<Idx>(d1.z);
+        u32 node = 0;
+        const int pr = probe_pair(d1, d2, pre_idx, kmer_hash_key(window >> (64 - 2 * k)), node);
+        const bool settled = pr >= 0;
+        if (pr == 1) {
+          found_at = static_cast<Idx>(node);
           found = true;
-        } else if (use0 && k0 == 0) {
-          notfound = true;
-        } else if (k1 == key) {
-          found_at = static_cast<Idx>(d2.z);
-          found = true;
-        } else if (k1 == 0) {
+        } else if (pr == 0) {
           notfound = true;
         } else {
-          settled = false;
           if (++j < kHashPairs) {
             pre_idx = ((pre_idx | 1) + 1) & hash_mask;
           } else {  // a long probe run: the full search answers
@@ -901,18 +913,23 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
       }
     }
     };
-    // Then further stream steps in the same iteration (kStreamPasses in all),
-    // for the cases v's own line settles: the next k-mer on v's edge c, on the
-    // edge c of v's suffix-group start in the same line, or a miss. Every other
-    // state's code then runs once per several streamed k-mers of most lanes, not
-    // per one; any other outcome (a group that began in an earlier line) leaves
-    // the lane in kStream for the next iteration's full step. One copy of
-    // advance() serves every pass (two copies cost registers, then spills).
+    // Then further steps in the same iteration (kStreamPasses in all) for the
+    // two commonest states: a stream step that v's own line settles (the next
+    // k-mer on v's edge c, on the edge c of v's suffix-group start in the same
+    // line, or a miss), and a k-mer table probe. Every other state's code then
+    // runs once per several k-mers of most lanes, not per one; any other
+    // outcome (a group that began in an earlier line, a long probe run) waits
+    // for the next iteration's full step. Absence checks stay in the full step:
+    // settled there, their runs of skipped windows (up to k - kp + 1) would run
+    // once per pass (C2: 22.0 ms without them in the passes, 27.1 ms with).
+    // One copy of advance() serves every pass (two copies cost registers, then
+    // spills).
 #pragma clang loop unroll(disable)
     for (u32 pass = 0; pass < kStreamPasses; ++pass) {
       advance(res, emitted, decide);
       emitted = false;
       decide = false;
+      bool moved = false;
       if (pass + 1 < kStreamPasses && mode == kStream) {
         const Idx bs = block_of(v);
         const u32 os = static_cast<u32>(v - bs * kBlockNodes);
@@ -937,6 +954,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         if (hit || miss) {  // else the full step of the next iteration decides
           if (STATS) ++st_stream;
           emitted = true;
+          moved = true;
           if (hit) {
             v = cmap_of<Idx>(ix, c) + entry_base(ix, ds, bs, c) + entry_popc(ds, at);
             res = static_cast<u64>(v);
@@ -948,8 +966,39 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
             miss_at = p + k - 1;
           }
         }
+      } else if (pass + 1 < kStreamPasses && mode == kHash) {
+        // a k-mer table probe, as in the full step
+        const uint4 *pair = ix.kmer_hash + (pre_idx & ~1ULL);
+        const uint4 h0 = pair[0], h1 = pair[1];
+        if (STATS) ++st_hash;
+        u32 node = 0;
+        const int pr = probe_pair(h0, h1, pre_idx, kmer_hash_key(window >> (64 - 2 * k)), node);
+        moved = true;
+        if (pr >= 0) {
+          if (STATS) {
+            ++st_full;
+            --st_hash;  // the first pair stands in for the presearch line of a full search
+          }
+          emitted = true;
+          if (pr == 1) {
+            v = static_cast<Idx>(node);
+            prev = true;
+            hint = kHintNone;
+            res = static_cast<u64>(v);  // (KEY: the "look up" prediction is 0)
+          } else {  // where the k-mer fails is unknown: check its last kp-mer next
+            res = kNotFound;
+            prev = false;
+            hint = kHintMiss;
+            miss_at = p + k - 1;
+          }
+        } else if (++j < kHashPairs) {
+          pre_idx = ((pre_idx | 1) + 1) & hash_mask;
+        } else {  // a long probe run: the full search answers
+          pre_idx = window >> (64 - 2 * kp);
+          mode = kPresearch;
+        }
       }
-      if (__ballot(emitted) == 0) break;
+      if (__ballot(moved) == 0) break;
     }
   }
   if constexpr (STATS) {
