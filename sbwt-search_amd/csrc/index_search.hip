@@ -436,12 +436,13 @@ __global__ void k_reset_counter(u64 *c) { *c = 0; }
 // loads. Device-scope atomics on one address serialize at the memory side, so
 // a counter touched once per lane refill stalls every wave; per 64 chunks it
 // only balances the tail.
-// Stream steps per loop iteration at most: the first through the full state
-// machine, the others settled from the lane's node line alone (an edge of the
-// node, or of its suffix-group start in the same line, or a miss). Each pass
-// adds a dependent round trip to the iteration; on C2, 2 / 3 / 4 passes:
-// search kernel 23.3 / 22.8 / 22.9 ms.
-constexpr u32 kStreamPasses = 3;
+// Steps per loop iteration at most: the first through the full state
+// machine, the others stream steps settled from the lane's node line alone (an
+// edge of the node, or of its suffix-group start in the same line, or a miss)
+// and k-mer table probes. Each pass adds a dependent round trip to the
+// iteration; on C2, 2 / 3 / 4 / 5 / 6 passes: search kernel 23.2 / 22.0 /
+// 21.6 / 21.5 / 21.4 ms (C3's index stage 32.4 / 32.6 / 32.3 ms for 2-4).
+constexpr u32 kStreamPasses = 4;
 
 // One probe of the k-mer table: the aligned bucket pair {b0, b1} read for the
 // probe at bucket h (an odd h skips the pair's first bucket). 1: the key's
