@@ -477,14 +477,21 @@ __device__ __forceinline__ bool group_start_in_line(const uint4 dg, u32 o, u32 &
   return true;
 }
 
-// What v's key-walk code (from the walk slot d3 of v's line; o = v's offset in
-// the line) says about the k-mer found on v's edge c: v marked, v with no edge
+// The 32-bit word of line b's walk slots that holds the key-walk code of the
+// node at offset o: its three-node group's byte, four groups per word.
+__device__ __forceinline__ const u32 *walk_word(const IndexView &ix, u64 b, u32 o) {
+  const u32 o3 = o >= kWalkNodesPerSlot ? o - kWalkNodesPerSlot : o;
+  return reinterpret_cast<const u32 *>(line_slot(ix, b, kSlotWalk + (o >= kWalkNodesPerSlot ? 1u : 0u))) +
+         (o3 / 3 >> 2);
+}
+
+// What v's key-walk code (in `word`, read from walk_word of v's line and
+// offset o) says about the k-mer found on v's edge c: v marked, v with no edge
 // (stuck), v's first edge is c (the two k-mers share a key), or unknown.
-__device__ __forceinline__ u32 walk_pred(const uint4 d3, u32 o, bool has_c, u32 c) {
+__device__ __forceinline__ u32 walk_pred(u32 word, u32 o, bool has_c, u32 c) {
   // v's code: the byte of its three-node group, then its base-6 digit
   const u32 o3 = o >= kWalkNodesPerSlot ? o - kWalkNodesPerSlot : o;
   const u32 q3 = o3 / 3;
-  const u32 word = (q3 >> 2) == 0 ? d3.x : ((q3 >> 2) == 1 ? d3.y : ((q3 >> 2) == 2 ? d3.z : d3.w));
   const u32 byte = (word >> (8 * (q3 & 3))) & 255u;
   const u32 dig = o3 - 3 * q3;
   const u32 wcode = (dig == 0 ? byte : (dig == 1 ? byte / 6 : byte / 36)) % 6;
@@ -618,12 +625,11 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     // would add a dependent round trip
     asm volatile("" : "+v"(d1.x), "+v"(d1.y), "+v"(d1.z), "+v"(d1.w), "+v"(d2.x), "+v"(d2.y),
                  "+v"(d2.z), "+v"(d2.w));
-    // KEY: v's key-walk code, from the line the stream step reads anyway
-    uint4 d3 = make_uint4(0, 0, 0, 0);
+    // KEY: the word of v's key-walk code, from the line the stream step reads anyway
+    u32 d3 = 0;
     if constexpr (KEY) {
-      d3 = *(mode == kStream ? line_slot(ix, b1, kSlotWalk + (o1 >= kWalkNodesPerSlot ? 1u : 0u))
-                             : dummy);
-      asm volatile("" : "+v"(d3.x), "+v"(d3.y), "+v"(d3.z), "+v"(d3.w));
+      d3 = *(mode == kStream ? walk_word(ix, b1, o1) : reinterpret_cast<const u32 *>(dummy));
+      asm volatile("" : "+v"(d3));
     }
     if (refill) {
       tbase = dyn0 + __shfl(tbase, 0, 64);
@@ -938,8 +944,8 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         // the group slot of the same line (v's suffix-group start), and, KEY,
         // v's key-walk code, as in the full step
         const uint4 dg = *line_slot(ix, bs, ix.has_groups ? kSlotGroups : c);
-        uint4 dw = make_uint4(0, 0, 0, 0);
-        if constexpr (KEY) dw = *line_slot(ix, bs, kSlotWalk + (os >= kWalkNodesPerSlot ? 1u : 0u));
+        u32 dw = 0;
+        if constexpr (KEY) dw = *walk_word(ix, bs, os);
         const bool has_c = entry_bit(ds, os);
         u32 at = os;  // where edge c is read: v, or its group start in this line
         bool hit = has_c, miss = false;
