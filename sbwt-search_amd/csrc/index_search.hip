@@ -523,19 +523,20 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
   const u64 dyn0 = share * n_waves;
   u64 wnext = wave * share, wend = wnext + share;
   bool dry = false;  // the counter is past the last chunk
-  u64 ci = 0;        // the lane's chunk (valid once want is false in kChunk)
   u32 mode = kChunk;
   bool want = true;  // the lane waits for a chunk
   u64 g = 0, p = 0, buf = 0;           // output slot, char position, buffer start (64-aligned)
   u64 w0 = 0, w1 = 0, w2 = 0, w3 = 0;  // packed chars [buf, buf + 128)
   u64 i0 = 0, i1 = 0;                  // their invalid bits
   u64 window = 0, pre_idx = 0;       // current window; presearch entry to read
-  u64 dead_end = 0;                   // windows starting before dead_end are not found
-  u64 miss_at = 0, tq = 0;            // char that ended the last miss; checked kp-mer start
+  // windows starting before dead_end are not found; in kTail, the checked
+  // kp-mer's start (every window before it is past dead_end then)
+  u64 dead_end = 0;
+  u64 miss_at = 0;                    // char that ended the last miss
   u32 hint = kHintNone;
   u32 left = 0, j = 0, c = 0;
   u32 pred = kPredLookup;  // KEY: what the next found result says about v (set by kStream)
-  Idx l = 0, r = 0, v = 0, gs = 0;
+  Idx l = 0, r = 0, v = 0;  // interval; found node; l: also v's group start in kGsEntry
   bool prev = false;
   __shared__ u64 obuf[8][kSearchThreads];  // staged results, obuf[slot][thread]
   const u32 tid = threadIdx.x;
@@ -559,6 +560,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
   };
   for (;;) {
     // hand the wave's chunks to lanes that wait for one
+    u64 ci = 0;  // the lane's chunk, read in this iteration's kChunk
     const u64 need = __ballot(want);
     if (need != 0) {
       const u64 avail = wend - wnext;
@@ -579,7 +581,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
       tbase = atomicAdd(reinterpret_cast<unsigned long long *>(counter), 64ULL);
     // ---- the two loads of this iteration
     // nodes of the entry loads (0 in other states: indexes stay in range)
-    const Idx n1 = mode == kStream ? v : (mode == kGsEntry ? gs : (mode == kStep ? l : Idx(0)));
+    const Idx n1 = mode == kStream ? v : (mode == kGsEntry || mode == kStep ? l : Idx(0));
     const Idx n2 = mode == kStep ? static_cast<Idx>(r + 1) : Idx(0);
     const Idx b1 = block_of(n1), b2 = block_of(n2);
     const u32 o1 = static_cast<u32>(n1 - b1 * kBlockNodes), o2 = static_cast<u32>(n2 - b2 * kBlockNodes);
@@ -701,7 +703,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
             if (STATS) ++st_fb;
             start_search();
           } else {
-            gs = static_cast<Idx>(static_cast<u64>(b1) * kBlockNodes - d2.x);
+            l = static_cast<Idx>(static_cast<u64>(b1) * kBlockNodes - d2.x);
             if (STATS) ++st_group;
             mode = kGsEntry;
           }
@@ -722,13 +724,14 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
           r = static_cast<Idx>(static_cast<u64>(d1.z) | (static_cast<u64>(d1.w) << 32));
         }
         if (mode == kTail) {
-          // absence check: the kp-mer at tq lies in every window [p, tq]
+          // absence check: the kp-mer at dead_end lies in every window [p, dead_end]
           if (STATS) ++st_tail;
           if (l > r) {
             if (STATS) ++st_tail_hit;
-            dead_end = tq + 1;
+            dead_end += 1;
             notfound = true;
           } else {
+            dead_end = 0;
             start_search();
           }
           break;
@@ -908,7 +911,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         if (absence && hint == kHintNone && !ix.kmer_hash) t = p + k - kp;
         if (absence && hint == kHintMiss && miss_at >= p) t = min(miss_at, p + k - kp);
         if (t > p) {
-          tq = t;
+          dead_end = t;
           const u32 x = static_cast<u32>(t - buf);  // < 96: the kp-mer ends inside this window
           const u64 ta = x < 32 ? w0 : (x < 64 ? w1 : w2), tb = x < 32 ? w1 : (x < 64 ? w2 : w3);
           const u32 tsh = 2 * (x & 31);
