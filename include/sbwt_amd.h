@@ -219,8 +219,11 @@ int sbwt_searcher_last_key_runs_time(sbwt_searcher *searcher, float *ms);
  * a kp-letter substring of the k-mer whose interval is empty is in no k-mer
  * of the index, so every k-mer containing it is not found), stats[9] k-mers
  * settled by such a check, stats[10] further k-mers settled by an earlier
- * check without a lookup; stats[11..15] are zero. Results are unchanged. No counterpart in the reference (its
- * kernels keep no counters). */
+ * check without a lookup, stats[11] further 128-byte lines of k-mer table
+ * probes (a probe run's first line counts as its full search's presearch
+ * line; a bucket pair in a line the run already read adds none);
+ * stats[12..15] are zero. Results are unchanged. No counterpart in the
+ * reference (its kernels keep no counters). */
 int sbwt_searcher_set_stats(sbwt_searcher *searcher, int enable);
 int sbwt_searcher_last_stats(sbwt_searcher *searcher, uint64_t stats[16]);
 

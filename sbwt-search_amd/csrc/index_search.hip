@@ -749,7 +749,6 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         resolve = true;
         break;
       case kHash: {
-        if (STATS) ++st_hash;
         u32 node = 0;
         const int pr = probe_pair(d1, d2, pre_idx, kmer_hash_key(window >> (64 - 2 * k)), node);
         const bool settled = pr >= 0;
@@ -761,16 +760,15 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         } else {
           if (++j < kHashPairs) {
             pre_idx = ((pre_idx | 1) + 1) & hash_mask;
+            if (STATS && (pre_idx & 7) == 0) ++st_hash;  // the next pair opens a new line
           } else {  // a long probe run: the full search answers
+            if (STATS) ++st_hash;  // (its presearch line is the search's own)
             pre_idx = window >> (64 - 2 * kp);
             mode = kPresearch;
           }
         }
         if (settled) {
-          if (STATS) {
-            ++st_full;
-            --st_hash;  // the first pair stands in for the presearch line of a full search
-          }
+          if (STATS) ++st_full;  // the first line stands in for the presearch line of a full search
           if (notfound) {  // where the k-mer fails is unknown: check its last kp-mer next
             hint = kHintMiss;
             miss_at = p + k - 1;
@@ -980,15 +978,11 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         // a k-mer table probe, as in the full step
         const uint4 *pair = ix.kmer_hash + (pre_idx & ~1ULL);
         const uint4 h0 = pair[0], h1 = pair[1];
-        if (STATS) ++st_hash;
         u32 node = 0;
         const int pr = probe_pair(h0, h1, pre_idx, kmer_hash_key(window >> (64 - 2 * k)), node);
         moved = true;
         if (pr >= 0) {
-          if (STATS) {
-            ++st_full;
-            --st_hash;  // the first pair stands in for the presearch line of a full search
-          }
+          if (STATS) ++st_full;  // the first line stands in for the presearch line of a full search
           emitted = true;
           if (pr == 1) {
             v = static_cast<Idx>(node);
@@ -1003,7 +997,9 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
           }
         } else if (++j < kHashPairs) {
           pre_idx = ((pre_idx | 1) + 1) & hash_mask;
+          if (STATS && (pre_idx & 7) == 0) ++st_hash;  // the next pair opens a new line
         } else {  // a long probe run: the full search answers
+          if (STATS) ++st_hash;  // (its presearch line is the search's own)
           pre_idx = window >> (64 - 2 * kp);
           mode = kPresearch;
         }
