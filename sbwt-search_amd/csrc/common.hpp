@@ -78,8 +78,9 @@ struct IndexView {
   u32 hyper_shift;            // lines per hyperblock = 2^hyper_shift (kHyperShift)
   const void *presearch;      // [4^kp] of {left,right} (uint2 or ulonglong2)
   const void *key_dest;       // [n] key-k-mer move destination (u32 or u64), or null
-  const uint4 *kmer_hash;     // [2^kmer_hash_bits] {k-mer code + 1 (u64), node (u32), 0}, or null
-  u32 kmer_hash_bits;
+  const uint4 *kmer_hash;     // [2^kmer_hash_bits groups][2] (four 8-byte entries each), or null
+  u32 kmer_hash_bits;         // G: group index bits
+  u32 kmer_hash_node_bits;    // N: entry bits of node + 1
   u64 c_map[5];
   u64 num_nodes;
   u32 k;
@@ -99,18 +100,27 @@ constexpr u32 kColorMaskMaxColors = 61;  // masks leave bits 61-63 to the sentin
 constexpr u32 kNoKeyRank = 0xFFFFFFFFu;  // rank table entry of a node whose key walk is stuck
 
 // k-mer table: every node whose label is a whole k-mer (k <= 32, fewer than
-// 2^32 nodes), by its 2k-bit code (first letter highest), in open addressing
-// with linear probing from the code's multiplicative hash; a lookup reads
-// buckets in aligned pairs (one 32-byte load each) and gives up after
-// kHashPairs pairs (the full search then answers). A bucket holds code + 1,
-// so 0 marks it empty; the one code whose key would wrap to 0, poly-T at
-// k = 32 (2^64 - 1), is never inserted and always searched in full
-// (kmer_hash_key).
-constexpr u32 kHashPairs = 4;
-// Table key of a k-mer code: code + 1, 0 for the code that has no key.
-__host__ __device__ __forceinline__ u64 kmer_hash_key(u64 code) { return code + 1; }
-__host__ __device__ __forceinline__ u64 kmer_hash_slot(u64 code, u32 bits) {
-  return (code * 0x9E3779B97F4A7C15ULL) >> (64 - bits);
+// 2^32 nodes), in groups of four 8-byte entries (32 bytes, a quarter of a
+// 128-byte line). A k-mer's 2k-bit code times an odd constant modulo 4^k (a
+// bijection) is its mix; the top G bits of the mix pick its home group, and
+// its entry holds the remaining 2k - G bits (the remainder), the distance d
+// (0-3) from the home group to the group holding it, and node + 1 in the top
+// N bits (2^N > nodes, so a used entry is never 0):
+//   entry = (node + 1) << (64 - N) | remainder << 2 | d,  2 + (2k - G) <= 64 - N.
+// An insert takes the first free entry of groups home ... home + 3, in entry
+// order (a k-mer that finds none stays out); a lookup reads those groups in
+// turn, one 32-byte group each, and stops at the k-mer's entry, at an empty
+// entry (the k-mer is absent: it would have been put there or before) or
+// after kHashGroups groups (the full search answers).
+constexpr u32 kHashGroups = 4;
+__host__ __device__ __forceinline__ u64 kmer_hash_mix(u64 code, u32 k) {
+  const u64 m = code * 0x9E3779B97F4A7C15ULL;
+  return k >= 32 ? m : m & ((1ULL << (2 * k)) - 1);
+}
+// The low 64 - N bits of the k-mer's entry in the group d after its home
+// (rem_bits = 2k - G).
+__host__ __device__ __forceinline__ u64 kmer_hash_tag(u64 mix, u32 rem_bits, u32 d) {
+  return ((mix & ((1ULL << rem_bits) - 1)) << 2) | d;
 }
 
 // Words of the search census (sbwt_searcher_last_stats).

@@ -189,9 +189,11 @@ __global__ __launch_bounds__(256) void k_hash_seed(const uint2 *__restrict__ pre
 // letters) the single-node intervals go into the table.
 __global__ __launch_bounds__(256) void k_hash_extend(IndexView ix, const PrefixEnt *__restrict__ in,
                                                      const u64 *n_in, PrefixEnt *out, u64 *n_out,
-                                                     u64 cap, u32 last, uint4 *table, u32 bits) {
+                                                     u64 cap, u32 last, uint4 *table, u32 bits,
+                                                     u32 node_bits) {
   const u64 n = min(*n_in, cap);
-  const u64 mask = (1ULL << bits) - 1;
+  const u64 gmask = (1ULL << bits) - 1;
+  const u32 rem_bits = 2 * ix.k - bits;
   for (u64 b = static_cast<u64>(blockIdx.x) * 256; b < n; b += static_cast<u64>(gridDim.x) * 256) {
     const u64 i = b + threadIdx.x;
     PrefixEnt e{0, 1, 0};
@@ -217,18 +219,21 @@ __global__ __launch_bounds__(256) void k_hash_extend(IndexView ix, const PrefixE
 #pragma unroll
       for (u32 c = 0; c < 4; ++c) {
         if (!((keep >> c) & 1)) continue;
-        // a k-letter interval is one node; its bucket holds code + 1 (0 marks
-        // an empty bucket; the code without a key, poly-T at k = 32, stays out
-        // and is always searched in full)
-        const u64 key = kmer_hash_key(ch[c].code);
-        if (key == 0) continue;
-        u64 h = kmer_hash_slot(ch[c].code, bits);
-        for (u64 probe = 0; probe <= mask; ++probe, h = (h + 1) & mask) {
-          unsigned long long *slot = reinterpret_cast<unsigned long long *>(table + h);
-          const unsigned long long prev = atomicCAS(slot, 0ULL, static_cast<unsigned long long>(key));
-          if (prev == 0 || prev == key) {
-            table[h].z = ch[c].l;
-            break;
+        // a k-letter interval is one node: its entry takes the first free
+        // entry of its home group or of the three groups after it (none: the
+        // k-mer stays out and is searched in full). Entries are never freed,
+        // so an insert done again finds its own entry before any free one.
+        const u64 mix = kmer_hash_mix(ch[c].code, ix.k);
+        const u64 home = mix >> rem_bits;
+        const u64 node_part = (static_cast<u64>(ch[c].l) + 1) << (64 - node_bits);
+        unsigned long long *t64 = reinterpret_cast<unsigned long long *>(table);
+        bool placed = false;
+        for (u32 d = 0; d < kHashGroups && !placed; ++d) {
+          const unsigned long long entry = node_part | kmer_hash_tag(mix, rem_bits, d);
+          unsigned long long *group = t64 + 4 * ((home + d) & gmask);
+          for (u32 q = 0; q < 4 && !placed; ++q) {
+            const unsigned long long prev = atomicCAS(group + q, 0ULL, entry);
+            placed = prev == 0 || prev == entry;
           }
         }
       }
@@ -423,8 +428,9 @@ void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const
 }
 
 // The k-mer table, on request (narrow indexes with k <= 32 and kp < k):
-// 2^bits buckets of 16 bytes with 2^bits >= 2 (n + 1), built when it takes at
-// most half of device memory and leaves an eighth free. The build grows the
+// 2^G groups of four 8-byte entries, G = N (+ 2 at k = 32, so the remainder
+// fits the entry) with 2^N > n: at most a quarter of the entries used; built
+// when it takes at most half of device memory and leaves an eighth free. The build grows the
 // strings of each slice of presearch entries (a range of kp-letter prefixes)
 // one letter per level up to k in two level lists; slices are sized so their
 // lists fit beside the table, and a slice whose level overflows its list is
@@ -433,9 +439,11 @@ void DeviceIndex::build(const u64 *const acgt[4], u64 n_, u32 k_, u32 kp_, const
 bool DeviceIndex::build_kmer_hash() {
   if (kmer_hash) return true;
   if (wide || k > 32 || kp >= k) return false;
-  u32 bits = 1;
-  while ((1ULL << bits) < 2 * (n + 1)) ++bits;
-  const u64 table_bytes = (1ULL << bits) * sizeof(uint4);
+  u32 node_bits = 1;  // entry bits of node + 1 <= n
+  while (node_bits < 32 && (1ULL << node_bits) <= n) ++node_bits;
+  const u32 code_bits = 2 * k;
+  const u32 bits = std::min<u32>(code_bits, node_bits + (code_bits > 62 ? code_bits - 62 : 0));
+  const u64 table_bytes = (1ULL << bits) * 2 * sizeof(uint4);
   size_t free_b = 0, total_b = 0;
   SBWT_HIP(hipMemGetInfo(&free_b, &total_b));
   const u64 reserve = total_b / 8;
@@ -462,6 +470,7 @@ bool DeviceIndex::build_kmer_hash() {
   SBWT_HIP(hipMalloc(&kmer_hash, table_bytes));
   SBWT_HIP(hipMemset(kmer_hash, 0, table_bytes));
   kmer_hash_bits = bits;
+  kmer_hash_node_bits = node_bits;
   const IndexView ix = view();
   std::vector<std::pair<u64, u64>> todo;  // (first entry, entries), done last first
   for (u64 f = count; f > 0;) {
@@ -483,7 +492,7 @@ bool DeviceIndex::build_kmer_hash() {
         const u32 from = (l - kp) & 1, to = from ^ 1;
         k_hash_extend<<<grid, 256>>>(ix, lists[from], counts + (l - kp), lists[to],
                                      counts + (l - kp + 1), cap, l + 1 == k ? 1u : 0u, kmer_hash,
-                                     bits);
+                                     bits, node_bits);
         SBWT_HIP(hipGetLastError());
       }
       SBWT_HIP(hipMemcpy(level_counts.data(), counts, levels * sizeof(u64), hipMemcpyDeviceToHost));
@@ -539,6 +548,7 @@ IndexView DeviceIndex::view() const {
   v.key_dest = key_dest;
   v.kmer_hash = kmer_hash;
   v.kmer_hash_bits = kmer_hash_bits;
+  v.kmer_hash_node_bits = kmer_hash_node_bits;
   v.has_groups = has_groups;
   v.has_marks = has_marks;
   v.color_table = nullptr;  // searchers that output color-table entries set it

@@ -38,7 +38,8 @@ struct DeviceIndex {
   bool has_marks = false;   // key-k-mer marks are in the lines
   void *key_dest = nullptr; // per node: its key-k-mer move destination (u32, or u64 if wide)
   uint4 *kmer_hash = nullptr;  // k-mer code -> node table (narrow indexes), or null
-  u32 kmer_hash_bits = 0;
+  u32 kmer_hash_bits = 0;       // group index bits
+  u32 kmer_hash_node_bits = 0;  // entry bits of node + 1
   // per node: its key's color entry (masks or key ranks), built from a color
   // index by build_color_table (color_search.hip); searchers use it only when
   // asked to output color-table entries

@@ -206,6 +206,28 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_write(const u64 *__restri
   }
 }
 
+// One group of the k-mer table (its four entries in g0, g1) read for the
+// k-mer whose entry there would end in `tag` (its low 64 - node_bits bits):
+// 1 its entry (node set), 0 an empty entry first (the k-mer is absent), -1
+// four entries of other k-mers (read the next group).
+__device__ __forceinline__ int probe_group(const uint4 g0, const uint4 g1, u64 tag, u32 node_bits,
+                                           u32 &node) {
+  const u64 e[4] = {static_cast<u64>(g0.x) | (static_cast<u64>(g0.y) << 32),
+                    static_cast<u64>(g0.z) | (static_cast<u64>(g0.w) << 32),
+                    static_cast<u64>(g1.x) | (static_cast<u64>(g1.y) << 32),
+                    static_cast<u64>(g1.z) | (static_cast<u64>(g1.w) << 32)};
+  const u64 low = ~0ULL >> node_bits;
+#pragma unroll
+  for (u32 q = 0; q < 4; ++q) {
+    if (e[q] == 0) return 0;
+    if ((e[q] & low) == tag) {
+      node = static_cast<u32>(e[q] >> (64 - node_bits)) - 1;
+      return 1;
+    }
+  }
+  return -1;
+}
+
 // ---------------------------------------------------------------------------
 // Search of one k-mer whose 2k bits sit at the top of `window`:
 // src/IndexSearcher/IndexSearcher.cuh:36-71. The presearch table gives the
@@ -217,38 +239,26 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_write(const u64 *__restri
 template <typename Idx, bool KEY>
 __device__ __forceinline__ u64 search_window(const IndexView &ix, u64 window, u32 &steps,
                                              u32 &walked) {
-  const u64 code = window >> (64 - 2 * ix.k);
-  const u64 key = kmer_hash_key(code);
-  if (ix.kmer_hash && key != 0) {  // the k-mer table answers in a pair or two of buckets
-    const u64 mask = (1ULL << ix.kmer_hash_bits) - 1;
-    u64 h = kmer_hash_slot(code, ix.kmer_hash_bits);
-    // `steps` counts the pairs read beyond the first, which stands in for the
+  if (ix.kmer_hash) {  // the k-mer table answers in a group or two
+    const u64 mix = kmer_hash_mix(window >> (64 - 2 * ix.k), ix.k);
+    const u32 rem_bits = 2 * ix.k - ix.kmer_hash_bits;
+    const u64 gmask = (1ULL << ix.kmer_hash_bits) - 1;
+    u64 g = mix >> rem_bits;
+    // `steps` counts the further lines read, the first standing in for the
     // presearch line the caller counts
-    for (u32 pr = 0; pr < kHashPairs; ++pr) {
-      const uint4 *pair = ix.kmer_hash + (h & ~1ULL);
-      const uint4 b0 = pair[0], b1 = pair[1];
-      if (pr) ++steps;
-      const u64 k0 = static_cast<u64>(b0.x) | (static_cast<u64>(b0.y) << 32);
-      const u64 k1 = static_cast<u64>(b1.x) | (static_cast<u64>(b1.y) << 32);
-      const bool use0 = (h & 1) == 0;  // an odd start skips the pair's first bucket
-      u64 node = kNotFound;
-      bool done = false;
-      if (use0 && (k0 == key || k0 == 0)) {
-        node = k0 == key ? static_cast<u64>(b0.z) : kNotFound;
-        done = true;
-      } else if (k1 == key || k1 == 0) {
-        node = k1 == key ? static_cast<u64>(b1.z) : kNotFound;
-        done = true;
+    for (u32 d = 0; d < kHashGroups; ++d, g = (g + 1) & gmask) {
+      const uint4 *group = ix.kmer_hash + 2 * g;
+      if (d && (g & 3) == 0) ++steps;  // the group opens a new line
+      u32 node = 0;
+      const int pr = probe_group(group[0], group[1], kmer_hash_tag(mix, rem_bits, d),
+                                 ix.kmer_hash_node_bits, node);
+      if (pr == 1) {
+        if constexpr (KEY) return key_result<Idx>(ix, static_cast<Idx>(node), walked);
+        return static_cast<u64>(node);
       }
-      if (done) {
-        if constexpr (KEY) {
-          if (node != kNotFound) return key_result<Idx>(ix, static_cast<Idx>(node), walked);
-        }
-        return node;
-      }
-      h = ((h | 1) + 1) & mask;
+      if (pr == 0) return kNotFound;
     }
-    ++steps;  // all pairs were extra: the presearch line follows
+    ++steps;  // the lines read were all extra: the presearch line follows
   }
   const u64 pre = window >> (64 - 2 * ix.kp);
   Idx l, r;
@@ -444,26 +454,6 @@ __global__ void k_reset_counter(u64 *c) { *c = 0; }
 // 21.6 / 21.5 / 21.4 ms (C3's index stage 32.4 / 32.6 / 32.3 ms for 2-4).
 constexpr u32 kStreamPasses = 4;
 
-// One probe of the k-mer table: the aligned bucket pair {b0, b1} read for the
-// probe at bucket h (an odd h skips the pair's first bucket). 1: the key's
-// bucket (node set), 0: an empty bucket first (the k-mer is absent), -1: both
-// buckets hold other keys (probe the next pair).
-__device__ __forceinline__ int probe_pair(const uint4 b0, const uint4 b1, u64 h, u64 key, u32 &node) {
-  const u64 k0 = static_cast<u64>(b0.x) | (static_cast<u64>(b0.y) << 32);
-  const u64 k1 = static_cast<u64>(b1.x) | (static_cast<u64>(b1.y) << 32);
-  const bool use0 = (h & 1) == 0;
-  if (use0 && k0 == key) {
-    node = b0.z;
-    return 1;
-  }
-  if (use0 && k0 == 0) return 0;
-  if (k1 == key) {
-    node = b1.z;
-    return 1;
-  }
-  return k1 == 0 ? 0 : -1;
-}
-
 // v's suffix-group start when its line holds it: the last group-start bit at
 // or before offset o in the line's group slot dg (pos = its offset); false
 // when the group began in an earlier line.
@@ -545,12 +535,16 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
   u32 st_full = 0, st_steps = 0, st_walk = 0, st_inv = 0, st_stream = 0, st_extra = 0, st_fb = 0;
   u32 st_group = 0, st_tail = 0, st_tail_hit = 0, st_skip = 0, st_hash = 0;
   const u64 hash_mask = ix.kmer_hash ? (1ULL << ix.kmer_hash_bits) - 1 : 0;
-  // a fresh search of the current window: the k-mer table when there is one
-  // (pre_idx = bucket, j = pairs read), else the presearch entry and steps
+  const u32 rem_bits = 2 * k - ix.kmer_hash_bits;
+  // in kHash, pre_idx = the k-mer's mix and j = the groups read before this
+  // one: the group to read, and the low bits of the k-mer's entry there
+  auto hash_group = [&]() { return ((pre_idx >> rem_bits) + j) & hash_mask; };
+  auto hash_tag = [&]() { return kmer_hash_tag(pre_idx, rem_bits, j); };
+  // a fresh search of the current window: the k-mer table when there is one,
+  // else the presearch entry and steps
   auto start_search = [&]() {
-    // (the code without a table key, poly-T at k = 32, is searched in full)
-    if (ix.kmer_hash && kmer_hash_key(window >> (64 - 2 * k)) != 0) {
-      pre_idx = kmer_hash_slot(window >> (64 - 2 * k), ix.kmer_hash_bits);
+    if (ix.kmer_hash) {
+      pre_idx = kmer_hash_mix(window >> (64 - 2 * k), k);
       j = 0;
       mode = kHash;
     } else {
@@ -614,8 +608,8 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         a1 = e1;
         a2 = line_slot(ix, b2, c);
         break;
-      case kHash:  // an aligned pair of buckets
-        a1 = ix.kmer_hash + (pre_idx & ~1ULL);
+      case kHash:  // a group of four entries
+        a1 = ix.kmer_hash + 2 * hash_group();
         a2 = a1 + 1;
         break;
       default:
@@ -750,7 +744,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         break;
       case kHash: {
         u32 node = 0;
-        const int pr = probe_pair(d1, d2, pre_idx, kmer_hash_key(window >> (64 - 2 * k)), node);
+        const int pr = probe_group(d1, d2, hash_tag(), ix.kmer_hash_node_bits, node);
         const bool settled = pr >= 0;
         if (pr == 1) {
           found_at = static_cast<Idx>(node);
@@ -758,9 +752,8 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         } else if (pr == 0) {
           notfound = true;
         } else {
-          if (++j < kHashPairs) {
-            pre_idx = ((pre_idx | 1) + 1) & hash_mask;
-            if (STATS && (pre_idx & 7) == 0) ++st_hash;  // the next pair opens a new line
+          if (++j < kHashGroups) {
+            if (STATS && (hash_group() & 3) == 0) ++st_hash;  // the next group opens a new line
           } else {  // a long probe run: the full search answers
             if (STATS) ++st_hash;  // (its presearch line is the search's own)
             pre_idx = window >> (64 - 2 * kp);
@@ -976,10 +969,10 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         }
       } else if (pass + 1 < kStreamPasses && mode == kHash) {
         // a k-mer table probe, as in the full step
-        const uint4 *pair = ix.kmer_hash + (pre_idx & ~1ULL);
-        const uint4 h0 = pair[0], h1 = pair[1];
+        const uint4 *group = ix.kmer_hash + 2 * hash_group();
+        const uint4 h0 = group[0], h1 = group[1];
         u32 node = 0;
-        const int pr = probe_pair(h0, h1, pre_idx, kmer_hash_key(window >> (64 - 2 * k)), node);
+        const int pr = probe_group(h0, h1, hash_tag(), ix.kmer_hash_node_bits, node);
         moved = true;
         if (pr >= 0) {
           if (STATS) ++st_full;  // the first line stands in for the presearch line of a full search
@@ -995,9 +988,8 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
             hint = kHintMiss;
             miss_at = p + k - 1;
           }
-        } else if (++j < kHashPairs) {
-          pre_idx = ((pre_idx | 1) + 1) & hash_mask;
-          if (STATS && (pre_idx & 7) == 0) ++st_hash;  // the next pair opens a new line
+        } else if (++j < kHashGroups) {
+          if (STATS && (hash_group() & 3) == 0) ++st_hash;  // the next group opens a new line
         } else {  // a long probe run: the full search answers
           if (STATS) ++st_hash;  // (its presearch line is the search's own)
           pre_idx = window >> (64 - 2 * kp);
