@@ -102,15 +102,16 @@ int sbwt_suffix_groups_consistent(const uint64_t *const acgt[4], const uint64_t 
 
 /* Builds the index's k-mer table: every node whose label is a whole k-mer,
  * by its 2k-bit code, so that a search that cannot step from the previous
- * k-mer (a read start, a k-mer after a miss) reads about one 32-byte bucket
- * pair instead of the presearch entry and k - presearch_letters SBWT steps.
- * Results do not change. One-time cost (about 0.1 s for 10^8 nodes) and
- * up to 32 bytes per node of device memory, so it pays for an index that
- * serves many batches; the command line does not build it. *built (nullable)
- * receives 0 when the index cannot have one (nodes >= 2^32, k > 32, or the
- * table would take more than a quarter of device memory). No counterpart in
- * the reference (its searches always start from the presearch table,
- * src/IndexSearcher/IndexSearcher.cuh:36-55). */
+ * k-mer (a read start, a k-mer after a miss) reads about one 32-byte group of
+ * entries instead of the presearch entry and k - presearch_letters SBWT
+ * steps. Results do not change. One-time cost (about 0.1 s for 10^8 nodes)
+ * and 32 to 64 bytes per node of device memory (4x that at k = 32), so it
+ * pays for an index that serves many batches; the command line does not
+ * build it. *built (nullable) receives 0 when the index cannot have one
+ * (nodes >= 2^32, k > 32, presearch_letters >= k, or the table would take
+ * more than half of device memory or leave less than an eighth free). No
+ * counterpart in the reference (its searches always start from the
+ * presearch table, src/IndexSearcher/IndexSearcher.cuh:36-55). */
 int sbwt_index_build_kmer_table(sbwt_index *index, int *built);
 
 void sbwt_index_destroy(sbwt_index *index);
@@ -221,7 +222,7 @@ int sbwt_searcher_last_key_runs_time(sbwt_searcher *searcher, float *ms);
  * settled by such a check, stats[10] further k-mers settled by an earlier
  * check without a lookup, stats[11] further 128-byte lines of k-mer table
  * probes (a probe run's first line counts as its full search's presearch
- * line; a bucket pair in a line the run already read adds none);
+ * line; a group in a line the run already read adds none);
  * stats[12..15] are zero. Results are unchanged. No counterpart in the
  * reference (its kernels keep no counters). */
 int sbwt_searcher_set_stats(sbwt_searcher *searcher, int enable);
