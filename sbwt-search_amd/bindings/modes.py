@@ -76,8 +76,12 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect
         off = torch.empty(max_reads + 1, dtype=torch.int64, device=dev)
         cap = max_reads * nc
         ids = torch.empty(cap, dtype=torch.int32, device=dev)
+        # page-locked landing buffers for the copies back (a pageable copy is
+        # staged through the runtime's bounce buffer in pieces)
+        h_off = torch.empty(max_reads + 1, dtype=torch.int64, pin_memory=True)
+        h_ids = torch.empty(cap, dtype=torch.int32, pin_memory=True)
         return {"stream": stream, "s": s, "cs": cs, "res": res, "off": off, "ids": ids,
-                "cap": cap}
+                "cap": cap, "h_off": h_off, "h_ids": h_ids}
 
     def run_batch(lane, b):
         c0, n, d_ends = rel[b]
@@ -88,11 +92,15 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect
                                               lane["s"].kmer_ends_device(), r1 - r0, params,
                                               lane["off"].data_ptr(), lane["ids"].data_ptr(),
                                               lane["cap"])
-        # the batch's output back on the host (the same copy in every form)
+        # the batch's output back on the host (the same copies in every form),
+        # then out of the lane's landing buffers before its next batch
+        nr = r1 - r0
         with torch.cuda.stream(lane["stream"]):
-            off = lane["off"][:r1 - r0 + 1].to("cpu", non_blocking=False)
-            ids = lane["ids"][:n_ids].to("cpu", non_blocking=False)
-        return off.numpy().view(np.uint64), ids.numpy().view(np.uint32)
+            lane["h_off"][:nr + 1].copy_(lane["off"][:nr + 1], non_blocking=True)
+            lane["h_ids"][:n_ids].copy_(lane["ids"][:n_ids], non_blocking=True)
+        lane["stream"].synchronize()
+        return (lane["h_off"][:nr + 1].numpy().view(np.uint64).copy(),
+                lane["h_ids"][:n_ids].numpy().view(np.uint32).copy())
 
     import threading
 

@@ -752,29 +752,42 @@ __global__ __launch_bounds__(256) void k_color_quirk2(ColorView cv, const u64 *_
   }
 }
 
+// Lanes per read of k_color_compact: the words of a read's hit bitmap side
+// by side, a power of two up to a wave (several reads per wave when nc <= 2048).
+inline u32 compact_lanes(u32 nc) {
+  const u32 words = (nc + 63) / 64;
+  u32 seg = 1;
+  while (seg < words && seg < kWave) seg <<= 1;
+  return seg;
+}
+
 // Expand per-read hit bitmaps into sorted u32 color ids at CSR offsets; ids
-// past `capacity` are not written (offsets still give the full count).
+// past `capacity` are not written (offsets still give the full count). Each
+// read takes `seg` lanes (compact_lanes), one bitmap word per lane and pass.
 __global__ __launch_bounds__(256) void k_color_compact(const u64 *__restrict__ bitmap,
                                                         const u64 *__restrict__ incl, u64 n_reads,
-                                                        u32 nc, u32 *__restrict__ ids, u64 capacity,
-                                                        u64 *__restrict__ offsets) {
+                                                        u32 nc, u32 seg, u32 *__restrict__ ids,
+                                                        u64 capacity, u64 *__restrict__ offsets) {
   const u32 lane = threadIdx.x % kWave, wave = threadIdx.x / kWave, waves = blockDim.x / kWave;
   const u32 words = (nc + 63) / 64;
-  for (u64 r = static_cast<u64>(blockIdx.x) * waves + wave; r < n_reads;
-       r += static_cast<u64>(gridDim.x) * waves) {
-    u64 pos = r ? incl[r - 1] : 0;
-    if (lane == 0) {
+  const u32 per_wave = kWave / seg, sub = lane / seg, sl = lane % seg;
+  for (u64 r0 = (static_cast<u64>(blockIdx.x) * waves + wave) * per_wave; r0 < n_reads;
+       r0 += static_cast<u64>(gridDim.x) * waves * per_wave) {
+    const u64 r = r0 + sub;
+    const bool live = r < n_reads;
+    u64 pos = live && r ? incl[r - 1] : 0;
+    if (live && sl == 0) {
       offsets[r + 1] = incl[r];
       if (r == 0) offsets[0] = 0;
     }
-    for (u32 w0 = 0; w0 < words; w0 += kWave) {
-      const u32 w = w0 + lane;
-      u64 x = w < words ? bitmap[r * words + w] : 0;
+    for (u32 w0 = 0; w0 < words; w0 += seg) {
+      const u32 w = w0 + sl;
+      u64 x = live && w < words ? bitmap[r * words + w] : 0;
       const u32 pc = static_cast<u32>(__popcll(x));
-      u32 pre = pc;  // inclusive wave scan
-      for (u32 o = 1; o < kWave; o <<= 1) {
-        const u32 y = __shfl_up(pre, o, kWave);
-        if (lane >= o) pre += y;
+      u32 pre = pc;  // inclusive scan over the read's lanes
+      for (u32 o = 1; o < seg; o <<= 1) {
+        const u32 y = __shfl_up(pre, o, seg);
+        if (sl >= o) pre += y;
       }
       u64 at = pos + pre - pc;
       while (x) {
@@ -782,7 +795,7 @@ __global__ __launch_bounds__(256) void k_color_compact(const u64 *__restrict__ b
         ++at;
         x &= x - 1;
       }
-      pos += __shfl(pre, kWave - 1, kWave);
+      pos += __shfl(pre, sub * seg + seg - 1, kWave);
     }
   }
 }
@@ -1150,8 +1163,9 @@ u64 ColorSearcher::pseudoalign_device(const u64 *d_results, const u64 *d_kmer_en
   if (total > ids_capacity)
     throw std::runtime_error("color id output needs " + std::to_string(total) + " entries, capacity " +
                              std::to_string(ids_capacity));
-  k_color_compact<<<grid_for(n_reads, 4), 4 * kWave, 0, stream_>>>(d_bitmap_, d_incl, n_reads, nc,
-                                                                    d_ids, ids_capacity, d_offsets);
+  const u32 seg = compact_lanes(nc), per_wave = kWave / seg;
+  k_color_compact<<<grid_for((n_reads + per_wave - 1) / per_wave, 4), 4 * kWave, 0, stream_>>>(
+      d_bitmap_, d_incl, n_reads, nc, seg, d_ids, ids_capacity, d_offsets);
   SBWT_HIP(hipGetLastError());
   return total;
 }
@@ -1165,8 +1179,9 @@ void ColorSearcher::pseudoalign_device_async(const u64 *d_results, const u64 *d_
   launch_threshold(d_results, d_kmer_ends, n_reads, p, d_stats);
   u64 *d_incl = d_nout_ + max_reads_;
   scan_inclusive_u64(d_nout_, n_reads, d_incl, d_scan_tmp_, stream_);
-  k_color_compact<<<grid_for(n_reads, 4), 4 * kWave, 0, stream_>>>(d_bitmap_, d_incl, n_reads, nc,
-                                                                    d_ids, ids_capacity, d_offsets);
+  const u32 seg = compact_lanes(nc), per_wave = kWave / seg;
+  k_color_compact<<<grid_for((n_reads + per_wave - 1) / per_wave, 4), 4 * kWave, 0, stream_>>>(
+      d_bitmap_, d_incl, n_reads, nc, seg, d_ids, ids_capacity, d_offsets);
   SBWT_HIP(hipGetLastError());
 }
 
