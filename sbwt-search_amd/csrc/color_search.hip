@@ -67,6 +67,11 @@ constexpr u32 kGroup = 4;
 // before their counter adds), and dense 64-bit words per lane in flight.
 constexpr u32 kPerLane = 8;
 constexpr u32 kDenseUnroll = 2;
+// Sparse ids of at most this width are walked in chunks of kPerLane ids of
+// one set, each chunk one 16-byte load (8 * 12 bits + a 31-bit offset fit).
+constexpr u32 kChunkWidth = 12;
+static_assert(kPerLane * kChunkWidth + 31 <= 128 && kPerLane * kChunkWidth <= 96, "a chunk fits one load");
+static_assert(kGroup * kWave < 0x10000, "a run length fits 16 bits of the chunk table");
 // The wave's LDS table: 64 entries of 16 bytes.
 constexpr u32 kTabWords = kWave * 4;
 
@@ -142,12 +147,15 @@ __device__ __forceinline__ void touch(u32 &touched, u32 c) { touched |= 1u << ((
 //    lane in flight; then the wave steps through the words, each broadcast
 //    from its lane (readlane), and lane i adds the run to color 64 u + i when
 //    its bit is set: no bit loop and no two lanes on one counter;
-//  - sparse sets: the ids of all sets, concatenated, are cut into ranges of
-//    kPerLane consecutive ids, one range per lane; a lane finds the set of its
-//    first id once (a search over the lanes' unit counts) and steps on through
-//    the table of set ranges. All loads of a range are issued before any id
-//    is used (one memory round trip per range), then the ids are extracted
-//    and added.
+//  - sparse sets of ids up to kChunkWidth bits: each set is cut into chunks
+//    of kPerLane ids, one chunk per lane (a search over the lanes' chunk
+//    counts finds its set); the chunk is one 16-byte load, shifted to bit 0,
+//    and id k is the width bits at k * width;
+//  - wider sparse ids: the ids of all sets, concatenated, are cut into ranges
+//    of kPerLane consecutive ids, one range per lane; a lane finds the set of
+//    its first id once and steps on through the table of set ranges. All
+//    loads of a range are issued before any id is used (one memory round trip
+//    per range), then the ids are extracted and added.
 __device__ __forceinline__ void wave_payload(const ColorView &cv, const SetRef &set, u32 units,
                                              u32 run, u32 lane, u32 *cnt, uint4 *tab,
                                              u32 &touched) {
@@ -200,13 +208,51 @@ __device__ __forceinline__ void wave_payload(const ColorView &cv, const SetRef &
     }
   }
   if (__ballot(s_units != 0) == 0) return;
+  const u32 width = cv.sparse_width;
+  if (width <= kChunkWidth && __ballot(s_units > 0xFFFFu) == 0) {
+    // chunks of up to kPerLane ids of one set, one chunk per lane: the
+    // chunk's ids lie in 16 bytes (kPerLane * width + 31 <= 128 bits), one
+    // load; shifted to bit 0, id k sits at bit k * width of 96 bits
+    const u32 c_units = (s_units + kPerLane - 1) / kPerLane;
+    const u32 c_incl = wave_prefix(c_units);
+    const u32 c_total = __builtin_amdgcn_readlane(c_incl, kWave - 1);
+    tab[lane] = make_uint4(c_incl, static_cast<u32>(set.start), static_cast<u32>(set.start >> 32),
+                           run | (s_units << 16));
+    wave_lds_fence();
+    const u32 *sparse32 = reinterpret_cast<const u32 *>(cv.sparse_arrays);
+    const u32 wmask = (1u << width) - 1;
+    for (u32 b0 = 0; b0 < c_total; b0 += kWave) {
+      u32 u = 0;
+      const u32 o = unit_owner(c_incl, c_units, min(b0 + lane, c_total - 1), u);
+      const uint4 e = tab[o];
+      const u32 r = e.w & 0xFFFFu;
+      const u32 n = b0 + lane < c_total ? min(kPerLane, (e.w >> 16) - kPerLane * u) : 0u;
+      const u64 bit = ((static_cast<u64>(e.y) | (static_cast<u64>(e.z) << 32)) + kPerLane * u) * width;
+      const uint4 w = *reinterpret_cast<const uint4 *>(sparse32 + (bit >> 5));
+      const u32 sh = static_cast<u32>(bit) & 31u;
+      const u32 a0 = __builtin_amdgcn_alignbit(w.y, w.x, sh), a1 = __builtin_amdgcn_alignbit(w.z, w.y, sh),
+                a2 = __builtin_amdgcn_alignbit(w.w, w.z, sh);
+      const u64 lo = static_cast<u64>(a0) | (static_cast<u64>(a1) << 32);  // bits 0-63
+      const u64 hi = static_cast<u64>(a1) | (static_cast<u64>(a2) << 32);  // bits 32-95
+#pragma unroll
+      for (u32 k = 0; k < kPerLane; ++k) {
+        const u32 pos = k * width;  // wave-uniform
+        const u32 id = static_cast<u32>(pos + width <= 64 ? lo >> pos : hi >> (pos - 32)) & wmask;
+        if (k < n && id < nc) {
+          cnt_add(cnt + id, r);
+          touch(touched, id);
+        }
+      }
+    }
+    wave_lds_fence();  // the table is written again by the next chunk
+    return;
+  }
   const u32 s_incl = wave_prefix(s_units);
   const u32 s_total = __builtin_amdgcn_readlane(s_incl, kWave - 1);
   // the set ranges: {end of the set's ids in the concatenation, start lo,
   // start hi, run}, lane order
   tab[lane] = make_uint4(s_incl, static_cast<u32>(set.start), static_cast<u32>(set.start >> 32), run);
   wave_lds_fence();
-  const u32 width = cv.sparse_width;
   for (u32 b0 = 0; b0 < s_total; b0 += kPerLane * kWave) {
     const u32 t = b0 + lane * kPerLane;  // this lane's first id
     const u32 t_first = min(t, s_total - 1);
@@ -976,7 +1022,7 @@ void DeviceColors::build(const HostColors &c) {
   dense.push_back(0);
   dense_arrays = dev_upload(dense.data(), dense.size());
   std::vector<u64> sparse = c.sparse_arrays.words;
-  sparse.push_back(0);
+  sparse.insert(sparse.end(), 2, 0);  // a chunk's 16-byte load may reach 12 bytes past its last id
   sparse_arrays = dev_upload(sparse.data(), sparse.size());
   device_bytes = key_marks.n_blocks * 16 + csi.size() * 8 + table.size() * 16 + (num_keys + 1) * 16 +
                  dense.size() * 8 + sparse.size() * 8;
