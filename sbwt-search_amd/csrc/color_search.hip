@@ -928,9 +928,18 @@ void with_counters_input(u64 nc, int input, F f) {
 // among the key k-mers (ColorSearcher.cuh:99-105); masks (at most
 // kColorMaskMaxColors colors): the key's color set as a bit mask.
 template <typename Idx, bool MASKS>
-__global__ __launch_bounds__(256) void k_color_table(IndexView ix, ColorView cv, u64 n, void *out) {
+__global__ __launch_bounds__(256) void k_color_table(IndexView ix, ColorView cv, u64 n, void *out,
+                                                      u32 *marks_differ) {
   for (u64 v = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
        v += static_cast<u64>(gridDim.x) * blockDim.x) {
+    if constexpr (!MASKS) {
+      // key ranks: the lines' own marks rank the same way when they are the
+      // color index's marks, bit for bit
+      const ulonglong2 me = reinterpret_cast<const ulonglong2 *>(cv.key_marks)[v >> 6];
+      const Idx b = block_of(static_cast<Idx>(v));
+      const bool line_mark = entry_bit(*line_slot(ix, b, kSlotMarks), static_cast<u32>(v - b * kBlockNodes));
+      if (line_mark != (((me.y >> (v & 63)) & 1) != 0)) *marks_differ = 1;
+    }
     u32 walked = 0;
     const u64 key = key_move<Idx>(ix, static_cast<Idx>(v), walked);
     const bool stuck = key == kWalkStuck;
@@ -1091,22 +1100,32 @@ void build_color_table(DeviceIndex &index, const DeviceColors &colors) {
     SBWT_HIP(hipFree(index.color_table));
     index.color_table = nullptr;
   }
+  index.color_ranks_in_lines = false;
   const bool masks = colors.num_colors <= kColorMaskMaxColors;
   if (!masks && colors.num_keys >= kNoKeyRank) throw std::runtime_error("too many key k-mers for a rank table");
   const u64 bytes = index.n * (masks ? 8 : 4) + 16;
   SBWT_HIP(hipMalloc(&index.color_table, bytes));
+  u32 *d_differ = nullptr;
+  SBWT_HIP(hipMalloc(&d_differ, sizeof(u32)));
+  SBWT_HIP(hipMemset(d_differ, 0, sizeof(u32)));
   const IndexView ix = index.view();
   const u32 grid = grid_for(index.n, 256);
+  void *out = index.color_table;
   if (index.wide) {
-    if (masks) k_color_table<u64, true><<<grid, 256>>>(ix, colors.view(), index.n, index.color_table);
-    else k_color_table<u64, false><<<grid, 256>>>(ix, colors.view(), index.n, index.color_table);
+    if (masks) k_color_table<u64, true><<<grid, 256>>>(ix, colors.view(), index.n, out, d_differ);
+    else k_color_table<u64, false><<<grid, 256>>>(ix, colors.view(), index.n, out, d_differ);
   } else {
-    if (masks) k_color_table<u32, true><<<grid, 256>>>(ix, colors.view(), index.n, index.color_table);
-    else k_color_table<u32, false><<<grid, 256>>>(ix, colors.view(), index.n, index.color_table);
+    if (masks) k_color_table<u32, true><<<grid, 256>>>(ix, colors.view(), index.n, out, d_differ);
+    else k_color_table<u32, false><<<grid, 256>>>(ix, colors.view(), index.n, out, d_differ);
   }
   SBWT_HIP(hipGetLastError());
-  SBWT_HIP(hipDeviceSynchronize());
+  u32 differ = 1;
+  SBWT_HIP(hipMemcpy(&differ, d_differ, sizeof(u32), hipMemcpyDeviceToHost));
+  SBWT_HIP(hipFree(d_differ));
   index.color_table_kind = masks ? kColorTableMasks : kColorTableRanks;
+  // the streaming search then hands a marked node's rank on in the result
+  // after it (kRankShift), so the key pass need not read its entry
+  index.color_ranks_in_lines = !masks && !index.wide && differ == 0 && colors.num_keys <= kRankMax;
   index.color_table_source = &colors;
   index.device_bytes += bytes;
 }

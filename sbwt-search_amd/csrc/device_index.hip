@@ -533,6 +533,7 @@ void DeviceIndex::release() {
   key_dest = nullptr;
   if (color_table) (void)hipFree(color_table);
   color_table = nullptr;
+  color_ranks_in_lines = false;
   color_table_kind = kColorTableNone;
   blocks = nullptr;
   hyper_base = nullptr;

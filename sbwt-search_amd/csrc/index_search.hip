@@ -525,7 +525,9 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
   u64 miss_at = 0;                    // char that ended the last miss
   u32 hint = kHintNone;
   u32 left = 0, j = 0, c = 0;
-  u32 pred = kPredLookup;  // KEY: what the next found result says about v (set by kStream)
+  // KEY: what the next found result says about v (set by kStream): kPred* in
+  // bits 0-1, 1 + v's rank among the marked nodes above them (or 0)
+  u32 pred = kPredLookup;
   Idx l = 0, r = 0, v = 0;  // interval; found node; l: also v's group start in kGsEntry
   bool prev = false;
   __shared__ u64 obuf[8][kSearchThreads];  // staged results, obuf[slot][thread]
@@ -627,6 +629,14 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
       d3 = *(mode == kStream ? walk_word(ix, b1, o1) : reinterpret_cast<const u32 *>(dummy));
       asm volatile("" : "+v"(d3));
     }
+    // KEY with line ranks: v's rank among the marked nodes, from the same line
+    u32 mrank = 0;
+    if constexpr (KEY) {
+      if (ix.line_ranks) {
+        const uint4 dm = *(mode == kStream ? line_slot(ix, b1, kSlotMarks) : dummy);
+        mrank = dm.x + entry_popc(dm, o1);
+      }
+    }
     if (refill) {
       tbase = dyn0 + __shfl(tbase, 0, 64);
       if (tbase >= n_chunks) {
@@ -674,7 +684,12 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
         break;
       case kStream:
         if (STATS) ++st_stream;
-        if constexpr (KEY) pred = walk_pred(d3, o1, bit1, c);
+        if constexpr (KEY) {
+          // what v's line says about v (kPred*), and, when v is marked and the
+          // color table ranks keys like the lines' marks, 1 + v's rank above it
+          pred = walk_pred(d3, o1, bit1, c);
+          if (pred == kPredMarked && ix.line_ranks && mrank <= kRankMax) pred |= (mrank + 1) << 2;
+        }
         if (bit1) {
           found = true;
         } else if (!ix.has_groups) {
@@ -804,7 +819,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
       if constexpr (KEY) {
         // the node and what it says about the k-mer before it; k_key_runs
         // replaces it with the key node
-        res |= static_cast<u64>(pred) << kPredShift;
+        res |= (static_cast<u64>(pred & 3u) << kPredShift) | (static_cast<u64>(pred >> 2) << kRankShift);
         pred = kPredLookup;
       }
       emitted = true;
@@ -959,7 +974,15 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
           if (hit) {
             v = cmap_of<Idx>(ix, c) + entry_base(ix, ds, bs, c) + entry_popc(ds, at);
             res = static_cast<u64>(v);
-            if constexpr (KEY) res |= static_cast<u64>(walk_pred(dw, os, has_c, c)) << kPredShift;
+            if constexpr (KEY) {
+              const u32 pr = walk_pred(dw, os, has_c, c);
+              res |= static_cast<u64>(pr) << kPredShift;
+              if (pr == kPredMarked && ix.line_ranks) {  // v's rank, from the same line
+                const uint4 dm = *line_slot(ix, bs, kSlotMarks);
+                const u32 mr = dm.x + entry_popc(dm, os);
+                if (mr <= kRankMax) res |= static_cast<u64>(mr + 1) << kRankShift;
+              }
+            }
           } else {  // the k-mer's last char made the miss: its predecessor was found
             res = kNotFound;
             prev = false;
@@ -1039,13 +1062,16 @@ __global__ __launch_bounds__(kSearchThreads) void k_key_runs(IndexView ix,
   const u32 lane = threadIdx.x & 63;
   const u64 n_waves = static_cast<u64>(gridDim.x) * (kSearchThreads / 64);
   const u64 wave = static_cast<u64>(blockIdx.x) * (kSearchThreads / 64) + threadIdx.x / 64;
-  constexpr u64 kNodeMask = (1ULL << kPredShift) - 1;
+  // narrow indexes: bits 32-60 may hold a rank (kRankShift)
+  constexpr u64 kNodeMask = sizeof(Idx) == 4 ? 0xFFFFFFFFULL : (1ULL << kPredShift) - 1;
   u32 st_walk = 0;
   for (u64 ci = wave; ci < n_chunks; ci += n_waves) {
     const ulonglong2 item = chunks[ci];
     const u64 g0 = item.x;
     const u32 len = static_cast<u32>(item.y >> 48);
-    u32 carry_flag = kPredLookup;  // flag of the k-mer right of the block
+    // flag of the k-mer right of the block: its kPred* in bits 0-1, above
+    // them 1 + the rank it carries (kRankShift) or 0
+    u32 carry_flag = kPredLookup;
     u64 carry_key = 0;             // and its key
     // the block ending at hi_: lane 63 holds k-mer hi_ - 1, lane q k-mer
     // hi_ - 64 + q; none below hi_ - 64 or 0
@@ -1070,19 +1096,24 @@ __global__ __launch_bounds__(kSearchThreads) void k_key_runs(IndexView ix,
 #pragma unroll
       for (u32 b = 0; b < kRunBlocks; ++b) {
         const bool found = (rv[b] >> 63) == 0;
-        flag[b] = found ? static_cast<u32>(rv[b] >> kPredShift) & 3u : kPredLookup;
+        flag[b] = found ? (static_cast<u32>(rv[b] >> kPredShift) & 3u) |
+                              (sizeof(Idx) == 4 ? (static_cast<u32>(rv[b] >> kRankShift) & ((1u << 29) - 1)) << 2
+                                                : 0u)
+                        : kPredLookup;
         const u64 node = rv[b] & kNodeMask;
         u32 succ = static_cast<u32>(__shfl_down(static_cast<int>(flag[b]), 1, 64));
         if (lane == 63) succ = right_flag;
-        brk[b] = found && succ != kPredSameKey;
+        brk[b] = found && (succ & 3u) != kPredSameKey;
         base[b] = 0;
         if (brk[b]) {
-          if (succ == kPredStuck) {
+          if ((succ & 3u) == kPredStuck) {
             base[b] = kWalkStuck;
+          } else if (ix.color_table && (succ >> 2) != 0) {  // a marked node's rank, from the search
+            base[b] = (succ >> 2) - 1;
           } else if (ix.color_table) {  // the key's color entry (for a marked node, its own)
             base[b] = color_table_entry<Idx>(ix, static_cast<Idx>(node));
             if (STATS) ++st_walk;
-          } else if (succ == kPredMarked) {
+          } else if ((succ & 3u) == kPredMarked) {
             base[b] = node;
           } else {
             u32 w = 0;
@@ -1260,6 +1291,7 @@ IndexView Searcher::reads_view() const {
   if (color_output_) {
     v.color_table = index_->color_table;
     v.color_table_kind = index_->color_table_kind;
+    v.line_ranks = index_->color_ranks_in_lines ? 1u : 0u;
   }
   return v;
 }

@@ -13,7 +13,7 @@ import pytest
 import oracle_py as O
 import sbwt_amd as S
 import workloads as W
-from colorlib import ColoredIndex, colored_seq_ids, kmer_ends_of, read_stats
+from colorlib import K, ColoredIndex, colored_seq_ids, kmer_ends_of, read_stats
 
 pytestmark = pytest.mark.gpu
 
@@ -188,3 +188,29 @@ def test_fused_color_table_equals_oracle(tmp_path, n_genomes, algo):
     other = S.ColorIndex(ci.colors_path)
     with pytest.raises(S.SbwtError):
         other.searcher(100, 10).set_input(idx)
+
+
+def test_fused_ranks_with_other_index_marks_match_per_kmer(tmp_path):
+    """The streaming key pass takes a marked node's color-table rank from the search
+    (its line's mark rank) only when the index's own key marks are the color index's:
+    with one more node marked in the index, those ranks differ, and the streaming fused
+    entries must still equal the per-k-mer fused search's, which reads the table."""
+    say("fused ranks: index marks differ from the color index's")
+    ci = ColoredIndex(tmp_path, n_genomes=80, length=12_000, sub=0.02, d=9, seed=5)
+    km = ci.key_marks.copy()
+    bits = np.unpackbits(km.view(np.uint8), bitorder="little")[:ci.sbwt.n]
+    extra = int(np.flatnonzero(bits == 0)[0])
+    km.view(np.uint8)[extra // 8] |= np.uint8(1 << (extra % 8))
+    idx = S.Index.create([ci.sbwt.bits(c) for c in range(4)], ci.sbwt.n, K, key_marks=km,
+                         suffix_group_starts=ci.sbwt.bits(4))
+    col = S.ColorIndex(ci.colors_path)
+    assert idx.build_color_table(col) == 2
+    lens = np.concatenate([np.full(600, 120, np.uint32), W.uniform_lengths(63, 60, 31, 6000)])
+    chars, ends = ci.reads(len(lens), lens, seed=64, random_frac=0.1, err=0.01)
+    got = {}
+    for algo in (S.ALGO_STREAMING, S.ALGO_PER_KMER):
+        s = idx.searcher(len(chars), len(ends), move_to_key_kmer=True)
+        s.set_algorithm(algo)
+        s.set_color_output(True)
+        got[algo] = s.search_reads(chars, ends)
+    assert np.array_equal(got[S.ALGO_STREAMING], got[S.ALGO_PER_KMER])
