@@ -929,17 +929,16 @@ void with_counters_input(u64 nc, int input, F f) {
 // kColorMaskMaxColors colors): the key's color set as a bit mask.
 template <typename Idx, bool MASKS>
 __global__ __launch_bounds__(256) void k_color_table(IndexView ix, ColorView cv, u64 n, void *out,
-                                                      u32 *marks_differ) {
+                                                      u64 *key_masks, u32 *marks_differ) {
   for (u64 v = static_cast<u64>(blockIdx.x) * blockDim.x + threadIdx.x; v < n;
        v += static_cast<u64>(gridDim.x) * blockDim.x) {
-    if constexpr (!MASKS) {
-      // key ranks: the lines' own marks rank the same way when they are the
-      // color index's marks, bit for bit
-      const ulonglong2 me = reinterpret_cast<const ulonglong2 *>(cv.key_marks)[v >> 6];
-      const Idx b = block_of(static_cast<Idx>(v));
-      const bool line_mark = entry_bit(*line_slot(ix, b, kSlotMarks), static_cast<u32>(v - b * kBlockNodes));
-      if (line_mark != (((me.y >> (v & 63)) & 1) != 0)) *marks_differ = 1;
-    }
+    // the lines' own marks rank the key k-mers the same way when they are the
+    // color index's marks, bit for bit
+    const ulonglong2 me = reinterpret_cast<const ulonglong2 *>(cv.key_marks)[v >> 6];
+    const bool color_mark = ((me.y >> (v & 63)) & 1) != 0;
+    const Idx b = block_of(static_cast<Idx>(v));
+    const bool line_mark = entry_bit(*line_slot(ix, b, kSlotMarks), static_cast<u32>(v - b * kBlockNodes));
+    if (line_mark != color_mark) *marks_differ = 1;
     u32 walked = 0;
     const u64 key = key_move<Idx>(ix, static_cast<Idx>(v), walked);
     const bool stuck = key == kWalkStuck;
@@ -967,6 +966,7 @@ __global__ __launch_bounds__(256) void k_color_table(IndexView ix, ColorView cv,
         m &= (1ULL << cv.num_colors) - 1;
       }
       static_cast<u64 *>(out)[v] = m;
+      if (color_mark) key_masks[rank] = m;  // a key k-mer (key == v): its mask by its rank
     }
   }
 }
@@ -1100,6 +1100,10 @@ void build_color_table(DeviceIndex &index, const DeviceColors &colors) {
     SBWT_HIP(hipFree(index.color_table));
     index.color_table = nullptr;
   }
+  if (index.key_color_masks) {
+    SBWT_HIP(hipFree(index.key_color_masks));
+    index.key_color_masks = nullptr;
+  }
   index.color_ranks_in_lines = false;
   const bool masks = colors.num_colors <= kColorMaskMaxColors;
   if (!masks && colors.num_keys >= kNoKeyRank) throw std::runtime_error("too many key k-mers for a rank table");
@@ -1108,15 +1112,22 @@ void build_color_table(DeviceIndex &index, const DeviceColors &colors) {
   u32 *d_differ = nullptr;
   SBWT_HIP(hipMalloc(&d_differ, sizeof(u32)));
   SBWT_HIP(hipMemset(d_differ, 0, sizeof(u32)));
+  // masks: also each key k-mer's mask by its key rank (8 B per key)
+  const u64 key_mask_bytes = masks ? (colors.num_keys + 1) * 8 : 0;
+  if (masks) {
+    SBWT_HIP(hipMalloc(&index.key_color_masks, key_mask_bytes));
+    SBWT_HIP(hipMemset(index.key_color_masks, 0, key_mask_bytes));
+  }
   const IndexView ix = index.view();
   const u32 grid = grid_for(index.n, 256);
   void *out = index.color_table;
+  u64 *km = index.key_color_masks;
   if (index.wide) {
-    if (masks) k_color_table<u64, true><<<grid, 256>>>(ix, colors.view(), index.n, out, d_differ);
-    else k_color_table<u64, false><<<grid, 256>>>(ix, colors.view(), index.n, out, d_differ);
+    if (masks) k_color_table<u64, true><<<grid, 256>>>(ix, colors.view(), index.n, out, km, d_differ);
+    else k_color_table<u64, false><<<grid, 256>>>(ix, colors.view(), index.n, out, km, d_differ);
   } else {
-    if (masks) k_color_table<u32, true><<<grid, 256>>>(ix, colors.view(), index.n, out, d_differ);
-    else k_color_table<u32, false><<<grid, 256>>>(ix, colors.view(), index.n, out, d_differ);
+    if (masks) k_color_table<u32, true><<<grid, 256>>>(ix, colors.view(), index.n, out, km, d_differ);
+    else k_color_table<u32, false><<<grid, 256>>>(ix, colors.view(), index.n, out, km, d_differ);
   }
   SBWT_HIP(hipGetLastError());
   u32 differ = 1;
@@ -1124,8 +1135,14 @@ void build_color_table(DeviceIndex &index, const DeviceColors &colors) {
   SBWT_HIP(hipFree(d_differ));
   index.color_table_kind = masks ? kColorTableMasks : kColorTableRanks;
   // the streaming search then hands a marked node's rank on in the result
-  // after it (kRankShift), so the key pass need not read its entry
-  index.color_ranks_in_lines = !masks && !index.wide && differ == 0 && colors.num_keys <= kRankMax;
+  // after it (kRankShift), so the key pass need not read its node's entry
+  index.color_ranks_in_lines = !index.wide && differ == 0 && colors.num_keys <= kRankMax;
+  if (masks && !index.color_ranks_in_lines) {
+    SBWT_HIP(hipFree(index.key_color_masks));
+    index.key_color_masks = nullptr;
+  } else {
+    index.device_bytes += key_mask_bytes;
+  }
   index.color_table_source = &colors;
   index.device_bytes += bytes;
 }

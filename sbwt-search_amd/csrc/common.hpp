@@ -68,11 +68,11 @@ constexpr u32 kWalkMarked = 4, kWalkNoEdge = 5;
 // k-mer's node), marked (its own key), or stuck walk.
 constexpr u32 kPredShift = 61;
 constexpr u32 kPredLookup = 0, kPredSameKey = 1, kPredMarked = 2, kPredStuck = 3;
-// With a key-rank color table whose key ranks are the ranks of the lines' own
-// marks (IndexView::line_ranks, narrow indexes), bits 32-60 of such a result
-// hold 1 + the rank of the marked node before it (its color-table entry),
-// read from the same line; 0 when that node is not marked or the rank does
-// not fit.
+// With a color table built over the lines' own marks (IndexView::line_ranks,
+// narrow indexes), bits 32-60 of such a result hold 1 + the rank of the
+// marked node before it among the marked nodes, read from the same line (0
+// when that node is not marked or the rank does not fit): its color-table
+// entry with key ranks, the index of its mask in key_masks with masks.
 constexpr u32 kRankShift = 32;
 constexpr u32 kRankMax = (1u << 29) - 2;  // largest rank the field holds (as rank + 1)
 constexpr u32 kHyperShift = 25;  // lines per hyperblock = 2^25 (3.2 G nodes)
@@ -100,7 +100,8 @@ struct IndexView {
   // (kColorTableRanks); null for key nodes.
   const void *color_table;
   u32 color_table_kind;
-  u32 line_ranks;             // ranks color table equal to the lines' mark ranks (kRankShift)
+  u32 line_ranks;             // color table keyed like the lines' mark ranks (kRankShift)
+  const u64 *key_masks;       // masks color table: the mask of each key k-mer, by key rank
 };
 
 constexpr u32 kColorTableNone = 0, kColorTableMasks = 1, kColorTableRanks = 2;

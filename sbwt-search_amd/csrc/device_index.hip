@@ -534,6 +534,8 @@ void DeviceIndex::release() {
   if (color_table) (void)hipFree(color_table);
   color_table = nullptr;
   color_ranks_in_lines = false;
+  if (key_color_masks) (void)hipFree(key_color_masks);
+  key_color_masks = nullptr;
   color_table_kind = kColorTableNone;
   blocks = nullptr;
   hyper_base = nullptr;

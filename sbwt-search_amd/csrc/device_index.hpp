@@ -46,9 +46,11 @@ struct DeviceIndex {
   void *color_table = nullptr;
   u32 color_table_kind = kColorTableNone;
   const void *color_table_source = nullptr;  // the DeviceColors it was built from
-  // the color table holds key ranks equal to the ranks of the lines' own
-  // marks (checked when it is built): a marked node's entry is its line rank
+  // the color table was built over marks equal to the lines' own (checked
+  // when it is built): a marked node's line rank is its key rank, its entry
+  // with key ranks, the index of its mask in key_color_masks with masks
   bool color_ranks_in_lines = false;
+  u64 *key_color_masks = nullptr;  // masks color table: [key rank] = the key's mask
   bool has_groups = false;  // suffix-group starts are in the lines
   // starts were given but some non-start row has an edge: not an SBWT, so
   // every search runs per k-mer (the streaming search needs the SBWT property)

@@ -190,13 +190,15 @@ def test_fused_color_table_equals_oracle(tmp_path, n_genomes, algo):
         other.searcher(100, 10).set_input(idx)
 
 
-def test_fused_ranks_with_other_index_marks_match_per_kmer(tmp_path):
-    """The streaming key pass takes a marked node's color-table rank from the search
-    (its line's mark rank) only when the index's own key marks are the color index's:
-    with one more node marked in the index, those ranks differ, and the streaming fused
-    entries must still equal the per-k-mer fused search's, which reads the table."""
-    say("fused ranks: index marks differ from the color index's")
-    ci = ColoredIndex(tmp_path, n_genomes=80, length=12_000, sub=0.02, d=9, seed=5)
+@pytest.mark.parametrize("n_genomes", [8, 80], ids=["masks-8-colors", "ranks-80-colors"])
+def test_fused_entries_with_other_index_marks_match_per_kmer(tmp_path, n_genomes):
+    """The streaming key pass takes a marked node's key rank from the search (its line's
+    mark rank: the color-table entry with ranks, the index of its mask with masks) only
+    when the index's own key marks are the color index's: with one more node marked in
+    the index, those ranks differ, and the streaming fused entries must still equal the
+    per-k-mer fused search's, which reads the table."""
+    say(f"fused entries, {n_genomes} colors: index marks differ from the color index's")
+    ci = ColoredIndex(tmp_path, n_genomes=n_genomes, length=12_000, sub=0.02, d=9, seed=5)
     km = ci.key_marks.copy()
     bits = np.unpackbits(km.view(np.uint8), bitorder="little")[:ci.sbwt.n]
     extra = int(np.flatnonzero(bits == 0)[0])
@@ -204,7 +206,7 @@ def test_fused_ranks_with_other_index_marks_match_per_kmer(tmp_path):
     idx = S.Index.create([ci.sbwt.bits(c) for c in range(4)], ci.sbwt.n, K, key_marks=km,
                          suffix_group_starts=ci.sbwt.bits(4))
     col = S.ColorIndex(ci.colors_path)
-    assert idx.build_color_table(col) == 2
+    assert idx.build_color_table(col) == (1 if n_genomes <= 61 else 2)
     lens = np.concatenate([np.full(600, 120, np.uint32), W.uniform_lengths(63, 60, 31, 6000)])
     chars, ends = ci.reads(len(lens), lens, seed=64, random_frac=0.1, err=0.01)
     got = {}
