@@ -453,6 +453,11 @@ __global__ void k_reset_counter(u64 *c) { *c = 0; }
 // iteration; on C2, 2 / 3 / 4 / 5 / 6 passes: search kernel 23.2 / 22.0 /
 // 21.6 / 21.5 / 21.4 ms (C3's index stage 32.4 / 32.6 / 32.3 ms for 2-4).
 constexpr u32 kStreamPasses = 4;
+// Without a k-mer table, fresh searches step through the SBWT one full
+// iteration per letter, and every pass lengthens those iterations: on C2
+// without the table, 1 / 2 / 3 / 4 passes: search kernel 30.0 / 29.2 / 30.8 /
+// 32.8 ms.
+constexpr u32 kStreamPassesNoTable = 2;
 
 // v's suffix-group start when its line holds it: the last group-start bit at
 // or before offset o in the line's group slot dg (pos = its offset); false
@@ -537,6 +542,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
   u32 st_full = 0, st_steps = 0, st_walk = 0, st_inv = 0, st_stream = 0, st_extra = 0, st_fb = 0;
   u32 st_group = 0, st_tail = 0, st_tail_hit = 0, st_skip = 0, st_hash = 0;
   const u64 hash_mask = ix.kmer_hash ? (1ULL << ix.kmer_hash_bits) - 1 : 0;
+  const u32 passes = ix.kmer_hash ? kStreamPasses : kStreamPassesNoTable;  // steps per iteration
   const u32 rem_bits = 2 * k - ix.kmer_hash_bits;
   // in kHash, pre_idx = the k-mer's mix and j = the groups read before this
   // one: the group to read, and the low bits of the k-mer's entry there
@@ -924,7 +930,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
       }
     }
     };
-    // Then further steps in the same iteration (kStreamPasses in all) for the
+    // Then further steps in the same iteration (`passes` in all) for the
     // two commonest states: a stream step that v's own line settles (the next
     // k-mer on v's edge c, on the edge c of v's suffix-group start in the same
     // line, or a miss), and a k-mer table probe. Every other state's code then
@@ -936,12 +942,12 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
     // One copy of advance() serves every pass (two copies cost registers, then
     // spills).
 #pragma clang loop unroll(disable)
-    for (u32 pass = 0; pass < kStreamPasses; ++pass) {
+    for (u32 pass = 0; pass < passes; ++pass) {
       advance(res, emitted, decide);
       emitted = false;
       decide = false;
       bool moved = false;
-      if (pass + 1 < kStreamPasses && mode == kStream) {
+      if (pass + 1 < passes && mode == kStream) {
         const Idx bs = block_of(v);
         const u32 os = static_cast<u32>(v - bs * kBlockNodes);
         const uint4 ds = *line_slot(ix, bs, c);
@@ -988,7 +994,7 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
             miss_at = p + k - 1;
           }
         }
-      } else if (pass + 1 < kStreamPasses && mode == kHash) {
+      } else if (pass + 1 < passes && mode == kHash) {
         // a k-mer table probe, as in the full step
         const uint4 *group = ix.kmer_hash + 2 * hash_group();
         const uint4 h0 = group[0], h1 = group[1];
