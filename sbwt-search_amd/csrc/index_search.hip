@@ -1056,8 +1056,11 @@ __global__ __launch_bounds__(kSearchThreads) void k_search_stream(
 // found k-mer. Work items never share a chain: each starts with a full search.
 constexpr u32 kRunBlocks = 4;  // blocks of 64 k-mers whose chain ends are looked up together
 
+// Eight waves per SIMD: the pass is a chain of dependent round trips per
+// wave, so occupancy is its throughput (65 -> 63 registers for the eighth wave:
+// C3's key runs 3.84 -> 3.05 ms).
 template <typename Idx, bool STATS>
-__global__ __launch_bounds__(kSearchThreads) void k_key_runs(IndexView ix,
+__global__ __launch_bounds__(kSearchThreads, 8) void k_key_runs(IndexView ix,
                                                             const ulonglong2 *__restrict__ chunks,
                                                             const u64 *__restrict__ chunk_ends,
                                                             u64 n_reads, u64 *__restrict__ out,
