@@ -579,8 +579,10 @@ __device__ __forceinline__ void finish_read(u32 *cnt, u32 nc, u64 r, u32 f, u32 
 
 // Fused color search, one wave per read (grid-stride): the plain form.
 // COUNTS writes the raw u32 counts; otherwise finish_read thresholds them.
+// Key-node input holds five waves per SIMD under a register cap (98 -> 96:
+// C3's key-node color kernel 14.6 -> 13.9 ms); a cap on the other inputs spills.
 template <bool COUNTS, bool LDS, int IN>
-__global__ __launch_bounds__(256) void k_color_reads(ColorView cv, const u64 *__restrict__ results,
+__global__ __launch_bounds__(256, (IN == kInKeys ? 5 : 1)) void k_color_reads(ColorView cv, const u64 *__restrict__ results,
                                                       const u64 *__restrict__ kmer_ends, u64 n_reads,
                                                       KernelParams p, ReadOut o,
                                                       u32 *__restrict__ counts_out,
