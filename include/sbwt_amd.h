@@ -438,10 +438,24 @@ int sbwt_color_searcher_set_work_items(sbwt_color_searcher *searcher, uint32_t i
  * ranks with one key-set lookup per run instead of the key-mark rank and the
  * key set. Results are identical to the two-stage path. kind nullable. */
 int sbwt_index_build_color_table(sbwt_index *index, const sbwt_color_index *colors, int *kind);
+/* on: SBWT_COLOR_OUTPUT_ENTRIES (1) writes the table entries above;
+ * SBWT_COLOR_OUTPUT_CHAINS (2, indexes of fewer than 2^32 nodes, reads path)
+ * writes each found k-mer's node with what the search learnt about the k-mer
+ * before it (its key's color entry when that one is marked), leaving the key
+ * pass to a color searcher with chain input (sbwt_color_searcher_set_chain_input):
+ * one pass over the results fewer; SBWT_COLOR_OUTPUT_NONE (0) key nodes. */
+#define SBWT_COLOR_OUTPUT_NONE 0
+#define SBWT_COLOR_OUTPUT_ENTRIES 1
+#define SBWT_COLOR_OUTPUT_CHAINS 2
 int sbwt_searcher_set_color_output(sbwt_searcher *searcher, int on);
 /* index: the index whose color-table entries the results are (null: key nodes,
  * the default). */
 int sbwt_color_searcher_set_input(sbwt_color_searcher *searcher, const sbwt_index *index);
+/* The results are the key chains a searcher with SBWT_COLOR_OUTPUT_CHAINS
+ * writes over index: the color stage resolves each chain to its key's entry
+ * (as the searcher's key pass would) while it counts. Lists are identical to
+ * the other inputs'. No counterpart in the reference. */
+int sbwt_color_searcher_set_chain_input(sbwt_color_searcher *searcher, const sbwt_index *index);
 
 /* Device time (ms) of the last call's color kernel. */
 int sbwt_color_searcher_last_time(sbwt_color_searcher *searcher, float *ms);

@@ -43,9 +43,10 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect
       work items of item_kmers k-mers, the batches spread over two streams so one
       batch's tail overlaps the next one's start.
     batched: as batched_unfused, with index and color search fused through the
-      index's color table (idx.build_color_table(col) must have run): the key
-      pass writes each k-mer's color entry (a mask, or a key rank) and the color
-      stage counts masks by ballots (ranks: one key-set lookup per run).
+      index's color table (idx.build_color_table(col) must have run): the search
+      leaves its key chains (COLOR_OUTPUT_CHAINS) and the color stage resolves
+      them to each k-mer's color entry (a mask, or a key rank) while it counts,
+      masks by ballots (ranks: one key-set lookup per run): no key pass.
 
     Outputs (per batch: CSR offsets and color ids) are compared in full with the
     plain form's. With collect, the outputs are returned too."""
@@ -70,8 +71,8 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect
         cs = col.searcher(max_chars, max_reads, stream=stream.cuda_stream)
         cs.set_work_items(items)
         if fused:
-            s.set_color_output(True)
-            cs.set_input(idx)
+            s.set_color_output(S.COLOR_OUTPUT_CHAINS)
+            cs.set_input(idx, chains=True)
         res = torch.empty(max_chars, dtype=torch.int64, device=dev)
         off = torch.empty(max_reads + 1, dtype=torch.int64, device=dev)
         cap = max_reads * nc
@@ -142,8 +143,8 @@ def run_batched(S, torch, wl, idx, col, batch_chars, tau=0.7, log=print, collect
          "color_ids": n_ids, "outputs_equal": True,
          "forms": {"plain": "one stream; key nodes; color stage one wave per read",
                    "batched_unfused": f"two streams; key nodes; {item_kmers}-k-mer work items",
-                   "batched": f"two streams; color-table entries (fused); {item_kmers}-k-mer "
-                              "work items"},
+                   "batched": f"two streams; key chains resolved to color-table entries by the "
+                              f"color stage (fused); {item_kmers}-k-mer work items"},
          "plain_s": t_plain, "batched_unfused_s": t_unfused, "batched_s": t_batched,
          "plain_reads_per_s": wl.n_reads / t_plain,
          "batched_unfused_reads_per_s": wl.n_reads / t_unfused,

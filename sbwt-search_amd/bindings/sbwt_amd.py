@@ -18,6 +18,7 @@ ALGO_STREAMING = 0
 ALGO_PER_KMER = 1
 ALGO_AUTO = 2  # default: per k-mer below 2^22 chars per batch, streaming (adaptive work items) above
 ALGO_STREAMING_ADAPTIVE = 3  # adaptive work items from 2^16 chars: batches sharing the GPU
+COLOR_OUTPUT_NONE, COLOR_OUTPUT_ENTRIES, COLOR_OUTPUT_CHAINS = 0, 1, 2  # sbwt_searcher_set_color_output
 NOALLOC_MAX_CHARS = 1 << 22
 NOALLOC_MAX_READS = 1 << 18
 NOT_FOUND = 0xFFFFFFFFFFFFFFFF
@@ -129,6 +130,7 @@ def lib():
         L.sbwt_index_build_color_table.argtypes = [vp, vp, C.POINTER(C.c_int)]
         L.sbwt_searcher_set_color_output.argtypes = [vp, C.c_int]
         L.sbwt_color_searcher_set_input.argtypes = [vp, vp]
+        L.sbwt_color_searcher_set_chain_input.argtypes = [vp, vp]
         L.sbwt_color_searcher_last_time.argtypes = [vp, C.POINTER(C.c_float)]
         L.sbwt_color_searcher_set_census.argtypes = [vp, C.c_int]
         L.sbwt_searcher_last_key_runs_time.argtypes = [vp, C.POINTER(C.c_float)]
@@ -384,7 +386,9 @@ class Searcher:
         _check(lib().sbwt_searcher_set_stats(self.h, int(enable)))
 
     def set_color_output(self, on):
-        """Write each found k-mer's color-table entry instead of its key node."""
+        """COLOR_OUTPUT_ENTRIES (or True): write each found k-mer's color-table entry
+        instead of its key node; COLOR_OUTPUT_CHAINS: the key chains, for a color
+        searcher with chain input; COLOR_OUTPUT_NONE (or False): key nodes."""
         _check(lib().sbwt_searcher_set_color_output(self.h, int(on)))
 
     def last_stats(self):
@@ -689,9 +693,13 @@ class ColorSearcher:
             C.c_void_p(d_stats) if d_stats else None, C.byref(nids)))
         return nids.value
 
-    def set_input(self, index):
-        """None: results are key nodes; an Index: its color-table entries."""
-        _check(lib().sbwt_color_searcher_set_input(self.h, index.h if index is not None else None))
+    def set_input(self, index, chains=False):
+        """None: results are key nodes; an Index: its color-table entries, or with
+        chains, the key chains a searcher with COLOR_OUTPUT_CHAINS writes over it."""
+        if chains:
+            _check(lib().sbwt_color_searcher_set_chain_input(self.h, index.h))
+        else:
+            _check(lib().sbwt_color_searcher_set_input(self.h, index.h if index is not None else None))
 
     def set_work_items(self, item_kmers):
         """0: one wave per read; else a queue of (read, k-mer range) items."""

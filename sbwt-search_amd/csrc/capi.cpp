@@ -570,7 +570,7 @@ int sbwt_index_build_color_table(sbwt_index *index, const sbwt_color_index *colo
 int sbwt_searcher_set_color_output(sbwt_searcher *searcher, int on) {
   return guarded([&] {
     require(searcher, "null argument");
-    searcher->impl->set_color_output(on != 0);
+    searcher->impl->set_color_output(on);
   });
 }
 
@@ -578,6 +578,13 @@ int sbwt_color_searcher_set_input(sbwt_color_searcher *searcher, const sbwt_inde
   return guarded([&] {
     require(searcher, "null argument");
     searcher->impl->set_input(index ? &index->dev : nullptr);
+  });
+}
+
+int sbwt_color_searcher_set_chain_input(sbwt_color_searcher *searcher, const sbwt_index *index) {
+  return guarded([&] {
+    require(searcher && index, "null argument");
+    searcher->impl->set_input(&index->dev, true);
   });
 }
 

@@ -1,6 +1,7 @@
 // Color search kernels for gfx950: per-read color counts in LDS, threshold,
 // sorted-id compaction; plus the reference's warp-padded interface.
 #include <algorithm>
+#include <type_traits>
 #include <vector>
 
 #include "color_search.hpp"
@@ -318,8 +319,81 @@ __device__ __forceinline__ void wave_payload(const ColorView &cv, const SetRef &
 
 // Inputs of the color stage: each k-mer's key node (the reference's index
 // output), or its key's entry of the index's color table: a color mask (the
-// counts then come from ballots, with no lookups) or the key's rank.
-constexpr int kInKeys = 0, kInMasks = 1, kInRanks = 2;
+// counts then come from ballots, with no lookups) or the key's rank; or the
+// key chains of either (a searcher's kColorOutputChains), resolved into those
+// entries here.
+constexpr int kInKeys = 0, kInMasks = 1, kInRanks = 2, kInChainMasks = 3, kInChainRanks = 4;
+// The entries an input is counted as (after resolving its chains).
+constexpr int entry_input(int in) {
+  return in == kInChainMasks ? kInMasks : (in == kInChainRanks ? kInRanks : in);
+}
+constexpr bool chain_input(int in) { return in == kInChainMasks || in == kInChainRanks; }
+
+// Flag of a key-chain result for the k-mer before it: kPred* in bits 0-1, 1 +
+// the rank of that k-mer's (marked) node above them or 0; a k-mer not found
+// says nothing (kPredLookup).
+__device__ __forceinline__ u32 chain_flag(u64 v) {
+  return (v >> 63) ? kPredLookup
+                   : (static_cast<u32>(v >> kPredShift) & 3u) |
+                         ((static_cast<u32>(v >> kRankShift) & ((1u << 29) - 1)) << 2);
+}
+
+// The key chains of one group (kGroup windows of 64 results), right to left,
+// as k_key_runs resolves them: a found k-mer whose successor's flag does not
+// say "same key" ends a chain and takes its key's entry (a marked node's rank
+// from the flag: the rank itself, or the index of its mask; stuck: the stuck
+// sentinel; else the node's color-table entry); every other found k-mer takes
+// the entry of the first chain end to its right. carry_flag / carry_key: the
+// flag of the k-mer after the group and that k-mer's entry (in: from the
+// group to the right; out: for the group to the left). Lines read: cs.
+template <bool MASKS>
+__device__ __forceinline__ void resolve_chains(const ColorView &cv, u64 (&v)[kGroup], u32 lane,
+                                               u32 &carry_flag, u64 &carry_key, Census &cs) {
+  // entries as the table holds them: masks, or u32 ranks with kNoKeyRank for stuck
+  using Entry = std::conditional_t<MASKS, u64, u32>;
+  u32 flag[kGroup];
+  bool brk[kGroup];
+  Entry base[kGroup];
+#pragma unroll
+  for (u32 g = 0; g < kGroup; ++g) flag[g] = chain_flag(v[g]);
+  u32 right = carry_flag;  // flag of the slot after window g's last
+#pragma unroll
+  for (int g = kGroup - 1; g >= 0; --g) {
+    u32 succ = static_cast<u32>(__shfl_down(static_cast<int>(flag[g]), 1, kWave));
+    if (lane == kWave - 1) succ = right;
+    right = static_cast<u32>(__shfl(static_cast<int>(flag[g]), 0, kWave));
+    brk[g] = (v[g] >> 63) == 0 && (succ & 3u) != kPredSameKey;
+    base[g] = 0;
+    if (brk[g]) {
+      if ((succ & 3u) == kPredStuck) {
+        base[g] = MASKS ? static_cast<Entry>(kWalkStuck) : static_cast<Entry>(kNoKeyRank);
+      } else if ((succ >> 2) != 0) {
+        if constexpr (MASKS) {
+          base[g] = cv.chain_key_masks[(succ >> 2) - 1];
+          if (cs.on) ++cs.mark_lines;
+        } else {
+          base[g] = (succ >> 2) - 1;
+        }
+      } else {
+        base[g] = static_cast<const Entry *>(cv.chain_table)[static_cast<u32>(v[g])];
+        if (cs.on) ++cs.mark_lines;
+      }
+    }
+  }
+  Entry ckey = static_cast<Entry>(carry_key);
+#pragma unroll
+  for (int g = kGroup - 1; g >= 0; --g) {
+    const u64 breaks = __ballot(brk[g]);
+    const u64 rgt = lane == kWave - 1 ? 0 : breaks & (~0ULL << (lane + 1));
+    const int src = rgt ? __ffsll(static_cast<long long>(rgt)) - 1 : 0;
+    const Entry from = __shfl(base[g], src, kWave);
+    const Entry key = brk[g] ? base[g] : (rgt ? from : ckey);
+    if ((v[g] >> 63) == 0) v[g] = MASKS || key != kNoKeyRank ? static_cast<u64>(key) : kWalkStuck;
+    ckey = __shfl(key, 0, kWave);
+  }
+  carry_key = ckey;
+  carry_flag = static_cast<u32>(__shfl(static_cast<int>(flag[0]), 0, kWave));
+}
 
 // Up to kGroup windows of 64 result slots from slot `base` of a read ending
 // at `ke`: counts found / not-found / invalid slots and adds every found
@@ -335,7 +409,9 @@ constexpr int kInKeys = 0, kInMasks = 1, kInRanks = 2;
 template <int IN>
 __device__ __forceinline__ void group_add(const ColorView &cv, const u64 *__restrict__ results,
                                           u64 base, u64 ke, u32 lane, u32 *cnt, uint4 *tab,
-                                          u32 &f, u32 &nf, u32 &inv, u32 &touched, Census &cs) {
+                                          u32 &f, u32 &nf, u32 &inv, u32 &touched, Census &cs,
+                                          u32 &carry_flag, u64 &carry_key) {
+  constexpr int E = entry_input(IN);
   u64 v[kGroup];
   bool fd[kGroup];
 #pragma unroll
@@ -343,15 +419,16 @@ __device__ __forceinline__ void group_add(const ColorView &cv, const u64 *__rest
     const u64 i = base + g * kWave + lane;
     v[g] = i < ke ? results[i] : kInvalid;
   }
+  if constexpr (chain_input(IN)) resolve_chains<IN == kInChainMasks>(cv, v, lane, carry_flag, carry_key, cs);
 #pragma unroll
   for (u32 g = 0; g < kGroup; ++g) {
     const bool in = base + g * kWave + lane < ke;
-    fd[g] = in && (IN == kInKeys ? v[g] < cv.num_nodes : (v[g] >> 63) == 0);
+    fd[g] = in && (E == kInKeys ? v[g] < cv.num_nodes : (v[g] >> 63) == 0);
     f += static_cast<u32>(__popcll(__ballot(fd[g])));
     inv += static_cast<u32>(__popcll(__ballot(in && v[g] == kInvalid)));
     nf += static_cast<u32>(__popcll(__ballot(in && !fd[g] && v[g] != kInvalid)));
   }
-  if constexpr (IN == kInMasks) {
+  if constexpr (E == kInMasks) {
     u32 mine = 0;
 #pragma unroll
     for (u32 g = 0; g < kGroup; ++g) {
@@ -418,7 +495,7 @@ __device__ __forceinline__ void group_add(const ColorView &cv, const u64 *__rest
     SetRef set;
     u32 units = 0;
     if (has) {
-      const u64 rank = IN == kInKeys ? rank_marks(cv.key_marks, x) : x;
+      const u64 rank = E == kInKeys ? rank_marks(cv.key_marks, x) : x;
       const uint4 e = cv.key_sets[rank];
       set.start = static_cast<u64>(e.x) | (static_cast<u64>(e.y) << 32);
       set.len = e.z;
@@ -426,7 +503,7 @@ __device__ __forceinline__ void group_add(const ColorView &cv, const u64 *__rest
       units = set.dense ? (set.len + 63) / 64 : set.len;
       if (cs.on) {
         ++cs.set_lines;
-        cs.mark_lines += IN == kInKeys ? 1u : 0u;
+        cs.mark_lines += E == kInKeys ? 1u : 0u;
         const u64 w = set.dense ? 1 : cv.sparse_width;
         cs.payload_lines += lines_spanned(set.start * w, (set.start + set.len) * w);
       }
@@ -435,13 +512,31 @@ __device__ __forceinline__ void group_add(const ColorView &cv, const u64 *__rest
   }
 }
 
-// All k-mer results [ks, ke) of one read into the wave's counters.
+// The k-mer results [ks, ke) of one read (whose k-mers end at re) into the
+// wave's counters. Key chains run right to left, group by group; a chain
+// that continues past ke (into another work item) ends at ke - 1 with its
+// node's color-table entry.
 template <int IN>
 __device__ __forceinline__ void read_add(const ColorView &cv, const u64 *__restrict__ results, u64 ks,
-                                         u64 ke, u32 lane, u32 *cnt, uint4 *tab, u32 &f, u32 &nf,
-                                         u32 &inv, u32 &touched, Census &cs) {
-  for (u64 base = ks; base < ke; base += kGroup * kWave)
-    group_add<IN>(cv, results, base, ke, lane, cnt, tab, f, nf, inv, touched, cs);
+                                         u64 ke, u64 re, u32 lane, u32 *cnt, uint4 *tab, u32 &f,
+                                         u32 &nf, u32 &inv, u32 &touched, Census &cs) {
+  u32 carry_flag = kPredLookup;
+  u64 carry_key = 0;
+  if constexpr (chain_input(IN)) {
+    if (ke <= ks) return;
+    if (ke < re) {
+      const u32 nf_ = chain_flag(results[ke]);
+      carry_flag = (nf_ & 3u) == kPredSameKey ? kPredLookup : nf_;
+    }
+    constexpr u64 kStep = kGroup * kWave;
+    for (u64 gi = (ke - ks - 1) / kStep + 1; gi-- > 0;)
+      group_add<IN>(cv, results, ks + gi * kStep, ke, lane, cnt, tab, f, nf, inv, touched, cs,
+                    carry_flag, carry_key);
+  } else {
+    for (u64 base = ks; base < ke; base += kGroup * kWave)
+      group_add<IN>(cv, results, base, ke, lane, cnt, tab, f, nf, inv, touched, cs, carry_flag,
+                    carry_key);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -605,7 +700,7 @@ __global__ __launch_bounds__(256, (IN == kInKeys ? 5 : 1)) void k_color_reads(Co
        r += static_cast<u64>(gridDim.x) * waves) {
     const u64 ks = r ? kmer_ends[r - 1] : 0, ke = kmer_ends[r];
     u32 f = 0, nf = 0, inv = 0, touched = 0;
-    read_add<IN>(cv, results, ks, ke, lane, cnt, tab, f, nf, inv, touched, cs);
+    read_add<IN>(cv, results, ks, ke, ke, lane, cnt, tab, f, nf, inv, touched, cs);
     kmers += ke - ks;
     cnt_fence<LDS>();
     if constexpr (COUNTS) {
@@ -680,7 +775,7 @@ __global__ __launch_bounds__(256) void k_color_items(
     const u64 rs = r ? kmer_ends[r - 1] : 0, re = kmer_ends[r];
     const u64 ks = min(re, rs + (it - i0) * item_kmers), ke = min(re, ks + item_kmers);
     u32 f = 0, nf = 0, inv = 0, touched = 0;
-    read_add<IN>(cv, results, ks, ke, lane, cnt, tab, f, nf, inv, touched, cs);
+    read_add<IN>(cv, results, ks, ke, re, lane, cnt, tab, f, nf, inv, touched, cs);
     kmers += ke - ks;
     cnt_fence<LDS>();
     if (ni == 1) {
@@ -783,7 +878,7 @@ __global__ __launch_bounds__(256) void k_color_quirk2(ColorView cv, const u64 *_
     u32 f = 0, nf = 0, inv = 0;
     Census off;
     u32 touched = 0;
-    read_add<IN>(cv, results, ks, ke, lane, cnt, tab, f, nf, inv, touched, off);
+    read_add<IN>(cv, results, ks, ke, ke, lane, cnt, tab, f, nf, inv, touched, off);
     cnt_fence<LDS>();
     u64 total = 0;
     for (u32 w = 0; w < words; ++w) {
@@ -870,7 +965,7 @@ __global__ __launch_bounds__(256) void k_color_warped(ColorView cv, const u64 *_
     const u64 wbeg = intervals[q] * kWave, wend = intervals[q + 1] * kWave;
     Census off;
     u32 touched = 0;
-    read_add<kInKeys>(cv, warped, wbeg, wend, lane, cnt, tab, f, nf, inv, touched, off);
+    read_add<kInKeys>(cv, warped, wbeg, wend, wend, lane, cnt, tab, f, nf, inv, touched, off);
     cnt_fence<LDS>();
     for (u32 c = lane; c < nc; c += kWave) {
       const u32 v = cnt_get<LDS>(cnt + c);
@@ -920,6 +1015,8 @@ void with_counters_input(u64 nc, int input, F f) {
   with_counters(nc, [&]<bool LDS>() {
     if (input == kInMasks) f.template operator()<LDS, kInMasks>();
     else if (input == kInRanks) f.template operator()<LDS, kInRanks>();
+    else if (input == kInChainMasks) f.template operator()<LDS, kInChainMasks>();
+    else if (input == kInChainRanks) f.template operator()<LDS, kInChainRanks>();
     else f.template operator()<LDS, kInKeys>();
   });
 }
@@ -1149,14 +1246,31 @@ void build_color_table(DeviceIndex &index, const DeviceColors &colors) {
   index.device_bytes += bytes;
 }
 
-void ColorSearcher::set_input(const DeviceIndex *index) {
+void ColorSearcher::set_input(const DeviceIndex *index, bool chains) {
+  chain_index_ = nullptr;
   if (!index) {
     input_ = kInKeys;
     return;
   }
   if (!index->color_table || index->color_table_source != colors_)
     throw std::runtime_error("the index's color table was not built from this color index");
-  input_ = index->color_table_kind == kColorTableMasks ? kInMasks : kInRanks;
+  const bool masks = index->color_table_kind == kColorTableMasks;
+  if (chains) {
+    if (index->wide) throw std::runtime_error("key-chain input needs an index of fewer than 2^32 nodes");
+    chain_index_ = index;
+    input_ = masks ? kInChainMasks : kInChainRanks;
+  } else {
+    input_ = masks ? kInMasks : kInRanks;
+  }
+}
+
+ColorView ColorSearcher::input_view() const {
+  ColorView v = colors_->view();
+  if (chain_index_) {
+    v.chain_table = chain_index_->color_table;
+    v.chain_key_masks = chain_index_->key_color_masks;
+  }
+  return v;
 }
 
 void ColorSearcher::set_work_items(u32 item_kmers) {
@@ -1298,7 +1412,7 @@ void ColorSearcher::launch_threshold(const u64 *d_results, const u64 *d_kmer_end
     const u64 bound = n_reads + max_indexes_ / item_kmers_ + 1;
     with_counters_input(nc, input_, [&]<bool LDS, int IN>() {
       k_color_items<LDS, IN><<<color_grid(bound, nc), kColorWaves * kWave, color_lds_bytes(nc),
-                               stream_>>>(colors_->view(), d_results, d_kmer_ends, n_reads, item_ends,
+                               stream_>>>(input_view(), d_results, d_kmer_ends, n_reads, item_ends,
                                       item_kmers_, long_incl, kp, out, d_global_cnt_, d_long_cnt_,
                                       d_long_meta_, census);
     });
@@ -1309,7 +1423,7 @@ void ColorSearcher::launch_threshold(const u64 *d_results, const u64 *d_kmer_end
     with_counters_input(nc, input_, [&]<bool LDS, int IN>() {
       k_color_reads<false, LDS, IN><<<color_grid(n_reads, nc), kColorWaves * kWave,
                                       color_lds_bytes(nc), stream_>>>(
-          colors_->view(), d_results, d_kmer_ends, n_reads, kp, out, nullptr, d_global_cnt_, census);
+          input_view(), d_results, d_kmer_ends, n_reads, kp, out, nullptr, d_global_cnt_, census);
     });
   }
   SBWT_HIP(hipGetLastError());
@@ -1318,7 +1432,7 @@ void ColorSearcher::launch_threshold(const u64 *d_results, const u64 *d_kmer_end
     k_colored_slots<<<grid_for(n_reads, 256), 256, 0, stream_>>>(colored_incl, n_reads, slot_read);
     with_counters_input(nc, input_, [&]<bool LDS, int IN>() {
       k_color_quirk2<LDS, IN><<<color_grid(n_reads, nc), kColorWaves * kWave, color_lds_bytes(nc),
-                                stream_>>>(colors_->view(), d_results, d_kmer_ends, n_reads, blank_min,
+                                stream_>>>(input_view(), d_results, d_kmer_ends, n_reads, blank_min,
                                        colored_incl, slot_read, d_bitmap_, d_nout_, d_global_cnt_);
     });
     SBWT_HIP(hipGetLastError());
@@ -1338,7 +1452,7 @@ void ColorSearcher::counts_device(const u64 *d_results, const u64 *d_kmer_ends, 
   SBWT_HIP(hipEventRecord(ev_[0], stream_));
   with_counters_input(nc, input_, [&]<bool LDS, int IN>() {
     k_color_reads<true, LDS, IN><<<color_grid(n_reads, nc), kColorWaves * kWave,
-                                   color_lds_bytes(nc), stream_>>>(colors_->view(), d_results, d_kmer_ends, n_reads, kp, out,
+                                   color_lds_bytes(nc), stream_>>>(input_view(), d_results, d_kmer_ends, n_reads, kp, out,
                                           d_counts, d_global_cnt_, nullptr);
   });
   SBWT_HIP(hipGetLastError());

@@ -24,6 +24,10 @@ struct ColorView {
   u32 csi_width;
   u32 sparse_width;
   u32 num_colors;
+  // key-chain input (a searcher's color output kColorOutputChains): the
+  // index's color table (per node) and, with masks, its key masks by rank
+  const void *chain_table;
+  const u64 *chain_key_masks;
 };
 
 struct DeviceColors {
@@ -100,8 +104,11 @@ class ColorSearcher {
   u32 work_items() const { return item_kmers_; }
   // Input of the fused color stage: the key nodes of the index search (null,
   // default), or the color-table entries that a searcher with color output
-  // writes over index (whose table must come from this color index).
-  void set_input(const DeviceIndex *index);
+  // writes over index (whose table must come from this color index), or, with
+  // chains, the key chains such a searcher writes with kColorOutputChains
+  // (narrow indexes), resolved here instead of by the searcher's key pass.
+  void set_input(const DeviceIndex *index, bool chains = false);
+  ColorView input_view() const;  // the color index's view, with the chain input's tables
   // Census of the fused color stage (measurement; leaves results unchanged):
   // when on, each pseudoalign call counts {key-set entries read, key-mark
   // entries read, 128-byte payload lines spanned by the decoded sets, k-mers},
@@ -124,7 +131,8 @@ class ColorSearcher {
   u64 *d_quirk_ = nullptr;    // [4][max_reads]: blank_min, colored, colored_incl, slot_read
   u32 *d_global_cnt_ = nullptr;  // per-wave counters when the colors do not fit LDS
   u32 item_kmers_ = 0;
-  int input_ = 0;             // key nodes, masks or ranks (color_search.hip)
+  int input_ = 0;             // key nodes, masks, ranks or their chains (color_search.hip)
+  const DeviceIndex *chain_index_ = nullptr;  // the index of a key-chain input
   u64 *d_items_ = nullptr;     // [4][max_reads]: items, item ends, multi flags, their scan
   u32 *d_long_cnt_ = nullptr;  // [slots][num_colors] partial counts of multi-item reads
   u32 *d_long_meta_ = nullptr; // [slots][4]: finished items, found, not found, invalid

@@ -105,6 +105,11 @@ struct IndexView {
 };
 
 constexpr u32 kColorTableNone = 0, kColorTableMasks = 1, kColorTableRanks = 2;
+// What a searcher with the key move writes for a found k-mer: its key node,
+// its key's color-table entry, or (key chains, narrow indexes) its node with
+// what the search learnt about the k-mer before it (kPredShift, kRankShift),
+// for a color searcher to resolve.
+constexpr int kColorOutputNone = 0, kColorOutputEntries = 1, kColorOutputChains = 2;
 constexpr u32 kColorMaskMaxColors = 61;  // masks leave bits 61-63 to the sentinels
 constexpr u32 kNoKeyRank = 0xFFFFFFFFu;  // rank table entry of a node whose key walk is stuck
 

@@ -1288,10 +1288,13 @@ Searcher::Searcher(const DeviceIndex *index, u64 max_chars, u64 max_reads, bool 
   set_launch(kDefaultChunkKmers, kDefaultBlocksPerCu);
 }
 
-void Searcher::set_color_output(bool on) {
-  if (on && (!move_to_key_ || !index_->color_table))
+void Searcher::set_color_output(int mode) {
+  if (mode < kColorOutputNone || mode > kColorOutputChains) throw std::runtime_error("unknown color output");
+  if (mode != kColorOutputNone && (!move_to_key_ || !index_->color_table))
     throw std::runtime_error("color-table output needs the key-k-mer move and the index's color table");
-  color_output_ = on;
+  if (mode == kColorOutputChains && index_->wide)
+    throw std::runtime_error("key-chain output needs an index of fewer than 2^32 nodes");
+  color_output_ = mode;
 }
 
 IndexView Searcher::reads_view() const {
@@ -1487,7 +1490,8 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
       k_search_stream<Idx, KEY, STATS><<<blocks, kSearchThreads, 0, stream_>>>(
           ix, d_packed_, d_invalid_, d_chunks_, d_item_ends, n_reads, d_counter_, d_results,
           d_stats_, absence_checks(*index_) ? 1u : 0u);
-      if constexpr (KEY) {
+      // (key chains: the color searcher resolves them)
+      if (KEY && color_output_ != kColorOutputChains) {
         if (timing_) SBWT_HIP(hipEventRecord(ev_[4], stream_));
         // one wave per work item
         k_key_runs<Idx, STATS><<<grid_for(chunk_bound, kSearchThreads / 64), kSearchThreads, 0,
@@ -1496,8 +1500,10 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
     });
   } else {
     if (timing_) SBWT_HIP(hipEventRecord(ev_[2], stream_));
-    // n_chars bounds the k-mer count; surplus workgroups leave at once
-    dispatch_search3(index_->wide, move_to_key_, stats_, [&]<typename Idx, bool KEY, bool STATS>() {
+    // n_chars bounds the k-mer count; surplus workgroups leave at once. Key
+    // chains: each found k-mer's node, every one a chain end of its own
+    const bool key = move_to_key_ && color_output_ != kColorOutputChains;
+    dispatch_search3(index_->wide, key, stats_, [&]<typename Idx, bool KEY, bool STATS>() {
       k_search_reads<Idx, KEY, STATS>
           <<<grid_for(n_chars, kSearchThreads), kSearchThreads, 0, stream_>>>(
               ix, d_packed_, d_invalid_, d_read_ends, d_kmer_ends_, n_reads, n_chars, d_results,
@@ -1508,7 +1514,7 @@ u64 Searcher::search_reads_device(const char *d_chars, u64 n_chars, const u64 *d
   if (timing_) {
     SBWT_HIP(hipEventRecord(ev_[3], stream_));
     events_recorded_ = true;
-    key_runs_timed_ = streaming && move_to_key_;
+    key_runs_timed_ = streaming && move_to_key_ && color_output_ != kColorOutputChains;
   }
   u64 nk = 0;
   if (n_kmers_host) {

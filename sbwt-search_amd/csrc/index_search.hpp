@@ -100,8 +100,11 @@ class Searcher {
   // reference's output) or, with color_output, the key's entry in the
   // index's color table (built from a color index), the input of the fused
   // color stage. Needs move_to_key_kmer and a built table.
-  void set_color_output(bool on);
-  bool color_output() const { return color_output_; }
+  // kColorOutputNone (key nodes), kColorOutputEntries (each found k-mer's
+  // color-table entry) or kColorOutputChains (the key chains, for a color
+  // searcher with chain input: the key pass is left to it)
+  void set_color_output(int mode);
+  int color_output() const { return color_output_; }
   int algorithm() const { return algorithm_; }
   void last_stats(u64 out[kStatsWords]);
   const u64 *kmer_ends_device() const { return d_kmer_ends_; }
@@ -118,7 +121,7 @@ class Searcher {
   bool timing_ = true;
   bool stats_ = false;
   int algorithm_ = kAlgoAuto;
-  bool color_output_ = false;
+  int color_output_ = 0;
   IndexView reads_view() const;
   u32 lane_share_ = 1;
   u64 *d_stats_ = nullptr;

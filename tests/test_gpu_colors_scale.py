@@ -216,3 +216,62 @@ def test_fused_entries_with_other_index_marks_match_per_kmer(tmp_path, n_genomes
         s.set_color_output(True)
         got[algo] = s.search_reads(chars, ends)
     assert np.array_equal(got[S.ALGO_STREAMING], got[S.ALGO_PER_KMER])
+
+
+@pytest.mark.parametrize("algo", [S.ALGO_STREAMING, S.ALGO_PER_KMER], ids=["streaming", "per-kmer"])
+@pytest.mark.parametrize("n_genomes", [8, 80], ids=["masks-8-colors", "ranks-80-colors"])
+def test_fused_key_chains_equal_oracle(tmp_path, n_genomes, algo):
+    """Key-chain output (the search leaves the key pass to the color stage, which
+    resolves each chain while it counts, right to left; work items of 64 k-mers cut
+    chains at their ends) gives the oracle's lists, per read and per work item."""
+    say(f"fused key chains: {n_genomes} colors algo={algo}")
+    ci = ColoredIndex(tmp_path, n_genomes=n_genomes, length=12_000, sub=0.02, d=9, seed=5)
+    idx = S.Index.load(ci.sbwt_path, ci.colors_path)
+    col = S.ColorIndex(ci.colors_path)
+    idx.build_color_table(col)
+    lens = np.concatenate([np.full(600, 120, np.uint32), W.uniform_lengths(61, 60, 31, 6000)])
+    chars, ends = ci.reads(len(lens), lens, seed=62, random_frac=0.1, err=0.01)
+    keys = ci.index_results(chars, ends)
+    ke = kmer_ends_of(ends)
+    s = idx.searcher(len(chars), len(ends), move_to_key_kmer=True)
+    s.set_algorithm(algo)
+    s.set_color_output(S.COLOR_OUTPUT_CHAINS)
+    raw = s.search_reads(chars, ends)
+    for items in (0, 64):
+        for tau, inc_nf, quirk in ((0.7, False, True), (0.4, True, False)):
+            say(f"  items={items} tau={tau} nf={inc_nf}")
+            exp, _ = expected_lists(ci, keys, ke, tau, inc_nf, False, quirk)
+            cs = col.searcher(len(raw) + 64, len(ke))
+            cs.set_work_items(items)
+            cs.set_input(idx, chains=True)
+            lists, _, _, _ = cs.pseudoalign(raw, ke, S.color_params(tau, inc_nf, False, quirk))
+            assert lists == exp, (items, tau)
+
+
+def test_fused_key_chains_with_other_index_marks(tmp_path):
+    """Key chains over an index whose marks differ from the color index's (no ranks
+    from the search: every chain end reads the color table) give the same lists as
+    the color-table entries."""
+    say("fused key chains: index marks differ from the color index's")
+    ci = ColoredIndex(tmp_path, n_genomes=80, length=12_000, sub=0.02, d=9, seed=5)
+    km = ci.key_marks.copy()
+    bits = np.unpackbits(km.view(np.uint8), bitorder="little")[:ci.sbwt.n]
+    extra = int(np.flatnonzero(bits == 0)[0])
+    km.view(np.uint8)[extra // 8] |= np.uint8(1 << (extra % 8))
+    idx = S.Index.create([ci.sbwt.bits(c) for c in range(4)], ci.sbwt.n, K, key_marks=km,
+                         suffix_group_starts=ci.sbwt.bits(4))
+    col = S.ColorIndex(ci.colors_path)
+    idx.build_color_table(col)
+    lens = np.concatenate([np.full(600, 120, np.uint32), W.uniform_lengths(63, 60, 31, 6000)])
+    chars, ends = ci.reads(len(lens), lens, seed=64, random_frac=0.1, err=0.01)
+    ke = kmer_ends_of(ends)
+    got = {}
+    for mode in (S.COLOR_OUTPUT_ENTRIES, S.COLOR_OUTPUT_CHAINS):
+        s = idx.searcher(len(chars), len(ends), move_to_key_kmer=True)
+        s.set_algorithm(S.ALGO_STREAMING)
+        s.set_color_output(mode)
+        res = s.search_reads(chars, ends)
+        cs = col.searcher(len(res) + 64, len(ke))
+        cs.set_input(idx, chains=mode == S.COLOR_OUTPUT_CHAINS)
+        got[mode], _, _, _ = cs.pseudoalign(res, ke, S.color_params(0.7, False, False, True))
+    assert got[S.COLOR_OUTPUT_ENTRIES] == got[S.COLOR_OUTPUT_CHAINS]
