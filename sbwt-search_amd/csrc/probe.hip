@@ -1,6 +1,6 @@
 // Random-line probe: the ceiling of dependent random 128-byte line reads on
 // this GPU at a given working-set size, against which the search kernels'
-// line rates are judged (their cost unit is one 128-byte line per random
+// line rates are measured (their cost unit is one 128-byte line per random
 // access). No counterpart in the reference.
 #include <algorithm>
 

@@ -1,5 +1,5 @@
-"""The BASELINE workloads (sbwt-search_amd/bindings/configs.py) on the MI355X,
-checked against the oracle.
+"""The benchmark workloads C1-C5 (sbwt-search_amd/bindings/configs.py) on the
+MI355X, checked against the oracle.
 
 C1 runs at full size against the full oracle output. C2 runs at full size
 (95 M-node index, 10 M reads): the oracle checks a strided sample of 10^5
