@@ -578,6 +578,15 @@ def mode_colors(args, configs, S, torch, W):
                                   d_res.data_ptr())
         launch = {"chunk_kmers": ch, "blocks_per_cu": bpc}
         log(f"autotune: {ch} k-mers per work item, {bpc} workgroups per CU ({tms:.2f} ms)")
+        # work-item shape of the color stage, timed on the color-table form's input
+        cs.set_input(idx)
+        s.search_reads_device(d_chars.data_ptr(), wl.n_chars, d_ends.data_ptr(), wl.n_reads,
+                              d_res.data_ptr())
+        s.sync()
+        items, ims = cs.autotune(d_res.data_ptr(), s.kmer_ends_device(), wl.n_reads, params,
+                                 d_off.data_ptr(), d_ids.data_ptr(), cap)
+        launch["color_item_kmers"] = items
+        log(f"color autotune: {items} k-mers per work item ({ims:.2f} ms)")
     med_f, n_ids_f, lists_f, tm_f, color_ms_f = run(True)
     med, n_ids, lists, tm, color_ms = run(False)
     same = n_ids == n_ids_f and all(np.array_equal(a, b) for a, b in zip(lists, lists_f))

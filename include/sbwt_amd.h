@@ -425,6 +425,18 @@ int sbwt_color_searcher_counts_device(sbwt_color_searcher *searcher,
  * src/ColorSearcher/ColorSearcher.cuh:96-205). */
 int sbwt_color_searcher_set_work_items(sbwt_color_searcher *searcher, uint32_t item_kmers);
 
+/* Autotuner of the work-item shape above for one caller batch (the arguments
+ * of sbwt_color_searcher_pseudoalign_device, whose outputs it overwrites): one
+ * wave per read, then items of 256, 512, 1024, 2048 and 4096 k-mers, each
+ * timed on one pass after an untimed one; the fastest is kept. *item_kmers
+ * and *ms (nullable) receive it and its color-stage time. No counterpart in
+ * the reference. */
+int sbwt_color_searcher_autotune(sbwt_color_searcher *searcher, const uint64_t *d_kmer_results,
+                                 const uint64_t *d_read_kmer_ends, uint64_t n_reads,
+                                 const sbwt_color_params *params, uint64_t *d_offsets,
+                                 uint32_t *d_ids, uint64_t ids_capacity, uint32_t *item_kmers,
+                                 float *ms);
+
 /* Fused index + color path (no counterpart in the reference, whose stages
  * meet through index files): builds the index's color table from a color
  * index over the same SBWT (the index needs its key-k-mer marks): per node,

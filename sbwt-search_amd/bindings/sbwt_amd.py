@@ -127,6 +127,9 @@ def lib():
                                                              C.c_uint64, vp, u64p]
         L.sbwt_color_searcher_counts_device.argtypes = [vp, vp, vp, C.c_uint64, vp]
         L.sbwt_color_searcher_set_work_items.argtypes = [vp, C.c_uint32]
+        L.sbwt_color_searcher_autotune.argtypes = [vp, vp, vp, C.c_uint64, C.POINTER(ColorParams),
+                                                   vp, vp, C.c_uint64, C.POINTER(C.c_uint32),
+                                                   C.POINTER(C.c_float)]
         L.sbwt_index_build_color_table.argtypes = [vp, vp, C.POINTER(C.c_int)]
         L.sbwt_searcher_set_color_output.argtypes = [vp, C.c_int]
         L.sbwt_color_searcher_set_input.argtypes = [vp, vp]
@@ -704,6 +707,14 @@ class ColorSearcher:
     def set_work_items(self, item_kmers):
         """0: one wave per read; else a queue of (read, k-mer range) items."""
         _check(lib().sbwt_color_searcher_set_work_items(self.h, item_kmers))
+
+    def autotune(self, d_results, d_kmer_ends, n_reads, params, d_offsets, d_ids, ids_capacity):
+        """Times the work-item shapes on this batch and keeps the fastest: (item_kmers, ms)."""
+        ik, ms = C.c_uint32(), C.c_float()
+        _check(lib().sbwt_color_searcher_autotune(
+            self.h, C.c_void_p(d_results), C.c_void_p(d_kmer_ends), n_reads, C.byref(params),
+            C.c_void_p(d_offsets), C.c_void_p(d_ids), ids_capacity, C.byref(ik), C.byref(ms)))
+        return int(ik.value), float(ms.value)
 
     def counts_device(self, d_results, d_kmer_ends, n_reads, d_counts):
         _check(lib().sbwt_color_searcher_counts_device(self.h, C.c_void_p(d_results),

@@ -588,6 +588,20 @@ int sbwt_color_searcher_set_chain_input(sbwt_color_searcher *searcher, const sbw
   });
 }
 
+int sbwt_color_searcher_autotune(sbwt_color_searcher *searcher, const uint64_t *d_kmer_results,
+                                 const uint64_t *d_read_kmer_ends, uint64_t n_reads,
+                                 const sbwt_color_params *params, uint64_t *d_offsets,
+                                 uint32_t *d_ids, uint64_t ids_capacity, uint32_t *item_kmers,
+                                 float *ms) {
+  return guarded([&] {
+    require(searcher, "null argument");
+    const uint32_t ik = searcher->impl->autotune_items(d_kmer_results, d_read_kmer_ends, n_reads,
+                                                       to_params(params), d_offsets, d_ids,
+                                                       ids_capacity, ms);
+    if (item_kmers) *item_kmers = ik;
+  });
+}
+
 int sbwt_color_searcher_set_work_items(sbwt_color_searcher *searcher, uint32_t item_kmers) {
   return guarded([&] {
     require(searcher, "null argument");

@@ -1303,6 +1303,32 @@ void ColorSearcher::set_work_items(u32 item_kmers) {
   item_kmers_ = static_cast<u32>(ik);
 }
 
+u32 ColorSearcher::autotune_items(const u64 *d_results, const u64 *d_kmer_ends, u64 n_reads,
+                                  const ColorParams &p, u64 *d_offsets, u32 *d_ids,
+                                  u64 ids_capacity, float *best_ms) {
+  static constexpr u32 kCandidates[] = {0, 256, 512, 1024, 2048, 4096};
+  const bool timing = timing_;
+  timing_ = true;
+  float best = 0;
+  u32 best_items = item_kmers_;
+  bool any = false;
+  for (const u32 c : kCandidates) {
+    set_work_items(c);
+    pseudoalign_device(d_results, d_kmer_ends, n_reads, p, d_offsets, d_ids, ids_capacity, nullptr);
+    pseudoalign_device(d_results, d_kmer_ends, n_reads, p, d_offsets, d_ids, ids_capacity, nullptr);
+    const float ms = last_search_ms();
+    if (!any || ms < best) {
+      best = ms;
+      best_items = c;
+      any = true;
+    }
+  }
+  set_work_items(best_items);
+  timing_ = timing;
+  if (best_ms) *best_ms = best;
+  return item_kmers_;
+}
+
 ColorSearcher::~ColorSearcher() {
   for (auto &e : ev_) (void)hipEventDestroy(e);
   void *bufs[] = {d_bitmap_, d_nout_,      d_scan_tmp_, d_quirk_,    d_warped_,

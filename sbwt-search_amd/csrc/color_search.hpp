@@ -102,6 +102,12 @@ class ColorSearcher {
   // length. Results are identical.
   void set_work_items(u32 item_kmers);
   u32 work_items() const { return item_kmers_; }
+  // Work-item shape for a caller batch (the arguments of pseudoalign_device):
+  // one wave per read, then items of 256 to 4096 k-mers, each timed on one
+  // pseudoalignment after an untimed one; keeps and returns the fastest
+  // (*best_ms, nullable: its color-stage time). Outputs are overwritten.
+  u32 autotune_items(const u64 *d_results, const u64 *d_kmer_ends, u64 n_reads, const ColorParams &p,
+                     u64 *d_offsets, u32 *d_ids, u64 ids_capacity, float *best_ms);
   // Input of the fused color stage: the key nodes of the index search (null,
   // default), or the color-table entries that a searcher with color output
   // writes over index (whose table must come from this color index), or, with

@@ -80,6 +80,33 @@ def test_work_items_equal_per_read_and_oracle(long_reads_index, items, tau, inc_
     assert max(int(ke[i]) - (int(ke[i - 1]) if i else 0) for i in range(len(ke))) > 8 * items
 
 
+def test_autotuned_work_items_equal_oracle(long_reads_index):
+    """The work-item autotuner times every shape on the batch, keeps one of them, and
+    both its own runs and the searcher's later batches give the oracle's lists."""
+    import torch
+    ci, res, ke = long_reads_index
+    say("work-item autotune")
+    col = S.ColorIndex(ci.colors_path)
+    p = S.color_params(0.7)
+    exp, _ = expected_lists(ci, res, ke, 0.7)
+    cs = col.searcher(len(res) + 64, len(ke))
+    dr = torch.from_numpy(res.view(np.int64)).cuda()
+    dk = torch.from_numpy(ke.view(np.int64)).cuda()
+    d_off = torch.empty(len(ke) + 1, dtype=torch.int64, device="cuda")
+    cap = len(ke) * ci.num_colors
+    d_ids = torch.empty(cap, dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    items, ms = cs.autotune(dr.data_ptr(), dk.data_ptr(), len(ke), p, d_off.data_ptr(),
+                            d_ids.data_ptr(), cap)
+    say(f"  kept {items} k-mers per item, {ms:.3f} ms")
+    assert items in (0, 256, 512, 1024, 2048, 4096) and ms > 0
+    off = d_off.cpu().numpy().view(np.uint64)
+    ids = d_ids.cpu().numpy().view(np.uint32)
+    assert [ids[int(off[r]):int(off[r + 1])].tolist() for r in range(len(ke))] == exp
+    lists, _, _, _ = cs.pseudoalign(res, ke, p)
+    assert lists == exp
+
+
 def _many_colors_index(tmp, n_genomes, length, n_colors, seed):
     # genome g gets color g * stride: the ids spread over [0, n_colors)
     stride = (n_colors - 1) // (n_genomes - 1)
