@@ -866,7 +866,7 @@ def mode_gather_probe(args, S):
     """The random-line probe over working-set sizes (the ceiling of dependent random
     128-byte line reads, cache-resident to HBM-resident)."""
     rows = []
-    for mib in (16, 64, 128, 256, 512, 1024, 4096, 16384, 65536):
+    for mib in (16, 64, 128, 256, 512, 1024, 4096, 16384, 65536, 131072):
         for bpc in (2, 4, 8):
             r = S.random_line_probe(mib << 20, bpc)
             rows.append({"working_set_mib": r["table_bytes"] >> 20, "blocks_per_cu": bpc,
